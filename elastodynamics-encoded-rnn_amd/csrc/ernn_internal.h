@@ -1,0 +1,91 @@
+// Internal layout shared by the HIP kernels (ernn_kernels.hip) and the host
+// plan builder (ernn_plan.cpp).  Plain structs, identical on host and device.
+//
+// State layout in the workspace ("padded field"): a field of an nx x ny grid
+// occupies (nx + 3) rows of `pitch` floats.  Row r of the grid (-1 <= r <= nx+1)
+// starts at (r + 1) * pitch, and column j (-32 <= j < pitch - 32) sits at
+// +32 + j, so interior cell (i, j) is at (i + 1) * pitch + 32 + j.  Everything
+// outside 0 <= i < nx, 0 <= j < ny is a ghost cell that stays zero: the
+// reference's zero padding (Parameter_inversion/Inversion_adam.py:50).
+// pitch = 32 + round_up(ny, 256), so every interior row starts 128-byte
+// aligned and a wavefront's last halo read (column round_up(ny, 64 V)) lands
+// at most on the zero left pad of the next row.
+#pragma once
+#include <stdint.h>
+
+namespace ernn {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kLeftPad = 32;
+constexpr int kColAlign = 256;
+
+enum ParamMode : int { kConst6 = 0, kDField = 1, kField6 = 2 };
+
+// One instance = one shot on one grid with one parameter set.
+struct InstDev {
+  int nx, ny, pitch, nt;
+  int set;          // parameter set index
+  int n_rec;        // receivers (receiver mode)
+  int n_frames;     // recorded frames, ceil(nt / P)
+  int pad0;
+  float hinv2;      // fl32(h^-2)
+  float dt2;        // fl32(dt^2)
+  float rho;        // fl32(rho)
+  float pad1;
+  double gscale;    // (double)dt2 / (double)rho: d(u^{n+1})/dC = gscale * dA/dC u^n
+  int64_t field;    // floats per padded field, (nx + 3) * pitch
+  int64_t pool_off; // float offset of this instance's state slots in the workspace
+  int64_t coef_off; // float offset of the packed coefficient fields of its set
+  int64_t acc_off;  // float offset of its fp32 per-cell segment accumulators
+  int64_t acc64_off;// double offset of its fp64 per-cell accumulators
+  int64_t trace_off;// float offset of its traces in the trace array
+  int64_t wav_off;  // float offset of its source time function
+};
+
+// One tile = R rows x (64 * V) columns of one instance, swept by one wavefront.
+struct TileDev {
+  int inst, i0, j0, set;
+  int src_begin, src_end;   // source events of this tile (sorted by row)
+  int rec_begin, rec_end;   // receiver events of this tile (sorted by row)
+};
+
+struct SrcEv { int i, j; float w; int comp; };
+struct RecEv { int i, j, rec, pad; };
+
+// Per-call device pointers, written by a one-thread kernel before each
+// forward/adjoint pass so that captured graphs stay valid across calls.
+struct CallDev {
+  float* ws;               // workspace
+  const float* params;     // [n_sets][6] constants (kConst6, kDField)
+  const float* field;      // [n_sets][n_coef][nx*ny] stiffness fields (kDField: D, kField6: 6 C)
+  const float* wav;        // source time functions
+  float* traces;           // forward output
+  const float* gtr;        // adjoint input: dJ/d traces (same layout)
+  double* grad_params;     // [n_sets][6]
+  double* grad_field;      // [n_sets][nx*ny] (kDField) or [n_sets][6][nx*ny] (kField6)
+};
+
+// Per-step scalars (kernel argument by value).
+struct StepArgs {
+  const InstDev* inst;
+  const TileDev* tiles;
+  const SrcEv* src;
+  const RecEv* rec;
+  const CallDev* call;
+  int64_t partial_off;     // double offset in the workspace of [n_tiles][6] fp64 partials
+  int n_tiles;
+  int mode;
+  int step;                // forward: n (produces u^{n+1}); adjoint: m (produces lam^m)
+  int s_prev, s_cur, s_next; // slots: fwd u^{n-1}, u^n, u^{n+1}; adj lam^{m+2}, lam^{m+1}, lam^m
+  int s_u;                 // adjoint: slot holding u^m
+  int stride;              // recording stride P
+  int record;              // forward: write traces on recorded steps
+  int snapshot;            // traces are whole fields
+  int seg_top, seg_bot;    // adjoint: the segment runs m = seg_top .. seg_bot
+  int rows;                // R
+  int pad;
+};
+
+}  // namespace ernn

@@ -1,0 +1,761 @@
+// HIP kernels for the ERNN forward recurrence and its reverse-time adjoint
+// on gfx950 (CDNA4, wave64).
+//
+// Reference semantics (paths relative to the reference repository root):
+//   PI = Parameter_inversion/Inversion_adam.py, DI = Damage_inversion/Inversion_adam.py
+//   stencils            PI:56-69, PI:78-83 (zero ghosts PI:50)
+//   operator, scalar C  PI:86-98;  per-cell C (non-divergence form) DI:151-163
+//   force and density   PI:103-104, source term PI:270-277
+//   leapfrog            PI:286-291;  recorded frame k = u^{kP+1}  PI:294-301
+//   gradient path       replaces autograd at PI:447 / DI:511
+//
+// Work decomposition: one wavefront owns a tile of R rows x 64*V columns and
+// sweeps it row by row, keeping rows i-1, i, i+1 of each stencilled field in
+// registers.  Lane l holds V consecutive columns; the column neighbours come
+// from the adjacent lanes by cross-lane shuffles, and the two edge lanes fetch
+// the tile's halo columns with one masked load.  Ghost cells are real zeros
+// in the padded layout (ernn_internal.h), so the sweep has no boundary
+// branches.  No LDS and no block barriers: the four wavefronts of a block
+// work on independent tiles.  Block b runs on XCD b % 8; the tile order maps
+// each XCD to one contiguous band of tiles so that halo rows and the next
+// step's reads of the same band hit that XCD's L2.
+//
+// The forward arithmetic follows the reference op by op in FP32 with
+// contraction disabled (-ffp-contract=off): each 3-tap stencil sums its taps
+// right to left as PyTorch's CPU conv2d does, the six stencil products are
+// summed left to right as written in PI:86-98, then (L + f) / rho and
+// (2u - u_prev) + fl32(dt^2) * a.
+#include <hip/hip_runtime.h>
+#include "ernn_internal.h"
+
+namespace ernn {
+
+// --------------------------------------------------------------------------
+// Register rows
+// --------------------------------------------------------------------------
+template <int V>
+struct Row {
+  float v[V + 2];  // v[0]: column j0+lV-1, v[1..V]: own columns, v[V+1]: j0+lV+V
+};
+
+template <int V>
+__device__ __forceinline__ void load_vec(const float* __restrict__ p, float (&c)[V]) {
+  if constexpr (V == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    c[0] = t.x; c[1] = t.y; c[2] = t.z; c[3] = t.w;
+  } else if constexpr (V == 2) {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    c[0] = t.x; c[1] = t.y;
+  } else {
+    c[0] = *p;
+  }
+}
+
+template <int V>
+__device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&c)[V]) {
+  if constexpr (V == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(c[0], c[1], c[2], c[3]);
+  } else if constexpr (V == 2) {
+    *reinterpret_cast<float2*>(p) = make_float2(c[0], c[1]);
+  } else {
+    *p = c[0];
+  }
+}
+
+// rowp points at column j0 of one padded row.
+template <int V>
+__device__ __forceinline__ Row<V> load_row(const float* __restrict__ rowp, int lane) {
+  Row<V> r;
+  float c[V];
+  load_vec<V>(rowp + lane * V, c);
+  const bool edge = (lane == 0) | (lane == kWave - 1);
+  const int hoff = (lane == 0) ? -1 : kWave * V;
+  float halo = 0.f;
+  if (edge) halo = rowp[hoff];
+  const float left = __shfl_up(c[V - 1], 1, kWave);
+  const float right = __shfl_down(c[0], 1, kWave);
+  r.v[0] = (lane == 0) ? halo : left;
+#pragma unroll
+  for (int e = 0; e < V; ++e) r.v[e + 1] = c[e];
+  r.v[V + 1] = (lane == kWave - 1) ? halo : right;
+  return r;
+}
+
+template <int V>
+__device__ __forceinline__ Row<V> mul_row(const Row<V>& a, const Row<V>& b) {
+  Row<V> r;
+#pragma unroll
+  for (int e = 0; e < V + 2; ++e) r.v[e] = a.v[e] * b.v[e];
+  return r;
+}
+
+// --------------------------------------------------------------------------
+// Stencils (PI:56-69): weights a = fl32(h^-2), -2a, q = a/4, -q (all exact)
+// --------------------------------------------------------------------------
+struct Weights {
+  float a, m2a, q, mq;
+};
+
+__device__ __forceinline__ Weights make_weights(float hinv2) {
+  return Weights{hinv2, -2.f * hinv2, 0.25f * hinv2, -0.25f * hinv2};
+}
+
+// x = position in Row::v of the output column
+template <int V>
+__device__ __forceinline__ float d_xx(const Row<V>& m, const Row<V>& c, const Row<V>& p, int x,
+                                      const Weights& w) {
+  return (w.a * p.v[x] + w.m2a * c.v[x]) + w.a * m.v[x];
+}
+template <int V>
+__device__ __forceinline__ float d_yy(const Row<V>& c, int x, const Weights& w) {
+  return (w.a * c.v[x + 1] + w.m2a * c.v[x]) + w.a * c.v[x - 1];
+}
+template <int V>
+__device__ __forceinline__ float d_xy(const Row<V>& m, const Row<V>& p, int x, const Weights& w) {
+  return ((w.q * p.v[x + 1] + w.mq * m.v[x + 1]) + w.mq * p.v[x - 1]) + w.q * m.v[x - 1];
+}
+
+struct Deriv {
+  float ux_xx, ux_yy, ux_xy, uy_xx, uy_yy, uy_xy;
+};
+
+template <int V>
+__device__ __forceinline__ Deriv derivs(const Row<V>& xm, const Row<V>& xc, const Row<V>& xp,
+                                        const Row<V>& ym, const Row<V>& yc, const Row<V>& yp,
+                                        int x, const Weights& w) {
+  Deriv d;
+  d.ux_xx = d_xx<V>(xm, xc, xp, x, w);
+  d.ux_yy = d_yy<V>(xc, x, w);
+  d.ux_xy = d_xy<V>(xm, xp, x, w);
+  d.uy_xx = d_xx<V>(ym, yc, yp, x, w);
+  d.uy_yy = d_yy<V>(yc, x, w);
+  d.uy_xy = d_xy<V>(ym, yp, x, w);
+  return d;
+}
+
+// Stiffness at one cell in the reference's order, with the derived sums the
+// operator uses: C12 + C66, 2 C16, 2 C26 (PI:88-98).
+struct Stiff {
+  float c11, c22, c12, c16, c26, c66, c1266, c216, c226;
+};
+
+__device__ __forceinline__ Stiff make_stiff(float c11, float c22, float c12, float c16, float c26,
+                                            float c66) {
+  Stiff s;
+  s.c11 = c11; s.c22 = c22; s.c12 = c12; s.c16 = c16; s.c26 = c26; s.c66 = c66;
+  s.c1266 = c12 + c66;
+  s.c216 = 2.f * c16;
+  s.c226 = 2.f * c26;
+  return s;
+}
+
+// Lx, Ly of PI:86-98 summed left to right.
+__device__ __forceinline__ void apply_op(const Stiff& s, const Deriv& d, float& lx, float& ly) {
+  lx = ((((s.c11 * d.ux_xx + s.c66 * d.ux_yy) + s.c1266 * d.uy_xy) + s.c216 * d.ux_xy) +
+        s.c16 * d.uy_xx) + s.c26 * d.uy_yy;
+  ly = ((((s.c66 * d.uy_xx + s.c22 * d.uy_yy) + s.c1266 * d.ux_xy) + s.c16 * d.ux_xx) +
+        s.c26 * d.ux_yy) + s.c226 * d.uy_xy;
+}
+
+// dA/dC_p u contracted with (lx, ly) at one cell, p in C11 C22 C12 C16 C26 C66
+// order.  With d = derivatives of u these are the per-cell partials; with d =
+// derivatives of the adjoint field and (lx, ly) = u the same expressions give
+// the summation-by-parts form used for constant C (all stencils symmetric).
+__device__ __forceinline__ void grad_terms(const Deriv& d, float lx, float ly, float (&g)[6]) {
+  g[0] = lx * d.ux_xx;
+  g[1] = ly * d.uy_yy;
+  g[2] = lx * d.uy_xy + ly * d.ux_xy;
+  g[3] = lx * (2.f * d.ux_xy + d.uy_xx) + ly * d.ux_xx;
+  g[4] = lx * d.uy_yy + ly * (d.ux_yy + 2.f * d.uy_xy);
+  g[5] = lx * (d.ux_yy + d.uy_xy) + ly * (d.uy_xx + d.ux_xy);
+}
+
+// Summation-by-parts variant: <lam, dA/dC_p u> = <D lam, u> per stencil.
+// dl = derivatives of lam (or of D*lam), (ux, uy) = u at the cell.
+__device__ __forceinline__ void grad_terms_sbp(const Deriv& dl, float ux, float uy, float (&g)[6]) {
+  g[0] = dl.ux_xx * ux;
+  g[1] = dl.uy_yy * uy;
+  g[2] = dl.ux_xy * uy + dl.uy_xy * ux;
+  g[3] = 2.f * dl.ux_xy * ux + dl.ux_xx * uy + dl.uy_xx * ux;
+  g[4] = dl.ux_yy * uy + dl.uy_yy * ux + 2.f * dl.uy_xy * uy;
+  g[5] = dl.ux_yy * ux + dl.ux_xy * uy + dl.uy_xx * uy + dl.uy_xy * ux;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Tile index of this wavefront.  gridDim.x is a multiple of 8; block b runs
+// on XCD b % 8 and XCD x takes the contiguous logical blocks
+// [x * G/8, (x+1) * G/8).
+__device__ __forceinline__ int tile_index() {
+  const int b = blockIdx.x;
+  const int lb = (b & 7) * (gridDim.x >> 3) + (b >> 3);
+  return __builtin_amdgcn_readfirstlane(lb * kWavesPerBlock + (threadIdx.x >> 6));
+}
+
+__device__ __forceinline__ const float* prow(const float* base, int64_t pitch, int r, int j0) {
+  return base + (int64_t)(r + 1) * pitch + kLeftPad + j0;
+}
+__device__ __forceinline__ float* prow(float* base, int64_t pitch, int r, int j0) {
+  return base + (int64_t)(r + 1) * pitch + kLeftPad + j0;
+}
+
+// --------------------------------------------------------------------------
+// Forward step: u^{n+1} from u^n, u^{n-1}  (PI:268-301 / DI:355-387)
+// --------------------------------------------------------------------------
+template <int MODE, int V>
+__global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = tile_index();
+  if (t >= a.n_tiles) return;
+  const TileDev tl = a.tiles[t];
+  const InstDev I = a.inst[tl.inst];
+  const int n = a.step;
+  if (n >= I.nt) return;
+  const CallDev C = *a.call;
+
+  const int64_t pitch = I.pitch;
+  const int j0 = tl.j0;
+  float* pool = C.ws + I.pool_off;
+  const float* u0x = pool + (int64_t)a.s_cur * 2 * I.field;
+  const float* u0y = u0x + I.field;
+  const float* u1x = pool + (int64_t)a.s_prev * 2 * I.field;
+  const float* u1y = u1x + I.field;
+  float* unx = pool + (int64_t)a.s_next * 2 * I.field;
+  float* uny = unx + I.field;
+  const float* coef = C.ws + I.coef_off;
+
+  const Weights w = make_weights(I.hinv2);
+  float c0[6];
+  if constexpr (MODE != kField6) {
+#pragma unroll
+    for (int p = 0; p < 6; ++p) c0[p] = C.params[tl.set * 6 + p];
+  }
+  const Stiff sc = make_stiff(c0[0], c0[1], c0[2], c0[3], c0[4], c0[5]);
+  const float s_n = C.wav[I.wav_off + n];
+  const bool rec_step = a.record && (n % a.stride == 0);
+  const int k = n / a.stride;
+  const int nf = I.n_frames;
+
+  Row<V> xm = load_row<V>(prow(u0x, pitch, tl.i0 - 1, j0), lane);
+  Row<V> xc = load_row<V>(prow(u0x, pitch, tl.i0, j0), lane);
+  Row<V> ym = load_row<V>(prow(u0y, pitch, tl.i0 - 1, j0), lane);
+  Row<V> yc = load_row<V>(prow(u0y, pitch, tl.i0, j0), lane);
+
+  int sp = tl.src_begin;
+  int rp = tl.rec_begin;
+  const int iend = min(tl.i0 + a.rows, I.nx);
+  for (int i = tl.i0; i < iend; ++i) {
+    const Row<V> xp = load_row<V>(prow(u0x, pitch, i + 1, j0), lane);
+    const Row<V> yp = load_row<V>(prow(u0y, pitch, i + 1, j0), lane);
+    float pvx[V], pvy[V];
+    load_vec<V>(prow(u1x, pitch, i, j0) + lane * V, pvx);
+    load_vec<V>(prow(u1y, pitch, i, j0) + lane * V, pvy);
+
+    float cf[6][V];
+    if constexpr (MODE == kDField) {
+      load_vec<V>(prow(coef, pitch, i, j0) + lane * V, cf[0]);
+    } else if constexpr (MODE == kField6) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) load_vec<V>(prow(coef + q * I.field, pitch, i, j0) + lane * V, cf[q]);
+    }
+
+    float lx[V], ly[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const Deriv d = derivs<V>(xm, xc, xp, ym, yc, yp, e + 1, w);
+      if constexpr (MODE == kConst6) {
+        apply_op(sc, d, lx[e], ly[e]);
+      } else if constexpr (MODE == kDField) {
+        const float D = cf[0][e];
+        const Stiff s = make_stiff(D * c0[0], D * c0[1], D * c0[2], D * c0[3], D * c0[4], D * c0[5]);
+        apply_op(s, d, lx[e], ly[e]);
+      } else {
+        const Stiff s = make_stiff(cf[0][e], cf[1][e], cf[2][e], cf[3][e], cf[4][e], cf[5][e]);
+        apply_op(s, d, lx[e], ly[e]);
+      }
+    }
+
+    // point forces on this row: f = s_n * w added to L before / rho (PI:103-104)
+    while (sp < tl.src_end) {
+      const SrcEv ev = a.src[sp];
+      if (ev.i != i) break;
+      const int lc = ev.j - j0;
+      if (lane == lc / V) {
+        const float f = s_n * ev.w;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          if (e == lc % V) {
+            if (ev.comp == 0) lx[e] = lx[e] + f;
+            else ly[e] = ly[e] + f;
+          }
+        }
+      }
+      ++sp;
+    }
+
+    float nvx[V], nvy[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const float ax = lx[e] / I.rho;
+      const float ay = ly[e] / I.rho;
+      float vx = (2.f * xc.v[e + 1] - pvx[e]) + I.dt2 * ax;
+      float vy = (2.f * yc.v[e + 1] - pvy[e]) + I.dt2 * ay;
+      if (j0 + lane * V + e >= I.ny) { vx = 0.f; vy = 0.f; }  // keep the right ghosts zero
+      nvx[e] = vx;
+      nvy[e] = vy;
+    }
+    store_vec<V>(prow(unx, pitch, i, j0) + lane * V, nvx);
+    store_vec<V>(prow(uny, pitch, i, j0) + lane * V, nvy);
+
+    if (rec_step) {
+      if (a.snapshot) {
+        float* tx = C.traces + I.trace_off + ((int64_t)k * I.nx + i) * I.ny;
+        float* ty = tx + (int64_t)nf * I.nx * I.ny;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const int j = j0 + lane * V + e;
+          if (j < I.ny) { tx[j] = nvx[e]; ty[j] = nvy[e]; }
+        }
+      } else {
+        while (rp < tl.rec_end) {
+          const RecEv ev = a.rec[rp];
+          if (ev.i != i) break;
+          const int lc = ev.j - j0;
+          if (lane == lc / V) {
+            float* tx = C.traces + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+            float* ty = tx + (int64_t)nf * I.n_rec;
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              if (e == lc % V) { *tx = nvx[e]; *ty = nvy[e]; }
+            }
+          }
+          ++rp;
+        }
+      }
+    }
+    xm = xc; xc = xp;
+    ym = yc; yc = yp;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Adjoint step: lam^m = r^m + (2I + (dt^2/rho) A^T) lam^{m+1} - lam^{m+2}
+// with the gradient terms (dt^2/rho) <lam^{m+1}, dA/dC u^m>.
+// --------------------------------------------------------------------------
+template <int MODE, int V>
+__global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int t = tile_index();
+  if (t >= a.n_tiles) return;
+  const TileDev tl = a.tiles[t];
+  const InstDev I = a.inst[tl.inst];
+  const int m = a.step;
+  if (m > I.nt) return;  // this instance has not started its adjoint yet
+  const CallDev C = *a.call;
+
+  const int64_t pitch = I.pitch;
+  const int j0 = tl.j0;
+  float* pool = C.ws + I.pool_off;
+  const float* l1x = pool + (int64_t)a.s_cur * 2 * I.field;   // lam^{m+1}
+  const float* l1y = l1x + I.field;
+  const float* l2x = pool + (int64_t)a.s_prev * 2 * I.field;  // lam^{m+2}
+  const float* l2y = l2x + I.field;
+  float* lnx = pool + (int64_t)a.s_next * 2 * I.field;        // lam^m
+  float* lny = lnx + I.field;
+  const float* umx = pool + (int64_t)a.s_u * 2 * I.field;     // u^m
+  const float* umy = umx + I.field;
+  const float* coef = C.ws + I.coef_off;
+
+  const Weights w = make_weights(I.hinv2);
+  float c0[6];
+  if constexpr (MODE != kField6) {
+#pragma unroll
+    for (int p = 0; p < 6; ++p) c0[p] = C.params[tl.set * 6 + p];
+  }
+  const Stiff sc = make_stiff(c0[0], c0[1], c0[2], c0[3], c0[4], c0[5]);
+
+  // residual injection on recorded steps m = kP + 1
+  const bool inj = ((m - 1) % a.stride == 0) && ((m - 1) / a.stride < I.n_frames);
+  const int k = (m - 1) / a.stride;
+  const int nf = I.n_frames;
+
+  // per-cell accumulators (field modes): first / last adjoint step of the segment
+  const int top = min(a.seg_top, I.nt);
+  const bool seg_first = (m == top);
+  const bool seg_last = (m == a.seg_bot);
+  constexpr int NACC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
+  float* acc32 = C.ws + I.acc_off;
+  double* acc64 = reinterpret_cast<double*>(C.ws) + I.acc64_off;
+
+  float S[6];
+#pragma unroll
+  for (int p = 0; p < 6; ++p) S[p] = 0.f;
+
+  // Stencilled adjoint rows.  kConst6: lam itself.  kDField: D * lam.
+  // kField6: the six weighted fields of A^T (see apply_AT in the oracle).
+  constexpr int NW = (MODE == kField6) ? 6 : 2;
+  Row<V> wm[NW], wc[NW];
+  // raw lam centres of rows i and i+1 (needed besides the weighted rows)
+  float lcx[V], lcy[V];
+  // u^m rows for the per-cell gradient (field modes)
+  Row<V> um_m[2], um_c[2];
+
+  auto load_w = [&](int r, Row<V> (&out)[NW], float (&cx)[V], float (&cy)[V]) {
+    const Row<V> lx = load_row<V>(prow(l1x, pitch, r, j0), lane);
+    const Row<V> ly = load_row<V>(prow(l1y, pitch, r, j0), lane);
+#pragma unroll
+    for (int e = 0; e < V; ++e) { cx[e] = lx.v[e + 1]; cy[e] = ly.v[e + 1]; }
+    if constexpr (MODE == kConst6) {
+      out[0] = lx;
+      out[1] = ly;
+    } else if constexpr (MODE == kDField) {
+      const Row<V> D = load_row<V>(prow(coef, pitch, r, j0), lane);
+      out[0] = mul_row<V>(D, lx);
+      out[1] = mul_row<V>(D, ly);
+    } else {
+      Row<V> cr[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) cr[q] = load_row<V>(prow(coef + q * I.field, pitch, r, j0), lane);
+#pragma unroll
+      for (int e = 0; e < V + 2; ++e) {
+        const float c11 = cr[0].v[e], c22 = cr[1].v[e], c12 = cr[2].v[e];
+        const float c16 = cr[3].v[e], c26 = cr[4].v[e], c66 = cr[5].v[e];
+        const float x = lx.v[e], y = ly.v[e];
+        out[0].v[e] = c11 * x + c16 * y;                 // -> Dxx, x row
+        out[1].v[e] = c66 * x + c26 * y;                 // -> Dyy, x row
+        out[2].v[e] = 2.f * c16 * x + (c12 + c66) * y;   // -> Dxy, x row
+        out[3].v[e] = c16 * x + c66 * y;                 // -> Dxx, y row
+        out[4].v[e] = c26 * x + c22 * y;                 // -> Dyy, y row
+        out[5].v[e] = (c12 + c66) * x + 2.f * c26 * y;   // -> Dxy, y row
+      }
+    }
+  };
+
+  float dummy_x[V], dummy_y[V];
+  load_w(tl.i0 - 1, wm, dummy_x, dummy_y);
+  load_w(tl.i0, wc, lcx, lcy);
+  if constexpr (MODE != kConst6) {
+    um_m[0] = load_row<V>(prow(umx, pitch, tl.i0 - 1, j0), lane);
+    um_m[1] = load_row<V>(prow(umy, pitch, tl.i0 - 1, j0), lane);
+    um_c[0] = load_row<V>(prow(umx, pitch, tl.i0, j0), lane);
+    um_c[1] = load_row<V>(prow(umy, pitch, tl.i0, j0), lane);
+  }
+
+  int rp = tl.rec_begin;
+  const int iend = min(tl.i0 + a.rows, I.nx);
+  for (int i = tl.i0; i < iend; ++i) {
+    Row<V> wp[NW];
+    float lpx[V], lpy[V];
+    load_w(i + 1, wp, lpx, lpy);
+    float l2vx[V], l2vy[V];
+    load_vec<V>(prow(l2x, pitch, i, j0) + lane * V, l2vx);
+    load_vec<V>(prow(l2y, pitch, i, j0) + lane * V, l2vy);
+    float uvx[V], uvy[V];
+    Row<V> um_p[2];
+    if constexpr (MODE == kConst6) {
+      load_vec<V>(prow(umx, pitch, i, j0) + lane * V, uvx);
+      load_vec<V>(prow(umy, pitch, i, j0) + lane * V, uvy);
+    } else {
+      um_p[0] = load_row<V>(prow(umx, pitch, i + 1, j0), lane);
+      um_p[1] = load_row<V>(prow(umy, pitch, i + 1, j0), lane);
+#pragma unroll
+      for (int e = 0; e < V; ++e) { uvx[e] = um_c[0].v[e + 1]; uvy[e] = um_c[1].v[e + 1]; }
+    }
+    float atx[V], aty[V];
+    float G[NACC > 0 ? NACC : 1][V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const int x = e + 1;
+      float g[6];
+      if constexpr (MODE == kConst6 || MODE == kDField) {
+        // A^T lam = A(C0) (D * lam); constant stencils on the (weighted) adjoint field
+        const Deriv dl = derivs<V>(wm[0], wc[0], wp[0], wm[1], wc[1], wp[1], x, w);
+        apply_op(sc, dl, atx[e], aty[e]);
+        grad_terms_sbp(dl, uvx[e], uvy[e], g);
+#pragma unroll
+        for (int p = 0; p < 6; ++p) S[p] += g[p];
+        if constexpr (MODE == kDField) {
+          // dJ/dD(cell) = lam . A(C0) u^m  at the cell
+          const Deriv du = derivs<V>(um_m[0], um_c[0], um_p[0], um_m[1], um_c[1], um_p[1], x, w);
+          float ox, oy;
+          apply_op(sc, du, ox, oy);
+          G[0][e] = lcx[e] * ox + lcy[e] * oy;
+        }
+      } else {
+        atx[e] = (d_xx<V>(wm[0], wc[0], wp[0], x, w) + d_yy<V>(wc[1], x, w)) + d_xy<V>(wm[2], wp[2], x, w);
+        aty[e] = (d_xx<V>(wm[3], wc[3], wp[3], x, w) + d_yy<V>(wc[4], x, w)) + d_xy<V>(wm[5], wp[5], x, w);
+        const Deriv du = derivs<V>(um_m[0], um_c[0], um_p[0], um_m[1], um_c[1], um_p[1], x, w);
+        grad_terms(du, lcx[e], lcy[e], g);
+#pragma unroll
+        for (int p = 0; p < 6; ++p) G[p][e] = g[p];
+      }
+    }
+
+    float nvx[V], nvy[V];
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      float vx = (2.f * lcx[e] - l2vx[e]) + I.dt2 * (atx[e] / I.rho);
+      float vy = (2.f * lcy[e] - l2vy[e]) + I.dt2 * (aty[e] / I.rho);
+      nvx[e] = vx;
+      nvy[e] = vy;
+    }
+    if (inj) {
+      if (a.snapshot) {
+        const float* gx = C.gtr + I.trace_off + ((int64_t)k * I.nx + i) * I.ny;
+        const float* gy = gx + (int64_t)nf * I.nx * I.ny;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const int j = j0 + lane * V + e;
+          if (j < I.ny) { nvx[e] += gx[j]; nvy[e] += gy[j]; }
+        }
+      } else {
+        while (rp < tl.rec_end) {
+          const RecEv ev = a.rec[rp];
+          if (ev.i != i) break;
+          const int lc = ev.j - j0;
+          if (lane == lc / V) {
+            const float* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+            const float* gy = gx + (int64_t)nf * I.n_rec;
+            const float vgx = *gx, vgy = *gy;
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              if (e == lc % V) { nvx[e] += vgx; nvy[e] += vgy; }
+            }
+          }
+          ++rp;
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      if (j0 + lane * V + e >= I.ny) { nvx[e] = 0.f; nvy[e] = 0.f; }
+    }
+    store_vec<V>(prow(lnx, pitch, i, j0) + lane * V, nvx);
+    store_vec<V>(prow(lny, pitch, i, j0) + lane * V, nvy);
+
+    if constexpr (NACC > 0) {
+#pragma unroll
+      for (int q = 0; q < NACC; ++q) {
+        float* a32 = prow(acc32 + q * I.field, pitch, i, j0) + lane * V;
+        float v[V];
+        if (seg_first) {
+#pragma unroll
+          for (int e = 0; e < V; ++e) v[e] = G[q][e];
+        } else {
+          load_vec<V>(a32, v);
+#pragma unroll
+          for (int e = 0; e < V; ++e) v[e] += G[q][e];
+        }
+        if (seg_last) {
+          double* a64 = acc64 + q * I.field + (int64_t)(i + 1) * pitch + kLeftPad + j0 + lane * V;
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            if (j0 + lane * V + e < I.ny) a64[e] += I.gscale * (double)v[e];
+          }
+        } else {
+          store_vec<V>(a32, v);
+        }
+      }
+    }
+
+#pragma unroll
+    for (int q = 0; q < NW; ++q) { wm[q] = wc[q]; wc[q] = wp[q]; }
+#pragma unroll
+    for (int e = 0; e < V; ++e) { lcx[e] = lpx[e]; lcy[e] = lpy[e]; }
+    if constexpr (MODE != kConst6) {
+      um_m[0] = um_c[0]; um_m[1] = um_c[1];
+      um_c[0] = um_p[0]; um_c[1] = um_p[1];
+    }
+  }
+
+  if constexpr (MODE != kField6) {
+    // per-tile fp64 partials of the six constant gradients (deterministic: one
+    // wavefront owns slot t in every launch)
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      const double s = wave_sum((double)S[p]);
+      if (lane == 0) (reinterpret_cast<double*>(C.ws) + a.partial_off)[(int64_t)t * 6 + p] += I.gscale * s;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Small helpers
+// --------------------------------------------------------------------------
+__global__ void set_call_kernel(CallDev* dst, CallDev v) { *dst = v; }
+
+__global__ void fill_zero_kernel(float* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
+
+// zero n floats at float offset `off` of the call's workspace
+__global__ void fill_zero_ws_kernel(const CallDev* call, int64_t off, int64_t n) {
+  float* p = call->ws + off;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
+
+// external [n_coef][nx*ny] fp32 fields of one set -> padded coefficient rows
+__global__ void pack_fields_kernel(const CallDev* call, const InstDev* inst, const int* set_inst,
+                                   const int64_t* set_ext_off, int n_coef, int set) {
+  const CallDev C = *call;
+  const InstDev I = inst[set_inst[set]];
+  const int64_t ncell = (int64_t)I.nx * I.ny;
+  const float* src = C.field + set_ext_off[set];
+  float* dst = C.ws + I.coef_off;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < ncell * n_coef;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(idx / ncell);
+    const int64_t c = idx - q * ncell;
+    const int i = (int)(c / I.ny), j = (int)(c - (int64_t)i * I.ny);
+    dst[q * I.field + (int64_t)(i + 1) * I.pitch + kLeftPad + j] = src[idx];
+  }
+}
+
+// grad_params[set][p] = sum over the set's tiles of partial[t][p]
+__global__ void finalize_const_kernel(const CallDev* call, int64_t partial_off,
+                                      const TileDev* __restrict__ tiles, int n_tiles) {
+  const int set = blockIdx.x / 6, p = blockIdx.x % 6;
+  const double* partial = reinterpret_cast<const double*>(call->ws) + partial_off;
+  __shared__ double red[kBlock];
+  double s = 0.0;
+  for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
+    if (tiles[t].set == set) s += partial[(int64_t)t * 6 + p];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) call->grad_params[set * 6 + p] = red[0];
+}
+
+// grad_field[set][q][i*ny+j] = sum over the set's instances of acc64 (padded)
+__global__ void finalize_field_kernel(const CallDev* call, const InstDev* inst, const int* set_inst_list,
+                                      const int* set_inst_begin, const int64_t* set_grad_off,
+                                      int n_acc, int set) {
+  const CallDev Cd = *call;
+  const int b = set_inst_begin[set], e = set_inst_begin[set + 1];
+  const InstDev I0 = inst[set_inst_list[b]];
+  const int64_t ncell = (int64_t)I0.nx * I0.ny;
+  double* out = Cd.grad_field + set_grad_off[set];
+  const double* acc = reinterpret_cast<const double*>(Cd.ws);
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < ncell * n_acc;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(idx / ncell);
+    const int64_t c = idx - q * ncell;
+    const int i = (int)(c / I0.ny), j = (int)(c - (int64_t)i * I0.ny);
+    const int64_t off = q * I0.field + (int64_t)(i + 1) * I0.pitch + kLeftPad + j;
+    double s = 0.0;
+    for (int u = b; u < e; ++u) s += acc[inst[set_inst_list[u]].acc64_off + off];
+    out[idx] = s;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Launchers (host side, called from ernn_plan.cpp)
+// --------------------------------------------------------------------------
+template <int MODE, int V>
+static hipError_t launch_fwd_t(const StepArgs& a, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL((fwd_step_kernel<MODE, V>), dim3(blocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+template <int MODE, int V>
+static hipError_t launch_adj_t(const StepArgs& a, int blocks, hipStream_t st) {
+  hipLaunchKernelGGL((adj_step_kernel<MODE, V>), dim3(blocks), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_fwd_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
+  switch (V) {
+    case 1: return launch_fwd_t<MODE, 1>(a, blocks, st);
+    case 2: return launch_fwd_t<MODE, 2>(a, blocks, st);
+    case 4: return launch_fwd_t<MODE, 4>(a, blocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+template <int MODE>
+static hipError_t launch_adj_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
+  switch (V) {
+    case 1: return launch_adj_t<MODE, 1>(a, blocks, st);
+    case 2: return launch_adj_t<MODE, 2>(a, blocks, st);
+    case 4: return launch_adj_t<MODE, 4>(a, blocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, hipStream_t st) {
+  switch (a.mode) {
+    case kConst6: return launch_fwd_m<kConst6>(a, V, blocks, st);
+    case kDField: return launch_fwd_m<kDField>(a, V, blocks, st);
+    case kField6: return launch_fwd_m<kField6>(a, V, blocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, hipStream_t st) {
+  switch (a.mode) {
+    case kConst6: return launch_adj_m<kConst6>(a, V, blocks, st);
+    case kDField: return launch_adj_m<kDField>(a, V, blocks, st);
+    case kField6: return launch_adj_m<kField6>(a, V, blocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st) {
+  hipLaunchKernelGGL(set_call_kernel, dim3(1), dim3(1), 0, st, dst, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_ws(const CallDev* call, int64_t off, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_zero_ws_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, call, off, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_fields(const CallDev* call, const InstDev* inst, const int* set_inst,
+                              const int64_t* set_ext_off, int n_coef, int set, int64_t ncell,
+                              hipStream_t st) {
+  int64_t blocks = (ncell * n_coef + kBlock - 1) / kBlock;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(pack_fields_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, call, inst,
+                     set_inst, set_ext_off, n_coef, set);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_const(const CallDev* call, int64_t partial_off, const TileDev* tiles,
+                                 int n_tiles, int n_sets, hipStream_t st) {
+  hipLaunchKernelGGL(finalize_const_kernel, dim3(n_sets * 6), dim3(kBlock), 0, st, call, partial_off,
+                     tiles, n_tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_field(const CallDev* call, const InstDev* inst, const int* set_inst_list,
+                                 const int* set_inst_begin, const int64_t* set_grad_off, int n_acc,
+                                 int set, int64_t ncell, hipStream_t st) {
+  int64_t blocks = (ncell * n_acc + kBlock - 1) / kBlock;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(finalize_field_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, call, inst,
+                     set_inst_list, set_inst_begin, set_grad_off, n_acc, set);
+  return hipGetLastError();
+}
+
+}  // namespace ernn

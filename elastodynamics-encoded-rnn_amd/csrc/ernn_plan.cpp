@@ -1,0 +1,708 @@
+// Host side of libernn_hip.so: descriptor validation, workspace layout, tile
+// and event tables, the per-pass launch schedule (forward, checkpointed
+// recompute, reverse-time adjoint), optional hipGraph capture, and the C ABI
+// declared in include/ernn.h.
+//
+// Schedule (K = checkpoint interval, S = ceil(nt_max / K) checkpoints):
+//  forward   steps n = 0 .. nt-1 produce u^{n+1} from (u^n, u^{n-1}).  State m
+//            lives in a 3-slot ring unless m = sK - 1 or m = sK (1 <= s < S),
+//            which go straight into checkpoint s; checkpoint 0 is the zero
+//            initial state (u^-1, u^0) of PI:256-259.
+//  adjoint   for s = S-1 .. 0: recompute u^{sK+1} .. u^{top} (top =
+//            min((s+1)K, nt)) from checkpoint s into K segment slots, then run
+//            the adjoint steps m = top .. sK+1, which read u^m from the segment
+//            and rotate lam through a 3-slot ring.  The forward has no atomics
+//            and a fixed order, so the recomputed states equal the forward's
+//            bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ernn.h"
+#include "ernn_internal.h"
+
+namespace ernn {
+hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, hipStream_t st);
+hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, hipStream_t st);
+hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st);
+hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t st);
+hipError_t launch_fill_ws(const CallDev* call, int64_t off, int64_t n, hipStream_t st);
+hipError_t launch_pack_fields(const CallDev* call, const InstDev* inst, const int* set_inst,
+                              const int64_t* set_ext_off, int n_coef, int set, int64_t ncell,
+                              hipStream_t st);
+hipError_t launch_finalize_const(const CallDev* call, int64_t partial_off, const TileDev* tiles,
+                                 int n_tiles, int n_sets, hipStream_t st);
+hipError_t launch_finalize_field(const CallDev* call, const InstDev* inst, const int* set_inst_list,
+                                 const int* set_inst_begin, const int64_t* set_grad_off, int n_acc,
+                                 int set, int64_t ncell, hipStream_t st);
+}  // namespace ernn
+
+using namespace ernn;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define ERNN_HIP(x)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      return fail(ERNN_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define ERNN_TRY(x)             \
+  do {                          \
+    int r_ = (x);               \
+    if (r_ != ERNN_OK) return r_; \
+  } while (0)
+
+constexpr int kMinRows = 2;
+constexpr int kMaxRows = 4096;
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+struct Tiling {
+  int V = 0, R = 0;
+  int n_tiles = 0, blocks = 0;
+  TileDev* d_tiles = nullptr;
+  SrcEv* d_src = nullptr;
+  RecEv* d_rec = nullptr;
+};
+
+struct Plan {
+  int device = 0;
+  int mode = 0, n_inst = 0, n_sets = 0, P = 1, snapshot = 0, K = 2, S = 1, nt_max = 0;
+  int n_coef = 0, n_acc = 0;
+  bool use_graph = false;
+  std::vector<ernn_instance_t> inst_h;
+  std::vector<ernn_source_t> src_h;
+  std::vector<ernn_receiver_t> rec_h;
+  std::vector<InstDev> inst;
+  InstDev* d_inst = nullptr;
+  CallDev* d_call = nullptr;
+  Tiling fwd, adj;
+  int64_t ws_floats = 0;
+  int64_t partial_off = 0;  // in doubles
+  int64_t partial_cap = 0;  // tiles
+  std::vector<int64_t> trace_off;
+  std::vector<int64_t> set_ext_off, set_grad_off;
+  std::vector<int> set_first, set_list, set_begin;
+  int* d_set_first = nullptr;
+  int* d_set_list = nullptr;
+  int* d_set_begin = nullptr;
+  int64_t* d_set_ext_off = nullptr;
+  int64_t* d_set_grad_off = nullptr;
+  int64_t field_floats = 0, grad_field_doubles = 0;
+  hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;
+  hipStream_t cap_stream = nullptr;
+
+  // slots of one instance's state pool
+  int ring(int r) const { return r; }
+  int ck(int s, int q) const { return 3 + 2 * s + q; }
+  int seg(int q) const { return 3 + 2 * S + q; }
+  int lam(int r) const { return 3 + 2 * S + K + r; }
+  int n_slots() const { return 3 + 2 * S + K + 3; }
+
+  int slot_fwd(int m) const {
+    if (m == -1) return ck(0, 0);
+    if (m == 0) return ck(0, 1);
+    if ((m + 1) % K == 0 && (m + 1) / K < S) return ck((m + 1) / K, 0);
+    if (m % K == 0 && m / K < S) return ck(m / K, 1);
+    return ring(m % 3);
+  }
+  int slot_seg(int s, int m) const {
+    if (m == s * K - 1) return ck(s, 0);
+    if (m == s * K) return ck(s, 1);
+    return seg(m - s * K - 1);
+  }
+};
+
+template <typename T>
+int dev_upload(T** dst, const std::vector<T>& v) {
+  *dst = nullptr;
+  if (v.empty()) return ERNN_OK;
+  ERNN_HIP(hipMalloc(reinterpret_cast<void**>(dst), sizeof(T) * v.size()));
+  ERNN_HIP(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return ERNN_OK;
+}
+
+void free_tiling(Tiling& t) {
+  if (t.d_tiles) (void)hipFree(t.d_tiles);
+  if (t.d_src) (void)hipFree(t.d_src);
+  if (t.d_rec) (void)hipFree(t.d_rec);
+  t = Tiling{};
+}
+
+void drop_graphs(Plan* p) {
+  if (p->g_fwd) (void)hipGraphExecDestroy(p->g_fwd);
+  if (p->g_adj) (void)hipGraphExecDestroy(p->g_adj);
+  p->g_fwd = p->g_adj = nullptr;
+}
+
+int64_t tiles_for(const Plan& p, int V, int R) {
+  int64_t n = 0;
+  for (const auto& I : p.inst_h)
+    n += ((I.nx + R - 1) / R) * (int64_t)((I.ny + kWave * V - 1) / (kWave * V));
+  return n;
+}
+
+int build_tiling(Plan* p, Tiling* t, int V, int R) {
+  if (!(V == 1 || V == 2 || V == 4)) return fail(ERNN_E_INVALID, "tile V must be 1, 2 or 4");
+  if (R < kMinRows || R > kMaxRows) return fail(ERNN_E_INVALID, "tile R out of range [2, 4096]");
+  const int64_t nt64 = tiles_for(*p, V, R);
+  if (nt64 > (int64_t)1 << 30) return fail(ERNN_E_CAPACITY, "too many tiles");
+  std::vector<TileDev> tiles;
+  tiles.reserve((size_t)nt64);
+  std::vector<SrcEv> sev;
+  std::vector<RecEv> rev;
+  const int W = kWave * V;
+  for (int b = 0; b < p->n_inst; ++b) {
+    const ernn_instance_t& I = p->inst_h[b];
+    const int nti = (I.nx + R - 1) / R, ntj = (I.ny + W - 1) / W;
+    // bucket this instance's sources and receivers by local tile, sorted by (i, j)
+    // key: (local tile, row, column); value: index into the source / receiver slice
+    struct Key { int64_t tile; int i, j, idx; };
+    const auto by_key = [](const Key& a, const Key& b) {
+      if (a.tile != b.tile) return a.tile < b.tile;
+      if (a.i != b.i) return a.i < b.i;
+      return a.j < b.j;
+    };
+    std::vector<Key> sk, rk;
+    for (int e = 0; e < I.src_count; ++e) {
+      const ernn_source_t& s = p->src_h[I.src_begin + e];
+      sk.push_back(Key{(int64_t)(s.i / R) * ntj + s.j / W, s.i, s.j, I.src_begin + e});
+    }
+    if (!p->snapshot) {
+      for (int e = 0; e < I.rec_count; ++e) {
+        const ernn_receiver_t& r = p->rec_h[I.rec_begin + e];
+        rk.push_back(Key{(int64_t)(r.i / R) * ntj + r.j / W, r.i, r.j, e});
+      }
+    }
+    std::stable_sort(sk.begin(), sk.end(), by_key);
+    std::stable_sort(rk.begin(), rk.end(), by_key);
+    size_t si = 0, ri = 0;
+    for (int ti = 0; ti < nti; ++ti) {
+      for (int tj = 0; tj < ntj; ++tj) {
+        const int64_t tl = (int64_t)ti * ntj + tj;
+        TileDev td{};
+        td.inst = b;
+        td.i0 = ti * R;
+        td.j0 = tj * W;
+        td.set = I.set;
+        td.src_begin = (int)sev.size();
+        while (si < sk.size() && sk[si].tile == tl) {
+          const ernn_source_t& s = p->src_h[sk[si].idx];
+          sev.push_back(SrcEv{s.i, s.j, s.w, s.comp});
+          ++si;
+        }
+        td.src_end = (int)sev.size();
+        td.rec_begin = (int)rev.size();
+        while (ri < rk.size() && rk[ri].tile == tl) {
+          const ernn_receiver_t& r = p->rec_h[I.rec_begin + rk[ri].idx];
+          rev.push_back(RecEv{r.i, r.j, rk[ri].idx, 0});
+          ++ri;
+        }
+        td.rec_end = (int)rev.size();
+        tiles.push_back(td);
+      }
+    }
+  }
+  free_tiling(*t);
+  t->V = V;
+  t->R = R;
+  t->n_tiles = (int)tiles.size();
+  const int64_t lblocks = (t->n_tiles + kWavesPerBlock - 1) / kWavesPerBlock;
+  t->blocks = (int)round_up(std::max<int64_t>(lblocks, 1), 8);
+  ERNN_TRY(dev_upload(&t->d_tiles, tiles));
+  ERNN_TRY(dev_upload(&t->d_src, sev));
+  ERNN_TRY(dev_upload(&t->d_rec, rev));
+  return ERNN_OK;
+}
+
+void destroy_plan(Plan* p) {
+  if (!p) return;
+  drop_graphs(p);
+  free_tiling(p->fwd);
+  free_tiling(p->adj);
+  if (p->d_inst) (void)hipFree(p->d_inst);
+  if (p->d_call) (void)hipFree(p->d_call);
+  if (p->d_set_first) (void)hipFree(p->d_set_first);
+  if (p->d_set_list) (void)hipFree(p->d_set_list);
+  if (p->d_set_begin) (void)hipFree(p->d_set_begin);
+  if (p->d_set_ext_off) (void)hipFree(p->d_set_ext_off);
+  if (p->d_set_grad_off) (void)hipFree(p->d_set_grad_off);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  delete p;
+}
+
+int validate(const ernn_desc_t* d) {
+  if (!d) return fail(ERNN_E_INVALID, "null descriptor");
+  if (d->n_inst < 1 || d->n_sets < 1) return fail(ERNN_E_INVALID, "need n_inst >= 1 and n_sets >= 1");
+  if (d->param_mode < ERNN_PARAM_CONST6 || d->param_mode > ERNN_PARAM_FIELD6)
+    return fail(ERNN_E_INVALID, "unknown param_mode");
+  if (d->rec_stride < 1) return fail(ERNN_E_INVALID, "rec_stride must be >= 1");
+  if (d->ckpt_every < 2) return fail(ERNN_E_INVALID, "ckpt_every must be >= 2");
+  if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
+  if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
+  if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))
+    return fail(ERNN_E_INVALID, "null source/receiver array");
+  std::vector<int> set_nx(d->n_sets, -1), set_ny(d->n_sets, -1);
+  for (int b = 0; b < d->n_inst; ++b) {
+    const ernn_instance_t& I = d->inst[b];
+    char buf[160];
+    if (I.nx < 1 || I.ny < 1 || I.nx > 65536 || I.ny > 65536) {
+      snprintf(buf, sizeof buf, "instance %d: grid %dx%d outside [1, 65536]", b, I.nx, I.ny);
+      return fail(ERNN_E_INVALID, buf);
+    }
+    if (I.nt < 1 || I.nt > 100000000) return fail(ERNN_E_INVALID, "nt outside [1, 1e8]");
+    if (I.set < 0 || I.set >= d->n_sets) return fail(ERNN_E_INVALID, "instance set out of range");
+    if (!(I.h > 0) || !(I.dt > 0) || !(I.rho > 0) || !std::isfinite(I.h) || !std::isfinite(I.dt) ||
+        !std::isfinite(I.rho))
+      return fail(ERNN_E_INVALID, "h, dt and rho must be finite and positive");
+    if (I.src_begin < 0 || I.src_count < 0 || (int64_t)I.src_begin + I.src_count > d->n_src)
+      return fail(ERNN_E_INVALID, "instance source slice out of range");
+    if (!d->snapshot &&
+        (I.rec_begin < 0 || I.rec_count < 0 || (int64_t)I.rec_begin + I.rec_count > d->n_rec))
+      return fail(ERNN_E_INVALID, "instance receiver slice out of range");
+    for (int e = 0; e < I.src_count; ++e) {
+      const ernn_source_t& s = d->src[I.src_begin + e];
+      if (s.i < 0 || s.i >= I.nx || s.j < 0 || s.j >= I.ny || (s.comp != 0 && s.comp != 1) ||
+          !std::isfinite(s.w)) {
+        snprintf(buf, sizeof buf, "instance %d: source %d at (%d,%d) comp %d invalid", b, e, s.i, s.j, s.comp);
+        return fail(ERNN_E_INVALID, buf);
+      }
+    }
+    if (!d->snapshot) {
+      for (int e = 0; e < I.rec_count; ++e) {
+        const ernn_receiver_t& r = d->rec[I.rec_begin + e];
+        if (r.i < 0 || r.i >= I.nx || r.j < 0 || r.j >= I.ny) {
+          snprintf(buf, sizeof buf, "instance %d: receiver %d at (%d,%d) outside the grid", b, e, r.i, r.j);
+          return fail(ERNN_E_INVALID, buf);
+        }
+      }
+    }
+    if (d->param_mode != ERNN_PARAM_CONST6) {
+      if (set_nx[I.set] < 0) { set_nx[I.set] = I.nx; set_ny[I.set] = I.ny; }
+      else if (set_nx[I.set] != I.nx || set_ny[I.set] != I.ny)
+        return fail(ERNN_E_INVALID, "instances of one field set must share the grid size");
+    }
+  }
+  for (int s = 0; s < d->n_sets; ++s) {
+    bool used = false;
+    for (int b = 0; b < d->n_inst && !used; ++b) used = d->inst[b].set == s;
+    if (!used) return fail(ERNN_E_INVALID, "every parameter set needs at least one instance");
+  }
+  return ERNN_OK;
+}
+
+int build_plan(const ernn_desc_t* d, Plan* p) {
+  ERNN_HIP(hipGetDevice(&p->device));
+  p->mode = d->param_mode;
+  p->n_inst = d->n_inst;
+  p->n_sets = d->n_sets;
+  p->P = d->rec_stride;
+  p->snapshot = d->snapshot ? 1 : 0;
+  p->K = d->ckpt_every;
+  p->use_graph = d->use_graph != 0;
+  p->n_coef = p->mode == ERNN_PARAM_DFIELD ? 1 : (p->mode == ERNN_PARAM_FIELD6 ? 6 : 0);
+  p->n_acc = p->n_coef;
+  p->inst_h.assign(d->inst, d->inst + d->n_inst);
+  if (d->n_src) p->src_h.assign(d->src, d->src + d->n_src);
+  if (d->n_rec && !p->snapshot) p->rec_h.assign(d->rec, d->rec + d->n_rec);
+  p->nt_max = 0;
+  for (const auto& I : p->inst_h) p->nt_max = std::max(p->nt_max, I.nt);
+  p->S = (p->nt_max + p->K - 1) / p->K;
+
+  // sets: first instance, instance lists, external field / gradient offsets
+  p->set_first.assign(p->n_sets, -1);
+  p->set_begin.assign(p->n_sets + 1, 0);
+  for (int b = 0; b < p->n_inst; ++b) {
+    const int s = p->inst_h[b].set;
+    if (p->set_first[s] < 0) p->set_first[s] = b;
+    p->set_begin[s + 1]++;
+  }
+  for (int s = 0; s < p->n_sets; ++s) p->set_begin[s + 1] += p->set_begin[s];
+  p->set_list.assign(p->n_inst, 0);
+  {
+    std::vector<int> fill(p->set_begin.begin(), p->set_begin.end() - 1);
+    for (int b = 0; b < p->n_inst; ++b) p->set_list[fill[p->inst_h[b].set]++] = b;
+  }
+  p->set_ext_off.assign(p->n_sets, 0);
+  p->set_grad_off.assign(p->n_sets, 0);
+  int64_t ext = 0;
+  for (int s = 0; s < p->n_sets; ++s) {
+    const auto& I = p->inst_h[p->set_first[s]];
+    p->set_ext_off[s] = ext;
+    p->set_grad_off[s] = ext;
+    ext += (int64_t)p->n_coef * I.nx * I.ny;
+  }
+  p->field_floats = ext;
+  p->grad_field_doubles = ext;
+
+  // traces
+  p->trace_off.assign(p->n_inst + 1, 0);
+  for (int b = 0; b < p->n_inst; ++b) {
+    const auto& I = p->inst_h[b];
+    const int64_t nf = (I.nt + p->P - 1) / p->P;
+    const int64_t X = p->snapshot ? (int64_t)I.nx * I.ny : (int64_t)I.rec_count;
+    p->trace_off[b + 1] = p->trace_off[b] + 2 * nf * X;
+  }
+
+  // workspace layout (floats, 64-float = 256-byte aligned pieces)
+  p->inst.assign(p->n_inst, InstDev{});
+  int64_t off = 0;
+  for (int b = 0; b < p->n_inst; ++b) {
+    const auto& I = p->inst_h[b];
+    InstDev& D = p->inst[b];
+    D.nx = I.nx;
+    D.ny = I.ny;
+    D.pitch = (int)(kLeftPad + round_up(I.ny, kColAlign));
+    D.nt = I.nt;
+    D.set = I.set;
+    D.n_rec = p->snapshot ? 0 : I.rec_count;
+    D.n_frames = (I.nt + p->P - 1) / p->P;
+    D.hinv2 = (float)(1.0 / (I.h * I.h));
+    D.dt2 = (float)(I.dt * I.dt);
+    D.rho = (float)I.rho;
+    D.gscale = (double)D.dt2 / (double)D.rho;
+    D.field = (int64_t)(I.nx + 3) * D.pitch;
+    D.pool_off = off;
+    off += round_up((int64_t)p->n_slots() * 2 * D.field, 64);
+    D.trace_off = p->trace_off[b];
+    D.wav_off = 0;
+  }
+  {
+    int64_t w = 0;
+    for (int b = 0; b < p->n_inst; ++b) {
+      p->inst[b].wav_off = w;
+      w += p->inst_h[b].nt;
+    }
+  }
+  if (p->n_coef) {
+    std::vector<int64_t> coef_off(p->n_sets);
+    for (int s = 0; s < p->n_sets; ++s) {
+      coef_off[s] = off;
+      off += round_up((int64_t)p->n_coef * p->inst[p->set_first[s]].field, 64);
+    }
+    for (int b = 0; b < p->n_inst; ++b) p->inst[b].coef_off = coef_off[p->inst[b].set];
+    for (int b = 0; b < p->n_inst; ++b) {
+      p->inst[b].acc_off = off;
+      off += round_up((int64_t)p->n_acc * p->inst[b].field, 64);
+    }
+    for (int b = 0; b < p->n_inst; ++b) {
+      p->inst[b].acc64_off = off / 2;
+      off += round_up((int64_t)2 * p->n_acc * p->inst[b].field, 64);
+    }
+  }
+  p->partial_cap = tiles_for(*p, 1, kMinRows);
+  p->partial_off = off / 2;
+  off += round_up(2 * 6 * p->partial_cap, 64);
+  p->ws_floats = off;
+
+  ERNN_TRY(dev_upload(&p->d_inst, p->inst));
+  ERNN_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_call), sizeof(CallDev)));
+  ERNN_HIP(hipMemset(p->d_call, 0, sizeof(CallDev)));
+  ERNN_TRY(dev_upload(&p->d_set_first, p->set_first));
+  ERNN_TRY(dev_upload(&p->d_set_list, p->set_list));
+  ERNN_TRY(dev_upload(&p->d_set_begin, p->set_begin));
+  ERNN_TRY(dev_upload(&p->d_set_ext_off, p->set_ext_off));
+  ERNN_TRY(dev_upload(&p->d_set_grad_off, p->set_grad_off));
+  ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 2, d->fwd_r ? d->fwd_r : 8));
+  ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 2, d->adj_r ? d->adj_r : 8));
+  ERNN_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+  return ERNN_OK;
+}
+
+CallDev to_dev(const ernn_call_t* c) {
+  CallDev v{};
+  v.ws = static_cast<float*>(c->workspace);
+  v.params = c->params;
+  v.field = c->field;
+  v.wav = c->wavelets;
+  v.traces = c->traces;
+  v.gtr = c->grad_traces;
+  v.grad_params = c->grad_params;
+  v.grad_field = c->grad_field;
+  return v;
+}
+
+int check_call(const Plan* p, const ernn_call_t* c, bool adjoint) {
+  if (!c) return fail(ERNN_E_INVALID, "null call");
+  if (!c->workspace) return fail(ERNN_E_INVALID, "null workspace");
+  if ((reinterpret_cast<uintptr_t>(c->workspace) & 255) != 0)
+    return fail(ERNN_E_INVALID, "workspace must be 256-byte aligned");
+  if (!c->wavelets) return fail(ERNN_E_INVALID, "null wavelets");
+  if (p->mode != ERNN_PARAM_FIELD6 && !c->params) return fail(ERNN_E_INVALID, "null params");
+  if (p->mode != ERNN_PARAM_CONST6 && !c->field) return fail(ERNN_E_INVALID, "null field");
+  if (!adjoint && !c->traces) return fail(ERNN_E_INVALID, "null traces");
+  if (adjoint) {
+    if (!c->grad_traces) return fail(ERNN_E_INVALID, "null grad_traces");
+    if (p->mode != ERNN_PARAM_FIELD6 && !c->grad_params) return fail(ERNN_E_INVALID, "null grad_params");
+    if (p->mode != ERNN_PARAM_CONST6 && !c->grad_field) return fail(ERNN_E_INVALID, "null grad_field");
+  }
+  return ERNN_OK;
+}
+
+StepArgs base_args(const Plan* p, const Tiling& t) {
+  StepArgs a{};
+  a.inst = p->d_inst;
+  a.tiles = t.d_tiles;
+  a.src = t.d_src;
+  a.rec = t.d_rec;
+  a.call = p->d_call;
+  a.partial_off = p->partial_off;
+  a.n_tiles = t.n_tiles;
+  a.mode = p->mode;
+  a.stride = p->P;
+  a.snapshot = p->snapshot;
+  a.rows = t.R;
+  return a;
+}
+
+int enqueue_pack(const Plan* p, hipStream_t st) {
+  for (int s = 0; s < p->n_sets && p->n_coef; ++s) {
+    const auto& I = p->inst_h[p->set_first[s]];
+    ERNN_HIP(launch_pack_fields(p->d_call, p->d_inst, p->d_set_first, p->d_set_ext_off, p->n_coef, s,
+                                (int64_t)I.nx * I.ny, st));
+  }
+  return ERNN_OK;
+}
+
+int enqueue_forward(const Plan* p, hipStream_t st) {
+  ERNN_TRY(enqueue_pack(p, st));
+  StepArgs a = base_args(p, p->fwd);
+  a.record = 1;
+  for (int n = 0; n < p->nt_max; ++n) {
+    a.step = n;
+    a.s_prev = p->slot_fwd(n - 1);
+    a.s_cur = p->slot_fwd(n);
+    a.s_next = p->slot_fwd(n + 1);
+    ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, st));
+  }
+  return ERNN_OK;
+}
+
+int enqueue_adjoint(const Plan* p, hipStream_t st) {
+  ERNN_TRY(enqueue_pack(p, st));
+  // zero: lam ring, per-tile partials, fp64 per-cell accumulators
+  for (int b = 0; b < p->n_inst; ++b) {
+    const InstDev& D = p->inst[b];
+    ERNN_HIP(launch_fill_ws(p->d_call, D.pool_off + (int64_t)p->lam(0) * 2 * D.field, 3 * 2 * D.field, st));
+    if (p->n_acc)
+      ERNN_HIP(launch_fill_ws(p->d_call, 2 * D.acc64_off, 2 * (int64_t)p->n_acc * D.field, st));
+  }
+  ERNN_HIP(launch_fill_ws(p->d_call, 2 * p->partial_off, 2 * 6 * (int64_t)p->adj.n_tiles, st));
+
+  StepArgs f = base_args(p, p->fwd);
+  f.record = 0;
+  StepArgs a = base_args(p, p->adj);
+  for (int s = p->S - 1; s >= 0; --s) {
+    const int top = std::min((s + 1) * p->K, p->nt_max);
+    const int bot = s * p->K + 1;
+    for (int n = s * p->K; n < top; ++n) {
+      f.step = n;
+      f.s_prev = p->slot_seg(s, n - 1);
+      f.s_cur = p->slot_seg(s, n);
+      f.s_next = p->slot_seg(s, n + 1);
+      ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, st));
+    }
+    a.seg_top = top;
+    a.seg_bot = bot;
+    for (int m = top; m >= bot; --m) {
+      a.step = m;
+      a.s_prev = p->lam((m + 2) % 3);
+      a.s_cur = p->lam((m + 1) % 3);
+      a.s_next = p->lam(m % 3);
+      a.s_u = p->seg(m - s * p->K - 1);
+      ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, st));
+    }
+  }
+  if (p->mode != ERNN_PARAM_FIELD6)
+    ERNN_HIP(launch_finalize_const(p->d_call, p->partial_off, p->adj.d_tiles, p->adj.n_tiles, p->n_sets, st));
+  if (p->n_acc) {
+    for (int s = 0; s < p->n_sets; ++s) {
+      const auto& I = p->inst_h[p->set_first[s]];
+      ERNN_HIP(launch_finalize_field(p->d_call, p->d_inst, p->d_set_list, p->d_set_begin, p->d_set_grad_off,
+                                     p->n_acc, s, (int64_t)I.nx * I.ny, st));
+    }
+  }
+  return ERNN_OK;
+}
+
+using Enqueue = int (*)(const Plan*, hipStream_t);
+
+// Capture an enqueue function into a graph on the plan's private stream.
+int capture(Plan* p, Enqueue fn, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  ERNN_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+  const int r = fn(p, p->cap_stream);
+  const hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+  if (r != ERNN_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return fail(ERNN_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(ERNN_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  return ERNN_OK;
+}
+
+int run_pass(Plan* p, const ernn_call_t* c, hipStream_t st, bool adjoint) {
+  ERNN_HIP(hipSetDevice(p->device));
+  ERNN_HIP(launch_set_call(p->d_call, to_dev(c), st));
+  Enqueue fn = adjoint ? enqueue_adjoint : enqueue_forward;
+  if (!p->use_graph) return fn(p, st);
+  hipGraphExec_t* g = adjoint ? &p->g_adj : &p->g_fwd;
+  // Every captured kernel reaches the workspace and the per-call buffers only
+  // through d_call, so one capture serves every call.
+  if (!*g) ERNN_TRY(capture(p, fn, g));
+  ERNN_HIP(hipGraphLaunch(*g, st));
+  return ERNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ernn_abi_version(void) { return ERNN_ABI_VERSION; }
+
+const char* ernn_last_error(void) { return g_err.c_str(); }
+
+int ernn_plan_create(const ernn_desc_t* desc, void** plan_out) {
+  if (!plan_out) return fail(ERNN_E_INVALID, "null plan_out");
+  *plan_out = nullptr;
+  ERNN_TRY(validate(desc));
+  Plan* p = new Plan();
+  const int r = build_plan(desc, p);
+  if (r != ERNN_OK) {
+    destroy_plan(p);
+    return r;
+  }
+  *plan_out = p;
+  return ERNN_OK;
+}
+
+int ernn_plan_destroy(void* plan) {
+  destroy_plan(static_cast<Plan*>(plan));
+  return ERNN_OK;
+}
+
+int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
+  if (!plan || !info) return fail(ERNN_E_INVALID, "null argument");
+  const Plan* p = static_cast<const Plan*>(plan);
+  std::memset(info, 0, sizeof *info);
+  info->workspace_bytes = p->ws_floats * (int64_t)sizeof(float);
+  info->trace_floats = p->trace_off.back();
+  info->grad_field_doubles = p->grad_field_doubles;
+  info->field_floats = p->field_floats;
+  info->n_frames_max = (p->nt_max + p->P - 1) / p->P;
+  info->ckpt_every = p->K;
+  info->n_ckpt = p->S;
+  info->fwd_v = p->fwd.V;
+  info->fwd_r = p->fwd.R;
+  info->adj_v = p->adj.V;
+  info->adj_r = p->adj.R;
+  info->fwd_tiles = p->fwd.n_tiles;
+  info->adj_tiles = p->adj.n_tiles;
+  info->use_graph = p->use_graph ? 1 : 0;
+  return ERNN_OK;
+}
+
+int ernn_trace_offsets(const void* plan, int64_t* trace_off) {
+  if (!plan || !trace_off) return fail(ERNN_E_INVALID, "null argument");
+  const Plan* p = static_cast<const Plan*>(plan);
+  std::copy(p->trace_off.begin(), p->trace_off.end(), trace_off);
+  return ERNN_OK;
+}
+
+int ernn_workspace_init(void* plan, void* workspace, void* stream) {
+  if (!plan || !workspace) return fail(ERNN_E_INVALID, "null argument");
+  Plan* p = static_cast<Plan*>(plan);
+  if ((reinterpret_cast<uintptr_t>(workspace) & 255) != 0)
+    return fail(ERNN_E_INVALID, "workspace must be 256-byte aligned");
+  ERNN_HIP(hipSetDevice(p->device));
+  ERNN_HIP(launch_fill_zero(static_cast<float*>(workspace), p->ws_floats, static_cast<hipStream_t>(stream)));
+  return ERNN_OK;
+}
+
+int ernn_forward(void* plan, const ernn_call_t* call, void* stream) {
+  if (!plan) return fail(ERNN_E_INVALID, "null plan");
+  Plan* p = static_cast<Plan*>(plan);
+  ERNN_TRY(check_call(p, call, false));
+  return run_pass(p, call, static_cast<hipStream_t>(stream), false);
+}
+
+int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream) {
+  if (!plan) return fail(ERNN_E_INVALID, "null plan");
+  Plan* p = static_cast<Plan*>(plan);
+  ERNN_TRY(check_call(p, call, true));
+  return run_pass(p, call, static_cast<hipStream_t>(stream), true);
+}
+
+int ernn_plan_set_tiling(void* plan, int which, int v, int r) {
+  if (!plan || (which != 0 && which != 1)) return fail(ERNN_E_INVALID, "bad plan or which");
+  Plan* p = static_cast<Plan*>(plan);
+  ERNN_HIP(hipSetDevice(p->device));
+  ERNN_HIP(hipDeviceSynchronize());
+  drop_graphs(p);
+  return build_tiling(p, which == 0 ? &p->fwd : &p->adj, v, r);
+}
+
+int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
+                    float* ms_out) {
+  if (!plan || !ms_out || steps < 1 || (which != 0 && which != 1))
+    return fail(ERNN_E_INVALID, "bad argument");
+  Plan* p = static_cast<Plan*>(plan);
+  ERNN_TRY(check_call(p, call, which == 1));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  ERNN_HIP(hipSetDevice(p->device));
+  ERNN_HIP(launch_set_call(p->d_call, to_dev(call), st));
+  hipEvent_t e0, e1;
+  ERNN_HIP(hipEventCreate(&e0));
+  ERNN_HIP(hipEventCreate(&e1));
+  ERNN_HIP(hipEventRecord(e0, st));
+  const int nsteps = std::min(steps, p->nt_max);
+  if (which == 0) {
+    StepArgs a = base_args(p, p->fwd);
+    a.record = 1;
+    for (int n = 0; n < nsteps; ++n) {
+      a.step = n;
+      a.s_prev = p->slot_fwd(n - 1);
+      a.s_cur = p->slot_fwd(n);
+      a.s_next = p->slot_fwd(n + 1);
+      ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, st));
+    }
+  } else {
+    StepArgs a = base_args(p, p->adj);
+    a.seg_top = p->nt_max;
+    a.seg_bot = 1;
+    for (int q = 0; q < nsteps; ++q) {
+      const int m = p->nt_max - q;
+      a.step = m;
+      a.s_prev = p->lam((m + 2) % 3);
+      a.s_cur = p->lam((m + 1) % 3);
+      a.s_next = p->lam(m % 3);
+      a.s_u = p->seg(q % p->K);
+      ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, st));
+    }
+  }
+  ERNN_HIP(hipEventRecord(e1, st));
+  ERNN_HIP(hipEventSynchronize(e1));
+  ERNN_HIP(hipEventElapsedTime(ms_out, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ERNN_OK;
+}
+
+}  // extern "C"

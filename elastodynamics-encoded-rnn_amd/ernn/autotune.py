@@ -1,0 +1,79 @@
+"""Autotuning: checkpoint interval under a memory cap, and tile shapes by timing.
+
+Checkpoint interval K.  Each instance keeps 3 ring slots, 2 states per
+checkpoint (S = ceil(nt / K) checkpoints), K recompute slots and 3 adjoint
+slots, each slot two padded fields.  Recompute cost is one extra forward
+whatever K is, so K only trades memory: slots(K) = 6 + 2 ceil(nt/K) + K is
+minimal near K = sqrt(2 nt).  ``choose_ckpt`` returns that K, or raises if
+even the minimum exceeds the cap.
+
+Tiles.  ``tune_tiles`` times every candidate (V columns per lane, R rows per
+wavefront) of the forward and the adjoint kernel on the real problem for a
+few steps (hipEvent timing inside the library) and keeps the fastest.
+"""
+from __future__ import annotations
+
+import math
+from typing import Iterable, Optional, Sequence, Tuple
+
+import torch
+
+from .problem import Survey
+
+LEFT_PAD = 32
+COL_ALIGN = 256
+
+
+def padded_field_floats(nx: int, ny: int) -> int:
+    pitch = LEFT_PAD + (ny + COL_ALIGN - 1) // COL_ALIGN * COL_ALIGN
+    return (nx + 3) * pitch
+
+
+def slots(nt: int, K: int) -> int:
+    return 6 + 2 * math.ceil(nt / K) + K
+
+
+def state_bytes(survey: Survey, K: int) -> int:
+    nt = survey.nt_max
+    return sum(slots(nt, K) * 2 * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
+
+
+def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
+    """Checkpoint interval minimizing state memory; raises if it exceeds the cap."""
+    nt = survey.nt_max
+    if nt < 4:
+        return 2
+    best = min(range(2, nt + 1), key=lambda K: (slots(nt, K), -K))
+    if mem_cap_bytes is not None and state_bytes(survey, best) > mem_cap_bytes:
+        raise MemoryError(f"checkpointed state needs {state_bytes(survey, best) / 2**30:.2f} GiB "
+                          f"> cap {mem_cap_bytes / 2**30:.2f} GiB")
+    return best
+
+
+DEFAULT_CANDIDATES: Tuple[Tuple[int, int], ...] = (
+    (1, 4), (1, 8), (1, 16), (2, 4), (2, 8), (2, 16), (2, 32), (4, 4), (4, 8), (4, 16), (4, 32))
+
+
+def tune_tiles(engine, params, field, wavelets, steps: int = 20,
+               candidates: Sequence[Tuple[int, int]] = DEFAULT_CANDIDATES,
+               which: Iterable[int] = (0, 1), repeats: int = 2) -> dict:
+    """Time candidates for the forward (0) and adjoint (1) kernels; keep the best.
+
+    Returns {which: (V, R, ms_per_step, {candidate: ms})}.
+    """
+    ws = engine.acquire_workspace()
+    traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)
+    result = {}
+    for w in which:
+        times = {}
+        for (v, r) in candidates:
+            engine.set_tiling(w, v, r)
+            engine.time_steps(w, 2, params, field, wavelets, ws, traces, traces)   # warm-up
+            ms = min(engine.time_steps(w, steps, params, field, wavelets, ws, traces, traces)
+                     for _ in range(repeats))
+            times[(v, r)] = ms / steps
+        best = min(times, key=times.get)
+        engine.set_tiling(w, *best)
+        result[w] = (best[0], best[1], times[best], times)
+    engine.release_workspace(ws)
+    return result

@@ -1,0 +1,211 @@
+"""Engine: one C-ABI plan (libernn_hip.so) bound to torch device memory.
+
+An Engine owns the plan built from a Survey and a small pool of workspaces.
+All buffers it passes to the library are torch CUDA tensors, so PyTorch's
+caching allocator sees every byte; the library itself allocates only its
+tables at plan creation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from .problem import MODES, Survey
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+class Engine:
+    """Plan + workspaces for one Survey on one CUDA device."""
+
+    def __init__(self, survey: Survey, device: Optional[torch.device] = None, ckpt_every: Optional[int] = None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("ernn.Engine needs a ROCm GPU: the HIP kernels are the only implementation")
+        self.survey = survey
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        if self.device.type != "cuda":
+            raise RuntimeError(f"ernn.Engine needs a CUDA (ROCm) device, got {self.device}")
+        self.mode = MODES[survey.mode]
+        self.n_coef = {0: 0, 1: 1, 2: 6}[self.mode]
+        K = ckpt_every or survey.ckpt_every
+        if not K:
+            from .autotune import choose_ckpt
+            K = choose_ckpt(survey)
+        self._build(K)
+        self._pool: List[torch.Tensor] = []
+
+    # ------------------------------------------------------------------ plan
+    def _build(self, K: int) -> None:
+        sv = self.survey
+        shots = sv.shots
+        n = len(shots)
+        inst = (_lib.Instance * n)()
+        src_list, rec_list = [], []
+        for b, s in enumerate(shots):
+            I = inst[b]
+            I.nx, I.ny, I.nt, I.set = s.nx, s.ny, s.nt, s.set
+            I.h, I.dt, I.rho = s.h, s.dt, s.rho
+            I.src_begin, I.src_count = len(src_list), len(s.sources)
+            src_list.extend(s.sources)
+            if sv.snapshot:
+                I.rec_begin, I.rec_count = 0, 0
+            else:
+                r = np.asarray(s.receivers, dtype=np.int64).reshape(-1, 2)
+                I.rec_begin, I.rec_count = len(rec_list), len(r)
+                rec_list.extend(map(tuple, r))
+        src = (_lib.Source * max(len(src_list), 1))()
+        for k, (i, j, w, comp) in enumerate(src_list):
+            src[k].i, src[k].j, src[k].w, src[k].comp = int(i), int(j), float(w), int(comp)
+        rec = (_lib.Receiver * max(len(rec_list), 1))()
+        for k, (i, j) in enumerate(rec_list):
+            rec[k].i, rec[k].j = int(i), int(j)
+        d = _lib.Desc()
+        d.n_inst, d.n_sets, d.param_mode = n, sv.n_sets, self.mode
+        d.rec_stride, d.snapshot, d.ckpt_every = sv.rec_stride, int(sv.snapshot), int(K)
+        d.fwd_v, d.fwd_r, d.adj_v, d.adj_r = sv.tiles
+        d.use_graph = int(sv.use_graph)
+        d.inst = inst
+        d.src, d.n_src = src, len(src_list)
+        d.rec, d.n_rec = rec, len(rec_list)
+        handle = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().ernn_plan_create(C.byref(d), C.byref(handle)), "ernn_plan_create")
+        self._plan = handle
+        self.info = _lib.PlanInfo()
+        _lib.check(_lib.lib().ernn_plan_info(self._plan, C.byref(self.info)), "ernn_plan_info")
+        off = (C.c_int64 * (n + 1))()
+        _lib.check(_lib.lib().ernn_trace_offsets(self._plan, off), "ernn_trace_offsets")
+        self.trace_off = [int(x) for x in off]
+
+    def close(self) -> None:
+        if getattr(self, "_plan", None):
+            torch.cuda.synchronize(self.device)
+            _lib.lib().ernn_plan_destroy(self._plan)
+            self._plan = None
+        self._pool.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------- buffers
+    @property
+    def workspace_bytes(self) -> int:
+        return int(self.info.workspace_bytes)
+
+    @property
+    def trace_floats(self) -> int:
+        return int(self.info.trace_floats)
+
+    def acquire_workspace(self) -> torch.Tensor:
+        if self._pool:
+            return self._pool.pop()
+        ws = torch.empty(self.workspace_bytes, dtype=torch.uint8, device=self.device)
+        if ws.data_ptr() % 256:
+            raise RuntimeError("workspace allocation is not 256-byte aligned")
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().ernn_workspace_init(self._plan, C.c_void_p(ws.data_ptr()),
+                                                      C.c_void_p(_lib.stream_handle(self.device))),
+                       "ernn_workspace_init")
+        return ws
+
+    def release_workspace(self, ws: torch.Tensor) -> None:
+        if len(self._pool) < 2:
+            self._pool.append(ws)
+
+    def _call(self, ws, params=None, field=None, wavelets=None, traces=None, grad_traces=None,
+              grad_params=None, grad_field=None) -> _lib.Call:
+        c = _lib.Call()
+        c.workspace = _ptr(ws)
+        c.params = _ptr(params)
+        c.field = _ptr(field)
+        c.wavelets = _ptr(wavelets)
+        c.traces = _ptr(traces)
+        c.grad_traces = _ptr(grad_traces)
+        c.grad_params = _ptr(grad_params)
+        c.grad_field = _ptr(grad_field)
+        return c
+
+    def _check_inputs(self, params, field, wavelets):
+        sv = self.survey
+        dev = self.device
+        def chk(t, name, numel):
+            if t is None:
+                raise ValueError(f"{name} is required for mode {sv.mode}")
+            if t.device != dev or t.dtype != torch.float32 or not t.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous float32 tensor on {dev}")
+            if t.numel() != numel:
+                raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
+        if self.mode != 2:
+            chk(params, "params", sv.n_sets * 6)
+        if self.mode != 0:
+            chk(field, "field", int(self.info.field_floats))
+        chk(wavelets, "wavelets", sum(s.nt for s in sv.shots))
+
+    # ----------------------------------------------------------------- passes
+    def forward(self, params, field, wavelets, ws: torch.Tensor, traces: Optional[torch.Tensor] = None):
+        """Run the forward recurrence; returns the flat trace tensor."""
+        self._check_inputs(params, field, wavelets)
+        if traces is None:
+            traces = torch.empty(self.trace_floats, dtype=torch.float32, device=self.device)
+        c = self._call(ws, params, field, wavelets, traces=traces)
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().ernn_forward(self._plan, C.byref(c),
+                                               C.c_void_p(_lib.stream_handle(self.device))), "ernn_forward")
+        return traces
+
+    def adjoint(self, params, field, wavelets, grad_traces: torch.Tensor, ws: torch.Tensor
+                ) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor]]:
+        """Gradient of J given dJ/d traces; returns (grad_params, grad_field) in fp64."""
+        self._check_inputs(params, field, wavelets)
+        if grad_traces.numel() != self.trace_floats or grad_traces.dtype != torch.float32:
+            raise ValueError("grad_traces must be float32 with the trace layout")
+        grad_traces = grad_traces.contiguous()
+        gp = torch.empty(self.survey.n_sets * 6, dtype=torch.float64, device=self.device) if self.mode != 2 else None
+        gf = (torch.empty(int(self.info.grad_field_doubles), dtype=torch.float64, device=self.device)
+              if self.mode != 0 else None)
+        c = self._call(ws, params, field, wavelets, grad_traces=grad_traces, grad_params=gp, grad_field=gf)
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().ernn_adjoint(self._plan, C.byref(c),
+                                               C.c_void_p(_lib.stream_handle(self.device))), "ernn_adjoint")
+        return gp, gf
+
+    # ----------------------------------------------------------- layouts
+    def shot_traces(self, traces: torch.Tensor, b: int) -> torch.Tensor:
+        """View of shot b's traces: (2, n_frames, n_rec) or (2, n_frames, nx, ny)."""
+        s = self.survey.shots[b]
+        nf = self.survey.n_frames(s)
+        block = traces[self.trace_off[b]: self.trace_off[b + 1]]
+        if self.survey.snapshot:
+            return block.view(2, nf, s.nx, s.ny)
+        return block.view(2, nf, -1)
+
+    def stacked(self, traces: torch.Tensor) -> torch.Tensor:
+        """(n_shots, 2, n_frames, X...) view when all shots share one block shape."""
+        shapes = {tuple(self.shot_traces(traces, b).shape) for b in range(len(self.survey.shots))}
+        if len(shapes) != 1:
+            raise ValueError("shots have different trace shapes; use shot_traces()")
+        return traces.view(len(self.survey.shots), *shapes.pop())
+
+    # ----------------------------------------------------------- tuning
+    def set_tiling(self, which: int, v: int, r: int) -> None:
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().ernn_plan_set_tiling(self._plan, which, v, r), "ernn_plan_set_tiling")
+            _lib.check(_lib.lib().ernn_plan_info(self._plan, C.byref(self.info)), "ernn_plan_info")
+
+    def time_steps(self, which: int, steps: int, params, field, wavelets, ws, traces, grad_traces) -> float:
+        c = self._call(ws, params, field, wavelets, traces=traces, grad_traces=grad_traces)
+        ms = C.c_float()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().ernn_time_steps(self._plan, which, steps, C.byref(c),
+                                                  C.c_void_p(_lib.stream_handle(self.device)), C.byref(ms)),
+                       "ernn_time_steps")
+        return float(ms.value)

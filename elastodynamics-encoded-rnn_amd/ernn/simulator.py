@@ -1,0 +1,304 @@
+"""nn.Module front ends with the reference's interface.
+
+CFRPWaveSimulatorRicker               mirrors Parameter_inversion/Inversion_adam.py:132-307
+SpatiallyVaryingAnisotropicSimulator  mirrors Damage_inversion/Inversion_adam.py:198-393
+ERNN                                  receiver-trace module for the benchmark surveys
+
+The first two keep the reference's constructor arguments, parameters
+(log-stiffness, PI:166-171 / DI:245-258), bounds, buffers, helper methods
+and forward() contract: forward(save_all_steps=False) returns
+(ux_fields, uy_fields) of shape (1, ceil(nt / P), nx, ny) holding u^{kP+1}
+(PI:294-305).  The time loop and its gradient run in the HIP library through
+ERNNFunction instead of ~60 tensor ops per step; get_material_params() stays
+in torch, so autograd chains exp/clamp (PI:223-232) onto the HIP gradient.
+
+Differences from the reference, by design:
+  * the Gaussian load is passed as a sparse list of cells whose weight exceeds
+    1e-12 of the peak (the dropped tail is below FP32 resolution of the force);
+  * the per-cell path adds the force untransposed (PI semantics); the
+    reference's damage path transposes it (DI:169-170), which is the same for
+    its centred source on a square grid and mirrors the source otherwise.
+"""
+from __future__ import annotations
+
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from .engine import Engine
+from .function import ernn_traces
+from .problem import Shot, Survey, gaussian_sources
+
+
+def _gaussian_dist(nx, ny, h, x0, y0, a):
+    """PI:185-195 on the CPU in FP32 (the module is moved with .to(device))."""
+    Lx, Ly = h * (nx - 1), h * (ny - 1)
+    x = torch.linspace(-Lx / 2, Lx / 2, nx)
+    y = torch.linspace(-Ly / 2, Ly / 2, ny)
+    X, Y = torch.meshgrid(x, y, indexing="ij")
+    return (a / np.pi) * torch.exp(-a * ((X - x0) ** 2 + (Y - y0) ** 2))
+
+
+def _ricker_signal(nt, dt, f0, t0, amplitude):
+    """PI:210-217: FP64 samples stored as a (1, nt) FP32 buffer."""
+    t = np.arange(nt) * dt
+    q = (np.pi * f0 * (t - t0)) ** 2
+    return torch.tensor(amplitude * (1 - 2 * q) * np.exp(-q), dtype=torch.float32).unsqueeze(0)
+
+
+class _SnapshotEngines:
+    """Engines per (device, recording stride) for a one-shot snapshot survey."""
+
+    def __init__(self):
+        self._engines: Dict[Tuple[str, int], Engine] = {}
+
+    def get(self, key, make_survey) -> Engine:
+        if key not in self._engines:
+            self._engines[key] = Engine(make_survey(), device=torch.device(key[0]))
+        return self._engines[key]
+
+
+class CFRPWaveSimulatorRicker(nn.Module):
+    """Six-constant ERNN (PI:132-307) on the HIP forward/adjoint."""
+
+    def __init__(self, domain_shape, h, dt, nt, primary_time_stride,
+                 source_component='y', source_f0=100e3, source_t0=6e-6, source_amplitude=1.0,
+                 C11_init=100e9, C22_init=10e9, C12_init=8e9,
+                 C16_init=25e9, C26_init=3e9, C66_init=12e9,
+                 rho_known=1600.0, gaussian_param=5e6, source_position=(0.0, 0.0),
+                 ckpt_every: int = 0):
+        super().__init__()
+        self.domain_shape = tuple(domain_shape)
+        self.h = h
+        self.dt = dt
+        self.nt = nt
+        self.primary_time_stride = primary_time_stride
+        self.source_component = source_component
+        self.source_f0 = source_f0
+        self.source_t0 = source_t0
+        self.source_amplitude = source_amplitude
+        self.gaussian_param = gaussian_param
+        self.source_position = source_position
+        self.log_C11 = nn.Parameter(torch.log(torch.tensor(C11_init, dtype=torch.float32)))
+        self.log_C22 = nn.Parameter(torch.log(torch.tensor(C22_init, dtype=torch.float32)))
+        self.log_C12 = nn.Parameter(torch.log(torch.tensor(C12_init, dtype=torch.float32)))
+        self.log_C16 = nn.Parameter(torch.log(torch.tensor(C16_init, dtype=torch.float32)))
+        self.log_C26 = nn.Parameter(torch.log(torch.tensor(C26_init, dtype=torch.float32)))
+        self.log_C66 = nn.Parameter(torch.log(torch.tensor(C66_init, dtype=torch.float32)))
+        self.register_buffer('rho', torch.tensor(rho_known, dtype=torch.float32))
+        self.rho_known = rho_known
+        self.C11_bounds = (1e9, 10000e9)
+        self.C22_bounds = (1e9, 10000e9)
+        self.C12_bounds = (1e9, 10000e9)
+        self.C16_bounds = (1e9, 10000e9)
+        self.C26_bounds = (1e9, 10000e9)
+        self.C66_bounds = (1e9, 10000e9)
+        nx, ny = self.domain_shape
+        self.gaussian_distribution = _gaussian_dist(nx, ny, h, source_position[0], source_position[1],
+                                                    gaussian_param)
+        self.register_buffer('gaussian_dist', self.gaussian_distribution)
+        self.register_buffer('source_signal', _ricker_signal(nt, dt, source_f0, source_t0, source_amplitude))
+        self._ckpt_every = ckpt_every
+        self._engines = _SnapshotEngines()
+
+    def get_material_params(self):
+        """PI:223-232."""
+        C11 = torch.clamp(torch.exp(self.log_C11), *self.C11_bounds)
+        C22 = torch.clamp(torch.exp(self.log_C22), *self.C22_bounds)
+        C12 = torch.clamp(torch.exp(self.log_C12), *self.C12_bounds)
+        C16 = torch.clamp(torch.exp(self.log_C16), *self.C16_bounds)
+        C26 = torch.clamp(torch.exp(self.log_C26), *self.C26_bounds)
+        C66 = torch.clamp(torch.exp(self.log_C66), *self.C66_bounds)
+        return C11, C22, C12, C16, C26, C66, self.rho
+
+    def get_current_params_values(self):
+        """PI:234-238 (GPa)."""
+        C11, C22, C12, C16, C26, C66, rho = self.get_material_params()
+        return (C11.item() / 1e9, C22.item() / 1e9, C12.item() / 1e9,
+                C16.item() / 1e9, C26.item() / 1e9, C66.item() / 1e9, rho.item())
+
+    def _survey(self, stride: int) -> Survey:
+        nx, ny = self.domain_shape
+        comp = 0 if self.source_component == 'x' else 1
+        g = self.gaussian_dist.detach().float().cpu().numpy()
+        keep = np.argwhere(g > 1e-12 * g.max())
+        src = [(int(i), int(j), float(g[i, j]), comp) for i, j in keep]
+        shot = Shot(nx=nx, ny=ny, nt=self.nt, h=self.h, dt=self.dt, rho=float(self.rho.item()),
+                    sources=src, wavelet=self.source_signal[0].detach().cpu().numpy())
+        return Survey([shot], mode="const6", rec_stride=stride, snapshot=True,
+                      ckpt_every=self._ckpt_every)
+
+    def engine(self, save_all_steps: bool = False) -> Engine:
+        stride = 1 if save_all_steps else self.primary_time_stride
+        dev = self.source_signal.device
+        return self._engines.get((str(dev), stride), lambda: self._survey(stride))
+
+    def forward(self, save_all_steps=False):
+        """PI:240-307: returns (ux_fields, uy_fields), each (1, ceil(nt/P), nx, ny)."""
+        eng = self.engine(save_all_steps)
+        C11, C22, C12, C16, C26, C66, rho = self.get_material_params()
+        params = torch.stack([C11, C22, C12, C16, C26, C66]).to(torch.float32).contiguous()
+        traces = ernn_traces(eng, params=params, wavelets=self.source_signal[0].contiguous())
+        blk = eng.shot_traces(traces, 0)            # (2, n_frames, nx, ny)
+        return blk[0].unsqueeze(0), blk[1].unsqueeze(0)
+
+
+def total_variation_2d(field):
+    """DI:179-195."""
+    dx = field[:, 1:, :] - field[:, :-1, :]
+    dy = field[:, :, 1:] - field[:, :, :-1]
+    return torch.sum(torch.abs(dx)) + torch.sum(torch.abs(dy))
+
+
+class SpatiallyVaryingAnisotropicSimulator(nn.Module):
+    """Per-cell six-field ERNN (DI:198-393) on the HIP forward/adjoint."""
+
+    def __init__(self, domain_shape, h, dt, nt, primary_time_stride,
+                 source_component='y', source_f0=100e3, source_t0=10e-6, source_amplitude=1e10,
+                 total_time=60e-6,
+                 C11_init=130.74e9, C22_init=11.50e9, C12_init=10.49e9,
+                 C16_init=30.67e9, C26_init=3.75e9, C66_init=14.77e9,
+                 rho_known=1610.0, gaussian_param=5e6, use_log_params=True,
+                 ckpt_every: int = 0):
+        super().__init__()
+        self.domain_shape = tuple(domain_shape)
+        self.h = h
+        self.dt = dt
+        self.nt = nt
+        self.total_time = total_time
+        self.primary_time_stride = primary_time_stride
+        self.use_log_params = use_log_params
+        self.source_component = source_component
+        self.source_f0 = source_f0
+        self.source_t0 = source_t0
+        self.source_amplitude = source_amplitude
+        self.gaussian_param = gaussian_param
+        nx, ny = self.domain_shape
+        if use_log_params:  # DI:245-251
+            self.log_C11 = nn.Parameter(torch.full((1, nx, ny), np.log(C11_init), dtype=torch.float32))
+            self.log_C22 = nn.Parameter(torch.full((1, nx, ny), np.log(C22_init), dtype=torch.float32))
+            self.log_C12 = nn.Parameter(torch.full((1, nx, ny), np.log(C12_init), dtype=torch.float32))
+            self.log_C16 = nn.Parameter(torch.full((1, nx, ny), np.log(abs(C16_init) + 1e6), dtype=torch.float32))
+            self.log_C26 = nn.Parameter(torch.full((1, nx, ny), np.log(abs(C26_init) + 1e6), dtype=torch.float32))
+            self.log_C66 = nn.Parameter(torch.full((1, nx, ny), np.log(C66_init), dtype=torch.float32))
+        else:  # DI:252-258
+            self.C11 = nn.Parameter(torch.full((1, nx, ny), C11_init, dtype=torch.float32))
+            self.C22 = nn.Parameter(torch.full((1, nx, ny), C22_init, dtype=torch.float32))
+            self.C12 = nn.Parameter(torch.full((1, nx, ny), C12_init, dtype=torch.float32))
+            self.C16 = nn.Parameter(torch.full((1, nx, ny), C16_init, dtype=torch.float32))
+            self.C26 = nn.Parameter(torch.full((1, nx, ny), C26_init, dtype=torch.float32))
+            self.C66 = nn.Parameter(torch.full((1, nx, ny), C66_init, dtype=torch.float32))
+        self.register_buffer('rho', torch.tensor(rho_known, dtype=torch.float32))
+        self.rho_known = rho_known
+        self.C11_bounds = (50e9, 250e9)   # DI:265-270
+        self.C22_bounds = (5e9, 50e9)
+        self.C12_bounds = (5e9, 50e9)
+        self.C16_bounds = (0, 60e9)
+        self.C26_bounds = (0, 20e9)
+        self.C66_bounds = (5e9, 30e9)
+        self.gaussian_distribution = _gaussian_dist(nx, ny, h, 0.0, 0.0, gaussian_param)  # DI:273-283
+        self.register_buffer('gaussian_dist', self.gaussian_distribution)
+        self.register_buffer('source_signal', _ricker_signal(nt, dt, source_f0, source_t0, source_amplitude))
+        self._ckpt_every = ckpt_every
+        self._engines = _SnapshotEngines()
+
+    def get_material_params(self):
+        """DI:304-322."""
+        if self.use_log_params:
+            f = [torch.exp(p) for p in (self.log_C11, self.log_C22, self.log_C12, self.log_C16,
+                                        self.log_C26, self.log_C66)]
+        else:
+            f = [self.C11, self.C22, self.C12, self.C16, self.C26, self.C66]
+        bounds = (self.C11_bounds, self.C22_bounds, self.C12_bounds, self.C16_bounds,
+                  self.C26_bounds, self.C66_bounds)
+        C11, C22, C12, C16, C26, C66 = (torch.clamp(x, *b) for x, b in zip(f, bounds))
+        return C11, C22, C12, C16, C26, C66, self.rho
+
+    def get_mean_params(self):
+        """DI:324-329 (GPa)."""
+        C11, C22, C12, C16, C26, C66, rho = self.get_material_params()
+        return (C11.mean().item() / 1e9, C22.mean().item() / 1e9, C12.mean().item() / 1e9,
+                C16.mean().item() / 1e9, C26.mean().item() / 1e9, C66.mean().item() / 1e9,
+                rho.item())
+
+    def _survey(self, stride: int) -> Survey:
+        nx, ny = self.domain_shape
+        comp = 0 if self.source_component == 'x' else 1
+        g = self.gaussian_dist.detach().float().cpu().numpy()
+        keep = np.argwhere(g > 1e-12 * g.max())
+        src = [(int(i), int(j), float(g[i, j]), comp) for i, j in keep]
+        shot = Shot(nx=nx, ny=ny, nt=self.nt, h=self.h, dt=self.dt, rho=float(self.rho.item()),
+                    sources=src, wavelet=self.source_signal[0].detach().cpu().numpy())
+        return Survey([shot], mode="field6", rec_stride=stride, snapshot=True,
+                      ckpt_every=self._ckpt_every)
+
+    def engine(self, save_all_steps: bool = False) -> Engine:
+        stride = 1 if save_all_steps else self.primary_time_stride
+        dev = self.source_signal.device
+        return self._engines.get((str(dev), stride), lambda: self._survey(stride))
+
+    def forward(self, save_all_steps=False):
+        """DI:331-393: returns (ux_fields, uy_fields), each (1, ceil(nt/P), nx, ny)."""
+        eng = self.engine(save_all_steps)
+        C11, C22, C12, C16, C26, C66, rho = self.get_material_params()
+        field = torch.cat([c.reshape(-1) for c in (C11, C22, C12, C16, C26, C66)]).to(torch.float32)
+        traces = ernn_traces(eng, field=field.contiguous(), wavelets=self.source_signal[0].contiguous())
+        blk = eng.shot_traces(traces, 0)
+        return blk[0].unsqueeze(0), blk[1].unsqueeze(0)
+
+
+class ERNN(nn.Module):
+    """Receiver-trace ERNN over a batch of shots (the benchmark surveys).
+
+    Parameters are linear and normalised: C = theta * scale per set and
+    constant (scale = |C_init|, or `zero_scale` for constants that start at 0),
+    which, unlike exp/clamp (PI:223-232), can reach zero and negative values.
+    In dfield mode the per-cell factor D multiplies all six constants and is a
+    parameter too (``learn_damage``).
+    forward() returns the flat trace tensor; ``shot_traces``/``stacked`` give views.
+    """
+
+    def __init__(self, survey: Survey, C_init: np.ndarray, scale: Optional[np.ndarray] = None,
+                 zero_scale: float = 5e9, D_init: Optional[np.ndarray] = None,
+                 learn_constants: bool = True, learn_damage: bool = True, device=None,
+                 autotune: bool = False):
+        super().__init__()
+        self.survey = survey
+        dev = torch.device(device) if device is not None else torch.device("cuda")
+        C_init = np.asarray(C_init, dtype=np.float64).reshape(survey.n_sets, 6)
+        if scale is None:
+            scale = np.where(np.abs(C_init) > 0, np.abs(C_init), zero_scale)
+        self.register_buffer("scale", torch.tensor(np.asarray(scale).reshape(-1, 6), dtype=torch.float32, device=dev))
+        self.theta = nn.Parameter(torch.tensor(C_init / scale, dtype=torch.float32, device=dev),
+                                  requires_grad=learn_constants)
+        if survey.mode == "dfield":
+            if D_init is None:
+                D_init = np.ones(sum(int(np.prod(survey.set_grid(s))) for s in range(survey.n_sets)), np.float32)
+            self.damage = nn.Parameter(torch.tensor(np.asarray(D_init, np.float32).reshape(-1), device=dev),
+                                       requires_grad=learn_damage)
+        elif survey.mode == "field6":
+            raise ValueError("ERNN covers const6 and dfield; use SpatiallyVaryingAnisotropicSimulator for field6")
+        self.register_buffer("wavelets", torch.tensor(survey.wavelets(), device=dev))
+        self.engine = Engine(survey, device=dev)
+        if autotune:
+            from .autotune import tune_tiles
+            with torch.no_grad():
+                tune_tiles(self.engine, self.stiffness().reshape(-1).contiguous(),
+                           self.damage.detach() if survey.mode == "dfield" else None, self.wavelets)
+
+    def stiffness(self) -> torch.Tensor:
+        """(n_sets, 6) stiffness in Pa, order C11 C22 C12 C16 C26 C66."""
+        return self.theta * self.scale
+
+    def forward(self) -> torch.Tensor:
+        params = self.stiffness().reshape(-1).contiguous()
+        field = self.damage if self.survey.mode == "dfield" else None
+        return ernn_traces(self.engine, params=params, field=field, wavelets=self.wavelets)
+
+    def shot_traces(self, traces, b):
+        return self.engine.shot_traces(traces, b)
+
+    def stacked(self, traces):
+        return self.engine.stacked(traces)

@@ -1,0 +1,181 @@
+/* C ABI of libernn_hip.so: the ERNN forward recurrence and its reverse-time
+ * adjoint on AMD Instinct MI355X (gfx950).
+ *
+ * The reference (guoziye97/Elastodynamics-encoded-RNN) has no FFI layer: its
+ * boundary is the nn.Module contract of
+ *   CFRPWaveSimulatorRicker.forward           Parameter_inversion/Inversion_adam.py:240-307
+ *   SpatiallyVaryingAnisotropicSimulator.forward  Damage_inversion/Inversion_adam.py:331-393
+ * and the autograd reverse pass that loss.backward() runs through it
+ *   Parameter_inversion/Inversion_adam.py:447, Damage_inversion/Inversion_adam.py:511.
+ * ernn_forward replaces the time loop of those forward methods (one call per
+ * forward instead of ~60 tensor ops per step); ernn_adjoint replaces the
+ * autograd replay (one call per backward).  The Python package binds these
+ * entry points with ctypes behind a torch.autograd.Function.
+ *
+ * Conventions
+ *  - A field is nx x ny, row-major with y (index j) contiguous, exactly the
+ *    reference's (1, nx, ny) tensors (PI:186-191, PI:255).
+ *  - Stiffness order is the reference's C11, C22, C12, C16, C26, C66
+ *    (PI:86-98), i.e. C11, C33, C13, C15, C35, C55 in x-z notation.
+ *  - All pointers in ernn_call_t are device pointers (HBM).  The library
+ *    allocates no device memory per call: the caller provides the workspace
+ *    (size from ernn_plan_info) and the outputs.  Device memory is allocated
+ *    only by ernn_plan_create (tables) and freed by ernn_plan_destroy.
+ *  - All work is enqueued on the caller's stream; no call synchronizes the
+ *    host except ernn_plan_create/destroy and ernn_time_steps.
+ *  - Every function returns 0 on success or a negative ERNN_E* code; the
+ *    message of the last failure on the calling thread is ernn_last_error().
+ *  - A plan must not be used by two streams at the same time.
+ */
+#ifndef ERNN_H_
+#define ERNN_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ERNN_ABI_VERSION 1
+
+#define ERNN_OK 0
+#define ERNN_E_INVALID (-1)   /* bad argument or descriptor */
+#define ERNN_E_HIP (-2)       /* HIP runtime error */
+#define ERNN_E_CAPACITY (-3)  /* size beyond what the plan supports */
+
+/* Parameter modes. */
+#define ERNN_PARAM_CONST6 0  /* six constants per set: PI:166-171 */
+#define ERNN_PARAM_DFIELD 1  /* C_p(cell) = D(cell) * C0_p: one damage field + six constants */
+#define ERNN_PARAM_FIELD6 2  /* six per-cell fields per set: DI:245-258 */
+
+/* One instance = one shot (source time function + point forces) on one grid
+ * with one parameter set.  Instances of a set share the set's parameters and
+ * sum their gradients into it (shots of one model); sets of the field modes
+ * require equal grid sizes for all their instances. */
+typedef struct {
+  int32_t nx, ny;       /* grid (PI:186) */
+  int32_t nt;           /* time steps (PI:268) */
+  int32_t set;          /* parameter set index, 0 <= set < n_sets */
+  double h, dt, rho;    /* spacing, step, density (PI:139-147) */
+  int32_t src_begin, src_count;  /* slice of the source array */
+  int32_t rec_begin, rec_count;  /* slice of the receiver array (receiver mode) */
+} ernn_instance_t;
+
+/* A point force: weight w on cell (i, j), component comp (0 = x, 1 = y).
+ * At step n it adds wavelet[n] * w to the operator before the division by
+ * rho, like force_amplitude * gaussian_dist (PI:270-277, PI:103-104). */
+typedef struct {
+  int32_t i, j;
+  float w;
+  int32_t comp;
+} ernn_source_t;
+
+typedef struct {
+  int32_t i, j;
+} ernn_receiver_t;
+
+typedef struct {
+  int32_t n_inst;
+  int32_t n_sets;
+  int32_t param_mode;   /* ERNN_PARAM_* */
+  int32_t rec_stride;   /* P: frame k records u^{kP+1} (PI:294-301); 1 = save_all_steps */
+  int32_t snapshot;     /* 1: frames are whole fields (PI:304-305); 0: receiver traces */
+  int32_t ckpt_every;   /* K >= 2: checkpoint interval of the adjoint recompute */
+  int32_t fwd_v, fwd_r; /* forward tile: V in {1,2,4} columns per lane, R rows */
+  int32_t adj_v, adj_r; /* adjoint tile */
+  int32_t use_graph;    /* 1: replay each pass as a captured hipGraph */
+  int32_t reserved;
+  const ernn_instance_t* inst;  /* host array [n_inst] */
+  const ernn_source_t* src;     /* host array [n_src] */
+  int64_t n_src;
+  const ernn_receiver_t* rec;   /* host array [n_rec] */
+  int64_t n_rec;
+} ernn_desc_t;
+
+typedef struct {
+  int64_t workspace_bytes;  /* caller-provided workspace, 256-byte aligned */
+  int64_t trace_floats;     /* total floats of the trace array */
+  int64_t grad_field_doubles; /* total doubles of grad_field (field modes) */
+  int64_t field_floats;     /* total floats of the external field input (field modes) */
+  int32_t n_frames_max;
+  int32_t ckpt_every;
+  int32_t n_ckpt;           /* checkpoints per instance */
+  int32_t fwd_v, fwd_r, adj_v, adj_r;
+  int32_t fwd_tiles, adj_tiles;
+  int32_t use_graph;
+  int32_t reserved;
+} ernn_plan_info_t;
+
+/* Per-call device pointers.
+ *  params      [n_sets][6] fp32 constants (CONST6, DFIELD; C0 for DFIELD)
+ *  field       per set, [n_coef][nx*ny] fp32, sets back to back (DFIELD: D,
+ *              n_coef = 1; FIELD6: the six C fields, n_coef = 6)
+ *  wavelets    per instance, [nt] fp32 source time function (PI:210-217),
+ *              instances back to back
+ *  traces      forward output.  Instance b's block starts at
+ *              ernn_trace_offsets()[b] and is [2][n_frames][X] with X =
+ *              rec_count (receiver mode, ordered as the instance's receiver
+ *              slice) or nx*ny (snapshot mode); component 0 = ux, 1 = uy.
+ *  grad_traces adjoint input dJ/d traces, same layout as traces
+ *  grad_params [n_sets][6] fp64 output (CONST6, DFIELD)
+ *  grad_field  fp64 output, per set [n_coef][nx*ny] (DFIELD: dJ/dD; FIELD6:
+ *              dJ/dC_p fields), sets back to back */
+typedef struct {
+  void* workspace;
+  const float* params;
+  const float* field;
+  const float* wavelets;
+  float* traces;
+  const float* grad_traces;
+  double* grad_params;
+  double* grad_field;
+} ernn_call_t;
+
+/* Version of this ABI (ERNN_ABI_VERSION). */
+int ernn_abi_version(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* ernn_last_error(void);
+
+/* Validate a descriptor, build tile and event tables on the current device.
+ * Replaces the set-up of CFRPWaveSimulatorRicker.__init__ (PI:139-208) and
+ * SpatiallyVaryingAnisotropicSimulator.__init__ (DI:208-296). */
+int ernn_plan_create(const ernn_desc_t* desc, void** plan_out);
+
+int ernn_plan_destroy(void* plan);
+
+int ernn_plan_info(const void* plan, ernn_plan_info_t* info);
+
+/* trace_off[b] for b in [0, n_inst] (n_inst + 1 entries). */
+int ernn_trace_offsets(const void* plan, int64_t* trace_off);
+
+/* Zero a new workspace (ghost cells must be zero; the kernels never write
+ * them).  Uses a fill kernel, never hipMemset, so it may be graph-captured. */
+int ernn_workspace_init(void* plan, void* workspace, void* stream);
+
+/* Forward recurrence of every instance (PI:240-307, DI:331-393): writes the
+ * traces and the checkpoints the adjoint needs into the workspace. */
+int ernn_forward(void* plan, const ernn_call_t* call, void* stream);
+
+/* Gradient of a scalar loss J given dJ/d traces (replaces loss.backward(),
+ * PI:447 / DI:511).  Needs the workspace of the matching ernn_forward.
+ * Recomputes the forward from checkpoints segment by segment and runs the
+ * reverse-time adjoint recurrence; FP64 accumulation. */
+int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream);
+
+/* Change the tile of the forward (which = 0) or adjoint (which = 1) kernel.
+ * Used by the autotuner; drops captured graphs. */
+int ernn_plan_set_tiling(void* plan, int which, int v, int r);
+
+/* Time `steps` forward (which = 0) or adjoint (which = 1) steps with the
+ * current tiling using hipEvents on `stream`; synchronizes.  The workspace
+ * must be initialized.  ms_out receives the elapsed milliseconds. */
+int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
+                    float* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ERNN_H_ */
