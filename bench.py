@@ -1,0 +1,312 @@
+"""Benchmark: fwd+adjoint Gcell-steps/s and s/Adam-iter, HIP adjoint vs autograd, 1 GPU.
+
+One step = one Adam iteration of the waveform-misfit inversion
+(Parameter_inversion/Inversion_adam.py:436-450): forward recurrence over all
+shots, MSE on receiver traces, gradient (checkpointed recompute + reverse-time
+adjoint), Adam update.  Default workload: single-512 (512^2 grid, 2000 steps,
+one shot, 256 receivers, checkpoint every 50), FP32, synthetic data (observed
+traces = the same solver at the true constants + 1 % RMS seeded noise).
+
+Also reported on the same line:
+  autograd_baseline  the same iteration through the reference's own method
+                     (op-for-op F.pad/F.conv2d restatement + autograd) on the
+                     same GPU, in the reference-faithful form (per-step stencil
+                     weight tensors, snapshot history) and with the weights
+                     hoisted; the faster one is the denominator of `speedup`.
+  roofline           dominant kernel's algorithmic bytes per launch / its average
+                     launch duration (hipEvents on the launch stream, back-to-back
+                     launches), against 8 TB/s HBM.
+  cpu_baseline       the op-for-op CPU port (oracle/plain_torch.py, autograd) on
+                     the host cores over a bounded window of steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "elastodynamics-encoded-rnn_amd"))
+
+METRIC = "fwd+adjoint Gcell-steps/s & s/Adam-iter: HIP adjoint vs autograd, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+# algorithmic FP32 bytes per cell and launch: forward reads u^n, u^{n-1} and
+# writes u^{n+1} (2 components each); the adjoint reads lam^{m+1}, lam^{m+2},
+# u^m and writes lam^m; per-cell fields add their reads / accumulator traffic.
+BYTES_PER_CELL = {
+    ("const6", 0): 24, ("const6", 1): 32,
+    ("dfield", 0): 28, ("dfield", 1): 44,      # + D read; adj + D + fp32 accumulator r/w
+    ("field6", 0): 48, ("field6", 1): 104,
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="single-512")
+    ap.add_argument("--nt", type=int, default=0, help="override the step count (profiling runs)")
+    ap.add_argument("--graph", type=int, default=1, help="replay passes as captured hipGraphs")
+    ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--baseline", type=int, default=1, help="time the autograd plain path on the GPU")
+    ap.add_argument("--cpu", type=int, default=1, help="time the CPU port")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+# ---------------------------------------------------------------------------
+# The reference's own method on the GPU (autograd plain path), self-contained
+# restatement of PI:44-106 and PI:240-307 used only as the timing baseline.
+# ---------------------------------------------------------------------------
+def _weights(torch, h, device, dtype):
+    s = h ** (-2)
+    a = torch.tensor([[[[0.0, 0.0, 0.0], [1.0, -2.0, 1.0], [0.0, 0.0, 0.0]]]], device=device, dtype=dtype)
+    b = torch.tensor([[[[0.0, 1.0, 0.0], [0.0, -2.0, 0.0], [0.0, 1.0, 0.0]]]], device=device, dtype=dtype)
+    c = torch.tensor([[[[0.25, 0.0, -0.25], [0.0, 0.0, 0.0], [-0.25, 0.0, 0.25]]]], device=device, dtype=dtype)
+    return s * a, s * b, s * c
+
+
+def autograd_traces(torch, shot, C, faithful):
+    """Receiver traces (2, nt, n_rec) through the reference's op sequence."""
+    import torch.nn.functional as F
+    dev = C[0].device
+    nx, ny = shot.nx, shot.ny
+    sig = torch.tensor(shot.wavelet, device=dev)
+    g = torch.zeros(1, nx, ny, device=dev)
+    for (i, j, w, comp) in shot.sources:
+        g[0, i, j] += w
+    rho = torch.tensor(shot.rho, device=dev)
+    ri = torch.as_tensor(shot.receivers[:, 0], device=dev).long()
+    rj = torch.as_tensor(shot.receivers[:, 1], device=dev).long()
+    C11, C22, C12, C16, C26, C66 = C
+    W = None if faithful else _weights(torch, shot.h, dev, torch.float32)
+    ux1 = torch.zeros(1, nx, ny, device=dev); uy1 = torch.zeros_like(ux1)
+    ux2 = torch.zeros_like(ux1); uy2 = torch.zeros_like(ux1)
+    hx, hy = [], []
+    conv = lambda f, k: F.conv2d(F.pad(f, (1, 1, 1, 1)), k)
+    for n in range(shot.nt):
+        fy = sig[n] * g
+        d2dx2, d2dy2, d2dxdy = W if W is not None else _weights(torch, shot.h, dev, torch.float32)
+        ux = ux1.transpose(1, 2).unsqueeze(1); uy = uy1.transpose(1, 2).unsqueeze(1)
+        uxx = conv(ux, d2dx2).squeeze(1); uxy_ = conv(ux, d2dy2).squeeze(1)
+        uyx = conv(uy, d2dx2).squeeze(1); uyy = conv(uy, d2dy2).squeeze(1)
+        uxxy = conv(ux, d2dxdy).squeeze(1); uyxy = conv(uy, d2dxdy).squeeze(1)
+        Lx = (C11 * uxx + C66 * uxy_ + (C12 + C66) * uyxy + 2 * C16 * uxxy + C16 * uyx + C26 * uyy).transpose(1, 2)
+        Ly = (C66 * uyx + C22 * uyy + (C12 + C66) * uxxy + C16 * uxx + C26 * uxy_ + 2 * C26 * uyxy).transpose(1, 2)
+        Lx = (Lx + torch.zeros_like(fy)) / rho
+        Ly = (Ly + fy) / rho
+        uxn = 2 * ux1 - ux2 + shot.dt ** 2 * Lx
+        uyn = 2 * uy1 - uy2 + shot.dt ** 2 * Ly
+        ux2, uy2, ux1, uy1 = ux1, uy1, uxn, uyn
+        if faithful:
+            hx.append(ux1); hy.append(uy1)
+        else:
+            hx.append(ux1[0, ri, rj]); hy.append(uy1[0, ri, rj])
+    if faithful:
+        X = torch.stack(hx, dim=1); Y = torch.stack(hy, dim=1)
+        return torch.stack([X[0][:, ri, rj], Y[0][:, ri, rj]])
+    return torch.stack([torch.stack(hx), torch.stack(hy)])
+
+
+def time_autograd(torch, wl, theta0, scale, observed, iters=1, warmup=1):
+    """s/Adam-iter of the plain path (one shot at a time, summed over shots)."""
+    res = {}
+    for faithful in (True, False):
+        theta = torch.nn.Parameter(theta0.clone())
+        opt = torch.optim.Adam([theta], lr=1e-2)
+
+        def it():
+            opt.zero_grad()
+            C = list(theta[0] * scale[0])
+            loss = 0.0
+            for b, s in enumerate(wl.survey.shots):
+                tr = autograd_traces(torch, s, C, faithful)
+                loss = loss + torch.mean((tr - observed[b]) ** 2)
+            loss.backward()
+            opt.step()
+        for _ in range(warmup):
+            it()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            it()
+        torch.cuda.synchronize()
+        res["faithful" if faithful else "hoisted"] = (time.perf_counter() - t0) / iters
+        log(f"autograd plain path ({'faithful' if faithful else 'hoisted'}): "
+            f"{res['faithful' if faithful else 'hoisted']:.3f} s/iter")
+    return res
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the op-for-op port (oracle/plain_torch.py) on the host cores
+# ---------------------------------------------------------------------------
+def cpu_baseline(torch, wl, budget_s):
+    sys.path.insert(0, REPO)
+    from oracle import plain_torch as pt
+    s = wl.survey.shots[0]
+    threads = torch.get_num_threads()
+
+    def run(nsteps):
+        C = [torch.tensor(float(c), requires_grad=True) for c in wl.C_true[0]]
+        sig = torch.tensor(s.wavelet[:nsteps]).unsqueeze(0)
+        spatial = torch.zeros(s.nx, s.ny)
+        for (i, j, w, comp) in s.sources:
+            spatial[i, j] += w
+        t0 = time.perf_counter()
+        ux, uy = pt.forward(C, s.h, s.dt, nsteps, 1, sig, spatial, torch.tensor(s.rho), hoist_weights=True)
+        (ux.square().sum() + uy.square().sum()).backward()
+        return time.perf_counter() - t0
+
+    # the source fires near t0 = 1 us, so short windows still carry the same op count
+    t_probe = run(10)
+    n = int(max(10, min(s.nt, 10 * budget_s / max(t_probe, 1e-3))))
+    t = run(n)
+    rate = s.nx * s.ny * n / t / 1e9
+    return {"value": round(rate, 5), "unit": "Gcell-steps/s (fwd+adjoint)", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle/plain_torch.py forward + autograd backward, {s.nx}x{s.ny} grid, "
+                      f"{n} of {s.nt} steps, torch {threads} threads, {t:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Shots would shard across GPUs with one gradient all-reduce per Adam iteration.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import ernn
+    from ernn import configs
+    from ernn.autotune import tune_tiles
+
+    dev = torch.device("cuda:0")
+    kw = {"nt": args.nt} if args.nt else {}
+    wl = configs.WORKLOADS[args.workload](**kw)
+    sv = wl.survey
+    sv.use_graph = bool(args.graph)
+    rng = np.random.default_rng(0)
+    C_true = wl.C_true
+    pert = 1.0 + rng.uniform(-0.2, 0.2, size=C_true.shape)
+    C_init = C_true * pert
+    C_init[:, 3:5] = C_true[:, 3:5] + rng.uniform(-0.2, 0.2, size=(C_true.shape[0], 2)) * C_true[:, 5:6]
+    D_init = None if sv.mode == "const6" else np.ones_like(wl.D_true)
+    log(f"workload {wl.name}: {wl.description}; {sv.cell_steps() / 1e9:.3f} Gcell-steps per pass")
+
+    model = ernn.ERNN(sv, C_init, D_init=D_init, device=dev)
+    eng = model.engine
+    params_true = torch.tensor(C_true.reshape(-1), dtype=torch.float32, device=dev)
+    field_true = None if sv.mode == "const6" else torch.tensor(wl.D_true.reshape(-1), device=dev)
+    tune = None
+    if args.autotune:
+        tune = tune_tiles(eng, params_true, field_true, model.wavelets, steps=20)
+        log("autotune: " + ", ".join(f"{'fwd' if w == 0 else 'adj'} V={v} R={r} {ms * 1e3:.1f} us/step"
+                                     for w, (v, r, ms, _) in tune.items()))
+    with torch.no_grad():
+        obs = ernn.ernn_traces(eng, params=params_true, field=field_true, wavelets=model.wavelets)
+        gen = torch.Generator(device="cpu").manual_seed(1)
+        noise = torch.randn(obs.shape, generator=gen).to(dev)
+        for b in range(len(sv.shots)):
+            blk = eng.shot_traces(obs, b)
+            nblk = eng.shot_traces(noise, b)
+            for c in range(2):
+                blk[c] += 0.01 * blk[c].pow(2).mean().sqrt() * nblk[c]
+    opt = torch.optim.Adam(model.parameters(), lr=1e-2)
+
+    def adam_iter():
+        opt.zero_grad()
+        tr = model()
+        loss = torch.mean((tr - obs) ** 2)
+        loss.backward()
+        opt.step()
+        return loss
+
+    for _ in range(args.warmup):
+        adam_iter()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        loss = adam_iter()
+    torch.cuda.synchronize()
+    dt_iter = (time.perf_counter() - t0) / args.steps
+    value = sv.cell_steps() / dt_iter / 1e9
+    log(f"HIP: {dt_iter * 1e3:.2f} ms/Adam-iter, {value:.2f} Gcell-steps/s fwd+adjoint, loss {loss.item():.4e}")
+
+    # per-kernel launch durations (hipEvents around back-to-back launches on the launch stream)
+    ws = eng.acquire_workspace()
+    tr0 = torch.zeros(eng.trace_floats, device=dev)
+    nk = min(200, sv.nt_max)
+    p = model.stiffness().detach().reshape(-1).contiguous()
+    f = None if sv.mode == "const6" else model.damage.detach()
+    eng.time_steps(0, 10, p, f, model.wavelets, ws, tr0, tr0)
+    fwd_ms = eng.time_steps(0, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
+    adj_ms = eng.time_steps(1, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
+    eng.release_workspace(ws)
+    cells = sum(s.nx * s.ny for s in sv.shots)
+    kern = {"fwd_step_kernel": (fwd_ms, 2 * sv.nt_max, BYTES_PER_CELL[(sv.mode, 0)]),
+            "adj_step_kernel": (adj_ms, sv.nt_max, BYTES_PER_CELL[(sv.mode, 1)])}
+    name = max(kern, key=lambda k: kern[k][0] * kern[k][1])
+    ms, launches, bpc = kern[name]
+    achieved = cells * bpc / (ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        traffic = tj.get(f"{wl.name}/{name}")
+    roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_cell": bpc, "cells_per_launch": cells, "avg_launch_us": round(ms * 1e3, 3),
+                "launches_per_iter": launches,
+                "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3)}
+    log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3:.2f} us per launch; "
+        f"{name}: {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.2%} of HBM peak)")
+
+    baseline = None
+    if args.baseline and sv.mode == "const6" and len(sv.shots) == 1:
+        theta0 = torch.tensor(C_init / np.where(np.abs(C_init) > 0, np.abs(C_init), 5e9),
+                              dtype=torch.float32, device=dev)
+        obs_shots = [eng.shot_traces(obs, b) for b in range(len(sv.shots))]
+        ag = time_autograd(torch, wl, theta0, model.scale, obs_shots)
+        best = min(ag.values())
+        baseline = {"s_per_adam_iter_faithful": round(ag["faithful"], 4),
+                    "s_per_adam_iter_hoisted": round(ag["hoisted"], 4),
+                    "value": round(sv.cell_steps() / best / 1e9, 4), "unit": "Gcell-steps/s"}
+
+    cpu = None
+    if args.cpu:
+        torch.set_num_threads(min(os.cpu_count() or 1, 16))
+        cpu = cpu_baseline(torch, wl, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']} Gcell-steps/s on {cpu['cores']} threads")
+
+    s0 = sv.shots[0]
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt_iter * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
+                   "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
+                   "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every), "graph": bool(sv.use_graph),
+                   "tiles": [int(eng.info.fwd_v), int(eng.info.fwd_r), int(eng.info.adj_v), int(eng.info.adj_r)],
+                   "parallelism": "none"},
+        "s_per_adam_iter": round(dt_iter, 5),
+        "autograd_baseline": baseline,
+        "speedup_vs_autograd": (round(value / baseline["value"], 2) if baseline else None),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -252,6 +252,11 @@ int validate(const ernn_desc_t* d) {
     return fail(ERNN_E_INVALID, "unknown param_mode");
   if (d->rec_stride < 1) return fail(ERNN_E_INVALID, "rec_stride must be >= 1");
   if (d->ckpt_every < 2) return fail(ERNN_E_INVALID, "ckpt_every must be >= 2");
+  for (int v : {d->fwd_v, d->adj_v})
+    if (!(v == 0 || v == 1 || v == 2 || v == 4)) return fail(ERNN_E_INVALID, "tile V must be 1, 2 or 4");
+  for (int r : {d->fwd_r, d->adj_r})
+    if (r != 0 && (r < kMinRows || r > kMaxRows))
+      return fail(ERNN_E_INVALID, "tile R out of range [2, 4096]");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))

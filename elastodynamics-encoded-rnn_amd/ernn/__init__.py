@@ -12,6 +12,8 @@ from .engine import Engine
 from .function import ERNNFunction, ernn_traces
 from .simulator import (ERNN, CFRPWaveSimulatorRicker, SpatiallyVaryingAnisotropicSimulator,
                         total_variation_2d)
+from .inversion import (CFRPParameterInverterRicker, SpatiallyVaryingParameterInverter, TraceInversion,
+                        add_noise_rms, add_noise_snr)
 from . import autotune, configs
 
 __all__ = [
@@ -19,4 +21,6 @@ __all__ = [
     "gaussian_sources", "max_phase_velocity", "point_source", "ricker", "row_receivers",
     "Engine", "ERNNFunction", "ernn_traces", "ERNN", "CFRPWaveSimulatorRicker",
     "SpatiallyVaryingAnisotropicSimulator", "total_variation_2d", "autotune", "configs",
+    "CFRPParameterInverterRicker", "SpatiallyVaryingParameterInverter", "TraceInversion",
+    "add_noise_rms", "add_noise_snr",
 ]

@@ -1,0 +1,188 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every
+declared symbol, descriptor validation rejects bad input without touching a
+GPU, and the problem / autotune helpers compute what they document."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import ernn
+from ernn import _lib, autotune, configs
+from ernn.problem import Survey
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "ernn.h")
+
+
+def header_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ernn_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.EXPORTS), "ctypes binding and header disagree"
+    assert lib.ernn_abi_version() == _lib.ERNN_ABI_VERSION
+
+
+def _desc(inst, src=(), rec=(), **kw):
+    n = len(inst)
+    I = (_lib.Instance * n)()
+    for k, d in enumerate(inst):
+        for f, v in d.items():
+            setattr(I[k], f, v)
+    S = (_lib.Source * max(len(src), 1))()
+    for k, (i, j, w, c) in enumerate(src):
+        S[k].i, S[k].j, S[k].w, S[k].comp = i, j, w, c
+    R = (_lib.Receiver * max(len(rec), 1))()
+    for k, (i, j) in enumerate(rec):
+        R[k].i, R[k].j = i, j
+    d = _lib.Desc()
+    d.n_inst, d.n_sets, d.param_mode, d.rec_stride, d.ckpt_every = n, 1, 0, 1, 4
+    d.fwd_v, d.fwd_r, d.adj_v, d.adj_r = 2, 8, 2, 8
+    for f, v in kw.items():
+        setattr(d, f, v)
+    d.inst, d.src, d.n_src, d.rec, d.n_rec = I, S, len(src), R, len(rec)
+    return d, (I, S, R)
+
+
+GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
+            src_begin=0, src_count=1, rec_begin=0, rec_count=1)
+
+
+@pytest.mark.parametrize("patch,kw,msg", [
+    ({"nx": 0}, {}, "grid"),
+    ({"nt": 0}, {}, "nt"),
+    ({"set": 1}, {}, "set"),
+    ({"dt": -1.0}, {}, "finite and positive"),
+    ({"src_count": 2}, {}, "source slice"),
+    ({"rec_count": 3}, {}, "receiver slice"),
+    ({}, {"ckpt_every": 1}, "ckpt_every"),
+    ({}, {"rec_stride": 0}, "rec_stride"),
+    ({}, {"param_mode": 7}, "param_mode"),
+    ({}, {"fwd_v": 3}, "tile V"),
+    ({}, {"adj_r": 1}, "tile R"),
+])
+def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
+    inst = dict(GOOD, **patch)
+    d, keep = _desc([inst], src=[(1, 2, 1.0, 1)], rec=[(0, 15)], **kw)
+    h = C.c_void_p()
+    rc = _lib.lib().ernn_plan_create(C.byref(d), C.byref(h))
+    assert rc == -1
+    assert msg in _lib.lib().ernn_last_error().decode()
+    assert not h.value
+
+
+def test_plan_create_rejects_out_of_grid_source_and_receiver():
+    for src, rec, msg in [([(16, 2, 1.0, 1)], [(0, 15)], "source"),
+                          ([(1, 2, 1.0, 2)], [(0, 15)], "source"),
+                          ([(1, 2, 1.0, 1)], [(0, 16)], "receiver")]:
+        d, keep = _desc([GOOD], src=src, rec=rec)
+        h = C.c_void_p()
+        assert _lib.lib().ernn_plan_create(C.byref(d), C.byref(h)) == -1
+        assert msg in _lib.lib().ernn_last_error().decode()
+
+
+def test_field_sets_must_share_grid():
+    a = dict(GOOD)
+    b = dict(GOOD, nx=20)
+    d, keep = _desc([a, b], src=[(1, 2, 1.0, 1)], rec=[(0, 15)], param_mode=1)
+    h = C.c_void_p()
+    assert _lib.lib().ernn_plan_create(C.byref(d), C.byref(h)) == -1
+    assert "share the grid" in _lib.lib().ernn_last_error().decode()
+
+
+def test_null_arguments_are_errors():
+    lib = _lib.lib()
+    assert lib.ernn_plan_create(None, None) == -1
+    info = _lib.PlanInfo()
+    assert lib.ernn_plan_info(None, C.byref(info)) == -1
+    assert lib.ernn_forward(None, None, None) == -1
+    assert lib.ernn_adjoint(None, None, None) == -1
+    assert lib.ernn_plan_destroy(None) == 0
+
+
+def test_engine_refuses_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    wl = configs.plain_128(nt=10)
+    with pytest.raises(RuntimeError, match="GPU"):
+        ernn.Engine(wl.survey)
+
+
+# ------------------------------------------------------------------ helpers
+def test_checkpoint_interval_minimizes_slots():
+    wl = configs.single_512(nt=2000)
+    K = autotune.choose_ckpt(wl.survey)
+    assert autotune.slots(2000, K) == min(autotune.slots(2000, k) for k in range(2, 2001))
+    assert abs(K - math.sqrt(2 * 2000)) < 8
+    with pytest.raises(MemoryError):
+        autotune.choose_ckpt(wl.survey, mem_cap_bytes=1 << 20)
+
+
+def test_damage_1024_fits_the_32_gb_cap():
+    wl = configs.damage_1024()
+    K = autotune.choose_ckpt(wl.survey, mem_cap_bytes=32 << 30)
+    assert autotune.state_bytes(wl.survey, K) < 32 << 30
+
+
+def test_padded_layout_matches_library_convention():
+    # pitch = 32 + round_up(ny, 256); rows nx + 3
+    assert autotune.padded_field_floats(128, 128) == 131 * 288
+    assert autotune.padded_field_floats(10, 256) == 13 * 288
+    assert autotune.padded_field_floats(10, 257) == 13 * 544
+
+
+def test_workload_presets():
+    wl = configs.plain_128()
+    s = wl.survey.shots[0]
+    assert (s.nx, s.ny, s.nt) == (128, 128, 600)
+    assert s.dt == pytest.approx(1.1094e-8, rel=1e-4)
+    assert len(s.receivers) == 64 and set(s.receivers[:, 1]) == {127}
+    assert s.sources[0][:2] == (64, 127)
+    assert wl.survey.cell_steps() == 128 * 128 * 600
+    wl = configs.batched_uneven()
+    assert len(wl.survey.shots) == 24 and wl.survey.n_sets == 24
+    assert {s.nx for s in wl.survey.shots} <= {256, 512, 768, 1024}
+    assert np.all(np.abs(wl.C_true[:, [0, 1, 2, 5]] / np.array(configs.C_ORTHO)[[0, 1, 2, 5]] - 1) <= 0.3)
+    wl = configs.damage_1024()
+    assert wl.D_true.shape == (1024, 1024)
+    assert 0.29 <= wl.D_true.min() <= 0.71 and wl.D_true.max() == 1.0
+    assert len(wl.survey.shots) == 8 and len(wl.survey.shots[0].receivers) == 512
+
+
+def test_bond_rotation_known_values_and_invariants():
+    r = np.array(ernn.bond_rotate((130, 12, 6, 0, 0, 5), 30.0))
+    np.testing.assert_allclose(r, [79.875, 20.875, 26.625, 37.456, 13.640, 25.625], atol=1e-3)
+    assert r[0] + r[1] + 2 * r[2] == pytest.approx(154)
+    assert r[0] + r[1] + 2 * r[5] == pytest.approx(152)
+    # rotating back recovers the original
+    np.testing.assert_allclose(ernn.bond_rotate(tuple(r), -30.0), [130, 12, 6, 0, 0, 5], atol=1e-9)
+    # the fastest direction speed is rotation invariant
+    C = np.array([130e9, 12e9, 6e9, 0, 0, 5e9])
+    v0 = ernn.max_phase_velocity(C, 1600)
+    v1 = ernn.max_phase_velocity(np.array(ernn.bond_rotate(C, 30.0)), 1600)
+    assert v1 == pytest.approx(v0, rel=1e-6)
+    assert v0 == pytest.approx(math.sqrt(130e9 / 1600), rel=1e-9)
+
+
+def test_gaussian_sources_match_reference_formula():
+    src, g = ernn.gaussian_sources(21, 21, 1.333e-3, a=5e6)
+    assert g[10, 10] == pytest.approx(5e6 / math.pi, rel=1e-6)
+    assert all(w > 1e-12 * g.max() for (_, _, w, _) in src)
+    assert max(w for (_, _, w, _) in src) == pytest.approx(g.max())
+
+
+def test_ellipse_damage_properties():
+    D = ernn.ellipse_damage(200, 180, 3, np.random.default_rng(0), semi=(20, 40), level=(0.3, 0.7))
+    assert D.shape == (200, 180) and D.dtype == np.float32
+    assert D.max() == 1.0 and 0.3 <= D.min() <= 0.7
+    assert (D < 1).mean() > 0.02
