@@ -2,14 +2,16 @@
 // plan builder (ernn_plan.cpp).  Plain structs, identical on host and device.
 //
 // State layout in the workspace ("padded field"): a field of an nx x ny grid
-// occupies (nx + 3) rows of `pitch` floats.  Row r of the grid (-1 <= r <= nx+1)
-// starts at (r + 1) * pitch, and column j (-32 <= j < pitch - 32) sits at
-// +32 + j, so interior cell (i, j) is at (i + 1) * pitch + 32 + j.  Everything
-// outside 0 <= i < nx, 0 <= j < ny is a ghost cell that stays zero: the
-// reference's zero padding (Parameter_inversion/Inversion_adam.py:50).
-// pitch = 32 + round_up(ny, 256), so every interior row starts 128-byte
-// aligned and a wavefront's last halo read (column round_up(ny, 64 V)) lands
-// at most on the zero left pad of the next row.
+// occupies round_up(nx, 16) + 3 rows of `pitch` floats.  Row r of the grid
+// (-1 <= r <= round_up(nx, 16) + 1) starts at (r + 1) * pitch, and column j
+// (-32 <= j < pitch - 32) sits at +32 + j, so interior cell (i, j) is at
+// (i + 1) * pitch + 32 + j.  Everything outside 0 <= i < nx, 0 <= j < ny is a
+// ghost cell that stays zero: the reference's zero padding
+// (Parameter_inversion/Inversion_adam.py:50).  pitch = 32 + round_up(ny, 256),
+// so every interior row starts 128-byte aligned and a wavefront's last halo
+// read (column round_up(ny, 64 V)) lands at most on the zero left pad of the
+// next row; the extra ghost rows let a tile of up to 16 rows load all its
+// rows without bounds checks.
 #pragma once
 #include <stdint.h>
 
@@ -20,6 +22,7 @@ constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
 constexpr int kLeftPad = 32;
 constexpr int kColAlign = 256;
+constexpr int kRowAlign = 16;
 
 enum ParamMode : int { kConst6 = 0, kDField = 1, kField6 = 2 };
 

@@ -205,8 +205,14 @@ __device__ __forceinline__ float* prow(float* base, int64_t pitch, int r, int j0
 
 // --------------------------------------------------------------------------
 // Forward step: u^{n+1} from u^n, u^{n-1}  (PI:268-301 / DI:355-387)
+//
+// R is a compile-time row count: the wavefront issues every load of its tile
+// (R + 2 rows of u^n, R rows of u^{n-1} and of the coefficients) before any
+// arithmetic, so a tile costs about one memory latency instead of R.  Rows
+// past the grid are zero ghost rows (the layout pads nx up to kRowAlign), so
+// the loads need no bounds checks; stores and events stop at row nx.
 // --------------------------------------------------------------------------
-template <int MODE, int V>
+template <int MODE, int V, int R>
 __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int t = tile_index();
@@ -240,41 +246,41 @@ __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
   const int k = n / a.stride;
   const int nf = I.n_frames;
 
-  Row<V> xm = load_row<V>(prow(u0x, pitch, tl.i0 - 1, j0), lane);
-  Row<V> xc = load_row<V>(prow(u0x, pitch, tl.i0, j0), lane);
-  Row<V> ym = load_row<V>(prow(u0y, pitch, tl.i0 - 1, j0), lane);
-  Row<V> yc = load_row<V>(prow(u0y, pitch, tl.i0, j0), lane);
+  constexpr int NC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
+  Row<V> xr[R + 2], yr[R + 2];
+  float pvx[R][V], pvy[R][V];
+  float cf[R][NC > 0 ? NC : 1][V];
+#pragma unroll
+  for (int r = 0; r < R + 2; ++r) {
+    xr[r] = load_row<V>(prow(u0x, pitch, tl.i0 - 1 + r, j0), lane);
+    yr[r] = load_row<V>(prow(u0y, pitch, tl.i0 - 1 + r, j0), lane);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    load_vec<V>(prow(u1x, pitch, tl.i0 + r, j0) + lane * V, pvx[r]);
+    load_vec<V>(prow(u1y, pitch, tl.i0 + r, j0) + lane * V, pvy[r]);
+#pragma unroll
+    for (int q = 0; q < NC; ++q) load_vec<V>(prow(coef + q * I.field, pitch, tl.i0 + r, j0) + lane * V, cf[r][q]);
+  }
 
   int sp = tl.src_begin;
   int rp = tl.rec_begin;
-  const int iend = min(tl.i0 + a.rows, I.nx);
-  for (int i = tl.i0; i < iend; ++i) {
-    const Row<V> xp = load_row<V>(prow(u0x, pitch, i + 1, j0), lane);
-    const Row<V> yp = load_row<V>(prow(u0y, pitch, i + 1, j0), lane);
-    float pvx[V], pvy[V];
-    load_vec<V>(prow(u1x, pitch, i, j0) + lane * V, pvx);
-    load_vec<V>(prow(u1y, pitch, i, j0) + lane * V, pvy);
-
-    float cf[6][V];
-    if constexpr (MODE == kDField) {
-      load_vec<V>(prow(coef, pitch, i, j0) + lane * V, cf[0]);
-    } else if constexpr (MODE == kField6) {
 #pragma unroll
-      for (int q = 0; q < 6; ++q) load_vec<V>(prow(coef + q * I.field, pitch, i, j0) + lane * V, cf[q]);
-    }
-
+  for (int r = 0; r < R; ++r) {
+    const int i = tl.i0 + r;
+    if (i >= I.nx) break;
     float lx[V], ly[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      const Deriv d = derivs<V>(xm, xc, xp, ym, yc, yp, e + 1, w);
+      const Deriv d = derivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1, w);
       if constexpr (MODE == kConst6) {
         apply_op(sc, d, lx[e], ly[e]);
       } else if constexpr (MODE == kDField) {
-        const float D = cf[0][e];
+        const float D = cf[r][0][e];
         const Stiff s = make_stiff(D * c0[0], D * c0[1], D * c0[2], D * c0[3], D * c0[4], D * c0[5]);
         apply_op(s, d, lx[e], ly[e]);
       } else {
-        const Stiff s = make_stiff(cf[0][e], cf[1][e], cf[2][e], cf[3][e], cf[4][e], cf[5][e]);
+        const Stiff s = make_stiff(cf[r][0][e], cf[r][1][e], cf[r][2][e], cf[r][3][e], cf[r][4][e], cf[r][5][e]);
         apply_op(s, d, lx[e], ly[e]);
       }
     }
@@ -302,8 +308,8 @@ __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
     for (int e = 0; e < V; ++e) {
       const float ax = lx[e] / I.rho;
       const float ay = ly[e] / I.rho;
-      float vx = (2.f * xc.v[e + 1] - pvx[e]) + I.dt2 * ax;
-      float vy = (2.f * yc.v[e + 1] - pvy[e]) + I.dt2 * ay;
+      float vx = (2.f * xr[r + 1].v[e + 1] - pvx[r][e]) + I.dt2 * ax;
+      float vy = (2.f * yr[r + 1].v[e + 1] - pvy[r][e]) + I.dt2 * ay;
       if (j0 + lane * V + e >= I.ny) { vx = 0.f; vy = 0.f; }  // keep the right ghosts zero
       nvx[e] = vx;
       nvy[e] = vy;
@@ -337,16 +343,60 @@ __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
         }
       }
     }
-    xm = xc; xc = xp;
-    ym = yc; yc = yp;
   }
 }
 
 // --------------------------------------------------------------------------
 // Adjoint step: lam^m = r^m + (2I + (dt^2/rho) A^T) lam^{m+1} - lam^{m+2}
-// with the gradient terms (dt^2/rho) <lam^{m+1}, dA/dC u^m>.
+// with the gradient terms (dt^2/rho) <lam^{m+1}, dA/dC u^m>.  Same tile
+// scheme as the forward: all loads first, then R rows of arithmetic.
 // --------------------------------------------------------------------------
 template <int MODE, int V>
+struct AdjRows {
+  // stencilled adjoint rows.  kConst6: lam itself.  kDField: D * lam.
+  // kField6: the six weighted fields of A^T (apply_AT in the oracle).
+  static constexpr int NW = (MODE == kField6) ? 6 : 2;
+  Row<V> w[NW];
+  float cx[V], cy[V];  // raw lam centres
+};
+
+template <int MODE, int V>
+__device__ __forceinline__ AdjRows<MODE, V> load_adj_row(const float* l1x, const float* l1y,
+                                                          const float* coef, int64_t field, int64_t pitch,
+                                                          int r, int j0, int lane) {
+  AdjRows<MODE, V> o;
+  const Row<V> lx = load_row<V>(prow(l1x, pitch, r, j0), lane);
+  const Row<V> ly = load_row<V>(prow(l1y, pitch, r, j0), lane);
+#pragma unroll
+  for (int e = 0; e < V; ++e) { o.cx[e] = lx.v[e + 1]; o.cy[e] = ly.v[e + 1]; }
+  if constexpr (MODE == kConst6) {
+    o.w[0] = lx;
+    o.w[1] = ly;
+  } else if constexpr (MODE == kDField) {
+    const Row<V> D = load_row<V>(prow(coef, pitch, r, j0), lane);
+    o.w[0] = mul_row<V>(D, lx);
+    o.w[1] = mul_row<V>(D, ly);
+  } else {
+    Row<V> cr[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) cr[q] = load_row<V>(prow(coef + q * field, pitch, r, j0), lane);
+#pragma unroll
+    for (int e = 0; e < V + 2; ++e) {
+      const float c11 = cr[0].v[e], c22 = cr[1].v[e], c12 = cr[2].v[e];
+      const float c16 = cr[3].v[e], c26 = cr[4].v[e], c66 = cr[5].v[e];
+      const float x = lx.v[e], y = ly.v[e];
+      o.w[0].v[e] = c11 * x + c16 * y;                 // -> Dxx, x row
+      o.w[1].v[e] = c66 * x + c26 * y;                 // -> Dyy, x row
+      o.w[2].v[e] = 2.f * c16 * x + (c12 + c66) * y;   // -> Dxy, x row
+      o.w[3].v[e] = c16 * x + c66 * y;                 // -> Dxx, y row
+      o.w[4].v[e] = c26 * x + c22 * y;                 // -> Dyy, y row
+      o.w[5].v[e] = (c12 + c66) * x + 2.f * c26 * y;   // -> Dxy, y row
+    }
+  }
+  return o;
+}
+
+template <int MODE, int V, int R>
 __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int t = tile_index();
@@ -388,83 +438,50 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
   const bool seg_first = (m == top);
   const bool seg_last = (m == a.seg_bot);
   constexpr int NACC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
+  constexpr int NW = AdjRows<MODE, V>::NW;
   float* acc32 = C.ws + I.acc_off;
   double* acc64 = reinterpret_cast<double*>(C.ws) + I.acc64_off;
+
+  // ---- loads
+  AdjRows<MODE, V> lr[R + 2];
+#pragma unroll
+  for (int r = 0; r < R + 2; ++r)
+    lr[r] = load_adj_row<MODE, V>(l1x, l1y, coef, I.field, pitch, tl.i0 - 1 + r, j0, lane);
+  float l2vx[R][V], l2vy[R][V];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    load_vec<V>(prow(l2x, pitch, tl.i0 + r, j0) + lane * V, l2vx[r]);
+    load_vec<V>(prow(l2y, pitch, tl.i0 + r, j0) + lane * V, l2vy[r]);
+  }
+  constexpr int RU = (MODE == kConst6) ? 1 : R + 2;   // u^m rows with halo (field modes)
+  float uvx[R][V], uvy[R][V];
+  Row<V> um[RU][2];
+  if constexpr (MODE == kConst6) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      load_vec<V>(prow(umx, pitch, tl.i0 + r, j0) + lane * V, uvx[r]);
+      load_vec<V>(prow(umy, pitch, tl.i0 + r, j0) + lane * V, uvy[r]);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      um[r][0] = load_row<V>(prow(umx, pitch, tl.i0 - 1 + r, j0), lane);
+      um[r][1] = load_row<V>(prow(umy, pitch, tl.i0 - 1 + r, j0), lane);
+    }
+  }
 
   float S[6];
 #pragma unroll
   for (int p = 0; p < 6; ++p) S[p] = 0.f;
 
-  // Stencilled adjoint rows.  kConst6: lam itself.  kDField: D * lam.
-  // kField6: the six weighted fields of A^T (see apply_AT in the oracle).
-  constexpr int NW = (MODE == kField6) ? 6 : 2;
-  Row<V> wm[NW], wc[NW];
-  // raw lam centres of rows i and i+1 (needed besides the weighted rows)
-  float lcx[V], lcy[V];
-  // u^m rows for the per-cell gradient (field modes)
-  Row<V> um_m[2], um_c[2];
-
-  auto load_w = [&](int r, Row<V> (&out)[NW], float (&cx)[V], float (&cy)[V]) {
-    const Row<V> lx = load_row<V>(prow(l1x, pitch, r, j0), lane);
-    const Row<V> ly = load_row<V>(prow(l1y, pitch, r, j0), lane);
-#pragma unroll
-    for (int e = 0; e < V; ++e) { cx[e] = lx.v[e + 1]; cy[e] = ly.v[e + 1]; }
-    if constexpr (MODE == kConst6) {
-      out[0] = lx;
-      out[1] = ly;
-    } else if constexpr (MODE == kDField) {
-      const Row<V> D = load_row<V>(prow(coef, pitch, r, j0), lane);
-      out[0] = mul_row<V>(D, lx);
-      out[1] = mul_row<V>(D, ly);
-    } else {
-      Row<V> cr[6];
-#pragma unroll
-      for (int q = 0; q < 6; ++q) cr[q] = load_row<V>(prow(coef + q * I.field, pitch, r, j0), lane);
-#pragma unroll
-      for (int e = 0; e < V + 2; ++e) {
-        const float c11 = cr[0].v[e], c22 = cr[1].v[e], c12 = cr[2].v[e];
-        const float c16 = cr[3].v[e], c26 = cr[4].v[e], c66 = cr[5].v[e];
-        const float x = lx.v[e], y = ly.v[e];
-        out[0].v[e] = c11 * x + c16 * y;                 // -> Dxx, x row
-        out[1].v[e] = c66 * x + c26 * y;                 // -> Dyy, x row
-        out[2].v[e] = 2.f * c16 * x + (c12 + c66) * y;   // -> Dxy, x row
-        out[3].v[e] = c16 * x + c66 * y;                 // -> Dxx, y row
-        out[4].v[e] = c26 * x + c22 * y;                 // -> Dyy, y row
-        out[5].v[e] = (c12 + c66) * x + 2.f * c26 * y;   // -> Dxy, y row
-      }
-    }
-  };
-
-  float dummy_x[V], dummy_y[V];
-  load_w(tl.i0 - 1, wm, dummy_x, dummy_y);
-  load_w(tl.i0, wc, lcx, lcy);
-  if constexpr (MODE != kConst6) {
-    um_m[0] = load_row<V>(prow(umx, pitch, tl.i0 - 1, j0), lane);
-    um_m[1] = load_row<V>(prow(umy, pitch, tl.i0 - 1, j0), lane);
-    um_c[0] = load_row<V>(prow(umx, pitch, tl.i0, j0), lane);
-    um_c[1] = load_row<V>(prow(umy, pitch, tl.i0, j0), lane);
-  }
-
   int rp = tl.rec_begin;
-  const int iend = min(tl.i0 + a.rows, I.nx);
-  for (int i = tl.i0; i < iend; ++i) {
-    Row<V> wp[NW];
-    float lpx[V], lpy[V];
-    load_w(i + 1, wp, lpx, lpy);
-    float l2vx[V], l2vy[V];
-    load_vec<V>(prow(l2x, pitch, i, j0) + lane * V, l2vx);
-    load_vec<V>(prow(l2y, pitch, i, j0) + lane * V, l2vy);
-    float uvx[V], uvy[V];
-    Row<V> um_p[2];
-    if constexpr (MODE == kConst6) {
-      load_vec<V>(prow(umx, pitch, i, j0) + lane * V, uvx);
-      load_vec<V>(prow(umy, pitch, i, j0) + lane * V, uvy);
-    } else {
-      um_p[0] = load_row<V>(prow(umx, pitch, i + 1, j0), lane);
-      um_p[1] = load_row<V>(prow(umy, pitch, i + 1, j0), lane);
 #pragma unroll
-      for (int e = 0; e < V; ++e) { uvx[e] = um_c[0].v[e + 1]; uvy[e] = um_c[1].v[e + 1]; }
-    }
+  for (int r = 0; r < R; ++r) {
+    const int i = tl.i0 + r;
+    if (i >= I.nx) break;
+    const AdjRows<MODE, V>& Lm = lr[r];
+    const AdjRows<MODE, V>& Lc = lr[r + 1];
+    const AdjRows<MODE, V>& Lp = lr[r + 2];
     float atx[V], aty[V];
     float G[NACC > 0 ? NACC : 1][V];
 #pragma unroll
@@ -472,24 +489,31 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
       const int x = e + 1;
       float g[6];
       if constexpr (MODE == kConst6 || MODE == kDField) {
-        // A^T lam = A(C0) (D * lam); constant stencils on the (weighted) adjoint field
-        const Deriv dl = derivs<V>(wm[0], wc[0], wp[0], wm[1], wc[1], wp[1], x, w);
+        // A^T lam = A(C0) (D * lam): constant stencils on the (weighted) adjoint field
+        const Deriv dl = derivs<V>(Lm.w[0], Lc.w[0], Lp.w[0], Lm.w[1], Lc.w[1], Lp.w[1], x, w);
         apply_op(sc, dl, atx[e], aty[e]);
-        grad_terms_sbp(dl, uvx[e], uvy[e], g);
+        float ux, uy;
+        if constexpr (MODE == kConst6) { ux = uvx[r][e]; uy = uvy[r][e]; }
+        else { ux = um[r + 1][0].v[x]; uy = um[r + 1][1].v[x]; }
+        grad_terms_sbp(dl, ux, uy, g);
 #pragma unroll
         for (int p = 0; p < 6; ++p) S[p] += g[p];
         if constexpr (MODE == kDField) {
           // dJ/dD(cell) = lam . A(C0) u^m  at the cell
-          const Deriv du = derivs<V>(um_m[0], um_c[0], um_p[0], um_m[1], um_c[1], um_p[1], x, w);
+          const Deriv du = derivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
+                                     um[r + 2][1], x, w);
           float ox, oy;
           apply_op(sc, du, ox, oy);
-          G[0][e] = lcx[e] * ox + lcy[e] * oy;
+          G[0][e] = Lc.cx[e] * ox + Lc.cy[e] * oy;
         }
       } else {
-        atx[e] = (d_xx<V>(wm[0], wc[0], wp[0], x, w) + d_yy<V>(wc[1], x, w)) + d_xy<V>(wm[2], wp[2], x, w);
-        aty[e] = (d_xx<V>(wm[3], wc[3], wp[3], x, w) + d_yy<V>(wc[4], x, w)) + d_xy<V>(wm[5], wp[5], x, w);
-        const Deriv du = derivs<V>(um_m[0], um_c[0], um_p[0], um_m[1], um_c[1], um_p[1], x, w);
-        grad_terms(du, lcx[e], lcy[e], g);
+        atx[e] = (d_xx<V>(Lm.w[0], Lc.w[0], Lp.w[0], x, w) + d_yy<V>(Lc.w[1], x, w)) +
+                 d_xy<V>(Lm.w[2], Lp.w[2], x, w);
+        aty[e] = (d_xx<V>(Lm.w[3], Lc.w[3], Lp.w[3], x, w) + d_yy<V>(Lc.w[4], x, w)) +
+                 d_xy<V>(Lm.w[5], Lp.w[5], x, w);
+        const Deriv du = derivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
+                                   um[r + 2][1], x, w);
+        grad_terms(du, Lc.cx[e], Lc.cy[e], g);
 #pragma unroll
         for (int p = 0; p < 6; ++p) G[p][e] = g[p];
       }
@@ -498,10 +522,8 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
     float nvx[V], nvy[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      float vx = (2.f * lcx[e] - l2vx[e]) + I.dt2 * (atx[e] / I.rho);
-      float vy = (2.f * lcy[e] - l2vy[e]) + I.dt2 * (aty[e] / I.rho);
-      nvx[e] = vx;
-      nvy[e] = vy;
+      nvx[e] = (2.f * Lc.cx[e] - l2vx[r][e]) + I.dt2 * (atx[e] / I.rho);
+      nvy[e] = (2.f * Lc.cy[e] - l2vy[r][e]) + I.dt2 * (aty[e] / I.rho);
     }
     if (inj) {
       if (a.snapshot) {
@@ -560,15 +582,6 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
           store_vec<V>(a32, v);
         }
       }
-    }
-
-#pragma unroll
-    for (int q = 0; q < NW; ++q) { wm[q] = wc[q]; wc[q] = wp[q]; }
-#pragma unroll
-    for (int e = 0; e < V; ++e) { lcx[e] = lpx[e]; lcy[e] = lpy[e]; }
-    if constexpr (MODE != kConst6) {
-      um_m[0] = um_c[0]; um_m[1] = um_c[1];
-      um_c[0] = um_p[0]; um_c[1] = um_p[1];
     }
   }
 
@@ -662,33 +675,31 @@ __global__ void finalize_field_kernel(const CallDev* call, const InstDev* inst, 
 // --------------------------------------------------------------------------
 // Launchers (host side, called from ernn_plan.cpp)
 // --------------------------------------------------------------------------
-template <int MODE, int V>
-static hipError_t launch_fwd_t(const StepArgs& a, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL((fwd_step_kernel<MODE, V>), dim3(blocks), dim3(kBlock), 0, st, a);
-  return hipGetLastError();
-}
-template <int MODE, int V>
-static hipError_t launch_adj_t(const StepArgs& a, int blocks, hipStream_t st) {
-  hipLaunchKernelGGL((adj_step_kernel<MODE, V>), dim3(blocks), dim3(kBlock), 0, st, a);
-  return hipGetLastError();
-}
+// Tile shapes with a compiled kernel: V columns per lane x R rows, V * R <= 16.
+#define ERNN_TILE_CASES(X) \
+  X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(1, 16) X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(4, 4)
 
 template <int MODE>
 static hipError_t launch_fwd_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
-  switch (V) {
-    case 1: return launch_fwd_t<MODE, 1>(a, blocks, st);
-    case 2: return launch_fwd_t<MODE, 2>(a, blocks, st);
-    case 4: return launch_fwd_t<MODE, 4>(a, blocks, st);
+#define ERNN_FWD_CASE(v, r)                                                                       \
+  if (V == v && a.rows == r) {                                                                    \
+    hipLaunchKernelGGL((fwd_step_kernel<MODE, v, r>), dim3(blocks), dim3(kBlock), 0, st, a);     \
+    return hipGetLastError();                                                                     \
   }
+  ERNN_TILE_CASES(ERNN_FWD_CASE)
+#undef ERNN_FWD_CASE
   return hipErrorInvalidValue;
 }
+
 template <int MODE>
 static hipError_t launch_adj_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
-  switch (V) {
-    case 1: return launch_adj_t<MODE, 1>(a, blocks, st);
-    case 2: return launch_adj_t<MODE, 2>(a, blocks, st);
-    case 4: return launch_adj_t<MODE, 4>(a, blocks, st);
+#define ERNN_ADJ_CASE(v, r)                                                                       \
+  if (V == v && a.rows == r) {                                                                    \
+    hipLaunchKernelGGL((adj_step_kernel<MODE, v, r>), dim3(blocks), dim3(kBlock), 0, st, a);     \
+    return hipGetLastError();                                                                     \
   }
+  ERNN_TILE_CASES(ERNN_ADJ_CASE)
+#undef ERNN_ADJ_CASE
   return hipErrorInvalidValue;
 }
 

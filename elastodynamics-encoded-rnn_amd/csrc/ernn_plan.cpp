@@ -66,8 +66,13 @@ int fail(int code, const std::string& msg) {
     if (r_ != ERNN_OK) return r_; \
   } while (0)
 
-constexpr int kMinRows = 2;
-constexpr int kMaxRows = 4096;
+// Tile shapes with a compiled kernel (ERNN_TILE_CASES in ernn_kernels.hip):
+// V in {1, 2, 4} columns per lane, R a power of two rows, V * R <= 16.
+bool tile_ok(int V, int R) {
+  if (!(V == 1 || V == 2 || V == 4)) return false;
+  if (R < 1 || R > kRowAlign || (R & (R - 1)) != 0) return false;
+  return V * R <= 16;
+}
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
@@ -157,8 +162,8 @@ int64_t tiles_for(const Plan& p, int V, int R) {
 }
 
 int build_tiling(Plan* p, Tiling* t, int V, int R) {
-  if (!(V == 1 || V == 2 || V == 4)) return fail(ERNN_E_INVALID, "tile V must be 1, 2 or 4");
-  if (R < kMinRows || R > kMaxRows) return fail(ERNN_E_INVALID, "tile R out of range [2, 4096]");
+  if (!tile_ok(V, R))
+    return fail(ERNN_E_INVALID, "tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
   const int64_t nt64 = tiles_for(*p, V, R);
   if (nt64 > (int64_t)1 << 30) return fail(ERNN_E_CAPACITY, "too many tiles");
   std::vector<TileDev> tiles;
@@ -252,11 +257,10 @@ int validate(const ernn_desc_t* d) {
     return fail(ERNN_E_INVALID, "unknown param_mode");
   if (d->rec_stride < 1) return fail(ERNN_E_INVALID, "rec_stride must be >= 1");
   if (d->ckpt_every < 2) return fail(ERNN_E_INVALID, "ckpt_every must be >= 2");
-  for (int v : {d->fwd_v, d->adj_v})
-    if (!(v == 0 || v == 1 || v == 2 || v == 4)) return fail(ERNN_E_INVALID, "tile V must be 1, 2 or 4");
-  for (int r : {d->fwd_r, d->adj_r})
-    if (r != 0 && (r < kMinRows || r > kMaxRows))
-      return fail(ERNN_E_INVALID, "tile R out of range [2, 4096]");
+  if ((d->fwd_v != 0 || d->fwd_r != 0) && !tile_ok(d->fwd_v, d->fwd_r))
+    return fail(ERNN_E_INVALID, "forward tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
+  if ((d->adj_v != 0 || d->adj_r != 0) && !tile_ok(d->adj_v, d->adj_r))
+    return fail(ERNN_E_INVALID, "adjoint tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))
@@ -380,7 +384,7 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
     D.dt2 = (float)(I.dt * I.dt);
     D.rho = (float)I.rho;
     D.gscale = (double)D.dt2 / (double)D.rho;
-    D.field = (int64_t)(I.nx + 3) * D.pitch;
+    D.field = (round_up(I.nx, kRowAlign) + 3) * (int64_t)D.pitch;
     D.pool_off = off;
     off += round_up((int64_t)p->n_slots() * 2 * D.field, 64);
     D.trace_off = p->trace_off[b];
@@ -409,7 +413,7 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
       off += round_up((int64_t)2 * p->n_acc * p->inst[b].field, 64);
     }
   }
-  p->partial_cap = tiles_for(*p, 1, kMinRows);
+  p->partial_cap = tiles_for(*p, 1, 1);
   p->partial_off = off / 2;
   off += round_up(2 * 6 * p->partial_cap, 64);
   p->ws_floats = off;
@@ -422,8 +426,8 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   ERNN_TRY(dev_upload(&p->d_set_begin, p->set_begin));
   ERNN_TRY(dev_upload(&p->d_set_ext_off, p->set_ext_off));
   ERNN_TRY(dev_upload(&p->d_set_grad_off, p->set_grad_off));
-  ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 2, d->fwd_r ? d->fwd_r : 8));
-  ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 2, d->adj_r ? d->adj_r : 8));
+  ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 1, d->fwd_r ? d->fwd_r : 4));
+  ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 1, d->adj_r ? d->adj_r : 4));
   ERNN_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
   return ERNN_OK;
 }
@@ -441,7 +445,7 @@ CallDev to_dev(const ernn_call_t* c) {
   return v;
 }
 
-int check_call(const Plan* p, const ernn_call_t* c, bool adjoint) {
+int check_call(const Plan* p, const ernn_call_t* c, bool adjoint, bool grads = true) {
   if (!c) return fail(ERNN_E_INVALID, "null call");
   if (!c->workspace) return fail(ERNN_E_INVALID, "null workspace");
   if ((reinterpret_cast<uintptr_t>(c->workspace) & 255) != 0)
@@ -452,8 +456,10 @@ int check_call(const Plan* p, const ernn_call_t* c, bool adjoint) {
   if (!adjoint && !c->traces) return fail(ERNN_E_INVALID, "null traces");
   if (adjoint) {
     if (!c->grad_traces) return fail(ERNN_E_INVALID, "null grad_traces");
-    if (p->mode != ERNN_PARAM_FIELD6 && !c->grad_params) return fail(ERNN_E_INVALID, "null grad_params");
-    if (p->mode != ERNN_PARAM_CONST6 && !c->grad_field) return fail(ERNN_E_INVALID, "null grad_field");
+    if (grads && p->mode != ERNN_PARAM_FIELD6 && !c->grad_params)
+      return fail(ERNN_E_INVALID, "null grad_params");
+    if (grads && p->mode != ERNN_PARAM_CONST6 && !c->grad_field)
+      return fail(ERNN_E_INVALID, "null grad_field");
   }
   return ERNN_OK;
 }
@@ -669,7 +675,7 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   if (!plan || !ms_out || steps < 1 || (which != 0 && which != 1))
     return fail(ERNN_E_INVALID, "bad argument");
   Plan* p = static_cast<Plan*>(plan);
-  ERNN_TRY(check_call(p, call, which == 1));
+  ERNN_TRY(check_call(p, call, which == 1, /*grads=*/false));  // timing never runs the finalize kernels
   hipStream_t st = static_cast<hipStream_t>(stream);
   ERNN_HIP(hipSetDevice(p->device));
   ERNN_HIP(launch_set_call(p->d_call, to_dev(call), st));

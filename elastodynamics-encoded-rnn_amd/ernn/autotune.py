@@ -22,11 +22,13 @@ from .problem import Survey
 
 LEFT_PAD = 32
 COL_ALIGN = 256
+ROW_ALIGN = 16
 
 
 def padded_field_floats(nx: int, ny: int) -> int:
+    """Floats of one padded field (the library's layout, ernn_internal.h)."""
     pitch = LEFT_PAD + (ny + COL_ALIGN - 1) // COL_ALIGN * COL_ALIGN
-    return (nx + 3) * pitch
+    return ((nx + ROW_ALIGN - 1) // ROW_ALIGN * ROW_ALIGN + 3) * pitch
 
 
 def slots(nt: int, K: int) -> int:
@@ -50,19 +52,22 @@ def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
     return best
 
 
+# every compiled tile shape: V columns per lane x R rows, V * R <= 16
 DEFAULT_CANDIDATES: Tuple[Tuple[int, int], ...] = (
-    (1, 4), (1, 8), (1, 16), (2, 4), (2, 8), (2, 16), (2, 32), (4, 4), (4, 8), (4, 16), (4, 32))
+    (1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (2, 1), (2, 2), (2, 4), (2, 8), (4, 1), (4, 2), (4, 4))
 
 
-def tune_tiles(engine, params, field, wavelets, steps: int = 20,
+def tune_tiles(engine, params, field, wavelets, steps: int = 50,
                candidates: Sequence[Tuple[int, int]] = DEFAULT_CANDIDATES,
-               which: Iterable[int] = (0, 1), repeats: int = 2) -> dict:
+               which: Iterable[int] = (0, 1), repeats: int = 3) -> dict:
     """Time candidates for the forward (0) and adjoint (1) kernels; keep the best.
 
     Returns {which: (V, R, ms_per_step, {candidate: ms})}.
     """
     ws = engine.acquire_workspace()
     traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)
+    for w in which:   # bring clocks and caches to steady state before the first candidate
+        engine.time_steps(w, steps, params, field, wavelets, ws, traces, traces)
     result = {}
     for w in which:
         times = {}

@@ -82,7 +82,8 @@ typedef struct {
   int32_t rec_stride;   /* P: frame k records u^{kP+1} (PI:294-301); 1 = save_all_steps */
   int32_t snapshot;     /* 1: frames are whole fields (PI:304-305); 0: receiver traces */
   int32_t ckpt_every;   /* K >= 2: checkpoint interval of the adjoint recompute */
-  int32_t fwd_v, fwd_r; /* forward tile: V in {1,2,4} columns per lane, R rows */
+  int32_t fwd_v, fwd_r; /* forward tile: V in {1,2,4} columns per lane, R in {1,2,4,8,16}
+                           rows, V * R <= 16; 0, 0 = default (1, 4) */
   int32_t adj_v, adj_r; /* adjoint tile */
   int32_t use_graph;    /* 1: replay each pass as a captured hipGraph */
   int32_t reserved;

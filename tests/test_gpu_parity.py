@@ -32,7 +32,7 @@ def _sources(nx, ny, h):
 
 
 GRIDS = [(37, 70), (20, 130), (64, 64), (5, 3)]
-TILES = [(1, 2), (1, 3), (2, 8), (4, 5), (4, 32)]
+TILES = [(1, 1), (1, 2), (2, 8), (4, 4), (1, 16)]
 
 
 @pytest.mark.parametrize("nx,ny", GRIDS)
@@ -63,7 +63,7 @@ def test_const6_gradient_matches_oracle_and_is_checkpoint_invariant():
     ref, gcell = oracle_run(shot, F32(C_ANISO), 1, False, G)
     gref = orc.const_gradient(gcell)
     first = None
-    for K, tiles in [(2, (2, 8, 2, 8)), (3, (1, 2, 4, 5)), (7, (4, 32, 1, 3)), (41, (2, 8, 2, 8)), (64, (2, 8, 2, 8))]:
+    for K, tiles in [(2, (2, 8, 2, 8)), (3, (1, 2, 4, 4)), (7, (4, 4, 1, 16)), (41, (2, 8, 2, 8)), (64, (2, 8, 2, 8))]:
         sv = ernn.Survey([shot], mode="const6", ckpt_every=K, tiles=tiles)
         eng, tr, (gp, gf) = run_hip(sv, params=C_ANISO, G=G)
         assert gf is None
@@ -96,7 +96,7 @@ def test_dfield_forward_and_gradients_match_oracle():
     D = rng.uniform(0.4, 1.0, (nx, ny)).astype(np.float32)
     shots = [make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=_receivers(nx, ny)),
              make_shot(nx, ny, nt, h, sources=ernn.point_source(3, ny - 1, h), receivers=_receivers(nx, ny))]
-    sv = ernn.Survey(shots, mode="dfield", ckpt_every=5, tiles=(2, 4, 1, 3))
+    sv = ernn.Survey(shots, mode="dfield", ckpt_every=5, tiles=(2, 4, 1, 8))
     Gs = [rng.standard_normal((2, nt, len(s.receivers))) for s in shots]
     C32 = F32(C_ANISO)
     cells = [np.float32(D) * np.float32(c) for c in C_ANISO]     # fl32(D * C0) like the kernel
@@ -120,7 +120,7 @@ def test_field6_forward_and_gradients_match_oracle():
     shot = make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=_receivers(nx, ny))
     G = rng.standard_normal((2, nt, len(shot.receivers)))
     ref, gcell = oracle_run(shot, [f.astype(np.float64) for f in fields], 1, False, G)
-    for tiles in [(1, 2, 1, 2), (2, 8, 2, 8), (4, 5, 4, 5)]:
+    for tiles in [(1, 2, 1, 2), (2, 8, 2, 8), (4, 4, 4, 4), (1, 1, 2, 1)]:
         sv = ernn.Survey([shot], mode="field6", ckpt_every=6, tiles=tiles)
         eng, tr, (gp, gf) = run_hip(sv, field=fields, G=G)
         assert gp is None

@@ -67,8 +67,9 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
     ({}, {"ckpt_every": 1}, "ckpt_every"),
     ({}, {"rec_stride": 0}, "rec_stride"),
     ({}, {"param_mode": 7}, "param_mode"),
-    ({}, {"fwd_v": 3}, "tile V"),
-    ({}, {"adj_r": 1}, "tile R"),
+    ({}, {"fwd_v": 3}, "tile (V, R)"),
+    ({}, {"adj_r": 3}, "tile (V, R)"),
+    ({}, {"fwd_v": 4, "fwd_r": 8}, "tile (V, R)"),
 ])
 def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
     inst = dict(GOOD, **patch)
@@ -135,10 +136,10 @@ def test_damage_1024_fits_the_32_gb_cap():
 
 
 def test_padded_layout_matches_library_convention():
-    # pitch = 32 + round_up(ny, 256); rows nx + 3
+    # pitch = 32 + round_up(ny, 256); rows round_up(nx, 16) + 3
     assert autotune.padded_field_floats(128, 128) == 131 * 288
-    assert autotune.padded_field_floats(10, 256) == 13 * 288
-    assert autotune.padded_field_floats(10, 257) == 13 * 544
+    assert autotune.padded_field_floats(10, 256) == 19 * 288
+    assert autotune.padded_field_floats(10, 257) == 19 * 544
 
 
 def test_workload_presets():
