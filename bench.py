@@ -61,6 +61,10 @@ def parse():
     ap.add_argument("--baseline", type=int, default=1, help="time the autograd plain path on the GPU")
     ap.add_argument("--cpu", type=int, default=1, help="time the CPU port")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--baseline-window", type=int, default=2000,
+                    help="steps of one shot timed for the autograd baseline (scaled to the survey)")
+    ap.add_argument("--per-instance", type=int, default=1,
+                    help="batched surveys: time every instance alone for p50 / max")
     return ap.parse_args()
 
 
@@ -76,11 +80,15 @@ def _weights(torch, h, device, dtype):
     return s * a, s * b, s * c
 
 
-def autograd_traces(torch, shot, C, faithful):
-    """Receiver traces (2, nt, n_rec) through the reference's op sequence."""
+def autograd_traces(torch, shot, C, faithful, D=None, nsteps=None):
+    """Receiver traces (2, nsteps, n_rec) through the reference's op sequence.
+
+    C: six 0-dim tensors; D: optional (nx, ny) damage field, in which case the
+    per-cell operator of DI:100-176 runs on the fields D * C_p."""
     import torch.nn.functional as F
     dev = C[0].device
     nx, ny = shot.nx, shot.ny
+    nt = shot.nt if nsteps is None else nsteps
     sig = torch.tensor(shot.wavelet, device=dev)
     g = torch.zeros(1, nx, ny, device=dev)
     for (i, j, w, comp) in shot.sources:
@@ -88,13 +96,16 @@ def autograd_traces(torch, shot, C, faithful):
     rho = torch.tensor(shot.rho, device=dev)
     ri = torch.as_tensor(shot.receivers[:, 0], device=dev).long()
     rj = torch.as_tensor(shot.receivers[:, 1], device=dev).long()
-    C11, C22, C12, C16, C26, C66 = C
+    if D is None:
+        C11, C22, C12, C16, C26, C66 = C
+    else:   # per-cell fields on the transposed view (DI:143-148)
+        C11, C22, C12, C16, C26, C66 = ((D * c).unsqueeze(0).transpose(1, 2) for c in C)
     W = None if faithful else _weights(torch, shot.h, dev, torch.float32)
     ux1 = torch.zeros(1, nx, ny, device=dev); uy1 = torch.zeros_like(ux1)
     ux2 = torch.zeros_like(ux1); uy2 = torch.zeros_like(ux1)
     hx, hy = [], []
     conv = lambda f, k: F.conv2d(F.pad(f, (1, 1, 1, 1)), k)
-    for n in range(shot.nt):
+    for n in range(nt):
         fy = sig[n] * g
         d2dx2, d2dy2, d2dxdy = W if W is not None else _weights(torch, shot.h, dev, torch.float32)
         ux = ux1.transpose(1, 2).unsqueeze(1); uy = uy1.transpose(1, 2).unsqueeze(1)
@@ -118,32 +129,44 @@ def autograd_traces(torch, shot, C, faithful):
     return torch.stack([torch.stack(hx), torch.stack(hy)])
 
 
-def time_autograd(torch, wl, theta0, scale, observed, iters=1, warmup=1):
-    """s/Adam-iter of the plain path (one shot at a time, summed over shots)."""
+def time_autograd(torch, wl, theta0, scale, D0, observed, window):
+    """s/Adam-iter of the plain path (shots one at a time, as the reference).
+
+    If the survey is longer than `window` steps or has several shots, one shot
+    is timed over `window` steps and the per-step cost is scaled to
+    n_shots * nt steps (per-step cost is constant in n)."""
     res = {}
+    shots = wl.survey.shots
+    s0 = shots[0]
+    W = min(window, s0.nt)
+    scale_f = sum(s.nt for s in shots) / W
     for faithful in (True, False):
         theta = torch.nn.Parameter(theta0.clone())
-        opt = torch.optim.Adam([theta], lr=1e-2)
+        params = [theta]
+        D = None
+        if D0 is not None:
+            D = torch.nn.Parameter(D0.clone())
+            params.append(D)
+        opt = torch.optim.Adam(params, lr=1e-2)
 
         def it():
             opt.zero_grad()
             C = list(theta[0] * scale[0])
-            loss = 0.0
-            for b, s in enumerate(wl.survey.shots):
-                tr = autograd_traces(torch, s, C, faithful)
-                loss = loss + torch.mean((tr - observed[b]) ** 2)
+            tr = autograd_traces(torch, s0, C, faithful, D=D, nsteps=W)
+            loss = torch.mean((tr - observed[0][:, :W]) ** 2)
             loss.backward()
             opt.step()
-        for _ in range(warmup):
-            it()
+        it()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(iters):
-            it()
+        it()
         torch.cuda.synchronize()
-        res["faithful" if faithful else "hoisted"] = (time.perf_counter() - t0) / iters
-        log(f"autograd plain path ({'faithful' if faithful else 'hoisted'}): "
-            f"{res['faithful' if faithful else 'hoisted']:.3f} s/iter")
+        key = "faithful" if faithful else "hoisted"
+        res[key] = (time.perf_counter() - t0) * scale_f
+        log(f"autograd plain path ({key}): {res[key]:.3f} s/iter"
+            + (f" (extrapolated from {W} steps of one shot)" if scale_f != 1 else ""))
+    res["window_steps"] = W
+    res["extrapolated"] = scale_f != 1
     return res
 
 
@@ -176,6 +199,29 @@ def cpu_baseline(torch, wl, budget_s):
             "kind": "port",
             "sample": f"oracle/plain_torch.py forward + autograd backward, {s.nx}x{s.ny} grid, "
                       f"{n} of {s.nt} steps, torch {threads} threads, {t:.1f} s"}
+
+
+def time_instances(torch, ernn, wl, C_init, D_init, dev, iters):
+    """Per-instance fwd+adjoint time (each instance alone on the GPU)."""
+    import numpy as np
+    times = []
+    for b, s in enumerate(wl.survey.shots):
+        one = ernn.Shot(**{**s.__dict__, "set": 0})
+        sv1 = ernn.Survey([one], mode=wl.survey.mode, n_sets=1, ckpt_every=wl.survey.ckpt_every,
+                          use_graph=wl.survey.use_graph)
+        m = ernn.ERNN(sv1, C_init[s.set:s.set + 1], D_init=D_init, device=dev)
+        for k in range(iters + 1):
+            if k == 1:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+            m.zero_grad()
+            m().square().mean().backward()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) / iters)
+        m.engine.close()
+    t = np.array(times)
+    return {"p50_s": round(float(np.median(t)), 5), "max_s": round(float(t.max()), 5),
+            "sum_s": round(float(t.sum()), 5), "n": len(t)}
 
 
 def main():
@@ -272,15 +318,21 @@ def main():
         f"{name}: {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.2%} of HBM peak)")
 
     baseline = None
-    if args.baseline and sv.mode == "const6" and len(sv.shots) == 1:
+    if args.baseline and sv.n_sets == 1:
         theta0 = torch.tensor(C_init / np.where(np.abs(C_init) > 0, np.abs(C_init), 5e9),
                               dtype=torch.float32, device=dev)
         obs_shots = [eng.shot_traces(obs, b) for b in range(len(sv.shots))]
-        ag = time_autograd(torch, wl, theta0, model.scale, obs_shots)
-        best = min(ag.values())
+        D0 = None if sv.mode == "const6" else torch.tensor(D_init, device=dev)
+        ag = time_autograd(torch, wl, theta0, model.scale, D0, obs_shots, args.baseline_window)
+        best = min(ag["faithful"], ag["hoisted"])
         baseline = {"s_per_adam_iter_faithful": round(ag["faithful"], 4),
                     "s_per_adam_iter_hoisted": round(ag["hoisted"], 4),
-                    "value": round(sv.cell_steps() / best / 1e9, 4), "unit": "Gcell-steps/s"}
+                    "value": round(sv.cell_steps() / best / 1e9, 4), "unit": "Gcell-steps/s",
+                    "window_steps": ag["window_steps"], "extrapolated": ag["extrapolated"]}
+
+    per_instance = None
+    if args.per_instance and len(sv.shots) > 1:
+        per_instance = time_instances(torch, ernn, wl, C_init, D_init, dev, args.steps)
 
     cpu = None
     if args.cpu:
@@ -302,6 +354,7 @@ def main():
         "s_per_adam_iter": round(dt_iter, 5),
         "autograd_baseline": baseline,
         "speedup_vs_autograd": (round(value / baseline["value"], 2) if baseline else None),
+        "per_instance_alone": per_instance,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

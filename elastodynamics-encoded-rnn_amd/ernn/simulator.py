@@ -256,14 +256,15 @@ class ERNN(nn.Module):
     constant (scale = |C_init|, or `zero_scale` for constants that start at 0),
     which, unlike exp/clamp (PI:223-232), can reach zero and negative values.
     In dfield mode the per-cell factor D multiplies all six constants and is a
-    parameter too (``learn_damage``).
+    parameter too (``learn_damage``).  ``trainable`` (six booleans) freezes
+    constants known a priori, e.g. C16 = C26 = 0 in an orthotropic frame.
     forward() returns the flat trace tensor; ``shot_traces``/``stacked`` give views.
     """
 
     def __init__(self, survey: Survey, C_init: np.ndarray, scale: Optional[np.ndarray] = None,
                  zero_scale: float = 5e9, D_init: Optional[np.ndarray] = None,
                  learn_constants: bool = True, learn_damage: bool = True, device=None,
-                 autotune: bool = False):
+                 autotune: bool = False, trainable: Optional[Sequence[bool]] = None):
         super().__init__()
         self.survey = survey
         dev = torch.device(device) if device is not None else torch.device("cuda")
@@ -273,6 +274,11 @@ class ERNN(nn.Module):
         self.register_buffer("scale", torch.tensor(np.asarray(scale).reshape(-1, 6), dtype=torch.float32, device=dev))
         self.theta = nn.Parameter(torch.tensor(C_init / scale, dtype=torch.float32, device=dev),
                                   requires_grad=learn_constants)
+        if trainable is not None and learn_constants:
+            # constants known a priori (e.g. C16 = C26 = 0 for an orthotropic frame) keep their value
+            mask = torch.tensor([float(bool(t)) for t in trainable], device=dev).expand_as(self.theta)
+            self.register_buffer("trainable_mask", mask.contiguous())
+            self.theta.register_hook(lambda g: g * self.trainable_mask)
         if survey.mode == "dfield":
             if D_init is None:
                 D_init = np.ones(sum(int(np.prod(survey.set_grid(s))) for s in range(survey.n_sets)), np.float32)
