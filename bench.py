@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--nt", type=int, default=0, help="override the step count (profiling runs)")
     ap.add_argument("--graph", type=int, default=1, help="replay passes as captured hipGraphs")
     ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--tiles", default="", help="fixed tiles 'Vf,Rf,Va,Ra,Bf,Ba' (skips autotuning)")
     ap.add_argument("--baseline", type=int, default=1, help="time the autograd plain path on the GPU")
     ap.add_argument("--cpu", type=int, default=1, help="time the CPU port")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -254,10 +255,15 @@ def main():
     params_true = torch.tensor(C_true.reshape(-1), dtype=torch.float32, device=dev)
     field_true = None if sv.mode == "const6" else torch.tensor(wl.D_true.reshape(-1), device=dev)
     tune = None
-    if args.autotune:
+    if args.tiles:
+        vf, rf, va, ra, bf, ba = (int(x) for x in args.tiles.split(","))
+        eng.set_tiling(0, vf, rf, bf)
+        eng.set_tiling(1, va, ra, ba)
+        log(f"tiles fixed: fwd V={vf} R={rf} B={bf}, adj V={va} R={ra} B={ba}")
+    elif args.autotune:
         tune = tune_tiles(eng, params_true, field_true, model.wavelets, steps=20)
-        log("autotune: " + ", ".join(f"{'fwd' if w == 0 else 'adj'} V={v} R={r} {ms * 1e3:.1f} us/step"
-                                     for w, (v, r, ms, _) in tune.items()))
+        log("autotune: " + ", ".join(f"{'fwd' if w == 0 else 'adj'} V={v} R={r} B={b} {ms * 1e3:.1f} us/step"
+                                     for w, (v, r, b, ms, _) in tune.items()))
     with torch.no_grad():
         obs = ernn.ernn_traces(eng, params=params_true, field=field_true, wavelets=model.wavelets)
         gen = torch.Generator(device="cpu").manual_seed(1)
@@ -349,7 +355,7 @@ def main():
         "config": {"workload": wl.name, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
                    "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
                    "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every), "graph": bool(sv.use_graph),
-                   "tiles": [int(eng.info.fwd_v), int(eng.info.fwd_r), int(eng.info.adj_v), int(eng.info.adj_r)],
+                   "tiles": list(eng.tiles),
                    "parallelism": "none"},
         "s_per_adam_iter": round(dt_iter, 5),
         "autograd_baseline": baseline,

@@ -187,13 +187,25 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Tile index of this wavefront.  gridDim.x is a multiple of 8; block b runs
-// on XCD b % 8 and XCD x takes the contiguous logical blocks
-// [x * G/8, (x+1) * G/8).
-__device__ __forceinline__ int tile_index() {
+// Tiles of this wavefront.  Block b runs on XCD b % 8 (dispatch is round
+// robin over the 8 XCDs); XCD x owns the contiguous band of tiles
+// [x * C, (x + 1) * C), C = ceil(n_tiles / 8), so halo rows and the next
+// step's reads of the same band hit that XCD's L2.  Its waves stride through
+// the band, so a grid capped below the tile count makes them persistent.
+struct TileRange {
+  int t, end, step;
+};
+
+__device__ __forceinline__ TileRange tile_range(int n_tiles) {
   const int b = blockIdx.x;
-  const int lb = (b & 7) * (gridDim.x >> 3) + (b >> 3);
-  return __builtin_amdgcn_readfirstlane(lb * kWavesPerBlock + (threadIdx.x >> 6));
+  const int per_xcd = gridDim.x >> 3;
+  const int x = b & 7;
+  const int chunk = (n_tiles + 7) >> 3;
+  TileRange r;
+  r.t = __builtin_amdgcn_readfirstlane(x * chunk + (b >> 3) * kWavesPerBlock + (int)(threadIdx.x >> 6));
+  r.end = min((x + 1) * chunk, n_tiles);
+  r.step = per_xcd * kWavesPerBlock;
+  return r;
 }
 
 __device__ __forceinline__ const float* prow(const float* base, int64_t pitch, int r, int j0) {
@@ -213,15 +225,9 @@ __device__ __forceinline__ float* prow(float* base, int64_t pitch, int r, int j0
 // the loads need no bounds checks; stores and events stop at row nx.
 // --------------------------------------------------------------------------
 template <int MODE, int V, int R>
-__global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int t = tile_index();
-  if (t >= a.n_tiles) return;
-  const TileDev tl = a.tiles[t];
-  const InstDev I = a.inst[tl.inst];
+__device__ __forceinline__ void fwd_tile(const StepArgs& a, const CallDev& C, const TileDev& tl,
+                                         const InstDev& I, int lane) {
   const int n = a.step;
-  if (n >= I.nt) return;
-  const CallDev C = *a.call;
 
   const int64_t pitch = I.pitch;
   const int j0 = tl.j0;
@@ -346,6 +352,25 @@ __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
   }
 }
 
+template <int MODE, int V, int R>
+__global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const TileRange tr = tile_range(a.n_tiles);
+  if (tr.t >= tr.end) return;
+  const CallDev C = *a.call;
+  int cur = -1;
+  InstDev I;
+  for (int t = tr.t; t < tr.end; t += tr.step) {
+    const TileDev tl = a.tiles[t];
+    if (tl.inst != cur) {
+      I = a.inst[tl.inst];
+      cur = tl.inst;
+    }
+    if (a.step >= I.nt) continue;   // this instance has finished its time loop
+    fwd_tile<MODE, V, R>(a, C, tl, I, lane);
+  }
+}
+
 // --------------------------------------------------------------------------
 // Adjoint step: lam^m = r^m + (2I + (dt^2/rho) A^T) lam^{m+1} - lam^{m+2}
 // with the gradient terms (dt^2/rho) <lam^{m+1}, dA/dC u^m>.  Same tile
@@ -397,15 +422,12 @@ __device__ __forceinline__ AdjRows<MODE, V> load_adj_row(const float* l1x, const
 }
 
 template <int MODE, int V, int R>
-__global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int t = tile_index();
-  if (t >= a.n_tiles) return;
-  const TileDev tl = a.tiles[t];
-  const InstDev I = a.inst[tl.inst];
+__device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, const TileDev& tl,
+                                         const InstDev& I, int t, int lane) {
+  // The adjoint is held to the gradient tolerance, not to the reference's
+  // rounding order, so it lets the compiler fuse multiply-adds.
+#pragma clang fp contract(fast)
   const int m = a.step;
-  if (m > I.nt) return;  // this instance has not started its adjoint yet
-  const CallDev C = *a.call;
 
   const int64_t pitch = I.pitch;
   const int j0 = tl.j0;
@@ -427,6 +449,7 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
     for (int p = 0; p < 6; ++p) c0[p] = C.params[tl.set * 6 + p];
   }
   const Stiff sc = make_stiff(c0[0], c0[1], c0[2], c0[3], c0[4], c0[5]);
+  const float kdt = (float)I.gscale;   // dt^2 / rho
 
   // residual injection on recorded steps m = kP + 1
   const bool inj = ((m - 1) % a.stride == 0) && ((m - 1) / a.stride < I.n_frames);
@@ -522,8 +545,8 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
     float nvx[V], nvy[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      nvx[e] = (2.f * Lc.cx[e] - l2vx[r][e]) + I.dt2 * (atx[e] / I.rho);
-      nvy[e] = (2.f * Lc.cy[e] - l2vy[r][e]) + I.dt2 * (aty[e] / I.rho);
+      nvx[e] = (2.f * Lc.cx[e] - l2vx[r][e]) + kdt * atx[e];
+      nvy[e] = (2.f * Lc.cy[e] - l2vy[r][e]) + kdt * aty[e];
     }
     if (inj) {
       if (a.snapshot) {
@@ -593,6 +616,25 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
       const double s = wave_sum((double)S[p]);
       if (lane == 0) (reinterpret_cast<double*>(C.ws) + a.partial_off)[(int64_t)t * 6 + p] += I.gscale * s;
     }
+  }
+}
+
+template <int MODE, int V, int R>
+__global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const TileRange tr = tile_range(a.n_tiles);
+  if (tr.t >= tr.end) return;
+  const CallDev C = *a.call;
+  int cur = -1;
+  InstDev I;
+  for (int t = tr.t; t < tr.end; t += tr.step) {
+    const TileDev tl = a.tiles[t];
+    if (tl.inst != cur) {
+      I = a.inst[tl.inst];
+      cur = tl.inst;
+    }
+    if (a.step > I.nt) continue;   // this instance has not started its adjoint yet
+    adj_tile<MODE, V, R>(a, C, tl, I, t, lane);
   }
 }
 
@@ -676,8 +718,13 @@ __global__ void finalize_field_kernel(const CallDev* call, const InstDev* inst, 
 // Launchers (host side, called from ernn_plan.cpp)
 // --------------------------------------------------------------------------
 // Tile shapes with a compiled kernel: V columns per lane x R rows, V * R <= 16.
+// The six-field adjoint at (1, 16) and (2, 8) would spill to scratch memory;
+// those shapes are not built (the build rejects any kernel that spills to
+// scratch) and the host refuses them (tile_ok in ernn_plan.cpp).
 #define ERNN_TILE_CASES(X) \
   X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(1, 16) X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(4, 4)
+#define ERNN_TILE_CASES_FIELD6_ADJ(X) \
+  X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 4) X(4, 1) X(4, 2) X(4, 4)
 
 template <int MODE>
 static hipError_t launch_fwd_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
@@ -691,17 +738,21 @@ static hipError_t launch_fwd_m(const StepArgs& a, int V, int blocks, hipStream_t
   return hipErrorInvalidValue;
 }
 
-template <int MODE>
-static hipError_t launch_adj_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
 #define ERNN_ADJ_CASE(v, r)                                                                       \
   if (V == v && a.rows == r) {                                                                    \
     hipLaunchKernelGGL((adj_step_kernel<MODE, v, r>), dim3(blocks), dim3(kBlock), 0, st, a);     \
     return hipGetLastError();                                                                     \
   }
-  ERNN_TILE_CASES(ERNN_ADJ_CASE)
-#undef ERNN_ADJ_CASE
+template <int MODE>
+static hipError_t launch_adj_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
+  if constexpr (MODE == kField6) {
+    ERNN_TILE_CASES_FIELD6_ADJ(ERNN_ADJ_CASE)
+  } else {
+    ERNN_TILE_CASES(ERNN_ADJ_CASE)
+  }
   return hipErrorInvalidValue;
 }
+#undef ERNN_ADJ_CASE
 
 hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, hipStream_t st) {
   switch (a.mode) {

@@ -73,11 +73,16 @@ bool tile_ok(int V, int R) {
   if (R < 1 || R > kRowAlign || (R & (R - 1)) != 0) return false;
   return V * R <= 16;
 }
+// The six-field adjoint is not built at (1, 16) and (2, 8) (scratch spills).
+bool adj_tile_ok(int mode, int V, int R) {
+  if (!tile_ok(V, R)) return false;
+  return !(mode == ERNN_PARAM_FIELD6 && V * R == 16 && V <= 2);
+}
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 struct Tiling {
-  int V = 0, R = 0;
+  int V = 0, R = 0, bpc = 0;
   int n_tiles = 0, blocks = 0;
   TileDev* d_tiles = nullptr;
   SrcEv* d_src = nullptr;
@@ -86,6 +91,7 @@ struct Tiling {
 
 struct Plan {
   int device = 0;
+  int n_cu = 0;
   int mode = 0, n_inst = 0, n_sets = 0, P = 1, snapshot = 0, K = 2, S = 1, nt_max = 0;
   int n_coef = 0, n_acc = 0;
   bool use_graph = false;
@@ -161,9 +167,12 @@ int64_t tiles_for(const Plan& p, int V, int R) {
   return n;
 }
 
-int build_tiling(Plan* p, Tiling* t, int V, int R) {
+int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
   if (!tile_ok(V, R))
     return fail(ERNN_E_INVALID, "tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
+  if (t == &p->adj && !adj_tile_ok(p->mode, V, R))
+    return fail(ERNN_E_INVALID, "adjoint tile (V, R) = (1, 16) and (2, 8) are not built for six fields");
+  if (bpc < 0 || bpc > 64) return fail(ERNN_E_INVALID, "blocks per CU must be in [0, 64]");
   const int64_t nt64 = tiles_for(*p, V, R);
   if (nt64 > (int64_t)1 << 30) return fail(ERNN_E_CAPACITY, "too many tiles");
   std::vector<TileDev> tiles;
@@ -225,9 +234,13 @@ int build_tiling(Plan* p, Tiling* t, int V, int R) {
   free_tiling(*t);
   t->V = V;
   t->R = R;
+  t->bpc = bpc;
   t->n_tiles = (int)tiles.size();
-  const int64_t lblocks = (t->n_tiles + kWavesPerBlock - 1) / kWavesPerBlock;
-  t->blocks = (int)round_up(std::max<int64_t>(lblocks, 1), 8);
+  // 8 XCDs, each owning ceil(n_tiles / 8) tiles; a wavefront per tile unless capped
+  const int64_t chunk = (t->n_tiles + 7) / 8;
+  int64_t per_xcd = std::max<int64_t>((chunk + kWavesPerBlock - 1) / kWavesPerBlock, 1);
+  if (bpc > 0) per_xcd = std::min<int64_t>(per_xcd, std::max<int64_t>((int64_t)bpc * p->n_cu / 8, 1));
+  t->blocks = (int)(8 * per_xcd);
   ERNN_TRY(dev_upload(&t->d_tiles, tiles));
   ERNN_TRY(dev_upload(&t->d_src, sev));
   ERNN_TRY(dev_upload(&t->d_rec, rev));
@@ -257,9 +270,11 @@ int validate(const ernn_desc_t* d) {
     return fail(ERNN_E_INVALID, "unknown param_mode");
   if (d->rec_stride < 1) return fail(ERNN_E_INVALID, "rec_stride must be >= 1");
   if (d->ckpt_every < 2) return fail(ERNN_E_INVALID, "ckpt_every must be >= 2");
+  if (d->fwd_bpc < 0 || d->fwd_bpc > 64 || d->adj_bpc < 0 || d->adj_bpc > 64)
+    return fail(ERNN_E_INVALID, "blocks per CU must be in [0, 64]");
   if ((d->fwd_v != 0 || d->fwd_r != 0) && !tile_ok(d->fwd_v, d->fwd_r))
     return fail(ERNN_E_INVALID, "forward tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
-  if ((d->adj_v != 0 || d->adj_r != 0) && !tile_ok(d->adj_v, d->adj_r))
+  if ((d->adj_v != 0 || d->adj_r != 0) && !adj_tile_ok(d->param_mode, d->adj_v, d->adj_r))
     return fail(ERNN_E_INVALID, "adjoint tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
@@ -316,6 +331,7 @@ int validate(const ernn_desc_t* d) {
 
 int build_plan(const ernn_desc_t* d, Plan* p) {
   ERNN_HIP(hipGetDevice(&p->device));
+  ERNN_HIP(hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device));
   p->mode = d->param_mode;
   p->n_inst = d->n_inst;
   p->n_sets = d->n_sets;
@@ -426,8 +442,8 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   ERNN_TRY(dev_upload(&p->d_set_begin, p->set_begin));
   ERNN_TRY(dev_upload(&p->d_set_ext_off, p->set_ext_off));
   ERNN_TRY(dev_upload(&p->d_set_grad_off, p->set_grad_off));
-  ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 1, d->fwd_r ? d->fwd_r : 4));
-  ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 1, d->adj_r ? d->adj_r : 4));
+  ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 1, d->fwd_r ? d->fwd_r : 4, d->fwd_bpc));
+  ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 1, d->adj_r ? d->adj_r : 4, d->adj_bpc));
   ERNN_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
   return ERNN_OK;
 }
@@ -627,6 +643,11 @@ int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
   info->fwd_tiles = p->fwd.n_tiles;
   info->adj_tiles = p->adj.n_tiles;
   info->use_graph = p->use_graph ? 1 : 0;
+  info->fwd_bpc = p->fwd.bpc;
+  info->adj_bpc = p->adj.bpc;
+  info->fwd_blocks = p->fwd.blocks;
+  info->adj_blocks = p->adj.blocks;
+  info->n_cu = p->n_cu;
   return ERNN_OK;
 }
 
@@ -661,13 +682,13 @@ int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream) {
   return run_pass(p, call, static_cast<hipStream_t>(stream), true);
 }
 
-int ernn_plan_set_tiling(void* plan, int which, int v, int r) {
+int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc) {
   if (!plan || (which != 0 && which != 1)) return fail(ERNN_E_INVALID, "bad plan or which");
   Plan* p = static_cast<Plan*>(plan);
   ERNN_HIP(hipSetDevice(p->device));
   ERNN_HIP(hipDeviceSynchronize());
   drop_graphs(p);
-  return build_tiling(p, which == 0 ? &p->fwd : &p->adj, v, r);
+  return build_tiling(p, which == 0 ? &p->fwd : &p->adj, v, r, bpc);
 }
 
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,

@@ -39,7 +39,8 @@ class Desc(C.Structure):
     _fields_ = [("n_inst", C.c_int32), ("n_sets", C.c_int32), ("param_mode", C.c_int32),
                 ("rec_stride", C.c_int32), ("snapshot", C.c_int32), ("ckpt_every", C.c_int32),
                 ("fwd_v", C.c_int32), ("fwd_r", C.c_int32), ("adj_v", C.c_int32),
-                ("adj_r", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32),
+                ("adj_r", C.c_int32), ("use_graph", C.c_int32), ("fwd_bpc", C.c_int32),
+                ("adj_bpc", C.c_int32), ("reserved", C.c_int32),
                 ("inst", C.POINTER(Instance)), ("src", C.POINTER(Source)), ("n_src", C.c_int64),
                 ("rec", C.POINTER(Receiver)), ("n_rec", C.c_int64)]
 
@@ -50,7 +51,9 @@ class PlanInfo(C.Structure):
                 ("n_frames_max", C.c_int32), ("ckpt_every", C.c_int32), ("n_ckpt", C.c_int32),
                 ("fwd_v", C.c_int32), ("fwd_r", C.c_int32), ("adj_v", C.c_int32),
                 ("adj_r", C.c_int32), ("fwd_tiles", C.c_int32), ("adj_tiles", C.c_int32),
-                ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+                ("use_graph", C.c_int32), ("fwd_bpc", C.c_int32), ("adj_bpc", C.c_int32),
+                ("fwd_blocks", C.c_int32), ("adj_blocks", C.c_int32), ("n_cu", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Call(C.Structure):
@@ -69,7 +72,7 @@ EXPORTS = {
     "ernn_workspace_init": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "ernn_forward": ([C.c_void_p, C.POINTER(Call), C.c_void_p], C.c_int),
     "ernn_adjoint": ([C.c_void_p, C.POINTER(Call), C.c_void_p], C.c_int),
-    "ernn_plan_set_tiling": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
+    "ernn_plan_set_tiling": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "ernn_time_steps": ([C.c_void_p, C.c_int, C.c_int, C.POINTER(Call), C.c_void_p,
                          C.POINTER(C.c_float)], C.c_int),
 }

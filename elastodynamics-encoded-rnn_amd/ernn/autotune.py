@@ -53,32 +53,44 @@ def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
 
 
 # every compiled tile shape: V columns per lane x R rows, V * R <= 16
-DEFAULT_CANDIDATES: Tuple[Tuple[int, int], ...] = (
+TILE_SHAPES: Tuple[Tuple[int, int], ...] = (
     (1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (2, 1), (2, 2), (2, 4), (2, 8), (4, 1), (4, 2), (4, 4))
+# grid caps in blocks per CU (0: one tile per wavefront; otherwise persistent wavefronts)
+GRID_CAPS: Tuple[int, ...] = (0, 2, 4, 8)
+DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
+    (v, r, b) for (v, r) in TILE_SHAPES for b in GRID_CAPS)
+# six-field adjoint shapes that are not built (they would spill to scratch)
+FIELD6_ADJ_EXCLUDED = {(1, 16), (2, 8)}
 
 
 def tune_tiles(engine, params, field, wavelets, steps: int = 50,
-               candidates: Sequence[Tuple[int, int]] = DEFAULT_CANDIDATES,
-               which: Iterable[int] = (0, 1), repeats: int = 3) -> dict:
+               candidates: Sequence[Tuple[int, int, int]] = DEFAULT_CANDIDATES,
+               which: Iterable[int] = (0, 1), rounds: int = 5) -> dict:
     """Time candidates for the forward (0) and adjoint (1) kernels; keep the best.
 
-    Returns {which: (V, R, ms_per_step, {candidate: ms})}.
+    Candidates are timed in `rounds` interleaved passes (so clock and cache
+    drift hit every candidate alike) and ranked by their median.
+    A candidate is (V, R, blocks per CU).
+    Returns {which: (V, R, bpc, ms_per_step, {candidate: ms})}.
     """
     ws = engine.acquire_workspace()
     traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)
     for w in which:   # bring clocks and caches to steady state before the first candidate
-        engine.time_steps(w, steps, params, field, wavelets, ws, traces, traces)
+        engine.time_steps(w, 4 * steps, params, field, wavelets, ws, traces, traces)
     result = {}
     for w in which:
-        times = {}
-        for (v, r) in candidates:
-            engine.set_tiling(w, v, r)
-            engine.time_steps(w, 2, params, field, wavelets, ws, traces, traces)   # warm-up
-            ms = min(engine.time_steps(w, steps, params, field, wavelets, ws, traces, traces)
-                     for _ in range(repeats))
-            times[(v, r)] = ms / steps
+        cands = [c for c in candidates
+                 if not (w == 1 and engine.survey.mode == "field6" and tuple(c[:2]) in FIELD6_ADJ_EXCLUDED)]
+        samples = {c: [] for c in cands}
+        for _ in range(rounds):
+            for c in cands:
+                engine.set_tiling(w, *c)
+                engine.time_steps(w, 2, params, field, wavelets, ws, traces, traces)   # warm-up
+                samples[c].append(engine.time_steps(w, steps, params, field, wavelets, ws,
+                                                    traces, traces) / steps)
+        times = {c: sorted(t)[len(t) // 2] for c, t in samples.items()}
         best = min(times, key=times.get)
         engine.set_tiling(w, *best)
-        result[w] = (best[0], best[1], times[best], times)
+        result[w] = (best[0], best[1], best[2], times[best], times)
     engine.release_workspace(ws)
     return result

@@ -9,6 +9,7 @@ same libamdhip64 that PyTorch-ROCm uses.
 from __future__ import annotations
 
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -45,8 +46,24 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (set HIPCC or install ROCm)")
 
 
+def _scratch_kernels(remarks: str):
+    """Kernels whose resource-usage remark reports private (scratch) memory."""
+    bad, name = [], None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and int(m.group(1)) > 0 and name:
+            bad.append((name, int(m.group(1))))
+    return bad
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every source for gfx950 and link LIB_PATH; returns its path."""
+    """Compile every source for gfx950 and link LIB_PATH; returns its path.
+
+    Fails if any kernel spills to scratch memory: every tile shape the
+    library launches must keep its working set in registers."""
     cc = hipcc()
     objs = []
     build_dir = os.path.join(ROOT, "_build")
@@ -54,10 +71,17 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         cmd = [cc, *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-x", "hip",
+               "-Rpass-analysis=kernel-resource-usage",
                "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            raise subprocess.CalledProcessError(r.returncode, cmd)
+        bad = _scratch_kernels(r.stderr)
+        if bad:
+            raise RuntimeError("kernels spill to scratch memory: " + ", ".join(f"{n} ({b} B/lane)" for n, b in bad))
         objs.append(obj)
     tmp = LIB_PATH + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]

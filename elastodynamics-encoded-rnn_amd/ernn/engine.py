@@ -68,7 +68,8 @@ class Engine:
         d = _lib.Desc()
         d.n_inst, d.n_sets, d.param_mode = n, sv.n_sets, self.mode
         d.rec_stride, d.snapshot, d.ckpt_every = sv.rec_stride, int(sv.snapshot), int(K)
-        d.fwd_v, d.fwd_r, d.adj_v, d.adj_r = sv.tiles
+        t = tuple(sv.tiles) + (0,) * (6 - len(sv.tiles))
+        d.fwd_v, d.fwd_r, d.adj_v, d.adj_r, d.fwd_bpc, d.adj_bpc = t[:6]
         d.use_graph = int(sv.use_graph)
         d.inst = inst
         d.src, d.n_src = src, len(src_list)
@@ -196,9 +197,14 @@ class Engine:
         return traces.view(len(self.survey.shots), *shapes.pop())
 
     # ----------------------------------------------------------- tuning
-    def set_tiling(self, which: int, v: int, r: int) -> None:
+    @property
+    def tiles(self) -> Tuple[int, int, int, int, int, int]:
+        i = self.info
+        return (int(i.fwd_v), int(i.fwd_r), int(i.adj_v), int(i.adj_r), int(i.fwd_bpc), int(i.adj_bpc))
+
+    def set_tiling(self, which: int, v: int, r: int, bpc: int = 0) -> None:
         with torch.cuda.device(self.device):
-            _lib.check(_lib.lib().ernn_plan_set_tiling(self._plan, which, v, r), "ernn_plan_set_tiling")
+            _lib.check(_lib.lib().ernn_plan_set_tiling(self._plan, which, v, r, bpc), "ernn_plan_set_tiling")
             _lib.check(_lib.lib().ernn_plan_info(self._plan, C.byref(self.info)), "ernn_plan_info")
 
     def time_steps(self, which: int, steps: int, params, field, wavelets, ws, traces, grad_traces) -> float:

@@ -169,7 +169,9 @@ class Survey:
     rec_stride: int = 1       # frame k records u^{kP+1} (PI:294-301)
     snapshot: bool = False    # record whole fields (the reference's output)
     ckpt_every: int = 0       # 0 = choose by memory (autotune.choose_ckpt)
-    tiles: Tuple[int, int, int, int] = (0, 0, 0, 0)   # fwd V, R, adj V, R (0 = default)
+    # forward V, R, adjoint V, R (0 = default), then grid caps in blocks per CU
+    # for the forward and adjoint (0 = one tile per wavefront)
+    tiles: Tuple[int, ...] = (0, 0, 0, 0, 0, 0)
     use_graph: bool = False
 
     def __post_init__(self):

@@ -86,6 +86,9 @@ typedef struct {
                            rows, V * R <= 16; 0, 0 = default (1, 4) */
   int32_t adj_v, adj_r; /* adjoint tile */
   int32_t use_graph;    /* 1: replay each pass as a captured hipGraph */
+  int32_t fwd_bpc;      /* forward grid cap in blocks per CU (0 = one tile per wavefront);
+                           below the tile count the wavefronts loop over tiles */
+  int32_t adj_bpc;      /* adjoint grid cap */
   int32_t reserved;
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
@@ -105,6 +108,9 @@ typedef struct {
   int32_t fwd_v, fwd_r, adj_v, adj_r;
   int32_t fwd_tiles, adj_tiles;
   int32_t use_graph;
+  int32_t fwd_bpc, adj_bpc;
+  int32_t fwd_blocks, adj_blocks;
+  int32_t n_cu;             /* compute units of the plan's device */
   int32_t reserved;
 } ernn_plan_info_t;
 
@@ -165,9 +171,10 @@ int ernn_forward(void* plan, const ernn_call_t* call, void* stream);
  * reverse-time adjoint recurrence; FP64 accumulation. */
 int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream);
 
-/* Change the tile of the forward (which = 0) or adjoint (which = 1) kernel.
- * Used by the autotuner; drops captured graphs. */
-int ernn_plan_set_tiling(void* plan, int which, int v, int r);
+/* Change the tile (V, R) and grid cap (blocks per CU, 0 = none) of the
+ * forward (which = 0) or adjoint (which = 1) kernel.  Used by the autotuner;
+ * synchronizes the device and drops captured graphs. */
+int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc);
 
 /* Time `steps` forward (which = 0) or adjoint (which = 1) steps with the
  * current tiling using hipEvents on `stream`; synchronizes.  The workspace

@@ -32,7 +32,8 @@ def _sources(nx, ny, h):
 
 
 GRIDS = [(37, 70), (20, 130), (64, 64), (5, 3)]
-TILES = [(1, 1), (1, 2), (2, 8), (4, 4), (1, 16)]
+# (V, R, blocks per CU): one tile per wavefront, and persistent wavefronts looping over tiles
+TILES = [(1, 1, 0), (1, 2, 1), (2, 8, 0), (4, 4, 2), (1, 16, 0), (2, 1, 1)]
 
 
 @pytest.mark.parametrize("nx,ny", GRIDS)
@@ -43,16 +44,16 @@ def test_const6_forward_matches_oracle_and_is_tiling_invariant(nx, ny, stride, s
     shot = make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=rec)
     ref, _ = oracle_run(shot, F32(C_ANISO), stride, snapshot)
     first = None
-    for (v, r) in TILES:
+    for (v, r, b) in TILES:
         sv = ernn.Survey([shot], mode="const6", rec_stride=stride, snapshot=snapshot, ckpt_every=4,
-                         tiles=(v, r, v, r))
+                         tiles=(v, r, v, r, b, b))
         eng, tr, _ = run_hip(sv, params=C_ANISO)
         got = tr.reshape(ref.shape)
-        assert rel_l2(got, ref) < 2e-6, (v, r)
+        assert rel_l2(got, ref) < 2e-6, (v, r, b)
         if first is None:
             first = tr
         else:
-            assert np.array_equal(tr, first), f"tiling {(v, r)} changed the bits"
+            assert np.array_equal(tr, first), f"tiling {(v, r, b)} changed the bits"
         eng.close()
 
 
@@ -63,7 +64,8 @@ def test_const6_gradient_matches_oracle_and_is_checkpoint_invariant():
     ref, gcell = oracle_run(shot, F32(C_ANISO), 1, False, G)
     gref = orc.const_gradient(gcell)
     first = None
-    for K, tiles in [(2, (2, 8, 2, 8)), (3, (1, 2, 4, 4)), (7, (4, 4, 1, 16)), (41, (2, 8, 2, 8)), (64, (2, 8, 2, 8))]:
+    for K, tiles in [(2, (2, 8, 2, 8)), (3, (1, 2, 4, 4, 1, 1)), (7, (4, 4, 1, 16)), (41, (2, 8, 2, 8, 1, 1)),
+                     (64, (2, 8, 2, 8))]:
         sv = ernn.Survey([shot], mode="const6", ckpt_every=K, tiles=tiles)
         eng, tr, (gp, gf) = run_hip(sv, params=C_ANISO, G=G)
         assert gf is None
@@ -72,7 +74,7 @@ def test_const6_gradient_matches_oracle_and_is_checkpoint_invariant():
         assert err < 1e-4, (K, err)
         if first is None:
             first = (tiles, gp)
-        elif tiles[2:] == first[0][2:]:
+        elif tiles[2:4] == first[0][2:4]:
             assert np.array_equal(gp, first[1]), f"K={K} changed the gradient bits"
         eng.close()
 
@@ -96,7 +98,7 @@ def test_dfield_forward_and_gradients_match_oracle():
     D = rng.uniform(0.4, 1.0, (nx, ny)).astype(np.float32)
     shots = [make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=_receivers(nx, ny)),
              make_shot(nx, ny, nt, h, sources=ernn.point_source(3, ny - 1, h), receivers=_receivers(nx, ny))]
-    sv = ernn.Survey(shots, mode="dfield", ckpt_every=5, tiles=(2, 4, 1, 8))
+    sv = ernn.Survey(shots, mode="dfield", ckpt_every=5, tiles=(2, 4, 1, 8, 1, 1))
     Gs = [rng.standard_normal((2, nt, len(s.receivers))) for s in shots]
     C32 = F32(C_ANISO)
     cells = [np.float32(D) * np.float32(c) for c in C_ANISO]     # fl32(D * C0) like the kernel
@@ -120,7 +122,7 @@ def test_field6_forward_and_gradients_match_oracle():
     shot = make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=_receivers(nx, ny))
     G = rng.standard_normal((2, nt, len(shot.receivers)))
     ref, gcell = oracle_run(shot, [f.astype(np.float64) for f in fields], 1, False, G)
-    for tiles in [(1, 2, 1, 2), (2, 8, 2, 8), (4, 4, 4, 4), (1, 1, 2, 1)]:
+    for tiles in [(1, 2, 1, 2), (2, 8, 2, 4), (4, 4, 4, 4, 1, 1), (1, 1, 2, 1, 2, 2), (2, 8, 1, 8)]:
         sv = ernn.Survey([shot], mode="field6", ckpt_every=6, tiles=tiles)
         eng, tr, (gp, gf) = run_hip(sv, field=fields, G=G)
         assert gp is None
@@ -141,7 +143,7 @@ def test_uneven_batch_equals_single_instance_runs():
              for nx, ny, nt, s in spec]
     Cs = np.stack([C_ANISO * rng.uniform(0.8, 1.2, 6) for _ in range(3)]).astype(np.float32)
     Gs = [rng.standard_normal((2, s.nt, len(s.receivers))) for s in shots]
-    sv = ernn.Survey(shots, mode="const6", n_sets=3, ckpt_every=6)
+    sv = ernn.Survey(shots, mode="const6", n_sets=3, ckpt_every=6, tiles=(1, 4, 2, 2, 1, 1))
     eng, tr, (gp, _) = run_hip(sv, params=Cs, G=np.concatenate([g.ravel() for g in Gs]))
     gp = gp.reshape(3, 6)
     gsum = np.zeros((3, 6))
@@ -211,3 +213,28 @@ def test_linearity_in_source_amplitude_is_exact():
     assert d.max() <= 1e-6 * np.abs(t1).max()
     big = np.abs(t1) > 1e-6 * np.abs(t1).max()
     assert np.array_equal(t2[big], 2 * t1[big])
+
+
+@pytest.mark.parametrize("mode", ["const6", "dfield"])
+def test_persistent_wavefronts_are_bitwise_equal(mode):
+    """A 512^2 grid has 4096 one-row tiles; capping the grid at 1 or 2 blocks
+    per CU makes every wavefront loop over 2-4 tiles of its XCD band.  Traces
+    and gradients must equal the one-tile-per-wavefront launch bit for bit."""
+    n, nt, h = 512, 30, 0.05e-3
+    rng = np.random.default_rng(6)
+    rec = ernn.row_receivers(n, n - 1, 128)
+    shots = [make_shot(n, n, nt, h, C=C_ANISO, receivers=rec),
+             make_shot(n, n, nt, h, C=C_ANISO, sources=ernn.point_source(40, n - 1, h), receivers=rec)]
+    D = rng.uniform(0.5, 1.0, (n, n)).astype(np.float32) if mode == "dfield" else None
+    G = rng.standard_normal((len(shots), 2, nt, len(rec)))
+    out = {}
+    for bpc in (0, 1, 2):
+        sv = ernn.Survey(shots, mode=mode, ckpt_every=7, tiles=(1, 1, 1, 1, bpc, bpc))
+        eng, tr, grads = run_hip(sv, params=C_ANISO, field=D, G=G)
+        assert int(eng.info.fwd_blocks) <= (bpc * eng.info.n_cu if bpc else 10 ** 9)
+        out[bpc] = (tr, grads)
+        eng.close()
+    for bpc in (1, 2):
+        assert np.array_equal(out[bpc][0], out[0][0]), bpc
+        for a, b in zip(out[bpc][1], out[0][1]):
+            assert (a is None and b is None) or np.array_equal(a, b), bpc
