@@ -255,14 +255,16 @@ def main():
     params_true = torch.tensor(C_true.reshape(-1), dtype=torch.float32, device=dev)
     field_true = None if sv.mode == "const6" else torch.tensor(wl.D_true.reshape(-1), device=dev)
     tune = None
+    cells_k = max(1, sum(s.nx * s.ny for s in sv.shots) // 1000)   # thousands of cells per step
     if args.tiles:
         vf, rf, va, ra, bf, ba = (int(x) for x in args.tiles.split(","))
         eng.set_tiling(0, vf, rf, bf)
         eng.set_tiling(1, va, ra, ba)
         log(f"tiles fixed: fwd V={vf} R={rf} B={bf}, adj V={va} R={ra} B={ba}")
     elif args.autotune:
-        tune = tune_tiles(eng, params_true, field_true, model.wavelets, steps=20)
-        log("autotune: " + ", ".join(f"{'fwd' if w == 0 else 'adj'} V={v} R={r} B={b} {ms * 1e3:.1f} us/step"
+        tune = tune_tiles(eng, params_true, field_true, model.wavelets, steps=max(20, min(100, 40000 // cells_k)),
+                          which=(0, 2) if sv.fused else (0, 1))
+        log("autotune: " + ", ".join(f"{('fwd', 'adj', 'fused')[w]} V={v} R={r} B={b} {ms * 1e3:.1f} us/step"
                                      for w, (v, r, b, ms, _) in tune.items()))
     with torch.no_grad():
         obs = ernn.ernn_traces(eng, params=params_true, field=field_true, wavelets=model.wavelets)
@@ -303,10 +305,16 @@ def main():
     eng.time_steps(0, 10, p, f, model.wavelets, ws, tr0, tr0)
     fwd_ms = eng.time_steps(0, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
     adj_ms = eng.time_steps(1, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
+    fused_ms = eng.time_steps(2, nk, p, f, model.wavelets, ws, tr0, tr0) / nk if sv.fused else None
     eng.release_workspace(ws)
     cells = sum(s.nx * s.ny for s in sv.shots)
-    kern = {"fwd_step_kernel": (fwd_ms, 2 * sv.nt_max, BYTES_PER_CELL[(sv.mode, 0)]),
-            "adj_step_kernel": (adj_ms, sv.nt_max, BYTES_PER_CELL[(sv.mode, 1)])}
+    bf, ba = BYTES_PER_CELL[(sv.mode, 0)], BYTES_PER_CELL[(sv.mode, 1)]
+    if sv.fused:   # forward pass: nt fwd launches; backward: ~nt fused launches (+ one segment alone)
+        kern = {"fwd_step_kernel": (fwd_ms, sv.nt_max + int(eng.info.ckpt_every), bf),
+                "fused_step_kernel": (fused_ms, sv.nt_max, bf + ba)}
+    else:
+        kern = {"fwd_step_kernel": (fwd_ms, 2 * sv.nt_max, bf),
+                "adj_step_kernel": (adj_ms, sv.nt_max, ba)}
     name = max(kern, key=lambda k: kern[k][0] * kern[k][1])
     ms, launches, bpc = kern[name]
     achieved = cells * bpc / (ms * 1e-3) / 1e9
@@ -319,8 +327,10 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_cell": bpc, "cells_per_launch": cells, "avg_launch_us": round(ms * 1e3, 3),
                 "launches_per_iter": launches,
-                "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3)}
-    log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3:.2f} us per launch; "
+                "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),
+                "fused_step_us": round(fused_ms * 1e3, 3) if fused_ms else None}
+    log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3:.2f} us, "
+        f"fused {fused_ms * 1e3 if fused_ms else float('nan'):.2f} us per launch; "
         f"{name}: {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.2%} of HBM peak)")
 
     baseline = None
@@ -355,6 +365,7 @@ def main():
         "config": {"workload": wl.name, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
                    "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
                    "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every), "graph": bool(sv.use_graph),
+                   "fused_backward": bool(sv.fused),
                    "tiles": list(eng.tiles),
                    "parallelism": "none"},
         "s_per_adam_iter": round(dt_iter, 5),

@@ -91,4 +91,11 @@ struct StepArgs {
   int pad;
 };
 
+// A launch that runs one adjoint step and, on the same tiles, one recompute
+// step of the previous checkpoint segment (they touch disjoint slots).
+struct FusedArgs {
+  StepArgs adj;
+  StepArgs fwd;
+};
+
 }  // namespace ernn

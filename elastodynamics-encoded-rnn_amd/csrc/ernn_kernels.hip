@@ -10,10 +10,9 @@
 //   gradient path       replaces autograd at PI:447 / DI:511
 //
 // Work decomposition: one wavefront owns a tile of R rows x 64*V columns and
-// sweeps it row by row, keeping rows i-1, i, i+1 of each stencilled field in
-// registers.  Lane l holds V consecutive columns; the column neighbours come
-// from the adjacent lanes by cross-lane shuffles, and the two edge lanes fetch
-// the tile's halo columns with one masked load.  Ghost cells are real zeros
+// computes it from registers.  Lane l holds V consecutive columns; the column
+// neighbours come from the adjacent lanes by cross-lane shuffles, and the two
+// edge lanes fetch the tile's halo columns with one masked load.  Ghost cells are real zeros
 // in the padded layout (ernn_internal.h), so the sweep has no boundary
 // branches.  No LDS and no block barriers: the four wavefronts of a block
 // work on independent tiles.  Block b runs on XCD b % 8; the tile order maps
@@ -62,7 +61,10 @@ __device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&c
   }
 }
 
-// rowp points at column j0 of one padded row.
+// rowp points at column j0 of one padded row.  The two edge lanes load the
+// tile's halo columns; every other column neighbour comes from the adjacent
+// lane by a cross-lane shuffle.  (DPP whole-wavefront shifts were measured
+// slower than ds_bpermute here: 147 vs 122 us on the 1024^2 damage adjoint.)
 template <int V>
 __device__ __forceinline__ Row<V> load_row(const float* __restrict__ rowp, int lane) {
   Row<V> r;
@@ -352,22 +354,32 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const CallDev& C, co
   }
 }
 
-template <int MODE, int V, int R>
+// LOOP = false: the grid covers every tile (one tile per wavefront, straight-
+// line code); LOOP = true: a capped grid whose wavefronts loop over their
+// XCD band, keeping the instance record in scalar registers between tiles.
+template <int MODE, int V, int R, bool LOOP>
 __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const TileRange tr = tile_range(a.n_tiles);
   if (tr.t >= tr.end) return;
   const CallDev C = *a.call;
-  int cur = -1;
-  InstDev I;
-  for (int t = tr.t; t < tr.end; t += tr.step) {
-    const TileDev tl = a.tiles[t];
-    if (tl.inst != cur) {
-      I = a.inst[tl.inst];
-      cur = tl.inst;
-    }
-    if (a.step >= I.nt) continue;   // this instance has finished its time loop
+  if constexpr (!LOOP) {
+    const TileDev tl = a.tiles[tr.t];
+    const InstDev I = a.inst[tl.inst];
+    if (a.step >= I.nt) return;   // this instance has finished its time loop
     fwd_tile<MODE, V, R>(a, C, tl, I, lane);
+  } else {
+    int cur = -1;
+    InstDev I;
+    for (int t = tr.t; t < tr.end; t += tr.step) {
+      const TileDev tl = a.tiles[t];
+      if (tl.inst != cur) {
+        I = a.inst[tl.inst];
+        cur = tl.inst;
+      }
+      if (a.step >= I.nt) continue;
+      fwd_tile<MODE, V, R>(a, C, tl, I, lane);
+    }
   }
 }
 
@@ -619,22 +631,60 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, co
   }
 }
 
-template <int MODE, int V, int R>
+template <int MODE, int V, int R, bool LOOP>
 __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const TileRange tr = tile_range(a.n_tiles);
   if (tr.t >= tr.end) return;
   const CallDev C = *a.call;
+  if constexpr (!LOOP) {
+    const TileDev tl = a.tiles[tr.t];
+    const InstDev I = a.inst[tl.inst];
+    if (a.step > I.nt) return;   // this instance has not started its adjoint yet
+    adj_tile<MODE, V, R>(a, C, tl, I, tr.t, lane);
+  } else {
+    int cur = -1;
+    InstDev I;
+    for (int t = tr.t; t < tr.end; t += tr.step) {
+      const TileDev tl = a.tiles[t];
+      if (tl.inst != cur) {
+        I = a.inst[tl.inst];
+        cur = tl.inst;
+      }
+      if (a.step > I.nt) continue;
+      adj_tile<MODE, V, R>(a, C, tl, I, t, lane);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Fused step: adjoint step m of checkpoint segment s and recompute step n of
+// segment s - 1 in one launch.  The two read and write disjoint slots (the
+// recompute fills the other bank of segment slots), so they are independent;
+// running them together halves the launches of the backward pass and lets the
+// bandwidth-bound recompute overlap the issue-bound adjoint on the same CUs.
+// Fused index u covers 2 x tiles: tile u / 2, adjoint if u is even.
+// --------------------------------------------------------------------------
+template <int MODE, int V, int R>
+__global__ void __launch_bounds__(kBlock) fused_step_kernel(FusedArgs fa) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const TileRange tr = tile_range(2 * fa.adj.n_tiles);
+  if (tr.t >= tr.end) return;
+  const CallDev C = *fa.adj.call;
   int cur = -1;
   InstDev I;
-  for (int t = tr.t; t < tr.end; t += tr.step) {
-    const TileDev tl = a.tiles[t];
+  for (int u = tr.t; u < tr.end; u += tr.step) {
+    const int t = u >> 1;
+    const TileDev tl = fa.adj.tiles[t];
     if (tl.inst != cur) {
-      I = a.inst[tl.inst];
+      I = fa.adj.inst[tl.inst];
       cur = tl.inst;
     }
-    if (a.step > I.nt) continue;   // this instance has not started its adjoint yet
-    adj_tile<MODE, V, R>(a, C, tl, I, t, lane);
+    if (u & 1) {
+      if (fa.fwd.step < I.nt) fwd_tile<MODE, V, R>(fa.fwd, C, tl, I, lane);
+    } else {
+      if (fa.adj.step <= I.nt) adj_tile<MODE, V, R>(fa.adj, C, tl, I, t, lane);
+    }
   }
 }
 
@@ -718,35 +768,51 @@ __global__ void finalize_field_kernel(const CallDev* call, const InstDev* inst, 
 // Launchers (host side, called from ernn_plan.cpp)
 // --------------------------------------------------------------------------
 // Tile shapes with a compiled kernel: V columns per lane x R rows, V * R <= 16.
-// The six-field adjoint at (1, 16) and (2, 8) would spill to scratch memory;
-// those shapes are not built (the build rejects any kernel that spills to
-// scratch) and the host refuses them (tile_ok in ernn_plan.cpp).
+// Every built kernel keeps its working set in architectural VGPRs: shapes
+// whose adjoint would spill (to AGPRs or scratch) are not built, the build
+// rejects any kernel that spills, and the host refuses the missing shapes
+// (adj_tile_ok in ernn_plan.cpp).
 #define ERNN_TILE_CASES(X) \
   X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(1, 16) X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(4, 4)
-#define ERNN_TILE_CASES_FIELD6_ADJ(X) \
-  X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 4) X(4, 1) X(4, 2) X(4, 4)
+#define ERNN_TILE_CASES_DFIELD_ADJ(X) \
+  X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 4) X(4, 1) X(4, 2)
+#define ERNN_TILE_CASES_FIELD6_ADJ(X) X(1, 1) X(1, 2) X(1, 4) X(2, 1) X(2, 2) X(4, 1)
+#define ERNN_TILE_CASES_FIELD6_FWD(X) \
+  X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 4) X(4, 1) X(4, 2)
 
-template <int MODE>
-static hipError_t launch_fwd_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
 #define ERNN_FWD_CASE(v, r)                                                                       \
   if (V == v && a.rows == r) {                                                                    \
-    hipLaunchKernelGGL((fwd_step_kernel<MODE, v, r>), dim3(blocks), dim3(kBlock), 0, st, a);     \
+    if (loop)                                                                                     \
+      hipLaunchKernelGGL((fwd_step_kernel<MODE, v, r, true>), dim3(blocks), dim3(kBlock), 0, st, a); \
+    else                                                                                          \
+      hipLaunchKernelGGL((fwd_step_kernel<MODE, v, r, false>), dim3(blocks), dim3(kBlock), 0, st, a); \
     return hipGetLastError();                                                                     \
   }
-  ERNN_TILE_CASES(ERNN_FWD_CASE)
-#undef ERNN_FWD_CASE
+template <int MODE>
+static hipError_t launch_fwd_m(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st) {
+  if constexpr (MODE == kField6) {
+    ERNN_TILE_CASES_FIELD6_FWD(ERNN_FWD_CASE)
+  } else {
+    ERNN_TILE_CASES(ERNN_FWD_CASE)
+  }
   return hipErrorInvalidValue;
 }
+#undef ERNN_FWD_CASE
 
 #define ERNN_ADJ_CASE(v, r)                                                                       \
   if (V == v && a.rows == r) {                                                                    \
-    hipLaunchKernelGGL((adj_step_kernel<MODE, v, r>), dim3(blocks), dim3(kBlock), 0, st, a);     \
+    if (loop)                                                                                     \
+      hipLaunchKernelGGL((adj_step_kernel<MODE, v, r, true>), dim3(blocks), dim3(kBlock), 0, st, a); \
+    else                                                                                          \
+      hipLaunchKernelGGL((adj_step_kernel<MODE, v, r, false>), dim3(blocks), dim3(kBlock), 0, st, a); \
     return hipGetLastError();                                                                     \
   }
 template <int MODE>
-static hipError_t launch_adj_m(const StepArgs& a, int V, int blocks, hipStream_t st) {
+static hipError_t launch_adj_m(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st) {
   if constexpr (MODE == kField6) {
     ERNN_TILE_CASES_FIELD6_ADJ(ERNN_ADJ_CASE)
+  } else if constexpr (MODE == kDField) {
+    ERNN_TILE_CASES_DFIELD_ADJ(ERNN_ADJ_CASE)
   } else {
     ERNN_TILE_CASES(ERNN_ADJ_CASE)
   }
@@ -754,20 +820,48 @@ static hipError_t launch_adj_m(const StepArgs& a, int V, int blocks, hipStream_t
 }
 #undef ERNN_ADJ_CASE
 
-hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, hipStream_t st) {
-  switch (a.mode) {
-    case kConst6: return launch_fwd_m<kConst6>(a, V, blocks, st);
-    case kDField: return launch_fwd_m<kDField>(a, V, blocks, st);
-    case kField6: return launch_fwd_m<kField6>(a, V, blocks, st);
+#define ERNN_FUSED_CASE(v, r)                                                                     \
+  if (V == v && fa.adj.rows == r) {                                                               \
+    hipLaunchKernelGGL((fused_step_kernel<MODE, v, r>), dim3(blocks), dim3(kBlock), 0, st, fa);   \
+    return hipGetLastError();                                                                     \
+  }
+template <int MODE>
+static hipError_t launch_fused_m(const FusedArgs& fa, int V, int blocks, hipStream_t st) {
+  if constexpr (MODE == kField6) {
+    ERNN_TILE_CASES_FIELD6_ADJ(ERNN_FUSED_CASE)
+  } else if constexpr (MODE == kDField) {
+    ERNN_TILE_CASES_DFIELD_ADJ(ERNN_FUSED_CASE)
+  } else {
+    ERNN_TILE_CASES(ERNN_FUSED_CASE)
+  }
+  return hipErrorInvalidValue;
+}
+#undef ERNN_FUSED_CASE
+
+hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t st) {
+  switch (fa.adj.mode) {
+    case kConst6: return launch_fused_m<kConst6>(fa, V, blocks, st);
+    case kDField: return launch_fused_m<kDField>(fa, V, blocks, st);
+    case kField6: return launch_fused_m<kField6>(fa, V, blocks, st);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, hipStream_t st) {
+// loop: the grid is capped below the tile count (persistent wavefronts)
+hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st) {
   switch (a.mode) {
-    case kConst6: return launch_adj_m<kConst6>(a, V, blocks, st);
-    case kDField: return launch_adj_m<kDField>(a, V, blocks, st);
-    case kField6: return launch_adj_m<kField6>(a, V, blocks, st);
+    case kConst6: return launch_fwd_m<kConst6>(a, V, blocks, loop, st);
+    case kDField: return launch_fwd_m<kDField>(a, V, blocks, loop, st);
+    case kField6: return launch_fwd_m<kField6>(a, V, blocks, loop, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st) {
+  switch (a.mode) {
+    case kConst6: return launch_adj_m<kConst6>(a, V, blocks, loop, st);
+    case kDField: return launch_adj_m<kDField>(a, V, blocks, loop, st);
+    case kField6: return launch_adj_m<kField6>(a, V, blocks, loop, st);
   }
   return hipErrorInvalidValue;
 }

@@ -27,8 +27,9 @@
 #include "ernn_internal.h"
 
 namespace ernn {
-hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, hipStream_t st);
-hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, hipStream_t st);
+hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st);
+hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st);
+hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t st);
 hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st);
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t st);
 hipError_t launch_fill_ws(const CallDev* call, int64_t off, int64_t n, hipStream_t st);
@@ -73,10 +74,19 @@ bool tile_ok(int V, int R) {
   if (R < 1 || R > kRowAlign || (R & (R - 1)) != 0) return false;
   return V * R <= 16;
 }
-// The six-field adjoint is not built at (1, 16) and (2, 8) (scratch spills).
+// Adjoint shapes that would spill registers are not built (ERNN_TILE_CASES_*
+// in ernn_kernels.hip): the damage-field adjoint needs V * R <= 8, the
+// six-field adjoint V * R <= 4.
 bool adj_tile_ok(int mode, int V, int R) {
   if (!tile_ok(V, R)) return false;
-  return !(mode == ERNN_PARAM_FIELD6 && V * R == 16 && V <= 2);
+  if (mode == ERNN_PARAM_DFIELD) return V * R <= 8;
+  if (mode == ERNN_PARAM_FIELD6) return V * R <= 4;
+  return true;
+}
+// The six-field forward is built for V * R <= 8 (larger shapes would spill).
+bool fwd_tile_ok(int mode, int V, int R) {
+  if (!tile_ok(V, R)) return false;
+  return !(mode == ERNN_PARAM_FIELD6 && V * R > 8);
 }
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
@@ -84,6 +94,8 @@ int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 struct Tiling {
   int V = 0, R = 0, bpc = 0;
   int n_tiles = 0, blocks = 0;
+  bool loop = false;      // grid capped below the tile count: wavefronts loop over tiles
+  int fused_blocks = 0;   // grid of the fused kernel (2 x tiles)
   TileDev* d_tiles = nullptr;
   SrcEv* d_src = nullptr;
   RecEv* d_rec = nullptr;
@@ -95,6 +107,7 @@ struct Plan {
   int mode = 0, n_inst = 0, n_sets = 0, P = 1, snapshot = 0, K = 2, S = 1, nt_max = 0;
   int n_coef = 0, n_acc = 0;
   bool use_graph = false;
+  bool fuse = false;
   std::vector<ernn_instance_t> inst_h;
   std::vector<ernn_source_t> src_h;
   std::vector<ernn_receiver_t> rec_h;
@@ -117,12 +130,14 @@ struct Plan {
   hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;
   hipStream_t cap_stream = nullptr;
 
-  // slots of one instance's state pool
+  // slots of one instance's state pool: ring, checkpoints, segment bank(s), lam ring
+  int banks() const { return fuse ? 2 : 1; }
   int ring(int r) const { return r; }
   int ck(int s, int q) const { return 3 + 2 * s + q; }
-  int seg(int q) const { return 3 + 2 * S + q; }
-  int lam(int r) const { return 3 + 2 * S + K + r; }
-  int n_slots() const { return 3 + 2 * S + K + 3; }
+  int seg(int bank, int q) const { return 3 + 2 * S + bank * K + q; }
+  int lam(int r) const { return 3 + 2 * S + banks() * K + r; }
+  int n_slots() const { return 3 + 2 * S + banks() * K + 3; }
+  int bank_of(int s) const { return fuse ? (s & 1) : 0; }
 
   int slot_fwd(int m) const {
     if (m == -1) return ck(0, 0);
@@ -134,7 +149,7 @@ struct Plan {
   int slot_seg(int s, int m) const {
     if (m == s * K - 1) return ck(s, 0);
     if (m == s * K) return ck(s, 1);
-    return seg(m - s * K - 1);
+    return seg(bank_of(s), m - s * K - 1);
   }
 };
 
@@ -170,8 +185,11 @@ int64_t tiles_for(const Plan& p, int V, int R) {
 int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
   if (!tile_ok(V, R))
     return fail(ERNN_E_INVALID, "tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
+  if (t == &p->fwd && !fwd_tile_ok(p->mode, V, R))
+    return fail(ERNN_E_INVALID, "forward tile (V, R) with V*R > 8 is not built for six fields");
   if (t == &p->adj && !adj_tile_ok(p->mode, V, R))
-    return fail(ERNN_E_INVALID, "adjoint tile (V, R) = (1, 16) and (2, 8) are not built for six fields");
+    return fail(ERNN_E_INVALID, "adjoint tile (V, R) not built for this mode (damage field: V*R <= 8, "
+                                "six fields: V*R <= 4)");
   if (bpc < 0 || bpc > 64) return fail(ERNN_E_INVALID, "blocks per CU must be in [0, 64]");
   const int64_t nt64 = tiles_for(*p, V, R);
   if (nt64 > (int64_t)1 << 30) return fail(ERNN_E_CAPACITY, "too many tiles");
@@ -239,8 +257,14 @@ int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
   // 8 XCDs, each owning ceil(n_tiles / 8) tiles; a wavefront per tile unless capped
   const int64_t chunk = (t->n_tiles + 7) / 8;
   int64_t per_xcd = std::max<int64_t>((chunk + kWavesPerBlock - 1) / kWavesPerBlock, 1);
-  if (bpc > 0) per_xcd = std::min<int64_t>(per_xcd, std::max<int64_t>((int64_t)bpc * p->n_cu / 8, 1));
+  const int64_t cap = std::max<int64_t>((int64_t)bpc * p->n_cu / 8, 1);
+  t->loop = bpc > 0 && cap < per_xcd;
+  if (t->loop) per_xcd = cap;
   t->blocks = (int)(8 * per_xcd);
+  const int64_t fchunk = (2 * (int64_t)t->n_tiles + 7) / 8;
+  int64_t fper = std::max<int64_t>((fchunk + kWavesPerBlock - 1) / kWavesPerBlock, 1);
+  if (bpc > 0) fper = std::min<int64_t>(fper, cap);
+  t->fused_blocks = (int)(8 * fper);
   ERNN_TRY(dev_upload(&t->d_tiles, tiles));
   ERNN_TRY(dev_upload(&t->d_src, sev));
   ERNN_TRY(dev_upload(&t->d_rec, rev));
@@ -272,10 +296,12 @@ int validate(const ernn_desc_t* d) {
   if (d->ckpt_every < 2) return fail(ERNN_E_INVALID, "ckpt_every must be >= 2");
   if (d->fwd_bpc < 0 || d->fwd_bpc > 64 || d->adj_bpc < 0 || d->adj_bpc > 64)
     return fail(ERNN_E_INVALID, "blocks per CU must be in [0, 64]");
-  if ((d->fwd_v != 0 || d->fwd_r != 0) && !tile_ok(d->fwd_v, d->fwd_r))
-    return fail(ERNN_E_INVALID, "forward tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
+  if ((d->fwd_v != 0 || d->fwd_r != 0) && !fwd_tile_ok(d->param_mode, d->fwd_v, d->fwd_r))
+    return fail(ERNN_E_INVALID, "forward tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16 "
+                                "(six fields: <= 8)");
   if ((d->adj_v != 0 || d->adj_r != 0) && !adj_tile_ok(d->param_mode, d->adj_v, d->adj_r))
-    return fail(ERNN_E_INVALID, "adjoint tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
+    return fail(ERNN_E_INVALID, "adjoint tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16 "
+                                "(damage field: <= 8, six fields: <= 4)");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))
@@ -339,6 +365,7 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   p->snapshot = d->snapshot ? 1 : 0;
   p->K = d->ckpt_every;
   p->use_graph = d->use_graph != 0;
+  p->fuse = d->fuse != 0;
   p->n_coef = p->mode == ERNN_PARAM_DFIELD ? 1 : (p->mode == ERNN_PARAM_FIELD6 ? 6 : 0);
   p->n_acc = p->n_coef;
   p->inst_h.assign(d->inst, d->inst + d->n_inst);
@@ -514,7 +541,7 @@ int enqueue_forward(const Plan* p, hipStream_t st) {
     a.s_prev = p->slot_fwd(n - 1);
     a.s_cur = p->slot_fwd(n);
     a.s_next = p->slot_fwd(n + 1);
-    ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, st));
+    ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
   }
   return ERNN_OK;
 }
@@ -533,25 +560,70 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
   StepArgs f = base_args(p, p->fwd);
   f.record = 0;
   StepArgs a = base_args(p, p->adj);
-  for (int s = p->S - 1; s >= 0; --s) {
-    const int top = std::min((s + 1) * p->K, p->nt_max);
-    const int bot = s * p->K + 1;
-    for (int n = s * p->K; n < top; ++n) {
-      f.step = n;
-      f.s_prev = p->slot_seg(s, n - 1);
-      f.s_cur = p->slot_seg(s, n);
-      f.s_next = p->slot_seg(s, n + 1);
-      ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, st));
+  // recompute step n of segment s (u^{n+1} into the segment's bank)
+  auto recompute = [&](StepArgs& r, int s, int n) {
+    r.step = n;
+    r.s_prev = p->slot_seg(s, n - 1);
+    r.s_cur = p->slot_seg(s, n);
+    r.s_next = p->slot_seg(s, n + 1);
+  };
+  auto seg_top = [&](int s) { return std::min((s + 1) * p->K, p->nt_max); };
+  if (!p->fuse) {
+    for (int s = p->S - 1; s >= 0; --s) {
+      const int top = seg_top(s), bot = s * p->K + 1;
+      for (int n = s * p->K; n < top; ++n) {
+        recompute(f, s, n);
+        ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
+      }
+      a.seg_top = top;
+      a.seg_bot = bot;
+      for (int m = top; m >= bot; --m) {
+        a.step = m;
+        a.s_prev = p->lam((m + 2) % 3);
+        a.s_cur = p->lam((m + 1) % 3);
+        a.s_next = p->lam(m % 3);
+        a.s_u = p->seg(0, m - s * p->K - 1);
+        ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, p->adj.loop, st));
+      }
     }
-    a.seg_top = top;
-    a.seg_bot = bot;
-    for (int m = top; m >= bot; --m) {
-      a.step = m;
-      a.s_prev = p->lam((m + 2) % 3);
-      a.s_cur = p->lam((m + 1) % 3);
-      a.s_next = p->lam(m % 3);
-      a.s_u = p->seg(m - s * p->K - 1);
-      ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, st));
+  } else {
+    // The last segment is recomputed alone; afterwards every adjoint step of
+    // segment s shares its launch with one recompute step of segment s - 1,
+    // which fills the other bank (segment s - 1 always has K steps).
+    {
+      const int s = p->S - 1;
+      for (int n = s * p->K; n < seg_top(s); ++n) {
+        recompute(f, s, n);
+        ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
+      }
+    }
+    FusedArgs fa{};
+    fa.fwd = base_args(p, p->adj);   // the fused kernel runs both on the adjoint's tiles
+    fa.fwd.record = 0;
+    for (int s = p->S - 1; s >= 0; --s) {
+      const int top = seg_top(s), bot = s * p->K + 1;
+      a.seg_top = top;
+      a.seg_bot = bot;
+      int idx = 0;
+      for (int m = top; m >= bot; --m, ++idx) {
+        a.step = m;
+        a.s_prev = p->lam((m + 2) % 3);
+        a.s_cur = p->lam((m + 1) % 3);
+        a.s_next = p->lam(m % 3);
+        a.s_u = p->seg(p->bank_of(s), m - s * p->K - 1);
+        if (s >= 1 && idx < p->K) {
+          fa.adj = a;
+          recompute(fa.fwd, s - 1, (s - 1) * p->K + idx);
+          ERNN_HIP(launch_fused_step(fa, p->adj.V, p->adj.fused_blocks, st));
+        } else {
+          ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, p->adj.loop, st));
+        }
+      }
+      // a short last segment leaves recompute steps of segment s - 1 to run alone
+      for (; s >= 1 && idx < p->K; ++idx) {
+        recompute(f, s - 1, (s - 1) * p->K + idx);
+        ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
+      }
     }
   }
   if (p->mode != ERNN_PARAM_FIELD6)
@@ -648,6 +720,7 @@ int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
   info->fwd_blocks = p->fwd.blocks;
   info->adj_blocks = p->adj.blocks;
   info->n_cu = p->n_cu;
+  info->fuse = p->fuse ? 1 : 0;
   return ERNN_OK;
 }
 
@@ -693,10 +766,10 @@ int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc) {
 
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
                     float* ms_out) {
-  if (!plan || !ms_out || steps < 1 || (which != 0 && which != 1))
-    return fail(ERNN_E_INVALID, "bad argument");
+  if (!plan || !ms_out || steps < 1 || which < 0 || which > 2) return fail(ERNN_E_INVALID, "bad argument");
   Plan* p = static_cast<Plan*>(plan);
-  ERNN_TRY(check_call(p, call, which == 1, /*grads=*/false));  // timing never runs the finalize kernels
+  if (which == 2 && !p->fuse) return fail(ERNN_E_INVALID, "fused timing needs a plan with fuse = 1");
+  ERNN_TRY(check_call(p, call, which != 0, /*grads=*/false));  // timing never runs the finalize kernels
   hipStream_t st = static_cast<hipStream_t>(stream);
   ERNN_HIP(hipSetDevice(p->device));
   ERNN_HIP(launch_set_call(p->d_call, to_dev(call), st));
@@ -713,20 +786,34 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
       a.s_prev = p->slot_fwd(n - 1);
       a.s_cur = p->slot_fwd(n);
       a.s_next = p->slot_fwd(n + 1);
-      ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, st));
+      ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
     }
   } else {
     StepArgs a = base_args(p, p->adj);
     a.seg_top = p->nt_max;
     a.seg_bot = 1;
+    FusedArgs fa{};
+    fa.fwd = base_args(p, p->adj);
+    fa.fwd.record = 0;
     for (int q = 0; q < nsteps; ++q) {
       const int m = p->nt_max - q;
       a.step = m;
       a.s_prev = p->lam((m + 2) % 3);
       a.s_cur = p->lam((m + 1) % 3);
       a.s_next = p->lam(m % 3);
-      a.s_u = p->seg(q % p->K);
-      ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, st));
+      a.s_u = p->seg(0, q % p->K);
+      if (which == 1) {
+        ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, p->adj.loop, st));
+      } else {   // recompute steps of segment 0 into bank 0 while the adjoint reads bank 1
+        a.s_u = p->seg(1, q % p->K);
+        fa.adj = a;
+        const int n = q % p->K;
+        fa.fwd.step = n;
+        fa.fwd.s_prev = p->slot_seg(0, n - 1);
+        fa.fwd.s_cur = p->slot_seg(0, n);
+        fa.fwd.s_next = p->slot_seg(0, n + 1);
+        ERNN_HIP(launch_fused_step(fa, p->adj.V, p->adj.fused_blocks, st));
+      }
     }
   }
   ERNN_HIP(hipEventRecord(e1, st));

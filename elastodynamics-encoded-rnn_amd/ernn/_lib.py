@@ -40,7 +40,7 @@ class Desc(C.Structure):
                 ("rec_stride", C.c_int32), ("snapshot", C.c_int32), ("ckpt_every", C.c_int32),
                 ("fwd_v", C.c_int32), ("fwd_r", C.c_int32), ("adj_v", C.c_int32),
                 ("adj_r", C.c_int32), ("use_graph", C.c_int32), ("fwd_bpc", C.c_int32),
-                ("adj_bpc", C.c_int32), ("reserved", C.c_int32),
+                ("adj_bpc", C.c_int32), ("fuse", C.c_int32),
                 ("inst", C.POINTER(Instance)), ("src", C.POINTER(Source)), ("n_src", C.c_int64),
                 ("rec", C.POINTER(Receiver)), ("n_rec", C.c_int64)]
 
@@ -53,7 +53,7 @@ class PlanInfo(C.Structure):
                 ("adj_r", C.c_int32), ("fwd_tiles", C.c_int32), ("adj_tiles", C.c_int32),
                 ("use_graph", C.c_int32), ("fwd_bpc", C.c_int32), ("adj_bpc", C.c_int32),
                 ("fwd_blocks", C.c_int32), ("adj_blocks", C.c_int32), ("n_cu", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("fuse", C.c_int32)]
 
 
 class Call(C.Structure):

@@ -1,11 +1,12 @@
 """Autotuning: checkpoint interval under a memory cap, and tile shapes by timing.
 
 Checkpoint interval K.  Each instance keeps 3 ring slots, 2 states per
-checkpoint (S = ceil(nt / K) checkpoints), K recompute slots and 3 adjoint
-slots, each slot two padded fields.  Recompute cost is one extra forward
-whatever K is, so K only trades memory: slots(K) = 6 + 2 ceil(nt/K) + K is
-minimal near K = sqrt(2 nt).  ``choose_ckpt`` returns that K, or raises if
-even the minimum exceeds the cap.
+checkpoint (S = ceil(nt / K) checkpoints), K recompute slots per bank (two
+banks when the adjoint is fused with the next segment's recompute) and 3
+adjoint slots, each slot two padded fields.  Recompute cost is one extra
+forward whatever K is, so K only trades memory: slots(K) = 6 + 2 ceil(nt/K) +
+banks K is minimal near K = sqrt(2 nt / banks).  ``choose_ckpt`` returns that
+K, or raises if even the minimum exceeds the cap.
 
 Tiles.  ``tune_tiles`` times every candidate (V columns per lane, R rows per
 wavefront) of the forward and the adjoint kernel on the real problem for a
@@ -31,13 +32,14 @@ def padded_field_floats(nx: int, ny: int) -> int:
     return ((nx + ROW_ALIGN - 1) // ROW_ALIGN * ROW_ALIGN + 3) * pitch
 
 
-def slots(nt: int, K: int) -> int:
-    return 6 + 2 * math.ceil(nt / K) + K
+def slots(nt: int, K: int, banks: int = 1) -> int:
+    return 6 + 2 * math.ceil(nt / K) + banks * K
 
 
 def state_bytes(survey: Survey, K: int) -> int:
     nt = survey.nt_max
-    return sum(slots(nt, K) * 2 * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
+    b = 2 if survey.fused else 1
+    return sum(slots(nt, K, b) * 2 * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
 
 
 def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
@@ -45,7 +47,8 @@ def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
     nt = survey.nt_max
     if nt < 4:
         return 2
-    best = min(range(2, nt + 1), key=lambda K: (slots(nt, K), -K))
+    b = 2 if survey.fused else 1
+    best = min(range(2, nt + 1), key=lambda K: (slots(nt, K, b), -K))
     if mem_cap_bytes is not None and state_bytes(survey, best) > mem_cap_bytes:
         raise MemoryError(f"checkpointed state needs {state_bytes(survey, best) / 2**30:.2f} GiB "
                           f"> cap {mem_cap_bytes / 2**30:.2f} GiB")
@@ -59,8 +62,8 @@ TILE_SHAPES: Tuple[Tuple[int, int], ...] = (
 GRID_CAPS: Tuple[int, ...] = (0, 2, 4, 8)
 DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
     (v, r, b) for (v, r) in TILE_SHAPES for b in GRID_CAPS)
-# six-field adjoint shapes that are not built (they would spill to scratch)
-FIELD6_ADJ_EXCLUDED = {(1, 16), (2, 8)}
+# largest V * R of the built adjoint kernels per mode (larger shapes would spill registers)
+ADJ_MAX_CELLS = {"const6": 16, "dfield": 8, "field6": 4}
 
 
 def tune_tiles(engine, params, field, wavelets, steps: int = 50,
@@ -70,7 +73,8 @@ def tune_tiles(engine, params, field, wavelets, steps: int = 50,
 
     Candidates are timed in `rounds` interleaved passes (so clock and cache
     drift hit every candidate alike) and ranked by their median.
-    A candidate is (V, R, blocks per CU).
+    A candidate is (V, R, blocks per CU).  which: 0 forward, 1 adjoint,
+    2 fused adjoint + recompute (sets the adjoint tiling; fuse plans).
     Returns {which: (V, R, bpc, ms_per_step, {candidate: ms})}.
     """
     ws = engine.acquire_workspace()
@@ -79,18 +83,21 @@ def tune_tiles(engine, params, field, wavelets, steps: int = 50,
         engine.time_steps(w, 4 * steps, params, field, wavelets, ws, traces, traces)
     result = {}
     for w in which:
+        mode = engine.survey.mode
         cands = [c for c in candidates
-                 if not (w == 1 and engine.survey.mode == "field6" and tuple(c[:2]) in FIELD6_ADJ_EXCLUDED)]
+                 if (w == 0 and not (mode == "field6" and c[0] * c[1] > 8))
+                 or (w != 0 and c[0] * c[1] <= ADJ_MAX_CELLS[mode])]
+        kernel = 1 if w == 2 else w
         samples = {c: [] for c in cands}
         for _ in range(rounds):
             for c in cands:
-                engine.set_tiling(w, *c)
+                engine.set_tiling(kernel, *c)
                 engine.time_steps(w, 2, params, field, wavelets, ws, traces, traces)   # warm-up
                 samples[c].append(engine.time_steps(w, steps, params, field, wavelets, ws,
                                                     traces, traces) / steps)
         times = {c: sorted(t)[len(t) // 2] for c, t in samples.items()}
         best = min(times, key=times.get)
-        engine.set_tiling(w, *best)
+        engine.set_tiling(kernel, *best)
         result[w] = (best[0], best[1], best[2], times[best], times)
     engine.release_workspace(ws)
     return result

@@ -46,24 +46,26 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (set HIPCC or install ROCm)")
 
 
-def _scratch_kernels(remarks: str):
-    """Kernels whose resource-usage remark reports private (scratch) memory."""
+def _spilling_kernels(remarks: str):
+    """Kernels whose resource-usage remarks report scratch memory or AGPRs
+    (register spills): [(kernel, what, amount)]."""
     bad, name = [], None
     for line in remarks.splitlines():
         m = re.search(r"Function Name: (\S+)", line)
         if m:
             name = m.group(1)
-        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
-        if m and int(m.group(1)) > 0 and name:
-            bad.append((name, int(m.group(1))))
+        for what, pat in (("scratch B/lane", r"ScratchSize \[bytes/lane\]: (\d+)"), ("AGPRs", r"AGPRs: (\d+)")):
+            m = re.search(pat, line)
+            if m and int(m.group(1)) > 0 and name:
+                bad.append((name, what, int(m.group(1))))
     return bad
 
 
 def build(verbose: bool = False) -> str:
     """Compile every source for gfx950 and link LIB_PATH; returns its path.
 
-    Fails if any kernel spills to scratch memory: every tile shape the
-    library launches must keep its working set in registers."""
+    Fails if any kernel spills registers (to AGPRs or scratch): every tile
+    shape the library launches must keep its working set in VGPRs."""
     cc = hipcc()
     objs = []
     build_dir = os.path.join(ROOT, "_build")
@@ -79,9 +81,9 @@ def build(verbose: bool = False) -> str:
         if r.returncode != 0:
             sys.stderr.write(r.stderr)
             raise subprocess.CalledProcessError(r.returncode, cmd)
-        bad = _scratch_kernels(r.stderr)
+        bad = _spilling_kernels(r.stderr)
         if bad:
-            raise RuntimeError("kernels spill to scratch memory: " + ", ".join(f"{n} ({b} B/lane)" for n, b in bad))
+            raise RuntimeError("kernels spill registers: " + ", ".join(f"{n} ({a} {w})" for n, w, a in bad))
         objs.append(obj)
     tmp = LIB_PATH + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]

@@ -71,6 +71,7 @@ class Engine:
         t = tuple(sv.tiles) + (0,) * (6 - len(sv.tiles))
         d.fwd_v, d.fwd_r, d.adj_v, d.adj_r, d.fwd_bpc, d.adj_bpc = t[:6]
         d.use_graph = int(sv.use_graph)
+        d.fuse = int(sv.fused)
         d.inst = inst
         d.src, d.n_src = src, len(src_list)
         d.rec, d.n_rec = rec, len(rec_list)

@@ -89,7 +89,8 @@ typedef struct {
   int32_t fwd_bpc;      /* forward grid cap in blocks per CU (0 = one tile per wavefront);
                            below the tile count the wavefronts loop over tiles */
   int32_t adj_bpc;      /* adjoint grid cap */
-  int32_t reserved;
+  int32_t fuse;         /* 1: run each adjoint step together with one recompute step of the
+                           previous checkpoint segment (one launch; K more slots per instance) */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;
@@ -111,7 +112,7 @@ typedef struct {
   int32_t fwd_bpc, adj_bpc;
   int32_t fwd_blocks, adj_blocks;
   int32_t n_cu;             /* compute units of the plan's device */
-  int32_t reserved;
+  int32_t fuse;
 } ernn_plan_info_t;
 
 /* Per-call device pointers.
@@ -176,9 +177,10 @@ int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream);
  * synchronizes the device and drops captured graphs. */
 int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc);
 
-/* Time `steps` forward (which = 0) or adjoint (which = 1) steps with the
- * current tiling using hipEvents on `stream`; synchronizes.  The workspace
- * must be initialized.  ms_out receives the elapsed milliseconds. */
+/* Time `steps` forward (which = 0), adjoint (which = 1) or fused adjoint +
+ * recompute (which = 2, fuse plans) steps with the current tiling using
+ * hipEvents on `stream`; synchronizes.  The workspace must be initialized.
+ * ms_out receives the elapsed milliseconds. */
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
                     float* ms_out);
 

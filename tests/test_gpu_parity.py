@@ -3,7 +3,9 @@
 Tolerances: FP32 traces within 2e-6 relative L2 of the FP64 oracle at these
 sizes (the FP32 rounding drift itself); gradients within 1e-4 relative (the
 bar for the adjoint); bitwise equality where the arithmetic per cell is the
-same by construction (tilings, checkpoint intervals, batching, graph replay).
+same by construction (forward tilings, checkpoint intervals, batching, graph
+replay).  The adjoint contracts multiply-adds, so its rounding may depend on
+the adjoint tile; adjoint bit-equality is asserted only at equal adjoint tiles.
 """
 import numpy as np
 import pytest
@@ -122,7 +124,7 @@ def test_field6_forward_and_gradients_match_oracle():
     shot = make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=_receivers(nx, ny))
     G = rng.standard_normal((2, nt, len(shot.receivers)))
     ref, gcell = oracle_run(shot, [f.astype(np.float64) for f in fields], 1, False, G)
-    for tiles in [(1, 2, 1, 2), (2, 8, 2, 4), (4, 4, 4, 4, 1, 1), (1, 1, 2, 1, 2, 2), (2, 8, 1, 8)]:
+    for tiles in [(1, 2, 1, 2), (2, 4, 2, 2), (4, 2, 4, 1, 1, 1), (1, 1, 2, 1, 2, 2), (1, 8, 1, 4)]:
         sv = ernn.Survey([shot], mode="field6", ckpt_every=6, tiles=tiles)
         eng, tr, (gp, gf) = run_hip(sv, field=fields, G=G)
         assert gp is None
@@ -149,7 +151,8 @@ def test_uneven_batch_equals_single_instance_runs():
     gsum = np.zeros((3, 6))
     for b, s in enumerate(shots):
         one = ernn.Shot(**{**s.__dict__, "set": 0})
-        e1, t1, (g1, _) = run_hip(ernn.Survey([one], mode="const6", ckpt_every=6), params=Cs[s.set], G=Gs[b])
+        e1, t1, (g1, _) = run_hip(ernn.Survey([one], mode="const6", ckpt_every=6, tiles=sv.tiles),
+                                  params=Cs[s.set], G=Gs[b])
         assert np.array_equal(tr[eng.trace_off[b]:eng.trace_off[b + 1]], t1), f"instance {b}"
         gsum[s.set] += g1
         e1.close()
@@ -238,3 +241,28 @@ def test_persistent_wavefronts_are_bitwise_equal(mode):
         assert np.array_equal(out[bpc][0], out[0][0]), bpc
         for a, b in zip(out[bpc][1], out[0][1]):
             assert (a is None and b is None) or np.array_equal(a, b), bpc
+
+
+@pytest.mark.parametrize("mode", ["const6", "dfield", "field6"])
+def test_fused_backward_equals_unfused(mode):
+    """Sharing each adjoint launch with a recompute step of the next segment
+    (two segment banks) gives the same traces and gradients as the plain
+    schedule, including a short last segment (nt % K != 0) and K > nt / 2."""
+    nx, ny, h = 40, 70, 0.2e-3
+    rng = np.random.default_rng(7)
+    fields = np.stack([rng.uniform(0.6, 1.0, (nx, ny)) * c for c in C_ANISO]).astype(np.float32)
+    D = rng.uniform(0.4, 1.0, (nx, ny)).astype(np.float32)
+    for nt, K in [(47, 5), (40, 8), (30, 16), (25, 40)]:
+        shot = make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=_receivers(nx, ny))
+        G = rng.standard_normal((2, nt, len(shot.receivers)))
+        kw = dict(params=None if mode == "field6" else C_ANISO,
+                  field=fields if mode == "field6" else (D if mode == "dfield" else None), G=G)
+        outs = []
+        for fuse in (False, True):
+            sv = ernn.Survey([shot], mode=mode, ckpt_every=K, tiles=(2, 2, 2, 2), fuse=fuse)
+            eng, tr, grads = run_hip(sv, **kw)
+            outs.append((tr, grads))
+            eng.close()
+        assert np.array_equal(outs[0][0], outs[1][0]), (nt, K)
+        for a, b in zip(outs[0][1], outs[1][1]):
+            assert (a is None and b is None) or np.array_equal(a, b), (mode, nt, K)

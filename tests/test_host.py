@@ -46,7 +46,7 @@ def _desc(inst, src=(), rec=(), **kw):
         R[k].i, R[k].j = i, j
     d = _lib.Desc()
     d.n_inst, d.n_sets, d.param_mode, d.rec_stride, d.ckpt_every = n, 1, 0, 1, 4
-    d.fwd_v, d.fwd_r, d.adj_v, d.adj_r, d.fwd_bpc, d.adj_bpc = 2, 8, 2, 8, 0, 0
+    d.fwd_v, d.fwd_r, d.adj_v, d.adj_r, d.fwd_bpc, d.adj_bpc = 2, 4, 2, 2, 0, 0
     for f, v in kw.items():
         setattr(d, f, v)
     d.inst, d.src, d.n_src, d.rec, d.n_rec = I, S, len(src), R, len(rec)
@@ -71,8 +71,9 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
     ({}, {"adj_r": 3}, "tile (V, R)"),
     ({}, {"fwd_v": 4, "fwd_r": 8}, "tile (V, R)"),
     ({}, {"adj_bpc": -1}, "blocks per CU"),
-    ({}, {"param_mode": 2, "adj_v": 2, "adj_r": 8}, "adjoint tile"),
-    ({}, {"param_mode": 2, "adj_v": 1, "adj_r": 16}, "adjoint tile"),
+    ({}, {"param_mode": 2, "adj_v": 2, "adj_r": 4}, "adjoint tile"),
+    ({}, {"param_mode": 2, "adj_v": 1, "adj_r": 8}, "adjoint tile"),
+    ({}, {"param_mode": 1, "adj_v": 4, "adj_r": 4}, "adjoint tile"),
 ])
 def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
     inst = dict(GOOD, **patch)
@@ -125,9 +126,12 @@ def test_engine_refuses_without_gpu():
 # ------------------------------------------------------------------ helpers
 def test_checkpoint_interval_minimizes_slots():
     wl = configs.single_512(nt=2000)
-    K = autotune.choose_ckpt(wl.survey)
-    assert autotune.slots(2000, K) == min(autotune.slots(2000, k) for k in range(2, 2001))
-    assert abs(K - math.sqrt(2 * 2000)) < 8
+    for fuse, banks in ((False, 1), (True, 2)):
+        wl.survey.fuse = fuse
+        assert wl.survey.fused == fuse
+        K = autotune.choose_ckpt(wl.survey)
+        assert autotune.slots(2000, K, banks) == min(autotune.slots(2000, k, banks) for k in range(2, 2001))
+        assert abs(K - math.sqrt(2 * 2000 / banks)) < 8
     with pytest.raises(MemoryError):
         autotune.choose_ckpt(wl.survey, mem_cap_bytes=1 << 20)
 
