@@ -135,11 +135,15 @@ def time_autograd(torch, wl, theta0, scale, D0, observed, window):
 
     If the survey is longer than `window` steps or has several shots, one shot
     is timed over `window` steps and the per-step cost is scaled to
-    n_shots * nt steps (per-step cost is constant in n)."""
+    n_shots * nt steps (per-step cost is constant in n).  The window is also
+    capped so that the autograd tape (about 48 saved fields per step) fits in
+    half of the device memory."""
     res = {}
     shots = wl.survey.shots
     s0 = shots[0]
-    W = min(window, s0.nt)
+    tape_step = 48 * 4 * s0.nx * s0.ny
+    mem_cap = torch.cuda.get_device_properties(0).total_memory // 2
+    W = max(10, min(window, s0.nt, mem_cap // tape_step))
     scale_f = sum(s.nt for s in shots) / W
     for faithful in (True, False):
         theta = torch.nn.Parameter(theta0.clone())
@@ -318,13 +322,19 @@ def main():
     name = max(kern, key=lambda k: kern[k][0] * kern[k][1])
     ms, launches, bpc = kern[name]
     achieved = cells * bpc / (ms * 1e-3) / 1e9
-    traffic = None
+    # HBM bytes per launch from the committed counter passes (FETCH_SIZE and
+    # WRITE_SIZE, calibrated on a copy kernel), taken with the tile shapes
+    # recorded beside them (traffic_tiles; within a few percent across tile
+    # shapes, since every shape reads each field once per launch).
+    traffic, traffic_tiles = None, None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        tj = json.load(open(tpath))
-        traffic = tj.get(f"{wl.name}/{name}")
+        entry = json.load(open(tpath)).get(f"{wl.name}/{name}")
+        if entry:
+            traffic, traffic_tiles = entry["bytes"], entry["tiles"]
     roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_tiles": traffic_tiles,
                 "bytes_per_cell": bpc, "cells_per_launch": cells, "avg_launch_us": round(ms * 1e3, 3),
                 "launches_per_iter": launches,
                 "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),

@@ -26,6 +26,13 @@ constexpr int kRowAlign = 16;
 
 enum ParamMode : int { kConst6 = 0, kDField = 1, kField6 = 2 };
 
+// fp64 partial sums per tile for the constant gradients (kConst6, kDField).
+// Slots 0-5 hold S11, S22, S12, S16, S26, S66 with the stencil weight h^-2 (or
+// h^-2 / 4 for slot 2) applied; slots 6-7 hold the mixed-derivative parts of
+// C16 and C26 (weight h^-2 / 4).  dJ/dC = (S11, S22, S12, S16 + 2 s6,
+// S26 + 2 s7, S66 + S12).
+constexpr int kPartials = 8;
+
 // One instance = one shot on one grid with one parameter set.
 struct InstDev {
   int nx, ny, pitch, nt;
@@ -77,7 +84,7 @@ struct StepArgs {
   const SrcEv* src;
   const RecEv* rec;
   const CallDev* call;
-  int64_t partial_off;     // double offset in the workspace of [n_tiles][6] fp64 partials
+  int64_t partial_off;     // double offset in the workspace of [n_tiles][kPartials] fp64 partials
   int n_tiles;
   int mode;
   int step;                // forward: n (produces u^{n+1}); adjoint: m (produces lam^m)

@@ -29,6 +29,37 @@
 
 namespace ernn {
 
+// Global-memory views of the call's buffers.  The pointers reach the kernels
+// through the device-side argument block, so the compiler would treat them as
+// generic (flat) pointers; a flat access counts against both the vector-memory
+// and the LDS counters, which makes the first cross-lane shuffle of a tile wait
+// for every load in flight.  In the global address space, loads retire in
+// order and a row can be consumed while the next rows are still arriving.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) f32x2 gfloat2;
+typedef __attribute__((address_space(1))) f32x4 gfloat4;
+typedef __attribute__((address_space(1))) double gdouble;
+
+struct GCall {
+  gfloat* ws;
+  const gfloat* params;
+  const gfloat* wav;
+  gfloat* traces;
+  const gfloat* gtr;
+};
+
+__device__ __forceinline__ GCall global_call(const CallDev& c) {
+  GCall g;
+  g.ws = (gfloat*)c.ws;
+  g.params = (const gfloat*)c.params;
+  g.wav = (const gfloat*)c.wav;
+  g.traces = (gfloat*)c.traces;
+  g.gtr = (const gfloat*)c.gtr;
+  return g;
+}
+
 // --------------------------------------------------------------------------
 // Register rows
 // --------------------------------------------------------------------------
@@ -38,12 +69,12 @@ struct Row {
 };
 
 template <int V>
-__device__ __forceinline__ void load_vec(const float* __restrict__ p, float (&c)[V]) {
+__device__ __forceinline__ void load_vec(const gfloat* __restrict__ p, float (&c)[V]) {
   if constexpr (V == 4) {
-    const float4 t = *reinterpret_cast<const float4*>(p);
+    const f32x4 t = *reinterpret_cast<const gfloat4*>(p);
     c[0] = t.x; c[1] = t.y; c[2] = t.z; c[3] = t.w;
   } else if constexpr (V == 2) {
-    const float2 t = *reinterpret_cast<const float2*>(p);
+    const f32x2 t = *reinterpret_cast<const gfloat2*>(p);
     c[0] = t.x; c[1] = t.y;
   } else {
     c[0] = *p;
@@ -51,11 +82,11 @@ __device__ __forceinline__ void load_vec(const float* __restrict__ p, float (&c)
 }
 
 template <int V>
-__device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&c)[V]) {
+__device__ __forceinline__ void store_vec(gfloat* __restrict__ p, const float (&c)[V]) {
   if constexpr (V == 4) {
-    *reinterpret_cast<float4*>(p) = make_float4(c[0], c[1], c[2], c[3]);
+    *reinterpret_cast<gfloat4*>(p) = f32x4{c[0], c[1], c[2], c[3]};
   } else if constexpr (V == 2) {
-    *reinterpret_cast<float2*>(p) = make_float2(c[0], c[1]);
+    *reinterpret_cast<gfloat2*>(p) = f32x2{c[0], c[1]};
   } else {
     *p = c[0];
   }
@@ -66,7 +97,7 @@ __device__ __forceinline__ void store_vec(float* __restrict__ p, const float (&c
 // lane by a cross-lane shuffle.  (DPP whole-wavefront shifts were measured
 // slower than ds_bpermute here: 147 vs 122 us on the 1024^2 damage adjoint.)
 template <int V>
-__device__ __forceinline__ Row<V> load_row(const float* __restrict__ rowp, int lane) {
+__device__ __forceinline__ Row<V> load_row(const gfloat* __restrict__ rowp, int lane) {
   Row<V> r;
   float c[V];
   load_vec<V>(rowp + lane * V, c);
@@ -159,34 +190,125 @@ __device__ __forceinline__ void apply_op(const Stiff& s, const Deriv& d, float& 
         s.c26 * d.ux_yy) + s.c226 * d.uy_xy;
 }
 
-// dA/dC_p u contracted with (lx, ly) at one cell, p in C11 C22 C12 C16 C26 C66
-// order.  With d = derivatives of u these are the per-cell partials; with d =
-// derivatives of the adjoint field and (lx, ly) = u the same expressions give
-// the summation-by-parts form used for constant C (all stencils symmetric).
-__device__ __forceinline__ void grad_terms(const Deriv& d, float lx, float ly, float (&g)[6]) {
-  g[0] = lx * d.ux_xx;
-  g[1] = ly * d.uy_yy;
-  g[2] = lx * d.uy_xy + ly * d.ux_xy;
-  g[3] = lx * (2.f * d.ux_xy + d.uy_xx) + ly * d.ux_xx;
-  g[4] = lx * d.uy_yy + ly * (d.ux_yy + 2.f * d.uy_xy);
-  g[5] = lx * (d.ux_yy + d.uy_xy) + ly * (d.uy_xx + d.ux_xy);
+// --------------------------------------------------------------------------
+// Adjoint arithmetic.  The adjoint is held to the gradient tolerance, not to
+// the reference's rounding order, so it works with unit-weight stencils
+//   Uxx = (f[i-1][j] + f[i+1][j]) - 2 f[i][j],   Uyy likewise along j,
+//   Uxy = (f[i+1][j+1] - f[i+1][j-1]) - (f[i-1][j+1] - f[i-1][j-1]),
+// i.e. d_xx = a Uxx, d_yy = a Uyy, d_xy = (a/4) Uxy with a = h^-2, folds a,
+// a/4 and dt^2/rho into per-wavefront coefficients, and writes every
+// multiply-add as an explicit fma.
+// --------------------------------------------------------------------------
+struct UDeriv {
+  float xx_x, yy_x, xy_x, xx_y, yy_y, xy_y;   // _x: of the x component, _y: of the y component
+};
+
+template <int V>
+__device__ __forceinline__ float uxx(const Row<V>& m, const Row<V>& c, const Row<V>& p, int x) {
+  return __builtin_fmaf(-2.f, c.v[x], m.v[x] + p.v[x]);
+}
+template <int V>
+__device__ __forceinline__ float uyy(const Row<V>& c, int x) {
+  return __builtin_fmaf(-2.f, c.v[x], c.v[x - 1] + c.v[x + 1]);
+}
+template <int V>
+__device__ __forceinline__ float uxy(const Row<V>& m, const Row<V>& p, int x) {
+  return (p.v[x + 1] - p.v[x - 1]) - (m.v[x + 1] - m.v[x - 1]);
 }
 
-// Summation-by-parts variant: <lam, dA/dC_p u> = <D lam, u> per stencil.
-// dl = derivatives of lam (or of D*lam), (ux, uy) = u at the cell.
-__device__ __forceinline__ void grad_terms_sbp(const Deriv& dl, float ux, float uy, float (&g)[6]) {
-  g[0] = dl.ux_xx * ux;
-  g[1] = dl.uy_yy * uy;
-  g[2] = dl.ux_xy * uy + dl.uy_xy * ux;
-  g[3] = 2.f * dl.ux_xy * ux + dl.ux_xx * uy + dl.uy_xx * ux;
-  g[4] = dl.ux_yy * uy + dl.uy_yy * ux + 2.f * dl.uy_xy * uy;
-  g[5] = dl.ux_yy * ux + dl.ux_xy * uy + dl.uy_xx * uy + dl.uy_xy * ux;
+template <int V>
+__device__ __forceinline__ UDeriv uderivs(const Row<V>& xm, const Row<V>& xc, const Row<V>& xp,
+                                          const Row<V>& ym, const Row<V>& yc, const Row<V>& yp, int x) {
+  UDeriv d;
+  d.xx_x = uxx<V>(xm, xc, xp, x);
+  d.yy_x = uyy<V>(xc, x);
+  d.xy_x = uxy<V>(xm, xp, x);
+  d.xx_y = uxx<V>(ym, yc, yp, x);
+  d.yy_y = uyy<V>(yc, x);
+  d.xy_y = uxy<V>(ym, yp, x);
+  return d;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
+// Operator coefficients for unit-weight derivatives, scaled by f:
+// k_p = f a C_p on the xx / yy taps, q_* = f (a/4) C on the mixed taps.
+struct OpCoef {
+  float k11, k22, k16, k26, k66, q1266, q216, q226;
+};
+
+__device__ __forceinline__ OpCoef op_coef(const float (&c)[6], float a, float f) {
+  const float fa = f * a, fq = 0.25f * fa;
+  OpCoef o;
+  o.k11 = fa * c[0];
+  o.k22 = fa * c[1];
+  o.k16 = fa * c[3];
+  o.k26 = fa * c[4];
+  o.k66 = fa * c[5];
+  o.q1266 = fq * (c[2] + c[5]);
+  o.q216 = fq * (2.f * c[3]);
+  o.q226 = fq * (2.f * c[4]);
+  return o;
+}
+
+// (bx, by) + f (A u) at one cell: the six products of PI:86-98 per component.
+__device__ __forceinline__ void apply_op_u(const OpCoef& k, const UDeriv& d, float bx, float by,
+                                           float& ox, float& oy) {
+  ox = __builtin_fmaf(k.k11, d.xx_x, __builtin_fmaf(k.k66, d.yy_x, __builtin_fmaf(k.k16, d.xx_y,
+       __builtin_fmaf(k.k26, d.yy_y, __builtin_fmaf(k.q1266, d.xy_y, __builtin_fmaf(k.q216, d.xy_x, bx))))));
+  oy = __builtin_fmaf(k.k66, d.xx_y, __builtin_fmaf(k.k22, d.yy_y, __builtin_fmaf(k.q1266, d.xy_x,
+       __builtin_fmaf(k.k16, d.xx_x, __builtin_fmaf(k.k26, d.yy_x, __builtin_fmaf(k.q226, d.xy_y, by))))));
+}
+
+// Constant-C gradient sums at one cell (kPartials slots, weights applied at
+// the end): <lam, dA/dC_p u> = <D_p w, u> by summation by parts, every
+// stencil being symmetric; d = unit derivatives of w (lam, or D * lam),
+// (ux, uy) = u at the cell.
+__device__ __forceinline__ void sbp_sums(const UDeriv& d, float ux, float uy, float (&S)[kPartials]) {
+  S[0] = __builtin_fmaf(d.xx_x, ux, S[0]);
+  S[1] = __builtin_fmaf(d.yy_y, uy, S[1]);
+  S[2] = __builtin_fmaf(d.xy_x, uy, __builtin_fmaf(d.xy_y, ux, S[2]));
+  S[3] = __builtin_fmaf(d.xx_x, uy, __builtin_fmaf(d.xx_y, ux, S[3]));
+  S[4] = __builtin_fmaf(d.yy_x, uy, __builtin_fmaf(d.yy_y, ux, S[4]));
+  S[5] = __builtin_fmaf(d.yy_x, ux, __builtin_fmaf(d.xx_y, uy, S[5]));
+  S[6] = __builtin_fmaf(d.xy_x, ux, S[6]);
+  S[7] = __builtin_fmaf(d.xy_y, uy, S[7]);
+}
+
+// Per-cell gradient of the six stiffness fields, lam . dA/dC_p u at the cell,
+// divided by a (the caller scales by a once): d = unit derivatives of u.
+__device__ __forceinline__ void field_grads(const UDeriv& d, float lx, float ly, float (&g)[6]) {
+  g[0] = lx * d.xx_x;
+  g[1] = ly * d.yy_y;
+  g[2] = 0.25f * __builtin_fmaf(lx, d.xy_y, ly * d.xy_x);
+  g[3] = __builtin_fmaf(lx, __builtin_fmaf(0.5f, d.xy_x, d.xx_y), ly * d.xx_x);
+  g[4] = __builtin_fmaf(lx, d.yy_y, ly * __builtin_fmaf(0.5f, d.xy_y, d.yy_x));
+  g[5] = __builtin_fmaf(lx, __builtin_fmaf(0.25f, d.xy_y, d.yy_x), ly * __builtin_fmaf(0.25f, d.xy_x, d.xx_y));
+}
+
+// Wavefront sum of eight per-lane values by a halving butterfly: each step
+// swaps half of the values still held with the partner lane, so the eight
+// sums take 4 + 2 + 1 + 3 shuffles.  Every lane returns the total of value
+// lane / 8.
+__device__ __forceinline__ double wave_sum8(const double (&v)[8], int lane) {
+  const bool b32 = lane & 32, b16 = lane & 16, b8 = lane & 8;
+  double h4[4];
 #pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+  for (int i = 0; i < 4; ++i) {
+    const double send = b32 ? v[i] : v[i + 4];
+    const double keep = b32 ? v[i + 4] : v[i];
+    h4[i] = keep + __shfl_xor(send, 32, kWave);
+  }
+  double h2[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double send = b16 ? h4[i] : h4[i + 2];
+    const double keep = b16 ? h4[i + 2] : h4[i];
+    h2[i] = keep + __shfl_xor(send, 16, kWave);
+  }
+  double s = (b8 ? h2[1] : h2[0]) + __shfl_xor(b8 ? h2[0] : h2[1], 8, kWave);
+  s += __shfl_xor(s, 4, kWave);
+  s += __shfl_xor(s, 2, kWave);
+  s += __shfl_xor(s, 1, kWave);
+  return s;
 }
 
 // Tiles of this wavefront.  Block b runs on XCD b % 8 (dispatch is round
@@ -210,10 +332,10 @@ __device__ __forceinline__ TileRange tile_range(int n_tiles) {
   return r;
 }
 
-__device__ __forceinline__ const float* prow(const float* base, int64_t pitch, int r, int j0) {
+__device__ __forceinline__ const gfloat* prow(const gfloat* base, int64_t pitch, int r, int j0) {
   return base + (int64_t)(r + 1) * pitch + kLeftPad + j0;
 }
-__device__ __forceinline__ float* prow(float* base, int64_t pitch, int r, int j0) {
+__device__ __forceinline__ gfloat* prow(gfloat* base, int64_t pitch, int r, int j0) {
   return base + (int64_t)(r + 1) * pitch + kLeftPad + j0;
 }
 
@@ -227,20 +349,20 @@ __device__ __forceinline__ float* prow(float* base, int64_t pitch, int r, int j0
 // the loads need no bounds checks; stores and events stop at row nx.
 // --------------------------------------------------------------------------
 template <int MODE, int V, int R>
-__device__ __forceinline__ void fwd_tile(const StepArgs& a, const CallDev& C, const TileDev& tl,
+__device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, const TileDev& tl,
                                          const InstDev& I, int lane) {
   const int n = a.step;
 
   const int64_t pitch = I.pitch;
   const int j0 = tl.j0;
-  float* pool = C.ws + I.pool_off;
-  const float* u0x = pool + (int64_t)a.s_cur * 2 * I.field;
-  const float* u0y = u0x + I.field;
-  const float* u1x = pool + (int64_t)a.s_prev * 2 * I.field;
-  const float* u1y = u1x + I.field;
-  float* unx = pool + (int64_t)a.s_next * 2 * I.field;
-  float* uny = unx + I.field;
-  const float* coef = C.ws + I.coef_off;
+  gfloat* pool = C.ws + I.pool_off;
+  const gfloat* u0x = pool + (int64_t)a.s_cur * 2 * I.field;
+  const gfloat* u0y = u0x + I.field;
+  const gfloat* u1x = pool + (int64_t)a.s_prev * 2 * I.field;
+  const gfloat* u1y = u1x + I.field;
+  gfloat* unx = pool + (int64_t)a.s_next * 2 * I.field;
+  gfloat* uny = unx + I.field;
+  const gfloat* coef = C.ws + I.coef_off;
 
   const Weights w = make_weights(I.hinv2);
   float c0[6];
@@ -327,8 +449,8 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const CallDev& C, co
 
     if (rec_step) {
       if (a.snapshot) {
-        float* tx = C.traces + I.trace_off + ((int64_t)k * I.nx + i) * I.ny;
-        float* ty = tx + (int64_t)nf * I.nx * I.ny;
+        gfloat* tx = C.traces + I.trace_off + ((int64_t)k * I.nx + i) * I.ny;
+        gfloat* ty = tx + (int64_t)nf * I.nx * I.ny;
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           const int j = j0 + lane * V + e;
@@ -340,8 +462,8 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const CallDev& C, co
           if (ev.i != i) break;
           const int lc = ev.j - j0;
           if (lane == lc / V) {
-            float* tx = C.traces + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
-            float* ty = tx + (int64_t)nf * I.n_rec;
+            gfloat* tx = C.traces + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+            gfloat* ty = tx + (int64_t)nf * I.n_rec;
 #pragma unroll
             for (int e = 0; e < V; ++e) {
               if (e == lc % V) { *tx = nvx[e]; *ty = nvy[e]; }
@@ -362,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const TileRange tr = tile_range(a.n_tiles);
   if (tr.t >= tr.end) return;
-  const CallDev C = *a.call;
+  const GCall C = global_call(*a.call);
   if constexpr (!LOOP) {
     const TileDev tl = a.tiles[tr.t];
     const InstDev I = a.inst[tl.inst];
@@ -398,8 +520,8 @@ struct AdjRows {
 };
 
 template <int MODE, int V>
-__device__ __forceinline__ AdjRows<MODE, V> load_adj_row(const float* l1x, const float* l1y,
-                                                          const float* coef, int64_t field, int64_t pitch,
+__device__ __forceinline__ AdjRows<MODE, V> load_adj_row(const gfloat* l1x, const gfloat* l1y,
+                                                          const gfloat* coef, int64_t field, int64_t pitch,
                                                           int r, int j0, int lane) {
   AdjRows<MODE, V> o;
   const Row<V> lx = load_row<V>(prow(l1x, pitch, r, j0), lane);
@@ -422,46 +544,49 @@ __device__ __forceinline__ AdjRows<MODE, V> load_adj_row(const float* l1x, const
       const float c11 = cr[0].v[e], c22 = cr[1].v[e], c12 = cr[2].v[e];
       const float c16 = cr[3].v[e], c26 = cr[4].v[e], c66 = cr[5].v[e];
       const float x = lx.v[e], y = ly.v[e];
-      o.w[0].v[e] = c11 * x + c16 * y;                 // -> Dxx, x row
-      o.w[1].v[e] = c66 * x + c26 * y;                 // -> Dyy, x row
-      o.w[2].v[e] = 2.f * c16 * x + (c12 + c66) * y;   // -> Dxy, x row
-      o.w[3].v[e] = c16 * x + c66 * y;                 // -> Dxx, y row
-      o.w[4].v[e] = c26 * x + c22 * y;                 // -> Dyy, y row
-      o.w[5].v[e] = (c12 + c66) * x + 2.f * c26 * y;   // -> Dxy, y row
+      const float c1266 = c12 + c66;
+      o.w[0].v[e] = __builtin_fmaf(c11, x, c16 * y);           // -> Dxx, x row
+      o.w[1].v[e] = __builtin_fmaf(c66, x, c26 * y);           // -> Dyy, x row
+      o.w[2].v[e] = __builtin_fmaf(2.f * c16, x, c1266 * y);   // -> Dxy, x row
+      o.w[3].v[e] = __builtin_fmaf(c16, x, c66 * y);           // -> Dxx, y row
+      o.w[4].v[e] = __builtin_fmaf(c26, x, c22 * y);           // -> Dyy, y row
+      o.w[5].v[e] = __builtin_fmaf(c1266, x, 2.f * c26 * y);   // -> Dxy, y row
     }
   }
   return o;
 }
 
 template <int MODE, int V, int R>
-__device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, const TileDev& tl,
+__device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, const TileDev& tl,
                                          const InstDev& I, int t, int lane) {
   // The adjoint is held to the gradient tolerance, not to the reference's
-  // rounding order, so it lets the compiler fuse multiply-adds.
+  // rounding order: explicit fmas, and the compiler may fuse the rest.
 #pragma clang fp contract(fast)
   const int m = a.step;
 
   const int64_t pitch = I.pitch;
   const int j0 = tl.j0;
-  float* pool = C.ws + I.pool_off;
-  const float* l1x = pool + (int64_t)a.s_cur * 2 * I.field;   // lam^{m+1}
-  const float* l1y = l1x + I.field;
-  const float* l2x = pool + (int64_t)a.s_prev * 2 * I.field;  // lam^{m+2}
-  const float* l2y = l2x + I.field;
-  float* lnx = pool + (int64_t)a.s_next * 2 * I.field;        // lam^m
-  float* lny = lnx + I.field;
-  const float* umx = pool + (int64_t)a.s_u * 2 * I.field;     // u^m
-  const float* umy = umx + I.field;
-  const float* coef = C.ws + I.coef_off;
+  gfloat* pool = C.ws + I.pool_off;
+  const gfloat* l1x = pool + (int64_t)a.s_cur * 2 * I.field;   // lam^{m+1}
+  const gfloat* l1y = l1x + I.field;
+  const gfloat* l2x = pool + (int64_t)a.s_prev * 2 * I.field;  // lam^{m+2}
+  const gfloat* l2y = l2x + I.field;
+  gfloat* lnx = pool + (int64_t)a.s_next * 2 * I.field;        // lam^m
+  gfloat* lny = lnx + I.field;
+  const gfloat* umx = pool + (int64_t)a.s_u * 2 * I.field;     // u^m
+  const gfloat* umy = umx + I.field;
+  const gfloat* coef = C.ws + I.coef_off;
 
-  const Weights w = make_weights(I.hinv2);
+  const float kdt = (float)I.gscale;   // dt^2 / rho
   float c0[6];
+  OpCoef kl, ku;   // kl: kdt A(C0) on the adjoint field; ku: A(C0) / a on u (dJ/dD)
   if constexpr (MODE != kField6) {
 #pragma unroll
     for (int p = 0; p < 6; ++p) c0[p] = C.params[tl.set * 6 + p];
+    kl = op_coef(c0, I.hinv2, kdt);
+    ku = op_coef(c0, 1.f, 1.f);
   }
-  const Stiff sc = make_stiff(c0[0], c0[1], c0[2], c0[3], c0[4], c0[5]);
-  const float kdt = (float)I.gscale;   // dt^2 / rho
+  const float ka = kdt * I.hinv2, kq = 0.25f * ka;   // kField6: kdt a, kdt a / 4
 
   // residual injection on recorded steps m = kP + 1
   const bool inj = ((m - 1) % a.stride == 0) && ((m - 1) / a.stride < I.n_frames);
@@ -474,8 +599,9 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, co
   const bool seg_last = (m == a.seg_bot);
   constexpr int NACC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
   constexpr int NW = AdjRows<MODE, V>::NW;
-  float* acc32 = C.ws + I.acc_off;
-  double* acc64 = reinterpret_cast<double*>(C.ws) + I.acc64_off;
+  gfloat* acc32 = C.ws + I.acc_off;
+  const double gsa = I.gscale * (double)I.hinv2;   // per-cell sums carry 1 / a
+  gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
 
   // ---- loads
   AdjRows<MODE, V> lr[R + 2];
@@ -505,9 +631,29 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, co
     }
   }
 
-  float S[6];
+  // The accumulators this tile read-modify-writes are read together with its
+  // other loads, so the updates at the end of the tile wait for no memory.
+  float acc_in[NACC > 0 ? R : 1][NACC > 0 ? NACC : 1][V];
+  if constexpr (NACC > 0) {
 #pragma unroll
-  for (int p = 0; p < 6; ++p) S[p] = 0.f;
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int q = 0; q < NACC; ++q) {
+        if (seg_first) {
+#pragma unroll
+          for (int e = 0; e < V; ++e) acc_in[r][q][e] = 0.f;
+        } else {
+          load_vec<V>(prow(acc32 + q * I.field, pitch, tl.i0 + r, j0) + lane * V, acc_in[r][q]);
+        }
+      }
+  }
+  gdouble* part = reinterpret_cast<gdouble*>(C.ws) + a.partial_off + (int64_t)t * kPartials + (lane >> 3);
+  double part_in = 0.0;
+  if constexpr (MODE != kField6) part_in = *part;
+
+  float S[kPartials];
+#pragma unroll
+  for (int p = 0; p < kPartials; ++p) S[p] = 0.f;
 
   int rp = tl.rec_begin;
 #pragma unroll
@@ -517,53 +663,47 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, co
     const AdjRows<MODE, V>& Lm = lr[r];
     const AdjRows<MODE, V>& Lc = lr[r + 1];
     const AdjRows<MODE, V>& Lp = lr[r + 2];
-    float atx[V], aty[V];
+    float nvx[V], nvy[V];
     float G[NACC > 0 ? NACC : 1][V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
       const int x = e + 1;
-      float g[6];
+      const float bx = __builtin_fmaf(2.f, Lc.cx[e], -l2vx[r][e]);   // 2 lam^{m+1} - lam^{m+2}
+      const float by = __builtin_fmaf(2.f, Lc.cy[e], -l2vy[r][e]);
       if constexpr (MODE == kConst6 || MODE == kDField) {
         // A^T lam = A(C0) (D * lam): constant stencils on the (weighted) adjoint field
-        const Deriv dl = derivs<V>(Lm.w[0], Lc.w[0], Lp.w[0], Lm.w[1], Lc.w[1], Lp.w[1], x, w);
-        apply_op(sc, dl, atx[e], aty[e]);
+        const UDeriv dl = uderivs<V>(Lm.w[0], Lc.w[0], Lp.w[0], Lm.w[1], Lc.w[1], Lp.w[1], x);
+        apply_op_u(kl, dl, bx, by, nvx[e], nvy[e]);
         float ux, uy;
         if constexpr (MODE == kConst6) { ux = uvx[r][e]; uy = uvy[r][e]; }
         else { ux = um[r + 1][0].v[x]; uy = um[r + 1][1].v[x]; }
-        grad_terms_sbp(dl, ux, uy, g);
-#pragma unroll
-        for (int p = 0; p < 6; ++p) S[p] += g[p];
+        sbp_sums(dl, ux, uy, S);
         if constexpr (MODE == kDField) {
-          // dJ/dD(cell) = lam . A(C0) u^m  at the cell
-          const Deriv du = derivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
-                                     um[r + 2][1], x, w);
+          // dJ/dD(cell) = lam . A(C0) u^m at the cell (divided by a)
+          const UDeriv du = uderivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
+                                       um[r + 2][1], x);
           float ox, oy;
-          apply_op(sc, du, ox, oy);
-          G[0][e] = Lc.cx[e] * ox + Lc.cy[e] * oy;
+          apply_op_u(ku, du, 0.f, 0.f, ox, oy);
+          G[0][e] = __builtin_fmaf(Lc.cx[e], ox, Lc.cy[e] * oy);
         }
       } else {
-        atx[e] = (d_xx<V>(Lm.w[0], Lc.w[0], Lp.w[0], x, w) + d_yy<V>(Lc.w[1], x, w)) +
-                 d_xy<V>(Lm.w[2], Lp.w[2], x, w);
-        aty[e] = (d_xx<V>(Lm.w[3], Lc.w[3], Lp.w[3], x, w) + d_yy<V>(Lc.w[4], x, w)) +
-                 d_xy<V>(Lm.w[5], Lp.w[5], x, w);
-        const Deriv du = derivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
-                                   um[r + 2][1], x, w);
-        grad_terms(du, Lc.cx[e], Lc.cy[e], g);
+        const float sx = uxx<V>(Lm.w[0], Lc.w[0], Lp.w[0], x) + uyy<V>(Lc.w[1], x);
+        const float sy = uxx<V>(Lm.w[3], Lc.w[3], Lp.w[3], x) + uyy<V>(Lc.w[4], x);
+        nvx[e] = __builtin_fmaf(ka, sx, __builtin_fmaf(kq, uxy<V>(Lm.w[2], Lp.w[2], x), bx));
+        nvy[e] = __builtin_fmaf(ka, sy, __builtin_fmaf(kq, uxy<V>(Lm.w[5], Lp.w[5], x), by));
+        const UDeriv du = uderivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
+                                     um[r + 2][1], x);
+        float g[6];
+        field_grads(du, Lc.cx[e], Lc.cy[e], g);
 #pragma unroll
         for (int p = 0; p < 6; ++p) G[p][e] = g[p];
       }
     }
 
-    float nvx[V], nvy[V];
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-      nvx[e] = (2.f * Lc.cx[e] - l2vx[r][e]) + kdt * atx[e];
-      nvy[e] = (2.f * Lc.cy[e] - l2vy[r][e]) + kdt * aty[e];
-    }
     if (inj) {
       if (a.snapshot) {
-        const float* gx = C.gtr + I.trace_off + ((int64_t)k * I.nx + i) * I.ny;
-        const float* gy = gx + (int64_t)nf * I.nx * I.ny;
+        const gfloat* gx = C.gtr + I.trace_off + ((int64_t)k * I.nx + i) * I.ny;
+        const gfloat* gy = gx + (int64_t)nf * I.nx * I.ny;
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           const int j = j0 + lane * V + e;
@@ -575,8 +715,8 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, co
           if (ev.i != i) break;
           const int lc = ev.j - j0;
           if (lane == lc / V) {
-            const float* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
-            const float* gy = gx + (int64_t)nf * I.n_rec;
+            const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+            const gfloat* gy = gx + (int64_t)nf * I.n_rec;
             const float vgx = *gx, vgy = *gy;
 #pragma unroll
             for (int e = 0; e < V; ++e) {
@@ -597,21 +737,15 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, co
     if constexpr (NACC > 0) {
 #pragma unroll
       for (int q = 0; q < NACC; ++q) {
-        float* a32 = prow(acc32 + q * I.field, pitch, i, j0) + lane * V;
+        gfloat* a32 = prow(acc32 + q * I.field, pitch, i, j0) + lane * V;
         float v[V];
-        if (seg_first) {
 #pragma unroll
-          for (int e = 0; e < V; ++e) v[e] = G[q][e];
-        } else {
-          load_vec<V>(a32, v);
-#pragma unroll
-          for (int e = 0; e < V; ++e) v[e] += G[q][e];
-        }
+        for (int e = 0; e < V; ++e) v[e] = acc_in[r][q][e] + G[q][e];
         if (seg_last) {
-          double* a64 = acc64 + q * I.field + (int64_t)(i + 1) * pitch + kLeftPad + j0 + lane * V;
+          gdouble* a64 = acc64 + q * I.field + (int64_t)(i + 1) * pitch + kLeftPad + j0 + lane * V;
 #pragma unroll
           for (int e = 0; e < V; ++e) {
-            if (j0 + lane * V + e < I.ny) a64[e] += I.gscale * (double)v[e];
+            if (j0 + lane * V + e < I.ny) a64[e] += gsa * (double)v[e];
           }
         } else {
           store_vec<V>(a32, v);
@@ -621,13 +755,15 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const CallDev& C, co
   }
 
   if constexpr (MODE != kField6) {
-    // per-tile fp64 partials of the six constant gradients (deterministic: one
-    // wavefront owns slot t in every launch)
+    // per-tile fp64 partials of the constant gradients (deterministic: one
+    // wavefront owns slot t in every launch); lane 8q holds slot q
+    double Sd[kPartials];
 #pragma unroll
-    for (int p = 0; p < 6; ++p) {
-      const double s = wave_sum((double)S[p]);
-      if (lane == 0) (reinterpret_cast<double*>(C.ws) + a.partial_off)[(int64_t)t * 6 + p] += I.gscale * s;
-    }
+    for (int p = 0; p < kPartials; ++p) Sd[p] = (double)S[p];
+    const double tot = wave_sum8(Sd, lane);
+    const int q = lane >> 3;
+    const double wq = (q == 2 || q >= 6) ? 0.25 * gsa : gsa;   // a/4 on the mixed taps
+    if ((lane & 7) == 0) *part = part_in + wq * tot;
   }
 }
 
@@ -636,7 +772,7 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
   const TileRange tr = tile_range(a.n_tiles);
   if (tr.t >= tr.end) return;
-  const CallDev C = *a.call;
+  const GCall C = global_call(*a.call);
   if constexpr (!LOOP) {
     const TileDev tl = a.tiles[tr.t];
     const InstDev I = a.inst[tl.inst];
@@ -670,7 +806,7 @@ __global__ void __launch_bounds__(kBlock) fused_step_kernel(FusedArgs fa) {
   const int lane = threadIdx.x & (kWave - 1);
   const TileRange tr = tile_range(2 * fa.adj.n_tiles);
   if (tr.t >= tr.end) return;
-  const CallDev C = *fa.adj.call;
+  const GCall C = global_call(*fa.adj.call);
   int cur = -1;
   InstDev I;
   for (int u = tr.t; u < tr.end; u += tr.step) {
@@ -729,10 +865,16 @@ __global__ void finalize_const_kernel(const CallDev* call, int64_t partial_off,
                                       const TileDev* __restrict__ tiles, int n_tiles) {
   const int set = blockIdx.x / 6, p = blockIdx.x % 6;
   const double* partial = reinterpret_cast<const double*>(call->ws) + partial_off;
+  // dJ/dC = (s0, s1, s2, s3 + 2 s6, s4 + 2 s7, s5 + s2)
+  const int p2 = (p == 3) ? 6 : (p == 4) ? 7 : (p == 5) ? 2 : -1;
+  const double f2 = (p == 5) ? 1.0 : 2.0;
   __shared__ double red[kBlock];
   double s = 0.0;
   for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
-    if (tiles[t].set == set) s += partial[(int64_t)t * 6 + p];
+    if (tiles[t].set == set) {
+      const double* pt = partial + (int64_t)t * kPartials;
+      s += (p2 >= 0) ? pt[p] + f2 * pt[p2] : pt[p];
+    }
   red[threadIdx.x] = s;
   __syncthreads();
   for (int off = kBlock / 2; off > 0; off >>= 1) {
@@ -775,8 +917,8 @@ __global__ void finalize_field_kernel(const CallDev* call, const InstDev* inst, 
 #define ERNN_TILE_CASES(X) \
   X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(1, 16) X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(4, 4)
 #define ERNN_TILE_CASES_DFIELD_ADJ(X) \
-  X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 4) X(4, 1) X(4, 2)
-#define ERNN_TILE_CASES_FIELD6_ADJ(X) X(1, 1) X(1, 2) X(1, 4) X(2, 1) X(2, 2) X(4, 1)
+  X(1, 1) X(1, 2) X(1, 4) X(2, 1) X(2, 2) X(2, 4) X(4, 1) X(4, 2)
+#define ERNN_TILE_CASES_FIELD6_ADJ(X) X(1, 1) X(1, 2) X(2, 1) X(2, 2) X(4, 1)
 #define ERNN_TILE_CASES_FIELD6_FWD(X) \
   X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 4) X(4, 1) X(4, 2)
 

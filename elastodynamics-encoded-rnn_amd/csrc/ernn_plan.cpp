@@ -79,8 +79,8 @@ bool tile_ok(int V, int R) {
 // six-field adjoint V * R <= 4.
 bool adj_tile_ok(int mode, int V, int R) {
   if (!tile_ok(V, R)) return false;
-  if (mode == ERNN_PARAM_DFIELD) return V * R <= 8;
-  if (mode == ERNN_PARAM_FIELD6) return V * R <= 4;
+  if (mode == ERNN_PARAM_DFIELD) return V * R <= 8 && R <= 4;
+  if (mode == ERNN_PARAM_FIELD6) return V * R <= 4 && R <= 2;
   return true;
 }
 // The six-field forward is built for V * R <= 8 (larger shapes would spill).
@@ -188,8 +188,8 @@ int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
   if (t == &p->fwd && !fwd_tile_ok(p->mode, V, R))
     return fail(ERNN_E_INVALID, "forward tile (V, R) with V*R > 8 is not built for six fields");
   if (t == &p->adj && !adj_tile_ok(p->mode, V, R))
-    return fail(ERNN_E_INVALID, "adjoint tile (V, R) not built for this mode (damage field: V*R <= 8, "
-                                "six fields: V*R <= 4)");
+    return fail(ERNN_E_INVALID, "adjoint tile (V, R) not built for this mode (damage field: V*R <= 8 "
+                                "and R <= 4, six fields: V*R <= 4 and R <= 2)");
   if (bpc < 0 || bpc > 64) return fail(ERNN_E_INVALID, "blocks per CU must be in [0, 64]");
   const int64_t nt64 = tiles_for(*p, V, R);
   if (nt64 > (int64_t)1 << 30) return fail(ERNN_E_CAPACITY, "too many tiles");
@@ -301,7 +301,7 @@ int validate(const ernn_desc_t* d) {
                                 "(six fields: <= 8)");
   if ((d->adj_v != 0 || d->adj_r != 0) && !adj_tile_ok(d->param_mode, d->adj_v, d->adj_r))
     return fail(ERNN_E_INVALID, "adjoint tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16 "
-                                "(damage field: <= 8, six fields: <= 4)");
+                                "(damage field: V*R <= 8 and R <= 4, six fields: V*R <= 4 and R <= 2)");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))
@@ -458,7 +458,7 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   }
   p->partial_cap = tiles_for(*p, 1, 1);
   p->partial_off = off / 2;
-  off += round_up(2 * 6 * p->partial_cap, 64);
+  off += round_up(2 * kPartials * p->partial_cap, 64);
   p->ws_floats = off;
 
   ERNN_TRY(dev_upload(&p->d_inst, p->inst));
@@ -470,7 +470,8 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   ERNN_TRY(dev_upload(&p->d_set_ext_off, p->set_ext_off));
   ERNN_TRY(dev_upload(&p->d_set_grad_off, p->set_grad_off));
   ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 1, d->fwd_r ? d->fwd_r : 4, d->fwd_bpc));
-  ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 1, d->adj_r ? d->adj_r : 4, d->adj_bpc));
+  const int adj_r0 = (d->param_mode == ERNN_PARAM_FIELD6) ? 2 : 4;
+  ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 1, d->adj_r ? d->adj_r : adj_r0, d->adj_bpc));
   ERNN_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
   return ERNN_OK;
 }
@@ -555,7 +556,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
     if (p->n_acc)
       ERNN_HIP(launch_fill_ws(p->d_call, 2 * D.acc64_off, 2 * (int64_t)p->n_acc * D.field, st));
   }
-  ERNN_HIP(launch_fill_ws(p->d_call, 2 * p->partial_off, 2 * 6 * (int64_t)p->adj.n_tiles, st));
+  ERNN_HIP(launch_fill_ws(p->d_call, 2 * p->partial_off, 2 * kPartials * (int64_t)p->adj.n_tiles, st));
 
   StepArgs f = base_args(p, p->fwd);
   f.record = 0;

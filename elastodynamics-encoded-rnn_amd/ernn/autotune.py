@@ -62,8 +62,9 @@ TILE_SHAPES: Tuple[Tuple[int, int], ...] = (
 GRID_CAPS: Tuple[int, ...] = (0, 2, 4, 8)
 DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
     (v, r, b) for (v, r) in TILE_SHAPES for b in GRID_CAPS)
-# largest V * R of the built adjoint kernels per mode (larger shapes would spill registers)
+# largest V * R and R of the built adjoint kernels per mode (larger shapes would spill registers)
 ADJ_MAX_CELLS = {"const6": 16, "dfield": 8, "field6": 4}
+ADJ_MAX_ROWS = {"const6": 16, "dfield": 4, "field6": 2}
 
 
 def tune_tiles(engine, params, field, wavelets, steps: int = 50,
@@ -86,7 +87,7 @@ def tune_tiles(engine, params, field, wavelets, steps: int = 50,
         mode = engine.survey.mode
         cands = [c for c in candidates
                  if (w == 0 and not (mode == "field6" and c[0] * c[1] > 8))
-                 or (w != 0 and c[0] * c[1] <= ADJ_MAX_CELLS[mode])]
+                 or (w != 0 and c[0] * c[1] <= ADJ_MAX_CELLS[mode] and c[1] <= ADJ_MAX_ROWS[mode])]
         kernel = 1 if w == 2 else w
         samples = {c: [] for c in cands}
         for _ in range(rounds):

@@ -83,8 +83,9 @@ typedef struct {
   int32_t snapshot;     /* 1: frames are whole fields (PI:304-305); 0: receiver traces */
   int32_t ckpt_every;   /* K >= 2: checkpoint interval of the adjoint recompute */
   int32_t fwd_v, fwd_r; /* forward tile: V in {1,2,4} columns per lane, R in {1,2,4,8,16}
-                           rows, V * R <= 16; 0, 0 = default (1, 4) */
-  int32_t adj_v, adj_r; /* adjoint tile */
+                           rows, V * R <= 16 (six fields: V * R <= 8); 0, 0 = default (1, 4) */
+  int32_t adj_v, adj_r; /* adjoint tile: as above, damage field V * R <= 8 and R <= 4, six
+                           fields V * R <= 4 and R <= 2; 0, 0 = default (1, 4), six fields (1, 2) */
   int32_t use_graph;    /* 1: replay each pass as a captured hipGraph */
   int32_t fwd_bpc;      /* forward grid cap in blocks per CU (0 = one tile per wavefront);
                            below the tile count the wavefronts loop over tiles */

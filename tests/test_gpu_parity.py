@@ -100,7 +100,7 @@ def test_dfield_forward_and_gradients_match_oracle():
     D = rng.uniform(0.4, 1.0, (nx, ny)).astype(np.float32)
     shots = [make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=_receivers(nx, ny)),
              make_shot(nx, ny, nt, h, sources=ernn.point_source(3, ny - 1, h), receivers=_receivers(nx, ny))]
-    sv = ernn.Survey(shots, mode="dfield", ckpt_every=5, tiles=(2, 4, 1, 8, 1, 1))
+    sv = ernn.Survey(shots, mode="dfield", ckpt_every=5, tiles=(2, 4, 1, 4, 1, 1))
     Gs = [rng.standard_normal((2, nt, len(s.receivers))) for s in shots]
     C32 = F32(C_ANISO)
     cells = [np.float32(D) * np.float32(c) for c in C_ANISO]     # fl32(D * C0) like the kernel
@@ -124,7 +124,7 @@ def test_field6_forward_and_gradients_match_oracle():
     shot = make_shot(nx, ny, nt, h, sources=_sources(nx, ny, h), receivers=_receivers(nx, ny))
     G = rng.standard_normal((2, nt, len(shot.receivers)))
     ref, gcell = oracle_run(shot, [f.astype(np.float64) for f in fields], 1, False, G)
-    for tiles in [(1, 2, 1, 2), (2, 4, 2, 2), (4, 2, 4, 1, 1, 1), (1, 1, 2, 1, 2, 2), (1, 8, 1, 4)]:
+    for tiles in [(1, 2, 1, 2), (2, 4, 2, 2), (4, 2, 4, 1, 1, 1), (1, 1, 2, 1, 2, 2), (1, 8, 1, 2)]:
         sv = ernn.Survey([shot], mode="field6", ckpt_every=6, tiles=tiles)
         eng, tr, (gp, gf) = run_hip(sv, field=fields, G=G)
         assert gp is None

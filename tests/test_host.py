@@ -73,6 +73,8 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
     ({}, {"adj_bpc": -1}, "blocks per CU"),
     ({}, {"param_mode": 2, "adj_v": 2, "adj_r": 4}, "adjoint tile"),
     ({}, {"param_mode": 2, "adj_v": 1, "adj_r": 8}, "adjoint tile"),
+    ({}, {"param_mode": 2, "adj_v": 1, "adj_r": 4}, "adjoint tile"),
+    ({}, {"param_mode": 1, "adj_v": 1, "adj_r": 8}, "adjoint tile"),
     ({}, {"param_mode": 1, "adj_v": 4, "adj_r": 4}, "adjoint tile"),
 ])
 def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
