@@ -92,25 +92,40 @@ __device__ __forceinline__ void store_vec(gfloat* __restrict__ p, const float (&
   }
 }
 
-// rowp points at column j0 of one padded row.  The two edge lanes load the
-// tile's halo columns; every other column neighbour comes from the adjacent
-// lane by a cross-lane shuffle.  (DPP whole-wavefront shifts were measured
-// slower than ds_bpermute here: 147 vs 122 us on the 1024^2 damage adjoint.)
+// A row is read in two phases so that a tile can issue every load before
+// the first cross-lane shuffle waits for data.  load_raw: rowp points at
+// column j0 of one padded row; the lane loads its V columns and one more
+// value that becomes the halo on the two edge lanes (column j0 - 1 on lane 0,
+// j0 + 64 V on lane 63).  The inner lanes re-read their own first column, so
+// the load needs no branch; a branch there would keep the compiler from
+// hoisting the next rows' loads above this row's shuffles.
+// make_row: the column neighbours come from the adjacent lanes by cross-lane
+// shuffles.  (DPP whole-wavefront shifts were measured slower than
+// ds_bpermute here: 147 vs 122 us on the 1024^2 damage adjoint.)
 template <int V>
-__device__ __forceinline__ Row<V> load_row(const gfloat* __restrict__ rowp, int lane) {
-  Row<V> r;
+struct RawRow {
   float c[V];
-  load_vec<V>(rowp + lane * V, c);
-  const bool edge = (lane == 0) | (lane == kWave - 1);
-  const int hoff = (lane == 0) ? -1 : kWave * V;
-  float halo = 0.f;
-  if (edge) halo = rowp[hoff];
-  const float left = __shfl_up(c[V - 1], 1, kWave);
-  const float right = __shfl_down(c[0], 1, kWave);
-  r.v[0] = (lane == 0) ? halo : left;
+  float h;
+};
+
+template <int V>
+__device__ __forceinline__ RawRow<V> load_raw(const gfloat* __restrict__ rowp, int lane) {
+  RawRow<V> r;
+  load_vec<V>(rowp + lane * V, r.c);
+  const int hoff = lane * V - (int)(lane == 0) + V * (int)(lane == kWave - 1);
+  r.h = rowp[hoff];
+  return r;
+}
+
+template <int V>
+__device__ __forceinline__ Row<V> make_row(const RawRow<V>& raw, int lane) {
+  Row<V> r;
+  const float left = __shfl_up(raw.c[V - 1], 1, kWave);
+  const float right = __shfl_down(raw.c[0], 1, kWave);
+  r.v[0] = (lane == 0) ? raw.h : left;
 #pragma unroll
-  for (int e = 0; e < V; ++e) r.v[e + 1] = c[e];
-  r.v[V + 1] = (lane == kWave - 1) ? halo : right;
+  for (int e = 0; e < V; ++e) r.v[e + 1] = raw.c[e];
+  r.v[V + 1] = (lane == kWave - 1) ? raw.h : right;
   return r;
 }
 
@@ -377,13 +392,13 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
   const int nf = I.n_frames;
 
   constexpr int NC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
-  Row<V> xr[R + 2], yr[R + 2];
+  RawRow<V> xraw[R + 2], yraw[R + 2];
   float pvx[R][V], pvy[R][V];
   float cf[R][NC > 0 ? NC : 1][V];
 #pragma unroll
   for (int r = 0; r < R + 2; ++r) {
-    xr[r] = load_row<V>(prow(u0x, pitch, tl.i0 - 1 + r, j0), lane);
-    yr[r] = load_row<V>(prow(u0y, pitch, tl.i0 - 1 + r, j0), lane);
+    xraw[r] = load_raw<V>(prow(u0x, pitch, tl.i0 - 1 + r, j0), lane);
+    yraw[r] = load_raw<V>(prow(u0y, pitch, tl.i0 - 1 + r, j0), lane);
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -392,13 +407,22 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
     for (int q = 0; q < NC; ++q) load_vec<V>(prow(coef + q * I.field, pitch, tl.i0 + r, j0) + lane * V, cf[r][q]);
   }
+  Row<V> xr[R + 2], yr[R + 2];
+#pragma unroll
+  for (int r = 0; r < R + 2; ++r) {
+    xr[r] = make_row<V>(xraw[r], lane);
+    yr[r] = make_row<V>(yraw[r], lane);
+  }
 
+  // Rows past the grid (i >= nx) are computed like the others and stored as
+  // zeros, which keeps those ghost rows zero.  A branch around them would let
+  // the compiler sink the tile's later loads below the first row's shuffles.
   int sp = tl.src_begin;
   int rp = tl.rec_begin;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = tl.i0 + r;
-    if (i >= I.nx) break;
+    const bool live = i < I.nx;
     float lx[V], ly[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
@@ -440,14 +464,14 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
       const float ay = ly[e] / I.rho;
       float vx = (2.f * xr[r + 1].v[e + 1] - pvx[r][e]) + I.dt2 * ax;
       float vy = (2.f * yr[r + 1].v[e + 1] - pvy[r][e]) + I.dt2 * ay;
-      if (j0 + lane * V + e >= I.ny) { vx = 0.f; vy = 0.f; }  // keep the right ghosts zero
+      if (!live || j0 + lane * V + e >= I.ny) { vx = 0.f; vy = 0.f; }  // keep the ghosts zero
       nvx[e] = vx;
       nvy[e] = vy;
     }
     store_vec<V>(prow(unx, pitch, i, j0) + lane * V, nvx);
     store_vec<V>(prow(uny, pitch, i, j0) + lane * V, nvy);
 
-    if (rec_step) {
+    if (rec_step && live) {
       if (a.snapshot) {
         gfloat* tx = C.traces + I.trace_off + ((int64_t)k * I.nx + i) * I.ny;
         gfloat* ty = tx + (int64_t)nf * I.nx * I.ny;
@@ -519,26 +543,45 @@ struct AdjRows {
   float cx[V], cy[V];  // raw lam centres
 };
 
+// Raw loads of one adjoint stencil row: lam^{m+1} and the coefficient rows
+// the weighted fields need (kDField: D, kField6: the six fields).
 template <int MODE, int V>
-__device__ __forceinline__ AdjRows<MODE, V> load_adj_row(const gfloat* l1x, const gfloat* l1y,
-                                                          const gfloat* coef, int64_t field, int64_t pitch,
-                                                          int r, int j0, int lane) {
-  AdjRows<MODE, V> o;
-  const Row<V> lx = load_row<V>(prow(l1x, pitch, r, j0), lane);
-  const Row<V> ly = load_row<V>(prow(l1y, pitch, r, j0), lane);
+struct AdjRaw {
+  static constexpr int NCR = (MODE == kField6) ? 6 : (MODE == kDField ? 1 : 0);
+  RawRow<V> lx, ly;
+  RawRow<V> cr[NCR > 0 ? NCR : 1];
+};
+
+template <int MODE, int V>
+__device__ __forceinline__ AdjRaw<MODE, V> load_adj_raw(const gfloat* l1x, const gfloat* l1y,
+                                                         const gfloat* coef, int64_t field, int64_t pitch,
+                                                         int r, int j0, int lane) {
+  AdjRaw<MODE, V> o;
+  o.lx = load_raw<V>(prow(l1x, pitch, r, j0), lane);
+  o.ly = load_raw<V>(prow(l1y, pitch, r, j0), lane);
 #pragma unroll
-  for (int e = 0; e < V; ++e) { o.cx[e] = lx.v[e + 1]; o.cy[e] = ly.v[e + 1]; }
+  for (int q = 0; q < AdjRaw<MODE, V>::NCR; ++q) o.cr[q] = load_raw<V>(prow(coef + q * field, pitch, r, j0), lane);
+  return o;
+}
+
+template <int MODE, int V>
+__device__ __forceinline__ AdjRows<MODE, V> make_adj_row(const AdjRaw<MODE, V>& raw, int lane) {
+  AdjRows<MODE, V> o;
+  const Row<V> lx = make_row<V>(raw.lx, lane);
+  const Row<V> ly = make_row<V>(raw.ly, lane);
+#pragma unroll
+  for (int e = 0; e < V; ++e) { o.cx[e] = raw.lx.c[e]; o.cy[e] = raw.ly.c[e]; }
   if constexpr (MODE == kConst6) {
     o.w[0] = lx;
     o.w[1] = ly;
   } else if constexpr (MODE == kDField) {
-    const Row<V> D = load_row<V>(prow(coef, pitch, r, j0), lane);
+    const Row<V> D = make_row<V>(raw.cr[0], lane);
     o.w[0] = mul_row<V>(D, lx);
     o.w[1] = mul_row<V>(D, ly);
   } else {
     Row<V> cr[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) cr[q] = load_row<V>(prow(coef + q * field, pitch, r, j0), lane);
+    for (int q = 0; q < 6; ++q) cr[q] = make_row<V>(raw.cr[q], lane);
 #pragma unroll
     for (int e = 0; e < V + 2; ++e) {
       const float c11 = cr[0].v[e], c22 = cr[1].v[e], c12 = cr[2].v[e];
@@ -603,11 +646,11 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   const double gsa = I.gscale * (double)I.hinv2;   // per-cell sums carry 1 / a
   gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
 
-  // ---- loads
-  AdjRows<MODE, V> lr[R + 2];
+  // ---- loads (all issued before the first shuffle)
+  AdjRaw<MODE, V> lraw[R + 2];
 #pragma unroll
   for (int r = 0; r < R + 2; ++r)
-    lr[r] = load_adj_row<MODE, V>(l1x, l1y, coef, I.field, pitch, tl.i0 - 1 + r, j0, lane);
+    lraw[r] = load_adj_raw<MODE, V>(l1x, l1y, coef, I.field, pitch, tl.i0 - 1 + r, j0, lane);
   float l2vx[R][V], l2vy[R][V];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -616,7 +659,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   }
   constexpr int RU = (MODE == kConst6) ? 1 : R + 2;   // u^m rows with halo (field modes)
   float uvx[R][V], uvy[R][V];
-  Row<V> um[RU][2];
+  RawRow<V> uraw[RU][2];
   if constexpr (MODE == kConst6) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -626,8 +669,8 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   } else {
 #pragma unroll
     for (int r = 0; r < RU; ++r) {
-      um[r][0] = load_row<V>(prow(umx, pitch, tl.i0 - 1 + r, j0), lane);
-      um[r][1] = load_row<V>(prow(umy, pitch, tl.i0 - 1 + r, j0), lane);
+      uraw[r][0] = load_raw<V>(prow(umx, pitch, tl.i0 - 1 + r, j0), lane);
+      uraw[r][1] = load_raw<V>(prow(umy, pitch, tl.i0 - 1 + r, j0), lane);
     }
   }
 
@@ -651,15 +694,30 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   double part_in = 0.0;
   if constexpr (MODE != kField6) part_in = *part;
 
+  // ---- shuffles
+  AdjRows<MODE, V> lr[R + 2];
+#pragma unroll
+  for (int r = 0; r < R + 2; ++r) lr[r] = make_adj_row<MODE, V>(lraw[r], lane);
+  Row<V> um[RU][2];
+  if constexpr (MODE != kConst6) {
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      um[r][0] = make_row<V>(uraw[r][0], lane);
+      um[r][1] = make_row<V>(uraw[r][1], lane);
+    }
+  }
+
   float S[kPartials];
 #pragma unroll
   for (int p = 0; p < kPartials; ++p) S[p] = 0.f;
 
+  // Rows past the grid are computed, contribute nothing and are stored as
+  // zeros (see fwd_tile).
   int rp = tl.rec_begin;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = tl.i0 + r;
-    if (i >= I.nx) break;
+    const bool live = i < I.nx;
     const AdjRows<MODE, V>& Lm = lr[r];
     const AdjRows<MODE, V>& Lc = lr[r + 1];
     const AdjRows<MODE, V>& Lp = lr[r + 2];
@@ -677,6 +735,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
         float ux, uy;
         if constexpr (MODE == kConst6) { ux = uvx[r][e]; uy = uvy[r][e]; }
         else { ux = um[r + 1][0].v[x]; uy = um[r + 1][1].v[x]; }
+        if (!live) { ux = 0.f; uy = 0.f; }
         sbp_sums(dl, ux, uy, S);
         if constexpr (MODE == kDField) {
           // dJ/dD(cell) = lam . A(C0) u^m at the cell (divided by a)
@@ -707,7 +766,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           const int j = j0 + lane * V + e;
-          if (j < I.ny) { nvx[e] += gx[j]; nvy[e] += gy[j]; }
+          if (live && j < I.ny) { nvx[e] += gx[j]; nvy[e] += gy[j]; }
         }
       } else {
         while (rp < tl.rec_end) {
@@ -729,7 +788,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
     }
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      if (j0 + lane * V + e >= I.ny) { nvx[e] = 0.f; nvy[e] = 0.f; }
+      if (!live || j0 + lane * V + e >= I.ny) { nvx[e] = 0.f; nvy[e] = 0.f; }
     }
     store_vec<V>(prow(lnx, pitch, i, j0) + lane * V, nvx);
     store_vec<V>(prow(lny, pitch, i, j0) + lane * V, nvy);
@@ -740,12 +799,12 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
         gfloat* a32 = prow(acc32 + q * I.field, pitch, i, j0) + lane * V;
         float v[V];
 #pragma unroll
-        for (int e = 0; e < V; ++e) v[e] = acc_in[r][q][e] + G[q][e];
+        for (int e = 0; e < V; ++e) v[e] = live ? acc_in[r][q][e] + G[q][e] : 0.f;
         if (seg_last) {
           gdouble* a64 = acc64 + q * I.field + (int64_t)(i + 1) * pitch + kLeftPad + j0 + lane * V;
 #pragma unroll
           for (int e = 0; e < V; ++e) {
-            if (j0 + lane * V + e < I.ny) a64[e] += gsa * (double)v[e];
+            if (live && j0 + lane * V + e < I.ny) a64[e] += gsa * (double)v[e];
           }
         } else {
           store_vec<V>(a32, v);
