@@ -41,6 +41,20 @@ typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) f32x2 gfloat2;
 typedef __attribute__((address_space(1))) f32x4 gfloat4;
 typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) char gbyte;
+
+// base + off elements, with the byte offset formed in 32 bits (padded fields
+// hold fewer than 2^30 elements): the load or store then takes the 64-bit
+// base in scalar registers and a 32-bit offset register that every field of
+// the tile shares.
+template <typename T>
+__device__ __forceinline__ T* at(T* base, uint32_t off) {
+  return reinterpret_cast<T*>(reinterpret_cast<gbyte*>(base) + (off * (uint32_t)sizeof(float)));
+}
+template <typename T>
+__device__ __forceinline__ const T* at(const T* base, uint32_t off) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const gbyte*>(base) + (off * (uint32_t)sizeof(float)));
+}
 
 struct GCall {
   gfloat* ws;
@@ -93,7 +107,7 @@ __device__ __forceinline__ void store_vec(gfloat* __restrict__ p, const float (&
 }
 
 // A row is read in two phases so that a tile can issue every load before
-// the first cross-lane shuffle waits for data.  load_raw: rowp points at
+// the first cross-lane shuffle waits for data.  load_raw: ro is the offset of
 // column j0 of one padded row; the lane loads its V columns and one more
 // value that becomes the halo on the two edge lanes (column j0 - 1 on lane 0,
 // j0 + 64 V on lane 63).  The inner lanes re-read their own first column, so
@@ -109,11 +123,11 @@ struct RawRow {
 };
 
 template <int V>
-__device__ __forceinline__ RawRow<V> load_raw(const gfloat* __restrict__ rowp, int lane) {
+__device__ __forceinline__ RawRow<V> load_raw(const gfloat* __restrict__ base, uint32_t ro, uint32_t lo,
+                                              uint32_t ho) {
   RawRow<V> r;
-  load_vec<V>(rowp + lane * V, r.c);
-  const int hoff = lane * V - (int)(lane == 0) + V * (int)(lane == kWave - 1);
-  r.h = rowp[hoff];
+  load_vec<V>(at(base, ro + lo), r.c);
+  r.h = *at(base, ro + ho);
   return r;
 }
 
@@ -347,11 +361,22 @@ __device__ __forceinline__ TileRange tile_range(int n_tiles) {
   return r;
 }
 
-__device__ __forceinline__ const gfloat* prow(const gfloat* base, int64_t pitch, int r, int j0) {
-  return base + (int64_t)(r + 1) * pitch + kLeftPad + j0;
+// Element offset of column j0 of grid row r in a padded field.  Offsets are
+// 32-bit (the plan keeps every padded field below 2^30 elements), so a tile
+// addresses each field by one 64-bit base in scalar registers plus one 32-bit
+// offset per row that all fields share.
+__device__ __forceinline__ uint32_t row_off(uint32_t pitch, int r, int j0) {
+  return (uint32_t)(r + 1) * pitch + (uint32_t)(kLeftPad + j0);
 }
-__device__ __forceinline__ gfloat* prow(gfloat* base, int64_t pitch, int r, int j0) {
-  return base + (int64_t)(r + 1) * pitch + kLeftPad + j0;
+
+// Per-lane column offsets: lo = the lane's first column, ho = the value
+// load_raw fetches besides (column -1 on lane 0, 64 V on lane 63, else the
+// lane's own first column; -1 wraps, and row offsets are >= kLeftPad).
+template <int V>
+__device__ __forceinline__ uint32_t lane_off(int lane) { return (uint32_t)(lane * V); }
+template <int V>
+__device__ __forceinline__ uint32_t halo_off(int lane) {
+  return (uint32_t)(lane * V - (int)(lane == 0) + V * (int)(lane == kWave - 1));
 }
 
 // --------------------------------------------------------------------------
@@ -368,7 +393,8 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
                                          const InstDev& I, int lane) {
   const int n = a.step;
 
-  const int64_t pitch = I.pitch;
+  const uint32_t P = (uint32_t)I.pitch;
+  const uint32_t lo = lane_off<V>(lane), ho = halo_off<V>(lane);
   const int j0 = tl.j0;
   gfloat* pool = C.ws + I.pool_off;
   const gfloat* u0x = pool + (int64_t)a.s_cur * 2 * I.field;
@@ -397,15 +423,17 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
   float cf[R][NC > 0 ? NC : 1][V];
 #pragma unroll
   for (int r = 0; r < R + 2; ++r) {
-    xraw[r] = load_raw<V>(prow(u0x, pitch, tl.i0 - 1 + r, j0), lane);
-    yraw[r] = load_raw<V>(prow(u0y, pitch, tl.i0 - 1 + r, j0), lane);
+    const uint32_t ro = row_off(P, tl.i0 - 1 + r, j0);
+    xraw[r] = load_raw<V>(u0x, ro, lo, ho);
+    yraw[r] = load_raw<V>(u0y, ro, lo, ho);
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    load_vec<V>(prow(u1x, pitch, tl.i0 + r, j0) + lane * V, pvx[r]);
-    load_vec<V>(prow(u1y, pitch, tl.i0 + r, j0) + lane * V, pvy[r]);
+    const uint32_t o = row_off(P, tl.i0 + r, j0) + lo;
+    load_vec<V>(at(u1x, o), pvx[r]);
+    load_vec<V>(at(u1y, o), pvy[r]);
 #pragma unroll
-    for (int q = 0; q < NC; ++q) load_vec<V>(prow(coef + q * I.field, pitch, tl.i0 + r, j0) + lane * V, cf[r][q]);
+    for (int q = 0; q < NC; ++q) load_vec<V>(at(coef + q * I.field, o), cf[r][q]);
   }
   Row<V> xr[R + 2], yr[R + 2];
 #pragma unroll
@@ -468,8 +496,9 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
       nvx[e] = vx;
       nvy[e] = vy;
     }
-    store_vec<V>(prow(unx, pitch, i, j0) + lane * V, nvx);
-    store_vec<V>(prow(uny, pitch, i, j0) + lane * V, nvy);
+    const uint32_t so = row_off(P, i, j0) + lo;
+    store_vec<V>(at(unx, so), nvx);
+    store_vec<V>(at(uny, so), nvy);
 
     if (rec_step && live) {
       if (a.snapshot) {
@@ -554,13 +583,13 @@ struct AdjRaw {
 
 template <int MODE, int V>
 __device__ __forceinline__ AdjRaw<MODE, V> load_adj_raw(const gfloat* l1x, const gfloat* l1y,
-                                                         const gfloat* coef, int64_t field, int64_t pitch,
-                                                         int r, int j0, int lane) {
+                                                         const gfloat* coef, int64_t field, uint32_t ro,
+                                                         uint32_t lo, uint32_t ho) {
   AdjRaw<MODE, V> o;
-  o.lx = load_raw<V>(prow(l1x, pitch, r, j0), lane);
-  o.ly = load_raw<V>(prow(l1y, pitch, r, j0), lane);
+  o.lx = load_raw<V>(l1x, ro, lo, ho);
+  o.ly = load_raw<V>(l1y, ro, lo, ho);
 #pragma unroll
-  for (int q = 0; q < AdjRaw<MODE, V>::NCR; ++q) o.cr[q] = load_raw<V>(prow(coef + q * field, pitch, r, j0), lane);
+  for (int q = 0; q < AdjRaw<MODE, V>::NCR; ++q) o.cr[q] = load_raw<V>(coef + q * field, ro, lo, ho);
   return o;
 }
 
@@ -607,7 +636,8 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
 #pragma clang fp contract(fast)
   const int m = a.step;
 
-  const int64_t pitch = I.pitch;
+  const uint32_t P = (uint32_t)I.pitch;
+  const uint32_t lo = lane_off<V>(lane), ho = halo_off<V>(lane);
   const int j0 = tl.j0;
   gfloat* pool = C.ws + I.pool_off;
   const gfloat* l1x = pool + (int64_t)a.s_cur * 2 * I.field;   // lam^{m+1}
@@ -650,12 +680,13 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   AdjRaw<MODE, V> lraw[R + 2];
 #pragma unroll
   for (int r = 0; r < R + 2; ++r)
-    lraw[r] = load_adj_raw<MODE, V>(l1x, l1y, coef, I.field, pitch, tl.i0 - 1 + r, j0, lane);
+    lraw[r] = load_adj_raw<MODE, V>(l1x, l1y, coef, I.field, row_off(P, tl.i0 - 1 + r, j0), lo, ho);
   float l2vx[R][V], l2vy[R][V];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    load_vec<V>(prow(l2x, pitch, tl.i0 + r, j0) + lane * V, l2vx[r]);
-    load_vec<V>(prow(l2y, pitch, tl.i0 + r, j0) + lane * V, l2vy[r]);
+    const uint32_t o = row_off(P, tl.i0 + r, j0) + lo;
+    load_vec<V>(at(l2x, o), l2vx[r]);
+    load_vec<V>(at(l2y, o), l2vy[r]);
   }
   constexpr int RU = (MODE == kConst6) ? 1 : R + 2;   // u^m rows with halo (field modes)
   float uvx[R][V], uvy[R][V];
@@ -663,14 +694,16 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   if constexpr (MODE == kConst6) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      load_vec<V>(prow(umx, pitch, tl.i0 + r, j0) + lane * V, uvx[r]);
-      load_vec<V>(prow(umy, pitch, tl.i0 + r, j0) + lane * V, uvy[r]);
+      const uint32_t o = row_off(P, tl.i0 + r, j0) + lo;
+      load_vec<V>(at(umx, o), uvx[r]);
+      load_vec<V>(at(umy, o), uvy[r]);
     }
   } else {
 #pragma unroll
     for (int r = 0; r < RU; ++r) {
-      uraw[r][0] = load_raw<V>(prow(umx, pitch, tl.i0 - 1 + r, j0), lane);
-      uraw[r][1] = load_raw<V>(prow(umy, pitch, tl.i0 - 1 + r, j0), lane);
+      const uint32_t ro = row_off(P, tl.i0 - 1 + r, j0);
+      uraw[r][0] = load_raw<V>(umx, ro, lo, ho);
+      uraw[r][1] = load_raw<V>(umy, ro, lo, ho);
     }
   }
 
@@ -686,7 +719,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
           for (int e = 0; e < V; ++e) acc_in[r][q][e] = 0.f;
         } else {
-          load_vec<V>(prow(acc32 + q * I.field, pitch, tl.i0 + r, j0) + lane * V, acc_in[r][q]);
+          load_vec<V>(at(acc32 + q * I.field, row_off(P, tl.i0 + r, j0) + lo), acc_in[r][q]);
         }
       }
   }
@@ -790,18 +823,19 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
     for (int e = 0; e < V; ++e) {
       if (!live || j0 + lane * V + e >= I.ny) { nvx[e] = 0.f; nvy[e] = 0.f; }
     }
-    store_vec<V>(prow(lnx, pitch, i, j0) + lane * V, nvx);
-    store_vec<V>(prow(lny, pitch, i, j0) + lane * V, nvy);
+    const uint32_t so = row_off(P, i, j0) + lo;
+    store_vec<V>(at(lnx, so), nvx);
+    store_vec<V>(at(lny, so), nvy);
 
     if constexpr (NACC > 0) {
 #pragma unroll
       for (int q = 0; q < NACC; ++q) {
-        gfloat* a32 = prow(acc32 + q * I.field, pitch, i, j0) + lane * V;
+        gfloat* a32 = at(acc32 + q * I.field, so);
         float v[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) v[e] = live ? acc_in[r][q][e] + G[q][e] : 0.f;
         if (seg_last) {
-          gdouble* a64 = acc64 + q * I.field + (int64_t)(i + 1) * pitch + kLeftPad + j0 + lane * V;
+          gdouble* a64 = acc64 + q * I.field + so;
 #pragma unroll
           for (int e = 0; e < V; ++e) {
             if (live && j0 + lane * V + e < I.ny) a64[e] += gsa * (double)v[e];

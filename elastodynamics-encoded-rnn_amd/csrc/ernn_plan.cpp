@@ -75,8 +75,8 @@ bool tile_ok(int V, int R) {
   return V * R <= 16;
 }
 // Adjoint shapes that would spill registers are not built (ERNN_TILE_CASES_*
-// in ernn_kernels.hip): the damage-field adjoint needs V * R <= 8, the
-// six-field adjoint V * R <= 4.
+// in ernn_kernels.hip): the damage-field adjoint needs V * R <= 8 and
+// R <= 4, the six-field adjoint V * R <= 4 and R <= 2.
 bool adj_tile_ok(int mode, int V, int R) {
   if (!tile_ok(V, R)) return false;
   if (mode == ERNN_PARAM_DFIELD) return V * R <= 8 && R <= 4;
@@ -313,6 +313,12 @@ int validate(const ernn_desc_t* d) {
     if (I.nx < 1 || I.ny < 1 || I.nx > 65536 || I.ny > 65536) {
       snprintf(buf, sizeof buf, "instance %d: grid %dx%d outside [1, 65536]", b, I.nx, I.ny);
       return fail(ERNN_E_INVALID, buf);
+    }
+    // the kernels address a padded field with 32-bit element offsets
+    if ((round_up(I.nx, kRowAlign) + 3) * (kLeftPad + round_up(I.ny, kColAlign)) >= ((int64_t)1 << 30)) {
+      snprintf(buf, sizeof buf, "instance %d: grid %dx%d too large (a padded field must hold < 2^30 cells)", b,
+               I.nx, I.ny);
+      return fail(ERNN_E_CAPACITY, buf);
     }
     if (I.nt < 1 || I.nt > 100000000) return fail(ERNN_E_INVALID, "nt outside [1, 1e8]");
     if (I.set < 0 || I.set >= d->n_sets) return fail(ERNN_E_INVALID, "instance set out of range");
