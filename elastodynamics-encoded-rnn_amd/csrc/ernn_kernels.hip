@@ -11,8 +11,8 @@
 //
 // Work decomposition: one wavefront owns a tile of R rows x 64*V columns and
 // computes it from registers.  Lane l holds V consecutive columns; the column
-// neighbours come from the adjacent lanes by cross-lane shuffles, and the two
-// edge lanes fetch the tile's halo columns with one masked load.  Ghost cells are real zeros
+// neighbours come from the adjacent lanes by DPP wavefront shifts, and the two
+// edge lanes load the tile's halo columns.  Ghost cells are real zeros
 // in the padded layout (ernn_internal.h), so the sweep has no boundary
 // branches.  No LDS and no block barriers: the four wavefronts of a block
 // work on independent tiles.  Block b runs on XCD b % 8; the tile order maps
@@ -113,9 +113,9 @@ __device__ __forceinline__ void store_vec(gfloat* __restrict__ p, const float (&
 // j0 + 64 V on lane 63).  The inner lanes re-read their own first column, so
 // the load needs no branch; a branch there would keep the compiler from
 // hoisting the next rows' loads above this row's shuffles.
-// make_row: the column neighbours come from the adjacent lanes by cross-lane
-// shuffles.  (DPP whole-wavefront shifts were measured slower than
-// ds_bpermute here: 147 vs 122 us on the 1024^2 damage adjoint.)
+// make_row: the column neighbours come from the adjacent lanes by DPP
+// whole-wavefront shifts (one VALU move each, no LDS round trip; measured
+// 76 vs 78 us per 1024^2 damage-adjoint step against ds_bpermute shuffles).
 template <int V>
 struct RawRow {
   float c[V];
@@ -134,12 +134,14 @@ __device__ __forceinline__ RawRow<V> load_raw(const gfloat* __restrict__ base, u
 template <int V>
 __device__ __forceinline__ Row<V> make_row(const RawRow<V>& raw, int lane) {
   Row<V> r;
-  const float left = __shfl_up(raw.c[V - 1], 1, kWave);
-  const float right = __shfl_down(raw.c[0], 1, kWave);
-  r.v[0] = (lane == 0) ? raw.h : left;
+  // wave_shr:1 / wave_shl:1 (DPP controls 0x138 / 0x130): lane l receives
+  // lane l -+ 1's value; the lane shifted in from outside the wavefront keeps
+  // `old`, here the halo that lane 0 (left) and lane 63 (right) loaded.
+  const int h = __float_as_int(raw.h);
+  r.v[0] = __int_as_float(__builtin_amdgcn_update_dpp(h, __float_as_int(raw.c[V - 1]), 0x138, 0xf, 0xf, false));
 #pragma unroll
   for (int e = 0; e < V; ++e) r.v[e + 1] = raw.c[e];
-  r.v[V + 1] = (lane == kWave - 1) ? raw.h : right;
+  r.v[V + 1] = __int_as_float(__builtin_amdgcn_update_dpp(h, __float_as_int(raw.c[0]), 0x130, 0xf, 0xf, false));
   return r;
 }
 
