@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--graph", type=int, default=1, help="replay passes as captured hipGraphs")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--tiles", default="", help="fixed tiles 'Vf,Rf,Va,Ra,Bf,Ba' (skips autotuning)")
+    ap.add_argument("--blocking", default="auto",
+                    help="time-blocked forward 'V,RW,KS', '0' for one step per launch, or 'auto' (autotuned)")
     ap.add_argument("--baseline", type=int, default=1, help="time the autograd plain path on the GPU")
     ap.add_argument("--cpu", type=int, default=1, help="time the CPU port")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -239,7 +241,7 @@ def main():
     import torch
     import ernn
     from ernn import configs
-    from ernn.autotune import tune_tiles
+    from ernn.autotune import tune_blocking, tune_tiles
 
     dev = torch.device("cuda:0")
     kw = {"nt": args.nt} if args.nt else {}
@@ -270,6 +272,14 @@ def main():
                           which=(0, 2) if sv.fused else (0, 1))
         log("autotune: " + ", ".join(f"{('fwd', 'adj', 'fused')[w]} V={v} R={r} B={b} {ms * 1e3:.1f} us/step"
                                      for w, (v, r, b, ms, _) in tune.items()))
+    if args.blocking not in ("auto", "0"):
+        eng.set_blocking(*(int(x) for x in args.blocking.split(",")))
+        log(f"time-blocked forward fixed: (V, RW, KS) = {eng.blocking}")
+    elif args.blocking == "auto" and args.autotune:
+        bv, brw, bks, bms, btimes = tune_blocking(eng, params_true, field_true, model.wavelets,
+                                                  steps=max(16, min(96, 40000 // cells_k)))
+        log(f"autotune: forward blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
+            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f})")
     with torch.no_grad():
         obs = ernn.ernn_traces(eng, params=params_true, field=field_true, wavelets=model.wavelets)
         gen = torch.Generator(device="cpu").manual_seed(1)
@@ -310,15 +320,27 @@ def main():
     fwd_ms = eng.time_steps(0, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
     adj_ms = eng.time_steps(1, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
     fused_ms = eng.time_steps(2, nk, p, f, model.wavelets, ws, tr0, tr0) / nk if sv.fused else None
+    KS = eng.blocking[2]
+    blk_ms = None
+    if KS > 1:   # per time-blocked launch of KS steps
+        L = max(2, nk // KS)
+        eng.time_steps(3, 2, p, f, model.wavelets, ws, tr0, tr0)
+        blk_ms = eng.time_steps(3, L, p, f, model.wavelets, ws, tr0, tr0) / L
     eng.release_workspace(ws)
     cells = sum(s.nx * s.ny for s in sv.shots)
     bf, ba = BYTES_PER_CELL[(sv.mode, 0)], BYTES_PER_CELL[(sv.mode, 1)]
-    if sv.fused:   # forward pass: nt fwd launches; backward: ~nt fused launches (+ one segment alone)
-        kern = {"fwd_step_kernel": (fwd_ms, sv.nt_max + int(eng.info.ckpt_every), bf),
-                "fused_step_kernel": (fused_ms, sv.nt_max, bf + ba)}
+    nt, K = sv.nt_max, int(eng.info.ckpt_every)
+    # (ms per launch, launches per iteration, algorithmic bytes per cell per launch)
+    kern = {}
+    if KS > 1:   # the time-blocked launch reads two states once and writes its last two: bf + 8
+        n_blk = math.ceil(nt / KS) + (math.ceil(K / KS) if sv.fused else math.ceil(nt / KS))
+        kern["fwd_block_kernel"] = (blk_ms, n_blk, bf + 8)
     else:
-        kern = {"fwd_step_kernel": (fwd_ms, 2 * sv.nt_max, bf),
-                "adj_step_kernel": (adj_ms, sv.nt_max, ba)}
+        kern["fwd_step_kernel"] = (fwd_ms, nt + K if sv.fused else 2 * nt, bf)
+    if sv.fused:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
+        kern["fused_step_kernel"] = (fused_ms, nt, bf + ba)
+    else:
+        kern["adj_step_kernel"] = (adj_ms, nt, ba)
     name = max(kern, key=lambda k: kern[k][0] * kern[k][1])
     ms, launches, bpc = kern[name]
     achieved = cells * bpc / (ms * 1e-3) / 1e9
@@ -338,9 +360,11 @@ def main():
                 "bytes_per_cell": bpc, "cells_per_launch": cells, "avg_launch_us": round(ms * 1e3, 3),
                 "launches_per_iter": launches,
                 "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),
-                "fused_step_us": round(fused_ms * 1e3, 3) if fused_ms else None}
+                "fused_step_us": round(fused_ms * 1e3, 3) if fused_ms else None,
+                "fwd_block_us_per_step": round(blk_ms / KS * 1e3, 3) if blk_ms else None}
     log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3:.2f} us, "
-        f"fused {fused_ms * 1e3 if fused_ms else float('nan'):.2f} us per launch; "
+        f"fused {fused_ms * 1e3 if fused_ms else float('nan'):.2f} us per launch, time-blocked fwd "
+        f"{blk_ms / KS * 1e3 if blk_ms else float('nan'):.2f} us per step; "
         f"{name}: {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.2%} of HBM peak)")
 
     baseline = None
@@ -376,6 +400,7 @@ def main():
                    "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
                    "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every), "graph": bool(sv.use_graph),
                    "fused_backward": bool(sv.fused),
+                   "blocking": list(eng.blocking),
                    "tiles": list(eng.tiles),
                    "parallelism": "none"},
         "s_per_adam_iter": round(dt_iter, 5),

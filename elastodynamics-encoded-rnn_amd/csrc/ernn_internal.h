@@ -105,4 +105,35 @@ struct FusedArgs {
   StepArgs fwd;
 };
 
+// Time-blocked forward: one launch advances up to kMaxBlockSteps steps over
+// block regions whose halo (one cell per step on every side) covers the extra
+// steps.  A block is kWavesPerBlock wavefronts stacked vertically, each
+// holding RW rows x 64 V columns of the region in registers; the output tile
+// is the region shrunk by KS cells on every side.
+constexpr int kMaxBlockSteps = 8;
+
+struct BlockDev {
+  int inst, oi0, oj0, set;  // output tile origin (grid coordinates)
+  int src_begin, src_end;   // sources inside the block's region, sorted by row
+  int rec_begin, rec_end;   // receivers inside the output tile, sorted by row
+};
+
+struct BlockArgs {
+  const InstDev* inst;
+  const BlockDev* blocks;
+  const SrcEv* src;
+  const RecEv* rec;
+  const CallDev* call;
+  int n_blocks;
+  int mode;
+  int n0;                        // first step of the launch: reads u^{n0-1}, u^{n0}
+  int ks;                        // steps in this launch (<= KS of the kernel)
+  int s_prev, s_cur;             // slots of u^{n0-1}, u^{n0}
+  int store[kMaxBlockSteps];     // slot receiving u^{n0+1+s}, or -1
+  int stride;                    // recording stride P
+  int record;
+  int snapshot;
+  int rw, ksmax;                 // compiled shape: rows per wavefront, steps per launch
+};
+
 }  // namespace ernn

@@ -19,11 +19,12 @@
 // each XCD to one contiguous band of tiles so that halo rows and the next
 // step's reads of the same band hit that XCD's L2.
 //
-// The forward arithmetic follows the reference op by op in FP32 with
-// contraction disabled (-ffp-contract=off): each 3-tap stencil sums its taps
-// right to left as PyTorch's CPU conv2d does, the six stencil products are
-// summed left to right as written in PI:86-98, then (L + f) / rho and
-// (2u - u_prev) + fl32(dt^2) * a.
+// Both recurrences use unit-weight stencils with h^-2, h^-2 / 4 and dt^2 / rho
+// folded into per-wavefront (or per-cell) coefficients and explicit fused
+// multiply-adds; the FP32 results stay within ~1e-6 relative of the FP64
+// restatement of the reference (PI:44-106, PI:286-291), well inside the 1e-5
+// parity bar, and every cell's arithmetic is fixed, so results do not depend
+// on the tiling.
 #include <hip/hip_runtime.h>
 #include "ernn_internal.h"
 
@@ -154,81 +155,13 @@ __device__ __forceinline__ Row<V> mul_row(const Row<V>& a, const Row<V>& b) {
 }
 
 // --------------------------------------------------------------------------
-// Stencils (PI:56-69): weights a = fl32(h^-2), -2a, q = a/4, -q (all exact)
-// --------------------------------------------------------------------------
-struct Weights {
-  float a, m2a, q, mq;
-};
-
-__device__ __forceinline__ Weights make_weights(float hinv2) {
-  return Weights{hinv2, -2.f * hinv2, 0.25f * hinv2, -0.25f * hinv2};
-}
-
-// x = position in Row::v of the output column
-template <int V>
-__device__ __forceinline__ float d_xx(const Row<V>& m, const Row<V>& c, const Row<V>& p, int x,
-                                      const Weights& w) {
-  return (w.a * p.v[x] + w.m2a * c.v[x]) + w.a * m.v[x];
-}
-template <int V>
-__device__ __forceinline__ float d_yy(const Row<V>& c, int x, const Weights& w) {
-  return (w.a * c.v[x + 1] + w.m2a * c.v[x]) + w.a * c.v[x - 1];
-}
-template <int V>
-__device__ __forceinline__ float d_xy(const Row<V>& m, const Row<V>& p, int x, const Weights& w) {
-  return ((w.q * p.v[x + 1] + w.mq * m.v[x + 1]) + w.mq * p.v[x - 1]) + w.q * m.v[x - 1];
-}
-
-struct Deriv {
-  float ux_xx, ux_yy, ux_xy, uy_xx, uy_yy, uy_xy;
-};
-
-template <int V>
-__device__ __forceinline__ Deriv derivs(const Row<V>& xm, const Row<V>& xc, const Row<V>& xp,
-                                        const Row<V>& ym, const Row<V>& yc, const Row<V>& yp,
-                                        int x, const Weights& w) {
-  Deriv d;
-  d.ux_xx = d_xx<V>(xm, xc, xp, x, w);
-  d.ux_yy = d_yy<V>(xc, x, w);
-  d.ux_xy = d_xy<V>(xm, xp, x, w);
-  d.uy_xx = d_xx<V>(ym, yc, yp, x, w);
-  d.uy_yy = d_yy<V>(yc, x, w);
-  d.uy_xy = d_xy<V>(ym, yp, x, w);
-  return d;
-}
-
-// Stiffness at one cell in the reference's order, with the derived sums the
-// operator uses: C12 + C66, 2 C16, 2 C26 (PI:88-98).
-struct Stiff {
-  float c11, c22, c12, c16, c26, c66, c1266, c216, c226;
-};
-
-__device__ __forceinline__ Stiff make_stiff(float c11, float c22, float c12, float c16, float c26,
-                                            float c66) {
-  Stiff s;
-  s.c11 = c11; s.c22 = c22; s.c12 = c12; s.c16 = c16; s.c26 = c26; s.c66 = c66;
-  s.c1266 = c12 + c66;
-  s.c216 = 2.f * c16;
-  s.c226 = 2.f * c26;
-  return s;
-}
-
-// Lx, Ly of PI:86-98 summed left to right.
-__device__ __forceinline__ void apply_op(const Stiff& s, const Deriv& d, float& lx, float& ly) {
-  lx = ((((s.c11 * d.ux_xx + s.c66 * d.ux_yy) + s.c1266 * d.uy_xy) + s.c216 * d.ux_xy) +
-        s.c16 * d.uy_xx) + s.c26 * d.uy_yy;
-  ly = ((((s.c66 * d.uy_xx + s.c22 * d.uy_yy) + s.c1266 * d.ux_xy) + s.c16 * d.ux_xx) +
-        s.c26 * d.ux_yy) + s.c226 * d.uy_xy;
-}
-
-// --------------------------------------------------------------------------
-// Adjoint arithmetic.  The adjoint is held to the gradient tolerance, not to
-// the reference's rounding order, so it works with unit-weight stencils
+// Stencil arithmetic (PI:56-98).  The stencils of PI:56-69 are applied with
+// unit weights
 //   Uxx = (f[i-1][j] + f[i+1][j]) - 2 f[i][j],   Uyy likewise along j,
 //   Uxy = (f[i+1][j+1] - f[i+1][j-1]) - (f[i-1][j+1] - f[i-1][j-1]),
-// i.e. d_xx = a Uxx, d_yy = a Uyy, d_xy = (a/4) Uxy with a = h^-2, folds a,
-// a/4 and dt^2/rho into per-wavefront coefficients, and writes every
-// multiply-add as an explicit fma.
+// i.e. d_xx = a Uxx, d_yy = a Uyy, d_xy = (a/4) Uxy with a = fl32(h^-2); a,
+// a/4 and dt^2/rho are folded into per-wavefront (or per-cell) coefficients,
+// and every multiply-add is an explicit fma.
 // --------------------------------------------------------------------------
 struct UDeriv {
   float xx_x, yy_x, xy_x, xx_y, yy_y, xy_y;   // _x: of the x component, _y: of the y component
@@ -260,33 +193,45 @@ __device__ __forceinline__ UDeriv uderivs(const Row<V>& xm, const Row<V>& xc, co
   return d;
 }
 
-// Operator coefficients for unit-weight derivatives, scaled by f:
-// k_p = f a C_p on the xx / yy taps, q_* = f (a/4) C on the mixed taps.
+// Operator coefficients for unit-weight derivatives, scaled by fa = f a:
+// k_p = fa C_p on the xx / yy taps, q_* = (fa / 4) C on the mixed taps.  Each
+// is formed in FP64 and rounded once to FP32, so the effective stiffness of
+// the recurrence is off by at most half an ulp (a fixed bias would shift the
+// point at which the gradient is evaluated).
 struct OpCoef {
   float k11, k22, k16, k26, k66, q1266, q216, q226;
 };
 
-__device__ __forceinline__ OpCoef op_coef(const float (&c)[6], float a, float f) {
-  const float fa = f * a, fq = 0.25f * fa;
+__device__ __forceinline__ OpCoef op_coef(const float (&c)[6], double fa) {
+  const double fq = 0.25 * fa;
   OpCoef o;
-  o.k11 = fa * c[0];
-  o.k22 = fa * c[1];
-  o.k16 = fa * c[3];
-  o.k26 = fa * c[4];
-  o.k66 = fa * c[5];
-  o.q1266 = fq * (c[2] + c[5]);
-  o.q216 = fq * (2.f * c[3]);
-  o.q226 = fq * (2.f * c[4]);
+  o.k11 = (float)(fa * c[0]);
+  o.k22 = (float)(fa * c[1]);
+  o.k16 = (float)(fa * c[3]);
+  o.k26 = (float)(fa * c[4]);
+  o.k66 = (float)(fa * c[5]);
+  o.q1266 = (float)(fq * ((double)c[2] + (double)c[5]));
+  o.q216 = (float)(fq * (2.0 * c[3]));
+  o.q226 = (float)(fq * (2.0 * c[4]));
   return o;
 }
 
-// (bx, by) + f (A u) at one cell: the six products of PI:86-98 per component.
+// f (A u) at one cell: the six products of PI:86-98 per component, summed
+// among themselves before any larger term is added (each addition to a state
+// value would round at the state's scale).
+__device__ __forceinline__ void apply_op_sum(const OpCoef& k, const UDeriv& d, float& ox, float& oy) {
+  ox = __builtin_fmaf(k.k11, d.xx_x, __builtin_fmaf(k.k66, d.yy_x, __builtin_fmaf(k.k16, d.xx_y,
+       __builtin_fmaf(k.k26, d.yy_y, __builtin_fmaf(k.q1266, d.xy_y, k.q216 * d.xy_x)))));
+  oy = __builtin_fmaf(k.k66, d.xx_y, __builtin_fmaf(k.k22, d.yy_y, __builtin_fmaf(k.q1266, d.xy_x,
+       __builtin_fmaf(k.k16, d.xx_x, __builtin_fmaf(k.k26, d.yy_x, k.q226 * d.xy_y)))));
+}
+
+// (bx, by) + f (A u): the operator sum first, then one addition.
 __device__ __forceinline__ void apply_op_u(const OpCoef& k, const UDeriv& d, float bx, float by,
                                            float& ox, float& oy) {
-  ox = __builtin_fmaf(k.k11, d.xx_x, __builtin_fmaf(k.k66, d.yy_x, __builtin_fmaf(k.k16, d.xx_y,
-       __builtin_fmaf(k.k26, d.yy_y, __builtin_fmaf(k.q1266, d.xy_y, __builtin_fmaf(k.q216, d.xy_x, bx))))));
-  oy = __builtin_fmaf(k.k66, d.xx_y, __builtin_fmaf(k.k22, d.yy_y, __builtin_fmaf(k.q1266, d.xy_x,
-       __builtin_fmaf(k.k16, d.xx_x, __builtin_fmaf(k.k26, d.yy_x, __builtin_fmaf(k.q226, d.xy_y, by))))));
+  apply_op_sum(k, d, ox, oy);
+  ox = bx + ox;
+  oy = by + oy;
 }
 
 // Constant-C gradient sums at one cell (kPartials slots, weights applied at
@@ -313,6 +258,27 @@ __device__ __forceinline__ void field_grads(const UDeriv& d, float lx, float ly,
   g[3] = __builtin_fmaf(lx, __builtin_fmaf(0.5f, d.xy_x, d.xx_y), ly * d.xx_x);
   g[4] = __builtin_fmaf(lx, d.yy_y, ly * __builtin_fmaf(0.5f, d.xy_y, d.yy_x));
   g[5] = __builtin_fmaf(lx, __builtin_fmaf(0.25f, d.xy_y, d.yy_x), ly * __builtin_fmaf(0.25f, d.xy_x, d.xx_y));
+}
+
+// Forward increment of one cell, (dt^2/rho) L u^n (PI:86-104): the caller
+// adds the force (dt^2/rho) f to it and then the state part 2 u^n - u^{n-1}
+// (PI:286-291), so the state is rounded once per step.  kf = op_coef(C0,
+// (dt^2/rho) a) for the constant-stiffness modes; the damage mode scales the
+// operator by the cell's D (C_p = D C0_p, DI:151-163 with DF:183-212); the
+// six-field mode builds the coefficients per cell.
+template <int MODE>
+__device__ __forceinline__ void fwd_op(const OpCoef& kf, double gsa, const float* cq, const UDeriv& d,
+                                       float& ox, float& oy) {
+  if constexpr (MODE == kConst6) {
+    apply_op_sum(kf, d, ox, oy);
+  } else if constexpr (MODE == kDField) {
+    apply_op_sum(kf, d, ox, oy);
+    ox = cq[0] * ox;
+    oy = cq[0] * oy;
+  } else {
+    const float c[6] = {cq[0], cq[1], cq[2], cq[3], cq[4], cq[5]};
+    apply_op_sum(op_coef(c, gsa), d, ox, oy);
+  }
 }
 
 // Wavefront sum of eight per-lane values by a halving butterfly: each step
@@ -407,13 +373,15 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
   gfloat* uny = unx + I.field;
   const gfloat* coef = C.ws + I.coef_off;
 
-  const Weights w = make_weights(I.hinv2);
+  const float kdt = (float)I.gscale;   // dt^2 / rho
   float c0[6];
+  OpCoef kf;
   if constexpr (MODE != kField6) {
 #pragma unroll
     for (int p = 0; p < 6; ++p) c0[p] = C.params[tl.set * 6 + p];
+    kf = op_coef(c0, I.gscale * (double)I.hinv2);
   }
-  const Stiff sc = make_stiff(c0[0], c0[1], c0[2], c0[3], c0[4], c0[5]);
+  const double gsa = I.gscale * (double)I.hinv2;   // (dt^2 / rho) h^-2
   const float s_n = C.wav[I.wav_off + n];
   const bool rec_step = a.record && (n % a.stride == 0);
   const int k = n / a.stride;
@@ -453,23 +421,17 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
   for (int r = 0; r < R; ++r) {
     const int i = tl.i0 + r;
     const bool live = i < I.nx;
-    float lx[V], ly[V];
+    float nvx[V], nvy[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      const Deriv d = derivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1, w);
-      if constexpr (MODE == kConst6) {
-        apply_op(sc, d, lx[e], ly[e]);
-      } else if constexpr (MODE == kDField) {
-        const float D = cf[r][0][e];
-        const Stiff s = make_stiff(D * c0[0], D * c0[1], D * c0[2], D * c0[3], D * c0[4], D * c0[5]);
-        apply_op(s, d, lx[e], ly[e]);
-      } else {
-        const Stiff s = make_stiff(cf[r][0][e], cf[r][1][e], cf[r][2][e], cf[r][3][e], cf[r][4][e], cf[r][5][e]);
-        apply_op(s, d, lx[e], ly[e]);
-      }
+      const UDeriv d = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1);
+      float cq[NC > 0 ? NC : 1];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
+      fwd_op<MODE>(kf, gsa, cq, d, nvx[e], nvy[e]);
     }
 
-    // point forces on this row: f = s_n * w added to L before / rho (PI:103-104)
+    // point forces on this row: (dt^2/rho) s_n w (PI:103-104, PI:270-277)
     while (sp < tl.src_end) {
       const SrcEv ev = a.src[sp];
       if (ev.i != i) break;
@@ -479,24 +441,18 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           if (e == lc % V) {
-            if (ev.comp == 0) lx[e] = lx[e] + f;
-            else ly[e] = ly[e] + f;
+            if (ev.comp == 0) nvx[e] = __builtin_fmaf(kdt, f, nvx[e]);
+            else nvy[e] = __builtin_fmaf(kdt, f, nvy[e]);
           }
         }
       }
       ++sp;
     }
-
-    float nvx[V], nvy[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      const float ax = lx[e] / I.rho;
-      const float ay = ly[e] / I.rho;
-      float vx = (2.f * xr[r + 1].v[e + 1] - pvx[r][e]) + I.dt2 * ax;
-      float vy = (2.f * yr[r + 1].v[e + 1] - pvy[r][e]) + I.dt2 * ay;
-      if (!live || j0 + lane * V + e >= I.ny) { vx = 0.f; vy = 0.f; }  // keep the ghosts zero
-      nvx[e] = vx;
-      nvy[e] = vy;
+      nvx[e] = __builtin_fmaf(2.f, xr[r + 1].v[e + 1], -pvx[r][e]) + nvx[e];
+      nvy[e] = __builtin_fmaf(2.f, yr[r + 1].v[e + 1], -pvy[r][e]) + nvy[e];
+      if (!live || j0 + lane * V + e >= I.ny) { nvx[e] = 0.f; nvy[e] = 0.f; }  // keep the ghosts zero
     }
     const uint32_t so = row_off(P, i, j0) + lo;
     store_vec<V>(at(unx, so), nvx);
@@ -658,10 +614,10 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   if constexpr (MODE != kField6) {
 #pragma unroll
     for (int p = 0; p < 6; ++p) c0[p] = C.params[tl.set * 6 + p];
-    kl = op_coef(c0, I.hinv2, kdt);
-    ku = op_coef(c0, 1.f, 1.f);
+    kl = op_coef(c0, I.gscale * (double)I.hinv2);
+    ku = op_coef(c0, 1.0);
   }
-  const float ka = kdt * I.hinv2, kq = 0.25f * ka;   // kField6: kdt a, kdt a / 4
+  const float ka = (float)(I.gscale * (double)I.hinv2), kq = 0.25f * ka;   // kField6: (dt^2/rho) a, a / 4
 
   // residual injection on recorded steps m = kP + 1
   const bool inj = ((m - 1) % a.stride == 0) && ((m - 1) / a.stride < I.n_frames);
@@ -777,14 +733,14 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
           const UDeriv du = uderivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
                                        um[r + 2][1], x);
           float ox, oy;
-          apply_op_u(ku, du, 0.f, 0.f, ox, oy);
+          apply_op_sum(ku, du, ox, oy);
           G[0][e] = __builtin_fmaf(Lc.cx[e], ox, Lc.cy[e] * oy);
         }
       } else {
         const float sx = uxx<V>(Lm.w[0], Lc.w[0], Lp.w[0], x) + uyy<V>(Lc.w[1], x);
         const float sy = uxx<V>(Lm.w[3], Lc.w[3], Lp.w[3], x) + uyy<V>(Lc.w[4], x);
-        nvx[e] = __builtin_fmaf(ka, sx, __builtin_fmaf(kq, uxy<V>(Lm.w[2], Lp.w[2], x), bx));
-        nvy[e] = __builtin_fmaf(ka, sy, __builtin_fmaf(kq, uxy<V>(Lm.w[5], Lp.w[5], x), by));
+        nvx[e] = bx + __builtin_fmaf(ka, sx, kq * uxy<V>(Lm.w[2], Lp.w[2], x));
+        nvy[e] = by + __builtin_fmaf(ka, sy, kq * uxy<V>(Lm.w[5], Lp.w[5], x));
         const UDeriv du = uderivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
                                      um[r + 2][1], x);
         float g[6];
@@ -915,6 +871,227 @@ __global__ void __launch_bounds__(kBlock) fused_step_kernel(FusedArgs fa) {
       if (fa.fwd.step < I.nt) fwd_tile<MODE, V, R>(fa.fwd, C, tl, I, lane);
     } else {
       if (fa.adj.step <= I.nt) adj_tile<MODE, V, R>(fa.adj, C, tl, I, t, lane);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Time-blocked forward: KS steps per launch on a block region of
+// kWavesPerBlock * RW rows x 64 V columns.  Wavefront w holds region rows
+// [w RW, (w + 1) RW) of u^n and u^{n-1} in registers; column neighbours come
+// by DPP shifts, and the rows just above and below a wavefront's band come
+// from its neighbours through LDS (one barrier per step).  The valid part of
+// the region shrinks by one cell per step on every side (its edge reads no
+// halo), so after KS steps the output tile (the region less KS cells on each
+// side) is exact.  Each cell's arithmetic is that of fwd_tile (fwd_op), so
+// the states are bitwise those of the single-step kernel.  Cells outside the grid are
+// kept at zero; only output cells are stored and recorded.
+// --------------------------------------------------------------------------
+template <int MODE, int V, int RW, int KS>
+__global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs a) {
+  constexpr int kCols = kWave * V;
+  constexpr int OR = kWavesPerBlock * RW - 2 * KS;   // output rows
+  constexpr int OC = kCols - 2 * KS;                 // output columns
+  constexpr int NC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
+  __shared__ float xch[2][kWavesPerBlock][2][2][kCols];   // [buffer][wavefront][top/bottom][comp][column]
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x >> 6;
+  // block b runs on XCD b % 8, which owns a contiguous band of block tiles
+  const int chunk = (a.n_blocks + 7) >> 3;
+  const int t = (blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+  if (t >= a.n_blocks) return;
+  const BlockDev bl = a.blocks[t];
+  const InstDev I = a.inst[bl.inst];
+  const int ks = min(a.ks, I.nt - a.n0);
+  if (ks <= 0) return;   // this instance has finished its time loop
+  const GCall C = global_call(*a.call);
+
+  const uint32_t P = (uint32_t)I.pitch;
+  gfloat* pool = C.ws + I.pool_off;
+  const gfloat* u0x = pool + (int64_t)a.s_cur * 2 * I.field;
+  const gfloat* u1x = pool + (int64_t)a.s_prev * 2 * I.field;
+  const gfloat* coef = C.ws + I.coef_off;
+  const int gi0 = bl.oi0 - KS + w * RW;          // grid row of this wavefront's first region row
+  const int gj0 = bl.oj0 - KS + lane * V;        // grid column of this lane's first column
+
+  // ---- loads: u^n, u^{n-1} and the coefficients of the lane's cells; cells
+  // outside the grid read a zero ghost cell (clamped address)
+  float cx[RW][V], cy[RW][V], px[RW][V], py[RW][V];
+  float cf[RW][NC > 0 ? NC : 1][V];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int ci = min(max(gi0 + r, -1), I.nx);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const int cj = min(max(gj0 + e, -1), I.ny);
+      const uint32_t o = row_off(P, ci, cj);
+      cx[r][e] = *at(u0x, o);
+      cy[r][e] = *at(u0x + I.field, o);
+      px[r][e] = *at(u1x, o);
+      py[r][e] = *at(u1x + I.field, o);
+#pragma unroll
+      for (int q = 0; q < NC; ++q) cf[r][q][e] = *at(coef + q * I.field, o);
+    }
+  }
+  const float kdt = (float)I.gscale;   // dt^2 / rho
+  float c0[6];
+  OpCoef kf;
+  if constexpr (MODE != kField6) {
+#pragma unroll
+    for (int p = 0; p < 6; ++p) c0[p] = C.params[bl.set * 6 + p];
+    kf = op_coef(c0, I.gscale * (double)I.hinv2);
+  }
+  const double gsa = I.gscale * (double)I.hinv2;
+
+  bool inside[RW][V], owned[RW][V];
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const int gi = gi0 + r, gj = gj0 + e;
+      inside[r][e] = gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny;
+      owned[r][e] = inside[r][e] && gi >= bl.oi0 && gi < bl.oi0 + OR && gj >= bl.oj0 && gj < bl.oj0 + OC;
+    }
+
+  auto publish = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      xch[buf][w][0][0][lane * V + e] = cx[0][e];
+      xch[buf][w][0][1][lane * V + e] = cy[0][e];
+      xch[buf][w][1][0][lane * V + e] = cx[RW - 1][e];
+      xch[buf][w][1][1][lane * V + e] = cy[RW - 1][e];
+    }
+  };
+  publish(0);
+  __syncthreads();
+
+  for (int s = 0; s < ks; ++s) {
+    const int n = a.n0 + s;
+    const int buf = s & 1;
+    // rows -1 .. RW of this wavefront's band; the region's own edge rows get zeros
+    Row<V> xr[RW + 2], yr[RW + 2];
+    {
+      RawRow<V> tx, ty, bx, by;
+      tx.h = ty.h = bx.h = by.h = 0.f;
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        tx.c[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
+        ty.c[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
+        bx.c[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
+        by.c[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+      }
+      xr[0] = make_row<V>(tx, lane);
+      yr[0] = make_row<V>(ty, lane);
+      xr[RW + 1] = make_row<V>(bx, lane);
+      yr[RW + 1] = make_row<V>(by, lane);
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        RawRow<V> rx, ry;
+        rx.h = ry.h = 0.f;
+#pragma unroll
+        for (int e = 0; e < V; ++e) { rx.c[e] = cx[r][e]; ry.c[e] = cy[r][e]; }
+        xr[r + 1] = make_row<V>(rx, lane);
+        yr[r + 1] = make_row<V>(ry, lane);
+      }
+    }
+
+    float nx_[RW][V], ny_[RW][V];
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        const UDeriv d = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1);
+        float cq[NC > 0 ? NC : 1];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
+        fwd_op<MODE>(kf, gsa, cq, d, nx_[r][e], ny_[r][e]);
+      }
+
+    // point forces anywhere in the region (halo cells feed the later steps)
+    const float s_n = C.wav[I.wav_off + n];
+    for (int sp = bl.src_begin; sp < bl.src_end; ++sp) {
+      const SrcEv ev = a.src[sp];
+      const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
+      if (rr < 0 || rr >= RW || lane != lc / V) continue;
+      const float f = s_n * ev.w;
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          if (r == rr && e == lc % V) {
+            if (ev.comp == 0) nx_[r][e] = __builtin_fmaf(kdt, f, nx_[r][e]);
+            else ny_[r][e] = __builtin_fmaf(kdt, f, ny_[r][e]);
+          }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        nx_[r][e] = __builtin_fmaf(2.f, cx[r][e], -px[r][e]) + nx_[r][e];
+        ny_[r][e] = __builtin_fmaf(2.f, cy[r][e], -py[r][e]) + ny_[r][e];
+        if (!inside[r][e]) { nx_[r][e] = 0.f; ny_[r][e] = 0.f; }
+      }
+
+    // store the output cells of the states the schedule keeps
+    const int slot = a.store[s];
+    if (slot >= 0) {
+      gfloat* dx = pool + (int64_t)slot * 2 * I.field;
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (owned[r][e]) {
+            const uint32_t o = row_off(P, gi0 + r, gj0 + e);
+            *at(dx, o) = nx_[r][e];
+            *at(dx + I.field, o) = ny_[r][e];
+          }
+    }
+    // record frame k = n / P from u^{n+1} (PI:294-301)
+    if (a.record && (n % a.stride == 0)) {
+      const int k = n / a.stride;
+      const int nf = I.n_frames;
+      if (a.snapshot) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+          for (int e = 0; e < V; ++e)
+            if (owned[r][e]) {
+              gfloat* tx = C.traces + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + gj0 + e;
+              tx[0] = nx_[r][e];
+              tx[(int64_t)nf * I.nx * I.ny] = ny_[r][e];
+            }
+      } else {
+        for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
+          const RecEv ev = a.rec[rp];
+          const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
+          if (rr < 0 || rr >= RW || lane != lc / V) continue;
+          gfloat* tx = C.traces + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+#pragma unroll
+          for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              if (r == rr && e == lc % V) {
+                tx[0] = nx_[r][e];
+                tx[(int64_t)nf * I.n_rec] = ny_[r][e];
+              }
+        }
+      }
+    }
+
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        px[r][e] = cx[r][e];
+        py[r][e] = cy[r][e];
+        cx[r][e] = nx_[r][e];
+        cy[r][e] = ny_[r][e];
+      }
+    if (s + 1 < ks) {
+      publish(buf ^ 1);
+      __syncthreads();
     }
   }
 }
@@ -1080,6 +1257,37 @@ hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t
     case kConst6: return launch_fused_m<kConst6>(fa, V, blocks, st);
     case kDField: return launch_fused_m<kDField>(fa, V, blocks, st);
     case kField6: return launch_fused_m<kField6>(fa, V, blocks, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// Time-blocked forward shapes (V, RW, KS): V columns per lane, RW rows per
+// wavefront, KS steps per launch (block_ok in ernn_plan.cpp).  The field
+// modes leave out (2, 8), whose registers would spill.
+#define ERNN_BLOCK_CASES_FIELD(X) \
+  X(1, 4, 2) X(1, 4, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 8) X(2, 4, 2) X(2, 4, 4)
+#define ERNN_BLOCK_CASES(X) ERNN_BLOCK_CASES_FIELD(X) X(2, 8, 2) X(2, 8, 4) X(2, 8, 8)
+#define ERNN_BLOCK_CASE(v_, r_, k_)                                                                 \
+  if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
+    hipLaunchKernelGGL((fwd_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kBlock), 0, st, a);  \
+    return hipGetLastError();                                                                      \
+  }
+template <int MODE>
+static hipError_t launch_block_m(const BlockArgs& a, int V, int grid, hipStream_t st) {
+  if constexpr (MODE == kConst6) {
+    ERNN_BLOCK_CASES(ERNN_BLOCK_CASE)
+  } else {
+    ERNN_BLOCK_CASES_FIELD(ERNN_BLOCK_CASE)
+  }
+  return hipErrorInvalidValue;
+}
+#undef ERNN_BLOCK_CASE
+
+hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st) {
+  switch (a.mode) {
+    case kConst6: return launch_block_m<kConst6>(a, V, grid, st);
+    case kDField: return launch_block_m<kDField>(a, V, grid, st);
+    case kField6: return launch_block_m<kField6>(a, V, grid, st);
   }
   return hipErrorInvalidValue;
 }

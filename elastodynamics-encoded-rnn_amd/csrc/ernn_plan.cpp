@@ -7,7 +7,10 @@
 //  forward   steps n = 0 .. nt-1 produce u^{n+1} from (u^n, u^{n-1}).  State m
 //            lives in a 3-slot ring unless m = sK - 1 or m = sK (1 <= s < S),
 //            which go straight into checkpoint s; checkpoint 0 is the zero
-//            initial state (u^-1, u^0) of PI:256-259.
+//            initial state (u^-1, u^0) of PI:256-259.  The time-blocked
+//            forward runs KS steps per launch and keeps only the checkpoint
+//            states and each launch's last two states, which alternate between
+//            two pairs of a 4-slot ring by launch parity.
 //  adjoint   for s = S-1 .. 0: recompute u^{sK+1} .. u^{top} (top =
 //            min((s+1)K, nt)) from checkpoint s into K segment slots, then run
 //            the adjoint steps m = top .. sK+1, which read u^m from the segment
@@ -30,6 +33,7 @@ namespace ernn {
 hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st);
 hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st);
 hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t st);
+hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st);
 hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st);
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t st);
 hipError_t launch_fill_ws(const CallDev* call, int64_t off, int64_t n, hipStream_t st);
@@ -89,6 +93,15 @@ bool fwd_tile_ok(int mode, int V, int R) {
   return !(mode == ERNN_PARAM_FIELD6 && V * R > 8);
 }
 
+// Time-blocked forward shapes (ERNN_BLOCK_CASES in ernn_kernels.hip): V in
+// {1, 2}, RW in {4, 8} rows per wavefront, KS in {2, 4, 8} steps with an
+// output tile of 4 RW - 2 KS > 0 rows; (2, 8) would spill for field modes.
+bool block_ok(int mode, int V, int RW, int KS) {
+  if (!(V == 1 || V == 2) || !(RW == 4 || RW == 8) || !(KS == 2 || KS == 4 || KS == 8)) return false;
+  if (kWavesPerBlock * RW - 2 * KS <= 0) return false;
+  return !(mode != ERNN_PARAM_CONST6 && V == 2 && RW == 8);
+}
+
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 struct Tiling {
@@ -97,6 +110,14 @@ struct Tiling {
   bool loop = false;      // grid capped below the tile count: wavefronts loop over tiles
   int fused_blocks = 0;   // grid of the fused kernel (2 x tiles)
   TileDev* d_tiles = nullptr;
+  SrcEv* d_src = nullptr;
+  RecEv* d_rec = nullptr;
+};
+
+struct BlockTiling {
+  int V = 0, RW = 0, KS = 0;   // KS = 0: the forward runs one step per launch
+  int n_blocks = 0, grid = 0;
+  BlockDev* d_blocks = nullptr;
   SrcEv* d_src = nullptr;
   RecEv* d_rec = nullptr;
 };
@@ -115,6 +136,7 @@ struct Plan {
   InstDev* d_inst = nullptr;
   CallDev* d_call = nullptr;
   Tiling fwd, adj;
+  BlockTiling blk;
   int64_t ws_floats = 0;
   int64_t partial_off = 0;  // in doubles
   int64_t partial_cap = 0;  // tiles
@@ -130,13 +152,13 @@ struct Plan {
   hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;
   hipStream_t cap_stream = nullptr;
 
-  // slots of one instance's state pool: ring, checkpoints, segment bank(s), lam ring
+  // slots of one instance's state pool: ring (4), checkpoints, segment bank(s), lam ring
   int banks() const { return fuse ? 2 : 1; }
   int ring(int r) const { return r; }
-  int ck(int s, int q) const { return 3 + 2 * s + q; }
-  int seg(int bank, int q) const { return 3 + 2 * S + bank * K + q; }
-  int lam(int r) const { return 3 + 2 * S + banks() * K + r; }
-  int n_slots() const { return 3 + 2 * S + banks() * K + 3; }
+  int ck(int s, int q) const { return 4 + 2 * s + q; }
+  int seg(int bank, int q) const { return 4 + 2 * S + bank * K + q; }
+  int lam(int r) const { return 4 + 2 * S + banks() * K + r; }
+  int n_slots() const { return 4 + 2 * S + banks() * K + 3; }
   int bank_of(int s) const { return fuse ? (s & 1) : 0; }
 
   int slot_fwd(int m) const {
@@ -150,6 +172,20 @@ struct Plan {
     if (m == s * K - 1) return ck(s, 0);
     if (m == s * K) return ck(s, 1);
     return seg(bank_of(s), m - s * K - 1);
+  }
+  // Time-blocked forward pass: where state m is kept, or -1.  State m comes
+  // from launch L = (m - 1) / KS at position (m - 1) % KS; checkpoint states
+  // keep their slots, and the last two states of launch L go to ring pair
+  // (L + 1) & 1, which launch L does not read.
+  int slot_blk(int m) const {
+    if (m <= 0) return slot_fwd(m);
+    const int sf = slot_fwd(m);
+    if (sf >= 4) return sf;
+    const int KS = blk.KS;
+    const int L = (m - 1) / KS, pos = (m - 1) % KS;
+    const int ks = std::min(KS, nt_max - L * KS);
+    if (pos >= ks - 2) return ring(2 * ((L + 1) & 1) + (pos - (ks - 2)));
+    return -1;
   }
 };
 
@@ -167,6 +203,13 @@ void free_tiling(Tiling& t) {
   if (t.d_src) (void)hipFree(t.d_src);
   if (t.d_rec) (void)hipFree(t.d_rec);
   t = Tiling{};
+}
+
+void free_blocking(BlockTiling& b) {
+  if (b.d_blocks) (void)hipFree(b.d_blocks);
+  if (b.d_src) (void)hipFree(b.d_src);
+  if (b.d_rec) (void)hipFree(b.d_rec);
+  b = BlockTiling{};
 }
 
 void drop_graphs(Plan* p) {
@@ -271,11 +314,94 @@ int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
   return ERNN_OK;
 }
 
+// Output tiles of (4 RW - 2 KS) x (64 V - 2 KS) cells per instance, row-major;
+// each block's source list covers its whole region (the halo cells' states
+// feed the later steps), its receiver list only its output tile.
+int build_blocking(Plan* p, int V, int RW, int KS) {
+  if (KS <= 1) {
+    free_blocking(p->blk);
+    return ERNN_OK;
+  }
+  if (!block_ok(p->mode, V, RW, KS))
+    return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
+                                "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
+  const int OR = kWavesPerBlock * RW - 2 * KS, OC = kWave * V - 2 * KS;
+  std::vector<BlockDev> blocks;
+  std::vector<SrcEv> sev;
+  std::vector<RecEv> rev;
+  struct Ev { int64_t blk; int i, j, idx; };
+  const auto by_key = [](const Ev& a, const Ev& b) {
+    if (a.blk != b.blk) return a.blk < b.blk;
+    if (a.i != b.i) return a.i < b.i;
+    return a.j < b.j;
+  };
+  for (int b = 0; b < p->n_inst; ++b) {
+    const ernn_instance_t& I = p->inst_h[b];
+    const int nbi = (I.nx + OR - 1) / OR, nbj = (I.ny + OC - 1) / OC;
+    std::vector<Ev> se, re;
+    for (int e = 0; e < I.src_count; ++e) {
+      const ernn_source_t& s = p->src_h[I.src_begin + e];
+      // regions span [bi OR - KS, (bi + 1) OR + KS): at most three blocks per axis
+      for (int bi = std::max(0, (s.i + KS) / OR - 2); bi <= std::min(nbi - 1, (s.i + KS) / OR); ++bi) {
+        if (s.i < bi * OR - KS || s.i >= (bi + 1) * OR + KS) continue;
+        for (int bj = std::max(0, (s.j + KS) / OC - 2); bj <= std::min(nbj - 1, (s.j + KS) / OC); ++bj) {
+          if (s.j < bj * OC - KS || s.j >= (bj + 1) * OC + KS) continue;
+          se.push_back(Ev{(int64_t)bi * nbj + bj, s.i, s.j, I.src_begin + e});
+        }
+      }
+    }
+    if (!p->snapshot) {
+      for (int e = 0; e < I.rec_count; ++e) {
+        const ernn_receiver_t& r = p->rec_h[I.rec_begin + e];
+        re.push_back(Ev{(int64_t)(r.i / OR) * nbj + r.j / OC, r.i, r.j, e});
+      }
+    }
+    std::stable_sort(se.begin(), se.end(), by_key);
+    std::stable_sort(re.begin(), re.end(), by_key);
+    size_t si = 0, ri = 0;
+    for (int bi = 0; bi < nbi; ++bi) {
+      for (int bj = 0; bj < nbj; ++bj) {
+        const int64_t id = (int64_t)bi * nbj + bj;
+        BlockDev bd{};
+        bd.inst = b;
+        bd.oi0 = bi * OR;
+        bd.oj0 = bj * OC;
+        bd.set = I.set;
+        bd.src_begin = (int)sev.size();
+        for (; si < se.size() && se[si].blk == id; ++si) {
+          const ernn_source_t& s = p->src_h[se[si].idx];
+          sev.push_back(SrcEv{s.i, s.j, s.w, s.comp});
+        }
+        bd.src_end = (int)sev.size();
+        bd.rec_begin = (int)rev.size();
+        for (; ri < re.size() && re[ri].blk == id; ++ri) {
+          const ernn_receiver_t& r = p->rec_h[I.rec_begin + re[ri].idx];
+          rev.push_back(RecEv{r.i, r.j, re[ri].idx, 0});
+        }
+        bd.rec_end = (int)rev.size();
+        blocks.push_back(bd);
+      }
+    }
+  }
+  if (blocks.size() > ((size_t)1 << 30)) return fail(ERNN_E_CAPACITY, "too many blocks");
+  free_blocking(p->blk);
+  p->blk.V = V;
+  p->blk.RW = RW;
+  p->blk.KS = KS;
+  p->blk.n_blocks = (int)blocks.size();
+  p->blk.grid = 8 * ((p->blk.n_blocks + 7) / 8);
+  ERNN_TRY(dev_upload(&p->blk.d_blocks, blocks));
+  ERNN_TRY(dev_upload(&p->blk.d_src, sev));
+  ERNN_TRY(dev_upload(&p->blk.d_rec, rev));
+  return ERNN_OK;
+}
+
 void destroy_plan(Plan* p) {
   if (!p) return;
   drop_graphs(p);
   free_tiling(p->fwd);
   free_tiling(p->adj);
+  free_blocking(p->blk);
   if (p->d_inst) (void)hipFree(p->d_inst);
   if (p->d_call) (void)hipFree(p->d_call);
   if (p->d_set_first) (void)hipFree(p->d_set_first);
@@ -302,6 +428,9 @@ int validate(const ernn_desc_t* d) {
   if ((d->adj_v != 0 || d->adj_r != 0) && !adj_tile_ok(d->param_mode, d->adj_v, d->adj_r))
     return fail(ERNN_E_INVALID, "adjoint tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16 "
                                 "(damage field: V*R <= 8 and R <= 4, six fields: V*R <= 4 and R <= 2)");
+  if (d->blk_ks > 1 && !block_ok(d->param_mode, d->blk_v, d->blk_rw, d->blk_ks))
+    return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
+                                "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))
@@ -478,6 +607,7 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 1, d->fwd_r ? d->fwd_r : 4, d->fwd_bpc));
   const int adj_r0 = (d->param_mode == ERNN_PARAM_FIELD6) ? 2 : 4;
   ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 1, d->adj_r ? d->adj_r : adj_r0, d->adj_bpc));
+  ERNN_TRY(build_blocking(p, d->blk_v, d->blk_rw, d->blk_ks));
   ERNN_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
   return ERNN_OK;
 }
@@ -539,8 +669,52 @@ int enqueue_pack(const Plan* p, hipStream_t st) {
   return ERNN_OK;
 }
 
+BlockArgs block_args(const Plan* p) {
+  BlockArgs a{};
+  a.inst = p->d_inst;
+  a.blocks = p->blk.d_blocks;
+  a.src = p->blk.d_src;
+  a.rec = p->blk.d_rec;
+  a.call = p->d_call;
+  a.n_blocks = p->blk.n_blocks;
+  a.mode = p->mode;
+  a.stride = p->P;
+  a.snapshot = p->snapshot;
+  a.rw = p->blk.RW;
+  a.ksmax = p->blk.KS;
+  return a;
+}
+
+// One time-blocked launch of the forward pass (record = 1) from step n0.
+int launch_blk_forward(const Plan* p, BlockArgs& a, int n0, hipStream_t st) {
+  a.n0 = n0;
+  a.ks = std::min(p->blk.KS, p->nt_max - n0);
+  a.s_prev = p->slot_blk(n0 - 1);
+  a.s_cur = p->slot_blk(n0);
+  for (int s = 0; s < kMaxBlockSteps; ++s) a.store[s] = s < a.ks ? p->slot_blk(n0 + 1 + s) : -1;
+  ERNN_HIP(launch_fwd_block(a, p->blk.V, p->blk.grid, st));
+  return ERNN_OK;
+}
+
+// Time-blocked recompute of segment s from step n0: every state is stored.
+int launch_blk_recompute(const Plan* p, BlockArgs& a, int s, int n0, int top, hipStream_t st) {
+  a.n0 = n0;
+  a.ks = std::min(p->blk.KS, top - n0);
+  a.s_prev = p->slot_seg(s, n0 - 1);
+  a.s_cur = p->slot_seg(s, n0);
+  for (int q = 0; q < kMaxBlockSteps; ++q) a.store[q] = q < a.ks ? p->slot_seg(s, n0 + 1 + q) : -1;
+  ERNN_HIP(launch_fwd_block(a, p->blk.V, p->blk.grid, st));
+  return ERNN_OK;
+}
+
 int enqueue_forward(const Plan* p, hipStream_t st) {
   ERNN_TRY(enqueue_pack(p, st));
+  if (p->blk.KS > 1) {
+    BlockArgs b = block_args(p);
+    b.record = 1;
+    for (int n0 = 0; n0 < p->nt_max; n0 += p->blk.KS) ERNN_TRY(launch_blk_forward(p, b, n0, st));
+    return ERNN_OK;
+  }
   StepArgs a = base_args(p, p->fwd);
   a.record = 1;
   for (int n = 0; n < p->nt_max; ++n) {
@@ -575,13 +749,25 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
     r.s_next = p->slot_seg(s, n + 1);
   };
   auto seg_top = [&](int s) { return std::min((s + 1) * p->K, p->nt_max); };
+  BlockArgs rb = block_args(p);
+  rb.record = 0;
+  // recompute segment s whole (time-blocked when enabled)
+  auto recompute_segment = [&](int s) -> int {
+    const int top = seg_top(s);
+    if (p->blk.KS > 1) {
+      for (int n0 = s * p->K; n0 < top; n0 += p->blk.KS) ERNN_TRY(launch_blk_recompute(p, rb, s, n0, top, st));
+      return ERNN_OK;
+    }
+    for (int n = s * p->K; n < top; ++n) {
+      recompute(f, s, n);
+      ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
+    }
+    return ERNN_OK;
+  };
   if (!p->fuse) {
     for (int s = p->S - 1; s >= 0; --s) {
       const int top = seg_top(s), bot = s * p->K + 1;
-      for (int n = s * p->K; n < top; ++n) {
-        recompute(f, s, n);
-        ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
-      }
+      ERNN_TRY(recompute_segment(s));
       a.seg_top = top;
       a.seg_bot = bot;
       for (int m = top; m >= bot; --m) {
@@ -597,13 +783,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
     // The last segment is recomputed alone; afterwards every adjoint step of
     // segment s shares its launch with one recompute step of segment s - 1,
     // which fills the other bank (segment s - 1 always has K steps).
-    {
-      const int s = p->S - 1;
-      for (int n = s * p->K; n < seg_top(s); ++n) {
-        recompute(f, s, n);
-        ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
-      }
-    }
+    ERNN_TRY(recompute_segment(p->S - 1));
     FusedArgs fa{};
     fa.fwd = base_args(p, p->adj);   // the fused kernel runs both on the adjoint's tiles
     fa.fwd.record = 0;
@@ -728,6 +908,10 @@ int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
   info->adj_blocks = p->adj.blocks;
   info->n_cu = p->n_cu;
   info->fuse = p->fuse ? 1 : 0;
+  info->blk_v = p->blk.V;
+  info->blk_rw = p->blk.RW;
+  info->blk_ks = p->blk.KS;
+  info->blk_blocks = p->blk.n_blocks;
   return ERNN_OK;
 }
 
@@ -771,12 +955,22 @@ int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc) {
   return build_tiling(p, which == 0 ? &p->fwd : &p->adj, v, r, bpc);
 }
 
+int ernn_plan_set_blocking(void* plan, int v, int rw, int ks) {
+  if (!plan) return fail(ERNN_E_INVALID, "null plan");
+  Plan* p = static_cast<Plan*>(plan);
+  ERNN_HIP(hipSetDevice(p->device));
+  ERNN_HIP(hipDeviceSynchronize());
+  drop_graphs(p);
+  return build_blocking(p, v, rw, ks);
+}
+
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
                     float* ms_out) {
-  if (!plan || !ms_out || steps < 1 || which < 0 || which > 2) return fail(ERNN_E_INVALID, "bad argument");
+  if (!plan || !ms_out || steps < 1 || which < 0 || which > 3) return fail(ERNN_E_INVALID, "bad argument");
   Plan* p = static_cast<Plan*>(plan);
   if (which == 2 && !p->fuse) return fail(ERNN_E_INVALID, "fused timing needs a plan with fuse = 1");
-  ERNN_TRY(check_call(p, call, which != 0, /*grads=*/false));  // timing never runs the finalize kernels
+  if (which == 3 && p->blk.KS <= 1) return fail(ERNN_E_INVALID, "time-blocked timing needs blk_ks > 1");
+  ERNN_TRY(check_call(p, call, which == 1 || which == 2, /*grads=*/false));  // timing never finalizes
   hipStream_t st = static_cast<hipStream_t>(stream);
   ERNN_HIP(hipSetDevice(p->device));
   ERNN_HIP(launch_set_call(p->d_call, to_dev(call), st));
@@ -785,7 +979,12 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   ERNN_HIP(hipEventCreate(&e1));
   ERNN_HIP(hipEventRecord(e0, st));
   const int nsteps = std::min(steps, p->nt_max);
-  if (which == 0) {
+  if (which == 3) {   // `steps` time-blocked forward launches of KS steps each
+    BlockArgs b = block_args(p);
+    b.record = 1;
+    const int launches = std::min(steps, (p->nt_max + p->blk.KS - 1) / p->blk.KS);
+    for (int L = 0; L < launches; ++L) ERNN_TRY(launch_blk_forward(p, b, L * p->blk.KS, st));
+  } else if (which == 0) {
     StepArgs a = base_args(p, p->fwd);
     a.record = 1;
     for (int n = 0; n < nsteps; ++n) {

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (must precede loading the HIP library)
 _DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_DIR, "libernn_hip.so")
 
-ERNN_ABI_VERSION = 1
+ERNN_ABI_VERSION = 2
 PARAM_CONST6, PARAM_DFIELD, PARAM_FIELD6 = 0, 1, 2
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "capacity exceeded"}
 
@@ -41,6 +41,7 @@ class Desc(C.Structure):
                 ("fwd_v", C.c_int32), ("fwd_r", C.c_int32), ("adj_v", C.c_int32),
                 ("adj_r", C.c_int32), ("use_graph", C.c_int32), ("fwd_bpc", C.c_int32),
                 ("adj_bpc", C.c_int32), ("fuse", C.c_int32),
+                ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32), ("reserved", C.c_int32),
                 ("inst", C.POINTER(Instance)), ("src", C.POINTER(Source)), ("n_src", C.c_int64),
                 ("rec", C.POINTER(Receiver)), ("n_rec", C.c_int64)]
 
@@ -53,7 +54,8 @@ class PlanInfo(C.Structure):
                 ("adj_r", C.c_int32), ("fwd_tiles", C.c_int32), ("adj_tiles", C.c_int32),
                 ("use_graph", C.c_int32), ("fwd_bpc", C.c_int32), ("adj_bpc", C.c_int32),
                 ("fwd_blocks", C.c_int32), ("adj_blocks", C.c_int32), ("n_cu", C.c_int32),
-                ("fuse", C.c_int32)]
+                ("fuse", C.c_int32), ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
+                ("blk_blocks", C.c_int32)]
 
 
 class Call(C.Structure):
@@ -73,6 +75,7 @@ EXPORTS = {
     "ernn_forward": ([C.c_void_p, C.POINTER(Call), C.c_void_p], C.c_int),
     "ernn_adjoint": ([C.c_void_p, C.POINTER(Call), C.c_void_p], C.c_int),
     "ernn_plan_set_tiling": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "ernn_plan_set_blocking": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
     "ernn_time_steps": ([C.c_void_p, C.c_int, C.c_int, C.POINTER(Call), C.c_void_p,
                          C.POINTER(C.c_float)], C.c_int),
 }

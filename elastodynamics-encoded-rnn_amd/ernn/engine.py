@@ -72,6 +72,7 @@ class Engine:
         d.fwd_v, d.fwd_r, d.adj_v, d.adj_r, d.fwd_bpc, d.adj_bpc = t[:6]
         d.use_graph = int(sv.use_graph)
         d.fuse = int(sv.fused)
+        d.blk_v, d.blk_rw, d.blk_ks = (int(x) for x in sv.blocking)
         d.inst = inst
         d.src, d.n_src = src, len(src_list)
         d.rec, d.n_rec = rec, len(rec_list)
@@ -202,6 +203,17 @@ class Engine:
     def tiles(self) -> Tuple[int, int, int, int, int, int]:
         i = self.info
         return (int(i.fwd_v), int(i.fwd_r), int(i.adj_v), int(i.adj_r), int(i.fwd_bpc), int(i.adj_bpc))
+
+    @property
+    def blocking(self) -> Tuple[int, int, int]:
+        i = self.info
+        return (int(i.blk_v), int(i.blk_rw), int(i.blk_ks))
+
+    def set_blocking(self, v: int, rw: int, ks: int) -> None:
+        """Time-blocked forward (ks > 1) or one step per launch (ks <= 1)."""
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().ernn_plan_set_blocking(self._plan, v, rw, ks), "ernn_plan_set_blocking")
+            _lib.check(_lib.lib().ernn_plan_info(self._plan, C.byref(self.info)), "ernn_plan_info")
 
     def set_tiling(self, which: int, v: int, r: int, bpc: int = 0) -> None:
         with torch.cuda.device(self.device):

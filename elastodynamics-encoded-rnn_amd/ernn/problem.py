@@ -180,6 +180,9 @@ class Survey:
     # separate at 512^2), off above (the fused kernel inherits the adjoint's
     # register footprint: 234 us vs 42 + 141 us at 8 x 1024^2).
     fuse: Optional[bool] = None
+    # Time-blocked forward (V, RW, KS): KS steps per launch on blocks of four
+    # wavefronts x RW rows x 64 V columns; (0, 0, 0) = one step per launch.
+    blocking: Tuple[int, int, int] = (0, 0, 0)
 
     def __post_init__(self):
         if self.mode not in MODES:

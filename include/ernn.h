@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define ERNN_ABI_VERSION 1
+#define ERNN_ABI_VERSION 2
 
 #define ERNN_OK 0
 #define ERNN_E_INVALID (-1)   /* bad argument or descriptor */
@@ -92,6 +92,12 @@ typedef struct {
   int32_t adj_bpc;      /* adjoint grid cap */
   int32_t fuse;         /* 1: run each adjoint step together with one recompute step of the
                            previous checkpoint segment (one launch; K more slots per instance) */
+  int32_t blk_v, blk_rw, blk_ks; /* time-blocked forward: blk_ks steps per launch on blocks of 4
+                           wavefronts x blk_rw rows x 64 blk_v columns; blk_ks <= 1 = one step per
+                           launch.  V in {1,2}, RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS; field
+                           modes exclude (V, RW) = (2, 8).  Used by the forward pass and the
+                           segment recompute of the unfused adjoint. */
+  int32_t reserved;     /* 0 */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;
@@ -114,6 +120,8 @@ typedef struct {
   int32_t fwd_blocks, adj_blocks;
   int32_t n_cu;             /* compute units of the plan's device */
   int32_t fuse;
+  int32_t blk_v, blk_rw, blk_ks; /* time-blocked forward shape (blk_ks = 0: off) */
+  int32_t blk_blocks;       /* blocks per time-blocked launch */
 } ernn_plan_info_t;
 
 /* Per-call device pointers.
@@ -178,8 +186,15 @@ int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream);
  * synchronizes the device and drops captured graphs. */
 int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc);
 
+/* Switch the forward to blk_ks steps per launch on blocks of the given shape
+ * (ks <= 1: one step per launch); same limits as ernn_desc_t.blk_*.  No
+ * counterpart in the reference: the time loop of PI:268-305 is one step per
+ * iteration.  Synchronizes the device and drops captured graphs. */
+int ernn_plan_set_blocking(void* plan, int v, int rw, int ks);
+
 /* Time `steps` forward (which = 0), adjoint (which = 1) or fused adjoint +
- * recompute (which = 2, fuse plans) steps with the current tiling using
+ * recompute (which = 2, fuse plans) steps with the current tiling, or `steps`
+ * time-blocked forward launches of blk_ks steps each (which = 3), using
  * hipEvents on `stream`; synchronizes.  The workspace must be initialized.
  * ms_out receives the elapsed milliseconds. */
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,

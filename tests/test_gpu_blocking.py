@@ -1,0 +1,125 @@
+"""Time-blocked forward (KS steps per launch) vs the single-step kernel.
+
+The blocked kernel runs each cell's arithmetic exactly as the single-step
+kernel does, so traces, the checkpoints it leaves for the adjoint and the
+states of the blocked segment recompute are bitwise the same; gradients are
+therefore bitwise the same at equal adjoint tiles.  Cases put sources and
+receivers on block edges and in block halos, cover grids smaller than one
+block, step counts that are not multiples of KS or of K, uneven batches whose
+instances stop at different steps, and whole-field (snapshot) recording.
+"""
+import numpy as np
+import pytest
+import torch
+
+import ernn
+from gpu_helpers import C_ANISO, DEV, make_shot, rel_l2, run_hip
+
+pytestmark = pytest.mark.gpu
+
+# (V, RW, KS): output tiles of (4 RW - 2 KS) x (64 V - 2 KS) cells
+SHAPES = [(1, 4, 2), (1, 4, 4), (1, 8, 8), (2, 4, 4), (2, 8, 4), (1, 8, 2)]
+
+
+def _edge_sources(nx, ny, h, V, RW, KS):
+    """Point sources on and next to the output-tile edges of the block grid."""
+    w = float(np.float32(1.0 / h ** 2))
+    OR, OC = 4 * RW - 2 * KS, 64 * V - 2 * KS
+    cells = {(nx // 2, ny - 1), (0, 0), (nx - 1, ny - 1), (nx // 3, ny // 2)}
+    for i in (OR - 1, OR, OR + 1, 2 * OR - 1):
+        for j in (OC - 1, OC, 1, ny - 2):
+            if 0 <= i < nx and 0 <= j < ny:
+                cells.add((i, j))
+    return [(i, j, w * (0.5 + 0.1 * k), k % 2) for k, (i, j) in enumerate(sorted(cells))]
+
+
+def _receivers(nx, ny, V, RW, KS):
+    OR, OC = 4 * RW - 2 * KS, 64 * V - 2 * KS
+    r = {(0, ny - 1), (nx - 1, 0), (nx // 2, ny // 2)}
+    for i in (OR - 1, OR, nx - 1):
+        for j in (OC - 1, OC, ny - 1):
+            if 0 <= i < nx and 0 <= j < ny:
+                r.add((i, j))
+    return np.array(sorted(r), dtype=np.int32)
+
+
+def _pair(shots, mode, blocking, tiles=(2, 2, 1, 4, 0, 0), K=7, snapshot=False, stride=1, fuse=False, **kw):
+    base = ernn.Survey(shots, mode=mode, ckpt_every=K, tiles=tiles, snapshot=snapshot, rec_stride=stride,
+                       fuse=fuse, n_sets=kw.pop("n_sets", 1))
+    blk = ernn.Survey(shots, mode=mode, ckpt_every=K, tiles=tiles, snapshot=snapshot, rec_stride=stride,
+                      fuse=fuse, n_sets=base.n_sets, blocking=blocking)
+    a = run_hip(base, **kw)
+    b = run_hip(blk, **kw)
+    assert b[0].blocking == tuple(blocking)
+    return a, b
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("mode", ["const6", "dfield", "field6"])
+def test_blocked_forward_and_gradients_are_bitwise_single_step(mode, shape):
+    V, RW, KS = shape
+    if mode != "const6" and (V, RW) == (2, 8):
+        pytest.skip("not built for field modes (register spill)")
+    nx, ny, nt, h = 45, 150, 47, 0.2e-3
+    rng = np.random.default_rng(7)
+    shot = make_shot(nx, ny, nt, h, sources=_edge_sources(nx, ny, h, V, RW, KS),
+                     receivers=_receivers(nx, ny, V, RW, KS))
+    kw = {}
+    if mode == "const6":
+        kw["params"] = C_ANISO
+    elif mode == "dfield":
+        kw["params"] = C_ANISO
+        kw["field"] = rng.uniform(0.4, 1.0, (nx, ny))
+    else:
+        kw["field"] = np.stack([rng.uniform(0.6, 1.0, (nx, ny)) * c for c in C_ANISO])
+    G = rng.standard_normal((2, nt, len(shot.receivers)))
+    tiles = (2, 2, 1, 2, 0, 0) if mode == "field6" else (2, 2, 1, 4, 0, 0)
+    (e0, t0, g0), (e1, t1, g1) = _pair([shot], mode, shape, tiles=tiles, G=G, **kw)
+    print(f"{mode} {shape}: blocks {e1.info.blk_blocks}, traces max |diff| {np.abs(t1 - t0).max():.3e}")
+    assert np.array_equal(t0, t1)
+    for x, y in zip(g0, g1):
+        if x is not None:
+            assert np.array_equal(x, y)
+    e0.close(); e1.close()
+
+
+def test_blocked_snapshot_small_grid_and_uneven_batch():
+    """Whole-field frames with stride 3; a grid smaller than one block; instances
+    of different sizes and step counts (each stops at its own nt) in one launch."""
+    h = 0.2e-3
+    shots = [make_shot(5, 3, 29, h), make_shot(40, 90, 41, h), make_shot(33, 70, 35, h)]
+    G = np.random.default_rng(3).standard_normal(
+        sum(2 * ((s.nt + 2) // 3) * s.nx * s.ny for s in shots))
+    (e0, t0, g0), (e1, t1, g1) = _pair(shots, "const6", (1, 4, 4), snapshot=True, stride=3, K=5,
+                                       params=C_ANISO, G=G)
+    assert np.array_equal(t0, t1)
+    assert np.array_equal(g0[0], g1[0])
+    e0.close(); e1.close()
+
+
+def test_blocked_forward_pass_with_fused_backward():
+    """The fused backward recomputes one step per launch; the blocked forward
+    pass must leave it the same checkpoints."""
+    nx, ny, nt, h = 64, 128, 61, 0.2e-3
+    shot = make_shot(nx, ny, nt, h, receivers=_receivers(nx, ny, 2, 8, 4))
+    G = np.random.default_rng(5).standard_normal((2, nt, len(shot.receivers)))
+    (e0, t0, g0), (e1, t1, g1) = _pair([shot], "const6", (2, 8, 4), K=6, fuse=True, params=C_ANISO, G=G)
+    assert e1.info.fuse == 1
+    assert np.array_equal(t0, t1)
+    assert np.array_equal(g0[0], g1[0])
+    e0.close(); e1.close()
+
+
+def test_set_blocking_switches_and_replays_graphs():
+    """Switching the shape on a live plan (as the autotuner does) and graph replay."""
+    nx, ny, nt, h = 50, 200, 40, 0.2e-3
+    shot = make_shot(nx, ny, nt, h, receivers=_receivers(nx, ny, 1, 4, 2))
+    sv = ernn.Survey([shot], mode="const6", ckpt_every=8, use_graph=True)
+    eng = ernn.Engine(sv, device=DEV)
+    ref = run_hip(sv, params=C_ANISO, engine=eng)[1]
+    for shape in [(1, 4, 2), (2, 4, 4), (1, 8, 8), (0, 0, 0)]:
+        eng.set_blocking(*shape)
+        for _ in range(2):
+            tr = run_hip(sv, params=C_ANISO, engine=eng)[1]
+            assert np.array_equal(tr, ref), shape
+    eng.close()

@@ -153,7 +153,10 @@ def test_full_size_traces_and_gradient_vs_plain_autograd(name):
 
 def test_single_512_crop_vs_fp64_oracle_and_finite_differences():
     """single-512 physics (h 0.05 mm, dt, C, 2000 steps) on a 128^2 crop problem:
-    HIP vs the FP64 oracle, and the HIP gradient vs central FD (step 1e-4 C)."""
+    HIP vs the FP64 oracle, and the HIP gradient vs central FD (step 1e-3 C:
+    J sums 2000 FP32 steps, so a smaller step lets the FP32 noise of J(C +- d)
+    dominate the difference, while the O(d^2) truncation of a 1e-3 step stays
+    below 1e-5 of the gradient)."""
     wl = configs.single_512(nt=2000, n=128)
     s = wl.survey.shots[0]
     C = wl.C_true[0] * np.array([1.05, 0.97, 1.1, 1.0, 1.0, 0.93]) + np.array([0, 0, 0, 0.4e9, -0.3e9, 0])
@@ -177,7 +180,7 @@ def test_single_512_crop_vs_fp64_oracle_and_finite_differences():
     # central differences of the HIP forward, J = <traces, G>
     fd = []
     for k in range(6):
-        d = 1e-4 * max(abs(C[k]), 5e9)
+        d = 1e-3 * max(abs(C[k]), 5e9)
         cp = C.copy(); cp[k] += d
         cm = C.copy(); cm[k] -= d
         jp = float((eng.forward(torch.tensor(cp, dtype=torch.float32, device=DEV), None, wav, ws).double() * Gt.double()).sum())
@@ -189,7 +192,7 @@ def test_single_512_crop_vs_fp64_oracle_and_finite_differences():
           f"grad vs FP32 central FD {e_fd:.2e}")
     assert e_tr < 1e-5
     assert e_g < 1e-4
-    assert e_fd < 2e-2      # FP32 forward noise / 1e-4 step bounds the FD accuracy
+    assert e_fd < 2e-2      # FP32 forward noise / step bounds the FD accuracy
 
 
 def test_damage_1024_dfield_vs_plain_per_cell_autograd():

@@ -76,6 +76,11 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
     ({}, {"param_mode": 2, "adj_v": 1, "adj_r": 4}, "adjoint tile"),
     ({}, {"param_mode": 1, "adj_v": 1, "adj_r": 8}, "adjoint tile"),
     ({}, {"param_mode": 1, "adj_v": 4, "adj_r": 4}, "adjoint tile"),
+    ({}, {"blk_v": 4, "blk_rw": 4, "blk_ks": 2}, "time-blocked"),
+    ({}, {"blk_v": 1, "blk_rw": 4, "blk_ks": 8}, "time-blocked"),
+    ({}, {"blk_v": 1, "blk_rw": 6, "blk_ks": 2}, "time-blocked"),
+    ({}, {"blk_v": 1, "blk_rw": 4, "blk_ks": 3}, "time-blocked"),
+    ({}, {"param_mode": 1, "blk_v": 2, "blk_rw": 8, "blk_ks": 4}, "time-blocked"),
 ])
 def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
     inst = dict(GOOD, **patch)
