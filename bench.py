@@ -280,6 +280,11 @@ def main():
                                                   steps=max(16, min(96, 40000 // cells_k)))
         log(f"autotune: forward blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
             f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f})")
+        if not sv.fused and sv.mode != "field6":
+            bv, brw, bks, bms, btimes = tune_blocking(eng, params_true, field_true, model.wavelets,
+                                                      steps=max(16, min(96, 40000 // cells_k)), which=1)
+            log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
+                f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f})")
     with torch.no_grad():
         obs = ernn.ernn_traces(eng, params=params_true, field=field_true, wavelets=model.wavelets)
         gen = torch.Generator(device="cpu").manual_seed(1)
@@ -326,6 +331,12 @@ def main():
         L = max(2, nk // KS)
         eng.time_steps(3, 2, p, f, model.wavelets, ws, tr0, tr0)
         blk_ms = eng.time_steps(3, L, p, f, model.wavelets, ws, tr0, tr0) / L
+    AKS = eng.adj_blocking[2] if not sv.fused else 0
+    ablk_ms = None
+    if AKS > 1:
+        L = max(2, nk // AKS)
+        eng.time_steps(4, 2, p, f, model.wavelets, ws, tr0, tr0)
+        ablk_ms = eng.time_steps(4, L, p, f, model.wavelets, ws, tr0, tr0) / L
     eng.release_workspace(ws)
     cells = sum(s.nx * s.ny for s in sv.shots)
     bf, ba = BYTES_PER_CELL[(sv.mode, 0)], BYTES_PER_CELL[(sv.mode, 1)]
@@ -339,6 +350,10 @@ def main():
         kern["fwd_step_kernel"] = (fwd_ms, nt + K if sv.fused else 2 * nt, bf)
     if sv.fused:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
         kern["fused_step_kernel"] = (fused_ms, nt, bf + ba)
+    elif AKS > 1:   # reads lam^{m+1}, lam^{m+2} (+ D), u^m per step, writes two states, acc r/w once
+        abytes = 16 + 8 * AKS + 16 + (12 if sv.mode == "dfield" else 0)
+        kern["adj_block_kernel"] = (ablk_ms, sum(math.ceil(min(K, nt - s * K) / AKS)
+                                                for s in range(math.ceil(nt / K))), abytes)
     else:
         kern["adj_step_kernel"] = (adj_ms, nt, ba)
     name = max(kern, key=lambda k: kern[k][0] * kern[k][1])
@@ -361,10 +376,12 @@ def main():
                 "launches_per_iter": launches,
                 "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),
                 "fused_step_us": round(fused_ms * 1e3, 3) if fused_ms else None,
-                "fwd_block_us_per_step": round(blk_ms / KS * 1e3, 3) if blk_ms else None}
+                "fwd_block_us_per_step": round(blk_ms / KS * 1e3, 3) if blk_ms else None,
+                "adj_block_us_per_step": round(ablk_ms / AKS * 1e3, 3) if ablk_ms else None}
     log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3:.2f} us, "
         f"fused {fused_ms * 1e3 if fused_ms else float('nan'):.2f} us per launch, time-blocked fwd "
-        f"{blk_ms / KS * 1e3 if blk_ms else float('nan'):.2f} us per step; "
+        f"{blk_ms / KS * 1e3 if blk_ms else float('nan'):.2f} us per step, time-blocked adj "
+        f"{ablk_ms / AKS * 1e3 if ablk_ms else float('nan'):.2f} us per step; "
         f"{name}: {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.2%} of HBM peak)")
 
     baseline = None
@@ -401,6 +418,7 @@ def main():
                    "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every), "graph": bool(sv.use_graph),
                    "fused_backward": bool(sv.fused),
                    "blocking": list(eng.blocking),
+                   "adj_blocking": list(eng.adj_blocking),
                    "tiles": list(eng.tiles),
                    "parallelism": "none"},
         "s_per_adam_iter": round(dt_iter, 5),

@@ -136,4 +136,25 @@ struct BlockArgs {
   int rw, ksmax;                 // compiled shape: rows per wavefront, steps per launch
 };
 
+// Time-blocked adjoint (constant and damage modes): one launch runs the
+// reverse steps m = m0, m0 - 1, ..., m0 - ks + 1 on the block regions.
+struct BlockAdjArgs {
+  const InstDev* inst;
+  const BlockDev* blocks;        // receiver lists cover the whole region (injection)
+  const RecEv* rec;
+  const CallDev* call;
+  int64_t partial_off;           // [blocks][kWavesPerBlock][kPartials] fp64 partials
+  int n_blocks;
+  int mode;
+  int m0, ks;
+  int s_l1, s_l2;                // slots of lam^{m0+1}, lam^{m0+2}
+  int store_in;                  // slot receiving lam^{m0+1} (a one-step launch), or -1
+  int store[kMaxBlockSteps];     // slot receiving lam^{m0-s}, or -1
+  int u_slot[kMaxBlockSteps];    // slot holding u^{m0-s}
+  int seg_top, seg_bot;          // the checkpoint segment this launch lies in
+  int stride;
+  int snapshot;
+  int rw, ksmax;
+};
+
 }  // namespace ernn

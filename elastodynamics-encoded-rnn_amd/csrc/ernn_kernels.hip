@@ -1097,6 +1097,291 @@ __global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs a) {
 }
 
 // --------------------------------------------------------------------------
+// Time-blocked adjoint: KS reverse steps per launch on the block regions of
+// the time-blocked forward.  Wavefront w holds lam^{m+1}, lam^{m+2} (and D)
+// of its region rows in registers; the stencilled field (lam, or D * lam for
+// the damage mode) crosses band edges through LDS.  Residuals are injected
+// on the whole region (its halo cells feed the later steps), the gradient
+// terms only on the output tile, where they are summed over the launch's
+// steps in registers: one butterfly for the constant-gradient partials (one
+// slot per wavefront) and one read-modify-write of the per-cell accumulators
+// per launch.  The reverse recurrence is that of adj_tile.
+// --------------------------------------------------------------------------
+template <int MODE, int V, int RW, int KS>
+__global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
+#pragma clang fp contract(fast)
+  constexpr int kCols = kWave * V;
+  constexpr int OR = kWavesPerBlock * RW - 2 * KS;
+  constexpr int OC = kCols - 2 * KS;
+  constexpr bool kD = (MODE == kDField);
+  __shared__ float xch[2][kWavesPerBlock][2][2][kCols];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x >> 6;
+  const int chunk = (a.n_blocks + 7) >> 3;
+  const int t = (blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+  if (t >= a.n_blocks) return;
+  const BlockDev bl = a.blocks[t];
+  const InstDev I = a.inst[bl.inst];
+  if (a.m0 - a.ks + 1 > I.nt) return;   // this instance's reverse recurrence has not started
+  const GCall C = global_call(*a.call);
+
+  const uint32_t P = (uint32_t)I.pitch;
+  gfloat* pool = C.ws + I.pool_off;
+  const gfloat* coef = C.ws + I.coef_off;
+  const int gi0 = bl.oi0 - KS + w * RW;
+  const int gj0 = bl.oj0 - KS + lane * V;
+  const float kdt = (float)I.gscale;
+  const double gsa = I.gscale * (double)I.hinv2;
+  float c0[6];
+#pragma unroll
+  for (int p = 0; p < 6; ++p) c0[p] = C.params[bl.set * 6 + p];
+  const OpCoef kl = op_coef(c0, gsa);    // (dt^2/rho) A(C0) on the (weighted) adjoint field
+  const OpCoef ku = op_coef(c0, 1.0);    // A(C0) / a on u (dJ/dD)
+
+  // ---- loads: lam^{m0+1}, lam^{m0+2}, D; partial slot and accumulators
+  const gfloat* l1 = pool + (int64_t)a.s_l1 * 2 * I.field;
+  const gfloat* l2 = pool + (int64_t)a.s_l2 * 2 * I.field;
+  float cx[RW][V], cy[RW][V], px[RW][V], py[RW][V], dd[RW][V];
+  bool inside[RW][V], owned[RW][V];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int ci = min(max(gi0 + r, -1), I.nx);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const int cj = min(max(gj0 + e, -1), I.ny);
+      const uint32_t o = row_off(P, ci, cj);
+      cx[r][e] = *at(l1, o);
+      cy[r][e] = *at(l1 + I.field, o);
+      px[r][e] = *at(l2, o);
+      py[r][e] = *at(l2 + I.field, o);
+      dd[r][e] = kD ? *at(coef, o) : 1.f;
+      const int gi = gi0 + r, gj = gj0 + e;
+      inside[r][e] = gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny;
+      owned[r][e] = inside[r][e] && gi >= bl.oi0 && gi < bl.oi0 + OR && gj >= bl.oj0 && gj < bl.oj0 + OC;
+    }
+  }
+  gdouble* part = reinterpret_cast<gdouble*>(C.ws) + a.partial_off +
+                  ((int64_t)t * kWavesPerBlock + w) * kPartials + (lane >> 3);
+  const double part_in = *part;
+  // per-cell damage accumulator: zero on the launch holding the segment's first step
+  const int top = min(a.seg_top, I.nt);
+  const bool seg_first = a.m0 >= top;
+  const bool seg_last = (a.m0 - a.ks + 1 == a.seg_bot);
+  gfloat* acc32 = C.ws + I.acc_off;
+  float gacc[RW][V];
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      gacc[r][e] = 0.f;
+      if (kD && !seg_first && owned[r][e]) gacc[r][e] = *at(acc32, row_off(P, gi0 + r, gj0 + e));
+    }
+  if (a.store_in >= 0) {   // a one-step launch also keeps its input state for the next launch
+    gfloat* dx = pool + (int64_t)a.store_in * 2 * I.field;
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+        if (owned[r][e]) {
+          const uint32_t o = row_off(P, gi0 + r, gj0 + e);
+          *at(dx, o) = cx[r][e];
+          *at(dx + I.field, o) = cy[r][e];
+        }
+  }
+
+  auto publish = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      xch[buf][w][0][0][lane * V + e] = dd[0][e] * cx[0][e];
+      xch[buf][w][0][1][lane * V + e] = dd[0][e] * cy[0][e];
+      xch[buf][w][1][0][lane * V + e] = dd[RW - 1][e] * cx[RW - 1][e];
+      xch[buf][w][1][1][lane * V + e] = dd[RW - 1][e] * cy[RW - 1][e];
+    }
+  };
+  publish(0);
+  __syncthreads();
+
+  float S[kPartials];
+#pragma unroll
+  for (int p = 0; p < kPartials; ++p) S[p] = 0.f;
+  const int nf = I.n_frames;
+
+  for (int s = 0; s < a.ks; ++s) {
+    const int m = a.m0 - s;
+    const int buf = s & 1;
+    const bool act = m <= I.nt;   // block-uniform
+    float nx_[RW][V], ny_[RW][V];
+    if (act) {
+      // ---- u^m of the lane's cells (rows -1 .. RW with column halos for the damage mode)
+      // (output cells never sit on a region edge, so the edge lanes need no halo)
+      const gfloat* umx = pool + (int64_t)a.u_slot[s] * 2 * I.field;
+      constexpr int RU = kD ? RW + 2 : RW;
+      RawRow<V> urx[RU], ury[RU];
+#pragma unroll
+      for (int r = 0; r < RU; ++r) {
+        const int ci = min(max(gi0 + r - (kD ? 1 : 0), -1), I.nx);
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const uint32_t o = row_off(P, ci, min(max(gj0 + e, -1), I.ny));
+          urx[r].c[e] = *at(umx, o);
+          ury[r].c[e] = *at(umx + I.field, o);
+        }
+        urx[r].h = ury[r].h = 0.f;
+      }
+
+      // ---- stencil rows of the (weighted) adjoint field: own rows and the band-edge rows
+      Row<V> xr[RW + 2], yr[RW + 2];
+      {
+        RawRow<V> tx, ty, bx, by;
+        tx.h = ty.h = bx.h = by.h = 0.f;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          tx.c[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
+          ty.c[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
+          bx.c[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
+          by.c[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+        }
+        xr[0] = make_row<V>(tx, lane);
+        yr[0] = make_row<V>(ty, lane);
+        xr[RW + 1] = make_row<V>(bx, lane);
+        yr[RW + 1] = make_row<V>(by, lane);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          RawRow<V> rx, ry;
+          rx.h = ry.h = 0.f;
+#pragma unroll
+          for (int e = 0; e < V; ++e) { rx.c[e] = dd[r][e] * cx[r][e]; ry.c[e] = dd[r][e] * cy[r][e]; }
+          xr[r + 1] = make_row<V>(rx, lane);
+          yr[r + 1] = make_row<V>(ry, lane);
+        }
+      }
+      Row<V> uxr[kD ? RW + 2 : 1], uyr[kD ? RW + 2 : 1];
+      if constexpr (kD) {
+#pragma unroll
+        for (int r = 0; r < RW + 2; ++r) {
+          uxr[r] = make_row<V>(urx[r], lane);
+          uyr[r] = make_row<V>(ury[r], lane);
+        }
+      }
+
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const int x = e + 1;
+          const UDeriv dl = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], x);
+          apply_op_u(kl, dl, __builtin_fmaf(2.f, cx[r][e], -px[r][e]), __builtin_fmaf(2.f, cy[r][e], -py[r][e]),
+                     nx_[r][e], ny_[r][e]);
+          // gradient terms on the output tile (lam^{m+1} with u^m)
+          float ux = kD ? uxr[r + 1].v[x] : urx[r].c[e];
+          float uy = kD ? uyr[r + 1].v[x] : ury[r].c[e];
+          if (!owned[r][e]) { ux = 0.f; uy = 0.f; }
+          sbp_sums(dl, ux, uy, S);
+          if constexpr (kD) {
+            const UDeriv du = uderivs<V>(uxr[r], uxr[r + 1], uxr[r + 2], uyr[r], uyr[r + 1], uyr[r + 2], x);
+            float ox, oy;
+            apply_op_sum(ku, du, ox, oy);
+            if (owned[r][e]) gacc[r][e] += __builtin_fmaf(cx[r][e], ox, cy[r][e] * oy);
+          }
+        }
+
+      // residual injection on recorded steps m = kP + 1 (the whole region)
+      if (((m - 1) % a.stride == 0) && ((m - 1) / a.stride < nf)) {
+        const int k = (m - 1) / a.stride;
+        if (a.snapshot) {
+#pragma unroll
+          for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              if (inside[r][e]) {
+                const gfloat* gx = C.gtr + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + gj0 + e;
+                nx_[r][e] += gx[0];
+                ny_[r][e] += gx[(int64_t)nf * I.nx * I.ny];
+              }
+        } else {
+          for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
+            const RecEv ev = a.rec[rp];
+            const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
+            if (rr < 0 || rr >= RW || lane != lc / V) continue;
+            const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+            const float vgx = gx[0], vgy = gx[(int64_t)nf * I.n_rec];
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
+#pragma unroll
+              for (int e = 0; e < V; ++e)
+                if (r == rr && e == lc % V) {
+                  nx_[r][e] += vgx;
+                  ny_[r][e] += vgy;
+                }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (!inside[r][e]) { nx_[r][e] = 0.f; ny_[r][e] = 0.f; }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) { nx_[r][e] = 0.f; ny_[r][e] = 0.f; }
+    }
+
+    const int slot = a.store[s];
+    if (slot >= 0) {
+      gfloat* dx = pool + (int64_t)slot * 2 * I.field;
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (owned[r][e]) {
+            const uint32_t o = row_off(P, gi0 + r, gj0 + e);
+            *at(dx, o) = nx_[r][e];
+            *at(dx + I.field, o) = ny_[r][e];
+          }
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        px[r][e] = cx[r][e];
+        py[r][e] = cy[r][e];
+        cx[r][e] = nx_[r][e];
+        cy[r][e] = ny_[r][e];
+      }
+    if (s + 1 < a.ks) {
+      publish(buf ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // constant-gradient partials of this wavefront (slot t * 4 + w; lane 8q holds slot q)
+  {
+    double Sd[kPartials];
+#pragma unroll
+    for (int p = 0; p < kPartials; ++p) Sd[p] = (double)S[p];
+    const double tot = wave_sum8(Sd, lane);
+    const int q = lane >> 3;
+    const double wq = (q == 2 || q >= 6) ? 0.25 * gsa : gsa;
+    if ((lane & 7) == 0) *part = part_in + wq * tot;
+  }
+  if constexpr (kD) {
+    gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+        if (owned[r][e]) {
+          const uint32_t o = row_off(P, gi0 + r, gj0 + e);
+          if (seg_last) acc64[o] += gsa * (double)gacc[r][e];
+          else *at(acc32, o) = gacc[r][e];
+        }
+  }
+}
+
+// --------------------------------------------------------------------------
 // Small helpers
 // --------------------------------------------------------------------------
 __global__ void set_call_kernel(CallDev* dst, CallDev v) { *dst = v; }
@@ -1133,8 +1418,10 @@ __global__ void pack_fields_kernel(const CallDev* call, const InstDev* inst, con
 }
 
 // grad_params[set][p] = sum over the set's tiles of partial[t][p]
+// per_rec partial slots per record (tile: 1, block: kWavesPerBlock)
+template <typename Rec>
 __global__ void finalize_const_kernel(const CallDev* call, int64_t partial_off,
-                                      const TileDev* __restrict__ tiles, int n_tiles) {
+                                      const Rec* __restrict__ tiles, int n_tiles, int per_rec) {
   const int set = blockIdx.x / 6, p = blockIdx.x % 6;
   const double* partial = reinterpret_cast<const double*>(call->ws) + partial_off;
   // dJ/dC = (s0, s1, s2, s3 + 2 s6, s4 + 2 s7, s5 + s2)
@@ -1142,8 +1429,8 @@ __global__ void finalize_const_kernel(const CallDev* call, int64_t partial_off,
   const double f2 = (p == 5) ? 1.0 : 2.0;
   __shared__ double red[kBlock];
   double s = 0.0;
-  for (int t = threadIdx.x; t < n_tiles; t += blockDim.x)
-    if (tiles[t].set == set) {
+  for (int t = threadIdx.x; t < n_tiles * per_rec; t += blockDim.x)
+    if (tiles[t / per_rec].set == set) {
       const double* pt = partial + (int64_t)t * kPartials;
       s += (p2 >= 0) ? pt[p] + f2 * pt[p2] : pt[p];
     }
@@ -1292,6 +1579,34 @@ hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st)
   return hipErrorInvalidValue;
 }
 
+// Time-blocked adjoint shapes (constant and damage modes; register limits
+// checked by the build).
+#define ERNN_ADJ_BLOCK_CASES_DFIELD(X) X(1, 4, 2) X(1, 4, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 8)
+#define ERNN_ADJ_BLOCK_CASES(X) ERNN_ADJ_BLOCK_CASES_DFIELD(X) X(2, 4, 2) X(2, 4, 4)
+#define ERNN_ADJ_BLOCK_CASE(v_, r_, k_)                                                             \
+  if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
+    hipLaunchKernelGGL((adj_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kBlock), 0, st, a);  \
+    return hipGetLastError();                                                                      \
+  }
+template <int MODE>
+static hipError_t launch_adj_block_m(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
+  if constexpr (MODE == kConst6) {
+    ERNN_ADJ_BLOCK_CASES(ERNN_ADJ_BLOCK_CASE)
+  } else {
+    ERNN_ADJ_BLOCK_CASES_DFIELD(ERNN_ADJ_BLOCK_CASE)
+  }
+  return hipErrorInvalidValue;
+}
+#undef ERNN_ADJ_BLOCK_CASE
+
+hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
+  switch (a.mode) {
+    case kConst6: return launch_adj_block_m<kConst6>(a, V, grid, st);
+    case kDField: return launch_adj_block_m<kDField>(a, V, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
 // loop: the grid is capped below the tile count (persistent wavefronts)
 hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st) {
   switch (a.mode) {
@@ -1344,8 +1659,15 @@ hipError_t launch_pack_fields(const CallDev* call, const InstDev* inst, const in
 
 hipError_t launch_finalize_const(const CallDev* call, int64_t partial_off, const TileDev* tiles,
                                  int n_tiles, int n_sets, hipStream_t st) {
-  hipLaunchKernelGGL(finalize_const_kernel, dim3(n_sets * 6), dim3(kBlock), 0, st, call, partial_off,
-                     tiles, n_tiles);
+  hipLaunchKernelGGL(finalize_const_kernel<TileDev>, dim3(n_sets * 6), dim3(kBlock), 0, st, call, partial_off,
+                     tiles, n_tiles, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize_const_blocks(const CallDev* call, int64_t partial_off, const BlockDev* blocks,
+                                        int n_blocks, int n_sets, hipStream_t st) {
+  hipLaunchKernelGGL(finalize_const_kernel<BlockDev>, dim3(n_sets * 6), dim3(kBlock), 0, st, call, partial_off,
+                     blocks, n_blocks, kWavesPerBlock);
   return hipGetLastError();
 }
 

@@ -34,6 +34,9 @@ hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, bool loop, hipS
 hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st);
 hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t st);
 hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st);
+hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st);
+hipError_t launch_finalize_const_blocks(const CallDev* call, int64_t partial_off, const BlockDev* blocks,
+                                        int n_blocks, int n_sets, hipStream_t st);
 hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st);
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t st);
 hipError_t launch_fill_ws(const CallDev* call, int64_t off, int64_t n, hipStream_t st);
@@ -101,6 +104,13 @@ bool block_ok(int mode, int V, int RW, int KS) {
   if (kWavesPerBlock * RW - 2 * KS <= 0) return false;
   return !(mode != ERNN_PARAM_CONST6 && V == 2 && RW == 8);
 }
+// Time-blocked adjoint shapes (ERNN_ADJ_BLOCK_CASES): constant and damage
+// modes; V = 2 (RW = 4) for constant stiffness only.
+bool adj_block_ok(int mode, int V, int RW, int KS) {
+  if (mode == ERNN_PARAM_FIELD6 || !block_ok(mode, V, RW, KS)) return false;
+  if (V == 2) return mode == ERNN_PARAM_CONST6 && RW == 4;
+  return true;
+}
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
@@ -136,7 +146,7 @@ struct Plan {
   InstDev* d_inst = nullptr;
   CallDev* d_call = nullptr;
   Tiling fwd, adj;
-  BlockTiling blk;
+  BlockTiling blk, blk_adj;
   int64_t ws_floats = 0;
   int64_t partial_off = 0;  // in doubles
   int64_t partial_cap = 0;  // tiles
@@ -152,13 +162,13 @@ struct Plan {
   hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;
   hipStream_t cap_stream = nullptr;
 
-  // slots of one instance's state pool: ring (4), checkpoints, segment bank(s), lam ring
+  // slots of one instance's state pool: ring (4), checkpoints, segment bank(s), lam ring (4)
   int banks() const { return fuse ? 2 : 1; }
   int ring(int r) const { return r; }
   int ck(int s, int q) const { return 4 + 2 * s + q; }
   int seg(int bank, int q) const { return 4 + 2 * S + bank * K + q; }
   int lam(int r) const { return 4 + 2 * S + banks() * K + r; }
-  int n_slots() const { return 4 + 2 * S + banks() * K + 3; }
+  int n_slots() const { return 4 + 2 * S + banks() * K + 4; }
   int bank_of(int s) const { return fuse ? (s & 1) : 0; }
 
   int slot_fwd(int m) const {
@@ -317,14 +327,18 @@ int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
 // Output tiles of (4 RW - 2 KS) x (64 V - 2 KS) cells per instance, row-major;
 // each block's source list covers its whole region (the halo cells' states
 // feed the later steps), its receiver list only its output tile.
-int build_blocking(Plan* p, int V, int RW, int KS) {
+int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
+  const bool adjoint = bt == &p->blk_adj;
   if (KS <= 1) {
-    free_blocking(p->blk);
+    free_blocking(*bt);
     return ERNN_OK;
   }
-  if (!block_ok(p->mode, V, RW, KS))
+  if (!adjoint && !block_ok(p->mode, V, RW, KS))
     return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
                                 "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
+  if (adjoint && !adj_block_ok(p->mode, V, RW, KS))
+    return fail(ERNN_E_INVALID, "time-blocked adjoint shape (V, RW, KS): constant or damage mode, V = 1 with "
+                                "RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or (constant) V = 2, RW = 4, KS in {2,4}");
   const int OR = kWavesPerBlock * RW - 2 * KS, OC = kWave * V - 2 * KS;
   std::vector<BlockDev> blocks;
   std::vector<SrcEv> sev;
@@ -353,7 +367,18 @@ int build_blocking(Plan* p, int V, int RW, int KS) {
     if (!p->snapshot) {
       for (int e = 0; e < I.rec_count; ++e) {
         const ernn_receiver_t& r = p->rec_h[I.rec_begin + e];
-        re.push_back(Ev{(int64_t)(r.i / OR) * nbj + r.j / OC, r.i, r.j, e});
+        if (!adjoint) {   // recording: the output tile holding the receiver
+          re.push_back(Ev{(int64_t)(r.i / OR) * nbj + r.j / OC, r.i, r.j, e});
+          continue;
+        }
+        // injection: every block whose region holds the receiver
+        for (int bi = std::max(0, (r.i + KS) / OR - 2); bi <= std::min(nbi - 1, (r.i + KS) / OR); ++bi) {
+          if (r.i < bi * OR - KS || r.i >= (bi + 1) * OR + KS) continue;
+          for (int bj = std::max(0, (r.j + KS) / OC - 2); bj <= std::min(nbj - 1, (r.j + KS) / OC); ++bj) {
+            if (r.j < bj * OC - KS || r.j >= (bj + 1) * OC + KS) continue;
+            re.push_back(Ev{(int64_t)bi * nbj + bj, r.i, r.j, e});
+          }
+        }
       }
     }
     std::stable_sort(se.begin(), se.end(), by_key);
@@ -384,15 +409,17 @@ int build_blocking(Plan* p, int V, int RW, int KS) {
     }
   }
   if (blocks.size() > ((size_t)1 << 30)) return fail(ERNN_E_CAPACITY, "too many blocks");
-  free_blocking(p->blk);
-  p->blk.V = V;
-  p->blk.RW = RW;
-  p->blk.KS = KS;
-  p->blk.n_blocks = (int)blocks.size();
-  p->blk.grid = 8 * ((p->blk.n_blocks + 7) / 8);
-  ERNN_TRY(dev_upload(&p->blk.d_blocks, blocks));
-  ERNN_TRY(dev_upload(&p->blk.d_src, sev));
-  ERNN_TRY(dev_upload(&p->blk.d_rec, rev));
+  if (adjoint && (int64_t)blocks.size() * kWavesPerBlock > p->partial_cap)
+    return fail(ERNN_E_CAPACITY, "time-blocked adjoint needs more partial slots than the workspace holds");
+  free_blocking(*bt);
+  bt->V = V;
+  bt->RW = RW;
+  bt->KS = KS;
+  bt->n_blocks = (int)blocks.size();
+  bt->grid = 8 * ((bt->n_blocks + 7) / 8);
+  ERNN_TRY(dev_upload(&bt->d_blocks, blocks));
+  ERNN_TRY(dev_upload(&bt->d_src, sev));
+  ERNN_TRY(dev_upload(&bt->d_rec, rev));
   return ERNN_OK;
 }
 
@@ -402,6 +429,7 @@ void destroy_plan(Plan* p) {
   free_tiling(p->fwd);
   free_tiling(p->adj);
   free_blocking(p->blk);
+  free_blocking(p->blk_adj);
   if (p->d_inst) (void)hipFree(p->d_inst);
   if (p->d_call) (void)hipFree(p->d_call);
   if (p->d_set_first) (void)hipFree(p->d_set_first);
@@ -431,6 +459,9 @@ int validate(const ernn_desc_t* d) {
   if (d->blk_ks > 1 && !block_ok(d->param_mode, d->blk_v, d->blk_rw, d->blk_ks))
     return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
                                 "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
+  if (d->ablk_ks > 1 && !adj_block_ok(d->param_mode, d->ablk_v, d->ablk_rw, d->ablk_ks))
+    return fail(ERNN_E_INVALID, "time-blocked adjoint shape (V, RW, KS): constant or damage mode, V = 1 with "
+                                "RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or (constant) V = 2, RW = 4, KS in {2,4}");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))
@@ -591,7 +622,11 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
       off += round_up((int64_t)2 * p->n_acc * p->inst[b].field, 64);
     }
   }
-  p->partial_cap = tiles_for(*p, 1, 1);
+  {
+    int64_t nb = 0;   // blocks of the smallest time-blocked output tile (8 x 48 cells), 4 slots each
+    for (const auto& I : p->inst_h) nb += (int64_t)((I.nx + 7) / 8) * ((I.ny + 47) / 48);
+    p->partial_cap = std::max<int64_t>(tiles_for(*p, 1, 1), kWavesPerBlock * nb);
+  }
   p->partial_off = off / 2;
   off += round_up(2 * kPartials * p->partial_cap, 64);
   p->ws_floats = off;
@@ -607,7 +642,8 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 1, d->fwd_r ? d->fwd_r : 4, d->fwd_bpc));
   const int adj_r0 = (d->param_mode == ERNN_PARAM_FIELD6) ? 2 : 4;
   ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 1, d->adj_r ? d->adj_r : adj_r0, d->adj_bpc));
-  ERNN_TRY(build_blocking(p, d->blk_v, d->blk_rw, d->blk_ks));
+  ERNN_TRY(build_blocking(p, &p->blk, d->blk_v, d->blk_rw, d->blk_ks));
+  ERNN_TRY(build_blocking(p, &p->blk_adj, d->ablk_v, d->ablk_rw, d->ablk_ks));
   ERNN_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
   return ERNN_OK;
 }
@@ -707,6 +743,44 @@ int launch_blk_recompute(const Plan* p, BlockArgs& a, int s, int n0, int top, hi
   return ERNN_OK;
 }
 
+BlockAdjArgs block_adj_args(const Plan* p) {
+  BlockAdjArgs a{};
+  a.inst = p->d_inst;
+  a.blocks = p->blk_adj.d_blocks;
+  a.rec = p->blk_adj.d_rec;
+  a.call = p->d_call;
+  a.partial_off = p->partial_off;
+  a.n_blocks = p->blk_adj.n_blocks;
+  a.mode = p->mode;
+  a.stride = p->P;
+  a.snapshot = p->snapshot;
+  a.rw = p->blk_adj.RW;
+  a.ksmax = p->blk_adj.KS;
+  return a;
+}
+
+// Time-blocked adjoint launch L over m0 .. m0 - ks + 1: reads lam^{m0+1},
+// lam^{m0+2} from lam pair L & 1 (slots 1, 0), leaves its last two states in
+// pair (L + 1) & 1; u^m comes from slot u_of(m).
+template <typename UOf>
+int launch_blk_adjoint(const Plan* p, BlockAdjArgs& a, int L, int m0, int ks, int seg_top, int seg_bot, UOf u_of,
+                       hipStream_t st) {
+  const int pr = 2 * (L & 1), pw = 2 * ((L + 1) & 1);
+  a.m0 = m0;
+  a.ks = ks;
+  a.s_l1 = p->lam(pr + 1);
+  a.s_l2 = p->lam(pr);
+  a.store_in = ks == 1 ? p->lam(pw) : -1;
+  for (int j = 0; j < kMaxBlockSteps; ++j) {
+    a.store[j] = j == ks - 1 ? p->lam(pw + 1) : (j == ks - 2 ? p->lam(pw) : -1);
+    a.u_slot[j] = j < ks ? u_of(m0 - j) : 0;
+  }
+  a.seg_top = seg_top;
+  a.seg_bot = seg_bot;
+  ERNN_HIP(launch_adj_block(a, p->blk_adj.V, p->blk_adj.grid, st));
+  return ERNN_OK;
+}
+
 int enqueue_forward(const Plan* p, hipStream_t st) {
   ERNN_TRY(enqueue_pack(p, st));
   if (p->blk.KS > 1) {
@@ -732,11 +806,11 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
   // zero: lam ring, per-tile partials, fp64 per-cell accumulators
   for (int b = 0; b < p->n_inst; ++b) {
     const InstDev& D = p->inst[b];
-    ERNN_HIP(launch_fill_ws(p->d_call, D.pool_off + (int64_t)p->lam(0) * 2 * D.field, 3 * 2 * D.field, st));
+    ERNN_HIP(launch_fill_ws(p->d_call, D.pool_off + (int64_t)p->lam(0) * 2 * D.field, 4 * 2 * D.field, st));
     if (p->n_acc)
       ERNN_HIP(launch_fill_ws(p->d_call, 2 * D.acc64_off, 2 * (int64_t)p->n_acc * D.field, st));
   }
-  ERNN_HIP(launch_fill_ws(p->d_call, 2 * p->partial_off, 2 * kPartials * (int64_t)p->adj.n_tiles, st));
+  ERNN_HIP(launch_fill_ws(p->d_call, 2 * p->partial_off, 2 * kPartials * p->partial_cap, st));
 
   StepArgs f = base_args(p, p->fwd);
   f.record = 0;
@@ -764,7 +838,17 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
     }
     return ERNN_OK;
   };
-  if (!p->fuse) {
+  if (!p->fuse && p->blk_adj.KS > 1) {
+    BlockAdjArgs ba = block_adj_args(p);
+    int L = 0;
+    for (int s = p->S - 1; s >= 0; --s) {
+      const int top = seg_top(s), bot = s * p->K + 1;
+      ERNN_TRY(recompute_segment(s));
+      for (int m0 = top; m0 >= bot; m0 -= p->blk_adj.KS, ++L)
+        ERNN_TRY(launch_blk_adjoint(p, ba, L, m0, std::min(p->blk_adj.KS, m0 - bot + 1), top, bot,
+                                    [&](int m) { return p->slot_seg(s, m); }, st));
+    }
+  } else if (!p->fuse) {
     for (int s = p->S - 1; s >= 0; --s) {
       const int top = seg_top(s), bot = s * p->K + 1;
       ERNN_TRY(recompute_segment(s));
@@ -813,8 +897,13 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
       }
     }
   }
-  if (p->mode != ERNN_PARAM_FIELD6)
-    ERNN_HIP(launch_finalize_const(p->d_call, p->partial_off, p->adj.d_tiles, p->adj.n_tiles, p->n_sets, st));
+  if (p->mode != ERNN_PARAM_FIELD6) {
+    if (!p->fuse && p->blk_adj.KS > 1)
+      ERNN_HIP(launch_finalize_const_blocks(p->d_call, p->partial_off, p->blk_adj.d_blocks, p->blk_adj.n_blocks,
+                                            p->n_sets, st));
+    else
+      ERNN_HIP(launch_finalize_const(p->d_call, p->partial_off, p->adj.d_tiles, p->adj.n_tiles, p->n_sets, st));
+  }
   if (p->n_acc) {
     for (int s = 0; s < p->n_sets; ++s) {
       const auto& I = p->inst_h[p->set_first[s]];
@@ -912,6 +1001,10 @@ int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
   info->blk_rw = p->blk.RW;
   info->blk_ks = p->blk.KS;
   info->blk_blocks = p->blk.n_blocks;
+  info->ablk_v = p->blk_adj.V;
+  info->ablk_rw = p->blk_adj.RW;
+  info->ablk_ks = p->blk_adj.KS;
+  info->ablk_blocks = p->blk_adj.n_blocks;
   return ERNN_OK;
 }
 
@@ -955,22 +1048,23 @@ int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc) {
   return build_tiling(p, which == 0 ? &p->fwd : &p->adj, v, r, bpc);
 }
 
-int ernn_plan_set_blocking(void* plan, int v, int rw, int ks) {
-  if (!plan) return fail(ERNN_E_INVALID, "null plan");
+int ernn_plan_set_blocking(void* plan, int which, int v, int rw, int ks) {
+  if (!plan || (which != 0 && which != 1)) return fail(ERNN_E_INVALID, "bad plan or which");
   Plan* p = static_cast<Plan*>(plan);
   ERNN_HIP(hipSetDevice(p->device));
   ERNN_HIP(hipDeviceSynchronize());
   drop_graphs(p);
-  return build_blocking(p, v, rw, ks);
+  return build_blocking(p, which == 0 ? &p->blk : &p->blk_adj, v, rw, ks);
 }
 
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
                     float* ms_out) {
-  if (!plan || !ms_out || steps < 1 || which < 0 || which > 3) return fail(ERNN_E_INVALID, "bad argument");
+  if (!plan || !ms_out || steps < 1 || which < 0 || which > 4) return fail(ERNN_E_INVALID, "bad argument");
   Plan* p = static_cast<Plan*>(plan);
   if (which == 2 && !p->fuse) return fail(ERNN_E_INVALID, "fused timing needs a plan with fuse = 1");
   if (which == 3 && p->blk.KS <= 1) return fail(ERNN_E_INVALID, "time-blocked timing needs blk_ks > 1");
-  ERNN_TRY(check_call(p, call, which == 1 || which == 2, /*grads=*/false));  // timing never finalizes
+  if (which == 4 && p->blk_adj.KS <= 1) return fail(ERNN_E_INVALID, "time-blocked adjoint timing needs ablk_ks > 1");
+  ERNN_TRY(check_call(p, call, which == 1 || which == 2 || which == 4, /*grads=*/false));  // never finalizes
   hipStream_t st = static_cast<hipStream_t>(stream);
   ERNN_HIP(hipSetDevice(p->device));
   ERNN_HIP(launch_set_call(p->d_call, to_dev(call), st));
@@ -979,7 +1073,16 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   ERNN_HIP(hipEventCreate(&e1));
   ERNN_HIP(hipEventRecord(e0, st));
   const int nsteps = std::min(steps, p->nt_max);
-  if (which == 3) {   // `steps` time-blocked forward launches of KS steps each
+  if (which == 4) {   // `steps` time-blocked adjoint launches of KS steps each, u^m from segment bank 0
+    BlockAdjArgs ba = block_adj_args(p);
+    const int KS = p->blk_adj.KS;
+    const int launches = std::min(steps, (p->nt_max + KS - 1) / KS);
+    for (int L = 0; L < launches; ++L) {
+      const int m0 = p->nt_max - L * KS;
+      ERNN_TRY(launch_blk_adjoint(p, ba, L, m0, std::min(KS, m0), p->nt_max, 1,
+                                  [&](int m) { return p->seg(0, m % p->K); }, st));
+    }
+  } else if (which == 3) {   // `steps` time-blocked forward launches of KS steps each
     BlockArgs b = block_args(p);
     b.record = 1;
     const int launches = std::min(steps, (p->nt_max + p->blk.KS - 1) / p->blk.KS);

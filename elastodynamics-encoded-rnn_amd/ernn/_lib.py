@@ -41,7 +41,8 @@ class Desc(C.Structure):
                 ("fwd_v", C.c_int32), ("fwd_r", C.c_int32), ("adj_v", C.c_int32),
                 ("adj_r", C.c_int32), ("use_graph", C.c_int32), ("fwd_bpc", C.c_int32),
                 ("adj_bpc", C.c_int32), ("fuse", C.c_int32),
-                ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32), ("reserved", C.c_int32),
+                ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
+                ("ablk_v", C.c_int32), ("ablk_rw", C.c_int32), ("ablk_ks", C.c_int32),
                 ("inst", C.POINTER(Instance)), ("src", C.POINTER(Source)), ("n_src", C.c_int64),
                 ("rec", C.POINTER(Receiver)), ("n_rec", C.c_int64)]
 
@@ -55,7 +56,8 @@ class PlanInfo(C.Structure):
                 ("use_graph", C.c_int32), ("fwd_bpc", C.c_int32), ("adj_bpc", C.c_int32),
                 ("fwd_blocks", C.c_int32), ("adj_blocks", C.c_int32), ("n_cu", C.c_int32),
                 ("fuse", C.c_int32), ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
-                ("blk_blocks", C.c_int32)]
+                ("blk_blocks", C.c_int32), ("ablk_v", C.c_int32), ("ablk_rw", C.c_int32),
+                ("ablk_ks", C.c_int32), ("ablk_blocks", C.c_int32)]
 
 
 class Call(C.Structure):
@@ -75,7 +77,7 @@ EXPORTS = {
     "ernn_forward": ([C.c_void_p, C.POINTER(Call), C.c_void_p], C.c_int),
     "ernn_adjoint": ([C.c_void_p, C.POINTER(Call), C.c_void_p], C.c_int),
     "ernn_plan_set_tiling": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
-    "ernn_plan_set_blocking": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
+    "ernn_plan_set_blocking": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "ernn_time_steps": ([C.c_void_p, C.c_int, C.c_int, C.POINTER(Call), C.c_void_p,
                          C.POINTER(C.c_float)], C.c_int),
 }

@@ -2,9 +2,9 @@
 
 Checkpoint interval K.  Each instance keeps 4 ring slots, 2 states per
 checkpoint (S = ceil(nt / K) checkpoints), K recompute slots per bank (two
-banks when the adjoint is fused with the next segment's recompute) and 3
+banks when the adjoint is fused with the next segment's recompute) and 4
 adjoint slots, each slot two padded fields.  Recompute cost is one extra
-forward whatever K is, so K only trades memory: slots(K) = 7 + 2 ceil(nt/K) +
+forward whatever K is, so K only trades memory: slots(K) = 8 + 2 ceil(nt/K) +
 banks K is minimal near K = sqrt(2 nt / banks).  ``choose_ckpt`` returns that
 K, or raises if even the minimum exceeds the cap.
 
@@ -33,7 +33,7 @@ def padded_field_floats(nx: int, ny: int) -> int:
 
 
 def slots(nt: int, K: int, banks: int = 1) -> int:
-    return 7 + 2 * math.ceil(nt / K) + banks * K
+    return 8 + 2 * math.ceil(nt / K) + banks * K
 
 
 def state_bytes(survey: Survey, K: int) -> int:
@@ -65,6 +65,9 @@ DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
 # time-blocked forward shapes (V, RW, KS); field modes leave out (V, RW) = (2, 8)
 BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = (
     (1, 4, 2), (1, 4, 4), (1, 8, 2), (1, 8, 4), (1, 8, 8), (2, 4, 2), (2, 4, 4), (2, 8, 2), (2, 8, 4), (2, 8, 8))
+# time-blocked adjoint shapes (constant and damage modes; V = 2 for constant stiffness only)
+ADJ_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 8, 2), (1, 8, 4), (1, 8, 8),
+                                                      (2, 4, 2), (2, 4, 4))
 # largest V * R and R of the built adjoint kernels per mode (larger shapes would spill registers)
 ADJ_MAX_CELLS = {"const6": 16, "dfield": 8, "field6": 4}
 ADJ_MAX_ROWS = {"const6": 16, "dfield": 4, "field6": 2}
@@ -107,32 +110,39 @@ def tune_tiles(engine, params, field, wavelets, steps: int = 50,
     return result
 
 
-def tune_blocking(engine, params, field, wavelets, steps: int = 64,
-                  shapes: Sequence[Tuple[int, int, int]] = BLOCK_SHAPES, rounds: int = 5) -> tuple:
-    """Time the time-blocked forward shapes against the current single-step
-    forward tiling, per step, in interleaved rounds; keep the fastest.
+def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int = 0,
+                  shapes: Optional[Sequence[Tuple[int, int, int]]] = None, rounds: int = 5) -> tuple:
+    """Time the time-blocked forward (which 0) or unfused adjoint (which 1)
+    shapes against the current single-step tiling of that pass, per step, in
+    interleaved rounds; keep the fastest.
 
     Returns (V, RW, KS, ms_per_step, {shape: ms_per_step}); (0, 0, 0) is the
     single-step kernel."""
     mode = engine.survey.mode
-    cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or b[:2] != (2, 8)]
+    if which == 0:
+        shapes = BLOCK_SHAPES if shapes is None else shapes
+        cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or b[:2] != (2, 8)]
+    else:
+        shapes = ADJ_BLOCK_SHAPES if shapes is None else shapes
+        cands = [(0, 0, 0)] + ([b for b in shapes if mode == "const6" or b[0] == 1] if mode != "field6" else [])
     ws = engine.acquire_workspace()
     traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)
     samples = {c: [] for c in cands}
 
     def run(c):
         if c[2] <= 1:
-            return engine.time_steps(0, steps, params, field, wavelets, ws, traces, traces) / steps
+            return engine.time_steps(which, steps, params, field, wavelets, ws, traces, traces) / steps
         launches = max(2, steps // c[2])
-        return engine.time_steps(3, launches, params, field, wavelets, ws, traces, traces) / (launches * c[2])
+        return engine.time_steps(3 + which, launches, params, field, wavelets, ws, traces,
+                                 traces) / (launches * c[2])
 
     for _ in range(rounds):
         for c in cands:
-            engine.set_blocking(*c)
+            engine.set_blocking(*c, which=which)
             run(c)   # warm-up
             samples[c].append(run(c))
     times = {c: sorted(t)[len(t) // 2] for c, t in samples.items()}
     best = min(times, key=times.get)
-    engine.set_blocking(*best)
+    engine.set_blocking(*best, which=which)
     engine.release_workspace(ws)
     return (best[0], best[1], best[2], times[best], times)

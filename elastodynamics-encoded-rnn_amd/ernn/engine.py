@@ -73,6 +73,7 @@ class Engine:
         d.use_graph = int(sv.use_graph)
         d.fuse = int(sv.fused)
         d.blk_v, d.blk_rw, d.blk_ks = (int(x) for x in sv.blocking)
+        d.ablk_v, d.ablk_rw, d.ablk_ks = (int(x) for x in sv.adj_blocking)
         d.inst = inst
         d.src, d.n_src = src, len(src_list)
         d.rec, d.n_rec = rec, len(rec_list)
@@ -209,10 +210,16 @@ class Engine:
         i = self.info
         return (int(i.blk_v), int(i.blk_rw), int(i.blk_ks))
 
-    def set_blocking(self, v: int, rw: int, ks: int) -> None:
-        """Time-blocked forward (ks > 1) or one step per launch (ks <= 1)."""
+    @property
+    def adj_blocking(self) -> Tuple[int, int, int]:
+        i = self.info
+        return (int(i.ablk_v), int(i.ablk_rw), int(i.ablk_ks))
+
+    def set_blocking(self, v: int, rw: int, ks: int, which: int = 0) -> None:
+        """Time-blocked forward (which 0) or unfused adjoint (which 1) with ks
+        steps per launch (ks <= 1: one step per launch)."""
         with torch.cuda.device(self.device):
-            _lib.check(_lib.lib().ernn_plan_set_blocking(self._plan, v, rw, ks), "ernn_plan_set_blocking")
+            _lib.check(_lib.lib().ernn_plan_set_blocking(self._plan, which, v, rw, ks), "ernn_plan_set_blocking")
             _lib.check(_lib.lib().ernn_plan_info(self._plan, C.byref(self.info)), "ernn_plan_info")
 
     def set_tiling(self, which: int, v: int, r: int, bpc: int = 0) -> None:

@@ -183,6 +183,8 @@ class Survey:
     # Time-blocked forward (V, RW, KS): KS steps per launch on blocks of four
     # wavefronts x RW rows x 64 V columns; (0, 0, 0) = one step per launch.
     blocking: Tuple[int, int, int] = (0, 0, 0)
+    # Time-blocked adjoint of the unfused backward (V, RW, KS), const6 / dfield
+    adj_blocking: Tuple[int, int, int] = (0, 0, 0)
 
     def __post_init__(self):
         if self.mode not in MODES:

@@ -97,7 +97,9 @@ typedef struct {
                            launch.  V in {1,2}, RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS; field
                            modes exclude (V, RW) = (2, 8).  Used by the forward pass and the
                            segment recompute of the unfused adjoint. */
-  int32_t reserved;     /* 0 */
+  int32_t ablk_v, ablk_rw, ablk_ks; /* time-blocked adjoint of the unfused backward (CONST6,
+                           DFIELD): ablk_ks reverse steps per launch; V = 1 with RW in {4,8}, or
+                           (CONST6) V = 2 with RW = 4 and KS in {2,4}; ablk_ks <= 1 = off */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;
@@ -122,6 +124,8 @@ typedef struct {
   int32_t fuse;
   int32_t blk_v, blk_rw, blk_ks; /* time-blocked forward shape (blk_ks = 0: off) */
   int32_t blk_blocks;       /* blocks per time-blocked launch */
+  int32_t ablk_v, ablk_rw, ablk_ks; /* time-blocked adjoint shape (ablk_ks = 0: off) */
+  int32_t ablk_blocks;
 } ernn_plan_info_t;
 
 /* Per-call device pointers.
@@ -186,15 +190,16 @@ int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream);
  * synchronizes the device and drops captured graphs. */
 int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc);
 
-/* Switch the forward to blk_ks steps per launch on blocks of the given shape
- * (ks <= 1: one step per launch); same limits as ernn_desc_t.blk_*.  No
- * counterpart in the reference: the time loop of PI:268-305 is one step per
- * iteration.  Synchronizes the device and drops captured graphs. */
-int ernn_plan_set_blocking(void* plan, int v, int rw, int ks);
+/* Switch the forward (which = 0) or the unfused adjoint (which = 1) to ks
+ * steps per launch on blocks of the given shape (ks <= 1: one step per
+ * launch); same limits as ernn_desc_t.blk_* / ablk_*.  No counterpart in the
+ * reference: its time loop (PI:268-305) and autograd replay run one step at a
+ * time.  Synchronizes the device and drops captured graphs. */
+int ernn_plan_set_blocking(void* plan, int which, int v, int rw, int ks);
 
 /* Time `steps` forward (which = 0), adjoint (which = 1) or fused adjoint +
  * recompute (which = 2, fuse plans) steps with the current tiling, or `steps`
- * time-blocked forward launches of blk_ks steps each (which = 3), using
+ * time-blocked forward (which = 3) or adjoint (which = 4) launches, using
  * hipEvents on `stream`; synchronizes.  The workspace must be initialized.
  * ms_out receives the elapsed milliseconds. */
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
