@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--tiles", default="", help="fixed tiles 'Vf,Rf,Va,Ra,Bf,Ba' (skips autotuning)")
     ap.add_argument("--blocking", default="auto",
                     help="time-blocked forward 'V,RW,KS', '0' for one step per launch, or 'auto' (autotuned)")
+    ap.add_argument("--adj-blocking", default="auto",
+                    help="time-blocked unfused adjoint 'V,RW,KS', '0', or 'auto' (autotuned)")
     ap.add_argument("--baseline", type=int, default=1, help="time the autograd plain path on the GPU")
     ap.add_argument("--cpu", type=int, default=1, help="time the CPU port")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -280,11 +282,14 @@ def main():
                                                   steps=max(16, min(96, 40000 // cells_k)))
         log(f"autotune: forward blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
             f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f})")
-        if not sv.fused and sv.mode != "field6":
-            bv, brw, bks, bms, btimes = tune_blocking(eng, params_true, field_true, model.wavelets,
-                                                      steps=max(16, min(96, 40000 // cells_k)), which=1)
-            log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
-                f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f})")
+    if args.adj_blocking not in ("auto", "0"):
+        eng.set_blocking(*(int(x) for x in args.adj_blocking.split(",")), which=1)
+        log(f"time-blocked adjoint fixed: (V, RW, KS) = {eng.adj_blocking}")
+    elif args.adj_blocking == "auto" and args.autotune and not sv.fused and sv.mode != "field6":
+        bv, brw, bks, bms, btimes = tune_blocking(eng, params_true, field_true, model.wavelets,
+                                                  steps=max(16, min(96, 40000 // cells_k)), which=1)
+        log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
+            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f})")
     with torch.no_grad():
         obs = ernn.ernn_traces(eng, params=params_true, field=field_true, wavelets=model.wavelets)
         gen = torch.Generator(device="cpu").manual_seed(1)

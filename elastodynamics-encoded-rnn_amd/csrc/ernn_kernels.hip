@@ -1199,6 +1199,27 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
       xch[buf][w][1][1][lane * V + e] = dd[RW - 1][e] * cy[RW - 1][e];
     }
   };
+  // u^m of the lane's cells for step s (rows -1 .. RW for the damage mode's
+  // stencil; output cells never sit on a region edge, so no halo columns).
+  // Step s + 1's loads are issued as soon as step s has used its u, before
+  // the band-edge exchange, so they arrive while the next update computes.
+  constexpr int RU = kD ? RW + 2 : RW;
+  RawRow<V> urx[RU], ury[RU];
+  auto load_u = [&](int s) {
+    const gfloat* umx = pool + (int64_t)a.u_slot[s] * 2 * I.field;
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      const int ci = min(max(gi0 + r - (kD ? 1 : 0), -1), I.nx);
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        const uint32_t o = row_off(P, ci, min(max(gj0 + e, -1), I.ny));
+        urx[r].c[e] = *at(umx, o);
+        ury[r].c[e] = *at(umx + I.field, o);
+      }
+      urx[r].h = ury[r].h = 0.f;
+    }
+  };
+  if (a.m0 <= I.nt) load_u(0);
   publish(0);
   __syncthreads();
 
@@ -1213,22 +1234,6 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
     const bool act = m <= I.nt;   // block-uniform
     float nx_[RW][V], ny_[RW][V];
     if (act) {
-      // ---- u^m of the lane's cells (rows -1 .. RW with column halos for the damage mode)
-      // (output cells never sit on a region edge, so the edge lanes need no halo)
-      const gfloat* umx = pool + (int64_t)a.u_slot[s] * 2 * I.field;
-      constexpr int RU = kD ? RW + 2 : RW;
-      RawRow<V> urx[RU], ury[RU];
-#pragma unroll
-      for (int r = 0; r < RU; ++r) {
-        const int ci = min(max(gi0 + r - (kD ? 1 : 0), -1), I.nx);
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          const uint32_t o = row_off(P, ci, min(max(gj0 + e, -1), I.ny));
-          urx[r].c[e] = *at(umx, o);
-          ury[r].c[e] = *at(umx + I.field, o);
-        }
-        urx[r].h = ury[r].h = 0.f;
-      }
 
       // ---- stencil rows of the (weighted) adjoint field: own rows and the band-edge rows
       Row<V> xr[RW + 2], yr[RW + 2];
@@ -1352,6 +1357,7 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
         cy[r][e] = ny_[r][e];
       }
     if (s + 1 < a.ks) {
+      if (m - 1 <= I.nt) load_u(s + 1);
       publish(buf ^ 1);
       __syncthreads();
     }
