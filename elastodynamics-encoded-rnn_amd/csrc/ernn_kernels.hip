@@ -146,6 +146,20 @@ __device__ __forceinline__ Row<V> make_row(const RawRow<V>& raw, int lane) {
   return r;
 }
 
+// make_row for a row without halo values (the time-blocked kernels, whose
+// region edge cells leave the valid part after the first step): bound_ctrl
+// writes 0 into the lane shifted in from outside the wavefront, so no
+// register has to be preset with `old` before each shift.
+template <int V>
+__device__ __forceinline__ Row<V> make_row_z(const float (&c)[V]) {
+  Row<V> r;
+  r.v[0] = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(c[V - 1]), 0x138, 0xf, 0xf, true));
+#pragma unroll
+  for (int e = 0; e < V; ++e) r.v[e + 1] = c[e];
+  r.v[V + 1] = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(c[0]), 0x130, 0xf, 0xf, true));
+  return r;
+}
+
 template <int V>
 __device__ __forceinline__ Row<V> mul_row(const Row<V>& a, const Row<V>& b) {
   Row<V> r;
@@ -972,27 +986,22 @@ __global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs a) {
     // rows -1 .. RW of this wavefront's band; the region's own edge rows get zeros
     Row<V> xr[RW + 2], yr[RW + 2];
     {
-      RawRow<V> tx, ty, bx, by;
-      tx.h = ty.h = bx.h = by.h = 0.f;
+      float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
       for (int e = 0; e < V; ++e) {
-        tx.c[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
-        ty.c[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
-        bx.c[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
-        by.c[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+        tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
+        ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
+        bx[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
+        by[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
       }
-      xr[0] = make_row<V>(tx, lane);
-      yr[0] = make_row<V>(ty, lane);
-      xr[RW + 1] = make_row<V>(bx, lane);
-      yr[RW + 1] = make_row<V>(by, lane);
+      xr[0] = make_row_z<V>(tx);
+      yr[0] = make_row_z<V>(ty);
+      xr[RW + 1] = make_row_z<V>(bx);
+      yr[RW + 1] = make_row_z<V>(by);
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
-        RawRow<V> rx, ry;
-        rx.h = ry.h = 0.f;
-#pragma unroll
-        for (int e = 0; e < V; ++e) { rx.c[e] = cx[r][e]; ry.c[e] = cy[r][e]; }
-        xr[r + 1] = make_row<V>(rx, lane);
-        yr[r + 1] = make_row<V>(ry, lane);
+        xr[r + 1] = make_row_z<V>(cx[r]);
+        yr[r + 1] = make_row_z<V>(cy[r]);
       }
     }
 
@@ -1204,7 +1213,7 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
   // Step s + 1's loads are issued as soon as step s has used its u, before
   // the band-edge exchange, so they arrive while the next update computes.
   constexpr int RU = kD ? RW + 2 : RW;
-  RawRow<V> urx[RU], ury[RU];
+  float urx[RU][V], ury[RU][V];
   auto load_u = [&](int s) {
     const gfloat* umx = pool + (int64_t)a.u_slot[s] * 2 * I.field;
 #pragma unroll
@@ -1213,10 +1222,9 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
 #pragma unroll
       for (int e = 0; e < V; ++e) {
         const uint32_t o = row_off(P, ci, min(max(gj0 + e, -1), I.ny));
-        urx[r].c[e] = *at(umx, o);
-        ury[r].c[e] = *at(umx + I.field, o);
+        urx[r][e] = *at(umx, o);
+        ury[r][e] = *at(umx + I.field, o);
       }
-      urx[r].h = ury[r].h = 0.f;
     }
   };
   if (a.m0 <= I.nt) load_u(0);
@@ -1238,35 +1246,33 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
       // ---- stencil rows of the (weighted) adjoint field: own rows and the band-edge rows
       Row<V> xr[RW + 2], yr[RW + 2];
       {
-        RawRow<V> tx, ty, bx, by;
-        tx.h = ty.h = bx.h = by.h = 0.f;
+        float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          tx.c[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
-          ty.c[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
-          bx.c[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
-          by.c[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+          tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
+          ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
+          bx[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
+          by[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
         }
-        xr[0] = make_row<V>(tx, lane);
-        yr[0] = make_row<V>(ty, lane);
-        xr[RW + 1] = make_row<V>(bx, lane);
-        yr[RW + 1] = make_row<V>(by, lane);
+        xr[0] = make_row_z<V>(tx);
+        yr[0] = make_row_z<V>(ty);
+        xr[RW + 1] = make_row_z<V>(bx);
+        yr[RW + 1] = make_row_z<V>(by);
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
-          RawRow<V> rx, ry;
-          rx.h = ry.h = 0.f;
+          float rx[V], ry[V];
 #pragma unroll
-          for (int e = 0; e < V; ++e) { rx.c[e] = dd[r][e] * cx[r][e]; ry.c[e] = dd[r][e] * cy[r][e]; }
-          xr[r + 1] = make_row<V>(rx, lane);
-          yr[r + 1] = make_row<V>(ry, lane);
+          for (int e = 0; e < V; ++e) { rx[e] = dd[r][e] * cx[r][e]; ry[e] = dd[r][e] * cy[r][e]; }
+          xr[r + 1] = make_row_z<V>(rx);
+          yr[r + 1] = make_row_z<V>(ry);
         }
       }
       Row<V> uxr[kD ? RW + 2 : 1], uyr[kD ? RW + 2 : 1];
       if constexpr (kD) {
 #pragma unroll
         for (int r = 0; r < RW + 2; ++r) {
-          uxr[r] = make_row<V>(urx[r], lane);
-          uyr[r] = make_row<V>(ury[r], lane);
+          uxr[r] = make_row_z<V>(urx[r]);
+          uyr[r] = make_row_z<V>(ury[r]);
         }
       }
 
@@ -1279,8 +1285,8 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
           apply_op_u(kl, dl, __builtin_fmaf(2.f, cx[r][e], -px[r][e]), __builtin_fmaf(2.f, cy[r][e], -py[r][e]),
                      nx_[r][e], ny_[r][e]);
           // gradient terms on the output tile (lam^{m+1} with u^m)
-          float ux = kD ? uxr[r + 1].v[x] : urx[r].c[e];
-          float uy = kD ? uyr[r + 1].v[x] : ury[r].c[e];
+          float ux = kD ? uxr[r + 1].v[x] : urx[r][e];
+          float uy = kD ? uyr[r + 1].v[x] : ury[r][e];
           if (!owned[r][e]) { ux = 0.f; uy = 0.f; }
           sbp_sums(dl, ux, uy, S);
           if constexpr (kD) {
@@ -1290,6 +1296,20 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
             if (owned[r][e]) gacc[r][e] += __builtin_fmaf(cx[r][e], ox, cy[r][e] * oy);
           }
         }
+
+      // The gradient sums are complete here.  The empty asm statements pin
+      // them in place: left free, the compiler sinks the gradient arithmetic
+      // below the injection branch, holding every row's derivatives live
+      // across it (damage mode at RW = 6: 205 instead of 166 VGPRs, two
+      // wavefronts per SIMD instead of three).
+#pragma unroll
+      for (int p = 0; p < kPartials; ++p) asm volatile("" : "+v"(S[p]));
+      if constexpr (kD) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+          for (int e = 0; e < V; ++e) asm volatile("" : "+v"(gacc[r][e]));
+      }
 
       // residual injection on recorded steps m = kP + 1 (the whole region)
       if (((m - 1) % a.stride == 0) && ((m - 1) / a.stride < nf)) {
@@ -1586,8 +1606,9 @@ hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st)
 }
 
 // Time-blocked adjoint shapes (constant and damage modes; register limits
-// checked by the build).
-#define ERNN_ADJ_BLOCK_CASES_DFIELD(X) X(1, 4, 2) X(1, 4, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 8)
+// checked by the build).  RW = 6 keeps the damage adjoint at three
+// wavefronts per SIMD (166 VGPRs), which outruns RW = 8 at two.
+#define ERNN_ADJ_BLOCK_CASES_DFIELD(X) X(1, 4, 2) X(1, 4, 4) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 8)
 #define ERNN_ADJ_BLOCK_CASES(X) ERNN_ADJ_BLOCK_CASES_DFIELD(X) X(2, 4, 2) X(2, 4, 4)
 #define ERNN_ADJ_BLOCK_CASE(v_, r_, k_)                                                             \
   if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \

@@ -14,9 +14,13 @@
 //  adjoint   for s = S-1 .. 0: recompute u^{sK+1} .. u^{top} (top =
 //            min((s+1)K, nt)) from checkpoint s into K segment slots, then run
 //            the adjoint steps m = top .. sK+1, which read u^m from the segment
-//            and rotate lam through a 3-slot ring.  The forward has no atomics
-//            and a fixed order, so the recomputed states equal the forward's
-//            bit for bit.
+//            and rotate lam through a 3-slot ring.  The time-blocked adjoint
+//            runs KS reverse steps per launch and hands its last two lam
+//            states to the next launch through the other pair of a 4-slot
+//            ring.  The forward has no atomics and a fixed order, so the
+//            recomputed states equal the forward's bit for bit.
+//  fused     small grids: one launch runs adjoint step m of segment s and
+//            recompute step n of segment s - 1 (two segment banks).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -105,9 +109,12 @@ bool block_ok(int mode, int V, int RW, int KS) {
   return !(mode != ERNN_PARAM_CONST6 && V == 2 && RW == 8);
 }
 // Time-blocked adjoint shapes (ERNN_ADJ_BLOCK_CASES): constant and damage
-// modes; V = 2 (RW = 4) for constant stiffness only.
+// modes; the forward's V = 1 shapes and (1, 6, 4); V = 2 (RW = 4) for
+// constant stiffness only.
 bool adj_block_ok(int mode, int V, int RW, int KS) {
-  if (mode == ERNN_PARAM_FIELD6 || !block_ok(mode, V, RW, KS)) return false;
+  if (mode == ERNN_PARAM_FIELD6) return false;
+  if (RW == 6) return V == 1 && KS == 4;
+  if (!block_ok(mode, V, RW, KS)) return false;
   if (V == 2) return mode == ERNN_PARAM_CONST6 && RW == 4;
   return true;
 }
@@ -338,7 +345,8 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
                                 "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
   if (adjoint && !adj_block_ok(p->mode, V, RW, KS))
     return fail(ERNN_E_INVALID, "time-blocked adjoint shape (V, RW, KS): constant or damage mode, V = 1 with "
-                                "RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or (constant) V = 2, RW = 4, KS in {2,4}");
+                                "RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) V = 2, "
+                                "RW = 4, KS in {2,4}");
   const int OR = kWavesPerBlock * RW - 2 * KS, OC = kWave * V - 2 * KS;
   std::vector<BlockDev> blocks;
   std::vector<SrcEv> sev;
@@ -461,7 +469,8 @@ int validate(const ernn_desc_t* d) {
                                 "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
   if (d->ablk_ks > 1 && !adj_block_ok(d->param_mode, d->ablk_v, d->ablk_rw, d->ablk_ks))
     return fail(ERNN_E_INVALID, "time-blocked adjoint shape (V, RW, KS): constant or damage mode, V = 1 with "
-                                "RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or (constant) V = 2, RW = 4, KS in {2,4}");
+                                "RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) V = 2, "
+                                "RW = 4, KS in {2,4}");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))

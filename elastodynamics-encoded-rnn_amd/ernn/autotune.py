@@ -66,8 +66,8 @@ DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
 BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = (
     (1, 4, 2), (1, 4, 4), (1, 8, 2), (1, 8, 4), (1, 8, 8), (2, 4, 2), (2, 4, 4), (2, 8, 2), (2, 8, 4), (2, 8, 8))
 # time-blocked adjoint shapes (constant and damage modes; V = 2 for constant stiffness only)
-ADJ_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 8, 2), (1, 8, 4), (1, 8, 8),
-                                                      (2, 4, 2), (2, 4, 4))
+ADJ_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 6, 4), (1, 8, 2), (1, 8, 4),
+                                                      (1, 8, 8), (2, 4, 2), (2, 4, 4))
 # largest V * R and R of the built adjoint kernels per mode (larger shapes would spill registers)
 ADJ_MAX_CELLS = {"const6": 16, "dfield": 8, "field6": 4}
 ADJ_MAX_ROWS = {"const6": 16, "dfield": 4, "field6": 2}

@@ -98,8 +98,9 @@ typedef struct {
                            modes exclude (V, RW) = (2, 8).  Used by the forward pass and the
                            segment recompute of the unfused adjoint. */
   int32_t ablk_v, ablk_rw, ablk_ks; /* time-blocked adjoint of the unfused backward (CONST6,
-                           DFIELD): ablk_ks reverse steps per launch; V = 1 with RW in {4,8}, or
-                           (CONST6) V = 2 with RW = 4 and KS in {2,4}; ablk_ks <= 1 = off */
+                           DFIELD): ablk_ks reverse steps per launch; V = 1 with RW in {4,8},
+                           or (1, 6, 4), or (CONST6) V = 2 with RW = 4 and KS in {2,4};
+                           ablk_ks <= 1 = off */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;

@@ -126,7 +126,7 @@ def test_set_blocking_switches_and_replays_graphs():
 
 
 # ------------------------------------------------------------- blocked adjoint
-ADJ_SHAPES = [(1, 4, 2), (1, 4, 4), (1, 8, 4), (1, 8, 8), (2, 4, 4)]
+ADJ_SHAPES = [(1, 4, 2), (1, 4, 4), (1, 6, 4), (1, 8, 4), (1, 8, 8), (2, 4, 4)]
 
 
 def _adj_pair(shots, mode, ablk, K, snapshot=False, stride=1, **kw):
