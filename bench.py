@@ -35,6 +35,11 @@ sys.path.insert(0, os.path.join(REPO, "elastodynamics-encoded-rnn_amd"))
 
 METRIC = "fwd+adjoint Gcell-steps/s & s/Adam-iter: HIP adjoint vs autograd, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+# Vector-ALU issue peak in wavefront instructions per second: 256 CUs x 2.4
+# GHz x one wave64 instruction per CU per cycle (four SIMD16 units, four
+# cycles each).  The kernels are built without packed FP32, so this is also
+# their FP32 FMA peak (78.6 TFLOP/s, half the packed 157.3).
+VALU_PEAK_WIPS = 256 * 2.4e9
 # algorithmic FP32 bytes per cell and launch: forward reads u^n, u^{n-1} and
 # writes u^{n+1} (2 components each); the adjoint reads lam^{m+1}, lam^{m+2},
 # u^m and writes lam^m; per-cell fields add their reads / accumulator traffic.
@@ -368,15 +373,20 @@ def main():
     # WRITE_SIZE, calibrated on a copy kernel), taken with the tile shapes
     # recorded beside them (traffic_tiles; within a few percent across tile
     # shapes, since every shape reads each field once per launch).
-    traffic, traffic_tiles = None, None
+    traffic, traffic_tiles, valu = None, None, None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tpath):
         entry = json.load(open(tpath)).get(f"{wl.name}/{name}")
         if entry:
             traffic, traffic_tiles = entry["bytes"], entry["tiles"]
+            if entry.get("valu_insts"):   # SQ_INSTS_VALU per launch, same counter runs
+                rate = entry["valu_insts"] / (ms * 1e-3)
+                valu = {"insts_per_launch": entry["valu_insts"], "issue_rate": round(rate / 1e9, 1),
+                        "peak": VALU_PEAK_WIPS / 1e9, "unit": "G wave-instructions/s",
+                        "frac": round(rate / VALU_PEAK_WIPS, 4)}
     roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_tiles": traffic_tiles,
+                "traffic_tiles": traffic_tiles, "valu": valu,
                 "bytes_per_cell": bpc, "cells_per_launch": cells, "avg_launch_us": round(ms * 1e3, 3),
                 "launches_per_iter": launches,
                 "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),

@@ -99,7 +99,7 @@ def aniso_2048(nt: int = 4000, n: int = 2048, shots: int = 16, seed: int = 4) ->
     h = 0.05e-3
     C = np.array(bond_rotate(C_ORTHO, 30.0))
     dt = 0.5 * h / max_phase_velocity(C, RHO)
-    rec = row_receivers(n, n - 1, 1024)
+    rec = row_receivers(n, n - 1, min(1024, n))   # 1024 at full size; every cell of the row on crops
     sh = [_shot(n, h, nt, dt, int((k + 0.5) * n / shots), rec) for k in range(shots)]
     sv = Survey(sh, mode="dfield")
     D = ellipse_damage(n, n, 5, np.random.default_rng(seed), semi=(20, 80), level=(0.2, 0.8), taper=5)

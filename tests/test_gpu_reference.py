@@ -2,7 +2,9 @@
 (oracle/plain_torch.py) with autograd, run on the same GPU.  Bars: FP32 traces
 within 1e-5 relative L2 of the plain path, gradients within 1e-4 relative of
 its autograd gradient; the FP64 oracle error is printed per case."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -16,6 +18,7 @@ from oracle import plain_torch as pt
 from gpu_helpers import DEV, rel_l2
 
 pytestmark = pytest.mark.gpu
+FD_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fd_single512_crop.json")
 
 
 def _rel(a, b):
@@ -153,21 +156,24 @@ def test_full_size_traces_and_gradient_vs_plain_autograd(name):
 
 def test_single_512_crop_vs_fp64_oracle_and_finite_differences():
     """single-512 physics (h 0.05 mm, dt, C, 2000 steps) on a 128^2 crop problem:
-    HIP vs the FP64 oracle, and the HIP gradient vs central FD (step 1e-3 C:
-    J sums 2000 FP32 steps, so a smaller step lets the FP32 noise of J(C +- d)
-    dominate the difference, while the O(d^2) truncation of a 1e-3 step stays
-    below 1e-5 of the gradient)."""
+    HIP traces vs the FP64 oracle; the HIP gradient vs the FP64 discrete adjoint
+    and vs FP64 central differences at step 1e-4 relative (both from the
+    committed fixture tests/golden/fd_single512_crop.json); and vs central
+    differences of the HIP forward itself at step 1e-3 C (J sums 2000 FP32
+    steps, so a smaller step lets the FP32 noise of J(C +- d) dominate the
+    difference, while the O(d^2) truncation of a 1e-3 step stays below 1e-5 of
+    the gradient)."""
+    fx = json.load(open(FD_FIXTURE))
     wl = configs.single_512(nt=2000, n=128)
     s = wl.survey.shots[0]
     C = wl.C_true[0] * np.array([1.05, 0.97, 1.1, 1.0, 1.0, 0.93]) + np.array([0, 0, 0, 0.4e9, -0.3e9, 0])
     C32 = np.asarray(C, np.float32).astype(np.float64)
-    rng = np.random.default_rng(8)
+    assert np.array_equal(C32, np.array(fx["C"])) and fx["nt"] == s.nt and fx["grid"] == [s.nx, s.ny]
+    rng = np.random.default_rng(fx["g_seed"])
     G = rng.standard_normal((2, s.nt, len(s.receivers)))
-    ref, st = orc.forward(C32, s.h, s.dt, s.rho, s.nt, s.nx, s.ny, s.sources, s.wavelet.astype(np.float64),
-                          1, s.receivers, keep_states=True)
-    gref = orc.const_gradient(orc.adjoint_gradient(C32, s.h, s.dt, s.rho, s.nt, s.nx, s.ny, st, G, 1,
-                                                   s.receivers))
-    del st
+    ref, _ = orc.forward(C32, s.h, s.dt, s.rho, s.nt, s.nx, s.ny, s.sources, s.wavelet.astype(np.float64),
+                         1, s.receivers)
+    gref, fd64 = np.array(fx["adjoint"]), np.array(fx["fd"])
     eng = ernn.Engine(wl.survey, device=DEV)
     wav = torch.tensor(wl.survey.wavelets(), device=DEV)
     Gt = torch.tensor(G.ravel(), dtype=torch.float32, device=DEV)
@@ -175,8 +181,10 @@ def test_single_512_crop_vs_fp64_oracle_and_finite_differences():
     p = torch.tensor(C, dtype=torch.float32, device=DEV)
     tr = eng.forward(p, None, wav, ws)
     gp, _ = eng.adjoint(p, None, wav, Gt, ws)
+    gp = gp.cpu().numpy()
     e_tr = rel_l2(tr.cpu().numpy().reshape(ref.shape), ref)
-    e_g = np.abs(gp.cpu().numpy() - gref).max() / np.abs(gref).max()
+    e_g = np.abs(gp - gref).max() / np.abs(gref).max()
+    e_fd64 = np.abs(gp - fd64).max() / np.abs(fd64).max()
     # central differences of the HIP forward, J = <traces, G>
     fd = []
     for k in range(6):
@@ -187,12 +195,49 @@ def test_single_512_crop_vs_fp64_oracle_and_finite_differences():
         jm = float((eng.forward(torch.tensor(cm, dtype=torch.float32, device=DEV), None, wav, ws).double() * Gt.double()).sum())
         fd.append((jp - jm) / (2 * d))
     fd = np.array(fd)
-    e_fd = np.abs(gp.cpu().numpy() - fd).max() / np.abs(gp.cpu().numpy()).max()
+    e_fd = np.abs(gp - fd).max() / np.abs(gp).max()
     print(f"single-512 crop 128^2 x 2000: traces rel-L2 vs FP64 {e_tr:.2e}, grad vs FP64 adjoint {e_g:.2e}, "
-          f"grad vs FP32 central FD {e_fd:.2e}")
+          f"vs FP64 central FD (1e-4) {e_fd64:.2e}, vs FP32 central FD (1e-3) {e_fd:.2e}")
     assert e_tr < 1e-5
     assert e_g < 1e-4
+    assert e_fd64 < 1e-4
     assert e_fd < 2e-2      # FP32 forward noise / step bounds the FD accuracy
+
+
+def test_aniso_2048_crop_vs_fp64_oracle():
+    """aniso-2048 physics (30-degree fibre frame, so C15 and C35 are nonzero;
+    damage field of five ellipses) on a 256^2 crop at 500 steps, two shots:
+    HIP traces, dJ/dC0 and dJ/dD vs the FP64 oracle's forward and discrete
+    adjoint on the fields C_p = D C0_p."""
+    from gpu_helpers import run_hip
+    wl = configs.aniso_2048(nt=500, n=256, shots=2)
+    sv = wl.survey
+    C0 = np.asarray(wl.C_true[0], np.float32)
+    nx, ny = sv.shots[0].nx, sv.shots[0].ny
+    D = np.asarray(wl.D_true, np.float32).reshape(nx, ny)
+    rng = np.random.default_rng(12)
+    G = rng.standard_normal(sum(2 * s.nt * len(s.receivers) for s in sv.shots))
+    eng, tr, (gC, gD) = run_hip(sv, params=C0, field=D, G=G)
+    D64, C64 = D.astype(np.float64), C0.astype(np.float64)
+    cells = [D64 * c for c in C64]
+    e_tr, gcell = 0.0, 0.0
+    for b, s in enumerate(sv.shots):
+        lo, hi = eng.trace_off[b], eng.trace_off[b + 1]
+        ref, st = orc.forward(cells, s.h, s.dt, s.rho, s.nt, nx, ny, s.sources, s.wavelet.astype(np.float64),
+                              1, s.receivers, keep_states=True)
+        e_tr = max(e_tr, rel_l2(tr[lo:hi].reshape(ref.shape), ref))
+        gcell = gcell + orc.adjoint_gradient(cells, s.h, s.dt, s.rho, s.nt, nx, ny, st,
+                                             G[lo:hi].reshape(ref.shape), 1, s.receivers)
+        del st
+    gC_ref, gD_ref = orc.dfield_gradient(gcell, C64, D64)
+    e_gc = np.abs(gC - gC_ref).max() / np.abs(gC_ref).max()
+    e_gd = rel_l2(gD.reshape(nx, ny), gD_ref)
+    print(f"aniso-2048 crop 256^2 x 500, 2 shots: traces rel-L2 vs FP64 {e_tr:.2e}, dJ/dC0 max rel {e_gc:.2e}, "
+          f"dJ/dD rel-L2 {e_gd:.2e}")
+    assert e_tr < 1e-5
+    assert e_gc < 1e-4
+    assert e_gd < 1e-4
+    eng.close()
 
 
 def test_damage_1024_dfield_vs_plain_per_cell_autograd():

@@ -303,3 +303,24 @@ def test_total_variation_closed_form():
     # |dx|: |2|,|1|,|3| ; |dy|: 1,2,0,2
     assert orc.total_variation(f) == pytest.approx(6 + 5)
     assert float(pt.total_variation_2d(torch.tensor(f)[None])) == pytest.approx(11)
+
+
+def test_oracle_adjoint_matches_committed_fp64_finite_differences_single_512_crop():
+    """At the single-512 physics on a 128^2 crop (2000 steps, 64 receivers), the
+    FP64 discrete adjoint matches FP64 central differences of J = <traces, G>
+    at step 1e-4 relative (tests/golden/fd_single512_crop.json, written by
+    tests/golden/make_fd_single512_crop.py from the oracle forward)."""
+    import importlib.util
+    import json
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("make_fd", os.path.join(here, "make_fd_single512_crop.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    fx = json.load(open(os.path.join(here, "fd_single512_crop.json")))
+    s, C, G = mk.problem()
+    assert np.array_equal(C, np.array(fx["C"])) and fx["step_rel"] == 1e-4
+    g = mk.adjoint(s, C, G)
+    fd = np.array(fx["fd"])
+    assert np.abs(g - fd).max() / np.abs(fd).max() < 1e-6
+    assert np.allclose(g, fx["adjoint"], rtol=1e-10, atol=0)
