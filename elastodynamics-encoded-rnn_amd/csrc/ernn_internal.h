@@ -116,6 +116,7 @@ struct BlockDev {
   int inst, oi0, oj0, set;  // output tile origin (grid coordinates)
   int src_begin, src_end;   // sources inside the block's region, sorted by row
   int rec_begin, rec_end;   // receivers inside the output tile, sorted by row
+  int rec_shared;           // 1: two receivers share a cell (the kernels walk the list instead of a cell map)
 };
 
 struct BlockArgs {

@@ -889,6 +889,38 @@ __global__ void __launch_bounds__(kBlock) fused_step_kernel(FusedArgs fa) {
   }
 }
 
+// Receivers of a block mapped onto the cells of each wavefront's band, once
+// per launch: map[w][r][c] = receiver index at band row r, region column c,
+// or -1; bit r V + e of the result: the lane's cell (r, e) holds one.  A
+// recorded or injected step then serves every receiver of the wavefront with
+// one masked access per cell instead of a walk over the block's list, whose
+// dependent loads put the blocks on a receiver line on the critical path.
+// Call block-uniformly (it holds a block barrier); valid only when no two
+// receivers share a cell (BlockDev::rec_shared).
+template <int V, int RW>
+__device__ __forceinline__ uint32_t map_receivers(int (&map)[kWavesPerBlock][RW][kWave * V], const RecEv* rec,
+                                                  int rec_begin, int rec_end, int gi0, int gc0, int w,
+                                                  int lane) {
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int e = 0; e < V; ++e) map[w][r][lane * V + e] = -1;
+  __syncthreads();
+  for (int rp = rec_begin + lane; rp < rec_end; rp += kWave) {
+    const RecEv ev = rec[rp];
+    const int rr = ev.i - gi0, lc = ev.j - gc0;
+    if (rr >= 0 && rr < RW && lc >= 0 && lc < kWave * V) map[w][rr][lc] = ev.rec;
+  }
+  __syncthreads();
+  uint32_t bits = 0;
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+      if (map[w][r][lane * V + e] >= 0) bits |= 1u << (r * V + e);
+  return bits;
+}
+
 // --------------------------------------------------------------------------
 // Time-blocked forward: KS steps per launch on a block region of
 // kWavesPerBlock * RW rows x 64 V columns.  Wavefront w holds region rows
@@ -967,6 +999,11 @@ __global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs a) {
       inside[r][e] = gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny;
       owned[r][e] = inside[r][e] && gi >= bl.oi0 && gi < bl.oi0 + OR && gj >= bl.oj0 && gj < bl.oj0 + OC;
     }
+
+  __shared__ int recmap[kWavesPerBlock][RW][kCols];
+  const bool use_map = a.record && !a.snapshot && !bl.rec_shared && bl.rec_end > bl.rec_begin;   // block-uniform
+  const uint32_t rbits = use_map ? map_receivers<V, RW>(recmap, a.rec, bl.rec_begin, bl.rec_end, gi0,
+                                                        bl.oj0 - KS, w, lane) : 0u;
 
   auto publish = [&](int buf) {
 #pragma unroll
@@ -1071,6 +1108,20 @@ __global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs a) {
               tx[0] = nx_[r][e];
               tx[(int64_t)nf * I.nx * I.ny] = ny_[r][e];
             }
+      } else if (use_map) {
+        if (rbits) {
+          gfloat* tx = C.traces + I.trace_off + (int64_t)k * I.n_rec;
+          const int64_t cy = (int64_t)nf * I.n_rec;
+#pragma unroll
+          for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              if ((rbits >> (r * V + e)) & 1u) {
+                const int rec = recmap[w][r][lane * V + e];
+                tx[rec] = nx_[r][e];
+                tx[rec + cy] = ny_[r][e];
+              }
+        }
       } else {
         for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
           const RecEv ev = a.rec[rp];
@@ -1199,6 +1250,14 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
         }
   }
 
+  // The damage mode walks the receiver list: its parallel injection would
+  // take the (1, 6, 4) kernel from 166 to 172 VGPRs, below three wavefronts
+  // per SIMD, and on the large grids it runs on other blocks hide the walk.
+  __shared__ int recmap[kWavesPerBlock][RW][kCols];
+  const bool use_map = !kD && !a.snapshot && !bl.rec_shared && bl.rec_end > bl.rec_begin;   // block-uniform
+  const uint32_t rbits = use_map ? map_receivers<V, RW>(recmap, a.rec, bl.rec_begin, bl.rec_end, gi0,
+                                                        bl.oj0 - KS, w, lane) : 0u;
+
   auto publish = [&](int buf) {
 #pragma unroll
     for (int e = 0; e < V; ++e) {
@@ -1324,6 +1383,20 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
                 nx_[r][e] += gx[0];
                 ny_[r][e] += gx[(int64_t)nf * I.nx * I.ny];
               }
+        } else if (use_map) {
+          if (rbits) {
+            const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec;
+            const int64_t cy = (int64_t)nf * I.n_rec;
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
+#pragma unroll
+              for (int e = 0; e < V; ++e)
+                if ((rbits >> (r * V + e)) & 1u) {
+                  const int rec = recmap[w][r][lane * V + e];
+                  nx_[r][e] += gx[rec];
+                  ny_[r][e] += gx[rec + cy];
+                }
+          }
         } else {
           for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
             const RecEv ev = a.rec[rp];

@@ -409,6 +409,7 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
         bd.rec_begin = (int)rev.size();
         for (; ri < re.size() && re[ri].blk == id; ++ri) {
           const ernn_receiver_t& r = p->rec_h[I.rec_begin + re[ri].idx];
+          if ((int)rev.size() > bd.rec_begin && rev.back().i == r.i && rev.back().j == r.j) bd.rec_shared = 1;
           rev.push_back(RecEv{r.i, r.j, re[ri].idx, 0});
         }
         bd.rec_end = (int)rev.size();

@@ -23,8 +23,10 @@ LIB_NAME = "libernn_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 SOURCES = ["ernn_kernels.hip", "ernn_plan.cpp"]
 
-# -ffp-contract=off keeps the reference's separately rounded multiply/add
-# order; packed FP32 instructions are disabled for all device code.
+# -ffp-contract=off: every fused multiply-add is an explicit fmaf in the
+# source, so each cell's arithmetic is the same in every kernel shape and a
+# recomputed state equals the forward's bit for bit; packed FP32 instructions
+# are disabled for all device code.
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
