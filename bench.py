@@ -68,6 +68,9 @@ def parse():
                     help="time-blocked forward 'V,RW,KS', '0' for one step per launch, or 'auto' (autotuned)")
     ap.add_argument("--adj-blocking", default="auto",
                     help="time-blocked unfused adjoint 'V,RW,KS', '0', or 'auto' (autotuned)")
+    ap.add_argument("--fuse", default="auto",
+                    help="backward schedule: '1' fused single-step, '0' unfused (time-blocked when tuned), "
+                         "'auto' (small grids: timed both ways)")
     ap.add_argument("--baseline", type=int, default=1, help="time the autograd plain path on the GPU")
     ap.add_argument("--cpu", type=int, default=1, help="time the CPU port")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -248,7 +251,7 @@ def main():
     import torch
     import ernn
     from ernn import configs
-    from ernn.autotune import tune_blocking, tune_tiles
+    from ernn.autotune import tune_backward, tune_blocking, tune_tiles
 
     dev = torch.device("cuda:0")
     kw = {"nt": args.nt} if args.nt else {}
@@ -263,12 +266,22 @@ def main():
     D_init = None if sv.mode == "const6" else np.ones_like(wl.D_true)
     log(f"workload {wl.name}: {wl.description}; {sv.cell_steps() / 1e9:.3f} Gcell-steps per pass")
 
-    model = ernn.ERNN(sv, C_init, D_init=D_init, device=dev)
-    eng = model.engine
     params_true = torch.tensor(C_true.reshape(-1), dtype=torch.float32, device=dev)
     field_true = None if sv.mode == "const6" else torch.tensor(wl.D_true.reshape(-1), device=dev)
-    tune = None
     cells_k = max(1, sum(s.nx * s.ny for s in sv.shots) // 1000)   # thousands of cells per step
+    if args.fuse != "auto":
+        sv.fuse = bool(int(args.fuse))
+    elif args.autotune and not args.tiles and sv.fused and sv.mode != "field6":
+        # small grids: the fused single-step backward vs the time-blocked one
+        fuse, _, bt = tune_backward(sv, params_true, field_true, torch.tensor(sv.wavelets(), device=dev),
+                                    steps=max(16, min(96, 40000 // cells_k)))
+        sv.fuse = fuse
+        log(f"autotune: backward {'fused' if fuse else 'unfused'} (fused {bt['fused'] * 1e3:.2f} vs unfused "
+            f"time-blocked {bt['blocked'] * 1e3:.2f} us per reverse step)")
+
+    model = ernn.ERNN(sv, C_init, D_init=D_init, device=dev)
+    eng = model.engine
+    tune = None
     if args.tiles:
         vf, rf, va, ra, bf, ba = (int(x) for x in args.tiles.split(","))
         eng.set_tiling(0, vf, rf, bf)
@@ -290,7 +303,7 @@ def main():
     if args.adj_blocking not in ("auto", "0"):
         eng.set_blocking(*(int(x) for x in args.adj_blocking.split(",")), which=1)
         log(f"time-blocked adjoint fixed: (V, RW, KS) = {eng.adj_blocking}")
-    elif args.adj_blocking == "auto" and args.autotune and not sv.fused and sv.mode != "field6":
+    elif args.adj_blocking == "auto" and args.autotune and sv.mode != "field6":
         bv, brw, bks, bms, btimes = tune_blocking(eng, params_true, field_true, model.wavelets,
                                                   steps=max(16, min(96, 40000 // cells_k)), which=1)
         log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
@@ -341,7 +354,7 @@ def main():
         L = max(2, nk // KS)
         eng.time_steps(3, 2, p, f, model.wavelets, ws, tr0, tr0)
         blk_ms = eng.time_steps(3, L, p, f, model.wavelets, ws, tr0, tr0) / L
-    AKS = eng.adj_blocking[2] if not sv.fused else 0
+    AKS = eng.adj_blocking[2]
     ablk_ms = None
     if AKS > 1:
         L = max(2, nk // AKS)
@@ -358,7 +371,10 @@ def main():
         kern["fwd_block_kernel"] = (blk_ms, n_blk, bf + 8)
     else:
         kern["fwd_step_kernel"] = (fwd_ms, nt + K if sv.fused else 2 * nt, bf)
-    if sv.fused:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
+    if sv.fused and AKS > 1:   # fused time-blocked backward: adjoint half + recompute half (every state stored)
+        abytes = 16 + 8 * AKS + 16 + (12 if sv.mode == "dfield" else 0)
+        kern["fused_block_kernel"] = (ablk_ms, math.ceil(nt / AKS), abytes + (bf - 8) + 8 * AKS)
+    elif sv.fused:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
         kern["fused_step_kernel"] = (fused_ms, nt, bf + ba)
     elif AKS > 1:   # reads lam^{m+1}, lam^{m+2} (+ D), u^m per step, writes two states, acc r/w once
         abytes = 16 + 8 * AKS + 16 + (12 if sv.mode == "dfield" else 0)

@@ -158,4 +158,14 @@ struct BlockAdjArgs {
   int rw, ksmax;
 };
 
+// Fused time-blocked backward: a time-blocked adjoint launch of segment s and
+// a time-blocked recompute launch of segment s - 1 (same V, RW, KS) in one
+// grid; blocks below adj_grid run the adjoint.
+struct FusedBlockArgs {
+  BlockAdjArgs adj;
+  BlockArgs fwd;
+  int adj_grid;
+  int pad;
+};
+
 }  // namespace ernn

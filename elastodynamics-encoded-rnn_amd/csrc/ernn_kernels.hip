@@ -933,8 +933,16 @@ __device__ __forceinline__ uint32_t map_receivers(int (&map)[kWavesPerBlock][RW]
 // the states are bitwise those of the single-step kernel.  Cells outside the grid are
 // kept at zero; only output cells are stored and recorded.
 // --------------------------------------------------------------------------
+// The forward body reads its launch's arguments in place through a pointer
+// into the kernarg segment (address space 4): scalar loads, also for the
+// arrays indexed by the step, in both fwd_block_kernel and the recompute half
+// of fused_block_kernel.
+template <typename T>
+using KArg = const T __attribute__((address_space(4)));
+
 template <int MODE, int V, int RW, int KS>
-__global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs a) {
+__device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
+  const auto& a = *ap;
   constexpr int kCols = kWave * V;
   constexpr int OR = kWavesPerBlock * RW - 2 * KS;   // output rows
   constexpr int OC = kCols - 2 * KS;                 // output columns
@@ -945,7 +953,7 @@ __global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs a) {
   const int w = threadIdx.x >> 6;
   // block b runs on XCD b % 8, which owns a contiguous band of block tiles
   const int chunk = (a.n_blocks + 7) >> 3;
-  const int t = (blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+  const int t = (bid & 7) * chunk + (bid >> 3);
   if (t >= a.n_blocks) return;
   const BlockDev bl = a.blocks[t];
   const InstDev I = a.inst[bl.inst];
@@ -1170,314 +1178,37 @@ __global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs a) {
 template <int MODE, int V, int RW, int KS>
 __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
 #pragma clang fp contract(fast)
-  constexpr int kCols = kWave * V;
-  constexpr int OR = kWavesPerBlock * RW - 2 * KS;
-  constexpr int OC = kCols - 2 * KS;
-  constexpr bool kD = (MODE == kDField);
-  __shared__ float xch[2][kWavesPerBlock][2][2][kCols];
+  const int bid = blockIdx.x;
+#include "ernn_adj_block_body.inc"
+}
 
-  const int lane = threadIdx.x & (kWave - 1);
-  const int w = threadIdx.x >> 6;
-  const int chunk = (a.n_blocks + 7) >> 3;
-  const int t = (blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
-  if (t >= a.n_blocks) return;
-  const BlockDev bl = a.blocks[t];
-  const InstDev I = a.inst[bl.inst];
-  if (a.m0 - a.ks + 1 > I.nt) return;   // this instance's reverse recurrence has not started
-  const GCall C = global_call(*a.call);
+// the argument is read in place through the kernarg segment (offset 0)
+template <int MODE, int V, int RW, int KS>
+__global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs) {
+  fwd_block_body<MODE, V, RW, KS>((KArg<BlockArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
+}
 
-  const uint32_t P = (uint32_t)I.pitch;
-  gfloat* pool = C.ws + I.pool_off;
-  const gfloat* coef = C.ws + I.coef_off;
-  const int gi0 = bl.oi0 - KS + w * RW;
-  const int gj0 = bl.oj0 - KS + lane * V;
-  const float kdt = (float)I.gscale;
-  const double gsa = I.gscale * (double)I.hinv2;
-  float c0[6];
-#pragma unroll
-  for (int p = 0; p < 6; ++p) c0[p] = C.params[bl.set * 6 + p];
-  const OpCoef kl = op_coef(c0, gsa);    // (dt^2/rho) A(C0) on the (weighted) adjoint field
-  const OpCoef ku = op_coef(c0, 1.0);    // A(C0) / a on u (dJ/dD)
-
-  // ---- loads: lam^{m0+1}, lam^{m0+2}, D; partial slot and accumulators
-  const gfloat* l1 = pool + (int64_t)a.s_l1 * 2 * I.field;
-  const gfloat* l2 = pool + (int64_t)a.s_l2 * 2 * I.field;
-  float cx[RW][V], cy[RW][V], px[RW][V], py[RW][V], dd[RW][V];
-  bool inside[RW][V], owned[RW][V];
-#pragma unroll
-  for (int r = 0; r < RW; ++r) {
-    const int ci = min(max(gi0 + r, -1), I.nx);
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-      const int cj = min(max(gj0 + e, -1), I.ny);
-      const uint32_t o = row_off(P, ci, cj);
-      cx[r][e] = *at(l1, o);
-      cy[r][e] = *at(l1 + I.field, o);
-      px[r][e] = *at(l2, o);
-      py[r][e] = *at(l2 + I.field, o);
-      dd[r][e] = kD ? *at(coef, o) : 1.f;
-      const int gi = gi0 + r, gj = gj0 + e;
-      inside[r][e] = gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny;
-      owned[r][e] = inside[r][e] && gi >= bl.oi0 && gi < bl.oi0 + OR && gj >= bl.oj0 && gj < bl.oj0 + OC;
-    }
+// --------------------------------------------------------------------------
+// Fused time-blocked backward: one launch runs KS reverse steps of checkpoint
+// segment s (blocks [0, adj_grid)) and KS recompute steps of segment s - 1
+// (the remaining blocks), which fill the other segment bank.  The two halves
+// touch disjoint slots and have no order between them; on small grids, where
+// either launch alone leaves most of the chip idle and each step is bound by
+// latency, running them side by side overlaps the two chains.  adj_grid is a
+// multiple of 8, so both halves keep the XCD banding of their block tables.
+// --------------------------------------------------------------------------
+template <int MODE, int V, int RW, int KS>
+__global__ void __launch_bounds__(kBlock) fused_block_kernel(FusedBlockArgs fa) {
+#pragma clang fp contract(fast)
+  if (blockIdx.x >= fa.adj_grid) {   // block-uniform
+    const auto base = (const char __attribute__((address_space(4)))*)__builtin_amdgcn_kernarg_segment_ptr();
+    fwd_block_body<MODE, V, RW, KS>((KArg<BlockArgs>*)(base + offsetof(FusedBlockArgs, fwd)),
+                                    blockIdx.x - fa.adj_grid);
+    return;
   }
-  gdouble* part = reinterpret_cast<gdouble*>(C.ws) + a.partial_off +
-                  ((int64_t)t * kWavesPerBlock + w) * kPartials + (lane >> 3);
-  const double part_in = *part;
-  // per-cell damage accumulator: zero on the launch holding the segment's first step
-  const int top = min(a.seg_top, I.nt);
-  const bool seg_first = a.m0 >= top;
-  const bool seg_last = (a.m0 - a.ks + 1 == a.seg_bot);
-  gfloat* acc32 = C.ws + I.acc_off;
-  float gacc[RW][V];
-#pragma unroll
-  for (int r = 0; r < RW; ++r)
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-      gacc[r][e] = 0.f;
-      if (kD && !seg_first && owned[r][e]) gacc[r][e] = *at(acc32, row_off(P, gi0 + r, gj0 + e));
-    }
-  if (a.store_in >= 0) {   // a one-step launch also keeps its input state for the next launch
-    gfloat* dx = pool + (int64_t)a.store_in * 2 * I.field;
-#pragma unroll
-    for (int r = 0; r < RW; ++r)
-#pragma unroll
-      for (int e = 0; e < V; ++e)
-        if (owned[r][e]) {
-          const uint32_t o = row_off(P, gi0 + r, gj0 + e);
-          *at(dx, o) = cx[r][e];
-          *at(dx + I.field, o) = cy[r][e];
-        }
-  }
-
-  // The damage mode walks the receiver list: its parallel injection would
-  // take the (1, 6, 4) kernel from 166 to 172 VGPRs, below three wavefronts
-  // per SIMD, and on the large grids it runs on other blocks hide the walk.
-  __shared__ int recmap[kWavesPerBlock][RW][kCols];
-  const bool use_map = !kD && !a.snapshot && !bl.rec_shared && bl.rec_end > bl.rec_begin;   // block-uniform
-  const uint32_t rbits = use_map ? map_receivers<V, RW>(recmap, a.rec, bl.rec_begin, bl.rec_end, gi0,
-                                                        bl.oj0 - KS, w, lane) : 0u;
-
-  auto publish = [&](int buf) {
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-      xch[buf][w][0][0][lane * V + e] = dd[0][e] * cx[0][e];
-      xch[buf][w][0][1][lane * V + e] = dd[0][e] * cy[0][e];
-      xch[buf][w][1][0][lane * V + e] = dd[RW - 1][e] * cx[RW - 1][e];
-      xch[buf][w][1][1][lane * V + e] = dd[RW - 1][e] * cy[RW - 1][e];
-    }
-  };
-  // u^m of the lane's cells for step s (rows -1 .. RW for the damage mode's
-  // stencil; output cells never sit on a region edge, so no halo columns).
-  // Step s + 1's loads are issued as soon as step s has used its u, before
-  // the band-edge exchange, so they arrive while the next update computes.
-  constexpr int RU = kD ? RW + 2 : RW;
-  float urx[RU][V], ury[RU][V];
-  auto load_u = [&](int s) {
-    const gfloat* umx = pool + (int64_t)a.u_slot[s] * 2 * I.field;
-#pragma unroll
-    for (int r = 0; r < RU; ++r) {
-      const int ci = min(max(gi0 + r - (kD ? 1 : 0), -1), I.nx);
-#pragma unroll
-      for (int e = 0; e < V; ++e) {
-        const uint32_t o = row_off(P, ci, min(max(gj0 + e, -1), I.ny));
-        urx[r][e] = *at(umx, o);
-        ury[r][e] = *at(umx + I.field, o);
-      }
-    }
-  };
-  if (a.m0 <= I.nt) load_u(0);
-  publish(0);
-  __syncthreads();
-
-  float S[kPartials];
-#pragma unroll
-  for (int p = 0; p < kPartials; ++p) S[p] = 0.f;
-  const int nf = I.n_frames;
-
-  for (int s = 0; s < a.ks; ++s) {
-    const int m = a.m0 - s;
-    const int buf = s & 1;
-    const bool act = m <= I.nt;   // block-uniform
-    float nx_[RW][V], ny_[RW][V];
-    if (act) {
-
-      // ---- stencil rows of the (weighted) adjoint field: own rows and the band-edge rows
-      Row<V> xr[RW + 2], yr[RW + 2];
-      {
-        float tx[V], ty[V], bx[V], by[V];
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
-          ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
-          bx[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
-          by[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
-        }
-        xr[0] = make_row_z<V>(tx);
-        yr[0] = make_row_z<V>(ty);
-        xr[RW + 1] = make_row_z<V>(bx);
-        yr[RW + 1] = make_row_z<V>(by);
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-          float rx[V], ry[V];
-#pragma unroll
-          for (int e = 0; e < V; ++e) { rx[e] = dd[r][e] * cx[r][e]; ry[e] = dd[r][e] * cy[r][e]; }
-          xr[r + 1] = make_row_z<V>(rx);
-          yr[r + 1] = make_row_z<V>(ry);
-        }
-      }
-      Row<V> uxr[kD ? RW + 2 : 1], uyr[kD ? RW + 2 : 1];
-      if constexpr (kD) {
-#pragma unroll
-        for (int r = 0; r < RW + 2; ++r) {
-          uxr[r] = make_row_z<V>(urx[r]);
-          uyr[r] = make_row_z<V>(ury[r]);
-        }
-      }
-
-#pragma unroll
-      for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          const int x = e + 1;
-          const UDeriv dl = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], x);
-          apply_op_u(kl, dl, __builtin_fmaf(2.f, cx[r][e], -px[r][e]), __builtin_fmaf(2.f, cy[r][e], -py[r][e]),
-                     nx_[r][e], ny_[r][e]);
-          // gradient terms on the output tile (lam^{m+1} with u^m)
-          float ux = kD ? uxr[r + 1].v[x] : urx[r][e];
-          float uy = kD ? uyr[r + 1].v[x] : ury[r][e];
-          if (!owned[r][e]) { ux = 0.f; uy = 0.f; }
-          sbp_sums(dl, ux, uy, S);
-          if constexpr (kD) {
-            const UDeriv du = uderivs<V>(uxr[r], uxr[r + 1], uxr[r + 2], uyr[r], uyr[r + 1], uyr[r + 2], x);
-            float ox, oy;
-            apply_op_sum(ku, du, ox, oy);
-            if (owned[r][e]) gacc[r][e] += __builtin_fmaf(cx[r][e], ox, cy[r][e] * oy);
-          }
-        }
-
-      // The gradient sums are complete here.  The empty asm statements pin
-      // them in place: left free, the compiler sinks the gradient arithmetic
-      // below the injection branch, holding every row's derivatives live
-      // across it (damage mode at RW = 6: 205 instead of 166 VGPRs, two
-      // wavefronts per SIMD instead of three).
-#pragma unroll
-      for (int p = 0; p < kPartials; ++p) asm volatile("" : "+v"(S[p]));
-      if constexpr (kD) {
-#pragma unroll
-        for (int r = 0; r < RW; ++r)
-#pragma unroll
-          for (int e = 0; e < V; ++e) asm volatile("" : "+v"(gacc[r][e]));
-      }
-
-      // residual injection on recorded steps m = kP + 1 (the whole region)
-      if (((m - 1) % a.stride == 0) && ((m - 1) / a.stride < nf)) {
-        const int k = (m - 1) / a.stride;
-        if (a.snapshot) {
-#pragma unroll
-          for (int r = 0; r < RW; ++r)
-#pragma unroll
-            for (int e = 0; e < V; ++e)
-              if (inside[r][e]) {
-                const gfloat* gx = C.gtr + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + gj0 + e;
-                nx_[r][e] += gx[0];
-                ny_[r][e] += gx[(int64_t)nf * I.nx * I.ny];
-              }
-        } else if (use_map) {
-          if (rbits) {
-            const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec;
-            const int64_t cy = (int64_t)nf * I.n_rec;
-#pragma unroll
-            for (int r = 0; r < RW; ++r)
-#pragma unroll
-              for (int e = 0; e < V; ++e)
-                if ((rbits >> (r * V + e)) & 1u) {
-                  const int rec = recmap[w][r][lane * V + e];
-                  nx_[r][e] += gx[rec];
-                  ny_[r][e] += gx[rec + cy];
-                }
-          }
-        } else {
-          for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
-            const RecEv ev = a.rec[rp];
-            const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
-            if (rr < 0 || rr >= RW || lane != lc / V) continue;
-            const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
-            const float vgx = gx[0], vgy = gx[(int64_t)nf * I.n_rec];
-#pragma unroll
-            for (int r = 0; r < RW; ++r)
-#pragma unroll
-              for (int e = 0; e < V; ++e)
-                if (r == rr && e == lc % V) {
-                  nx_[r][e] += vgx;
-                  ny_[r][e] += vgy;
-                }
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e)
-          if (!inside[r][e]) { nx_[r][e] = 0.f; ny_[r][e] = 0.f; }
-    } else {
-#pragma unroll
-      for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e) { nx_[r][e] = 0.f; ny_[r][e] = 0.f; }
-    }
-
-    const int slot = a.store[s];
-    if (slot >= 0) {
-      gfloat* dx = pool + (int64_t)slot * 2 * I.field;
-#pragma unroll
-      for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e)
-          if (owned[r][e]) {
-            const uint32_t o = row_off(P, gi0 + r, gj0 + e);
-            *at(dx, o) = nx_[r][e];
-            *at(dx + I.field, o) = ny_[r][e];
-          }
-    }
-#pragma unroll
-    for (int r = 0; r < RW; ++r)
-#pragma unroll
-      for (int e = 0; e < V; ++e) {
-        px[r][e] = cx[r][e];
-        py[r][e] = cy[r][e];
-        cx[r][e] = nx_[r][e];
-        cy[r][e] = ny_[r][e];
-      }
-    if (s + 1 < a.ks) {
-      if (m - 1 <= I.nt) load_u(s + 1);
-      publish(buf ^ 1);
-      __syncthreads();
-    }
-  }
-
-  // constant-gradient partials of this wavefront (slot t * 4 + w; lane 8q holds slot q)
-  {
-    double Sd[kPartials];
-#pragma unroll
-    for (int p = 0; p < kPartials; ++p) Sd[p] = (double)S[p];
-    const double tot = wave_sum8(Sd, lane);
-    const int q = lane >> 3;
-    const double wq = (q == 2 || q >= 6) ? 0.25 * gsa : gsa;
-    if ((lane & 7) == 0) *part = part_in + wq * tot;
-  }
-  if constexpr (kD) {
-    gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
-#pragma unroll
-    for (int r = 0; r < RW; ++r)
-#pragma unroll
-      for (int e = 0; e < V; ++e)
-        if (owned[r][e]) {
-          const uint32_t o = row_off(P, gi0 + r, gj0 + e);
-          if (seg_last) acc64[o] += gsa * (double)gacc[r][e];
-          else *at(acc32, o) = gacc[r][e];
-        }
-  }
+  const BlockAdjArgs& a = fa.adj;
+  const int bid = blockIdx.x;
+#include "ernn_adj_block_body.inc"
 }
 
 // --------------------------------------------------------------------------
@@ -1703,6 +1434,32 @@ hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t 
   switch (a.mode) {
     case kConst6: return launch_adj_block_m<kConst6>(a, V, grid, st);
     case kDField: return launch_adj_block_m<kDField>(a, V, grid, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// Fused time-blocked backward shapes (fused_block_ok in ernn_plan.cpp): the
+// small-grid shapes of both halves, constant and damage modes.
+#define ERNN_FUSED_BLOCK_CASES(X) X(1, 4, 2) X(1, 4, 4) X(1, 8, 4)
+#define ERNN_FUSED_BLOCK_CASE(v_, r_, k_)                                                             \
+  if (V == v_ && fa.adj.rw == r_ && fa.adj.ksmax == k_) {                                            \
+    hipLaunchKernelGGL((fused_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kBlock), 0, st, fa); \
+    return hipGetLastError();                                                                        \
+  }
+template <int MODE>
+static hipError_t launch_fused_block_m(const FusedBlockArgs& fa, int V, int grid, hipStream_t st) {
+  ERNN_FUSED_BLOCK_CASES(ERNN_FUSED_BLOCK_CASE)
+  return hipErrorInvalidValue;
+}
+#undef ERNN_FUSED_BLOCK_CASE
+
+// grid = adjoint grid (a multiple of 8) + forward grid
+hipError_t launch_fused_block(const FusedBlockArgs& fa, int V, int grid, hipStream_t st) {
+  if (fa.fwd.rw != fa.adj.rw || fa.fwd.ksmax != fa.adj.ksmax || fa.fwd.mode != fa.adj.mode || (fa.adj_grid & 7))
+    return hipErrorInvalidValue;
+  switch (fa.adj.mode) {
+    case kConst6: return launch_fused_block_m<kConst6>(fa, V, grid, st);
+    case kDField: return launch_fused_block_m<kDField>(fa, V, grid, st);
   }
   return hipErrorInvalidValue;
 }

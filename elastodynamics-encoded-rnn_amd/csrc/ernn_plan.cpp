@@ -39,6 +39,7 @@ hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, bool loop, hipS
 hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t st);
 hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st);
 hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st);
+hipError_t launch_fused_block(const FusedBlockArgs& fa, int V, int grid, hipStream_t st);
 hipError_t launch_finalize_const_blocks(const CallDev* call, int64_t partial_off, const BlockDev* blocks,
                                         int n_blocks, int n_sets, hipStream_t st);
 hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st);
@@ -118,6 +119,19 @@ bool adj_block_ok(int mode, int V, int RW, int KS) {
   if (V == 2) return mode == ERNN_PARAM_CONST6 && RW == 4;
   return true;
 }
+// Fused time-blocked backward shapes (ERNN_FUSED_BLOCK_CASES): on a fuse
+// plan the adjoint blocking shape runs both halves of each fused launch.
+bool fused_block_ok(int mode, int V, int RW, int KS) {
+  if (mode == ERNN_PARAM_FIELD6 || V != 1) return false;
+  return (RW == 4 && (KS == 2 || KS == 4)) || (RW == 8 && KS == 4);
+}
+bool ablk_ok(int mode, bool fuse, int V, int RW, int KS) {
+  return fuse ? fused_block_ok(mode, V, RW, KS) : adj_block_ok(mode, V, RW, KS);
+}
+const char* const kAblkShapes =
+    "time-blocked adjoint shape (V, RW, KS): constant or damage mode; unfused: V = 1 with RW in {4,8}, "
+    "KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) V = 2, RW = 4, KS in {2,4}; fused: (1, 4, 2), "
+    "(1, 4, 4) or (1, 8, 4)";
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
@@ -153,7 +167,10 @@ struct Plan {
   InstDev* d_inst = nullptr;
   CallDev* d_call = nullptr;
   Tiling fwd, adj;
-  BlockTiling blk, blk_adj;
+  // blk: time-blocked forward; blk_adj: time-blocked adjoint; blk_ff: the
+  // forward-style table (sources over the region) with blk_adj's shape, for
+  // the recompute half of the fused time-blocked backward
+  BlockTiling blk, blk_adj, blk_ff;
   int64_t ws_floats = 0;
   int64_t partial_off = 0;  // in doubles
   int64_t partial_cap = 0;  // tiles
@@ -338,15 +355,13 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
   const bool adjoint = bt == &p->blk_adj;
   if (KS <= 1) {
     free_blocking(*bt);
+    if (adjoint) free_blocking(p->blk_ff);
     return ERNN_OK;
   }
   if (!adjoint && !block_ok(p->mode, V, RW, KS))
     return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
                                 "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
-  if (adjoint && !adj_block_ok(p->mode, V, RW, KS))
-    return fail(ERNN_E_INVALID, "time-blocked adjoint shape (V, RW, KS): constant or damage mode, V = 1 with "
-                                "RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) V = 2, "
-                                "RW = 4, KS in {2,4}");
+  if (adjoint && !ablk_ok(p->mode, p->fuse, V, RW, KS)) return fail(ERNN_E_INVALID, kAblkShapes);
   const int OR = kWavesPerBlock * RW - 2 * KS, OC = kWave * V - 2 * KS;
   std::vector<BlockDev> blocks;
   std::vector<SrcEv> sev;
@@ -429,6 +444,7 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
   ERNN_TRY(dev_upload(&bt->d_blocks, blocks));
   ERNN_TRY(dev_upload(&bt->d_src, sev));
   ERNN_TRY(dev_upload(&bt->d_rec, rev));
+  if (adjoint && p->fuse) return build_blocking(p, &p->blk_ff, V, RW, KS);
   return ERNN_OK;
 }
 
@@ -439,6 +455,7 @@ void destroy_plan(Plan* p) {
   free_tiling(p->adj);
   free_blocking(p->blk);
   free_blocking(p->blk_adj);
+  free_blocking(p->blk_ff);
   if (p->d_inst) (void)hipFree(p->d_inst);
   if (p->d_call) (void)hipFree(p->d_call);
   if (p->d_set_first) (void)hipFree(p->d_set_first);
@@ -468,10 +485,8 @@ int validate(const ernn_desc_t* d) {
   if (d->blk_ks > 1 && !block_ok(d->param_mode, d->blk_v, d->blk_rw, d->blk_ks))
     return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
                                 "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
-  if (d->ablk_ks > 1 && !adj_block_ok(d->param_mode, d->ablk_v, d->ablk_rw, d->ablk_ks))
-    return fail(ERNN_E_INVALID, "time-blocked adjoint shape (V, RW, KS): constant or damage mode, V = 1 with "
-                                "RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) V = 2, "
-                                "RW = 4, KS in {2,4}");
+  if (d->ablk_ks > 1 && !ablk_ok(d->param_mode, d->fuse != 0, d->ablk_v, d->ablk_rw, d->ablk_ks))
+    return fail(ERNN_E_INVALID, kAblkShapes);
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))
@@ -715,21 +730,22 @@ int enqueue_pack(const Plan* p, hipStream_t st) {
   return ERNN_OK;
 }
 
-BlockArgs block_args(const Plan* p) {
+BlockArgs block_args(const Plan* p, const BlockTiling& bt) {
   BlockArgs a{};
   a.inst = p->d_inst;
-  a.blocks = p->blk.d_blocks;
-  a.src = p->blk.d_src;
-  a.rec = p->blk.d_rec;
+  a.blocks = bt.d_blocks;
+  a.src = bt.d_src;
+  a.rec = bt.d_rec;
   a.call = p->d_call;
-  a.n_blocks = p->blk.n_blocks;
+  a.n_blocks = bt.n_blocks;
   a.mode = p->mode;
   a.stride = p->P;
   a.snapshot = p->snapshot;
-  a.rw = p->blk.RW;
-  a.ksmax = p->blk.KS;
+  a.rw = bt.RW;
+  a.ksmax = bt.KS;
   return a;
 }
+BlockArgs block_args(const Plan* p) { return block_args(p, p->blk); }
 
 // One time-blocked launch of the forward pass (record = 1) from step n0.
 int launch_blk_forward(const Plan* p, BlockArgs& a, int n0, hipStream_t st) {
@@ -742,14 +758,19 @@ int launch_blk_forward(const Plan* p, BlockArgs& a, int n0, hipStream_t st) {
   return ERNN_OK;
 }
 
-// Time-blocked recompute of segment s from step n0: every state is stored.
-int launch_blk_recompute(const Plan* p, BlockArgs& a, int s, int n0, int top, hipStream_t st) {
+// Time-blocked recompute arguments for segment s from step n0: every state
+// is stored.
+void blk_recompute_args(const Plan* p, BlockArgs& a, int s, int n0, int top) {
   a.n0 = n0;
-  a.ks = std::min(p->blk.KS, top - n0);
+  a.ks = std::min(a.ksmax, top - n0);
   a.s_prev = p->slot_seg(s, n0 - 1);
   a.s_cur = p->slot_seg(s, n0);
   for (int q = 0; q < kMaxBlockSteps; ++q) a.store[q] = q < a.ks ? p->slot_seg(s, n0 + 1 + q) : -1;
-  ERNN_HIP(launch_fwd_block(a, p->blk.V, p->blk.grid, st));
+}
+int launch_blk_recompute(const Plan* p, const BlockTiling& bt, BlockArgs& a, int s, int n0, int top,
+                         hipStream_t st) {
+  blk_recompute_args(p, a, s, n0, top);
+  ERNN_HIP(launch_fwd_block(a, bt.V, bt.grid, st));
   return ERNN_OK;
 }
 
@@ -773,8 +794,7 @@ BlockAdjArgs block_adj_args(const Plan* p) {
 // lam^{m0+2} from lam pair L & 1 (slots 1, 0), leaves its last two states in
 // pair (L + 1) & 1; u^m comes from slot u_of(m).
 template <typename UOf>
-int launch_blk_adjoint(const Plan* p, BlockAdjArgs& a, int L, int m0, int ks, int seg_top, int seg_bot, UOf u_of,
-                       hipStream_t st) {
+void blk_adjoint_args(const Plan* p, BlockAdjArgs& a, int L, int m0, int ks, int seg_top, int seg_bot, UOf u_of) {
   const int pr = 2 * (L & 1), pw = 2 * ((L + 1) & 1);
   a.m0 = m0;
   a.ks = ks;
@@ -787,6 +807,11 @@ int launch_blk_adjoint(const Plan* p, BlockAdjArgs& a, int L, int m0, int ks, in
   }
   a.seg_top = seg_top;
   a.seg_bot = seg_bot;
+}
+template <typename UOf>
+int launch_blk_adjoint(const Plan* p, BlockAdjArgs& a, int L, int m0, int ks, int seg_top, int seg_bot, UOf u_of,
+                       hipStream_t st) {
+  blk_adjoint_args(p, a, L, m0, ks, seg_top, seg_bot, u_of);
   ERNN_HIP(launch_adj_block(a, p->blk_adj.V, p->blk_adj.grid, st));
   return ERNN_OK;
 }
@@ -839,7 +864,8 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
   auto recompute_segment = [&](int s) -> int {
     const int top = seg_top(s);
     if (p->blk.KS > 1) {
-      for (int n0 = s * p->K; n0 < top; n0 += p->blk.KS) ERNN_TRY(launch_blk_recompute(p, rb, s, n0, top, st));
+      for (int n0 = s * p->K; n0 < top; n0 += p->blk.KS)
+        ERNN_TRY(launch_blk_recompute(p, p->blk, rb, s, n0, top, st));
       return ERNN_OK;
     }
     for (int n = s * p->K; n < top; ++n) {
@@ -848,7 +874,42 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
     }
     return ERNN_OK;
   };
-  if (!p->fuse && p->blk_adj.KS > 1) {
+  if (p->fuse && p->blk_adj.KS > 1) {
+    // The last segment is recomputed alone; afterwards every time-blocked
+    // adjoint launch of segment s carries the next KS recompute steps of
+    // segment s - 1 into the other bank (segment s - 1 has K steps, so a
+    // short last segment leaves recompute launches to run alone).
+    ERNN_TRY(recompute_segment(p->S - 1));
+    const int KS = p->blk_adj.KS;
+    FusedBlockArgs fb{};
+    fb.adj = block_adj_args(p);
+    fb.fwd = block_args(p, p->blk_ff);
+    fb.fwd.record = 0;
+    fb.adj_grid = p->blk_adj.grid;
+    const int grid = p->blk_adj.grid + p->blk_ff.grid;
+    int L = 0;
+    for (int s = p->S - 1; s >= 0; --s) {
+      const int top = seg_top(s), bot = s * p->K + 1;
+      const auto u_of = [&](int m) { return p->slot_seg(s, m); };
+      int n0 = (s - 1) * p->K;   // next recompute step of segment s - 1
+      for (int m0 = top; m0 >= bot; m0 -= KS, ++L) {
+        const int ks = std::min(KS, m0 - bot + 1);
+        if (s >= 1 && n0 < s * p->K) {
+          blk_adjoint_args(p, fb.adj, L, m0, ks, top, bot, u_of);
+          blk_recompute_args(p, fb.fwd, s - 1, n0, s * p->K);
+          ERNN_HIP(launch_fused_block(fb, p->blk_adj.V, grid, st));
+          n0 += KS;
+        } else {
+          ERNN_TRY(launch_blk_adjoint(p, fb.adj, L, m0, ks, top, bot, u_of, st));
+        }
+      }
+      for (; s >= 1 && n0 < s * p->K; n0 += KS) {
+        BlockArgs r = block_args(p, p->blk_ff);
+        r.record = 0;
+        ERNN_TRY(launch_blk_recompute(p, p->blk_ff, r, s - 1, n0, s * p->K, st));
+      }
+    }
+  } else if (!p->fuse && p->blk_adj.KS > 1) {
     BlockAdjArgs ba = block_adj_args(p);
     int L = 0;
     for (int s = p->S - 1; s >= 0; --s) {
@@ -908,7 +969,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
     }
   }
   if (p->mode != ERNN_PARAM_FIELD6) {
-    if (!p->fuse && p->blk_adj.KS > 1)
+    if (p->blk_adj.KS > 1)
       ERNN_HIP(launch_finalize_const_blocks(p->d_call, p->partial_off, p->blk_adj.d_blocks, p->blk_adj.n_blocks,
                                             p->n_sets, st));
     else
@@ -1084,13 +1145,27 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   ERNN_HIP(hipEventRecord(e0, st));
   const int nsteps = std::min(steps, p->nt_max);
   if (which == 4) {   // `steps` time-blocked adjoint launches of KS steps each, u^m from segment bank 0
+    // (fuse plans: fused launches, u^m from bank 1 while segment 0 is recomputed into bank 0)
     BlockAdjArgs ba = block_adj_args(p);
     const int KS = p->blk_adj.KS;
     const int launches = std::min(steps, (p->nt_max + KS - 1) / KS);
+    FusedBlockArgs fb{};
+    fb.fwd = block_args(p, p->blk_ff);
+    fb.fwd.record = 0;
+    fb.adj_grid = p->blk_adj.grid;
     for (int L = 0; L < launches; ++L) {
       const int m0 = p->nt_max - L * KS;
-      ERNN_TRY(launch_blk_adjoint(p, ba, L, m0, std::min(KS, m0), p->nt_max, 1,
-                                  [&](int m) { return p->seg(0, m % p->K); }, st));
+      const int bank = p->fuse ? 1 : 0;
+      const auto u_of = [&](int m) { return p->seg(bank, m % p->K); };
+      if (!p->fuse) {
+        ERNN_TRY(launch_blk_adjoint(p, ba, L, m0, std::min(KS, m0), p->nt_max, 1, u_of, st));
+        continue;
+      }
+      fb.adj = ba;
+      blk_adjoint_args(p, fb.adj, L, m0, std::min(KS, m0), p->nt_max, 1, u_of);
+      const int n0 = (L * KS) % std::max(KS, p->K - p->K % KS);
+      blk_recompute_args(p, fb.fwd, 0, n0, std::min(p->K, p->nt_max));
+      ERNN_HIP(launch_fused_block(fb, p->blk_adj.V, p->blk_adj.grid + p->blk_ff.grid, st));
     }
   } else if (which == 3) {   // `steps` time-blocked forward launches of KS steps each
     BlockArgs b = block_args(p);

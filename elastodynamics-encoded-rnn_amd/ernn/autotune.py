@@ -68,6 +68,8 @@ BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = (
 # time-blocked adjoint shapes (constant and damage modes; V = 2 for constant stiffness only)
 ADJ_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 6, 4), (1, 8, 2), (1, 8, 4),
                                                       (1, 8, 8), (2, 4, 2), (2, 4, 4))
+# time-blocked shapes of the fused backward (fuse plans): both halves of a launch share one shape
+FUSED_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 8, 4))
 # largest V * R and R of the built adjoint kernels per mode (larger shapes would spill registers)
 ADJ_MAX_CELLS = {"const6": 16, "dfield": 8, "field6": 4}
 ADJ_MAX_ROWS = {"const6": 16, "dfield": 4, "field6": 2}
@@ -112,18 +114,21 @@ def tune_tiles(engine, params, field, wavelets, steps: int = 50,
 
 def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int = 0,
                   shapes: Optional[Sequence[Tuple[int, int, int]]] = None, rounds: int = 5) -> tuple:
-    """Time the time-blocked forward (which 0) or unfused adjoint (which 1)
-    shapes against the current single-step tiling of that pass, per step, in
-    interleaved rounds; keep the fastest.
+    """Time the time-blocked forward (which 0) or adjoint (which 1) shapes
+    against the current single-step tiling of that pass, per step, in
+    interleaved rounds; keep the fastest.  On a fuse plan the adjoint shapes
+    are those of the fused time-blocked backward, timed per step of adjoint
+    plus recompute against the fused single-step kernel.
 
     Returns (V, RW, KS, ms_per_step, {shape: ms_per_step}); (0, 0, 0) is the
     single-step kernel."""
     mode = engine.survey.mode
+    fused = which == 1 and bool(engine.info.fuse)
     if which == 0:
         shapes = BLOCK_SHAPES if shapes is None else shapes
         cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or b[:2] != (2, 8)]
     else:
-        shapes = ADJ_BLOCK_SHAPES if shapes is None else shapes
+        shapes = (FUSED_BLOCK_SHAPES if fused else ADJ_BLOCK_SHAPES) if shapes is None else shapes
         cands = [(0, 0, 0)] + ([b for b in shapes if mode == "const6" or b[0] == 1] if mode != "field6" else [])
     ws = engine.acquire_workspace()
     traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)
@@ -131,7 +136,8 @@ def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int =
 
     def run(c):
         if c[2] <= 1:
-            return engine.time_steps(which, steps, params, field, wavelets, ws, traces, traces) / steps
+            return engine.time_steps(2 if fused else which, steps, params, field, wavelets, ws, traces,
+                                     traces) / steps
         launches = max(2, steps // c[2])
         return engine.time_steps(3 + which, launches, params, field, wavelets, ws, traces,
                                  traces) / (launches * c[2])
@@ -146,3 +152,47 @@ def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int =
     engine.set_blocking(*best, which=which)
     engine.release_workspace(ws)
     return (best[0], best[1], best[2], times[best], times)
+
+
+def tune_backward(survey, params, field, wavelets, steps: int = 48, rounds: int = 3) -> tuple:
+    """Choose the backward schedule of a survey by timing the candidates on
+    the real problem, per reverse step (adjoint + recompute):
+
+    - fused: each launch runs adjoint and recompute work of two consecutive
+      checkpoint segments, one step each (fused single-step kernel) or KS
+      steps each (fused time-blocked kernel), at the best tiles / shape;
+    - unfused: the segment recompute by the time-blocked forward, then the
+      time-blocked adjoint, each at its best shape.
+
+    The fused schedule fixes the plan's slot layout (two segment banks), so
+    each schedule is timed on its own temporary plan.  The survey's `fuse` is
+    left as it was.
+
+    Returns (fuse, adj_blocking, {"fused": ms_per_step, "blocked": ms_per_step})."""
+    from .engine import Engine
+    if survey.mode == "field6":
+        return survey.fused, (0, 0, 0), {}
+    saved = survey.fuse
+    times = {}
+    try:
+        survey.fuse = True
+        eng = Engine(survey, device=params.device)
+        try:
+            tune_tiles(eng, params, field, wavelets, steps=steps, which=(2,), rounds=rounds)
+            fz = tune_blocking(eng, params, field, wavelets, steps=steps, which=1, rounds=rounds)
+            times["fused"] = fz[3]
+        finally:
+            eng.close()
+        survey.fuse = False
+        eng = Engine(survey, device=params.device)
+        try:
+            tune_tiles(eng, params, field, wavelets, steps=steps, which=(0, 1), rounds=rounds)
+            fwd = tune_blocking(eng, params, field, wavelets, steps=steps, which=0, rounds=rounds)
+            adj = tune_blocking(eng, params, field, wavelets, steps=steps, which=1, rounds=rounds)
+            times["blocked"] = fwd[3] + adj[3]
+        finally:
+            eng.close()
+    finally:
+        survey.fuse = saved
+    fuse = times["fused"] <= times["blocked"]
+    return fuse, (fz[:3] if fuse else adj[:3]), times

@@ -97,9 +97,11 @@ typedef struct {
                            launch.  V in {1,2}, RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS; field
                            modes exclude (V, RW) = (2, 8).  Used by the forward pass and the
                            segment recompute of the unfused adjoint. */
-  int32_t ablk_v, ablk_rw, ablk_ks; /* time-blocked adjoint of the unfused backward (CONST6,
-                           DFIELD): ablk_ks reverse steps per launch; V = 1 with RW in {4,8},
-                           or (1, 6, 4), or (CONST6) V = 2 with RW = 4 and KS in {2,4};
+  int32_t ablk_v, ablk_rw, ablk_ks; /* time-blocked adjoint (CONST6, DFIELD): ablk_ks reverse
+                           steps per launch.  Unfused backward: V = 1 with RW in {4,8}, or
+                           (1, 6, 4), or (CONST6) V = 2 with RW = 4 and KS in {2,4}.  fuse = 1:
+                           each launch also runs ablk_ks recompute steps of the previous
+                           segment with the same shape, (1, 4, 2), (1, 4, 4) or (1, 8, 4).
                            ablk_ks <= 1 = off */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
@@ -191,7 +193,7 @@ int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream);
  * synchronizes the device and drops captured graphs. */
 int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc);
 
-/* Switch the forward (which = 0) or the unfused adjoint (which = 1) to ks
+/* Switch the forward (which = 0) or the adjoint (which = 1) to ks
  * steps per launch on blocks of the given shape (ks <= 1: one step per
  * launch); same limits as ernn_desc_t.blk_* / ablk_*.  No counterpart in the
  * reference: its time loop (PI:268-305) and autograd replay run one step at a
@@ -200,9 +202,10 @@ int ernn_plan_set_blocking(void* plan, int which, int v, int rw, int ks);
 
 /* Time `steps` forward (which = 0), adjoint (which = 1) or fused adjoint +
  * recompute (which = 2, fuse plans) steps with the current tiling, or `steps`
- * time-blocked forward (which = 3) or adjoint (which = 4) launches, using
- * hipEvents on `stream`; synchronizes.  The workspace must be initialized.
- * ms_out receives the elapsed milliseconds. */
+ * time-blocked forward (which = 3) or adjoint (which = 4; fuse plans: fused
+ * adjoint + recompute) launches, using hipEvents on `stream`; synchronizes.
+ * The workspace must be initialized.  ms_out receives the elapsed
+ * milliseconds. */
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
                     float* ms_out);
 

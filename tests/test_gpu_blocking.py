@@ -225,3 +225,74 @@ def test_receiver_line_and_shared_cells(mode):
         assert rel_l2(h1[1], h0[1]) < 1e-5
     for e in (e0, e1, a0, a1):
         e.close()
+
+
+# ------------------------------------------------------ fused time-blocked backward
+@pytest.mark.parametrize("shape", [(1, 4, 2), (1, 4, 4), (1, 8, 4)])
+@pytest.mark.parametrize("mode", ["const6", "dfield"])
+def test_fused_blocked_backward_matches_unfused_and_oracle(mode, shape):
+    """Each launch runs KS reverse steps of segment s beside KS recompute steps
+    of segment s - 1.  K = 7 with 47 steps leaves a short last segment (its
+    adjoint launches cannot carry all of segment s - 1's recompute, which then
+    runs alone) and one-step launches (7 mod KS); gradients vs the unfused
+    single-step backward (same recurrence) and the FP64 oracle."""
+    from gpu_helpers import oracle_run
+    from oracle import ernn_fp64 as orc
+    V, RW, KS = shape
+    nx, ny, nt, h = 45, 150, 47, 0.2e-3
+    rng = np.random.default_rng(13)
+    shot = make_shot(nx, ny, nt, h, sources=_edge_sources(nx, ny, h, V, RW, KS),
+                     receivers=_receivers(nx, ny, V, RW, KS))
+    G = rng.standard_normal((2, nt, len(shot.receivers)))
+    kw = {"params": C_ANISO}
+    if mode == "dfield":
+        kw["field"] = rng.uniform(0.4, 1.0, (nx, ny)).astype(np.float32)
+
+    def sv(fuse, ablk):
+        return ernn.Survey([shot], mode=mode, ckpt_every=7, tiles=(2, 2, 1, 4, 0, 0), fuse=fuse,
+                           adj_blocking=ablk, blocking=(1, 4, 4))
+    e0, t0, g0 = run_hip(sv(False, (0, 0, 0)), G=G, **kw)
+    e1, t1, g1 = run_hip(sv(True, shape), G=G, **kw)
+    assert e1.info.fuse == 1 and e1.adj_blocking == shape
+    assert np.array_equal(t0, t1)
+    err_ss = np.abs(g1[0] - g0[0]).max() / np.abs(g0[0]).max()
+    if mode == "const6":
+        _, gcell = oracle_run(shot, np.asarray(C_ANISO, np.float32).astype(np.float64), 1, False, G)
+        gref = orc.const_gradient(gcell)
+        err_f = 0.0
+    else:
+        D = kw["field"].astype(np.float64)
+        cells = [(np.float32(kw["field"]) * np.float32(c)).astype(np.float64) for c in C_ANISO]
+        _, gcell = oracle_run(shot, cells, 1, False, G)
+        gref, gD = orc.dfield_gradient(gcell, np.asarray(C_ANISO, np.float32).astype(np.float64), D)
+        err_f = rel_l2(g1[1].reshape(nx, ny), gD)
+        assert rel_l2(g1[1], g0[1]) < 1e-5
+    err_or = np.abs(g1[0] - gref).max() / np.abs(gref).max()
+    print(f"{mode} fused blocked {shape}: dJ/dC vs unfused single-step {err_ss:.2e}, vs FP64 {err_or:.2e}, "
+          f"dJ/dD rel-L2 vs FP64 {err_f:.2e}")
+    assert err_ss < 1e-5
+    assert err_or < 1e-4
+    assert err_f < 1e-4
+    e0.close(); e1.close()
+
+
+def test_fused_blocked_backward_uneven_batch_and_graph_replay():
+    """Instances of different sizes and step counts in one plan; captured
+    graph replays equal the eager pass."""
+    h = 0.2e-3
+    shots = [make_shot(nx, ny, nt, h, receivers=np.array([(0, ny - 1), (nx - 1, 0), (nx // 2, ny // 2)], np.int32))
+             for nx, ny, nt in ((5, 3, 29), (40, 90, 41), (33, 70, 36))]
+    G = np.random.default_rng(6).standard_normal(sum(2 * s.nt * len(s.receivers) for s in shots))
+
+    def sv(fuse, ablk, graph=False):
+        return ernn.Survey(shots, mode="const6", ckpt_every=6, fuse=fuse, adj_blocking=ablk, blocking=(1, 4, 4),
+                           use_graph=graph)
+    _, t0, g0 = run_hip(sv(False, (0, 0, 0)), params=C_ANISO, G=G)
+    eng = ernn.Engine(sv(True, (1, 4, 4), graph=True), device=DEV)
+    for _ in range(3):
+        _, t1, g1 = run_hip(eng.survey, params=C_ANISO, G=G, engine=eng)
+        assert np.array_equal(t0, t1)
+        err = np.abs(g1[0] - g0[0]).max() / np.abs(g0[0]).max()
+        assert err < 1e-5, err
+    print(f"fused blocked uneven batch, graph replays: dJ/dC vs unfused single-step {err:.2e}")
+    eng.close()
