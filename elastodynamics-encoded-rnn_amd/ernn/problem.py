@@ -174,16 +174,18 @@ class Survey:
     # for the forward and adjoint (0 = one tile per wavefront)
     tiles: Tuple[int, ...] = (0, 0, 0, 0, 0, 0)
     use_graph: bool = False
-    # Share each adjoint launch with a recompute step of the next checkpoint
-    # segment.  None = automatic: on while a step has at most FUSE_MAX_CELLS
-    # cells (launch-latency bound: measured 8.2 us fused vs 4.7 + 7.1 us
-    # separate at 512^2), off above (the fused kernel inherits the adjoint's
-    # register footprint: 234 us vs 42 + 141 us at 8 x 1024^2).
+    # Share each adjoint launch with recompute work of the previous checkpoint
+    # segment: one step each, or KS each with a time-blocked adj_blocking.
+    # None = automatic: on while a step has at most FUSE_MAX_CELLS cells
+    # (latency bound), off above (no gain at 8 x 1024^2, and the fused
+    # single-step kernel inherits the adjoint's register footprint).
+    # autotune.tune_backward picks it by timing on the real problem.
     fuse: Optional[bool] = None
     # Time-blocked forward (V, RW, KS): KS steps per launch on blocks of four
     # wavefronts x RW rows x 64 V columns; (0, 0, 0) = one step per launch.
     blocking: Tuple[int, int, int] = (0, 0, 0)
-    # Time-blocked adjoint of the unfused backward (V, RW, KS), const6 / dfield
+    # Time-blocked adjoint (V, RW, KS), const6 / dfield; with fuse, the shape of
+    # the fused time-blocked backward ((1, 4, 2), (1, 4, 4) or (1, 8, 4))
     adj_blocking: Tuple[int, int, int] = (0, 0, 0)
 
     def __post_init__(self):
