@@ -212,8 +212,14 @@ def cpu_baseline(torch, wl, budget_s):
     n = int(max(10, min(s.nt, 10 * budget_s / max(t_probe, 1e-3))))
     t = run(n)
     rate = s.nx * s.ny * n / t / 1e9
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu = next(ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": round(rate, 5), "unit": "Gcell-steps/s (fwd+adjoint)", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu": cpu,
             "sample": f"oracle/plain_torch.py forward + autograd backward, {s.nx}x{s.ny} grid, "
                       f"{n} of {s.nt} steps, torch {threads} threads, {t:.1f} s"}
 

@@ -296,3 +296,24 @@ def test_fused_blocked_backward_uneven_batch_and_graph_replay():
         assert err < 1e-5, err
     print(f"fused blocked uneven batch, graph replays: dJ/dC vs unfused single-step {err:.2e}")
     eng.close()
+
+
+@pytest.mark.parametrize("K", [2, 3])
+def test_fused_blocked_backward_short_segments_and_snapshot(K):
+    """Checkpoint intervals shorter than KS = 4 (every launch of both halves is
+    partial) with whole-field residuals at stride 3."""
+    nx, ny, nt, h = 40, 90, 31, 0.2e-3
+    shot = make_shot(nx, ny, nt, h)
+    nf = (nt + 2) // 3
+    G = np.random.default_rng(K).standard_normal(2 * nf * nx * ny)
+
+    def sv(fuse, ablk):
+        return ernn.Survey([shot], mode="const6", ckpt_every=K, snapshot=True, rec_stride=3, fuse=fuse,
+                           adj_blocking=ablk, blocking=(1, 4, 4))
+    _, t0, g0 = run_hip(sv(False, (0, 0, 0)), params=C_ANISO, G=G)
+    e1, t1, g1 = run_hip(sv(True, (1, 4, 4)), params=C_ANISO, G=G)
+    assert np.array_equal(t0, t1)
+    err = np.abs(g1[0] - g0[0]).max() / np.abs(g0[0]).max()
+    print(f"fused blocked K={K}, snapshot stride 3: dJ/dC vs unfused single-step {err:.2e}")
+    assert err < 1e-5
+    e1.close()
