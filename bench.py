@@ -224,6 +224,32 @@ def cpu_baseline(torch, wl, budget_s):
                       f"{n} of {s.nt} steps, torch {threads} threads, {t:.1f} s"}
 
 
+def cpu_oracle(wl, budget_s):
+    """The FP64 NumPy oracle (forward with kept states + discrete adjoint) on
+    one host core over a bounded window of the first shot's steps."""
+    import numpy as np
+    sys.path.insert(0, REPO)
+    from oracle import ernn_fp64 as orc
+    s = wl.survey.shots[0]
+    C0 = np.asarray(wl.C_true[0], np.float64)
+    C = C0 if wl.survey.mode == "const6" else [np.asarray(wl.D_true, np.float64) * c for c in C0]
+
+    def run(n):
+        t0 = time.perf_counter()
+        tr, st = orc.forward(C, s.h, s.dt, s.rho, n, s.nx, s.ny, s.sources, s.wavelet[:n].astype(np.float64), 1,
+                             s.receivers, keep_states=True)
+        orc.adjoint_gradient(C, s.h, s.dt, s.rho, n, s.nx, s.ny, st, np.ones_like(tr), 1, s.receivers)
+        return time.perf_counter() - t0
+
+    t_probe = run(4)
+    n = int(max(4, min(s.nt, 4 * budget_s / max(t_probe, 1e-3))))
+    t = run(n)
+    return {"value": round(s.nx * s.ny * n / t / 1e9, 6), "unit": "Gcell-steps/s (fwd+adjoint)", "cores": 1,
+            "kind": "oracle",
+            "sample": f"oracle/ernn_fp64.py FP64 forward + discrete adjoint, {s.nx}x{s.ny} grid, {n} of {s.nt} "
+                      f"steps, NumPy on one core, {t:.1f} s"}
+
+
 def time_instances(torch, ernn, wl, C_init, D_init, dev, iters):
     """Per-instance fwd+adjoint time (each instance alone on the GPU)."""
     import numpy as np
@@ -443,6 +469,8 @@ def main():
         torch.set_num_threads(min(os.cpu_count() or 1, 16))
         cpu = cpu_baseline(torch, wl, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']} Gcell-steps/s on {cpu['cores']} threads")
+        cpu["fp64_oracle"] = cpu_oracle(wl, args.cpu_seconds / 3)
+        log(f"cpu FP64 oracle: {cpu['fp64_oracle']['value']} Gcell-steps/s on one core")
 
     s0 = sv.shots[0]
     out = {
