@@ -103,8 +103,10 @@ bool fwd_tile_ok(int mode, int V, int R) {
 
 // Time-blocked forward shapes (ERNN_BLOCK_CASES in ernn_kernels.hip): V in
 // {1, 2}, RW in {4, 8} rows per wavefront, KS in {2, 4, 8} steps with an
-// output tile of 4 RW - 2 KS > 0 rows; (2, 8) would spill for field modes.
+// output tile of 4 RW - 2 KS > 0 rows, and (1, 8, 5); (2, 8) would spill for
+// field modes.
 bool block_ok(int mode, int V, int RW, int KS) {
+  if (V == 1 && RW == 8 && KS == 5) return true;
   if (!(V == 1 || V == 2) || !(RW == 4 || RW == 8) || !(KS == 2 || KS == 4 || KS == 8)) return false;
   if (kWavesPerBlock * RW - 2 * KS <= 0) return false;
   return !(mode != ERNN_PARAM_CONST6 && V == 2 && RW == 8);
@@ -115,6 +117,7 @@ bool block_ok(int mode, int V, int RW, int KS) {
 bool adj_block_ok(int mode, int V, int RW, int KS) {
   if (mode == ERNN_PARAM_FIELD6) return false;
   if (RW == 6) return V == 1 && KS == 4;
+  if (KS == 5) return mode == ERNN_PARAM_CONST6 && V == 1 && RW == 8;
   if (!block_ok(mode, V, RW, KS)) return false;
   if (V == 2) return mode == ERNN_PARAM_CONST6 && RW == 4;
   return true;
@@ -130,8 +133,8 @@ bool ablk_ok(int mode, bool fuse, int V, int RW, int KS) {
 }
 const char* const kAblkShapes =
     "time-blocked adjoint shape (V, RW, KS): constant or damage mode; unfused: V = 1 with RW in {4,8}, "
-    "KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) V = 2, RW = 4, KS in {2,4}; fused: (1, 4, 2), "
-    "(1, 4, 4) or (1, 8, 4)";
+    "KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) (1, 8, 5) or V = 2, RW = 4, KS in {2,4}; fused: "
+    "(1, 4, 2), (1, 4, 4) or (1, 8, 4)";
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
@@ -360,7 +363,8 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
   }
   if (!adjoint && !block_ok(p->mode, V, RW, KS))
     return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
-                                "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
+                                "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8), "
+                                "or be (1, 8, 5)");
   if (adjoint && !ablk_ok(p->mode, p->fuse, V, RW, KS)) return fail(ERNN_E_INVALID, kAblkShapes);
   const int OR = kWavesPerBlock * RW - 2 * KS, OC = kWave * V - 2 * KS;
   std::vector<BlockDev> blocks;
@@ -484,7 +488,8 @@ int validate(const ernn_desc_t* d) {
                                 "(damage field: V*R <= 8 and R <= 4, six fields: V*R <= 4 and R <= 2)");
   if (d->blk_ks > 1 && !block_ok(d->param_mode, d->blk_v, d->blk_rw, d->blk_ks))
     return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
-                                "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8)");
+                                "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8), "
+                                "or be (1, 8, 5)");
   if (d->ablk_ks > 1 && !ablk_ok(d->param_mode, d->fuse != 0, d->ablk_v, d->ablk_rw, d->ablk_ks))
     return fail(ERNN_E_INVALID, kAblkShapes);
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");

@@ -64,10 +64,11 @@ DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
     (v, r, b) for (v, r) in TILE_SHAPES for b in GRID_CAPS)
 # time-blocked forward shapes (V, RW, KS); field modes leave out (V, RW) = (2, 8)
 BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = (
-    (1, 4, 2), (1, 4, 4), (1, 8, 2), (1, 8, 4), (1, 8, 8), (2, 4, 2), (2, 4, 4), (2, 8, 2), (2, 8, 4), (2, 8, 8))
+    (1, 4, 2), (1, 4, 4), (1, 8, 2), (1, 8, 4), (1, 8, 5), (1, 8, 8), (2, 4, 2), (2, 4, 4), (2, 8, 2), (2, 8, 4),
+    (2, 8, 8))
 # time-blocked adjoint shapes (constant and damage modes; V = 2 for constant stiffness only)
 ADJ_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 6, 4), (1, 8, 2), (1, 8, 4),
-                                                      (1, 8, 8), (2, 4, 2), (2, 4, 4))
+                                                      (1, 8, 5), (1, 8, 8), (2, 4, 2), (2, 4, 4))
 # time-blocked shapes of the fused backward (fuse plans): both halves of a launch share one shape
 FUSED_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 8, 4))
 # largest V * R and R of the built adjoint kernels per mode (larger shapes would spill registers)
@@ -129,7 +130,8 @@ def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int =
         cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or b[:2] != (2, 8)]
     else:
         shapes = (FUSED_BLOCK_SHAPES if fused else ADJ_BLOCK_SHAPES) if shapes is None else shapes
-        cands = [(0, 0, 0)] + ([b for b in shapes if mode == "const6" or b[0] == 1] if mode != "field6" else [])
+        cands = [(0, 0, 0)] + ([b for b in shapes if mode == "const6" or (b[0] == 1 and b[2] != 5)]
+                               if mode != "field6" else [])
     ws = engine.acquire_workspace()
     traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)
     samples = {c: [] for c in cands}

@@ -94,12 +94,13 @@ typedef struct {
                            previous checkpoint segment (one launch; K more slots per instance) */
   int32_t blk_v, blk_rw, blk_ks; /* time-blocked forward: blk_ks steps per launch on blocks of 4
                            wavefronts x blk_rw rows x 64 blk_v columns; blk_ks <= 1 = one step per
-                           launch.  V in {1,2}, RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS; field
-                           modes exclude (V, RW) = (2, 8).  Used by the forward pass and the
-                           segment recompute of the unfused adjoint. */
+                           launch.  V in {1,2}, RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or
+                           (1, 8, 5); field modes exclude (V, RW) = (2, 8).  Used by the
+                           forward pass and the segment recompute. */
   int32_t ablk_v, ablk_rw, ablk_ks; /* time-blocked adjoint (CONST6, DFIELD): ablk_ks reverse
                            steps per launch.  Unfused backward: V = 1 with RW in {4,8}, or
-                           (1, 6, 4), or (CONST6) V = 2 with RW = 4 and KS in {2,4}.  fuse = 1:
+                           (1, 6, 4), or (CONST6) (1, 8, 5) or V = 2 with RW = 4 and KS in
+                           {2,4}.  fuse = 1:
                            each launch also runs ablk_ks recompute steps of the previous
                            segment with the same shape, (1, 4, 2), (1, 4, 4) or (1, 8, 4).
                            ablk_ks <= 1 = off */

@@ -18,7 +18,7 @@ from gpu_helpers import C_ANISO, DEV, make_shot, rel_l2, run_hip
 pytestmark = pytest.mark.gpu
 
 # (V, RW, KS): output tiles of (4 RW - 2 KS) x (64 V - 2 KS) cells
-SHAPES = [(1, 4, 2), (1, 4, 4), (1, 8, 8), (2, 4, 4), (2, 8, 4), (1, 8, 2)]
+SHAPES = [(1, 4, 2), (1, 4, 4), (1, 8, 8), (2, 4, 4), (2, 8, 4), (1, 8, 2), (1, 8, 5)]
 
 
 def _edge_sources(nx, ny, h, V, RW, KS):
@@ -126,7 +126,7 @@ def test_set_blocking_switches_and_replays_graphs():
 
 
 # ------------------------------------------------------------- blocked adjoint
-ADJ_SHAPES = [(1, 4, 2), (1, 4, 4), (1, 6, 4), (1, 8, 4), (1, 8, 8), (2, 4, 4)]
+ADJ_SHAPES = [(1, 4, 2), (1, 4, 4), (1, 6, 4), (1, 8, 4), (1, 8, 5), (1, 8, 8), (2, 4, 4)]
 
 
 def _adj_pair(shots, mode, ablk, K, snapshot=False, stride=1, **kw):
@@ -142,7 +142,7 @@ def test_blocked_adjoint_matches_single_step_and_oracle(mode, shape):
     """Gradients of the time-blocked adjoint vs the single-step adjoint (same
     recurrence, gradient sums taken in another order) and vs the FP64 oracle;
     K = 7 leaves one-step launches (7 mod KS = 1 or 3) at segment ends."""
-    if mode == "dfield" and shape[0] == 2:
+    if mode == "dfield" and (shape[0] == 2 or shape[2] == 5):
         pytest.skip("not built for the damage mode")
     from gpu_helpers import oracle_run
     from oracle import ernn_fp64 as orc
