@@ -276,7 +276,7 @@ def time_instances(torch, ernn, wl, C_init, D_init, dev, iters):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Shots would shard across GPUs with one gradient all-reduce per Adam iteration.", file=sys.stderr)
         sys.exit(2)
     import numpy as np
