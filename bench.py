@@ -273,12 +273,198 @@ def time_instances(torch, ernn, wl, C_init, D_init, dev, iters):
             "sum_s": round(float(t.sum()), 5), "n": len(t)}
 
 
+# FEM observed-data generator runs (the reference scripts' main()):
+# fem-pf  Parameter_inversion/FEM_generate_data.py:823-898, fem-df  Damage_inversion/FEM_generate_data.py:1486-1560
+FEM_WORKLOADS = {
+    "fem-pf": dict(t_total=60e-6, nt=2001, damage=None),
+    "fem-df": dict(t_total=120e-6, nt=4001, damage=[
+        {"type": "circle", "center": (0.1, 0.1), "size": 0.02, "reduction_factor": 0.5},
+        {"type": "square", "center": (-0.1, -0.1), "size": 0.02, "reduction_factor": 0.5},
+        {"type": "square", "center": (-0.1, -0.06), "size": 0.02, "reduction_factor": 0.5},
+        {"type": "square", "center": (-0.06, -0.1), "size": 0.02, "reduction_factor": 0.5}]),
+}
+FEM_C = (130.74e9, 11.50e9, 10.49e9, 30.67e9, 3.75e9, 14.77e9)   # PF:840-845
+FEM_RHO = 1610.0
+
+
+def fem_torch_program(torch, fem, nx, L, C6, rho, dt, g32, w32, steps, dev, D=None):
+    """Seconds per step of the reference's own FEM tensor program on the same
+    GPU, restated op for op (PF:322-339 mass, PF:363-397 Jacobians, PF:417-457
+    operator with three Gauss points and six index_add_ scatters, PF:481-497
+    update on the (nodes, nt) history array), over `steps` steps."""
+    import numpy as np
+    x, y, el_np, bnd_np, dx, dy = fem.create_structured_triangular_mesh(nx, nx, L, L)
+    N, E = nx * nx, el_np.shape[0]
+    el = torch.from_numpy(el_np).to(dev)
+    bnd = torch.from_numpy(bnd_np).to(dev)
+    M = torch.zeros(N, dtype=torch.float32, device=dev)
+    for i in range(3):
+        M.index_add_(0, el[:, i], torch.full((E,), rho * (dx * dy / 2.0) / 3.0, dtype=torch.float32, device=dev))
+    xc, yc = x[el_np % nx], y[el_np // nx]
+    J = np.stack([np.stack([xc[:, 1] - xc[:, 0], xc[:, 2] - xc[:, 0]], 1),
+                  np.stack([yc[:, 1] - yc[:, 0], yc[:, 2] - yc[:, 0]], 1)], 1)
+    detJ = torch.from_numpy(np.abs(np.linalg.det(J))).float().to(dev)
+    Jinv = torch.from_numpy(np.linalg.inv(J)).float().to(dev)
+    dxi = torch.tensor([-1.0, 1.0, 0.0], device=dev)
+    deta = torch.tensor([-1.0, 0.0, 1.0], device=dev)
+    wg = torch.tensor([1 / 6] * 3, device=dev)
+    if D is None:
+        Ce = [torch.tensor(c, dtype=torch.float32, device=dev) for c in C6]
+    else:   # per-element C at the centroid node (DF:1003-1023)
+        ci = np.abs(x[None, :] - xc.mean(1)[:, None]).argmin(1)
+        cj = np.abs(y[None, :] - yc.mean(1)[:, None]).argmin(1)
+        Ce = [torch.from_numpy(c * D.astype(np.float64)[ci, cj]).float().to(dev) for c in C6]
+    C11, C22, C12, C16, C26, C66 = Ce
+    gn = torch.from_numpy(g32.T.flatten()).float().to(dev)
+    tf = torch.from_numpy(w32).to(dev)
+    ux = torch.zeros((N, steps + 2), dtype=torch.float32, device=dev)
+    uy = torch.zeros((N, steps + 2), dtype=torch.float32, device=dev)
+
+    def K(u_x, u_y):
+        f_x = torch.zeros(N, dtype=torch.float32, device=dev)
+        f_y = torch.zeros(N, dtype=torch.float32, device=dev)
+        for gp in range(3):
+            uex, uey = u_x[el], u_y[el]
+            dNdx = Jinv[:, 0, 0].unsqueeze(1) * dxi.unsqueeze(0) + Jinv[:, 1, 0].unsqueeze(1) * deta.unsqueeze(0)
+            dNdy = Jinv[:, 0, 1].unsqueeze(1) * dxi.unsqueeze(0) + Jinv[:, 1, 1].unsqueeze(1) * deta.unsqueeze(0)
+            exx = torch.sum(dNdx * uex, dim=1)
+            eyy = torch.sum(dNdy * uey, dim=1)
+            gxy = torch.sum(dNdy * uex + dNdx * uey, dim=1)
+            sxx = C11 * exx + C12 * eyy + C16 * gxy
+            syy = C12 * exx + C22 * eyy + C26 * gxy
+            txy = C16 * exx + C26 * eyy + C66 * gxy
+            fac = detJ * wg[gp]
+            fex = fac.unsqueeze(1) * (dNdx * sxx.unsqueeze(1) + dNdy * txy.unsqueeze(1))
+            fey = fac.unsqueeze(1) * (dNdy * syy.unsqueeze(1) + dNdx * txy.unsqueeze(1))
+            for i in range(3):
+                f_x.index_add_(0, el[:, i], fex[:, i])
+                f_y.index_add_(0, el[:, i], fey[:, i])
+        return f_x, f_y
+
+    def run(n_steps):
+        for n in range(1, n_steps + 1):
+            fy_ext = tf[n] * gn * (1.0 / rho) * M
+            fix, fiy = K(ux[:, n], uy[:, n])
+            ux[:, n + 1] = 2 * ux[:, n] - ux[:, n - 1] - (dt * dt) * fix / M
+            uy[:, n + 1] = 2 * uy[:, n] - uy[:, n - 1] + (dt * dt) * (fy_ext - fiy) / M
+            ux[bnd, n + 1] = 0
+            uy[bnd, n + 1] = 0
+
+    run(3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(steps)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def fem_main(args):
+    """One FEM generator run of the reference's main() on the HIP path: tile
+    autotune, `warmup` + `steps` timed whole runs (graph replay), the step
+    kernel's roofline, the reference's tensor program on the same GPU and the
+    FP64 oracle on one CPU core over bounded windows."""
+    import numpy as np
+    import torch
+    from ernn import fem
+    dev = torch.device("cuda:0")
+    spec = FEM_WORKLOADS[args.workload]
+    nx, L = 301, 0.4
+    x = np.linspace(-L / 2, L / 2, nx)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    vp, vs, _ = fem.calculate_wave_velocities_anisotropic(*FEM_C, FEM_RHO)
+    nt_req = args.nt or spec["nt"]
+    dt, nt, t = fem.time_axis(nt_req, spec["t_total"] * (nt_req - 1) / (spec["nt"] - 1), L / 300, L / 300,
+                              max(vp, vs))
+    g32 = fem.gaussian_point_load(X, Y, 0.0, 0.0, 5e6).astype(np.float32)
+    w32 = np.asarray(fem.ricker_wavelet(t, 100e3, 10e-6, 1e10), np.float32)
+    D = None
+    if spec["damage"]:
+        mask, _ = fem.create_damage_regions(X, Y, spec["damage"])
+        D = np.where(mask, 0.5, 1.0).astype(np.float32)
+    Dt = torch.tensor(D, device=dev) if D is not None else None
+    stride = 5
+    eng = fem.FEMEngine(nx, nx, L / 300, L / 300, nt, dt, FEM_RHO, fem.nodal_loads(g32), damage=D is not None,
+                        stride=stride, use_graph=bool(args.graph), device=dev)
+    C32 = np.asarray(FEM_C, np.float32)
+    tune = eng.tune(C32, w32, Dt, steps=min(200, nt - 2)) if args.autotune else None
+    if tune:
+        log("fem tiles: " + ", ".join(f"{k}: {v / min(200, nt - 2) * 1e3:.2f} us" for k, v in sorted(tune.items())))
+    out = (torch.empty((eng.n_frames, nx, nx), dtype=torch.float32, device=dev),
+           torch.empty((eng.n_frames, nx, nx), dtype=torch.float32, device=dev))
+    for _ in range(args.warmup):
+        eng.run(C32, w32, Dt, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.run(C32, w32, Dt, out=out)
+    torch.cuda.synchronize()
+    run_s = (time.perf_counter() - t0) / args.steps
+    nodes = nx * nx
+    node_steps = nodes * (nt - 2)
+    value = node_steps / run_s / 1e9
+    step_ms = eng.time_steps(nt - 2, C32, w32, Dt, out=out) / (nt - 2)
+    # algorithmic bytes per node and step: read u^n, u^{n-1} (2 components), write u^{n+1}
+    # (+ the damage factor), and the frame written every `stride` steps
+    bpn = 24 + (4 if D is not None else 0) + 8.0 / stride
+    achieved = nodes * bpn / (step_ms * 1e-3) / 1e9
+    traffic = valu = None
+    tpath = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        entry = json.load(open(tpath)).get(f"{args.workload}/fem_step_kernel")
+        if entry:
+            traffic = entry["bytes"]
+            if entry.get("valu_insts"):
+                rate = entry["valu_insts"] / (step_ms * 1e-3)
+                valu = {"insts_per_launch": entry["valu_insts"], "issue_rate": round(rate / 1e9, 1),
+                        "peak": VALU_PEAK_WIPS / 1e9, "unit": "G wave-instructions/s",
+                        "frac": round(rate / VALU_PEAK_WIPS, 4)}
+    roofline = {"bound": "hbm", "kernel": "fem_step_kernel", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "valu": valu,
+                "bytes_per_node": bpn, "nodes_per_launch": nodes, "avg_launch_us": round(step_ms * 1e3, 3),
+                "launches_per_run": nt - 2}
+    log(f"fem run {run_s * 1e3:.2f} ms ({value:.2f} Gnode-steps/s), step kernel {step_ms * 1e3:.2f} us")
+    baseline = None
+    if args.baseline:
+        win = min(args.baseline_window, 200, nt - 2)
+        s = fem_torch_program(torch, fem, nx, L, FEM_C, FEM_RHO, dt, g32, w32, win, dev, D)
+        baseline = {"s_per_step": round(s, 6), "s_per_run": round(s * (nt - 2), 3), "window_steps": win,
+                    "value": round(nodes / s / 1e9, 5), "unit": "Gnode-steps/s"}
+        log(f"reference tensor program on the GPU: {s * 1e3:.3f} ms per step")
+    cpu = None
+    if args.cpu:
+        from oracle import fem_fp64
+        n_win = max(10, min(nt - 2, int(args.cpu_seconds * 20)))
+        t0 = time.perf_counter()
+        fem_fp64.solve(nx, nx, L, L, dt, n_win + 2, np.asarray(C32, np.float64), FEM_RHO, g32, w32[:n_win + 2],
+                       C_fields=None if D is None else [D.astype(np.float64) * float(c) for c in C32], stride=stride)
+        s = (time.perf_counter() - t0) / n_win
+        cpu = {"value": round(nodes / s / 1e9, 5), "unit": "Gnode-steps/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 oracle (sparse element assembly + {n_win} steps) of the same run"}
+        log(f"cpu FP64 oracle: {cpu['value']} Gnode-steps/s on one core")
+    line = {
+        "metric": "FEM generator Gnode-steps/s (1 GPU)", "value": round(value, 3), "unit": "Gnode-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(run_s * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "grid": f"{nx}x{nx}", "nt": nt, "frames_every": stride,
+                   "damage": D is not None, "tile": list(eng.tile), "graph": bool(args.graph),
+                   "parallelism": "none"},
+        "s_per_run": round(run_s, 5),
+        "reference_program_on_gpu": baseline,
+        "speedup_vs_reference_program": round(baseline["s_per_run"] / run_s, 1) if baseline else None,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    eng.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Shots would shard across GPUs with one gradient all-reduce per Adam iteration.", file=sys.stderr)
         sys.exit(2)
+    if args.workload in FEM_WORKLOADS:
+        return fem_main(args)
     import numpy as np
     import torch
     import ernn

@@ -1,0 +1,361 @@
+// Host side of the FEM observed-data generator: descriptor validation, the
+// tile and nodal-load tables, the run schedule (zero state and frames, pack
+// the damage factor, form the edge-flux matrix, nt - 2 step launches) with
+// optional hipGraph capture, and the ernn_fem_* C ABI of include/ernn.h.
+//
+// Reference: Parameter_inversion/FEM_generate_data.py (PF) :196-590 and
+// Damage_inversion/FEM_generate_data.py (DF) :770-1224.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ernn.h"
+#include "ernn_internal.h"
+
+namespace ernn {
+int set_last_error(int code, const std::string& msg);
+bool fem_shape_ok(int V, int R);
+hipError_t launch_fem_step(const FemStepArgs& a, int V, int R, int blocks, hipStream_t st);
+hipError_t launch_fem_setup(const FemCallDev* call, int64_t field, double dx, double dy, hipStream_t st);
+hipError_t launch_fem_pack(const FemCallDev* call, int64_t field, int nx, int ny, int pitch, hipStream_t st);
+hipError_t launch_fem_zero(const FemCallDev* call, int which, int64_t n, hipStream_t st);
+hipError_t launch_fem_set_call(FemCallDev* dst, const FemCallDev& v, hipStream_t st);
+}  // namespace ernn
+
+using namespace ernn;
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_last_error(code, msg); }
+
+#define FEM_HIP(x)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      return fail(ERNN_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define FEM_TRY(x)              \
+  do {                          \
+    int r_ = (x);               \
+    if (r_ != ERNN_OK) return r_; \
+  } while (0)
+
+struct FemPlan {
+  int device = 0;
+  int nx = 0, ny = 0, nt = 0, mode = 0, stride = 1;
+  double dx = 0, dy = 0, dt = 0, rho = 0;
+  float scale = 0.7f;
+  int V = 1, R = 2;
+  int pitch = 0;
+  int64_t field = 0;       // floats per padded field
+  int64_t ws_floats = 0;
+  int64_t frame = 0;       // nx * ny
+  int n_frames = 0;
+  int n_tiles = 0, blocks = 0;
+  float dt2 = 0, rinv = 0, m_even = 0, m_odd = 0;
+  int use_graph = 0;
+  std::vector<ernn_source_t> loads;   // interior loads
+  FemTile* d_tiles = nullptr;
+  SrcEv* d_loads = nullptr;
+  FemCallDev* d_call = nullptr;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+};
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+void free_tables(FemPlan* p) {
+  if (p->d_tiles) (void)hipFree(p->d_tiles);
+  if (p->d_loads) (void)hipFree(p->d_loads);
+  p->d_tiles = nullptr;
+  p->d_loads = nullptr;
+}
+
+void destroy(FemPlan* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  if (p->graph) (void)hipGraphExecDestroy(p->graph);
+  free_tables(p);
+  if (p->d_call) (void)hipFree(p->d_call);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  delete p;
+}
+
+// Tiles of R node rows x 64 V node columns over the interior rows 1 .. nx-2
+// (columns from 0, so that the rows stay aligned), ordered row band by row
+// band so that each XCD's contiguous share of tiles is a band of rows; the
+// loads of a tile sorted by row.
+int build_tables(FemPlan* p) {
+  free_tables(p);
+  const int cols = 64 * p->V;
+  const int tr = (p->nx - 2 + p->R - 1) / p->R;
+  const int tc = (p->ny - 1 + cols - 1) / cols;
+  p->n_tiles = tr * tc;
+  const int per_xcd = (p->n_tiles + 7) / 8;
+  p->blocks = 8 * std::max(1, (per_xcd + kWavesPerBlock - 1) / kWavesPerBlock);
+  std::vector<std::vector<SrcEv>> ev(p->n_tiles);
+  for (const auto& l : p->loads) {
+    const int t = ((l.i - 1) / p->R) * tc + l.j / cols;
+    ev[t].push_back(SrcEv{l.i, l.j, l.w, l.comp});
+  }
+  std::vector<FemTile> tiles(p->n_tiles);
+  std::vector<SrcEv> flat;
+  for (int a = 0; a < tr; ++a)
+    for (int b = 0; b < tc; ++b) {
+      const int t = a * tc + b;
+      auto& e = ev[t];
+      std::stable_sort(e.begin(), e.end(), [](const SrcEv& x, const SrcEv& y) { return x.i < y.i; });
+      tiles[t] = FemTile{1 + a * p->R, b * cols, (int)flat.size(), (int)(flat.size() + e.size())};
+      flat.insert(flat.end(), e.begin(), e.end());
+    }
+  FEM_HIP(hipMalloc(&p->d_tiles, sizeof(FemTile) * tiles.size()));
+  FEM_HIP(hipMemcpy(p->d_tiles, tiles.data(), sizeof(FemTile) * tiles.size(), hipMemcpyHostToDevice));
+  if (!flat.empty()) {
+    FEM_HIP(hipMalloc(&p->d_loads, sizeof(SrcEv) * flat.size()));
+    FEM_HIP(hipMemcpy(p->d_loads, flat.data(), sizeof(SrcEv) * flat.size(), hipMemcpyHostToDevice));
+  }
+  return ERNN_OK;
+}
+
+int check_call(const FemPlan* p, const ernn_fem_call_t* c) {
+  if (!c) return fail(ERNN_E_INVALID, "null call");
+  if (!c->workspace || !c->params || !c->wavelet || !c->frames_x || !c->frames_y)
+    return fail(ERNN_E_INVALID, "workspace, params, wavelet and both frame arrays are required");
+  if ((reinterpret_cast<uintptr_t>(c->workspace) & 255) != 0)
+    return fail(ERNN_E_INVALID, "workspace must be 256-byte aligned");
+  if (p->mode == ERNN_FEM_DAMAGE && !c->field) return fail(ERNN_E_INVALID, "damage mode needs the factor field");
+  return ERNN_OK;
+}
+
+FemCallDev to_dev(const ernn_fem_call_t* c) {
+  FemCallDev d;
+  d.ws = static_cast<float*>(c->workspace);
+  d.params = c->params;
+  d.field = c->field;
+  d.wav = c->wavelet;
+  d.frames_x = c->frames_x;
+  d.frames_y = c->frames_y;
+  return d;
+}
+
+FemStepArgs step_args(const FemPlan* p, int n) {
+  FemStepArgs a{};
+  a.call = p->d_call;
+  a.tiles = p->d_tiles;
+  a.loads = p->d_loads;
+  a.field = p->field;
+  a.rec_off = ((n + 1) % p->stride == 0) ? (int64_t)((n + 1) / p->stride) * p->frame : -1;
+  a.n_tiles = p->n_tiles;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.pitch = p->pitch;
+  a.step = n;
+  a.mode = p->mode;
+  a.dt2 = p->dt2;
+  a.rinv = p->rinv;
+  a.m_even = p->m_even;
+  a.m_odd = p->m_odd;
+  a.scale = p->scale;
+  return a;
+}
+
+// State, damage factor and flux matrix; frames zeroed (frame 0 = u^0 = 0 and
+// the boundary ring stays zero in every frame).
+int enqueue_setup(const FemPlan* p, hipStream_t st) {
+  FEM_HIP(launch_fem_zero(p->d_call, 0, (int64_t)kFemSlots * p->field, st));
+  if (p->mode == ERNN_FEM_DAMAGE) FEM_HIP(launch_fem_pack(p->d_call, p->field, p->nx, p->ny, p->pitch, st));
+  FEM_HIP(launch_fem_setup(p->d_call, p->field, p->dx, p->dy, st));
+  FEM_HIP(launch_fem_zero(p->d_call, 1, (int64_t)p->n_frames * p->frame, st));
+  FEM_HIP(launch_fem_zero(p->d_call, 2, (int64_t)p->n_frames * p->frame, st));
+  return ERNN_OK;
+}
+
+int enqueue_steps(const FemPlan* p, int n_end, hipStream_t st) {
+  for (int n = 1; n < n_end; ++n) FEM_HIP(launch_fem_step(step_args(p, n), p->V, p->R, p->blocks, st));
+  return ERNN_OK;
+}
+
+int enqueue_run(const FemPlan* p, hipStream_t st) {
+  FEM_TRY(enqueue_setup(p, st));
+  return enqueue_steps(p, p->nt - 1, st);
+}
+
+int capture(FemPlan* p) {
+  hipGraph_t g = nullptr;
+  FEM_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+  const int r = enqueue_run(p, p->cap_stream);
+  const hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+  if (r != ERNN_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return fail(ERNN_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(ERNN_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  return ERNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ernn_fem_plan_create(const ernn_fem_desc_t* d, void** plan_out) {
+  if (!d || !plan_out) return fail(ERNN_E_INVALID, "null argument");
+  *plan_out = nullptr;
+  if (d->nx < 3 || d->ny < 3) return fail(ERNN_E_INVALID, "nx and ny must be >= 3 (an interior node)");
+  if (d->nt < 2) return fail(ERNN_E_INVALID, "nt must be >= 2");
+  if (d->mode != ERNN_FEM_CONST && d->mode != ERNN_FEM_DAMAGE) return fail(ERNN_E_INVALID, "unknown mode");
+  if (!(d->dx > 0) || !(d->dy > 0) || !(d->dt > 0) || !(d->rho > 0) || !std::isfinite(d->dx) ||
+      !std::isfinite(d->dy) || !std::isfinite(d->dt) || !std::isfinite(d->rho))
+    return fail(ERNN_E_INVALID, "dx, dy, dt and rho must be positive and finite");
+  if (d->rec_stride < 1) return fail(ERNN_E_INVALID, "rec_stride must be >= 1");
+  if (!std::isfinite(d->rec_scale)) return fail(ERNN_E_INVALID, "rec_scale must be finite");
+  if (d->n_loads < 0 || (d->n_loads > 0 && !d->loads)) return fail(ERNN_E_INVALID, "bad load array");
+  int V = d->tile_v, R = d->tile_r;
+  if (V == 0 && R == 0) {
+    V = 1;
+    R = 2;
+  }
+  if (!fem_shape_ok(V, R))
+    return fail(ERNN_E_INVALID, "tile (V, R) must be one of (1,1) (1,2) (1,4) (1,8) (2,2) (2,4) (4,2) (4,4)");
+  const int64_t pitch = kLeftPad + round_up(d->ny, kColAlign);
+  const int64_t field = (int64_t)(d->nx + kFemExtraRows) * pitch;
+  if (field >= (int64_t(1) << 30)) return fail(ERNN_E_CAPACITY, "grid too large (a padded field must hold < 2^30 floats)");
+  FemPlan* p = new FemPlan();
+  p->nx = d->nx;
+  p->ny = d->ny;
+  p->nt = d->nt;
+  p->mode = d->mode;
+  p->stride = d->rec_stride;
+  p->dx = d->dx;
+  p->dy = d->dy;
+  p->dt = d->dt;
+  p->rho = d->rho;
+  p->scale = d->rec_scale;
+  p->V = V;
+  p->R = R;
+  p->pitch = (int)pitch;
+  p->field = field;
+  p->ws_floats = (int64_t)kFemSlots * field + 64;
+  p->frame = (int64_t)d->nx * d->ny;
+  p->n_frames = (d->nt + d->rec_stride - 1) / d->rec_stride;
+  p->use_graph = d->use_graph ? 1 : 0;
+  // the reference's scalars in its own rounding: dt*dt and 1/rho enter FP32
+  // arithmetic as FP32 (PF:482-493); the lumped mass is rho * area / 3 in
+  // FP32 (PF:335), added once per element touching the node (PF:333-337)
+  p->dt2 = (float)(d->dt * d->dt);
+  p->rinv = (float)(1.0 / d->rho);
+  const float m = (float)(d->rho * (d->dx * d->dy / 2.0) / 3.0);
+  float s = 0.f;
+  for (int k = 1; k <= 8; ++k) {
+    s += m;
+    if (k == 4) p->m_odd = s;
+  }
+  p->m_even = s;
+  for (int64_t k = 0; k < d->n_loads; ++k) {
+    const ernn_source_t& l = d->loads[k];
+    if (l.i < 0 || l.i >= d->nx || l.j < 0 || l.j >= d->ny || (l.comp != 0 && l.comp != 1) || !std::isfinite(l.w)) {
+      destroy(p);
+      return fail(ERNN_E_INVALID, "load " + std::to_string(k) + ": node outside the grid, bad component or weight");
+    }
+    if (l.i >= 1 && l.i <= d->nx - 2 && l.j >= 1 && l.j <= d->ny - 2) p->loads.push_back(l);
+  }
+  int r = ERNN_OK;
+  hipError_t e = hipGetDevice(&p->device);
+  if (e == hipSuccess) e = hipMalloc(&p->d_call, sizeof(FemCallDev));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) r = fail(ERNN_E_HIP, std::string("plan set-up: ") + hipGetErrorString(e));
+  if (r == ERNN_OK) r = build_tables(p);
+  if (r != ERNN_OK) {
+    destroy(p);
+    return r;
+  }
+  *plan_out = p;
+  return ERNN_OK;
+}
+
+int ernn_fem_plan_destroy(void* plan) {
+  if (!plan) return fail(ERNN_E_INVALID, "null plan");
+  FemPlan* p = static_cast<FemPlan*>(plan);
+  (void)hipSetDevice(p->device);
+  (void)hipDeviceSynchronize();
+  destroy(p);
+  return ERNN_OK;
+}
+
+int ernn_fem_plan_info(const void* plan, ernn_fem_info_t* info) {
+  if (!plan || !info) return fail(ERNN_E_INVALID, "null argument");
+  const FemPlan* p = static_cast<const FemPlan*>(plan);
+  *info = ernn_fem_info_t{};
+  info->workspace_bytes = p->ws_floats * (int64_t)sizeof(float);
+  info->frame_floats = (int64_t)p->n_frames * p->frame;
+  info->n_frames = p->n_frames;
+  info->tile_v = p->V;
+  info->tile_r = p->R;
+  info->n_tiles = p->n_tiles;
+  info->blocks = p->blocks;
+  info->n_loads = (int32_t)p->loads.size();
+  info->use_graph = p->use_graph;
+  return ERNN_OK;
+}
+
+int ernn_fem_run(void* plan, const ernn_fem_call_t* call, void* stream) {
+  if (!plan) return fail(ERNN_E_INVALID, "null plan");
+  FemPlan* p = static_cast<FemPlan*>(plan);
+  FEM_TRY(check_call(p, call));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  FEM_HIP(hipSetDevice(p->device));
+  FEM_HIP(launch_fem_set_call(p->d_call, to_dev(call), st));
+  if (!p->use_graph) return enqueue_run(p, st);
+  // the captured kernels reach every buffer through d_call: one capture serves every call
+  if (!p->graph) FEM_TRY(capture(p));
+  FEM_HIP(hipGraphLaunch(p->graph, st));
+  return ERNN_OK;
+}
+
+int ernn_fem_set_tiling(void* plan, int v, int r) {
+  if (!plan) return fail(ERNN_E_INVALID, "null plan");
+  FemPlan* p = static_cast<FemPlan*>(plan);
+  if (!fem_shape_ok(v, r))
+    return fail(ERNN_E_INVALID, "tile (V, R) must be one of (1,1) (1,2) (1,4) (1,8) (2,2) (2,4) (4,2) (4,4)");
+  FEM_HIP(hipSetDevice(p->device));
+  FEM_HIP(hipDeviceSynchronize());
+  if (p->graph) {
+    (void)hipGraphExecDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  p->V = v;
+  p->R = r;
+  return build_tables(p);
+}
+
+int ernn_fem_time_steps(void* plan, int steps, const ernn_fem_call_t* call, void* stream, float* ms_out) {
+  if (!plan || !ms_out || steps < 1) return fail(ERNN_E_INVALID, "bad argument");
+  FemPlan* p = static_cast<FemPlan*>(plan);
+  FEM_TRY(check_call(p, call));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  FEM_HIP(hipSetDevice(p->device));
+  FEM_HIP(launch_fem_set_call(p->d_call, to_dev(call), st));
+  FEM_TRY(enqueue_setup(p, st));
+  hipEvent_t e0, e1;
+  FEM_HIP(hipEventCreate(&e0));
+  FEM_HIP(hipEventCreate(&e1));
+  FEM_HIP(hipEventRecord(e0, st));
+  const int r = enqueue_steps(p, std::min(steps + 1, p->nt - 1), st);
+  FEM_HIP(hipEventRecord(e1, st));
+  FEM_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (r != ERNN_OK) return r;
+  *ms_out = ms;
+  return ERNN_OK;
+}
+
+}  // extern "C"

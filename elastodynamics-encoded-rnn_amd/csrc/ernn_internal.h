@@ -168,4 +168,50 @@ struct FusedBlockArgs {
   int pad;
 };
 
+// --------------------------------------------------------------------------
+// FEM generator (P1 triangles on the checkerboard mesh, lumped mass, central
+// differences).  Node (i, j), i along x, is at (i + 1) * pitch + kLeftPad + j
+// of a padded field of nx + kFemExtraRows rows (the layout above, with room
+// for a tile of up to kFemMaxRows rows past the last interior row).  The
+// boundary ring of nodes is held at zero, so it and the ghost cells are the
+// zero padding of the interior unknowns.
+// Workspace: ux slots 0, 1; uy slots 2, 3 (u^m in slot m & 1); the packed
+// damage factor field in slot 4; the 4 x 4 edge-flux matrix at 5 fields.
+constexpr int kFemMaxRows = 8;
+constexpr int kFemExtraRows = kFemMaxRows + 3;
+constexpr int kFemSlots = 5;
+
+enum FemMode : int { kFemConst = 0, kFemDamage = 1 };
+
+// One tile = R node rows x 64 V node columns, swept by one wavefront.
+struct FemTile {
+  int i0, j0;
+  int ev_begin, ev_end;   // nodal loads of the tile, sorted by row
+};
+
+struct FemCallDev {
+  float* ws;
+  const float* params;    // C11, C22, C12, C16, C26, C66 (damage mode: the intact C0)
+  const float* field;     // damage mode: per-node factor, nx * ny, row-major (x, y)
+  const float* wav;       // time factors, nt
+  float* frames_x;        // [n_frames][nx][ny]
+  float* frames_y;
+};
+
+struct FemStepArgs {
+  const FemCallDev* call;
+  const FemTile* tiles;
+  const SrcEv* loads;
+  int64_t field;          // floats per padded field
+  int64_t rec_off;        // float offset of the frame this step writes, or -1
+  int n_tiles;
+  int nx, ny, pitch;
+  int step;               // n: u^{n+1} from u^n (slot n & 1) and u^{n-1}, written over u^{n-1}
+  int mode;
+  float dt2;              // fl32(dt^2)
+  float rinv;             // fl32(1 / rho)
+  float m_even, m_odd;    // lumped mass of interior nodes with i + j even (8 elements) / odd (4)
+  float scale;            // frame scale
+};
+
 }  // namespace ernn

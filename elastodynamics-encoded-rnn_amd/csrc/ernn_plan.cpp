@@ -1024,6 +1024,12 @@ int run_pass(Plan* p, const ernn_call_t* c, hipStream_t st, bool adjoint) {
 
 }  // namespace
 
+// The other C-ABI source files (ernn_fem.cpp) report failures through the
+// same per-thread message.
+namespace ernn {
+int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace ernn
+
 extern "C" {
 
 int ernn_abi_version(void) { return ERNN_ABI_VERSION; }

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (must precede loading the HIP library)
 _DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_DIR, "libernn_hip.so")
 
-ERNN_ABI_VERSION = 2
+ERNN_ABI_VERSION = 3
 PARAM_CONST6, PARAM_DFIELD, PARAM_FIELD6 = 0, 1, 2
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "capacity exceeded"}
 
@@ -66,7 +66,36 @@ class Call(C.Structure):
                 ("grad_params", C.c_void_p), ("grad_field", C.c_void_p)]
 
 
+FEM_CONST, FEM_DAMAGE = 0, 1
+
+
+class FemDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nt", C.c_int32), ("mode", C.c_int32),
+                ("dx", C.c_double), ("dy", C.c_double), ("dt", C.c_double), ("rho", C.c_double),
+                ("rec_stride", C.c_int32), ("rec_scale", C.c_float), ("tile_v", C.c_int32),
+                ("tile_r", C.c_int32), ("use_graph", C.c_int32), ("loads", C.POINTER(Source)),
+                ("n_loads", C.c_int64)]
+
+
+class FemInfo(C.Structure):
+    _fields_ = [("workspace_bytes", C.c_int64), ("frame_floats", C.c_int64), ("n_frames", C.c_int32),
+                ("tile_v", C.c_int32), ("tile_r", C.c_int32), ("n_tiles", C.c_int32), ("blocks", C.c_int32),
+                ("n_loads", C.c_int32), ("use_graph", C.c_int32), ("pad", C.c_int32)]
+
+
+class FemCall(C.Structure):
+    _fields_ = [("workspace", C.c_void_p), ("params", C.c_void_p), ("field", C.c_void_p),
+                ("wavelet", C.c_void_p), ("frames_x", C.c_void_p), ("frames_y", C.c_void_p)]
+
+
 EXPORTS = {
+    "ernn_fem_plan_create": ([C.POINTER(FemDesc), C.POINTER(C.c_void_p)], C.c_int),
+    "ernn_fem_plan_destroy": ([C.c_void_p], C.c_int),
+    "ernn_fem_plan_info": ([C.c_void_p, C.POINTER(FemInfo)], C.c_int),
+    "ernn_fem_run": ([C.c_void_p, C.POINTER(FemCall), C.c_void_p], C.c_int),
+    "ernn_fem_set_tiling": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "ernn_fem_time_steps": ([C.c_void_p, C.c_int, C.POINTER(FemCall), C.c_void_p, C.POINTER(C.c_float)],
+                            C.c_int),
     "ernn_abi_version": ([], C.c_int),
     "ernn_last_error": ([], C.c_char_p),
     "ernn_plan_create": ([C.POINTER(Desc), C.POINTER(C.c_void_p)], C.c_int),

@@ -21,7 +21,7 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB_NAME = "libernn_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
-SOURCES = ["ernn_kernels.hip", "ernn_plan.cpp"]
+SOURCES = ["ernn_kernels.hip", "ernn_plan.cpp", "ernn_fem.hip", "ernn_fem.cpp"]
 
 # -ffp-contract=off: every fused multiply-add is an explicit fmaf in the
 # source, so each cell's arithmetic is the same in every kernel shape and a
@@ -73,7 +73,7 @@ def build(verbose: bool = False) -> str:
     build_dir = os.path.join(ROOT, "_build")
     os.makedirs(build_dir, exist_ok=True)
     for src in SOURCES:
-        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(build_dir, src.replace(".", "_") + ".o")   # ernn_fem.hip and ernn_fem.cpp differ
         cmd = [cc, *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-x", "hip",
                "-Rpass-analysis=kernel-resource-usage",
                "-c", os.path.join(CSRC, src), "-o", obj]

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define ERNN_ABI_VERSION 2
+#define ERNN_ABI_VERSION 3
 
 #define ERNN_OK 0
 #define ERNN_E_INVALID (-1)   /* bad argument or descriptor */
@@ -209,6 +209,82 @@ int ernn_plan_set_blocking(void* plan, int which, int v, int rw, int ks);
  * milliseconds. */
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
                     float* ms_out);
+
+/* ------------------------------------------------------------------------
+ * FEM observed-data generator: the reference's explicit P1-triangle solver
+ *   solve_2d_anisotropic_wave_fem_ricker_fixed
+ *     Parameter_inversion/FEM_generate_data.py:196-590 (constant C),
+ *     Damage_inversion/FEM_generate_data.py:770-1224 (damaged C)
+ * on its checkerboard mesh (PF:117-189): lumped mass (PF:327-339), element
+ * forces of PF:417-457 (per-element C at the node nearest the centroid,
+ * DF:1003-1026), central differences with the boundary held at zero
+ * (PF:481-497), frames scaled by rec_scale (0.7, PF:525-526).  ernn_fem_run
+ * replaces the whole time loop (PF:481-513, ~100 tensor ops per step) by one
+ * launch per step; nodes gather their forces, no atomics.
+ * Fields are nx x ny row-major with y contiguous, the reference's
+ * (nx, ny, t) output order (PF:517-523); frame k holds rec_scale * u^{k P}.
+ * ------------------------------------------------------------------------ */
+#define ERNN_FEM_CONST 0   /* one stiffness matrix (PF:402-406) */
+#define ERNN_FEM_DAMAGE 1  /* C = D(node nearest the element centroid) * C0 (DF:183-212, DF:1017-1022) */
+
+typedef struct {
+  int32_t nx, ny;        /* nodes per side, boundary included (PF:117-147) */
+  int32_t nt;            /* time samples; steps n = 1 .. nt-2 produce u^{n+1} (PF:481) */
+  int32_t mode;          /* ERNN_FEM_* */
+  double dx, dy;         /* Lx / (nx - 1), Ly / (ny - 1) (PF:140-141) */
+  double dt, rho;        /* time step after the CFL check (PF:280-296), density */
+  int32_t rec_stride;    /* P >= 1: frame k = u^{k P}, k < ceil(nt / P) (PF:926-930) */
+  float rec_scale;       /* frame factor (0.7 in PF:525-526) */
+  int32_t tile_v, tile_r; /* tile: V in {1,2,4} columns per lane, R rows, (V, R) in
+                            (1,1) (1,2) (1,4) (1,8) (2,2) (2,4) (4,2) (4,4); 0, 0 = (1, 2) */
+  int32_t use_graph;     /* 1: replay the run as a captured hipGraph */
+  const ernn_source_t* loads; /* host array: nodal loads w at (i, j) on component comp, the
+                            Gaussian of PF:298-301 at the nodes (PF:322); f_ext = w(t_n) w / rho * M
+                            (PF:482-484).  Loads on boundary nodes have no effect and are dropped */
+  int64_t n_loads;
+} ernn_fem_desc_t;
+
+typedef struct {
+  int64_t workspace_bytes; /* caller-provided workspace, 256-byte aligned */
+  int64_t frame_floats;    /* floats of each frame array, n_frames * nx * ny */
+  int32_t n_frames;
+  int32_t tile_v, tile_r;
+  int32_t n_tiles, blocks;
+  int32_t n_loads;         /* loads kept (interior nodes) */
+  int32_t use_graph;
+  int32_t pad;
+} ernn_fem_info_t;
+
+/* Per-run device pointers. */
+typedef struct {
+  void* workspace;
+  const float* params;   /* [6] C11, C22, C12, C16, C26, C66 (DAMAGE: the intact C0) */
+  const float* field;    /* DAMAGE: per-node factor D, nx * ny (1 intact, 0.5 damaged in DF) */
+  const float* wavelet;  /* [nt] time factors w(t_n) (PF:459-463) */
+  float* frames_x;       /* [n_frames][nx][ny] output, ux */
+  float* frames_y;       /* [n_frames][nx][ny] output, uy */
+} ernn_fem_call_t;
+
+/* Validate a descriptor and build the tile and load tables on the current
+ * device (replaces the set-up of PF:300-463). */
+int ernn_fem_plan_create(const ernn_fem_desc_t* desc, void** plan_out);
+
+int ernn_fem_plan_destroy(void* plan);
+
+int ernn_fem_plan_info(const void* plan, ernn_fem_info_t* info);
+
+/* The whole run (PF:479-526): zero state and frames, then steps n = 1 ..
+ * nt-2, writing frame k = rec_scale * u^{kP} (frame 0 = 0).  Enqueued on
+ * `stream`; no host synchronization. */
+int ernn_fem_run(void* plan, const ernn_fem_call_t* call, void* stream);
+
+/* Change the tile (V, R); synchronizes the device and drops the captured
+ * graph.  Used by the autotuner. */
+int ernn_fem_set_tiling(void* plan, int v, int r);
+
+/* Time steps n = 1 .. min(steps, nt - 2) after the run's set-up, with
+ * hipEvents on `stream`; synchronizes.  ms_out receives the milliseconds. */
+int ernn_fem_time_steps(void* plan, int steps, const ernn_fem_call_t* call, void* stream, float* ms_out);
 
 #ifdef __cplusplus
 }
