@@ -386,7 +386,11 @@ def fem_main(args):
     eng = fem.FEMEngine(nx, nx, L / 300, L / 300, nt, dt, FEM_RHO, fem.nodal_loads(g32), damage=D is not None,
                         stride=stride, use_graph=bool(args.graph), device=dev)
     C32 = np.asarray(FEM_C, np.float32)
-    tune = eng.tune(C32, w32, Dt, steps=min(200, nt - 2)) if args.autotune else None
+    tune = None
+    if args.tiles:   # fixed tile "V,R"
+        eng.set_tiling(*(int(s) for s in args.tiles.split(",")))
+    elif args.autotune:
+        tune = eng.tune(C32, w32, Dt, steps=min(200, nt - 2))
     if tune:
         log("fem tiles: " + ", ".join(f"{k}: {v / min(200, nt - 2) * 1e3:.2f} us" for k, v in sorted(tune.items())))
     out = (torch.empty((eng.n_frames, nx, nx), dtype=torch.float32, device=dev),

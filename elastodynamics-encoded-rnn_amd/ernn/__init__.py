@@ -14,7 +14,7 @@ from .simulator import (ERNN, CFRPWaveSimulatorRicker, SpatiallyVaryingAnisotrop
                         total_variation_2d)
 from .inversion import (CFRPParameterInverterRicker, SpatiallyVaryingParameterInverter, TraceInversion,
                         add_noise_rms, add_noise_snr)
-from . import autotune, configs, data
+from . import autotune, configs, data, fem
 
 __all__ = [
     "PARAM_NAMES", "XZ_NAMES", "Shot", "Survey", "bond_rotate", "cfl_dt", "ellipse_damage",
@@ -22,5 +22,5 @@ __all__ = [
     "Engine", "ERNNFunction", "ernn_traces", "ERNN", "CFRPWaveSimulatorRicker",
     "SpatiallyVaryingAnisotropicSimulator", "total_variation_2d", "autotune", "configs",
     "CFRPParameterInverterRicker", "SpatiallyVaryingParameterInverter", "TraceInversion",
-    "add_noise_rms", "add_noise_snr", "data",
+    "add_noise_rms", "add_noise_snr", "data", "fem",
 ]
