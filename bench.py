@@ -461,6 +461,111 @@ def fem_main(args):
     print(json.dumps(line), flush=True)
 
 
+def tv_adam_main(args):
+    """The damage inversion's regularised update (DI:179-195, DI:304-322,
+    DI:453-460, DI:478-483) on six 2048^2 log-stiffness fields: the fused HIP
+    step (FusedTVAdam) against autograd through clamp(exp) + TV and
+    torch.optim.Adam on the same GPU (the data gradient is given), and the FP64
+    oracle on one CPU core."""
+    import numpy as np
+    import torch
+    import ernn
+    from ernn import FusedTVAdam
+    dev = torch.device("cuda:0")
+    n = 2048
+    bounds = [(50e9, 250e9), (5e9, 50e9), (5e9, 50e9), (0.0, 60e9), (0.0, 20e9), (5e9, 30e9)]
+    C0 = [130.74e9, 11.5e9, 10.49e9, 30.67e9, 3.75e9, 14.77e9]
+    g = torch.Generator(device=dev).manual_seed(0)
+    th0 = [(torch.full((1, n, n), math.log(c), device=dev) + 0.05 * torch.randn((1, n, n), generator=g, device=dev))
+           .contiguous() for c in C0]
+    gd = [1e-3 * torch.randn((1, n, n), generator=g, device=dev) for _ in C0]
+    lam, lr = 1e-12, 1e-3
+    cells = 6 * n * n
+
+    fused_p = [t.clone() for t in th0]
+    opt = FusedTVAdam(fused_p, lr=lr, lambda_tv=lam, bounds=bounds)
+
+    def fused_step():
+        for p, d in zip(fused_p, gd):
+            p.grad = d.clone()
+        opt.step()
+
+    ref_p = [t.clone().requires_grad_() for t in th0]
+    ropt = torch.optim.Adam(ref_p, lr=lr)
+
+    def torch_step():
+        ropt.zero_grad()
+        for p, d in zip(ref_p, gd):
+            p.grad = d.clone()
+        C = [torch.clamp(torch.exp(p), *b) for p, b in zip(ref_p, bounds)]
+        (lam * sum(ernn.total_variation_2d(c) for c in C)).backward()
+        ropt.step()
+
+    def timed(fn, k):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k
+
+    k = max(args.steps, 5)
+    s_fused = timed(fused_step, k)
+    s_torch = timed(torch_step, k)
+    # kernel time alone (hipEvents on the stream), without the gradient copy of fused_step
+    for p, d in zip(fused_p, gd):
+        p.grad = d.clone()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(k):
+        opt.step()
+    e1.record()
+    torch.cuda.synchronize()
+    step_ms = e0.elapsed_time(e1) / k
+    # bytes per cell: TV pass reads theta (+ neighbours, cached) and grad, writes grad (12);
+    # Adam pass reads theta, grad, m, v and writes theta, m, v (28)
+    bpc = 40
+    achieved = cells * bpc / (step_ms * 1e-3) / 1e9
+    traffic = valu = None
+    tpath = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        entry = json.load(open(tpath)).get("tv-adam-2048/tv_grad_kernel+adam_kernel")
+        if entry:
+            traffic = entry["bytes"]
+    roofline = {"bound": "hbm", "kernel": "tv_grad_kernel+adam_kernel", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "valu": valu, "bytes_per_cell": bpc, "cells_per_launch": cells, "avg_launch_us": round(step_ms * 1e3, 2)}
+    log(f"tv+adam: fused {s_fused * 1e3:.3f} ms per update ({step_ms * 1e3:.1f} us kernels), "
+        f"autograd TV + torch Adam {s_torch * 1e3:.3f} ms")
+    cpu = None
+    if args.cpu:
+        from oracle import tv_adam_fp64 as T
+        th = th0[0][0].double().cpu().numpy()
+        gg = gd[0][0].double().cpu().numpy()
+        t0 = time.perf_counter()
+        Cf, chain = T.fields(th, *bounds[0])
+        gt = gg + lam * T.tv_subgradient(Cf) * chain
+        T.tv(Cf)
+        T.adam_step(th, gt, 0 * th, 0 * th, 1, lr, 0.9, 0.999, 1e-8)
+        s = time.perf_counter() - t0
+        cpu = {"value": round(n * n / s / 1e9, 5), "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+               "sample": "FP64 oracle: one 2048^2 field, TV subgradient + chain + one Adam step"}
+    line = {
+        "metric": "TV + Adam update Gcell-updates/s (1 GPU)", "value": round(cells / s_fused / 1e9, 3),
+        "unit": "Gcell-updates/s", "n_gpus": 1, "steps": k, "warmup": args.warmup,
+        "ms_per_step": round(s_fused * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "tv-adam-2048", "fields": 6, "grid": f"{n}x{n}", "parallelism": "none"},
+        "reference_path_on_gpu": {"ms_per_step": round(s_torch * 1e3, 4), "what": "autograd TV + torch.optim.Adam"},
+        "speedup_vs_reference_path": round(s_torch / s_fused, 2),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -469,6 +574,8 @@ def main():
         sys.exit(2)
     if args.workload in FEM_WORKLOADS:
         return fem_main(args)
+    if args.workload == "tv-adam-2048":
+        return tv_adam_main(args)
     import numpy as np
     import torch
     import ernn

@@ -1,0 +1,250 @@
+// Total-variation regularisation + Adam update of per-cell parameter fields
+// on gfx950, and its C ABI (ernn_tv_adam_* in include/ernn.h).
+//
+// Reference (paths relative to the reference repository root):
+//   DI = Damage_inversion/Inversion_adam.py
+//   fields            C_p = clamp(exp(theta_p), lo_p, hi_p)   DI:304-322 (log parameters DI:245-251)
+//   TV                sum |C[i+1,j] - C[i,j]| + sum |C[i,j+1] - C[i,j]|   DI:179-195
+//   loss              data + lambda * sum_p TV(C_p)   DI:441-463, gradient by autograd DI:511
+//   update            torch.optim.Adam over the six theta fields   DI:478-483, DI:514
+//
+// The reference builds TV and its gradient from ~20 tensor ops per field and
+// steps Adam with ~10 more.  Here one call runs two passes over the fields:
+//   tv_grad_kernel  g_theta += (lambda s(i,j)) m(i,j) e(i,j), the TV
+//                   subgradient s (sign(0) = 0 as abs' backward) chained
+//                   through the clamp mask m (inclusive bounds) and
+//                   d exp = e; per-block FP64 TV partial sums
+//   adam_kernel     m, v, theta in place with torch's arithmetic order
+//                   (lerp, mul + addcmul, sqrt / bias correction + eps,
+//                   addcdiv)
+// Two passes because a cell's TV gradient reads its neighbours' theta,
+// which an in-place update in the same pass would race with.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "ernn.h"
+#include "ernn_internal.h"
+
+namespace ernn {
+
+int set_last_error(int code, const std::string& msg);
+
+constexpr int kTvThreads = 256;
+
+struct TvArgs {
+  const float* theta[ERNN_TV_MAX_FIELDS];
+  float* grad[ERNN_TV_MAX_FIELDS];
+  float lo[ERNN_TV_MAX_FIELDS], hi[ERNN_TV_MAX_FIELDS];
+  double* partial;       // [n_fields][blocks_per_field] TV partial sums
+  int nx, ny, log_params, blocks_per_field;
+  float lambda;
+};
+
+struct AdamArgs {
+  float* theta[ERNN_TV_MAX_FIELDS];
+  const float* grad[ERNN_TV_MAX_FIELDS];
+  float* m[ERNN_TV_MAX_FIELDS];
+  float* v[ERNN_TV_MAX_FIELDS];
+  int64_t n;             // cells per field
+  float beta1w;          // lerp weight 1 - beta1
+  float beta2;
+  float beta2w;          // addcmul value 1 - beta2
+  float step_size;       // -lr / (1 - beta1^t)
+  float bc2_sqrt;        // sqrt(1 - beta2^t)
+  float eps;
+  float weight_decay;
+};
+
+// C = clamp(exp(theta)) (or clamp(theta)); the clamp's backward passes the
+// gradient where lo <= x <= hi, and exp's multiplies it by its result.
+__device__ __forceinline__ float field_value(float th, int log_params, float lo, float hi, float* chain) {
+  const float x = log_params ? expf(th) : th;
+  const bool in = x >= lo && x <= hi;
+  *chain = in ? (log_params ? x : 1.0f) : 0.0f;
+  return fminf(fmaxf(x, lo), hi);
+}
+
+__device__ __forceinline__ float sgn(float d) { return d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f); }
+
+__global__ void __launch_bounds__(kTvThreads) tv_grad_kernel(TvArgs a) {
+  const int f = blockIdx.y;
+  const float* __restrict__ th = a.theta[f];
+  float* __restrict__ g = a.grad[f];
+  const float lo = a.lo[f], hi = a.hi[f];
+  const int64_t n = (int64_t)a.nx * a.ny;
+  const int64_t k = (int64_t)blockIdx.x * kTvThreads + threadIdx.x;
+  double tv = 0.0;
+  if (k < n) {
+    const int i = (int)(k / a.ny), j = (int)(k % a.ny);
+    float chain, dummy;
+    const float c = field_value(th[k], a.log_params, lo, hi, &chain);
+    float s = 0.f;
+    if (i > 0) s += sgn(c - field_value(th[k - a.ny], a.log_params, lo, hi, &dummy));
+    if (i + 1 < a.nx) {
+      const float d = field_value(th[k + a.ny], a.log_params, lo, hi, &dummy) - c;
+      s -= sgn(d);
+      tv += fabs((double)d);
+    }
+    if (j > 0) s += sgn(c - field_value(th[k - 1], a.log_params, lo, hi, &dummy));
+    if (j + 1 < a.ny) {
+      const float d = field_value(th[k + 1], a.log_params, lo, hi, &dummy) - c;
+      s -= sgn(d);
+      tv += fabs((double)d);
+    }
+    g[k] = g[k] + (a.lambda * s) * chain;
+  }
+  // block sum of the TV terms, fixed order: wavefront butterflies, then the four wave sums
+  for (int o = 32; o > 0; o >>= 1) tv += __shfl_xor(tv, o, kWave);
+  __shared__ double ws[kTvThreads / kWave];
+  if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = tv;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kTvThreads / kWave; ++w) s += ws[w];
+    a.partial[(int64_t)f * a.blocks_per_field + blockIdx.x] = s;
+  }
+}
+
+// One block per field: TV = sum of the partials in block order.
+__global__ void __launch_bounds__(kTvThreads) tv_finalize_kernel(const double* partial, int blocks_per_field,
+                                                                  double* out) {
+  const int f = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < blocks_per_field; b += kTvThreads) s += partial[(int64_t)f * blocks_per_field + b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+  __shared__ double ws[kTvThreads / kWave];
+  if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kTvThreads / kWave; ++w) t += ws[w];
+    out[f] = t;
+  }
+}
+
+// torch.optim.Adam (foreach form, torch/optim/adam.py _multi_tensor_adam) for one element:
+//   g = g + wd theta                       (_foreach_add, alpha = weight_decay)
+//   m = lerp(m, g, w), w = 1 - b1          (w < 0.5: m + w (g - m); else g - (g - m)(1 - w))
+//   v = v b2 + (1 - b2) (g g)              (_foreach_mul_, _foreach_addcmul_)
+//   theta = theta + step_size (m / (sqrt(v) / sqrt(1 - b2^t) + eps))   (_foreach_addcdiv_)
+__global__ void __launch_bounds__(kTvThreads) adam_kernel(AdamArgs a) {
+  const int f = blockIdx.y;
+  float* __restrict__ th = a.theta[f];
+  const float* __restrict__ gr = a.grad[f];
+  float* __restrict__ m = a.m[f];
+  float* __restrict__ v = a.v[f];
+  for (int64_t k = (int64_t)blockIdx.x * kTvThreads + threadIdx.x; k < a.n; k += (int64_t)gridDim.x * kTvThreads) {
+    const float p = th[k];
+    float g = gr[k];
+    if (a.weight_decay != 0.f) g = g + a.weight_decay * p;
+    const float mk = a.beta1w < 0.5f ? m[k] + a.beta1w * (g - m[k]) : g - (g - m[k]) * (1.0f - a.beta1w);
+    const float vk = v[k] * a.beta2 + a.beta2w * (g * g);
+    const float denom = sqrtf(vk) / a.bc2_sqrt + a.eps;
+    m[k] = mk;
+    v[k] = vk;
+    th[k] = p + a.step_size * (mk / denom);
+  }
+}
+
+}  // namespace ernn
+
+using namespace ernn;
+
+namespace {
+
+int64_t tv_blocks(int nx, int ny) { return ((int64_t)nx * ny + kTvThreads - 1) / kTvThreads; }
+
+#define TV_HIP(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess)                                                                      \
+      return set_last_error(ERNN_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+int check_desc(const ernn_tv_adam_desc_t* d) {
+  if (!d) return set_last_error(ERNN_E_INVALID, "null descriptor");
+  if (d->nx < 1 || d->ny < 1) return set_last_error(ERNN_E_INVALID, "nx and ny must be >= 1");
+  if (d->n_fields < 1 || d->n_fields > ERNN_TV_MAX_FIELDS)
+    return set_last_error(ERNN_E_INVALID, "n_fields must be in 1 .. " + std::to_string(ERNN_TV_MAX_FIELDS));
+  if ((int64_t)d->nx * d->ny >= (int64_t(1) << 31)) return set_last_error(ERNN_E_CAPACITY, "field too large");
+  if (d->step < 1) return set_last_error(ERNN_E_INVALID, "step (the Adam step count) must be >= 1");
+  if (!(d->lr >= 0.0) || !(d->eps > 0.0) || !(d->beta1 >= 0.0 && d->beta1 < 1.0) ||
+      !(d->beta2 >= 0.0 && d->beta2 < 1.0) || !std::isfinite(d->lambda_tv) || !std::isfinite(d->weight_decay))
+    return set_last_error(ERNN_E_INVALID, "bad Adam hyper-parameters (lr >= 0, eps > 0, 0 <= beta < 1)");
+  for (int f = 0; f < d->n_fields; ++f)
+    if (!(d->lo[f] <= d->hi[f])) return set_last_error(ERNN_E_INVALID, "bounds of field " + std::to_string(f));
+  return ERNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ernn_tv_adam_workspace_bytes(const ernn_tv_adam_desc_t* desc, int64_t* bytes) {
+  if (!bytes) return set_last_error(ERNN_E_INVALID, "null argument");
+  const int r = check_desc(desc);
+  if (r != ERNN_OK) return r;
+  *bytes = tv_blocks(desc->nx, desc->ny) * desc->n_fields * (int64_t)sizeof(double);
+  return ERNN_OK;
+}
+
+int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c, void* stream) {
+  const int r = check_desc(d);
+  if (r != ERNN_OK) return r;
+  if (!c || !c->workspace) return set_last_error(ERNN_E_INVALID, "null call or workspace");
+  if ((reinterpret_cast<uintptr_t>(c->workspace) & 7) != 0)
+    return set_last_error(ERNN_E_INVALID, "workspace must be 8-byte aligned");
+  for (int f = 0; f < d->n_fields; ++f)
+    if (!c->theta[f] || !c->grad[f] || !c->m[f] || !c->v[f])
+      return set_last_error(ERNN_E_INVALID, "theta, grad, m and v of field " + std::to_string(f) + " are required");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t nb = tv_blocks(d->nx, d->ny);
+  const int64_t n = (int64_t)d->nx * d->ny;
+  TvArgs t{};
+  for (int f = 0; f < d->n_fields; ++f) {
+    t.theta[f] = c->theta[f];
+    t.grad[f] = c->grad[f];
+    t.lo[f] = d->lo[f];
+    t.hi[f] = d->hi[f];
+  }
+  t.partial = static_cast<double*>(c->workspace);
+  t.nx = d->nx;
+  t.ny = d->ny;
+  t.log_params = d->log_params ? 1 : 0;
+  t.blocks_per_field = (int)nb;
+  t.lambda = (float)d->lambda_tv;   // the mul's backward scales by the FP32 scalar
+  hipLaunchKernelGGL(tv_grad_kernel, dim3((unsigned)nb, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, t);
+  TV_HIP(hipGetLastError());
+  if (c->tv) {
+    hipLaunchKernelGGL(tv_finalize_kernel, dim3((unsigned)d->n_fields), dim3(kTvThreads), 0, st,
+                       static_cast<const double*>(c->workspace), (int)nb, c->tv);
+    TV_HIP(hipGetLastError());
+  }
+  // Adam's scalars as torch forms them: Python floats (double), cast to FP32 where a kernel uses them
+  const double bc1 = 1.0 - std::pow(d->beta1, (double)d->step);
+  const double bc2 = 1.0 - std::pow(d->beta2, (double)d->step);
+  AdamArgs a{};
+  for (int f = 0; f < d->n_fields; ++f) {
+    a.theta[f] = c->theta[f];
+    a.grad[f] = c->grad[f];
+    a.m[f] = c->m[f];
+    a.v[f] = c->v[f];
+  }
+  a.n = n;
+  a.beta1w = (float)(1.0 - d->beta1);
+  a.beta2 = (float)d->beta2;
+  a.beta2w = (float)(1.0 - d->beta2);
+  a.step_size = (float)((d->lr / bc1) * -1.0);
+  a.bc2_sqrt = (float)std::pow(bc2, 0.5);
+  a.eps = (float)d->eps;
+  a.weight_decay = (float)d->weight_decay;
+  const int64_t blocks = std::min<int64_t>((n + kTvThreads - 1) / kTvThreads, 16384);
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)blocks, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, a);
+  TV_HIP(hipGetLastError());
+  return ERNN_OK;
+}
+
+}  // extern "C"

@@ -14,6 +14,7 @@ from .simulator import (ERNN, CFRPWaveSimulatorRicker, SpatiallyVaryingAnisotrop
                         total_variation_2d)
 from .inversion import (CFRPParameterInverterRicker, SpatiallyVaryingParameterInverter, TraceInversion,
                         add_noise_rms, add_noise_snr)
+from .optim import FusedTVAdam
 from . import autotune, configs, data, fem
 
 __all__ = [
@@ -22,5 +23,5 @@ __all__ = [
     "Engine", "ERNNFunction", "ernn_traces", "ERNN", "CFRPWaveSimulatorRicker",
     "SpatiallyVaryingAnisotropicSimulator", "total_variation_2d", "autotune", "configs",
     "CFRPParameterInverterRicker", "SpatiallyVaryingParameterInverter", "TraceInversion",
-    "add_noise_rms", "add_noise_snr", "data", "fem",
+    "add_noise_rms", "add_noise_snr", "data", "fem", "FusedTVAdam",
 ]

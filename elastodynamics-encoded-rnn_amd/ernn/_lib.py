@@ -88,7 +88,25 @@ class FemCall(C.Structure):
                 ("wavelet", C.c_void_p), ("frames_x", C.c_void_p), ("frames_y", C.c_void_p)]
 
 
+TV_MAX_FIELDS = 8
+
+
+class TvAdamDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("n_fields", C.c_int32), ("log_params", C.c_int32),
+                ("step", C.c_int32), ("pad", C.c_int32), ("lambda_tv", C.c_double), ("lr", C.c_double),
+                ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double), ("weight_decay", C.c_double),
+                ("lo", C.c_float * TV_MAX_FIELDS), ("hi", C.c_float * TV_MAX_FIELDS)]
+
+
+class TvAdamCall(C.Structure):
+    _fields_ = [("theta", C.c_void_p * TV_MAX_FIELDS), ("grad", C.c_void_p * TV_MAX_FIELDS),
+                ("m", C.c_void_p * TV_MAX_FIELDS), ("v", C.c_void_p * TV_MAX_FIELDS),
+                ("tv", C.c_void_p), ("workspace", C.c_void_p)]
+
+
 EXPORTS = {
+    "ernn_tv_adam_workspace_bytes": ([C.POINTER(TvAdamDesc), C.POINTER(C.c_int64)], C.c_int),
+    "ernn_tv_adam_step": ([C.POINTER(TvAdamDesc), C.POINTER(TvAdamCall), C.c_void_p], C.c_int),
     "ernn_fem_plan_create": ([C.POINTER(FemDesc), C.POINTER(C.c_void_p)], C.c_int),
     "ernn_fem_plan_destroy": ([C.c_void_p], C.c_int),
     "ernn_fem_plan_info": ([C.c_void_p, C.POINTER(FemInfo)], C.c_int),

@@ -21,7 +21,7 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB_NAME = "libernn_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
-SOURCES = ["ernn_kernels.hip", "ernn_plan.cpp", "ernn_fem.hip", "ernn_fem.cpp"]
+SOURCES = ["ernn_kernels.hip", "ernn_plan.cpp", "ernn_fem.hip", "ernn_fem.cpp", "ernn_tv_adam.hip"]
 
 # -ffp-contract=off: every fused multiply-add is an explicit fmaf in the
 # source, so each cell's arithmetic is the same in every kernel shape and a

@@ -20,6 +20,7 @@ import numpy as np
 import torch
 import torch.optim as optim
 
+from .optim import FusedTVAdam
 from .simulator import ERNN, total_variation_2d
 
 
@@ -128,31 +129,56 @@ class SpatiallyVaryingParameterInverter:
                         'C12_mean': [], 'C16_mean': [], 'C26_mean': [], 'C66_mean': []}
         self.verbose = verbose
 
-    def compute_loss(self, pred_ux_primary, pred_uy_primary):
-        """DI:441-463: 5 MSE_x + MSE_y + lambda sum_p TV(C_p)."""
+    def data_loss(self, pred_ux_primary, pred_uy_primary):
+        """DI:445-451: 5 MSE_x + MSE_y on the secondary time samples."""
         s = self.secondary_time_stride
         px = pred_ux_primary[:, ::s, :, :]
         py = pred_uy_primary[:, ::s, :, :]
-        loss_data = 5 * torch.mean((px - self.target_ux_secondary) ** 2) + \
+        return 5 * torch.mean((px - self.target_ux_secondary) ** 2) + \
             torch.mean((py - self.target_uy_secondary) ** 2)
+
+    def compute_loss(self, pred_ux_primary, pred_uy_primary):
+        """DI:441-463: 5 MSE_x + MSE_y + lambda sum_p TV(C_p)."""
+        loss_data = self.data_loss(pred_ux_primary, pred_uy_primary)
         loss_tv = 0.0
         if self.lambda_tv > 0:
             C = self.simulator.get_material_params()[:6]
             loss_tv = self.lambda_tv * sum(total_variation_2d(c) for c in C)
         return loss_data + loss_tv, loss_data, loss_tv
 
+    def fused_optimizer(self, lr=1e-3, betas=(0.9, 0.999), weight_decay=0.0):
+        """FusedTVAdam over the simulator's six fields with its bounds and
+        parameterisation (DI:245-270, DI:304-322) and this inverter's lambda_tv."""
+        sim = self.simulator
+        names = ('C11', 'C22', 'C12', 'C16', 'C26', 'C66')
+        params = [getattr(sim, ('log_' if sim.use_log_params else '') + n) for n in names]
+        return FusedTVAdam(params, lr=lr, betas=betas, weight_decay=weight_decay, lambda_tv=self.lambda_tv,
+                           bounds=[getattr(sim, n + '_bounds') for n in names], log_params=sim.use_log_params)
+
     def invert(self, max_epochs=1000, lr=1e-3, betas=(0.9, 0.999), weight_decay=0.0,
                checkpoint_interval=50, output_dir="./checkpoints_fem_damage_adam_fixed_bc",
-               patience=100, min_epochs=50):
-        """DI:465-560."""
-        optimizer = optim.Adam(self.simulator.parameters(), lr=lr, betas=betas, weight_decay=weight_decay)
+               patience=100, min_epochs=50, fused_update=False):
+        """DI:465-560.  fused_update: the TV term's gradient and the Adam step
+        run in one HIP call (FusedTVAdam) after the data loss's backward,
+        instead of autograd through the TV graph and torch.optim.Adam."""
+        if fused_update:
+            optimizer = self.fused_optimizer(lr, betas, weight_decay)
+        else:
+            optimizer = optim.Adam(self.simulator.parameters(), lr=lr, betas=betas, weight_decay=weight_decay)
         best, counter = float('inf'), 0
         for epoch in range(max_epochs):
             optimizer.zero_grad()
             ux, uy = self.simulator(save_all_steps=False)
-            total, data, tv = self.compute_loss(ux, uy)
-            total.backward()
-            optimizer.step()
+            if fused_update:
+                data = self.data_loss(ux, uy)
+                data.backward()
+                optimizer.step()
+                tv = self.lambda_tv * float(optimizer.last_tv.sum()) if self.lambda_tv > 0 else 0.0
+                total = data.detach() + tv
+            else:
+                total, data, tv = self.compute_loss(ux, uy)
+                total.backward()
+                optimizer.step()
             cur = total.item()
             self.history['loss'].append(cur)
             self.history['loss_data'].append(data.item())

@@ -286,6 +286,49 @@ int ernn_fem_set_tiling(void* plan, int v, int r);
  * hipEvents on `stream`; synchronizes.  ms_out receives the milliseconds. */
 int ernn_fem_time_steps(void* plan, int steps, const ernn_fem_call_t* call, void* stream, float* ms_out);
 
+/* ------------------------------------------------------------------------
+ * Total-variation regularisation + Adam update of per-cell parameter fields
+ * (Damage_inversion/Inversion_adam.py): theta_p are the trainable fields,
+ * C_p = clamp(exp(theta_p), lo_p, hi_p) (DI:304-313; log_params = 0:
+ * clamp(theta_p), DI:315-320), the loss adds lambda * sum_p TV(C_p) with
+ * TV = sum |C[i+1,j] - C[i,j]| + sum |C[i,j+1] - C[i,j]| (DI:179-195,
+ * DI:453-460), and torch.optim.Adam steps the thetas (DI:478-483, DI:514).
+ * ernn_tv_adam_step replaces the TV forward, its autograd backward and the
+ * optimizer step (~30 tensor ops per field) by two passes over the fields.
+ * ------------------------------------------------------------------------ */
+#define ERNN_TV_MAX_FIELDS 8
+
+typedef struct {
+  int32_t nx, ny;        /* field shape, row-major, y contiguous */
+  int32_t n_fields;      /* 1 .. ERNN_TV_MAX_FIELDS */
+  int32_t log_params;    /* 1: C = clamp(exp(theta)); 0: C = clamp(theta) */
+  int32_t step;          /* Adam step count t >= 1, this update included */
+  int32_t pad;
+  double lambda_tv;      /* TV weight (DI:404) */
+  double lr, beta1, beta2, eps, weight_decay;   /* torch.optim.Adam arguments (DI:478-483) */
+  float lo[ERNN_TV_MAX_FIELDS], hi[ERNN_TV_MAX_FIELDS]; /* clamp bounds per field (DI:265-270) */
+} ernn_tv_adam_desc_t;
+
+/* Device pointers, one per field (fields need not be contiguous). */
+typedef struct {
+  float* theta[ERNN_TV_MAX_FIELDS]; /* updated in place */
+  float* grad[ERNN_TV_MAX_FIELDS];  /* in: dJ_data/dtheta; out: the total gradient
+                                       dJ_data/dtheta + lambda dTV/dtheta (what loss.backward()
+                                       leaves in theta.grad for the whole loss) */
+  float* m[ERNN_TV_MAX_FIELDS];     /* Adam first moments (zero before step 1), updated in place */
+  float* v[ERNN_TV_MAX_FIELDS];     /* Adam second moments, updated in place */
+  double* tv;                       /* optional [n_fields] output: TV(C_p) of the fields before the
+                                       update (the loss term without lambda); NULL = skip */
+  void* workspace;                  /* ernn_tv_adam_workspace_bytes, 8-byte aligned */
+} ernn_tv_adam_call_t;
+
+/* Workspace bytes of ernn_tv_adam_step for this descriptor (validates it). */
+int ernn_tv_adam_workspace_bytes(const ernn_tv_adam_desc_t* desc, int64_t* bytes);
+
+/* One regularised Adam step: grad += lambda dTV/dtheta, then Adam on theta,
+ * m, v (torch's foreach arithmetic order).  Enqueued on `stream`. */
+int ernn_tv_adam_step(const ernn_tv_adam_desc_t* desc, const ernn_tv_adam_call_t* call, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -111,3 +111,43 @@ def test_nodal_loads_keep_fp32_nonzeros():
         assert c == 1 and w == float(g32[i, j]) and w > 0
     # all of the load's mass is in the list: sum of kept weights = sum of the FP32 array
     assert abs(sum(w for _, _, w, _ in loads) - float(g32.astype(np.float64).sum())) < 1e-6 * float(g32.sum())
+
+
+# ------------------------------------------------------------------ TV + Adam descriptor validation
+def _tv_desc(**kw):
+    d = _lib.TvAdamDesc()
+    d.nx, d.ny, d.n_fields, d.log_params, d.step = 16, 12, 6, 1, 1
+    d.lambda_tv, d.lr, d.beta1, d.beta2, d.eps, d.weight_decay = 1e-12, 1e-3, 0.9, 0.999, 1e-8, 0.0
+    for f in range(8):
+        d.lo[f], d.hi[f] = 0.0, 1e12
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("kw,msg", [
+    ({"nx": 0}, "nx and ny"),
+    ({"n_fields": 0}, "n_fields"),
+    ({"n_fields": 9}, "n_fields"),
+    ({"step": 0}, "step"),
+    ({"eps": 0.0}, "hyper-parameters"),
+    ({"beta1": 1.0}, "hyper-parameters"),
+    ({"lr": -1.0}, "hyper-parameters"),
+    ({"lambda_tv": float("nan")}, "hyper-parameters"),
+])
+def test_tv_adam_rejects_bad_descriptors(kw, msg):
+    n = C.c_int64(-1)
+    assert _lib.lib().ernn_tv_adam_workspace_bytes(C.byref(_tv_desc(**kw)), C.byref(n)) == -1
+    assert msg in _lib.lib().ernn_last_error().decode()
+
+
+def test_tv_adam_workspace_size_and_bounds_check():
+    n = C.c_int64(-1)
+    assert _lib.lib().ernn_tv_adam_workspace_bytes(C.byref(_tv_desc()), C.byref(n)) == 0
+    assert n.value == 6 * ((16 * 12 + 255) // 256) * 8          # one FP64 partial per block and field
+    d = _tv_desc()
+    d.lo[2], d.hi[2] = 5.0, 1.0
+    assert _lib.lib().ernn_tv_adam_workspace_bytes(C.byref(d), C.byref(n)) == -1
+    assert "bounds of field 2" in _lib.lib().ernn_last_error().decode()
+    call = _lib.TvAdamCall()
+    assert _lib.lib().ernn_tv_adam_step(C.byref(_tv_desc()), C.byref(call), None) == -1   # null workspace
