@@ -40,6 +40,7 @@ struct TvArgs {
   float lo[ERNN_TV_MAX_FIELDS], hi[ERNN_TV_MAX_FIELDS];
   double* partial;       // [n_fields][blocks_per_field] TV partial sums
   int nx, ny, log_params, blocks_per_field;
+  int blocks_per_xcd;    // grid.x = 8 x blocks_per_xcd
   float lambda;
 };
 
@@ -69,34 +70,14 @@ __device__ __forceinline__ float field_value(float th, int log_params, float lo,
 
 __device__ __forceinline__ float sgn(float d) { return d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f); }
 
-__global__ void __launch_bounds__(kTvThreads) tv_grad_kernel(TvArgs a) {
-  const int f = blockIdx.y;
-  const float* __restrict__ th = a.theta[f];
-  float* __restrict__ g = a.grad[f];
-  const float lo = a.lo[f], hi = a.hi[f];
-  const int64_t n = (int64_t)a.nx * a.ny;
-  const int64_t k = (int64_t)blockIdx.x * kTvThreads + threadIdx.x;
-  double tv = 0.0;
-  if (k < n) {
-    const int i = (int)(k / a.ny), j = (int)(k % a.ny);
-    float chain, dummy;
-    const float c = field_value(th[k], a.log_params, lo, hi, &chain);
-    float s = 0.f;
-    if (i > 0) s += sgn(c - field_value(th[k - a.ny], a.log_params, lo, hi, &dummy));
-    if (i + 1 < a.nx) {
-      const float d = field_value(th[k + a.ny], a.log_params, lo, hi, &dummy) - c;
-      s -= sgn(d);
-      tv += fabs((double)d);
-    }
-    if (j > 0) s += sgn(c - field_value(th[k - 1], a.log_params, lo, hi, &dummy));
-    if (j + 1 < a.ny) {
-      const float d = field_value(th[k + 1], a.log_params, lo, hi, &dummy) - c;
-      s -= sgn(d);
-      tv += fabs((double)d);
-    }
-    g[k] = g[k] + (a.lambda * s) * chain;
-  }
-  // block sum of the TV terms, fixed order: wavefront butterflies, then the four wave sums
+__device__ __forceinline__ void load_cells(const float* p, float (&c)[4]) {
+  const float4 t = *reinterpret_cast<const float4*>(p);
+  c[0] = t.x; c[1] = t.y; c[2] = t.z; c[3] = t.w;
+}
+
+// Block sum of the threads' TV terms into partial[f][lb], fixed order:
+// wavefront butterflies, then the four wave sums.
+__device__ __forceinline__ void block_partial(double tv, double* partial, int f, int blocks_per_field, int lb) {
   for (int o = 32; o > 0; o >>= 1) tv += __shfl_xor(tv, o, kWave);
   __shared__ double ws[kTvThreads / kWave];
   if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = tv;
@@ -104,8 +85,73 @@ __global__ void __launch_bounds__(kTvThreads) tv_grad_kernel(TvArgs a) {
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int w = 0; w < kTvThreads / kWave; ++w) s += ws[w];
-    a.partial[(int64_t)f * a.blocks_per_field + blockIdx.x] = s;
+    partial[(int64_t)f * blocks_per_field + lb] = s;
   }
+}
+
+// V consecutive cells of one row per thread (V = 4: 16-byte loads and
+// stores, rows of ny % 4 == 0 floats on 16-byte aligned fields; V = 1 any).
+// Block b runs on XCD b % 8 (round-robin dispatch); it takes logical block
+// (b % 8) x blocks_per_xcd + b / 8, so each XCD sweeps one contiguous band
+// of rows and the rows above and below a block are in its own L2 (with a
+// plain order every neighbour row sat on another XCD: measured 2x the
+// algorithmic fetch, 101 us per 6 x 2048^2 update).
+template <int V>
+__global__ void __launch_bounds__(kTvThreads) tv_grad_kernel(TvArgs a) {
+  const int lb = (int)(blockIdx.x & 7) * a.blocks_per_xcd + (int)(blockIdx.x >> 3);
+  if (lb >= a.blocks_per_field) return;   // whole block: the barrier below is not reached by anyone
+  const int f = blockIdx.y;
+  const float* __restrict__ th = a.theta[f];
+  float* __restrict__ g = a.grad[f];
+  const float lo = a.lo[f], hi = a.hi[f];
+  const int lp = a.log_params;
+  const int groups = a.ny / V;
+  const int64_t t = (int64_t)lb * kTvThreads + threadIdx.x;
+  double tv = 0.0;
+  if (t < (int64_t)a.nx * groups) {
+    const int i = (int)(t / groups), j0 = (int)(t % groups) * V;
+    const int64_t k = (int64_t)i * a.ny + j0;
+    float raw[V], up[V], dn[V], gv[V];
+    if constexpr (V == 4) {
+      load_cells(th + k, raw);
+      load_cells(g + k, gv);
+      if (i > 0) load_cells(th + k - a.ny, up);
+      if (i + 1 < a.nx) load_cells(th + k + a.ny, dn);
+    } else {
+      raw[0] = th[k];
+      gv[0] = g[k];
+      if (i > 0) up[0] = th[k - a.ny];
+      if (i + 1 < a.nx) dn[0] = th[k + a.ny];
+    }
+    float cv[V + 2], chain[V], dummy;   // cv[m]: C at column j0 - 1 + m
+#pragma unroll
+    for (int e = 0; e < V; ++e) cv[e + 1] = field_value(raw[e], lp, lo, hi, &chain[e]);
+    cv[0] = j0 > 0 ? field_value(th[k - 1], lp, lo, hi, &dummy) : 0.f;
+    cv[V + 1] = j0 + V < a.ny ? field_value(th[k + V], lp, lo, hi, &dummy) : 0.f;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      const float c = cv[e + 1];
+      float s = 0.f;
+      if (i > 0) s += sgn(c - field_value(up[e], lp, lo, hi, &dummy));
+      if (i + 1 < a.nx) {
+        const float d = field_value(dn[e], lp, lo, hi, &dummy) - c;
+        s -= sgn(d);
+        tv += fabs((double)d);
+      }
+      if (j0 + e > 0) s += sgn(c - cv[e]);
+      if (j0 + e + 1 < a.ny) {
+        const float d = cv[e + 2] - c;
+        s -= sgn(d);
+        tv += fabs((double)d);
+      }
+      gv[e] = gv[e] + (a.lambda * s) * chain[e];
+    }
+    if constexpr (V == 4)
+      *reinterpret_cast<float4*>(g + k) = make_float4(gv[0], gv[1], gv[2], gv[3]);
+    else
+      g[k] = gv[0];
+  }
+  block_partial(tv, a.partial, f, a.blocks_per_field, lb);
 }
 
 // One block per field: TV = sum of the partials in block order.
@@ -130,22 +176,46 @@ __global__ void __launch_bounds__(kTvThreads) tv_finalize_kernel(const double* p
 //   m = lerp(m, g, w), w = 1 - b1          (w < 0.5: m + w (g - m); else g - (g - m)(1 - w))
 //   v = v b2 + (1 - b2) (g g)              (_foreach_mul_, _foreach_addcmul_)
 //   theta = theta + step_size (m / (sqrt(v) / sqrt(1 - b2^t) + eps))   (_foreach_addcdiv_)
+__device__ __forceinline__ void adam_cell(float& th, float g, float& m, float& v, const AdamArgs& a) {
+  const float p = th;
+  if (a.weight_decay != 0.f) g = g + a.weight_decay * p;
+  const float mk = a.beta1w < 0.5f ? m + a.beta1w * (g - m) : g - (g - m) * (1.0f - a.beta1w);
+  const float vk = v * a.beta2 + a.beta2w * (g * g);
+  const float denom = sqrtf(vk) / a.bc2_sqrt + a.eps;
+  m = mk;
+  v = vk;
+  th = p + a.step_size * (mk / denom);
+}
+
+// V = 4: 16-byte accesses over the first n - n % 4 cells (fields 16-byte
+// aligned), block 0 takes the tail; V = 1: every cell.
+template <int V>
 __global__ void __launch_bounds__(kTvThreads) adam_kernel(AdamArgs a) {
   const int f = blockIdx.y;
   float* __restrict__ th = a.theta[f];
   const float* __restrict__ gr = a.grad[f];
   float* __restrict__ m = a.m[f];
   float* __restrict__ v = a.v[f];
-  for (int64_t k = (int64_t)blockIdx.x * kTvThreads + threadIdx.x; k < a.n; k += (int64_t)gridDim.x * kTvThreads) {
-    const float p = th[k];
-    float g = gr[k];
-    if (a.weight_decay != 0.f) g = g + a.weight_decay * p;
-    const float mk = a.beta1w < 0.5f ? m[k] + a.beta1w * (g - m[k]) : g - (g - m[k]) * (1.0f - a.beta1w);
-    const float vk = v[k] * a.beta2 + a.beta2w * (g * g);
-    const float denom = sqrtf(vk) / a.bc2_sqrt + a.eps;
-    m[k] = mk;
-    v[k] = vk;
-    th[k] = p + a.step_size * (mk / denom);
+  const int64_t groups = a.n / V;
+  for (int64_t q = (int64_t)blockIdx.x * kTvThreads + threadIdx.x; q < groups; q += (int64_t)gridDim.x * kTvThreads) {
+    if constexpr (V == 4) {
+      float t4[4], g4[4], m4[4], v4[4];
+      load_cells(th + 4 * q, t4);
+      load_cells(gr + 4 * q, g4);
+      load_cells(m + 4 * q, m4);
+      load_cells(v + 4 * q, v4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) adam_cell(t4[e], g4[e], m4[e], v4[e], a);
+      *reinterpret_cast<float4*>(th + 4 * q) = make_float4(t4[0], t4[1], t4[2], t4[3]);
+      *reinterpret_cast<float4*>(m + 4 * q) = make_float4(m4[0], m4[1], m4[2], m4[3]);
+      *reinterpret_cast<float4*>(v + 4 * q) = make_float4(v4[0], v4[1], v4[2], v4[3]);
+    } else {
+      adam_cell(th[q], gr[q], m[q], v[q], a);
+    }
+  }
+  if (V > 1 && blockIdx.x == 0) {
+    const int64_t k = groups * V + threadIdx.x;
+    if (k < a.n) adam_cell(th[k], gr[k], m[k], v[k], a);
   }
 }
 
@@ -201,8 +271,14 @@ int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c
     if (!c->theta[f] || !c->grad[f] || !c->m[f] || !c->v[f])
       return set_last_error(ERNN_E_INVALID, "theta, grad, m and v of field " + std::to_string(f) + " are required");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int64_t nb = tv_blocks(d->nx, d->ny);
   const int64_t n = (int64_t)d->nx * d->ny;
+  // 16-byte accesses when every field pointer allows them
+  bool aligned = true;
+  for (int f = 0; f < d->n_fields; ++f)
+    for (const void* q : {(const void*)c->theta[f], (const void*)c->grad[f], (const void*)c->m[f], (const void*)c->v[f]})
+      aligned = aligned && (reinterpret_cast<uintptr_t>(q) & 15) == 0;
+  const bool tv4 = aligned && d->ny % 4 == 0;
+  const int64_t nb = tv4 ? (n / 4 + kTvThreads - 1) / kTvThreads : tv_blocks(d->nx, d->ny);
   TvArgs t{};
   for (int f = 0; f < d->n_fields; ++f) {
     t.theta[f] = c->theta[f];
@@ -215,8 +291,13 @@ int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c
   t.ny = d->ny;
   t.log_params = d->log_params ? 1 : 0;
   t.blocks_per_field = (int)nb;
+  t.blocks_per_xcd = (int)((nb + 7) / 8);
   t.lambda = (float)d->lambda_tv;   // the mul's backward scales by the FP32 scalar
-  hipLaunchKernelGGL(tv_grad_kernel, dim3((unsigned)nb, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, t);
+  const unsigned grid = 8u * (unsigned)t.blocks_per_xcd;
+  if (tv4)
+    hipLaunchKernelGGL(tv_grad_kernel<4>, dim3(grid, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, t);
+  else
+    hipLaunchKernelGGL(tv_grad_kernel<1>, dim3(grid, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, t);
   TV_HIP(hipGetLastError());
   if (c->tv) {
     hipLaunchKernelGGL(tv_finalize_kernel, dim3((unsigned)d->n_fields), dim3(kTvThreads), 0, st,
@@ -241,8 +322,12 @@ int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c
   a.bc2_sqrt = (float)std::pow(bc2, 0.5);
   a.eps = (float)d->eps;
   a.weight_decay = (float)d->weight_decay;
-  const int64_t blocks = std::min<int64_t>((n + kTvThreads - 1) / kTvThreads, 16384);
-  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)blocks, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, a);
+  const int V = aligned ? 4 : 1;
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n / V + kTvThreads - 1) / kTvThreads, 16384));
+  if (V == 4)
+    hipLaunchKernelGGL(adam_kernel<4>, dim3((unsigned)blocks, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, a);
+  else
+    hipLaunchKernelGGL(adam_kernel<1>, dim3((unsigned)blocks, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, a);
   TV_HIP(hipGetLastError());
   return ERNN_OK;
 }

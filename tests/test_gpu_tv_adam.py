@@ -152,3 +152,45 @@ def test_di_inverter_fused_update_matches_unfused():
     np.testing.assert_allclose(h1["loss_tv"], h0["loss_tv"], rtol=1e-5)
     for a, b in zip(p1, p0):
         assert float((a - b).abs().max()) < 2e-5
+
+
+def test_vector_and_scalar_paths_agree():
+    """16-byte-aligned fields with ny % 4 == 0 take the 4-cells-per-thread
+    kernels; fields at an odd float offset take the one-cell kernels.  The
+    per-cell arithmetic is the same, so theta, m, v and grad agree bit for
+    bit over several steps (the TV sums differ only in summation order)."""
+    nx, ny = 36, 44
+    th0 = make_fields(nx, ny, seed=5)
+    gs = torch.Generator().manual_seed(9)
+    gdata = [[(0.1 * torch.randn((1, nx, ny), generator=gs)).to(DEV) for _ in range(6)] for _ in range(3)]
+    adam = dict(lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    aligned = fused_path(th0, gdata, 1e-12, 3, **adam)
+    # the same values one float into larger buffers: every pointer is 4 bytes off 16-byte alignment
+    bufs = [torch.empty(nx * ny + 1, device=DEV) for _ in range(6)]
+    shifted = []
+    for b, t in zip(bufs, th0):
+        v = b[1:].view(1, nx, ny)
+        v.copy_(t)
+        shifted.append(v)
+    assert all(v.data_ptr() % 16 for v in shifted)
+    params = shifted
+    opt = FusedTVAdam(params, lambda_tv=1e-12, bounds=BOUNDS, **adam)
+    gbufs = [torch.empty(nx * ny + 1, device=DEV) for _ in range(6)]
+    for s in range(3):
+        for p, gb, gd in zip(params, gbufs, gdata[s]):
+            gv = gb[1:].view(1, nx, ny)
+            gv.copy_(gd)
+            p.grad = gv
+        # Adam state at an odd offset too
+        if s == 0:
+            mb = [torch.zeros(nx * ny + 1, device=DEV) for _ in range(12)]
+            opt.state["exp_avg"] = [b[1:].view(1, nx, ny) for b in mb[:6]]
+            opt.state["exp_avg_sq"] = [b[1:].view(1, nx, ny) for b in mb[6:]]
+        opt.step()
+        torch.cuda.synchronize()
+        for f in range(6):
+            assert torch.equal(params[f], aligned[s]["theta"][f]), (s, f)
+            assert torch.equal(params[f].grad, aligned[s]["grad"][f]), (s, f)
+            assert torch.equal(opt.state["exp_avg"][f], aligned[s]["m"][f]), (s, f)
+            assert torch.equal(opt.state["exp_avg_sq"][f], aligned[s]["v"][f]), (s, f)
+            assert abs(opt.last_tv[f].item() - aligned[s]["tv"][f]) <= 1e-12 * aligned[s]["tv"][f]
