@@ -78,6 +78,9 @@ def parse():
                     help="steps of one shot timed for the autograd baseline (scaled to the survey)")
     ap.add_argument("--per-instance", type=int, default=1,
                     help="batched surveys: time every instance alone for p50 / max")
+    ap.add_argument("--fem-n", type=int, default=301,
+                    help="FEM workloads: nodes per side (the reference's 301; larger grids keep the same square and "
+                         "physics, dt from the same CFL rule)")
     return ap.parse_args()
 
 
@@ -368,13 +371,13 @@ def fem_main(args):
     from ernn import fem
     dev = torch.device("cuda:0")
     spec = FEM_WORKLOADS[args.workload]
-    nx, L = 301, 0.4
+    nx, L = args.fem_n, 0.4
+    h = L / (nx - 1)
     x = np.linspace(-L / 2, L / 2, nx)
     X, Y = np.meshgrid(x, x, indexing="ij")
     vp, vs, _ = fem.calculate_wave_velocities_anisotropic(*FEM_C, FEM_RHO)
     nt_req = args.nt or spec["nt"]
-    dt, nt, t = fem.time_axis(nt_req, spec["t_total"] * (nt_req - 1) / (spec["nt"] - 1), L / 300, L / 300,
-                              max(vp, vs))
+    dt, nt, t = fem.time_axis(nt_req, spec["t_total"] * (nt_req - 1) / (spec["nt"] - 1), h, h, max(vp, vs))
     g32 = fem.gaussian_point_load(X, Y, 0.0, 0.0, 5e6).astype(np.float32)
     w32 = np.asarray(fem.ricker_wavelet(t, 100e3, 10e-6, 1e10), np.float32)
     D = None
@@ -383,7 +386,7 @@ def fem_main(args):
         D = np.where(mask, 0.5, 1.0).astype(np.float32)
     Dt = torch.tensor(D, device=dev) if D is not None else None
     stride = 5
-    eng = fem.FEMEngine(nx, nx, L / 300, L / 300, nt, dt, FEM_RHO, fem.nodal_loads(g32), damage=D is not None,
+    eng = fem.FEMEngine(nx, nx, h, h, nt, dt, FEM_RHO, fem.nodal_loads(g32), damage=D is not None,
                         stride=stride, use_graph=bool(args.graph), device=dev)
     C32 = np.asarray(FEM_C, np.float32)
     tune = None
@@ -414,7 +417,8 @@ def fem_main(args):
     traffic = valu = None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        entry = json.load(open(tpath)).get(f"{args.workload}/fem_step_kernel")
+        wname = args.workload if nx == 301 else f"{args.workload}-{nx}"
+        entry = json.load(open(tpath)).get(f"{wname}/fem_step_kernel")
         if entry:
             traffic = entry["bytes"]
             if entry.get("valu_insts"):
@@ -449,7 +453,8 @@ def fem_main(args):
         "metric": "FEM generator Gnode-steps/s (1 GPU)", "value": round(value, 3), "unit": "Gnode-steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(run_s * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": args.workload, "grid": f"{nx}x{nx}", "nt": nt, "frames_every": stride,
+        "config": {"workload": args.workload if nx == 301 else f"{args.workload}-{nx}", "grid": f"{nx}x{nx}",
+                   "nt": nt, "frames_every": stride,
                    "damage": D is not None, "tile": list(eng.tile), "graph": bool(args.graph),
                    "parallelism": "none"},
         "s_per_run": round(run_s, 5),

@@ -22,6 +22,8 @@ hipError_t launch_fem_step(const FemStepArgs& a, int V, int R, int blocks, hipSt
 hipError_t launch_fem_setup(const FemCallDev* call, int64_t field, double dx, double dy, hipStream_t st);
 hipError_t launch_fem_pack(const FemCallDev* call, int64_t field, int nx, int ny, int pitch, hipStream_t st);
 hipError_t launch_fem_zero(const FemCallDev* call, int which, int64_t n, hipStream_t st);
+hipError_t launch_fem_loads(const FemCallDev* call, const SrcEv* loads, int n, int64_t field, int pitch,
+                            hipStream_t st);
 hipError_t launch_fem_set_call(FemCallDev* dst, const FemCallDev& v, hipStream_t st);
 }  // namespace ernn
 
@@ -49,7 +51,7 @@ struct FemPlan {
   int nx = 0, ny = 0, nt = 0, mode = 0, stride = 1;
   double dx = 0, dy = 0, dt = 0, rho = 0;
   float scale = 0.7f;
-  int V = 1, R = 2;
+  int V = 1, R = 1;
   int pitch = 0;
   int64_t field = 0;       // floats per padded field
   int64_t ws_floats = 0;
@@ -85,36 +87,31 @@ void destroy(FemPlan* p) {
   delete p;
 }
 
-// Tiles of R node rows x 64 V node columns over the interior rows 1 .. nx-2
-// (columns from 0, so that the rows stay aligned), ordered row band by row
-// band so that each XCD's contiguous share of tiles is a band of rows; the
-// loads of a tile sorted by row.
+// Tiles of R node rows x 63 node columns over the interior nodes (rows 1 ..
+// nx-2 from row 1, columns 1 .. ny-2 from column 1: a wavefront's lane 0
+// holds the column before the tile), ordered row band by row band so that
+// each XCD's contiguous share of tiles is a band of rows; tiles holding a
+// nodal load are flagged (only they read the load fields).
+constexpr int kFemTileCols = kWave - 1;
+
 int build_tables(FemPlan* p) {
   free_tables(p);
-  const int cols = 64 * p->V;
+  const int cols = kFemTileCols;
   const int tr = (p->nx - 2 + p->R - 1) / p->R;
-  const int tc = (p->ny - 1 + cols - 1) / cols;
+  const int tc = (p->ny - 2 + cols - 1) / cols;
   p->n_tiles = tr * tc;
   const int per_xcd = (p->n_tiles + 7) / 8;
   p->blocks = 8 * std::max(1, (per_xcd + kWavesPerBlock - 1) / kWavesPerBlock);
-  std::vector<std::vector<SrcEv>> ev(p->n_tiles);
-  for (const auto& l : p->loads) {
-    const int t = ((l.i - 1) / p->R) * tc + l.j / cols;
-    ev[t].push_back(SrcEv{l.i, l.j, l.w, l.comp});
-  }
+  std::vector<int> has(p->n_tiles, 0);
+  for (const auto& l : p->loads) has[((l.i - 1) / p->R) * tc + (l.j - 1) / cols] = 1;
   std::vector<FemTile> tiles(p->n_tiles);
-  std::vector<SrcEv> flat;
   for (int a = 0; a < tr; ++a)
-    for (int b = 0; b < tc; ++b) {
-      const int t = a * tc + b;
-      auto& e = ev[t];
-      std::stable_sort(e.begin(), e.end(), [](const SrcEv& x, const SrcEv& y) { return x.i < y.i; });
-      tiles[t] = FemTile{1 + a * p->R, b * cols, (int)flat.size(), (int)(flat.size() + e.size())};
-      flat.insert(flat.end(), e.begin(), e.end());
-    }
+    for (int b = 0; b < tc; ++b) tiles[a * tc + b] = FemTile{1 + a * p->R, 1 + b * cols, has[a * tc + b], 0};
   FEM_HIP(hipMalloc(&p->d_tiles, sizeof(FemTile) * tiles.size()));
   FEM_HIP(hipMemcpy(p->d_tiles, tiles.data(), sizeof(FemTile) * tiles.size(), hipMemcpyHostToDevice));
-  if (!flat.empty()) {
+  if (!p->loads.empty()) {
+    std::vector<SrcEv> flat;
+    for (const auto& l : p->loads) flat.push_back(SrcEv{l.i, l.j, l.w, l.comp});
     FEM_HIP(hipMalloc(&p->d_loads, sizeof(SrcEv) * flat.size()));
     FEM_HIP(hipMemcpy(p->d_loads, flat.data(), sizeof(SrcEv) * flat.size(), hipMemcpyHostToDevice));
   }
@@ -146,7 +143,6 @@ FemStepArgs step_args(const FemPlan* p, int n) {
   FemStepArgs a{};
   a.call = p->d_call;
   a.tiles = p->d_tiles;
-  a.loads = p->d_loads;
   a.field = p->field;
   a.rec_off = ((n + 1) % p->stride == 0) ? (int64_t)((n + 1) / p->stride) * p->frame : -1;
   a.n_tiles = p->n_tiles;
@@ -168,6 +164,7 @@ FemStepArgs step_args(const FemPlan* p, int n) {
 int enqueue_setup(const FemPlan* p, hipStream_t st) {
   FEM_HIP(launch_fem_zero(p->d_call, 0, (int64_t)kFemSlots * p->field, st));
   if (p->mode == ERNN_FEM_DAMAGE) FEM_HIP(launch_fem_pack(p->d_call, p->field, p->nx, p->ny, p->pitch, st));
+  FEM_HIP(launch_fem_loads(p->d_call, p->d_loads, (int)p->loads.size(), p->field, p->pitch, st));
   FEM_HIP(launch_fem_setup(p->d_call, p->field, p->dx, p->dy, st));
   FEM_HIP(launch_fem_zero(p->d_call, 1, (int64_t)p->n_frames * p->frame, st));
   FEM_HIP(launch_fem_zero(p->d_call, 2, (int64_t)p->n_frames * p->frame, st));
@@ -219,10 +216,9 @@ int ernn_fem_plan_create(const ernn_fem_desc_t* d, void** plan_out) {
   int V = d->tile_v, R = d->tile_r;
   if (V == 0 && R == 0) {
     V = 1;
-    R = 2;
+    R = 1;
   }
-  if (!fem_shape_ok(V, R))
-    return fail(ERNN_E_INVALID, "tile (V, R) must be one of (1,1) (1,2) (1,4) (1,8) (2,2) (2,4) (4,2) (4,4)");
+  if (!fem_shape_ok(V, R)) return fail(ERNN_E_INVALID, "tile (V, R) must be (1, 1), (1, 2), (1, 4) or (1, 8)");
   const int64_t pitch = kLeftPad + round_up(d->ny, kColAlign);
   const int64_t field = (int64_t)(d->nx + kFemExtraRows) * pitch;
   if (field >= (int64_t(1) << 30)) return fail(ERNN_E_CAPACITY, "grid too large (a padded field must hold < 2^30 floats)");
@@ -265,6 +261,18 @@ int ernn_fem_plan_create(const ernn_fem_desc_t* d, void** plan_out) {
     }
     if (l.i >= 1 && l.i <= d->nx - 2 && l.j >= 1 && l.j <= d->ny - 2) p->loads.push_back(l);
   }
+  // one weight per (node, component): duplicates are summed in FP32 in array order
+  std::stable_sort(p->loads.begin(), p->loads.end(), [](const ernn_source_t& x, const ernn_source_t& y) {
+    return x.comp != y.comp ? x.comp < y.comp : (x.i != y.i ? x.i < y.i : x.j < y.j);
+  });
+  std::vector<ernn_source_t> merged;
+  for (const auto& l : p->loads) {
+    if (!merged.empty() && merged.back().i == l.i && merged.back().j == l.j && merged.back().comp == l.comp)
+      merged.back().w += l.w;
+    else
+      merged.push_back(l);
+  }
+  p->loads.swap(merged);
   int r = ERNN_OK;
   hipError_t e = hipGetDevice(&p->device);
   if (e == hipSuccess) e = hipMalloc(&p->d_call, sizeof(FemCallDev));
@@ -321,8 +329,7 @@ int ernn_fem_run(void* plan, const ernn_fem_call_t* call, void* stream) {
 int ernn_fem_set_tiling(void* plan, int v, int r) {
   if (!plan) return fail(ERNN_E_INVALID, "null plan");
   FemPlan* p = static_cast<FemPlan*>(plan);
-  if (!fem_shape_ok(v, r))
-    return fail(ERNN_E_INVALID, "tile (V, R) must be one of (1,1) (1,2) (1,4) (1,8) (2,2) (2,4) (4,2) (4,4)");
+  if (!fem_shape_ok(v, r)) return fail(ERNN_E_INVALID, "tile (V, R) must be (1, 1), (1, 2), (1, 4) or (1, 8)");
   FEM_HIP(hipSetDevice(p->device));
   FEM_HIP(hipDeviceSynchronize());
   if (p->graph) {

@@ -27,12 +27,18 @@
 // FP64 and rounded once).  Every cell's two triangles give four corner
 // forces; a node's force is the sum over its four cells.
 //
-// Work decomposition: one wavefront owns R node rows x 64 V node columns and
-// sweeps its cell rows top to bottom from registers; lane l holds V columns
-// and gets the column neighbours by DPP wavefront shifts (ernn_device.h), the
-// edge lanes load the halo columns.  A cell row's lower corner forces are
-// carried in registers to the next node row, so each u value is loaded once
-// per tile and no atomics or LDS are used.
+// Work decomposition: one wavefront owns R node rows x 63 node columns and
+// sweeps its cell rows top to bottom from registers.  Lane l holds node
+// column j0 - 1 + l and computes the one cell to its right, (ci, j0 - 1 + l),
+// per cell row; the right neighbour's u comes from lane l + 1 by a DPP
+// wavefront shift (lane 63 loads it), and the left cell's corner forces
+// F2, F3 come from lane l - 1 the same way, so every cell is computed once
+// per cell row.  Lane 0 only feeds lane 1 (its own left cell belongs to the
+// previous tile), which is why a tile stores 63 columns.  A cell row's lower
+// corner forces are carried in registers to the next node row: (R + 1) / R
+// cells per node, each u value loaded once per tile, no atomics, no LDS.
+// (A first version computed the V + 1 cells around V columns per lane:
+// four cells per node at R = 1, VALU-bound at 80 % of peak on 2049^2.)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -68,154 +74,121 @@ __device__ __forceinline__ float fem_update(float u, float up, float fext, float
   return base + (dt2 * (fext - fint)) / M;
 }
 
-template <int MODE, int V, int R>
+// DPP wave_shr:1 with bound_ctrl: lane l receives lane l - 1's value, lane 0 gets 0.
+__device__ __forceinline__ float from_left(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+template <int MODE, int R>
 __device__ __forceinline__ void fem_tile(const FemStepArgs& a, const FemTile& tl, gfloat* ws, const gfloat* wav,
                                          gfloat* frx, gfloat* fry, const float (&Q)[16]) {
   const int lane = threadIdx.x & (kWave - 1);
   const uint32_t pitch = (uint32_t)a.pitch;
-  const uint32_t lo = lane_off<V>(lane), ho = halo_off<V>(lane);
   const int i0 = tl.i0, j0 = tl.j0;
+  const int c = j0 - 1 + lane;                   // the lane's node column
+  const uint32_t lo = (uint32_t)lane;            // from column j0 - 1
+  const uint32_t ho = (uint32_t)(lane + (lane == kWave - 1));   // lane 63 also reads column j0 + 63
   const int cur = a.step & 1;
   const gfloat* uxc = ws + (int64_t)cur * a.field;
   const gfloat* uyc = ws + (int64_t)(2 + cur) * a.field;
   gfloat* uxp = ws + (int64_t)(1 - cur) * a.field;
   gfloat* uyp = ws + (int64_t)(3 - cur) * a.field;
   const gfloat* dfl = ws + 4 * a.field;
+  const gfloat* wlx = ws + (int64_t)kFemLoadSlot * a.field;
+  const gfloat* wly = ws + (int64_t)(kFemLoadSlot + 1) * a.field;
+  const bool loads = tl.has_loads != 0;   // wavefront-uniform
 
   // every load of the tile first: R + 2 rows of u^n (and of the damage
-  // factor), R rows of u^{n-1}
-  RawRow<V> rx[R + 2], ry[R + 2], rd[MODE == kFemDamage ? R + 2 : 1];
+  // factor), R rows of u^{n-1} (and of the load weights, in tiles that have any)
+  RawRow<1> rx[R + 2], ry[R + 2], rd[MODE == kFemDamage ? R + 2 : 1];
 #pragma unroll
   for (int r = 0; r < R + 2; ++r) {
-    const uint32_t ro = row_off(pitch, i0 - 1 + r, j0);
-    rx[r] = load_raw<V>(uxc, ro, lo, ho);
-    ry[r] = load_raw<V>(uyc, ro, lo, ho);
-    if constexpr (MODE == kFemDamage) rd[r] = load_raw<V>(dfl, ro, lo, ho);
+    const uint32_t ro = row_off(pitch, i0 - 1 + r, j0 - 1);
+    rx[r] = load_raw<1>(uxc, ro, lo, ho);
+    ry[r] = load_raw<1>(uyc, ro, lo, ho);
+    if constexpr (MODE == kFemDamage) rd[r] = load_raw<1>(dfl, ro, lo, ho);
   }
-  float px[R][V], py[R][V];
+  float px[R], py[R], lwx[R], lwy[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const uint32_t ro = row_off(pitch, i0 + r, j0) + lo;
-    load_vec<V>(at(uxp, ro), px[r]);
-    load_vec<V>(at(uyp, ro), py[r]);
+    const uint32_t ro = row_off(pitch, i0 + r, j0 - 1) + lo;
+    px[r] = *at(uxp, ro);
+    py[r] = *at(uyp, ro);
+    lwx[r] = loads ? *at(wlx, ro) : 0.f;
+    lwy[r] = loads ? *at(wly, ro) : 0.f;
   }
   const float wv = wav[a.step];
 
-  const int c0 = j0 + lane * V;     // the lane's first node column
-  Row<V> xa = make_row(rx[0], lane), ya = make_row(ry[0], lane);
-  Row<V> da;
+  // Row<1>: v[1] = the lane's column, v[2] = the column to its right
+  Row<1> xa = make_row(rx[0], lane), ya = make_row(ry[0], lane);
+  Row<1> da;
   if constexpr (MODE == kFemDamage) da = make_row(rd[0], lane);
-  float lx[V + 1], ly[V + 1];       // y-edge differences of the upper row of the cell row
-#pragma unroll
-  for (int k = 0; k <= V; ++k) {
-    lx[k] = xa.v[k + 1] - xa.v[k];
-    ly[k] = ya.v[k + 1] - ya.v[k];
-  }
-  float gx[V], gy[V];               // forces of node row ci from cell row ci - 1
-#pragma unroll
-  for (int e = 0; e < V; ++e) gx[e] = gy[e] = 0.f;
+  float lx = xa.v[2] - xa.v[1], ly = ya.v[2] - ya.v[1];   // y-edge of the cell row's upper node row
+  float gx = 0.f, gy = 0.f;                                // forces of node row ci from cell row ci - 1
 
 #pragma unroll
   for (int r = 0; r <= R; ++r) {
     const int ci = i0 - 1 + r;      // cell row: nodes ci (row a) and ci + 1 (row b)
-    const Row<V> xb = make_row(rx[r + 1], lane), yb = make_row(ry[r + 1], lane);
-    Row<V> db;
+    const Row<1> xb = make_row(rx[r + 1], lane), yb = make_row(ry[r + 1], lane);
+    Row<1> db;
     if constexpr (MODE == kFemDamage) db = make_row(rd[r + 1], lane);
-    float hx[V + 2], hy[V + 2];     // x-edge differences at columns c0 - 1 + k
-#pragma unroll
-    for (int k = 0; k < V + 2; ++k) {
-      hx[k] = xb.v[k] - xa.v[k];
-      hy[k] = yb.v[k] - ya.v[k];
+    // the cell (ci, c): x-edges at columns c (bottom) and c + 1 (top), y-edges of rows ci (left) and ci + 1 (right)
+    const float h0x = xb.v[1] - xa.v[1], h0y = yb.v[1] - ya.v[1];
+    const float h1x = xb.v[2] - xa.v[2], h1y = yb.v[2] - ya.v[2];
+    const float rx_ = xb.v[2] - xb.v[1], ry_ = yb.v[2] - yb.v[1];
+    const bool even = ((ci + c) & 1) == 0;
+    // triangle 1 holds the bottom x-edge n0 -> n1 and, for even cells, the
+    // right y-edge n1 -> n2 ([n0,n1,n2]), for odd cells the left n0 -> n3
+    // ([n0,n1,n3]); triangle 2 holds the top x-edge n3 -> n2 and the other y-edge
+    Flux t1 = tri_flux(Q, h0x, h0y, even ? rx_ : lx, even ? ry_ : ly);
+    Flux t2 = tri_flux(Q, h1x, h1y, even ? lx : rx_, even ? ly : ry_);
+    if constexpr (MODE == kFemDamage) {
+      // C of a triangle = factor at the node nearest its centroid:
+      // [n0,n1,n2] -> n1, [n0,n2,n3] -> n3, [n0,n1,n3] -> n0, [n1,n2,n3] -> n2
+      t1 = scale_flux(t1, even ? db.v[1] : da.v[1]);
+      t2 = scale_flux(t2, even ? da.v[2] : db.v[2]);
     }
-    float rxd[V + 1], ryd[V + 1];   // y-edge differences of row b
-#pragma unroll
-    for (int k = 0; k <= V; ++k) {
-      rxd[k] = xb.v[k + 1] - xb.v[k];
-      ryd[k] = yb.v[k + 1] - yb.v[k];
-    }
-    // cells k = 0 .. V (column c0 - 1 + k): corner forces F0 (n0), F1 (n1), F2 (n2), F3 (n3)
-    float f0x[V + 1], f0y[V + 1], f1x[V + 1], f1y[V + 1], f2x[V + 1], f2y[V + 1], f3x[V + 1], f3y[V + 1];
-#pragma unroll
-    for (int k = 0; k <= V; ++k) {
-      const bool even = ((ci + c0 - 1 + k) & 1) == 0;
-      // triangle 1 holds the bottom x-edge n0 -> n1 and, for even cells, the
-      // right y-edge n1 -> n2 ([n0,n1,n2]), for odd cells the left n0 -> n3
-      // ([n0,n1,n3]); triangle 2 holds the top x-edge n3 -> n2 and the other y-edge
-      const float y1x = even ? rxd[k] : lx[k], y1y = even ? ryd[k] : ly[k];
-      const float y2x = even ? lx[k] : rxd[k], y2y = even ? ly[k] : ryd[k];
-      Flux t1 = tri_flux(Q, hx[k], hy[k], y1x, y1y);
-      Flux t2 = tri_flux(Q, hx[k + 1], hy[k + 1], y2x, y2y);
-      if constexpr (MODE == kFemDamage) {
-        // C of a triangle = factor at the node nearest its centroid:
-        // [n0,n1,n2] -> n1, [n0,n2,n3] -> n3, [n0,n1,n3] -> n0, [n1,n2,n3] -> n2
-        t1 = scale_flux(t1, even ? db.v[k] : da.v[k]);
-        t2 = scale_flux(t2, even ? da.v[k + 1] : db.v[k + 1]);
-      }
-      const float lqx = even ? t2.qx : t1.qx, lqy = even ? t2.qy : t1.qy;   // PY of the left-edge triangle
-      const float rqx = even ? t1.qx : t2.qx, rqy = even ? t1.qy : t2.qy;   // PY of the right-edge triangle
-      f0x[k] = -t1.px - lqx;  f0y[k] = -t1.py - lqy;
-      f1x[k] = t1.px - rqx;   f1y[k] = t1.py - rqy;
-      f2x[k] = t2.px + rqx;   f2y[k] = t2.py + rqy;
-      f3x[k] = lqx - t2.px;   f3y[k] = lqy - t2.py;
-    }
+    const float lqx = even ? t2.qx : t1.qx, lqy = even ? t2.qy : t1.qy;   // PY of the left-edge triangle
+    const float rqx = even ? t1.qx : t2.qx, rqy = even ? t1.qy : t2.qy;   // PY of the right-edge triangle
+    const float f0x = -t1.px - lqx, f0y = -t1.py - lqy;   // corner n0 = node (ci, c)
+    const float f1x = t1.px - rqx, f1y = t1.py - rqy;     // n1 = (ci + 1, c)
+    const float f2x = t2.px + rqx, f2y = t2.py + rqy;     // n2 = (ci + 1, c + 1)
+    const float f3x = lqx - t2.px, f3y = lqy - t2.py;     // n3 = (ci, c + 1)
+    // the left cell (ci, c - 1) gives node (ci, c) its n3 and node (ci + 1, c) its n2
+    const float l2x = from_left(f2x), l2y = from_left(f2y);
+    const float l3x = from_left(f3x), l3y = from_left(f3y);
     if (r >= 1) {                   // node row i = ci: cells (ci - 1, .) carried in g, cells (ci, .) here
       const int i = ci;
-      float fex[V], fey[V];
-#pragma unroll
-      for (int e = 0; e < V; ++e) fex[e] = fey[e] = 0.f;
-      float M[V];
-#pragma unroll
-      for (int e = 0; e < V; ++e) M[e] = ((i + c0 + e) & 1) ? a.m_odd : a.m_even;
-      // nodal loads of this row: f_ext = ((w(t_n) g) / rho) M  (PF:482-484)
-      for (int q = tl.ev_begin; q < tl.ev_end; ++q) {
-        const SrcEv s = a.loads[q];
-        if (s.i != i) continue;
-        const int d = s.j - c0;
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          if (d == e) {
-            const float f = ((wv * s.w) * a.rinv) * M[e];
-            if (s.comp) fey[e] += f; else fex[e] += f;
-          }
-        }
-      }
-      const uint32_t ro = row_off(pitch, i, j0) + lo;
-      const bool row_in = i <= a.nx - 2;
-#pragma unroll
-      for (int e = 0; e < V; ++e) {
-        const float fx = (gx[e] + f0x[e + 1]) + f3x[e];
-        const float fy = (gy[e] + f0y[e + 1]) + f3y[e];
-        const float nx_ = fem_update(xa.v[e + 1], px[r - 1][e], fex[e], fx, a.dt2, M[e]);
-        const float ny_ = fem_update(ya.v[e + 1], py[r - 1][e], fey[e], fy, a.dt2, M[e]);
-        const int col = c0 + e;
-        if (row_in && col >= 1 && col <= a.ny - 2) {
-          *at(uxp, ro + e) = nx_;
-          *at(uyp, ro + e) = ny_;
-          if (a.rec_off >= 0) {
-            const int64_t fo = a.rec_off + (int64_t)i * a.ny + col;
-            frx[fo] = nx_ * a.scale;
-            fry[fo] = ny_ * a.scale;
-          }
+      const float M = ((i + c) & 1) ? a.m_odd : a.m_even;
+      // nodal load: f_ext = ((w(t_n) g) / rho) M  (PF:482-484); g = 0 gives f_ext = 0 exactly
+      const float fex = ((wv * lwx[r - 1]) * a.rinv) * M;
+      const float fey = ((wv * lwy[r - 1]) * a.rinv) * M;
+      const float fx = (gx + f0x) + l3x;
+      const float fy = (gy + f0y) + l3y;
+      const float nx_ = fem_update(xa.v[1], px[r - 1], fex, fx, a.dt2, M);
+      const float ny_ = fem_update(ya.v[1], py[r - 1], fey, fy, a.dt2, M);
+      if (lane >= 1 && i <= a.nx - 2 && c >= 1 && c <= a.ny - 2) {
+        const uint32_t ro = row_off(pitch, i, j0 - 1) + lo;
+        *at(uxp, ro) = nx_;
+        *at(uyp, ro) = ny_;
+        if (a.rec_off >= 0) {
+          const int64_t fo = a.rec_off + (int64_t)i * a.ny + c;
+          frx[fo] = nx_ * a.scale;
+          fry[fo] = ny_ * a.scale;
         }
       }
     }
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-      gx[e] = f1x[e + 1] + f2x[e];
-      gy[e] = f1y[e + 1] + f2y[e];
-    }
+    gx = f1x + l2x;
+    gy = f1y + l2y;
     xa = xb;
     ya = yb;
     if constexpr (MODE == kFemDamage) da = db;
-#pragma unroll
-    for (int k = 0; k <= V; ++k) {
-      lx[k] = rxd[k];
-      ly[k] = ryd[k];
-    }
+    lx = rx_;
+    ly = ry_;
   }
 }
 
-template <int MODE, int V, int R>
+template <int MODE, int R>
 __global__ void __launch_bounds__(kBlock) fem_step_kernel(FemStepArgs a) {
   const FemCallDev& c = *a.call;
   gfloat* ws = (gfloat*)c.ws;
@@ -225,7 +198,7 @@ __global__ void __launch_bounds__(kBlock) fem_step_kernel(FemStepArgs a) {
   for (int k = 0; k < 16; ++k) Q[k] = qp[k];
   const TileRange tr = tile_range(a.n_tiles);
   for (int t = tr.t; t < tr.end; t += tr.step)
-    fem_tile<MODE, V, R>(a, a.tiles[t], ws, (const gfloat*)c.wav, (gfloat*)c.frames_x, (gfloat*)c.frames_y, Q);
+    fem_tile<MODE, R>(a, a.tiles[t], ws, (const gfloat*)c.wav, (gfloat*)c.frames_x, (gfloat*)c.frames_y, Q);
 }
 
 // Q from the six constants (FP64, rounded once); one thread.
@@ -253,6 +226,16 @@ __global__ void fem_pack_kernel(const FemCallDev* call, int64_t field, int nx, i
   }
 }
 
+// Nodal load weights into workspace slots 5 (x) and 6 (y); one thread per
+// load, every (node, component) once (the plan merges duplicates).
+__global__ void fem_loads_kernel(const FemCallDev* call, const SrcEv* loads, int n, int64_t field, int pitch) {
+  const FemCallDev& c = *call;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const SrcEv s = loads[k];
+  c.ws[(int64_t)(kFemLoadSlot + (s.comp ? 1 : 0)) * field + (int64_t)(s.i + 1) * pitch + kLeftPad + s.j] = s.w;
+}
+
 // Zero n floats of the workspace (state slots) or of both frame arrays.
 __global__ void fem_zero_kernel(const FemCallDev* call, int which, int64_t n) {
   const FemCallDev& c = *call;
@@ -266,14 +249,14 @@ __global__ void fem_set_call_kernel(FemCallDev* dst, FemCallDev v) { *dst = v; }
 // --------------------------------------------------------------------------
 // Launchers
 // --------------------------------------------------------------------------
-#define ERNN_FEM_SHAPES(X) X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 2) X(2, 4) X(4, 2) X(4, 4)
+#define ERNN_FEM_SHAPES(X) X(1) X(2) X(4) X(8)
 
 template <int MODE>
-hipError_t launch_fem_m(const FemStepArgs& a, int V, int R, int blocks, hipStream_t st) {
-#define ERNN_FEM_CASE(v, r)                                                                         \
-  if (V == v && R == r) {                                                                           \
-    hipLaunchKernelGGL((fem_step_kernel<MODE, v, r>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a); \
-    return hipGetLastError();                                                                       \
+hipError_t launch_fem_m(const FemStepArgs& a, int R, int blocks, hipStream_t st) {
+#define ERNN_FEM_CASE(r)                                                                         \
+  if (R == r) {                                                                                  \
+    hipLaunchKernelGGL((fem_step_kernel<MODE, r>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a); \
+    return hipGetLastError();                                                                    \
   }
   ERNN_FEM_SHAPES(ERNN_FEM_CASE)
 #undef ERNN_FEM_CASE
@@ -281,15 +264,17 @@ hipError_t launch_fem_m(const FemStepArgs& a, int V, int R, int blocks, hipStrea
 }
 
 bool fem_shape_ok(int V, int R) {
-#define ERNN_FEM_OK(v, r) if (V == v && R == r) return true;
+  if (V != 1) return false;
+#define ERNN_FEM_OK(r) if (R == r) return true;
   ERNN_FEM_SHAPES(ERNN_FEM_OK)
 #undef ERNN_FEM_OK
   return false;
 }
 
 hipError_t launch_fem_step(const FemStepArgs& a, int V, int R, int blocks, hipStream_t st) {
-  return a.mode == kFemDamage ? launch_fem_m<kFemDamage>(a, V, R, blocks, st)
-                              : launch_fem_m<kFemConst>(a, V, R, blocks, st);
+  if (V != 1) return hipErrorInvalidValue;
+  return a.mode == kFemDamage ? launch_fem_m<kFemDamage>(a, R, blocks, st)
+                              : launch_fem_m<kFemConst>(a, R, blocks, st);
 }
 
 hipError_t launch_fem_setup(const FemCallDev* call, int64_t field, double dx, double dy, hipStream_t st) {
@@ -301,6 +286,14 @@ hipError_t launch_fem_pack(const FemCallDev* call, int64_t field, int nx, int ny
   const int64_t n = (int64_t)nx * ny;
   const int64_t blocks = std::min<int64_t>((n + kBlock - 1) / kBlock, 65536);
   hipLaunchKernelGGL(fem_pack_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, st, call, field, nx, ny, pitch);
+  return hipGetLastError();
+}
+
+hipError_t launch_fem_loads(const FemCallDev* call, const SrcEv* loads, int n, int64_t field, int pitch,
+                            hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fem_loads_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, call, loads, n,
+                     field, pitch);
   return hipGetLastError();
 }
 

@@ -176,17 +176,20 @@ struct FusedBlockArgs {
 // boundary ring of nodes is held at zero, so it and the ghost cells are the
 // zero padding of the interior unknowns.
 // Workspace: ux slots 0, 1; uy slots 2, 3 (u^m in slot m & 1); the packed
-// damage factor field in slot 4; the 4 x 4 edge-flux matrix at 5 fields.
+// damage factor field in slot 4; the nodal load weights of the x and y
+// components in slots 5, 6; the 4 x 4 edge-flux matrix at 7 fields.
 constexpr int kFemMaxRows = 8;
 constexpr int kFemExtraRows = kFemMaxRows + 3;
-constexpr int kFemSlots = 5;
+constexpr int kFemSlots = 7;
+constexpr int kFemLoadSlot = 5;
 
 enum FemMode : int { kFemConst = 0, kFemDamage = 1 };
 
-// One tile = R node rows x 64 V node columns, swept by one wavefront.
+// One tile = R node rows x 63 node columns, swept by one wavefront.
 struct FemTile {
   int i0, j0;
-  int ev_begin, ev_end;   // nodal loads of the tile, sorted by row
+  int has_loads;          // 1: a nodal load lies in the tile (the kernel reads the load fields)
+  int pad;
 };
 
 struct FemCallDev {
@@ -201,7 +204,6 @@ struct FemCallDev {
 struct FemStepArgs {
   const FemCallDev* call;
   const FemTile* tiles;
-  const SrcEv* loads;
   int64_t field;          // floats per padded field
   int64_t rec_off;        // float offset of the frame this step writes, or -1
   int n_tiles;

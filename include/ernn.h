@@ -235,8 +235,8 @@ typedef struct {
   double dt, rho;        /* time step after the CFL check (PF:280-296), density */
   int32_t rec_stride;    /* P >= 1: frame k = u^{k P}, k < ceil(nt / P) (PF:926-930) */
   float rec_scale;       /* frame factor (0.7 in PF:525-526) */
-  int32_t tile_v, tile_r; /* tile: V in {1,2,4} columns per lane, R rows, (V, R) in
-                            (1,1) (1,2) (1,4) (1,8) (2,2) (2,4) (4,2) (4,4); 0, 0 = (1, 2) */
+  int32_t tile_v, tile_r; /* tile: a wavefront sweeps R node rows x 63 node columns, one column per
+                            lane: (V, R) = (1, 1), (1, 2), (1, 4) or (1, 8); 0, 0 = (1, 1) */
   int32_t use_graph;     /* 1: replay the run as a captured hipGraph */
   const ernn_source_t* loads; /* host array: nodal loads w at (i, j) on component comp, the
                             Gaussian of PF:298-301 at the nodes (PF:322); f_ext = w(t_n) w / rho * M

@@ -35,7 +35,8 @@ def _fem_desc(loads=((5, 5, 1.0, 1),), **kw):
     ({"rec_stride": 0}, "rec_stride"),
     ({"rec_scale": float("inf")}, "rec_scale"),
     ({"tile_v": 3, "tile_r": 2}, "tile (V, R)"),
-    ({"tile_v": 4, "tile_r": 8}, "tile (V, R)"),
+    ({"tile_v": 2, "tile_r": 2}, "tile (V, R)"),
+    ({"tile_v": 1, "tile_r": 3}, "tile (V, R)"),
     ({"n_loads": -1}, "load array"),
 ])
 def test_fem_plan_create_rejects_bad_descriptors(kw, msg):
