@@ -394,8 +394,11 @@ def fem_main(args):
         eng.set_tiling(*(int(s) for s in args.tiles.split(",")))
     elif args.autotune:
         tune = eng.tune(C32, w32, Dt, steps=min(200, nt - 2))
+    if args.blocking not in ("auto", "0"):   # fixed time-blocked shape "RW,KS"
+        eng.set_blocking(*(int(s) for s in args.blocking.split(",")))
     if tune:
-        log("fem tiles: " + ", ".join(f"{k}: {v / min(200, nt - 2) * 1e3:.2f} us" for k, v in sorted(tune.items())))
+        log("fem shapes (us per step): " + ", ".join(f"{k}: {v / min(200, nt - 2) * 1e3:.2f}"
+                                                    for k, v in sorted(tune.items())))
     out = (torch.empty((eng.n_frames, nx, nx), dtype=torch.float32, device=dev),
            torch.empty((eng.n_frames, nx, nx), dtype=torch.float32, device=dev))
     for _ in range(args.warmup):
@@ -410,26 +413,36 @@ def fem_main(args):
     node_steps = nodes * (nt - 2)
     value = node_steps / run_s / 1e9
     step_ms = eng.time_steps(nt - 2, C32, w32, Dt, out=out) / (nt - 2)
-    # algorithmic bytes per node and step: read u^n, u^{n-1} (2 components), write u^{n+1}
-    # (+ the damage factor), and the frame written every `stride` steps
-    bpn = 24 + (4 if D is not None else 0) + 8.0 / stride
-    achieved = nodes * bpn / (step_ms * 1e-3) / 1e9
+    RW, KS = eng.blocking
+    if KS > 1:
+        # time-blocked launch of KS steps: reads u^{n0-1}, u^{n0} (+ the damage factor) and writes its last
+        # two states (2 components each), plus the frames of its steps
+        kname = "fem_block_kernel"
+        bpl = 32 + (4 if D is not None else 0) + 8.0 * KS / stride
+        launches = -(-(nt - 2) // KS)
+        launch_ms = step_ms * (nt - 2) / launches
+    else:
+        # one step: read u^n, u^{n-1} (2 components), write u^{n+1} (+ the damage factor), frame every `stride`
+        kname = "fem_step_kernel"
+        bpl = 24 + (4 if D is not None else 0) + 8.0 / stride
+        launches, launch_ms = nt - 2, step_ms
+    achieved = nodes * bpl / (launch_ms * 1e-3) / 1e9
     traffic = valu = None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tpath):
         wname = args.workload if nx == 301 else f"{args.workload}-{nx}"
-        entry = json.load(open(tpath)).get(f"{wname}/fem_step_kernel")
-        if entry:
+        entry = json.load(open(tpath)).get(f"{wname}/{kname}")
+        if entry and entry.get("shape") in (None, [RW, KS] if KS > 1 else list(eng.tile)):
             traffic = entry["bytes"]
             if entry.get("valu_insts"):
-                rate = entry["valu_insts"] / (step_ms * 1e-3)
+                rate = entry["valu_insts"] / (launch_ms * 1e-3)
                 valu = {"insts_per_launch": entry["valu_insts"], "issue_rate": round(rate / 1e9, 1),
                         "peak": VALU_PEAK_WIPS / 1e9, "unit": "G wave-instructions/s",
                         "frac": round(rate / VALU_PEAK_WIPS, 4)}
-    roofline = {"bound": "hbm", "kernel": "fem_step_kernel", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "valu": valu,
-                "bytes_per_node": bpn, "nodes_per_launch": nodes, "avg_launch_us": round(step_ms * 1e3, 3),
-                "launches_per_run": nt - 2}
+                "bytes_per_node": bpl, "nodes_per_launch": nodes, "avg_launch_us": round(launch_ms * 1e3, 3),
+                "launches_per_run": launches, "us_per_step": round(step_ms * 1e3, 3)}
     log(f"fem run {run_s * 1e3:.2f} ms ({value:.2f} Gnode-steps/s), step kernel {step_ms * 1e3:.2f} us")
     baseline = None
     if args.baseline:
@@ -454,7 +467,7 @@ def fem_main(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(run_s * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload if nx == 301 else f"{args.workload}-{nx}", "grid": f"{nx}x{nx}",
-                   "nt": nt, "frames_every": stride,
+                   "nt": nt, "frames_every": stride, "blocking": list(eng.blocking),
                    "damage": D is not None, "tile": list(eng.tile), "graph": bool(args.graph),
                    "parallelism": "none"},
         "s_per_run": round(run_s, 5),

@@ -24,6 +24,8 @@ hipError_t launch_fem_pack(const FemCallDev* call, int64_t field, int nx, int ny
 hipError_t launch_fem_zero(const FemCallDev* call, int which, int64_t n, hipStream_t st);
 hipError_t launch_fem_loads(const FemCallDev* call, const SrcEv* loads, int n, int64_t field, int pitch,
                             hipStream_t st);
+bool fem_block_shape_ok(int RW, int KS);
+hipError_t launch_fem_block(const FemBlockArgs& a, int grid, hipStream_t st);
 hipError_t launch_fem_set_call(FemCallDev* dst, const FemCallDev& v, hipStream_t st);
 }  // namespace ernn
 
@@ -60,7 +62,10 @@ struct FemPlan {
   int n_tiles = 0, blocks = 0;
   float dt2 = 0, rinv = 0, m_even = 0, m_odd = 0;
   int use_graph = 0;
+  int RW = 0, KS = 0;                 // time-blocked shape (KS = 0: one step per launch)
+  int n_blocks = 0, blk_grid = 0;
   std::vector<ernn_source_t> loads;   // interior loads
+  FemBlock* d_blocks = nullptr;
   FemTile* d_tiles = nullptr;
   SrcEv* d_loads = nullptr;
   FemCallDev* d_call = nullptr;
@@ -73,8 +78,34 @@ int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 void free_tables(FemPlan* p) {
   if (p->d_tiles) (void)hipFree(p->d_tiles);
   if (p->d_loads) (void)hipFree(p->d_loads);
+  if (p->d_blocks) (void)hipFree(p->d_blocks);
   p->d_tiles = nullptr;
   p->d_loads = nullptr;
+  p->d_blocks = nullptr;
+}
+
+// Time-blocked output tiles of (4 RW - 2 KS) rows x (64 - 2 KS) columns over
+// the interior, in row-band order; the grid maps block b to logical block
+// (b % 8) x blocks_per_xcd + b / 8 so that each XCD sweeps a band of rows.
+// A block is flagged when a load lies in its region (output tile + KS halo).
+int build_blocks(FemPlan* p) {
+  const int orows = kWavesPerBlock * p->RW - 2 * p->KS, ocols = kWave - 2 * p->KS;
+  const int br = (p->nx - 2 + orows - 1) / orows, bc = (p->ny - 2 + ocols - 1) / ocols;
+  std::vector<FemBlock> blocks;
+  for (int a = 0; a < br; ++a)
+    for (int b = 0; b < bc; ++b) {
+      FemBlock bl{1 + a * orows, 1 + b * ocols, 0, 0};
+      for (const auto& l : p->loads)
+        if (l.i >= bl.oi0 - p->KS && l.i < bl.oi0 - p->KS + kWavesPerBlock * p->RW && l.j >= bl.oj0 - p->KS &&
+            l.j < bl.oj0 - p->KS + kWave)
+          bl.has_loads = 1;
+      blocks.push_back(bl);
+    }
+  p->n_blocks = (int)blocks.size();
+  p->blk_grid = 8 * ((p->n_blocks + 7) / 8);
+  FEM_HIP(hipMalloc(&p->d_blocks, sizeof(FemBlock) * blocks.size()));
+  FEM_HIP(hipMemcpy(p->d_blocks, blocks.data(), sizeof(FemBlock) * blocks.size(), hipMemcpyHostToDevice));
+  return ERNN_OK;
 }
 
 void destroy(FemPlan* p) {
@@ -109,6 +140,7 @@ int build_tables(FemPlan* p) {
     for (int b = 0; b < tc; ++b) tiles[a * tc + b] = FemTile{1 + a * p->R, 1 + b * cols, has[a * tc + b], 0};
   FEM_HIP(hipMalloc(&p->d_tiles, sizeof(FemTile) * tiles.size()));
   FEM_HIP(hipMemcpy(p->d_tiles, tiles.data(), sizeof(FemTile) * tiles.size(), hipMemcpyHostToDevice));
+  if (p->KS > 1) FEM_TRY(build_blocks(p));
   if (!p->loads.empty()) {
     std::vector<SrcEv> flat;
     for (const auto& l : p->loads) flat.push_back(SrcEv{l.i, l.j, l.w, l.comp});
@@ -171,7 +203,44 @@ int enqueue_setup(const FemPlan* p, hipStream_t st) {
   return ERNN_OK;
 }
 
+FemBlockArgs block_args(const FemPlan* p) {
+  FemBlockArgs a{};
+  a.call = p->d_call;
+  a.blocks = p->d_blocks;
+  a.field = p->field;
+  a.frame = p->frame;
+  a.n_blocks = p->n_blocks;
+  a.blocks_per_xcd = p->blk_grid / 8;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.pitch = p->pitch;
+  a.mode = p->mode;
+  a.stride = p->stride;
+  a.dt2 = p->dt2;
+  a.rinv = p->rinv;
+  a.m_even = p->m_even;
+  a.m_odd = p->m_odd;
+  a.scale = p->scale;
+  a.rw = p->RW;
+  a.ksmax = p->KS;
+  return a;
+}
+
+// Steps n = 1 .. n_end - 1.  Time-blocked: launch L runs steps n0 .. n0 + ks - 1
+// from pair L & 1 into pair (L + 1) & 1 (u^0 = u^1 = 0 in pair 0).
 int enqueue_steps(const FemPlan* p, int n_end, hipStream_t st) {
+  if (p->KS > 1) {
+    FemBlockArgs a = block_args(p);
+    for (int n0 = 1, L = 0; n0 < n_end; ++L) {
+      a.n0 = n0;
+      a.ks = std::min(p->KS, n_end - n0);
+      a.src_pair = L & 1;
+      a.dst_pair = (L + 1) & 1;
+      FEM_HIP(launch_fem_block(a, p->blk_grid, st));
+      n0 += a.ks;
+    }
+    return ERNN_OK;
+  }
   for (int n = 1; n < n_end; ++n) FEM_HIP(launch_fem_step(step_args(p, n), p->V, p->R, p->blocks, st));
   return ERNN_OK;
 }
@@ -219,6 +288,9 @@ int ernn_fem_plan_create(const ernn_fem_desc_t* d, void** plan_out) {
     R = 1;
   }
   if (!fem_shape_ok(V, R)) return fail(ERNN_E_INVALID, "tile (V, R) must be (1, 1), (1, 2), (1, 4) or (1, 8)");
+  const int KS = d->blk_ks > 1 ? d->blk_ks : 0, RW = KS ? d->blk_rw : 0;
+  if (KS && !fem_block_shape_ok(RW, KS))
+    return fail(ERNN_E_INVALID, "time-blocked shape (RW, KS) must be (4, 2), (4, 4), (8, 2), (8, 4) or (8, 8)");
   const int64_t pitch = kLeftPad + round_up(d->ny, kColAlign);
   const int64_t field = (int64_t)(d->nx + kFemExtraRows) * pitch;
   if (field >= (int64_t(1) << 30)) return fail(ERNN_E_CAPACITY, "grid too large (a padded field must hold < 2^30 floats)");
@@ -235,6 +307,8 @@ int ernn_fem_plan_create(const ernn_fem_desc_t* d, void** plan_out) {
   p->scale = d->rec_scale;
   p->V = V;
   p->R = R;
+  p->RW = RW;
+  p->KS = KS;
   p->pitch = (int)pitch;
   p->field = field;
   p->ws_floats = (int64_t)kFemSlots * field + 64;
@@ -309,6 +383,9 @@ int ernn_fem_plan_info(const void* plan, ernn_fem_info_t* info) {
   info->blocks = p->blocks;
   info->n_loads = (int32_t)p->loads.size();
   info->use_graph = p->use_graph;
+  info->blk_rw = p->RW;
+  info->blk_ks = p->KS;
+  info->blk_blocks = p->n_blocks;
   return ERNN_OK;
 }
 
@@ -338,6 +415,22 @@ int ernn_fem_set_tiling(void* plan, int v, int r) {
   }
   p->V = v;
   p->R = r;
+  return build_tables(p);
+}
+
+int ernn_fem_set_blocking(void* plan, int rw, int ks) {
+  if (!plan) return fail(ERNN_E_INVALID, "null plan");
+  FemPlan* p = static_cast<FemPlan*>(plan);
+  if (ks > 1 && !fem_block_shape_ok(rw, ks))
+    return fail(ERNN_E_INVALID, "time-blocked shape (RW, KS) must be (4, 2), (4, 4), (8, 2), (8, 4) or (8, 8)");
+  FEM_HIP(hipSetDevice(p->device));
+  FEM_HIP(hipDeviceSynchronize());
+  if (p->graph) {
+    (void)hipGraphExecDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  p->KS = ks > 1 ? ks : 0;
+  p->RW = p->KS ? rw : 0;
   return build_tables(p);
 }
 

@@ -175,13 +175,18 @@ struct FusedBlockArgs {
 // for a tile of up to kFemMaxRows rows past the last interior row).  The
 // boundary ring of nodes is held at zero, so it and the ghost cells are the
 // zero padding of the interior unknowns.
-// Workspace: ux slots 0, 1; uy slots 2, 3 (u^m in slot m & 1); the packed
-// damage factor field in slot 4; the nodal load weights of the x and y
-// components in slots 5, 6; the 4 x 4 edge-flux matrix at 7 fields.
+// Workspace: ux slots 0-3, uy slots 4-7 as two pairs each (pair p = slots
+// 2p, 2p + 1 of the component).  One step per launch keeps u^m in slot m & 1
+// of pair 0; the time-blocked launches read (u^{n0-1}, u^{n0}) from pair
+// L & 1 (older first) and write their last two states to the other pair.
+// Slot 8: the packed damage factor; slots 9, 10: the nodal load weights of
+// the x and y components; the 4 x 4 edge-flux matrix at 11 fields.
 constexpr int kFemMaxRows = 8;
 constexpr int kFemExtraRows = kFemMaxRows + 3;
-constexpr int kFemSlots = 7;
-constexpr int kFemLoadSlot = 5;
+constexpr int kFemUySlot = 4;
+constexpr int kFemDSlot = 8;
+constexpr int kFemLoadSlot = 9;
+constexpr int kFemSlots = 11;
 
 enum FemMode : int { kFemConst = 0, kFemDamage = 1 };
 
@@ -199,6 +204,33 @@ struct FemCallDev {
   const float* wav;       // time factors, nt
   float* frames_x;        // [n_frames][nx][ny]
   float* frames_y;
+};
+
+// Time-blocked FEM launch: a block of kWavesPerBlock wavefronts x RW node
+// rows x 64 node columns advances up to KS steps from registers; the output
+// tile is the block region shrunk by KS nodes on every side.
+constexpr int kFemMaxBlockSteps = 8;
+
+struct FemBlock {
+  int oi0, oj0;           // output tile origin (grid node)
+  int has_loads;          // 1: a nodal load lies in the block region
+  int pad;
+};
+
+struct FemBlockArgs {
+  const FemCallDev* call;
+  const FemBlock* blocks;
+  int64_t field;
+  int64_t frame;          // nx * ny
+  int n_blocks, blocks_per_xcd;
+  int nx, ny, pitch;
+  int mode;
+  int n0;                 // first step: reads u^{n0-1}, u^{n0}, produces u^{n0+1} .. u^{n0+ks}
+  int ks;                 // steps in this launch (<= KS)
+  int src_pair, dst_pair;
+  int stride;             // frame k = u^{k stride}
+  float dt2, rinv, m_even, m_odd, scale;
+  int rw, ksmax;          // compiled shape
 };
 
 struct FemStepArgs {

@@ -73,14 +73,15 @@ class FemDesc(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nt", C.c_int32), ("mode", C.c_int32),
                 ("dx", C.c_double), ("dy", C.c_double), ("dt", C.c_double), ("rho", C.c_double),
                 ("rec_stride", C.c_int32), ("rec_scale", C.c_float), ("tile_v", C.c_int32),
-                ("tile_r", C.c_int32), ("use_graph", C.c_int32), ("loads", C.POINTER(Source)),
-                ("n_loads", C.c_int64)]
+                ("tile_r", C.c_int32), ("use_graph", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
+                ("loads", C.POINTER(Source)), ("n_loads", C.c_int64)]
 
 
 class FemInfo(C.Structure):
     _fields_ = [("workspace_bytes", C.c_int64), ("frame_floats", C.c_int64), ("n_frames", C.c_int32),
                 ("tile_v", C.c_int32), ("tile_r", C.c_int32), ("n_tiles", C.c_int32), ("blocks", C.c_int32),
-                ("n_loads", C.c_int32), ("use_graph", C.c_int32), ("pad", C.c_int32)]
+                ("n_loads", C.c_int32), ("use_graph", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
+                ("blk_blocks", C.c_int32)]
 
 
 class FemCall(C.Structure):
@@ -112,6 +113,7 @@ EXPORTS = {
     "ernn_fem_plan_info": ([C.c_void_p, C.POINTER(FemInfo)], C.c_int),
     "ernn_fem_run": ([C.c_void_p, C.POINTER(FemCall), C.c_void_p], C.c_int),
     "ernn_fem_set_tiling": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "ernn_fem_set_blocking": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "ernn_fem_time_steps": ([C.c_void_p, C.c_int, C.POINTER(FemCall), C.c_void_p, C.POINTER(C.c_float)],
                             C.c_int),
     "ernn_abi_version": ([], C.c_int),

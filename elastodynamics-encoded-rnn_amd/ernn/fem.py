@@ -31,6 +31,7 @@ from . import _lib
 from .data import save_reference_data
 
 FEM_SHAPES = ((1, 1), (1, 2), (1, 4), (1, 8))   # (V, R): R node rows x 63 columns per wavefront
+FEM_BLOCK_SHAPES = ((4, 2), (4, 4), (8, 2), (8, 4), (8, 8))   # (RW, KS): KS steps per launch
 STIFFNESS_NAMES = ("C11", "C22", "C12", "C16", "C26", "C66")
 DEFAULT_DAMAGE = [{"type": "circle", "center": (0.1, 0.1), "size": 0.02, "reduction_factor": 0.5}]   # DF:856-866
 
@@ -143,7 +144,7 @@ class FEMEngine:
     def __init__(self, nx: int, ny: int, dx: float, dy: float, nt: int, dt: float, rho: float,
                  loads: Sequence[Tuple[int, int, float, int]], damage: bool = False, stride: int = 1,
                  scale: float = 0.7, tile: Tuple[int, int] = (0, 0), use_graph: bool = False,
-                 device: Optional[torch.device] = None):
+                 device: Optional[torch.device] = None, blocking: Tuple[int, int] = (0, 0)):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         if self.device.type != "cuda":
             raise RuntimeError("FEMEngine runs on a ROCm device only (there is no CPU path)")
@@ -155,7 +156,8 @@ class FEMEngine:
         d = _lib.FemDesc(nx=self.nx, ny=self.ny, nt=self.nt, mode=_lib.FEM_DAMAGE if damage else _lib.FEM_CONST,
                          dx=float(dx), dy=float(dy), dt=float(dt), rho=float(rho), rec_stride=self.stride,
                          rec_scale=float(scale), tile_v=int(tile[0]), tile_r=int(tile[1]),
-                         use_graph=int(bool(use_graph)), loads=arr, n_loads=len(loads))
+                         use_graph=int(bool(use_graph)), blk_rw=int(blocking[0]), blk_ks=int(blocking[1]),
+                         loads=arr, n_loads=len(loads))
         lib = _lib.lib()
         self._plan = C.c_void_p()
         with torch.cuda.device(self.device):
@@ -187,6 +189,11 @@ class FEMEngine:
     @property
     def tile(self) -> Tuple[int, int]:
         return int(self.info.tile_v), int(self.info.tile_r)
+
+    @property
+    def blocking(self) -> Tuple[int, int]:
+        """(RW, KS) of the time-blocked launches; (0, 0) = one step per launch."""
+        return int(self.info.blk_rw), int(self.info.blk_ks)
 
     def _inputs(self, C6, wavelet, damage_field):
         f32 = dict(dtype=torch.float32, device=self.device)
@@ -244,19 +251,34 @@ class FEMEngine:
             _lib.check(_lib.lib().ernn_fem_set_tiling(self._plan, int(v), int(r)), "ernn_fem_set_tiling")
         self._refresh()
 
+    def set_blocking(self, rw: int, ks: int) -> None:
+        """KS steps per launch on blocks of 4 x RW x 64 nodes (ks <= 1: off)."""
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().ernn_fem_set_blocking(self._plan, int(rw), int(ks)), "ernn_fem_set_blocking")
+        self._refresh()
+
     def tune(self, C6, wavelet, damage_field=None, steps: int = 100, rounds: int = 2,
-             shapes: Sequence[Tuple[int, int]] = FEM_SHAPES) -> Dict[Tuple[int, int], float]:
-        """Time each tile shape over `steps` steps (best of `rounds`), keep the
-        fastest; returns {shape: ms}."""
+             shapes: Sequence[Tuple[int, int]] = FEM_SHAPES,
+             block_shapes: Sequence[Tuple[int, int]] = FEM_BLOCK_SHAPES) -> Dict[Tuple, float]:
+        """Time each one-step tile shape and each time-blocked shape over
+        `steps` steps (best of `rounds`) and keep the fastest; returns
+        {("tile", V, R) or ("block", RW, KS): ms}."""
         shape = (self.n_frames, self.nx, self.ny)
         out = (torch.empty(shape, dtype=torch.float32, device=self.device),
                torch.empty(shape, dtype=torch.float32, device=self.device))
         times = {}
+        self.set_blocking(0, 0)
         for v, r in shapes:
             self.set_tiling(v, r)
-            times[(v, r)] = min(self.time_steps(steps, C6, wavelet, damage_field, out) for _ in range(rounds))
+            times[("tile", v, r)] = min(self.time_steps(steps, C6, wavelet, damage_field, out) for _ in range(rounds))
+        best_tile = min((k for k in times if k[0] == "tile"), key=times.get)
+        self.set_tiling(*best_tile[1:])
+        for rw, ks in block_shapes:
+            self.set_blocking(rw, ks)
+            times[("block", rw, ks)] = min(self.time_steps(steps, C6, wavelet, damage_field, out)
+                                           for _ in range(rounds))
         best = min(times, key=times.get)
-        self.set_tiling(*best)
+        self.set_blocking(*(best[1:] if best[0] == "block" else (0, 0)))
         return times
 
 

@@ -238,6 +238,10 @@ typedef struct {
   int32_t tile_v, tile_r; /* tile: a wavefront sweeps R node rows x 63 node columns, one column per
                             lane: (V, R) = (1, 1), (1, 2), (1, 4) or (1, 8); 0, 0 = (1, 1) */
   int32_t use_graph;     /* 1: replay the run as a captured hipGraph */
+  int32_t blk_rw, blk_ks; /* time-blocked: blk_ks steps per launch on blocks of 4 wavefronts x blk_rw
+                            node rows x 64 columns, (RW, KS) in (4, 2) (4, 4) (8, 2) (8, 4) (8, 8);
+                            blk_ks <= 1: one step per launch with the tile above.  Same results bit
+                            for bit. */
   const ernn_source_t* loads; /* host array: nodal loads w at (i, j) on component comp, the
                             Gaussian of PF:298-301 at the nodes (PF:322); f_ext = w(t_n) w / rho * M
                             (PF:482-484).  Loads on boundary nodes have no effect and are dropped */
@@ -250,9 +254,10 @@ typedef struct {
   int32_t n_frames;
   int32_t tile_v, tile_r;
   int32_t n_tiles, blocks;
-  int32_t n_loads;         /* loads kept (interior nodes) */
+  int32_t n_loads;         /* loads kept (interior nodes, one per node and component) */
   int32_t use_graph;
-  int32_t pad;
+  int32_t blk_rw, blk_ks;  /* time-blocked shape (blk_ks = 0: off) */
+  int32_t blk_blocks;      /* blocks per time-blocked launch */
 } ernn_fem_info_t;
 
 /* Per-run device pointers. */
@@ -281,6 +286,11 @@ int ernn_fem_run(void* plan, const ernn_fem_call_t* call, void* stream);
 /* Change the tile (V, R); synchronizes the device and drops the captured
  * graph.  Used by the autotuner. */
 int ernn_fem_set_tiling(void* plan, int v, int r);
+
+/* Switch to ks steps per launch on blocks of the given shape (ks <= 1: one
+ * step per launch); same limits as ernn_fem_desc_t.blk_*.  Synchronizes the
+ * device and drops the captured graph.  Used by the autotuner. */
+int ernn_fem_set_blocking(void* plan, int rw, int ks);
 
 /* Time steps n = 1 .. min(steps, nt - 2) after the run's set-up, with
  * hipEvents on `stream`; synchronizes.  ms_out receives the milliseconds. */

@@ -18,7 +18,8 @@ def _fem_desc(loads=((5, 5, 1.0, 1),), **kw):
     for k, (i, j, w, c) in enumerate(loads):
         L[k] = _lib.Source(i, j, w, c)
     d = _lib.FemDesc(nx=17, ny=13, nt=20, mode=0, dx=1e-3, dy=1.2e-3, dt=1e-8, rho=1610.0, rec_stride=5,
-                     rec_scale=0.7, tile_v=0, tile_r=0, use_graph=0, loads=L, n_loads=len(loads))
+                     rec_scale=0.7, tile_v=0, tile_r=0, use_graph=0, blk_rw=0, blk_ks=0, loads=L,
+                     n_loads=len(loads))
     for f, v in kw.items():
         setattr(d, f, v)
     return d, L
@@ -38,6 +39,8 @@ def _fem_desc(loads=((5, 5, 1.0, 1),), **kw):
     ({"tile_v": 2, "tile_r": 2}, "tile (V, R)"),
     ({"tile_v": 1, "tile_r": 3}, "tile (V, R)"),
     ({"n_loads": -1}, "load array"),
+    ({"blk_rw": 4, "blk_ks": 3}, "time-blocked shape"),
+    ({"blk_rw": 6, "blk_ks": 4}, "time-blocked shape"),
 ])
 def test_fem_plan_create_rejects_bad_descriptors(kw, msg):
     d, keep = _fem_desc(**kw)

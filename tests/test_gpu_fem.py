@@ -45,11 +45,13 @@ def small_case(nx=37, ny=29, Lx=0.04, Ly=0.03, nt=301, a=4e5, f0=4e5, amp=1e10, 
                 C32=np.asarray(C, np.float32), X=X, Y=Y)
 
 
-def gpu_run(c, stride=1, tile=(0, 0), D=None, use_graph=False, runs=1):
-    print(f"[fem] nx={c['nx']} ny={c['ny']} nt={c['nt']} stride={stride} tile={tile} damage={D is not None} "
-          f"graph={use_graph}", flush=True)
+def gpu_run(c, stride=1, tile=(0, 0), D=None, use_graph=False, runs=1, blocking=(0, 0)):
+    print(f"[fem] nx={c['nx']} ny={c['ny']} nt={c['nt']} stride={stride} tile={tile} blocking={blocking} "
+          f"damage={D is not None} graph={use_graph}", flush=True)
     eng = fem.FEMEngine(c["nx"], c["ny"], c["dx"], c["dy"], c["nt"], c["dt"], RHO, fem.nodal_loads(c["g"]),
-                        damage=D is not None, stride=stride, tile=tile, use_graph=use_graph, device=DEV)
+                        damage=D is not None, stride=stride, tile=tile, use_graph=use_graph, device=DEV,
+                        blocking=blocking)
+    assert eng.blocking == (tuple(blocking) if blocking[1] > 1 else (0, 0))
     outs = []
     for _ in range(runs):
         fx, fy = eng.run(c["C32"], c["w"], D)
@@ -84,6 +86,33 @@ def test_fem_small_vs_fp64_oracle_all_tiles(nx, ny, Ly):
         else:   # a node's arithmetic does not depend on the tiling
             np.testing.assert_array_equal(gx, ref[0])
             np.testing.assert_array_equal(gy, ref[1])
+
+
+@pytest.mark.parametrize("nx,ny,Ly", [(37, 29, 0.03), (23, 131, 0.13), (70, 61, 0.05)])
+def test_fem_time_blocked_equals_one_step(nx, ny, Ly):
+    """KS steps per launch give the one-step kernel's frames bit for bit
+    (83 samples = 81 steps: the last launch of every shape is partial)."""
+    c = small_case(nx=nx, ny=ny, Ly=Ly, nt=83)
+    ref = gpu_run(c, stride=3, tile=(1, 1))
+    assert np.abs(ref[1]).max() > 0
+    for blk in fem.FEM_BLOCK_SHAPES:
+        got = gpu_run(c, stride=3, blocking=blk)
+        np.testing.assert_array_equal(got[0], ref[0], err_msg=f"blocking {blk}")
+        np.testing.assert_array_equal(got[1], ref[1], err_msg=f"blocking {blk}")
+
+
+def test_fem_time_blocked_damage_and_graph():
+    c = small_case(nx=41, ny=70, Ly=0.05, nt=64)
+    mask, _ = F.damage_regions(c["X"], c["Y"], [{"type": "circle", "center": (0.006, 0.004), "size": 0.006}])
+    D = torch.tensor(np.where(mask, 0.5, 1.0).astype(np.float32))
+    ref = gpu_run(c, stride=2, tile=(1, 2), D=D)
+    for blk in ((4, 4), (8, 8)):
+        got = gpu_run(c, stride=2, D=D, blocking=blk)
+        np.testing.assert_array_equal(got[0], ref[0])
+        np.testing.assert_array_equal(got[1], ref[1])
+    for gx, gy in gpu_run(c, stride=2, D=D, blocking=(8, 4), use_graph=True, runs=3):
+        np.testing.assert_array_equal(gx, ref[0])
+        np.testing.assert_array_equal(gy, ref[1])
 
 
 def test_fem_frames_boundary_and_first_steps():
@@ -146,6 +175,9 @@ def test_fem_reference_size_vs_fp64_oracle_and_mirror_symmetry():
     e = rel_l2(np.stack([gx, gy]), np.stack([ox, oy]))
     print(f"[fem] 301^2 x 2001 rel L2 vs FP64 oracle {e:.3e}", flush=True)
     assert e < 2e-5
+    bx, by = gpu_run(c, stride=5, blocking=(4, 4))   # time-blocked: the same frames bit for bit
+    np.testing.assert_array_equal(bx, gx)
+    np.testing.assert_array_equal(by, gy)
     c["C32"] = np.asarray((130.74e9, 11.50e9, 10.49e9, 0.0, 0.0, 14.77e9), np.float32)
     sx, sy = gpu_run(c, stride=5, tile=(1, 4))
     scale = np.abs(sy).max()
