@@ -10,7 +10,7 @@
 //
 // The reference builds TV and its gradient from ~20 tensor ops per field and
 // steps Adam with ~10 more.  Here one call runs two passes over the fields:
-//   tv_grad_kernel  g_theta += (lambda s(i,j)) m(i,j) e(i,j), the TV
+//   tv_grad_kernel* g_theta += (lambda s(i,j)) m(i,j) e(i,j), the TV
 //                   subgradient s (sign(0) = 0 as abs' backward) chained
 //                   through the clamp mask m (inclusive bounds) and
 //                   d exp = e; per-block FP64 TV partial sums
@@ -89,15 +89,11 @@ __device__ __forceinline__ void block_partial(double tv, double* partial, int f,
   }
 }
 
-// V consecutive cells of one row per thread (V = 4: 16-byte loads and
-// stores, rows of ny % 4 == 0 floats on 16-byte aligned fields; V = 1 any).
-// Block b runs on XCD b % 8 (round-robin dispatch); it takes logical block
+// One-cell variant (any alignment, any ny): thread = one cell.  Block b runs
+// on XCD b % 8 (round-robin dispatch); it takes logical block
 // (b % 8) x blocks_per_xcd + b / 8, so each XCD sweeps one contiguous band
-// of rows and the rows above and below a block are in its own L2 (with a
-// plain order every neighbour row sat on another XCD: measured 2x the
-// algorithmic fetch, 101 us per 6 x 2048^2 update).
-template <int V>
-__global__ void __launch_bounds__(kTvThreads) tv_grad_kernel(TvArgs a) {
+// of rows and the rows above and below a block are in its own L2.
+__global__ void __launch_bounds__(kTvThreads) tv_grad_kernel1(TvArgs a) {
   const int lb = (int)(blockIdx.x & 7) * a.blocks_per_xcd + (int)(blockIdx.x >> 3);
   if (lb >= a.blocks_per_field) return;   // whole block: the barrier below is not reached by anyone
   const int f = blockIdx.y;
@@ -105,51 +101,105 @@ __global__ void __launch_bounds__(kTvThreads) tv_grad_kernel(TvArgs a) {
   float* __restrict__ g = a.grad[f];
   const float lo = a.lo[f], hi = a.hi[f];
   const int lp = a.log_params;
-  const int groups = a.ny / V;
+  const int64_t k = (int64_t)lb * kTvThreads + threadIdx.x;
+  double tv = 0.0;
+  if (k < (int64_t)a.nx * a.ny) {
+    const int i = (int)(k / a.ny), j = (int)(k % a.ny);
+    float chain, dummy;
+    const float c = field_value(th[k], lp, lo, hi, &chain);
+    float s = 0.f;
+    if (i > 0) s += sgn(c - field_value(th[k - a.ny], lp, lo, hi, &dummy));
+    if (i + 1 < a.nx) {
+      const float d = field_value(th[k + a.ny], lp, lo, hi, &dummy) - c;
+      s -= sgn(d);
+      tv += fabs((double)d);
+    }
+    if (j > 0) s += sgn(c - field_value(th[k - 1], lp, lo, hi, &dummy));
+    if (j + 1 < a.ny) {
+      const float d = field_value(th[k + 1], lp, lo, hi, &dummy) - c;
+      s -= sgn(d);
+      tv += fabs((double)d);
+    }
+    g[k] = g[k] + (a.lambda * s) * chain;
+  }
+  block_partial(tv, a.partial, f, a.blocks_per_field, lb);
+}
+
+// Strip variant (16-byte aligned fields, ny % 4 == 0): a thread sweeps a
+// strip of kTvStrip rows x 4 columns top to bottom, so each row's fields are
+// formed once (exp of the cell, of its two side neighbours, reused as the
+// next row's upper neighbours): 1.75 exps per cell instead of 3.5 when every
+// neighbour is formed on its own, which made the one-row version VALU-bound
+// (90 % of the issue peak at 6 x 2048^2).  Same arithmetic per cell as above.
+constexpr int kTvStrip = 8;
+
+__global__ void __launch_bounds__(kTvThreads) tv_grad_kernel4(TvArgs a) {
+  const int lb = (int)(blockIdx.x & 7) * a.blocks_per_xcd + (int)(blockIdx.x >> 3);
+  if (lb >= a.blocks_per_field) return;
+  const int f = blockIdx.y;
+  const float* __restrict__ th = a.theta[f];
+  float* __restrict__ g = a.grad[f];
+  const float lo = a.lo[f], hi = a.hi[f];
+  const int lp = a.log_params;
+  const int groups = a.ny / 4;
+  const int strips = (a.nx + kTvStrip - 1) / kTvStrip;
   const int64_t t = (int64_t)lb * kTvThreads + threadIdx.x;
   double tv = 0.0;
-  if (t < (int64_t)a.nx * groups) {
-    const int i = (int)(t / groups), j0 = (int)(t % groups) * V;
-    const int64_t k = (int64_t)i * a.ny + j0;
-    float raw[V], up[V], dn[V], gv[V];
-    if constexpr (V == 4) {
-      load_cells(th + k, raw);
-      load_cells(g + k, gv);
-      if (i > 0) load_cells(th + k - a.ny, up);
-      if (i + 1 < a.nx) load_cells(th + k + a.ny, dn);
-    } else {
-      raw[0] = th[k];
-      gv[0] = g[k];
-      if (i > 0) up[0] = th[k - a.ny];
-      if (i + 1 < a.nx) dn[0] = th[k + a.ny];
+  if (t < (int64_t)strips * groups) {
+    const int i0 = (int)(t / groups) * kTvStrip, j0 = (int)(t % groups) * 4;
+    const int i1 = min(i0 + kTvStrip, a.nx);
+    float up[4], cur[4], raw[4], dummy;
+    if (i0 > 0) {
+      load_cells(th + (int64_t)(i0 - 1) * a.ny + j0, raw);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) up[e] = field_value(raw[e], lp, lo, hi, &dummy);
     }
-    float cv[V + 2], chain[V], dummy;   // cv[m]: C at column j0 - 1 + m
+    float chain[4];
+    load_cells(th + (int64_t)i0 * a.ny + j0, raw);
 #pragma unroll
-    for (int e = 0; e < V; ++e) cv[e + 1] = field_value(raw[e], lp, lo, hi, &chain[e]);
-    cv[0] = j0 > 0 ? field_value(th[k - 1], lp, lo, hi, &dummy) : 0.f;
-    cv[V + 1] = j0 + V < a.ny ? field_value(th[k + V], lp, lo, hi, &dummy) : 0.f;
+    for (int e = 0; e < 4; ++e) cur[e] = field_value(raw[e], lp, lo, hi, &chain[e]);
 #pragma unroll
-    for (int e = 0; e < V; ++e) {
-      const float c = cv[e + 1];
-      float s = 0.f;
-      if (i > 0) s += sgn(c - field_value(up[e], lp, lo, hi, &dummy));
+    for (int r = 0; r < kTvStrip; ++r) {   // unrolled: the rows' loads are independent and issue early
+      const int i = i0 + r;
+      if (i >= i1) break;
+      const int64_t k = (int64_t)i * a.ny + j0;
+      float dn[4], dchain[4];
       if (i + 1 < a.nx) {
-        const float d = field_value(dn[e], lp, lo, hi, &dummy) - c;
-        s -= sgn(d);
-        tv += fabs((double)d);
+        load_cells(th + k + a.ny, raw);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) dn[e] = field_value(raw[e], lp, lo, hi, &dchain[e]);
       }
-      if (j0 + e > 0) s += sgn(c - cv[e]);
-      if (j0 + e + 1 < a.ny) {
-        const float d = cv[e + 2] - c;
-        s -= sgn(d);
-        tv += fabs((double)d);
+      const float left = j0 > 0 ? field_value(th[k - 1], lp, lo, hi, &dummy) : 0.f;
+      const float right = j0 + 4 < a.ny ? field_value(th[k + 4], lp, lo, hi, &dummy) : 0.f;
+      float gv[4];
+      load_cells(g + k, gv);
+      float cv[6] = {left, cur[0], cur[1], cur[2], cur[3], right};   // cv[m]: column j0 - 1 + m
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float c = cv[e + 1];
+        float s = 0.f;
+        if (i > 0) s += sgn(c - up[e]);
+        if (i + 1 < a.nx) {
+          const float d = dn[e] - c;
+          s -= sgn(d);
+          tv += fabs((double)d);
+        }
+        if (j0 + e > 0) s += sgn(c - cv[e]);
+        if (j0 + e + 1 < a.ny) {
+          const float d = cv[e + 2] - c;
+          s -= sgn(d);
+          tv += fabs((double)d);
+        }
+        gv[e] = gv[e] + (a.lambda * s) * chain[e];
       }
-      gv[e] = gv[e] + (a.lambda * s) * chain[e];
-    }
-    if constexpr (V == 4)
       *reinterpret_cast<float4*>(g + k) = make_float4(gv[0], gv[1], gv[2], gv[3]);
-    else
-      g[k] = gv[0];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        up[e] = cur[e];
+        cur[e] = dn[e];
+        chain[e] = dchain[e];
+      }
+    }
   }
   block_partial(tv, a.partial, f, a.blocks_per_field, lb);
 }
@@ -278,7 +328,8 @@ int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c
     for (const void* q : {(const void*)c->theta[f], (const void*)c->grad[f], (const void*)c->m[f], (const void*)c->v[f]})
       aligned = aligned && (reinterpret_cast<uintptr_t>(q) & 15) == 0;
   const bool tv4 = aligned && d->ny % 4 == 0;
-  const int64_t nb = tv4 ? (n / 4 + kTvThreads - 1) / kTvThreads : tv_blocks(d->nx, d->ny);
+  const int64_t strips = (d->nx + kTvStrip - 1) / kTvStrip;
+  const int64_t nb = tv4 ? (strips * (d->ny / 4) + kTvThreads - 1) / kTvThreads : tv_blocks(d->nx, d->ny);
   TvArgs t{};
   for (int f = 0; f < d->n_fields; ++f) {
     t.theta[f] = c->theta[f];
@@ -295,9 +346,9 @@ int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c
   t.lambda = (float)d->lambda_tv;   // the mul's backward scales by the FP32 scalar
   const unsigned grid = 8u * (unsigned)t.blocks_per_xcd;
   if (tv4)
-    hipLaunchKernelGGL(tv_grad_kernel<4>, dim3(grid, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, t);
+    hipLaunchKernelGGL(tv_grad_kernel4, dim3(grid, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, t);
   else
-    hipLaunchKernelGGL(tv_grad_kernel<1>, dim3(grid, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, t);
+    hipLaunchKernelGGL(tv_grad_kernel1, dim3(grid, (unsigned)d->n_fields), dim3(kTvThreads), 0, st, t);
   TV_HIP(hipGetLastError());
   if (c->tv) {
     hipLaunchKernelGGL(tv_finalize_kernel, dim3((unsigned)d->n_fields), dim3(kTvThreads), 0, st,
