@@ -84,19 +84,20 @@ void free_tables(FemPlan* p) {
   p->d_blocks = nullptr;
 }
 
-// Time-blocked output tiles of (4 RW - 2 KS) rows x (64 - 2 KS) columns over
+// Time-blocked output tiles of (WB RW - 2 KS) rows x (64 - 2 KS) columns over
 // the interior, in row-band order; the grid maps block b to logical block
 // (b % 8) x blocks_per_xcd + b / 8 so that each XCD sweeps a band of rows.
 // A block is flagged when a load lies in its region (output tile + KS halo).
 int build_blocks(FemPlan* p) {
-  const int orows = kWavesPerBlock * p->RW - 2 * p->KS, ocols = kWave - 2 * p->KS;
+  const int wb = p->RW == 2 ? 8 : kWavesPerBlock;   // wavefronts per block (fem_block_waves in ernn_fem.hip)
+  const int orows = wb * p->RW - 2 * p->KS, ocols = kWave - 2 * p->KS;
   const int br = (p->nx - 2 + orows - 1) / orows, bc = (p->ny - 2 + ocols - 1) / ocols;
   std::vector<FemBlock> blocks;
   for (int a = 0; a < br; ++a)
     for (int b = 0; b < bc; ++b) {
       FemBlock bl{1 + a * orows, 1 + b * ocols, 0, 0};
       for (const auto& l : p->loads)
-        if (l.i >= bl.oi0 - p->KS && l.i < bl.oi0 - p->KS + kWavesPerBlock * p->RW && l.j >= bl.oj0 - p->KS &&
+        if (l.i >= bl.oi0 - p->KS && l.i < bl.oi0 - p->KS + wb * p->RW && l.j >= bl.oj0 - p->KS &&
             l.j < bl.oj0 - p->KS + kWave)
           bl.has_loads = 1;
       blocks.push_back(bl);
@@ -290,7 +291,7 @@ int ernn_fem_plan_create(const ernn_fem_desc_t* d, void** plan_out) {
   if (!fem_shape_ok(V, R)) return fail(ERNN_E_INVALID, "tile (V, R) must be (1, 1), (1, 2), (1, 4) or (1, 8)");
   const int KS = d->blk_ks > 1 ? d->blk_ks : 0, RW = KS ? d->blk_rw : 0;
   if (KS && !fem_block_shape_ok(RW, KS))
-    return fail(ERNN_E_INVALID, "time-blocked shape (RW, KS) must be (4, 2), (4, 4), (8, 2), (8, 4) or (8, 8)");
+    return fail(ERNN_E_INVALID, "time-blocked shape (RW, KS) must be (2, 2), (2, 4), (4, 2), (4, 4), (8, 2), (8, 4) or (8, 8)");
   const int64_t pitch = kLeftPad + round_up(d->ny, kColAlign);
   const int64_t field = (int64_t)(d->nx + kFemExtraRows) * pitch;
   if (field >= (int64_t(1) << 30)) return fail(ERNN_E_CAPACITY, "grid too large (a padded field must hold < 2^30 floats)");
@@ -422,7 +423,7 @@ int ernn_fem_set_blocking(void* plan, int rw, int ks) {
   if (!plan) return fail(ERNN_E_INVALID, "null plan");
   FemPlan* p = static_cast<FemPlan*>(plan);
   if (ks > 1 && !fem_block_shape_ok(rw, ks))
-    return fail(ERNN_E_INVALID, "time-blocked shape (RW, KS) must be (4, 2), (4, 4), (8, 2), (8, 4) or (8, 8)");
+    return fail(ERNN_E_INVALID, "time-blocked shape (RW, KS) must be (2, 2), (2, 4), (4, 2), (4, 4), (8, 2), (8, 4) or (8, 8)");
   FEM_HIP(hipSetDevice(p->device));
   FEM_HIP(hipDeviceSynchronize());
   if (p->graph) {

@@ -230,9 +230,11 @@ __global__ void __launch_bounds__(kBlock) fem_step_kernel(FemStepArgs a) {
 }
 
 // --------------------------------------------------------------------------
-// Time-blocked step: KS steps per launch.  A block of 4 wavefronts holds a
-// region of 4 RW node rows x 64 node columns (wavefront w: rows w RW ..
-// w RW + RW - 1, lane l: column l) with u^n and u^{n-1} in registers.  Each
+// Time-blocked step: KS steps per launch.  A block of WB wavefronts (4, or 8
+// for RW = 2: a shorter serial chain per wavefront and twice the wavefronts
+// in flight) holds a region of WB RW node rows x 64 node columns (wavefront
+// w: rows w RW .. w RW + RW - 1, lane l: column l) with u^n and u^{n-1} in
+// registers.  Each
 // step a wavefront takes the row above and below its band from its
 // neighbours through LDS (double-buffered by step parity: one barrier per
 // step), the column neighbour and the left cell's corners by DPP with zero
@@ -242,8 +244,12 @@ __global__ void __launch_bounds__(kBlock) fem_step_kernel(FemStepArgs a) {
 // ring held at zero).  Every node's arithmetic is fem_tile's, so the frames
 // and states equal the one-step kernel's bit for bit.
 // --------------------------------------------------------------------------
+template <int RW>
+constexpr int fem_block_waves() { return RW == 2 ? 8 : kWavesPerBlock; }
+
 template <int MODE, int RW, int KS>
-__global__ void __launch_bounds__(kBlock) fem_block_kernel(FemBlockArgs a) {
+__global__ void __launch_bounds__(kWave * fem_block_waves<RW>()) fem_block_kernel(FemBlockArgs a) {
+  constexpr int WB = fem_block_waves<RW>();
   const int lb = (int)(blockIdx.x & 7) * a.blocks_per_xcd + (int)(blockIdx.x >> 3);
   if (lb >= a.n_blocks) return;   // whole block: no barrier is reached by anyone
   const FemCallDev& C = *a.call;
@@ -299,7 +305,7 @@ __global__ void __launch_bounds__(kBlock) fem_block_kernel(FemBlockArgs a) {
   }
 
   // edge rows: [buffer][wavefront][top, bottom][x, y][lane]
-  __shared__ float edge[2][kWavesPerBlock][2][2][kWave];
+  __shared__ float edge[2][WB][2][2][kWave];
   edge[0][w][0][0][lane] = cx[0];
   edge[0][w][0][1][lane] = cy[0];
   edge[0][w][1][0][lane] = cx[RW - 1];
@@ -309,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) fem_block_kernel(FemBlockArgs a) {
   gfloat* frx = (gfloat*)C.frames_x;
   gfloat* fry = (gfloat*)C.frames_y;
   const gfloat* wav = (const gfloat*)C.wav;
-  // the output tile: region rows KS .. 4 RW - KS - 1, lanes KS .. 63 - KS
+  // the output tile: region rows KS .. WB RW - KS - 1, lanes KS .. 63 - KS
   const bool lane_out = lane >= KS && lane < kWave - KS && col_int;
   for (int s = 0; s < a.ks; ++s) {
     const int buf = s & 1;
@@ -317,8 +323,8 @@ __global__ void __launch_bounds__(kBlock) fem_block_kernel(FemBlockArgs a) {
     const float wv = wav[n];
     const float ax = w > 0 ? edge[buf][w - 1][1][0][lane] : 0.f;
     const float ay = w > 0 ? edge[buf][w - 1][1][1][lane] : 0.f;
-    const float bx = w < kWavesPerBlock - 1 ? edge[buf][w + 1][0][0][lane] : 0.f;
-    const float by = w < kWavesPerBlock - 1 ? edge[buf][w + 1][0][1][lane] : 0.f;
+    const float bx = w < WB - 1 ? edge[buf][w + 1][0][0][lane] : 0.f;
+    const float by = w < WB - 1 ? edge[buf][w + 1][0][1][lane] : 0.f;
     // cell rows r = 0 .. RW: nodes r0 - 1 + r (row a) and r0 + r (row b)
     float xa0 = ax, ya0 = ay, xa1 = from_right(ax), ya1 = from_right(ay);
     float lx = xa1 - xa0, ly = ya1 - ya0;
@@ -366,7 +372,7 @@ __global__ void __launch_bounds__(kBlock) fem_block_kernel(FemBlockArgs a) {
       for (int k = 0; k < RW; ++k) {
         const int rr = w * RW + k;
         const int gi = r0 + k;
-        if (lane_out && rr >= KS && rr < kWavesPerBlock * RW - KS && gi >= 1 && gi <= a.nx - 2) {
+        if (lane_out && rr >= KS && rr < WB * RW - KS && gi >= 1 && gi <= a.nx - 2) {
           const int64_t fo = fo0 + (int64_t)gi * a.ny + c;
           frx[fo] = cx[k] * a.scale;
           fry[fo] = cy[k] * a.scale;
@@ -388,7 +394,7 @@ __global__ void __launch_bounds__(kBlock) fem_block_kernel(FemBlockArgs a) {
   for (int k = 0; k < RW; ++k) {
     const int rr = w * RW + k;
     const int gi = r0 + k;
-    if (lane_out && rr >= KS && rr < kWavesPerBlock * RW - KS && gi >= 1 && gi <= a.nx - 2) {
+    if (lane_out && rr >= KS && rr < WB * RW - KS && gi >= 1 && gi <= a.nx - 2) {
       const uint32_t o = row_off(pitch, gi, rj0) + (uint32_t)lane;
       *at(dx_old, o) = ox[k];
       *at(dx_new, o) = cx[k];
@@ -474,13 +480,14 @@ hipError_t launch_fem_step(const FemStepArgs& a, int V, int R, int blocks, hipSt
                               : launch_fem_m<kFemConst>(a, R, blocks, st);
 }
 
-#define ERNN_FEM_BLOCK_SHAPES(X) X(4, 2) X(4, 4) X(8, 2) X(8, 4) X(8, 8)
+#define ERNN_FEM_BLOCK_SHAPES(X) X(2, 2) X(2, 4) X(4, 2) X(4, 4) X(8, 2) X(8, 4) X(8, 8)
 
 template <int MODE>
 hipError_t launch_fem_block_m(const FemBlockArgs& a, int grid, hipStream_t st) {
 #define ERNN_FEM_BCASE(RW_, KS_)                                                                      \
   if (a.rw == RW_ && a.ksmax == KS_) {                                                                \
-    hipLaunchKernelGGL((fem_block_kernel<MODE, RW_, KS_>), dim3((unsigned)grid), dim3(kBlock), 0, st, a); \
+    hipLaunchKernelGGL((fem_block_kernel<MODE, RW_, KS_>), dim3((unsigned)grid),                      \
+                       dim3(kWave * fem_block_waves<RW_>()), 0, st, a);                               \
     return hipGetLastError();                                                                         \
   }
   ERNN_FEM_BLOCK_SHAPES(ERNN_FEM_BCASE)

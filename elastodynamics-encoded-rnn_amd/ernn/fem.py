@@ -31,7 +31,7 @@ from . import _lib
 from .data import save_reference_data
 
 FEM_SHAPES = ((1, 1), (1, 2), (1, 4), (1, 8))   # (V, R): R node rows x 63 columns per wavefront
-FEM_BLOCK_SHAPES = ((4, 2), (4, 4), (8, 2), (8, 4), (8, 8))   # (RW, KS): KS steps per launch
+FEM_BLOCK_SHAPES = ((2, 2), (2, 4), (4, 2), (4, 4), (8, 2), (8, 4), (8, 8))   # (RW, KS): KS steps per launch
 STIFFNESS_NAMES = ("C11", "C22", "C12", "C16", "C26", "C66")
 DEFAULT_DAMAGE = [{"type": "circle", "center": (0.1, 0.1), "size": 0.02, "reduction_factor": 0.5}]   # DF:856-866
 

@@ -238,8 +238,9 @@ typedef struct {
   int32_t tile_v, tile_r; /* tile: a wavefront sweeps R node rows x 63 node columns, one column per
                             lane: (V, R) = (1, 1), (1, 2), (1, 4) or (1, 8); 0, 0 = (1, 1) */
   int32_t use_graph;     /* 1: replay the run as a captured hipGraph */
-  int32_t blk_rw, blk_ks; /* time-blocked: blk_ks steps per launch on blocks of 4 wavefronts x blk_rw
-                            node rows x 64 columns, (RW, KS) in (4, 2) (4, 4) (8, 2) (8, 4) (8, 8);
+  int32_t blk_rw, blk_ks; /* time-blocked: blk_ks steps per launch on blocks of 4 wavefronts (8 for
+                            RW = 2) x blk_rw node rows x 64 columns, (RW, KS) in (2, 2) (2, 4) (4, 2)
+                            (4, 4) (8, 2) (8, 4) (8, 8);
                             blk_ks <= 1: one step per launch with the tile above.  Same results bit
                             for bit. */
   const ernn_source_t* loads; /* host array: nodal loads w at (i, j) on component comp, the
