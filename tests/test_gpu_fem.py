@@ -210,3 +210,34 @@ def test_fem_solver_api_and_reference_file_round_trip(tmp_path):
     dx_, dy_, dmeta = data.load_fem_damage_data(dpath, device=DEV)
     assert dmeta["damage_mask"].sum() > 0 and int(dmeta["damage_info"]["total_damaged_points"]) > 0
     assert rel_l2(dy_.cpu().numpy(), fy.cpu().numpy()) > 1e-4
+
+
+def test_fem_damage_reference_size_vs_fp64_oracle():
+    """DF main (DF:1486-1545): 301 x 301 nodes, 0.4 m, 4001 samples of 30 ns,
+    four damage regions with the stiffness halved (DF:877-880), frames every 5
+    steps; FP32 vs the FP64 oracle, and time-blocked = one step per launch."""
+    nx = ny = 301
+    L = 0.4
+    x = np.linspace(-L / 2, L / 2, nx)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    dt, nt, t = fem.time_axis(4001, 120e-6, L / 300, L / 300, np.sqrt(130.74e9 / RHO))
+    g = fem.gaussian_point_load(X, Y, 0.0, 0.0, 5e6).astype(np.float32)
+    w = fem.ricker_wavelet(t, 100e3, 10e-6, 1e10).astype(np.float32)
+    mask, info = fem.create_damage_regions(X, Y, [
+        {"type": "circle", "center": (0.1, 0.1), "size": 0.02},
+        {"type": "square", "center": (-0.1, -0.1), "size": 0.02},
+        {"type": "square", "center": (-0.1, -0.06), "size": 0.02},
+        {"type": "square", "center": (-0.06, -0.1), "size": 0.02}])
+    assert info["total_damaged_points"] > 0
+    D = np.where(mask, 0.5, 1.0).astype(np.float32)
+    c = dict(nx=nx, ny=ny, Lx=L, Ly=L, dx=L / 300, dy=L / 300, dt=dt, nt=nt, g=g, w=w,
+             C32=np.asarray(C_PF, np.float32))
+    gx, gy = gpu_run(c, stride=5, tile=(1, 1), D=torch.tensor(D))
+    assert gx.shape == (801, nx, ny)
+    ox, oy = oracle_run(c, stride=5, D=D)
+    e = rel_l2(np.stack([gx, gy]), np.stack([ox, oy]))
+    print(f"[fem] DF main 301^2 x 4001 rel L2 vs FP64 oracle {e:.3e}", flush=True)
+    assert e < 2e-5
+    bx, by = gpu_run(c, stride=5, D=torch.tensor(D), blocking=(2, 4))
+    np.testing.assert_array_equal(bx, gx)
+    np.testing.assert_array_equal(by, gy)
