@@ -191,13 +191,14 @@ def damaged_stiffness(C, mask, reduction_factor=0.5):
     return out
 
 
-def solve(nx, ny, Lx, Ly, dt, nt, C, rho, load, wavelet, *, C_fields=None, stride=1, scale=0.7):
+def solve(nx, ny, Lx, Ly, dt, nt, C, rho, load, wavelet, *, C_fields=None, stride=1, scale=0.7, load_x=None):
     """Central-difference FEM run of PF:479-526 in FP64.
 
     C: six constants (PF:402-406), or C_fields: six (nx, ny) fields sampled at
     each element's centroid node (DF:1003-1026).  load: (nx, ny) nodal
     Gaussian weights (PF:298-301), applied to the y component; wavelet: (nt,)
-    time factors (PF:459-463).  Steps n = 1 .. nt-2 produce u^{n+1}; u^0 =
+    time factors (PF:459-463); load_x: an optional x-component load of the
+    same form (the reference loads y only).  Steps n = 1 .. nt-2 produce u^{n+1}; u^0 =
     u^1 = 0 and boundary nodes are held at zero (PF:495-497).  Returns
     (ux, uy) frames of shape (ceil(nt / stride), nx, ny) holding scale x
     u^{k stride} (PF:525-526, PF:926-930)."""
@@ -213,6 +214,7 @@ def solve(nx, ny, Lx, Ly, dt, nt, C, rho, load, wavelet, *, C_fields=None, strid
     K = stiffness_matrix(x, y, elements, nx, C_elem)
     M = lumped_mass(elements, N, rho, dx, dy)
     g = np.asarray(load, np.float64).T.ravel()          # node order j * nx + i (PF:322)
+    gx = None if load_x is None else np.asarray(load_x, np.float64).T.ravel()
     tf = np.asarray(wavelet, np.float64)
     n_frames = (nt + stride - 1) // stride
     fx_out = np.zeros((n_frames, nx, ny))
@@ -223,7 +225,8 @@ def solve(nx, ny, Lx, Ly, dt, nt, C, rho, load, wavelet, *, C_fields=None, strid
         fext_y = (tf[n] * g / rho) * M                  # PF:482-484
         f = K @ u_cur
         u_next = 2 * u_cur - u_prev
-        u_next[:N] -= dt * dt * f[:N] / M               # PF:489-493
+        fext_x = 0.0 if gx is None else (tf[n] * gx / rho) * M
+        u_next[:N] += dt * dt * (fext_x - f[:N]) / M    # PF:489-493
         u_next[N:] += dt * dt * (fext_y - f[N:]) / M
         u_next[boundary] = 0.0                          # PF:495-497
         u_next[N + boundary] = 0.0
