@@ -54,6 +54,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _ranked(times, n=4):
+    """The n fastest time-blocked shapes of a tuner's {shape: ms per step}."""
+    best = sorted((t, s) for s, t in times.items() if s != (0, 0, 0))[:n]
+    return "ranked " + ", ".join(f"{s} {t * 1e3:.2f}" for t, s in best)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -646,7 +652,7 @@ def main():
         bv, brw, bks, bms, btimes = tune_blocking(eng, params_true, field_true, model.wavelets,
                                                   steps=max(16, min(96, 40000 // cells_k)))
         log(f"autotune: forward blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
-            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f})")
+            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f}; {_ranked(btimes)})")
     if args.adj_blocking not in ("auto", "0"):
         eng.set_blocking(*(int(x) for x in args.adj_blocking.split(",")), which=1)
         log(f"time-blocked adjoint fixed: (V, RW, KS) = {eng.adj_blocking}")
@@ -654,7 +660,7 @@ def main():
         bv, brw, bks, bms, btimes = tune_blocking(eng, params_true, field_true, model.wavelets,
                                                   steps=max(16, min(96, 40000 // cells_k)), which=1)
         log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
-            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f})")
+            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f}; {_ranked(btimes)})")
     with torch.no_grad():
         obs = ernn.ernn_traces(eng, params=params_true, field=field_true, wavelets=model.wavelets)
         gen = torch.Generator(device="cpu").manual_seed(1)

@@ -89,7 +89,7 @@ void free_tables(FemPlan* p) {
 // (b % 8) x blocks_per_xcd + b / 8 so that each XCD sweeps a band of rows.
 // A block is flagged when a load lies in its region (output tile + KS halo).
 int build_blocks(FemPlan* p) {
-  const int wb = p->RW == 2 ? 8 : kWavesPerBlock;   // wavefronts per block (fem_block_waves in ernn_fem.hip)
+  const int wb = blk_waves(p->RW);   // wavefronts per block
   const int orows = wb * p->RW - 2 * p->KS, ocols = kWave - 2 * p->KS;
   const int br = (p->nx - 2 + orows - 1) / orows, bc = (p->ny - 2 + ocols - 1) / ocols;
   std::vector<FemBlock> blocks;

@@ -244,12 +244,9 @@ __global__ void __launch_bounds__(kBlock) fem_step_kernel(FemStepArgs a) {
 // ring held at zero).  Every node's arithmetic is fem_tile's, so the frames
 // and states equal the one-step kernel's bit for bit.
 // --------------------------------------------------------------------------
-template <int RW>
-constexpr int fem_block_waves() { return RW == 2 ? 8 : kWavesPerBlock; }
-
 template <int MODE, int RW, int KS>
-__global__ void __launch_bounds__(kWave * fem_block_waves<RW>()) fem_block_kernel(FemBlockArgs a) {
-  constexpr int WB = fem_block_waves<RW>();
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) fem_block_kernel(FemBlockArgs a) {
+  constexpr int WB = blk_waves(RW);
   const int lb = (int)(blockIdx.x & 7) * a.blocks_per_xcd + (int)(blockIdx.x >> 3);
   if (lb >= a.n_blocks) return;   // whole block: no barrier is reached by anyone
   const FemCallDev& C = *a.call;
@@ -487,7 +484,7 @@ hipError_t launch_fem_block_m(const FemBlockArgs& a, int grid, hipStream_t st) {
 #define ERNN_FEM_BCASE(RW_, KS_)                                                                      \
   if (a.rw == RW_ && a.ksmax == KS_) {                                                                \
     hipLaunchKernelGGL((fem_block_kernel<MODE, RW_, KS_>), dim3((unsigned)grid),                      \
-                       dim3(kWave * fem_block_waves<RW_>()), 0, st, a);                               \
+                       dim3(kWave * blk_waves(RW_)), 0, st, a);                               \
     return hipGetLastError();                                                                         \
   }
   ERNN_FEM_BLOCK_SHAPES(ERNN_FEM_BCASE)

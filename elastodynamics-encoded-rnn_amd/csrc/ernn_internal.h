@@ -107,10 +107,19 @@ struct FusedArgs {
 
 // Time-blocked forward: one launch advances up to kMaxBlockSteps steps over
 // block regions whose halo (one cell per step on every side) covers the extra
-// steps.  A block is kWavesPerBlock wavefronts stacked vertically, each
+// steps.  A block is blk_waves(RW) wavefronts stacked vertically, each
 // holding RW rows x 64 V columns of the region in registers; the output tile
 // is the region shrunk by KS cells on every side.
 constexpr int kMaxBlockSteps = 8;
+
+// Wavefronts per time-blocked block: two-row bands (RW = 2) stack 8, which
+// keeps the region as tall as RW = 4 with half the serial work per
+// wavefront; every other shape stacks kWavesPerBlock.  Only the FEM kernel
+// builds RW = 2: on the ERNN stencil the two-row shapes lost to RW = 4
+// (one barrier per step across 8 wavefronts costs more than the shorter
+// chain saves).
+constexpr int kMaxBlockWaves = 8;
+constexpr int blk_waves(int rw) { return rw == 2 ? kMaxBlockWaves : kWavesPerBlock; }
 
 struct BlockDev {
   int inst, oi0, oj0, set;  // output tile origin (grid coordinates)
@@ -144,7 +153,7 @@ struct BlockAdjArgs {
   const BlockDev* blocks;        // receiver lists cover the whole region (injection)
   const RecEv* rec;
   const CallDev* call;
-  int64_t partial_off;           // [blocks][kWavesPerBlock][kPartials] fp64 partials
+  int64_t partial_off;           // [blocks][blk_waves(RW)][kPartials] fp64 partials
   int n_blocks;
   int mode;
   int m0, ks;
@@ -206,7 +215,7 @@ struct FemCallDev {
   float* frames_y;
 };
 
-// Time-blocked FEM launch: a block of kWavesPerBlock wavefronts x RW node
+// Time-blocked FEM launch: a block of blk_waves(RW) wavefronts x RW node
 // rows x 64 node columns advances up to KS steps from registers; the output
 // tile is the block region shrunk by KS nodes on every side.
 constexpr int kFemMaxBlockSteps = 8;

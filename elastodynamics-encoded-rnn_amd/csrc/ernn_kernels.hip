@@ -753,8 +753,8 @@ __global__ void __launch_bounds__(kBlock) fused_step_kernel(FusedArgs fa) {
 // dependent loads put the blocks on a receiver line on the critical path.
 // Call block-uniformly (it holds a block barrier); valid only when no two
 // receivers share a cell (BlockDev::rec_shared).
-template <int V, int RW>
-__device__ __forceinline__ uint32_t map_receivers(int (&map)[kWavesPerBlock][RW][kWave * V], const RecEv* rec,
+template <int V, int RW, int WB>
+__device__ __forceinline__ uint32_t map_receivers(int (&map)[WB][RW][kWave * V], const RecEv* rec,
                                                   int rec_begin, int rec_end, int gi0, int gc0, int w,
                                                   int lane) {
 #pragma unroll
@@ -779,7 +779,7 @@ __device__ __forceinline__ uint32_t map_receivers(int (&map)[kWavesPerBlock][RW]
 
 // --------------------------------------------------------------------------
 // Time-blocked forward: KS steps per launch on a block region of
-// kWavesPerBlock * RW rows x 64 V columns.  Wavefront w holds region rows
+// blk_waves(RW) * RW rows x 64 V columns.  Wavefront w holds region rows
 // [w RW, (w + 1) RW) of u^n and u^{n-1} in registers; column neighbours come
 // by DPP shifts, and the rows just above and below a wavefront's band come
 // from its neighbours through LDS (one barrier per step).  The valid part of
@@ -800,10 +800,11 @@ template <int MODE, int V, int RW, int KS>
 __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   const auto& a = *ap;
   constexpr int kCols = kWave * V;
-  constexpr int OR = kWavesPerBlock * RW - 2 * KS;   // output rows
+  constexpr int WB = blk_waves(RW);                  // wavefronts
+  constexpr int OR = WB * RW - 2 * KS;               // output rows
   constexpr int OC = kCols - 2 * KS;                 // output columns
   constexpr int NC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
-  __shared__ float xch[2][kWavesPerBlock][2][2][kCols];   // [buffer][wavefront][top/bottom][comp][column]
+  __shared__ float xch[2][WB][2][2][kCols];          // [buffer][wavefront][top/bottom][comp][column]
 
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x >> 6;
@@ -864,9 +865,9 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
       owned[r][e] = inside[r][e] && gi >= bl.oi0 && gi < bl.oi0 + OR && gj >= bl.oj0 && gj < bl.oj0 + OC;
     }
 
-  __shared__ int recmap[kWavesPerBlock][RW][kCols];
+  __shared__ int recmap[WB][RW][kCols];
   const bool use_map = a.record && !a.snapshot && !bl.rec_shared && bl.rec_end > bl.rec_begin;   // block-uniform
-  const uint32_t rbits = use_map ? map_receivers<V, RW>(recmap, a.rec, bl.rec_begin, bl.rec_end, gi0,
+  const uint32_t rbits = use_map ? map_receivers<V, RW, WB>(recmap, a.rec, bl.rec_begin, bl.rec_end, gi0,
                                                         bl.oj0 - KS, w, lane) : 0u;
 
   auto publish = [&](int buf) {
@@ -892,8 +893,8 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
       for (int e = 0; e < V; ++e) {
         tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
         ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
-        bx[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
-        by[e] = (w < kWavesPerBlock - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+        bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
+        by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
       }
       xr[0] = make_row_z<V>(tx);
       yr[0] = make_row_z<V>(ty);
@@ -1032,7 +1033,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
 // per launch.  The reverse recurrence is that of adj_tile.
 // --------------------------------------------------------------------------
 template <int MODE, int V, int RW, int KS>
-__global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockAdjArgs a) {
 #pragma clang fp contract(fast)
   const int bid = blockIdx.x;
 #include "ernn_adj_block_body.inc"
@@ -1040,7 +1041,7 @@ __global__ void __launch_bounds__(kBlock) adj_block_kernel(BlockAdjArgs a) {
 
 // the argument is read in place through the kernarg segment (offset 0)
 template <int MODE, int V, int RW, int KS>
-__global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs) {
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) fwd_block_kernel(BlockArgs) {
   fwd_block_body<MODE, V, RW, KS>((KArg<BlockArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
 
@@ -1054,7 +1055,7 @@ __global__ void __launch_bounds__(kBlock) fwd_block_kernel(BlockArgs) {
 // multiple of 8, so both halves keep the XCD banding of their block tables.
 // --------------------------------------------------------------------------
 template <int MODE, int V, int RW, int KS>
-__global__ void __launch_bounds__(kBlock) fused_block_kernel(FusedBlockArgs fa) {
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) fused_block_kernel(FusedBlockArgs fa) {
 #pragma clang fp contract(fast)
   if (blockIdx.x >= fa.adj_grid) {   // block-uniform
     const auto base = (const char __attribute__((address_space(4)))*)__builtin_amdgcn_kernarg_segment_ptr();
@@ -1104,7 +1105,7 @@ __global__ void pack_fields_kernel(const CallDev* call, const InstDev* inst, con
 }
 
 // grad_params[set][p] = sum over the set's tiles of partial[t][p]
-// per_rec partial slots per record (tile: 1, block: kWavesPerBlock)
+// per_rec partial slots per record (tile: 1, block: its wavefronts)
 template <typename Rec>
 __global__ void finalize_const_kernel(const CallDev* call, int64_t partial_off,
                                       const Rec* __restrict__ tiles, int n_tiles, int per_rec) {
@@ -1238,12 +1239,14 @@ hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t
 // wavefront, KS steps per launch (block_ok in ernn_plan.cpp).  The field
 // modes leave out (2, 8), whose registers would spill.  (1, 8, 5) measured
 // 19.1 against 20.6 us per step for (1, 8, 4) on 8 x 1024^2 damage shots.
+// Two-row bands of 8 wavefronts (blk_waves(2)) measured slower on 512^2:
+// (1, 2, 4) 2.41 against 2.27 us per step for (1, 4, 4).
 #define ERNN_BLOCK_CASES_FIELD(X) \
   X(1, 4, 2) X(1, 4, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 5) X(1, 8, 8) X(2, 4, 2) X(2, 4, 4)
 #define ERNN_BLOCK_CASES(X) ERNN_BLOCK_CASES_FIELD(X) X(2, 8, 2) X(2, 8, 4) X(2, 8, 8)
 #define ERNN_BLOCK_CASE(v_, r_, k_)                                                                 \
   if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
-    hipLaunchKernelGGL((fwd_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kBlock), 0, st, a);  \
+    hipLaunchKernelGGL((fwd_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
     return hipGetLastError();                                                                      \
   }
 template <int MODE>
@@ -1273,7 +1276,7 @@ hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st)
 #define ERNN_ADJ_BLOCK_CASES(X) ERNN_ADJ_BLOCK_CASES_DFIELD(X) X(1, 8, 5) X(2, 4, 2) X(2, 4, 4)
 #define ERNN_ADJ_BLOCK_CASE(v_, r_, k_)                                                             \
   if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
-    hipLaunchKernelGGL((adj_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kBlock), 0, st, a);  \
+    hipLaunchKernelGGL((adj_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
     return hipGetLastError();                                                                      \
   }
 template <int MODE>
@@ -1300,7 +1303,8 @@ hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t 
 #define ERNN_FUSED_BLOCK_CASES(X) X(1, 4, 2) X(1, 4, 4) X(1, 8, 4)
 #define ERNN_FUSED_BLOCK_CASE(v_, r_, k_)                                                             \
   if (V == v_ && fa.adj.rw == r_ && fa.adj.ksmax == k_) {                                            \
-    hipLaunchKernelGGL((fused_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kBlock), 0, st, fa); \
+    hipLaunchKernelGGL((fused_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, \
+                       fa);                                                                          \
     return hipGetLastError();                                                                        \
   }
 template <int MODE>
@@ -1379,9 +1383,9 @@ hipError_t launch_finalize_const(const CallDev* call, int64_t partial_off, const
 }
 
 hipError_t launch_finalize_const_blocks(const CallDev* call, int64_t partial_off, const BlockDev* blocks,
-                                        int n_blocks, int n_sets, hipStream_t st) {
+                                        int n_blocks, int waves, int n_sets, hipStream_t st) {
   hipLaunchKernelGGL(finalize_const_kernel<BlockDev>, dim3(n_sets * 6), dim3(kBlock), 0, st, call, partial_off,
-                     blocks, n_blocks, kWavesPerBlock);
+                     blocks, n_blocks, waves);
   return hipGetLastError();
 }
 

@@ -41,7 +41,7 @@ hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st)
 hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st);
 hipError_t launch_fused_block(const FusedBlockArgs& fa, int V, int grid, hipStream_t st);
 hipError_t launch_finalize_const_blocks(const CallDev* call, int64_t partial_off, const BlockDev* blocks,
-                                        int n_blocks, int n_sets, hipStream_t st);
+                                        int n_blocks, int waves, int n_sets, hipStream_t st);
 hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st);
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t st);
 hipError_t launch_fill_ws(const CallDev* call, int64_t off, int64_t n, hipStream_t st);
@@ -108,7 +108,7 @@ bool fwd_tile_ok(int mode, int V, int R) {
 bool block_ok(int mode, int V, int RW, int KS) {
   if (V == 1 && RW == 8 && KS == 5) return true;
   if (!(V == 1 || V == 2) || !(RW == 4 || RW == 8) || !(KS == 2 || KS == 4 || KS == 8)) return false;
-  if (kWavesPerBlock * RW - 2 * KS <= 0) return false;
+  if (blk_waves(RW) * RW - 2 * KS <= 0) return false;
   return !(mode != ERNN_PARAM_CONST6 && V == 2 && RW == 8);
 }
 // Time-blocked adjoint shapes (ERNN_ADJ_BLOCK_CASES): constant and damage
@@ -351,7 +351,7 @@ int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
   return ERNN_OK;
 }
 
-// Output tiles of (4 RW - 2 KS) x (64 V - 2 KS) cells per instance, row-major;
+// Output tiles of (blk_waves(RW) RW - 2 KS) x (64 V - 2 KS) cells per instance, row-major;
 // each block's source list covers its whole region (the halo cells' states
 // feed the later steps), its receiver list only its output tile.
 int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
@@ -366,7 +366,7 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
                                 "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8), "
                                 "or be (1, 8, 5)");
   if (adjoint && !ablk_ok(p->mode, p->fuse, V, RW, KS)) return fail(ERNN_E_INVALID, kAblkShapes);
-  const int OR = kWavesPerBlock * RW - 2 * KS, OC = kWave * V - 2 * KS;
+  const int OR = blk_waves(RW) * RW - 2 * KS, OC = kWave * V - 2 * KS;
   std::vector<BlockDev> blocks;
   std::vector<SrcEv> sev;
   std::vector<RecEv> rev;
@@ -437,7 +437,7 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
     }
   }
   if (blocks.size() > ((size_t)1 << 30)) return fail(ERNN_E_CAPACITY, "too many blocks");
-  if (adjoint && (int64_t)blocks.size() * kWavesPerBlock > p->partial_cap)
+  if (adjoint && (int64_t)blocks.size() * blk_waves(RW) > p->partial_cap)
     return fail(ERNN_E_CAPACITY, "time-blocked adjoint needs more partial slots than the workspace holds");
   free_blocking(*bt);
   bt->V = V;
@@ -653,9 +653,10 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
     }
   }
   {
-    int64_t nb = 0;   // blocks of the smallest time-blocked output tile (8 x 48 cells), 4 slots each
+    // blocks of the smallest time-blocked output tile (8 x 48 cells), one slot per wavefront
+    int64_t nb = 0;
     for (const auto& I : p->inst_h) nb += (int64_t)((I.nx + 7) / 8) * ((I.ny + 47) / 48);
-    p->partial_cap = std::max<int64_t>(tiles_for(*p, 1, 1), kWavesPerBlock * nb);
+    p->partial_cap = std::max<int64_t>(tiles_for(*p, 1, 1), kMaxBlockWaves * nb);
   }
   p->partial_off = off / 2;
   off += round_up(2 * kPartials * p->partial_cap, 64);
@@ -976,6 +977,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
   if (p->mode != ERNN_PARAM_FIELD6) {
     if (p->blk_adj.KS > 1)
       ERNN_HIP(launch_finalize_const_blocks(p->d_call, p->partial_off, p->blk_adj.d_blocks, p->blk_adj.n_blocks,
+                                            blk_waves(p->blk_adj.RW),
                                             p->n_sets, st));
     else
       ERNN_HIP(launch_finalize_const(p->d_call, p->partial_off, p->adj.d_tiles, p->adj.n_tiles, p->n_sets, st));
