@@ -122,10 +122,13 @@ constexpr int kMaxBlockWaves = 8;
 constexpr int blk_waves(int rw) { return rw == 2 ? kMaxBlockWaves : kWavesPerBlock; }
 
 struct BlockDev {
+  InstDev I;                // the instance's record, carried inline: a block's first field loads then wait
+                            // on one dependent load (this record) instead of two (record, then instance)
   int inst, oi0, oj0, set;  // output tile origin (grid coordinates)
   int src_begin, src_end;   // sources inside the block's region, sorted by row
   int rec_begin, rec_end;   // receivers inside the output tile, sorted by row
   int rec_shared;           // 1: two receivers share a cell (the kernels walk the list instead of a cell map)
+  int pad;
 };
 
 struct BlockArgs {

@@ -811,12 +811,23 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   // block b runs on XCD b % 8, which owns a contiguous band of block tiles
   const int chunk = (a.n_blocks + 7) >> 3;
   const int t = (bid & 7) * chunk + (bid >> 3);
+  // Bands of up to 4 rows load the call and block records side by side ahead
+  // of the exits (a clamped, always valid index), pinned there by the empty
+  // asm, so the state loads wait on one scalar round trip after the kernel
+  // arguments instead of four: single-512 forward 2.30 -> 2.23 us per step.
+  // 8-row bands keep the plain order, which the compiler gives fewer
+  // registers (96 against 100 VGPRs: 5 wavefronts per SIMD instead of 4).
+  constexpr bool kEarly = RW <= 4;
+  if (!kEarly && t >= a.n_blocks) return;
+  const GCall C = global_call(*a.call);
+  const BlockDev bl = a.blocks[max(min(t, a.n_blocks - 1), 0)];
+  if constexpr (kEarly)
+    asm volatile("" ::"s"((uint64_t)C.ws), "s"(bl.I.nt), "s"(bl.I.field), "s"(bl.I.pool_off), "s"(bl.oi0),
+                 "s"(a.n0), "s"(a.ks), "s"(a.s_prev), "s"(a.s_cur));
   if (t >= a.n_blocks) return;
-  const BlockDev bl = a.blocks[t];
-  const InstDev I = a.inst[bl.inst];
+  const InstDev I = bl.I;
   const int ks = min(a.ks, I.nt - a.n0);
   if (ks <= 0) return;   // this instance has finished its time loop
-  const GCall C = global_call(*a.call);
 
   const uint32_t P = (uint32_t)I.pitch;
   gfloat* pool = C.ws + I.pool_off;

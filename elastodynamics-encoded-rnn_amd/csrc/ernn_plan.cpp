@@ -415,6 +415,7 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
       for (int bj = 0; bj < nbj; ++bj) {
         const int64_t id = (int64_t)bi * nbj + bj;
         BlockDev bd{};
+        bd.I = p->inst[b];
         bd.inst = b;
         bd.oi0 = bi * OR;
         bd.oj0 = bj * OC;
