@@ -248,14 +248,21 @@ template <int MODE, int RW, int KS>
 __global__ void __launch_bounds__(kWave * blk_waves(RW)) fem_block_kernel(FemBlockArgs a) {
   constexpr int WB = blk_waves(RW);
   const int lb = (int)(blockIdx.x & 7) * a.blocks_per_xcd + (int)(blockIdx.x >> 3);
-  if (lb >= a.n_blocks) return;   // whole block: no barrier is reached by anyone
+  // the call and block records load side by side ahead of the exit (a
+  // clamped, always valid index), so the field loads wait on one scalar
+  // round trip after the kernel arguments: the exit tests a copy of n_blocks
+  // passed through an empty asm that takes the loaded values as inputs (not
+  // volatile, so later uniform loads stay scalar)
   const FemCallDev& C = *a.call;
   gfloat* ws = (gfloat*)C.ws;
+  const FemBlock bl = a.blocks[max(min(lb, a.n_blocks - 1), 0)];
+  int n_blocks = a.n_blocks;
+  asm("" : "+s"(n_blocks) : "s"((uint64_t)ws), "s"(bl.oi0), "s"(bl.oj0), "s"(bl.has_loads));
+  if (lb >= n_blocks) return;   // whole block: no barrier is reached by anyone
   const gfloat* qp = ws + (int64_t)kFemSlots * a.field;
   float Q[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) Q[k] = qp[k];
-  const FemBlock bl = a.blocks[lb];
   const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & (kWave - 1));
   const int rj0 = bl.oj0 - KS;
   const int c = rj0 + lane;                       // the lane's grid column
