@@ -747,16 +747,17 @@ __global__ void __launch_bounds__(kBlock) fused_step_kernel(FusedArgs fa) {
 
 // Receivers of a block mapped onto the cells of each wavefront's band, once
 // per launch: map[w][r][c] = receiver index at band row r, region column c,
-// or -1; bit r V + e of the result: the lane's cell (r, e) holds one.  A
+// or -1, built in LDS; idx[r][e] = the entry of the lane's cell (r, e), kept
+// in registers; bit r V + e of the result: that cell holds a receiver.  A
 // recorded or injected step then serves every receiver of the wavefront with
 // one masked access per cell instead of a walk over the block's list, whose
 // dependent loads put the blocks on a receiver line on the critical path.
 // Call block-uniformly (it holds a block barrier); valid only when no two
 // receivers share a cell (BlockDev::rec_shared).
 template <int V, int RW, int WB>
-__device__ __forceinline__ uint32_t map_receivers(int (&map)[WB][RW][kWave * V], const RecEv* rec,
-                                                  int rec_begin, int rec_end, int gi0, int gc0, int w,
-                                                  int lane) {
+__device__ __forceinline__ uint32_t map_receivers(int (&map)[WB][RW][kWave * V], int (&idx)[RW][V],
+                                                  const RecEv* rec, int rec_begin, int rec_end, int gi0, int gc0,
+                                                  int w, int lane) {
 #pragma unroll
   for (int r = 0; r < RW; ++r)
 #pragma unroll
@@ -772,8 +773,10 @@ __device__ __forceinline__ uint32_t map_receivers(int (&map)[WB][RW][kWave * V],
 #pragma unroll
   for (int r = 0; r < RW; ++r)
 #pragma unroll
-    for (int e = 0; e < V; ++e)
-      if (map[w][r][lane * V + e] >= 0) bits |= 1u << (r * V + e);
+    for (int e = 0; e < V; ++e) {
+      idx[r][e] = map[w][r][lane * V + e];
+      if (idx[r][e] >= 0) bits |= 1u << (r * V + e);
+    }
   return bits;
 }
 
@@ -882,8 +885,9 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
 
   __shared__ int recmap[WB][RW][kCols];
   const bool use_map = a.record && !a.snapshot && !bl.rec_shared && bl.rec_end > bl.rec_begin;   // block-uniform
-  const uint32_t rbits = use_map ? map_receivers<V, RW, WB>(recmap, a.rec, bl.rec_begin, bl.rec_end, gi0,
-                                                        bl.oj0 - KS, w, lane) : 0u;
+  int ridx[RW][V];
+  const uint32_t rbits = use_map ? map_receivers<V, RW, WB>(recmap, ridx, a.rec, bl.rec_begin, bl.rec_end,
+                                                            gi0, bl.oj0 - KS, w, lane) : 0u;
 
   auto publish = [&](int buf) {
 #pragma unroll
