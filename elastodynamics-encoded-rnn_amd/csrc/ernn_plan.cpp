@@ -186,7 +186,12 @@ struct Plan {
   int64_t* d_set_ext_off = nullptr;
   int64_t* d_set_grad_off = nullptr;
   int64_t field_floats = 0, grad_field_doubles = 0;
-  hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;
+  // g_adj_skip: the adjoint that starts on the last segment's stored states
+  hipGraphExec_t g_fwd = nullptr, g_adj = nullptr, g_adj_skip = nullptr;
+  // The workspace whose last forward pass left every state of the last
+  // checkpoint segment in that segment's bank, not yet used up by an adjoint
+  // (its recompute of earlier segments reuses the bank), or null.
+  const void* seg_ws = nullptr;
   hipStream_t cap_stream = nullptr;
 
   // slots of one instance's state pool: ring (4), checkpoints, segment bank(s), lam ring (4)
@@ -252,7 +257,8 @@ void free_blocking(BlockTiling& b) {
 void drop_graphs(Plan* p) {
   if (p->g_fwd) (void)hipGraphExecDestroy(p->g_fwd);
   if (p->g_adj) (void)hipGraphExecDestroy(p->g_adj);
-  p->g_fwd = p->g_adj = nullptr;
+  if (p->g_adj_skip) (void)hipGraphExecDestroy(p->g_adj_skip);
+  p->g_fwd = p->g_adj = p->g_adj_skip = nullptr;
 }
 
 int64_t tiles_for(const Plan& p, int V, int R) {
@@ -754,10 +760,11 @@ BlockArgs block_args(const Plan* p, const BlockTiling& bt) {
 }
 BlockArgs block_args(const Plan* p) { return block_args(p, p->blk); }
 
-// One time-blocked launch of the forward pass (record = 1) from step n0.
-int launch_blk_forward(const Plan* p, BlockArgs& a, int n0, hipStream_t st) {
+// One time-blocked launch of the forward pass (record = 1) from step n0, up
+// to step `end` at most.
+int launch_blk_forward(const Plan* p, BlockArgs& a, int n0, int end, hipStream_t st) {
   a.n0 = n0;
-  a.ks = std::min(p->blk.KS, p->nt_max - n0);
+  a.ks = std::min(p->blk.KS, end - n0);
   a.s_prev = p->slot_blk(n0 - 1);
   a.s_cur = p->slot_blk(n0);
   for (int s = 0; s < kMaxBlockSteps; ++s) a.store[s] = s < a.ks ? p->slot_blk(n0 + 1 + s) : -1;
@@ -823,27 +830,40 @@ int launch_blk_adjoint(const Plan* p, BlockAdjArgs& a, int L, int m0, int ks, in
   return ERNN_OK;
 }
 
+// The forward pass keeps the checkpoints and, for the last checkpoint
+// segment (steps from B = (S - 1) K on), every state in that segment's bank,
+// as its recompute would: the adjoint that follows on the same workspace
+// starts on them instead of recomputing the segment (seg_ws).  States B - 1
+// and B are the segment's checkpoints under both slot maps.
 int enqueue_forward(const Plan* p, hipStream_t st) {
   ERNN_TRY(enqueue_pack(p, st));
+  const int B = (p->S - 1) * p->K;
   if (p->blk.KS > 1) {
     BlockArgs b = block_args(p);
     b.record = 1;
-    for (int n0 = 0; n0 < p->nt_max; n0 += p->blk.KS) ERNN_TRY(launch_blk_forward(p, b, n0, st));
+    for (int n0 = 0; n0 < B; n0 += p->blk.KS) ERNN_TRY(launch_blk_forward(p, b, n0, B, st));
+    for (int n0 = B; n0 < p->nt_max; n0 += p->blk.KS) {
+      blk_recompute_args(p, b, p->S - 1, n0, p->nt_max);
+      ERNN_HIP(launch_fwd_block(b, p->blk.V, p->blk.grid, st));
+    }
     return ERNN_OK;
   }
+  const auto slot = [&](int m) { return m >= B - 1 ? p->slot_seg(p->S - 1, m) : p->slot_fwd(m); };
   StepArgs a = base_args(p, p->fwd);
   a.record = 1;
   for (int n = 0; n < p->nt_max; ++n) {
     a.step = n;
-    a.s_prev = p->slot_fwd(n - 1);
-    a.s_cur = p->slot_fwd(n);
-    a.s_next = p->slot_fwd(n + 1);
+    a.s_prev = slot(n - 1);
+    a.s_cur = slot(n);
+    a.s_next = slot(n + 1);
     ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
   }
   return ERNN_OK;
 }
 
-int enqueue_adjoint(const Plan* p, hipStream_t st) {
+// skip_last: the last segment's states are in its bank (left by the forward
+// pass on this workspace), so its recompute is skipped.
+int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
   ERNN_TRY(enqueue_pack(p, st));
   // zero: lam ring, per-tile partials, fp64 per-cell accumulators
   for (int b = 0; b < p->n_inst; ++b) {
@@ -882,11 +902,11 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
     return ERNN_OK;
   };
   if (p->fuse && p->blk_adj.KS > 1) {
-    // The last segment is recomputed alone; afterwards every time-blocked
-    // adjoint launch of segment s carries the next KS recompute steps of
-    // segment s - 1 into the other bank (segment s - 1 has K steps, so a
-    // short last segment leaves recompute launches to run alone).
-    ERNN_TRY(recompute_segment(p->S - 1));
+    // The last segment is recomputed alone (unless stored); afterwards every
+    // time-blocked adjoint launch of segment s carries the next KS recompute
+    // steps of segment s - 1 into the other bank (segment s - 1 has K steps,
+    // so a short last segment leaves recompute launches to run alone).
+    if (!skip_last) ERNN_TRY(recompute_segment(p->S - 1));
     const int KS = p->blk_adj.KS;
     FusedBlockArgs fb{};
     fb.adj = block_adj_args(p);
@@ -921,7 +941,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
     int L = 0;
     for (int s = p->S - 1; s >= 0; --s) {
       const int top = seg_top(s), bot = s * p->K + 1;
-      ERNN_TRY(recompute_segment(s));
+      if (s < p->S - 1 || !skip_last) ERNN_TRY(recompute_segment(s));
       for (int m0 = top; m0 >= bot; m0 -= p->blk_adj.KS, ++L)
         ERNN_TRY(launch_blk_adjoint(p, ba, L, m0, std::min(p->blk_adj.KS, m0 - bot + 1), top, bot,
                                     [&](int m) { return p->slot_seg(s, m); }, st));
@@ -929,7 +949,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
   } else if (!p->fuse) {
     for (int s = p->S - 1; s >= 0; --s) {
       const int top = seg_top(s), bot = s * p->K + 1;
-      ERNN_TRY(recompute_segment(s));
+      if (s < p->S - 1 || !skip_last) ERNN_TRY(recompute_segment(s));
       a.seg_top = top;
       a.seg_bot = bot;
       for (int m = top; m >= bot; --m) {
@@ -942,10 +962,11 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
       }
     }
   } else {
-    // The last segment is recomputed alone; afterwards every adjoint step of
-    // segment s shares its launch with one recompute step of segment s - 1,
-    // which fills the other bank (segment s - 1 always has K steps).
-    ERNN_TRY(recompute_segment(p->S - 1));
+    // The last segment is recomputed alone (unless stored); afterwards every
+    // adjoint step of segment s shares its launch with one recompute step of
+    // segment s - 1, which fills the other bank (segment s - 1 always has K
+    // steps).
+    if (!skip_last) ERNN_TRY(recompute_segment(p->S - 1));
     FusedArgs fa{};
     fa.fwd = base_args(p, p->adj);   // the fused kernel runs both on the adjoint's tiles
     fa.fwd.record = 0;
@@ -994,6 +1015,8 @@ int enqueue_adjoint(const Plan* p, hipStream_t st) {
 }
 
 using Enqueue = int (*)(const Plan*, hipStream_t);
+int enqueue_adjoint_full(const Plan* p, hipStream_t st) { return enqueue_adjoint(p, st, false); }
+int enqueue_adjoint_skip(const Plan* p, hipStream_t st) { return enqueue_adjoint(p, st, true); }
 
 // Capture an enqueue function into a graph on the plan's private stream.
 int capture(Plan* p, Enqueue fn, hipGraphExec_t* out) {
@@ -1014,14 +1037,22 @@ int capture(Plan* p, Enqueue fn, hipGraphExec_t* out) {
 
 int run_pass(Plan* p, const ernn_call_t* c, hipStream_t st, bool adjoint) {
   ERNN_HIP(hipSetDevice(p->device));
+  // An adjoint uses up the stored last segment of its workspace; a forward
+  // pass stores it anew.
+  const bool skip = adjoint && c->workspace == p->seg_ws;
+  p->seg_ws = nullptr;
   ERNN_HIP(launch_set_call(p->d_call, to_dev(c), st));
-  Enqueue fn = adjoint ? enqueue_adjoint : enqueue_forward;
-  if (!p->use_graph) return fn(p, st);
-  hipGraphExec_t* g = adjoint ? &p->g_adj : &p->g_fwd;
-  // Every captured kernel reaches the workspace and the per-call buffers only
-  // through d_call, so one capture serves every call.
-  if (!*g) ERNN_TRY(capture(p, fn, g));
-  ERNN_HIP(hipGraphLaunch(*g, st));
+  Enqueue fn = !adjoint ? enqueue_forward : (skip ? enqueue_adjoint_skip : enqueue_adjoint_full);
+  if (!p->use_graph) {
+    ERNN_TRY(fn(p, st));
+  } else {
+    hipGraphExec_t* g = !adjoint ? &p->g_fwd : (skip ? &p->g_adj_skip : &p->g_adj);
+    // Every captured kernel reaches the workspace and the per-call buffers
+    // only through d_call, so one capture serves every call.
+    if (!*g) ERNN_TRY(capture(p, fn, g));
+    ERNN_HIP(hipGraphLaunch(*g, st));
+  }
+  if (!adjoint) p->seg_ws = c->workspace;
   return ERNN_OK;
 }
 
@@ -1106,6 +1137,7 @@ int ernn_workspace_init(void* plan, void* workspace, void* stream) {
   if ((reinterpret_cast<uintptr_t>(workspace) & 255) != 0)
     return fail(ERNN_E_INVALID, "workspace must be 256-byte aligned");
   ERNN_HIP(hipSetDevice(p->device));
+  if (workspace == p->seg_ws) p->seg_ws = nullptr;
   ERNN_HIP(launch_fill_zero(static_cast<float*>(workspace), p->ws_floats, static_cast<hipStream_t>(stream)));
   return ERNN_OK;
 }
@@ -1152,6 +1184,7 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   ERNN_TRY(check_call(p, call, which == 1 || which == 2 || which == 4, /*grads=*/false));  // never finalizes
   hipStream_t st = static_cast<hipStream_t>(stream);
   ERNN_HIP(hipSetDevice(p->device));
+  if (call->workspace == p->seg_ws) p->seg_ws = nullptr;   // the timed launches overwrite the state slots
   ERNN_HIP(launch_set_call(p->d_call, to_dev(call), st));
   hipEvent_t e0, e1;
   ERNN_HIP(hipEventCreate(&e0));
@@ -1185,7 +1218,7 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
     BlockArgs b = block_args(p);
     b.record = 1;
     const int launches = std::min(steps, (p->nt_max + p->blk.KS - 1) / p->blk.KS);
-    for (int L = 0; L < launches; ++L) ERNN_TRY(launch_blk_forward(p, b, L * p->blk.KS, st));
+    for (int L = 0; L < launches; ++L) ERNN_TRY(launch_blk_forward(p, b, L * p->blk.KS, p->nt_max, st));
   } else if (which == 0) {
     StepArgs a = base_args(p, p->fwd);
     a.record = 1;

@@ -186,7 +186,11 @@ int ernn_forward(void* plan, const ernn_call_t* call, void* stream);
 /* Gradient of a scalar loss J given dJ/d traces (replaces loss.backward(),
  * PI:447 / DI:511).  Needs the workspace of the matching ernn_forward.
  * Recomputes the forward from checkpoints segment by segment and runs the
- * reverse-time adjoint recurrence; FP64 accumulation. */
+ * reverse-time adjoint recurrence; FP64 accumulation.  The forward pass also
+ * leaves every state of the last checkpoint segment in the workspace; the
+ * first adjoint on that workspace after it starts from them instead of
+ * recomputing the segment (later adjoints, or a workspace touched by
+ * ernn_time_steps or ernn_workspace_init since, recompute it). */
 int ernn_adjoint(void* plan, const ernn_call_t* call, void* stream);
 
 /* Change the tile (V, R) and grid cap (blocks per CU, 0 = none) of the

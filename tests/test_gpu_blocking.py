@@ -317,3 +317,48 @@ def test_fused_blocked_backward_short_segments_and_snapshot(K):
     print(f"fused blocked K={K}, snapshot stride 3: dJ/dC vs unfused single-step {err:.2e}")
     assert err < 1e-5
     e1.close()
+
+
+# ------------------------------------------- stored last segment (forward -> adjoint)
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("blocking,fuse,ablk", [((1, 4, 4), True, (1, 4, 4)), ((1, 4, 4), False, (1, 4, 4)),
+                                                ((0, 0, 0), True, (0, 0, 0)), ((0, 0, 0), False, (0, 0, 0))])
+def test_stored_last_segment_is_used_once(blocking, fuse, ablk, graph):
+    """The forward pass leaves every state of the last checkpoint segment in
+    its bank; the first adjoint on that workspace starts from them, a second
+    adjoint (the bank was reused) or one on a workspace whose forward is not
+    the latest recomputes the segment.  Every order gives the gradient of the
+    fresh forward -> adjoint bit for bit.  K = 7 with 37 steps: the time-blocked
+    forward's launch before the last segment is partial (33..35)."""
+    nx, ny, nt, h = 40, 90, 37, 0.2e-3
+    shot = make_shot(nx, ny, nt, h, receivers=np.array([(0, ny - 1), (nx - 1, 0), (nx // 2, ny // 2)], np.int32))
+    sv = ernn.Survey([shot], mode="const6", ckpt_every=7, fuse=fuse, blocking=blocking, adj_blocking=ablk,
+                     use_graph=graph)
+    eng = ernn.Engine(sv, device=DEV)
+    wav = torch.tensor(sv.wavelets(), device=DEV)
+    p1 = torch.tensor(np.asarray(C_ANISO, np.float32), device=DEV)
+    p2 = p1 * 1.1
+    G = torch.tensor(np.random.default_rng(8).standard_normal(eng.trace_floats).astype(np.float32), device=DEV)
+    wa, wb = eng.acquire_workspace(), eng.acquire_workspace()
+
+    def grad(p, ws):
+        return eng.adjoint(p, None, wav, G, ws)[0].cpu().numpy()
+
+    t1 = eng.forward(p1, None, wav, wa).cpu().numpy()
+    g_first = grad(p1, wa)           # starts on the stored segment
+    g_again = grad(p1, wa)           # recomputes it
+    eng.forward(p1, None, wav, wa)
+    eng.forward(p2, None, wav, wb)
+    g_older = grad(p1, wa)           # wa's forward is not the latest: recomputes
+    g_b = grad(p2, wb)               # stored
+    g_b2 = grad(p2, wb)              # recomputed
+    torch.cuda.synchronize()
+    assert np.abs(t1).max() > 0 and np.abs(g_first).max() > 0
+    assert np.array_equal(g_first, g_again)
+    assert np.array_equal(g_first, g_older)
+    assert np.array_equal(g_b, g_b2)
+    assert not np.array_equal(g_b, g_first)
+    print(f"blocking {blocking} fuse {fuse} adj {ablk} graph {graph}: stored / recomputed gradients equal")
+    eng.release_workspace(wa)
+    eng.release_workspace(wb)
+    eng.close()
