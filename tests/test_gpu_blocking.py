@@ -398,3 +398,35 @@ def test_stored_last_segment_uneven_batch(mode, fuse):
     print(f"{mode} fuse {fuse} uneven batch: stored and recomputed last segment give equal gradients")
     eng.release_workspace(ws)
     eng.close()
+
+
+@pytest.mark.parametrize("blocking,fuse,ablk", [((1, 4, 4), True, (1, 4, 4)), ((0, 0, 0), False, (0, 0, 0))])
+def test_stored_last_segment_single_segment(blocking, fuse, ablk):
+    """nt <= K: one checkpoint segment, so the whole forward pass runs as the
+    stored last segment.  Traces equal the plan with K < nt bit for bit; the
+    stored and recomputed adjoints are equal, and match the K < nt plan's."""
+    nx, ny, nt, h = 40, 90, 23, 0.2e-3
+    shot = make_shot(nx, ny, nt, h, receivers=np.array([(0, ny - 1), (nx - 1, 0), (nx // 2, ny // 2)], np.int32))
+    G = np.random.default_rng(9).standard_normal((2, nt, 3))
+
+    def sv(K):
+        return ernn.Survey([shot], mode="const6", ckpt_every=K, fuse=fuse, blocking=blocking, adj_blocking=ablk,
+                           use_graph=True)
+    e0, t0, g0 = run_hip(sv(5), params=C_ANISO, G=G)
+    eng = ernn.Engine(sv(32), device=DEV)
+    assert eng.info.n_ckpt == 1
+    wav = torch.tensor(eng.survey.wavelets(), device=DEV)
+    p = torch.tensor(np.asarray(C_ANISO, np.float32), device=DEV)
+    Gt = torch.tensor(G.astype(np.float32).reshape(-1), device=DEV)
+    ws = eng.acquire_workspace()
+    t1 = eng.forward(p, None, wav, ws).cpu().numpy()
+    g1 = eng.adjoint(p, None, wav, Gt, ws)[0].cpu().numpy()
+    g2 = eng.adjoint(p, None, wav, Gt, ws)[0].cpu().numpy()
+    torch.cuda.synchronize()
+    assert np.array_equal(t0, t1)
+    assert np.array_equal(g1, g2)
+    err = np.abs(g1 - g0[0]).max() / np.abs(g0[0]).max()
+    print(f"one segment, blocking {blocking} fuse {fuse}: dJ/dC vs K = 5 plan {err:.2e}")
+    assert err < 1e-5
+    eng.release_workspace(ws)
+    eng.close(); e0.close()
