@@ -885,6 +885,8 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
 
   __shared__ int recmap[WB][RW][kCols];
   const bool use_map = a.record && !a.snapshot && !bl.rec_shared && bl.rec_end > bl.rec_begin;   // block-uniform
+  // the recording stores read the LDS map: holding the indices in registers
+  // (ridx, left unused) cost the (1, 4) shapes 7 VGPRs for no measured gain
   int ridx[RW][V];
   const uint32_t rbits = use_map ? map_receivers<V, RW, WB>(recmap, ridx, a.rec, bl.rec_begin, bl.rec_end,
                                                             gi0, bl.oj0 - KS, w, lane) : 0u;
