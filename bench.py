@@ -74,6 +74,9 @@ def parse():
                     help="time-blocked forward 'V,RW,KS', '0' for one step per launch, or 'auto' (autotuned)")
     ap.add_argument("--adj-blocking", default="auto",
                     help="time-blocked unfused adjoint 'V,RW,KS', '0', or 'auto' (autotuned)")
+    ap.add_argument("--ff-rw", default="auto",
+                    help="fused backward: rows per wavefront of the recompute half (the adjoint's RW, or 8 beside "
+                         "RW = 4), or 'auto' (autotuned)")
     ap.add_argument("--fuse", default="auto",
                     help="backward schedule: '1' fused single-step, '0' unfused (time-blocked when tuned), "
                          "'auto' (small grids: timed both ways)")
@@ -604,7 +607,7 @@ def main():
     import torch
     import ernn
     from ernn import configs
-    from ernn.autotune import tune_backward, tune_blocking, tune_tiles
+    from ernn.autotune import tune_backward, tune_blocking, tune_fused_recompute, tune_tiles
 
     dev = torch.device("cuda:0")
     kw = {"nt": args.nt} if args.nt else {}
@@ -661,6 +664,17 @@ def main():
                                                   steps=max(16, min(96, 40000 // cells_k)), which=1)
         log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
             f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f}; {_ranked(btimes)})")
+    if eng.info.fuse and eng.adj_blocking[2] > 1:
+        if args.ff_rw != "auto":
+            v, _, ks = eng.adj_blocking
+            eng.set_blocking(v, int(args.ff_rw), ks, which=2)
+            log(f"fused recompute rows fixed: {eng.ff_rw}")
+        elif args.autotune:
+            frw, ftimes = tune_fused_recompute(eng, params_true, field_true, model.wavelets,
+                                               steps=max(16, min(96, 40000 // cells_k)))
+            if ftimes:
+                log(f"autotune: fused recompute rows per wavefront = {frw} ("
+                    + ", ".join(f"RW {c}: {t * 1e3:.2f}" for c, t in ftimes.items()) + " us per reverse step)")
     with torch.no_grad():
         obs = ernn.ernn_traces(eng, params=params_true, field=field_true, wavelets=model.wavelets)
         gen = torch.Generator(device="cpu").manual_seed(1)
@@ -804,7 +818,7 @@ def main():
                    "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every), "graph": bool(sv.use_graph),
                    "fused_backward": bool(sv.fused),
                    "blocking": list(eng.blocking),
-                   "adj_blocking": list(eng.adj_blocking),
+                   "adj_blocking": list(eng.adj_blocking), "fused_recompute_rw": eng.ff_rw,
                    "tiles": list(eng.tiles),
                    "parallelism": "none"},
         "s_per_adam_iter": round(dt_iter, 5),

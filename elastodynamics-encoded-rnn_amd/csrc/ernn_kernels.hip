@@ -1075,13 +1075,17 @@ __global__ void __launch_bounds__(kWave * blk_waves(RW)) fwd_block_kernel(BlockA
 // latency, running them side by side overlaps the two chains.  adj_grid is a
 // multiple of 8, so both halves keep the XCD banding of their block tables.
 // --------------------------------------------------------------------------
-template <int MODE, int V, int RW, int KS>
+// RWF: rows per wavefront of the recompute half (its own block table).  On
+// small grids taller recompute blocks (RWF = 8 beside RW = 4) cut the grid so
+// that every block of the launch is resident at once.
+template <int MODE, int V, int RW, int KS, int RWF>
 __global__ void __launch_bounds__(kWave * blk_waves(RW)) fused_block_kernel(FusedBlockArgs fa) {
 #pragma clang fp contract(fast)
+  static_assert(blk_waves(RWF) == blk_waves(RW), "both halves run blocks of one size");
   if (blockIdx.x >= fa.adj_grid) {   // block-uniform
     const auto base = (const char __attribute__((address_space(4)))*)__builtin_amdgcn_kernarg_segment_ptr();
-    fwd_block_body<MODE, V, RW, KS>((KArg<BlockArgs>*)(base + offsetof(FusedBlockArgs, fwd)),
-                                    blockIdx.x - fa.adj_grid);
+    fwd_block_body<MODE, V, RWF, KS>((KArg<BlockArgs>*)(base + offsetof(FusedBlockArgs, fwd)),
+                                     blockIdx.x - fa.adj_grid);
     return;
   }
   const BlockAdjArgs& a = fa.adj;
@@ -1319,12 +1323,14 @@ hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t 
   return hipErrorInvalidValue;
 }
 
-// Fused time-blocked backward shapes (fused_block_ok in ernn_plan.cpp): the
-// small-grid shapes of both halves, constant and damage modes.
-#define ERNN_FUSED_BLOCK_CASES(X) X(1, 4, 2) X(1, 4, 4) X(1, 8, 4)
-#define ERNN_FUSED_BLOCK_CASE(v_, r_, k_)                                                             \
-  if (V == v_ && fa.adj.rw == r_ && fa.adj.ksmax == k_) {                                            \
-    hipLaunchKernelGGL((fused_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, \
+// Fused time-blocked backward shapes (V, RW, KS, RWF) (fused_block_ok and
+// fused_ff_ok in ernn_plan.cpp): the small-grid shapes, constant and damage
+// modes; the recompute half runs RWF rows per wavefront (the adjoint's RW, or
+// 8 beside RW = 4).
+#define ERNN_FUSED_BLOCK_CASES(X) X(1, 4, 2, 4) X(1, 4, 4, 4) X(1, 8, 4, 8) X(1, 4, 2, 8) X(1, 4, 4, 8)
+#define ERNN_FUSED_BLOCK_CASE(v_, r_, k_, f_)                                                         \
+  if (V == v_ && fa.adj.rw == r_ && fa.adj.ksmax == k_ && fa.fwd.rw == f_) {                        \
+    hipLaunchKernelGGL((fused_block_kernel<MODE, v_, r_, k_, f_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, \
                        fa);                                                                          \
     return hipGetLastError();                                                                        \
   }
@@ -1337,7 +1343,7 @@ static hipError_t launch_fused_block_m(const FusedBlockArgs& fa, int V, int grid
 
 // grid = adjoint grid (a multiple of 8) + forward grid
 hipError_t launch_fused_block(const FusedBlockArgs& fa, int V, int grid, hipStream_t st) {
-  if (fa.fwd.rw != fa.adj.rw || fa.fwd.ksmax != fa.adj.ksmax || fa.fwd.mode != fa.adj.mode || (fa.adj_grid & 7))
+  if (fa.fwd.ksmax != fa.adj.ksmax || fa.fwd.mode != fa.adj.mode || (fa.adj_grid & 7))
     return hipErrorInvalidValue;
   switch (fa.adj.mode) {
     case kConst6: return launch_fused_block_m<kConst6>(fa, V, grid, st);

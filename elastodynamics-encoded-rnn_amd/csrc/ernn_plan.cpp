@@ -128,6 +128,11 @@ bool fused_block_ok(int mode, int V, int RW, int KS) {
   if (mode == ERNN_PARAM_FIELD6 || V != 1) return false;
   return (RW == 4 && (KS == 2 || KS == 4)) || (RW == 8 && KS == 4);
 }
+// Rows per wavefront of the fused kernel's recompute half (ERNN_FUSED_BLOCK_CASES):
+// the adjoint's, or 8 beside the 4-row adjoint shapes.
+bool fused_ff_ok(int V, int RW, int KS, int RWF) {
+  return RWF == RW || (V == 1 && RW == 4 && RWF == 8 && (KS == 2 || KS == 4));
+}
 bool ablk_ok(int mode, bool fuse, int V, int RW, int KS) {
   return fuse ? fused_block_ok(mode, V, RW, KS) : adj_block_ok(mode, V, RW, KS);
 }
@@ -163,6 +168,7 @@ struct Plan {
   int n_coef = 0, n_acc = 0;
   bool use_graph = false;
   bool fuse = false;
+  int ff_rw = 0;   // rows per wavefront of the fused recompute half (0: the adjoint's)
   std::vector<ernn_instance_t> inst_h;
   std::vector<ernn_source_t> src_h;
   std::vector<ernn_receiver_t> rec_h;
@@ -455,7 +461,7 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
   ERNN_TRY(dev_upload(&bt->d_blocks, blocks));
   ERNN_TRY(dev_upload(&bt->d_src, sev));
   ERNN_TRY(dev_upload(&bt->d_rec, rev));
-  if (adjoint && p->fuse) return build_blocking(p, &p->blk_ff, V, RW, KS);
+  if (adjoint && p->fuse) return build_blocking(p, &p->blk_ff, V, p->ff_rw ? p->ff_rw : RW, KS);
   return ERNN_OK;
 }
 
@@ -1166,11 +1172,23 @@ int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc) {
 }
 
 int ernn_plan_set_blocking(void* plan, int which, int v, int rw, int ks) {
-  if (!plan || (which != 0 && which != 1)) return fail(ERNN_E_INVALID, "bad plan or which");
+  if (!plan || which < 0 || which > 2) return fail(ERNN_E_INVALID, "bad plan or which");
   Plan* p = static_cast<Plan*>(plan);
+  if (which == 2) {   // the fused kernel's recompute half
+    if (!p->fuse || p->blk_adj.KS <= 1 || v != p->blk_adj.V || ks != p->blk_adj.KS ||
+        !fused_ff_ok(v, p->blk_adj.RW, ks, rw))
+      return fail(ERNN_E_INVALID, "fused recompute shape: needs a fuse plan with time-blocked adjoint (V, RW, KS); "
+                                  "(V, RW, KS) itself or (1, 8, KS) beside RW = 4, KS in {2,4}");
+    ERNN_HIP(hipSetDevice(p->device));
+    ERNN_HIP(hipDeviceSynchronize());
+    drop_graphs(p);
+    p->ff_rw = rw;
+    return build_blocking(p, &p->blk_ff, v, rw, ks);
+  }
   ERNN_HIP(hipSetDevice(p->device));
   ERNN_HIP(hipDeviceSynchronize());
   drop_graphs(p);
+  if (which == 1) p->ff_rw = 0;   // a new adjoint shape resets the recompute half to it
   return build_blocking(p, which == 0 ? &p->blk : &p->blk_adj, v, rw, ks);
 }
 

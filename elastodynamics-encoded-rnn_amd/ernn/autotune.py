@@ -156,6 +156,34 @@ def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int =
     return (best[0], best[1], best[2], times[best], times)
 
 
+def tune_fused_recompute(engine, params, field, wavelets, steps: int = 64, rounds: int = 5) -> tuple:
+    """On a fuse plan whose time-blocked adjoint has 4-row bands (V = 1, KS in
+    {2, 4}), time the fused kernel with its recompute half at the adjoint's
+    shape and at 8 rows per wavefront, per reverse step, in interleaved
+    rounds; keep the faster.  The taller recompute blocks cut the launch's
+    block count, so that on small grids every block is resident at once.
+
+    Returns (rw, {rw: ms_per_step}); ({}: not applicable, nothing timed)."""
+    v, rw, ks = engine.adj_blocking
+    if not engine.info.fuse or v != 1 or rw != 4 or ks not in (2, 4):
+        return engine.ff_rw, {}
+    ws = engine.acquire_workspace()
+    traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)
+    launches = max(2, steps // ks)
+    samples = {c: [] for c in (rw, 8)}
+    for _ in range(rounds):
+        for c in samples:
+            engine.set_blocking(v, c, ks, which=2)
+            engine.time_steps(4, launches, params, field, wavelets, ws, traces, traces)   # warm-up
+            samples[c].append(engine.time_steps(4, launches, params, field, wavelets, ws, traces, traces)
+                              / (launches * ks))
+    times = {c: sorted(t)[len(t) // 2] for c, t in samples.items()}
+    best = min(times, key=times.get)
+    engine.set_blocking(v, best, ks, which=2)
+    engine.release_workspace(ws)
+    return best, times
+
+
 def tune_backward(survey, params, field, wavelets, steps: int = 48, rounds: int = 3) -> tuple:
     """Choose the backward schedule of a survey by timing the candidates on
     the real problem, per reverse step (adjoint + recompute):

@@ -39,6 +39,8 @@ class Engine:
             K = choose_ckpt(survey)
         self._build(K)
         self._pool: List[torch.Tensor] = []
+        # rows per wavefront of the fused kernel's recompute half (set_blocking which 2)
+        self.ff_rw = int(self.info.ablk_rw) if (self.info.fuse and self.info.ablk_ks > 1) else 0
 
     # ------------------------------------------------------------------ plan
     def _build(self, K: int) -> None:
@@ -216,11 +218,16 @@ class Engine:
         return (int(i.ablk_v), int(i.ablk_rw), int(i.ablk_ks))
 
     def set_blocking(self, v: int, rw: int, ks: int, which: int = 0) -> None:
-        """Time-blocked forward (which 0) or unfused adjoint (which 1) with ks
-        steps per launch (ks <= 1: one step per launch)."""
+        """Time-blocked forward (which 0) or adjoint (which 1) with ks steps
+        per launch (ks <= 1: one step per launch); which 2: rows per wavefront
+        rw of the fused kernel's recompute half (v, ks: the adjoint's)."""
         with torch.cuda.device(self.device):
             _lib.check(_lib.lib().ernn_plan_set_blocking(self._plan, which, v, rw, ks), "ernn_plan_set_blocking")
             _lib.check(_lib.lib().ernn_plan_info(self._plan, C.byref(self.info)), "ernn_plan_info")
+        if which == 2:
+            self.ff_rw = rw
+        elif which == 1:
+            self.ff_rw = rw if ks > 1 else 0
 
     def set_tiling(self, which: int, v: int, r: int, bpc: int = 0) -> None:
         with torch.cuda.device(self.device):

@@ -200,9 +200,13 @@ int ernn_plan_set_tiling(void* plan, int which, int v, int r, int bpc);
 
 /* Switch the forward (which = 0) or the adjoint (which = 1) to ks
  * steps per launch on blocks of the given shape (ks <= 1: one step per
- * launch); same limits as ernn_desc_t.blk_* / ablk_*.  No counterpart in the
- * reference: its time loop (PI:268-305) and autograd replay run one step at a
- * time.  Synchronizes the device and drops captured graphs. */
+ * launch); same limits as ernn_desc_t.blk_* / ablk_*.  which = 2 (fuse plans
+ * with a time-blocked adjoint) sets the rows per wavefront of the fused
+ * kernel's recompute half: v and ks must be the adjoint's, rw the adjoint's
+ * or 8 beside an adjoint RW of 4 with ks in {2,4}; setting the adjoint shape
+ * resets it to the adjoint's.  No counterpart in the reference: its time loop
+ * (PI:268-305) and autograd replay run one step at a time.  Synchronizes the
+ * device and drops captured graphs. */
 int ernn_plan_set_blocking(void* plan, int which, int v, int rw, int ks);
 
 /* Time `steps` forward (which = 0), adjoint (which = 1) or fused adjoint +
