@@ -3,9 +3,11 @@
 One step = one Adam iteration of the waveform-misfit inversion
 (Parameter_inversion/Inversion_adam.py:436-450): forward recurrence over all
 shots, MSE on receiver traces, gradient (checkpointed recompute + reverse-time
-adjoint), Adam update.  Default workload: single-512 (512^2 grid, 2000 steps,
-one shot, 256 receivers, checkpoint every 50), FP32, synthetic data (observed
-traces = the same solver at the true constants + 1 % RMS seeded noise).
+adjoint), Adam update.  Default workload: damage-1024 (1024^2 grid, 3000
+steps, 8 top-edge shots, 512 receivers on the top and bottom rows, damage
+field D(x) of three ellipses scaling all six constants; the gradient covers
+the six constants and the 1 Mi-cell D field), FP32, synthetic data (observed
+traces = the same solver at the true constants and D + 1 % RMS seeded noise).
 
 Also reported on the same line:
   autograd_baseline  the same iteration through the reference's own method
@@ -15,7 +17,15 @@ Also reported on the same line:
                      hoisted; the faster one is the denominator of `speedup`.
   roofline           dominant kernel's algorithmic bytes per launch / its average
                      launch duration (hipEvents on the launch stream, back-to-back
-                     launches), against 8 TB/s HBM.
+                     launches), against 8 TB/s HBM.  Algorithmic bytes per cell
+                     and step: each state read once and written once, per-cell
+                     fields read once (forward 24 / 28 / 48 B for six constants /
+                     damage field / six fields, adjoint 32 / 36 / 56 B); a launch
+                     of KS steps is credited KS times that, the recompute half of
+                     a fused launch nothing (reported beside it).  Time-blocked
+                     launches move fewer bytes than that (states stay in
+                     registers between their steps), so the fraction can exceed
+                     1; `traffic` is the measured HBM bytes per launch.
   cpu_baseline       the op-for-op CPU port (oracle/plain_torch.py, autograd) on
                      the host cores over a bounded window of steps.
 
@@ -40,13 +50,15 @@ HBM_PEAK_GBS = 8000.0
 # cycles each).  The kernels are built without packed FP32, so this is also
 # their FP32 FMA peak (78.6 TFLOP/s, half the packed 157.3).
 VALU_PEAK_WIPS = 256 * 2.4e9
-# algorithmic FP32 bytes per cell and launch: forward reads u^n, u^{n-1} and
-# writes u^{n+1} (2 components each); the adjoint reads lam^{m+1}, lam^{m+2},
-# u^m and writes lam^m; per-cell fields add their reads / accumulator traffic.
+# Algorithmic FP32 bytes per cell and step (the minimum of one pass over the
+# grid per step): the forward reads u^n, u^{n-1} and writes u^{n+1} (2
+# components each); the adjoint reads lam^{m+1}, lam^{m+2}, u^m and writes
+# lam^m; per-cell fields add one read each (damage field: D; six fields: the
+# six C_p).  Gradient accumulators are amortised and not counted.
 BYTES_PER_CELL = {
     ("const6", 0): 24, ("const6", 1): 32,
-    ("dfield", 0): 28, ("dfield", 1): 44,      # + D read; adj + D + fp32 accumulator r/w
-    ("field6", 0): 48, ("field6", 1): 104,
+    ("dfield", 0): 28, ("dfield", 1): 36,
+    ("field6", 0): 48, ("field6", 1): 56,
 }
 
 
@@ -65,7 +77,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="single-512")
+    ap.add_argument("--workload", default="damage-1024")
     ap.add_argument("--nt", type=int, default=0, help="override the step count (profiling runs)")
     ap.add_argument("--graph", type=int, default=1, help="replay passes as captured hipGraphs")
     ap.add_argument("--autotune", type=int, default=1)
@@ -629,7 +641,7 @@ def main():
         sv.fuse = bool(int(args.fuse))
     elif args.autotune and not args.tiles and sv.fused and sv.mode != "field6":
         # small grids: the fused single-step backward vs the time-blocked one
-        fuse, _, bt = tune_backward(sv, params_true, field_true, torch.tensor(sv.wavelets(), device=dev),
+        fuse, _, _, bt = tune_backward(sv, params_true, field_true, torch.tensor(sv.wavelets(), device=dev),
                                     steps=max(16, min(96, 40000 // cells_k)))
         sv.fuse = fuse
         log(f"autotune: backward {'fused' if fuse else 'unfused'} (fused {bt['fused'] * 1e3:.2f} vs unfused "
@@ -731,36 +743,42 @@ def main():
     cells = sum(s.nx * s.ny for s in sv.shots)
     bf, ba = BYTES_PER_CELL[(sv.mode, 0)], BYTES_PER_CELL[(sv.mode, 1)]
     nt, K = sv.nt_max, int(eng.info.ckpt_every)
-    # (ms per launch, launches per iteration, algorithmic bytes per cell per launch)
+    # (ms per launch, launches per iteration, algorithmic bytes per cell per
+    # launch, recompute bytes per cell per launch not counted): a launch of KS
+    # steps is credited KS steps of algorithmic bytes; the recompute half of a
+    # fused launch is overhead and is reported apart.
     kern = {}
-    if KS > 1:   # the time-blocked launch reads two states once and writes its last two: bf + 8
+    if KS > 1:
         n_blk = math.ceil(nt / KS) + (math.ceil(K / KS) if sv.fused else math.ceil(nt / KS))
-        kern["fwd_block_kernel"] = (blk_ms, n_blk, bf + 8)
+        kern["fwd_block_kernel"] = (blk_ms, n_blk, bf * KS, 0)
     else:
-        kern["fwd_step_kernel"] = (fwd_ms, nt + K if sv.fused else 2 * nt, bf)
-    if sv.fused and AKS > 1:   # fused time-blocked backward: adjoint half + recompute half (every state stored)
-        abytes = 16 + 8 * AKS + 16 + (12 if sv.mode == "dfield" else 0)
-        kern["fused_block_kernel"] = (ablk_ms, math.ceil(nt / AKS), abytes + (bf - 8) + 8 * AKS)
+        kern["fwd_step_kernel"] = (fwd_ms, nt + K if sv.fused else 2 * nt, bf, 0)
+    if sv.fused and AKS > 1:   # fused time-blocked backward: AKS adjoint steps + AKS recompute steps
+        kern["fused_block_kernel"] = (ablk_ms, math.ceil(nt / AKS), ba * AKS, bf * AKS)
     elif sv.fused:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
-        kern["fused_step_kernel"] = (fused_ms, nt, bf + ba)
-    elif AKS > 1:   # reads lam^{m+1}, lam^{m+2} (+ D), u^m per step, writes two states, acc r/w once
-        abytes = 16 + 8 * AKS + 16 + (12 if sv.mode == "dfield" else 0)
+        kern["fused_step_kernel"] = (fused_ms, nt, ba, bf)
+    elif AKS > 1:
         kern["adj_block_kernel"] = (ablk_ms, sum(math.ceil(min(K, nt - s * K) / AKS)
-                                                for s in range(math.ceil(nt / K))), abytes)
+                                                for s in range(math.ceil(nt / K))), ba * AKS, 0)
     else:
-        kern["adj_step_kernel"] = (adj_ms, nt, ba)
+        kern["adj_step_kernel"] = (adj_ms, nt, ba, 0)
     name = max(kern, key=lambda k: kern[k][0] * kern[k][1])
-    ms, launches, bpc = kern[name]
+    ms, launches, bpc, rbpc = kern[name]
     achieved = cells * bpc / (ms * 1e-3) / 1e9
     # HBM bytes per launch from the committed counter passes (FETCH_SIZE and
     # WRITE_SIZE, calibrated on a copy kernel), taken with the tile shapes
     # recorded beside them (traffic_tiles; within a few percent across tile
     # shapes, since every shape reads each field once per launch).
+    # The entry is used only when it was taken with the shape the kernel has
+    # now (time-blocked forward: blocking; time-blocked adjoint: adj_blocking;
+    # fused time-blocked: adj_blocking + recompute rows; one-step: the tiles).
+    kshape = {"fwd_block_kernel": list(eng.blocking), "adj_block_kernel": list(eng.adj_blocking),
+              "fused_block_kernel": list(eng.adj_blocking) + [eng.ff_rw]}.get(name, list(eng.tiles))
     traffic, traffic_tiles, valu = None, None, None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tpath):
         entry = json.load(open(tpath)).get(f"{wl.name}/{name}")
-        if entry:
+        if entry and entry.get("shape") == kshape:
             traffic, traffic_tiles = entry["bytes"], entry["tiles"]
             if entry.get("valu_insts"):   # SQ_INSTS_VALU per launch, same counter runs
                 rate = entry["valu_insts"] / (ms * 1e-3)
@@ -769,9 +787,9 @@ def main():
                         "frac": round(rate / VALU_PEAK_WIPS, 4)}
     roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "traffic_tiles": traffic_tiles, "valu": valu,
-                "bytes_per_cell": bpc, "cells_per_launch": cells, "avg_launch_us": round(ms * 1e3, 3),
-                "launches_per_iter": launches,
+                "traffic_tiles": traffic_tiles, "kernel_shape": kshape, "valu": valu,
+                "bytes_per_cell": bpc, "recompute_bytes_per_cell": rbpc, "cells_per_launch": cells,
+                "avg_launch_us": round(ms * 1e3, 3), "launches_per_iter": launches,
                 "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),
                 "fused_step_us": round(fused_ms * 1e3, 3) if fused_ms else None,
                 "fwd_block_us_per_step": round(blk_ms / KS * 1e3, 3) if blk_ms else None,

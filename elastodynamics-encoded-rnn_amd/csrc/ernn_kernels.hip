@@ -1129,11 +1129,16 @@ __global__ void pack_fields_kernel(const CallDev* call, const InstDev* inst, con
   }
 }
 
-// grad_params[set][p] = sum over the set's tiles of partial[t][p]
-// per_rec partial slots per record (tile: 1, block: its wavefronts)
-template <typename Rec>
-__global__ void finalize_const_kernel(const CallDev* call, int64_t partial_off,
-                                      const Rec* __restrict__ tiles, int n_tiles, int per_rec) {
+// grad_params[set][p] = sum of the set's partial slots.  rec_first[b] is
+// the first record (tile or block) of instance b, records of one instance
+// being contiguous; each record holds per_rec slots (tile: 1, block: its
+// wavefronts).  The set's slots are numbered locally, instance after
+// instance in the set's order, and thread t sums local slots t, t + kBlock,
+// ..., so the order of the sum depends only on the set's own instances: an
+// instance batched with others gets the gradient of its lone run bit for bit.
+__global__ void finalize_const_kernel(const CallDev* call, int64_t partial_off, const int* __restrict__ set_inst_list,
+                                      const int* __restrict__ set_inst_begin, const int* __restrict__ rec_first,
+                                      int per_rec) {
   const int set = blockIdx.x / 6, p = blockIdx.x % 6;
   const double* partial = reinterpret_cast<const double*>(call->ws) + partial_off;
   // dJ/dC = (s0, s1, s2, s3 + 2 s6, s4 + 2 s7, s5 + s2)
@@ -1141,11 +1146,17 @@ __global__ void finalize_const_kernel(const CallDev* call, int64_t partial_off,
   const double f2 = (p == 5) ? 1.0 : 2.0;
   __shared__ double red[kBlock];
   double s = 0.0;
-  for (int t = threadIdx.x; t < n_tiles * per_rec; t += blockDim.x)
-    if (tiles[t / per_rec].set == set) {
-      const double* pt = partial + (int64_t)t * kPartials;
+  int64_t base = 0;   // local index of the current instance's first slot
+  for (int u = set_inst_begin[set]; u < set_inst_begin[set + 1]; ++u) {
+    const int b = set_inst_list[u];
+    const int64_t lo = (int64_t)rec_first[b] * per_rec;
+    const int64_t n = (int64_t)(rec_first[b + 1] - rec_first[b]) * per_rec;
+    for (int64_t k = (((int64_t)threadIdx.x - base) % kBlock + kBlock) % kBlock; k < n; k += kBlock) {
+      const double* pt = partial + (lo + k) * kPartials;
       s += (p2 >= 0) ? pt[p] + f2 * pt[p2] : pt[p];
     }
+    base += n;
+  }
   red[threadIdx.x] = s;
   __syncthreads();
   for (int off = kBlock / 2; off > 0; off >>= 1) {
@@ -1402,17 +1413,12 @@ hipError_t launch_pack_fields(const CallDev* call, const InstDev* inst, const in
   return hipGetLastError();
 }
 
-hipError_t launch_finalize_const(const CallDev* call, int64_t partial_off, const TileDev* tiles,
-                                 int n_tiles, int n_sets, hipStream_t st) {
-  hipLaunchKernelGGL(finalize_const_kernel<TileDev>, dim3(n_sets * 6), dim3(kBlock), 0, st, call, partial_off,
-                     tiles, n_tiles, 1);
-  return hipGetLastError();
-}
-
-hipError_t launch_finalize_const_blocks(const CallDev* call, int64_t partial_off, const BlockDev* blocks,
-                                        int n_blocks, int waves, int n_sets, hipStream_t st) {
-  hipLaunchKernelGGL(finalize_const_kernel<BlockDev>, dim3(n_sets * 6), dim3(kBlock), 0, st, call, partial_off,
-                     blocks, n_blocks, waves);
+// per_rec: partial slots per record (tile 1, block its wavefront count)
+hipError_t launch_finalize_const(const CallDev* call, int64_t partial_off, const int* set_inst_list,
+                                 const int* set_inst_begin, const int* rec_first, int per_rec, int n_sets,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(finalize_const_kernel, dim3(n_sets * 6), dim3(kBlock), 0, st, call, partial_off, set_inst_list,
+                     set_inst_begin, rec_first, per_rec);
   return hipGetLastError();
 }
 

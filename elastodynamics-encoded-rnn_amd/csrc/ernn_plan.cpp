@@ -40,16 +40,15 @@ hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t
 hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st);
 hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st);
 hipError_t launch_fused_block(const FusedBlockArgs& fa, int V, int grid, hipStream_t st);
-hipError_t launch_finalize_const_blocks(const CallDev* call, int64_t partial_off, const BlockDev* blocks,
-                                        int n_blocks, int waves, int n_sets, hipStream_t st);
 hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st);
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t st);
 hipError_t launch_fill_ws(const CallDev* call, int64_t off, int64_t n, hipStream_t st);
 hipError_t launch_pack_fields(const CallDev* call, const InstDev* inst, const int* set_inst,
                               const int64_t* set_ext_off, int n_coef, int set, int64_t ncell,
                               hipStream_t st);
-hipError_t launch_finalize_const(const CallDev* call, int64_t partial_off, const TileDev* tiles,
-                                 int n_tiles, int n_sets, hipStream_t st);
+hipError_t launch_finalize_const(const CallDev* call, int64_t partial_off, const int* set_inst_list,
+                                 const int* set_inst_begin, const int* rec_first, int per_rec, int n_sets,
+                                 hipStream_t st);
 hipError_t launch_finalize_field(const CallDev* call, const InstDev* inst, const int* set_inst_list,
                                  const int* set_inst_begin, const int64_t* set_grad_off, int n_acc,
                                  int set, int64_t ncell, hipStream_t st);
@@ -151,6 +150,7 @@ struct Tiling {
   TileDev* d_tiles = nullptr;
   SrcEv* d_src = nullptr;
   RecEv* d_rec = nullptr;
+  int* d_inst_first = nullptr;   // [n_inst + 1]: first tile of each instance
 };
 
 struct BlockTiling {
@@ -159,6 +159,7 @@ struct BlockTiling {
   BlockDev* d_blocks = nullptr;
   SrcEv* d_src = nullptr;
   RecEv* d_rec = nullptr;
+  int* d_inst_first = nullptr;   // [n_inst + 1]: first block of each instance
 };
 
 struct Plan {
@@ -168,7 +169,7 @@ struct Plan {
   int n_coef = 0, n_acc = 0;
   bool use_graph = false;
   bool fuse = false;
-  int ff_rw = 0;   // rows per wavefront of the fused recompute half (0: the adjoint's)
+  int ff_rw = 0;   // rows per wavefront of the fused recompute half (0: no fused time-blocked backward)
   std::vector<ernn_instance_t> inst_h;
   std::vector<ernn_source_t> src_h;
   std::vector<ernn_receiver_t> rec_h;
@@ -250,6 +251,7 @@ void free_tiling(Tiling& t) {
   if (t.d_tiles) (void)hipFree(t.d_tiles);
   if (t.d_src) (void)hipFree(t.d_src);
   if (t.d_rec) (void)hipFree(t.d_rec);
+  if (t.d_inst_first) (void)hipFree(t.d_inst_first);
   t = Tiling{};
 }
 
@@ -257,6 +259,7 @@ void free_blocking(BlockTiling& b) {
   if (b.d_blocks) (void)hipFree(b.d_blocks);
   if (b.d_src) (void)hipFree(b.d_src);
   if (b.d_rec) (void)hipFree(b.d_rec);
+  if (b.d_inst_first) (void)hipFree(b.d_inst_first);
   b = BlockTiling{};
 }
 
@@ -289,9 +292,11 @@ int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
   tiles.reserve((size_t)nt64);
   std::vector<SrcEv> sev;
   std::vector<RecEv> rev;
+  std::vector<int> first(p->n_inst + 1, 0);
   const int W = kWave * V;
   for (int b = 0; b < p->n_inst; ++b) {
     const ernn_instance_t& I = p->inst_h[b];
+    first[b] = (int)tiles.size();
     const int nti = (I.nx + R - 1) / R, ntj = (I.ny + W - 1) / W;
     // bucket this instance's sources and receivers by local tile, sorted by (i, j)
     // key: (local tile, row, column); value: index into the source / receiver slice
@@ -341,43 +346,45 @@ int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
       }
     }
   }
-  free_tiling(*t);
-  t->V = V;
-  t->R = R;
-  t->bpc = bpc;
-  t->n_tiles = (int)tiles.size();
+  first[p->n_inst] = (int)tiles.size();
+  // The new tables are uploaded into a temporary and swapped in only when
+  // every upload succeeded: a failure leaves the plan's tiling as it was.
+  Tiling nt;
+  nt.V = V;
+  nt.R = R;
+  nt.bpc = bpc;
+  nt.n_tiles = (int)tiles.size();
   // 8 XCDs, each owning ceil(n_tiles / 8) tiles; a wavefront per tile unless capped
-  const int64_t chunk = (t->n_tiles + 7) / 8;
+  const int64_t chunk = (nt.n_tiles + 7) / 8;
   int64_t per_xcd = std::max<int64_t>((chunk + kWavesPerBlock - 1) / kWavesPerBlock, 1);
   const int64_t cap = std::max<int64_t>((int64_t)bpc * p->n_cu / 8, 1);
-  t->loop = bpc > 0 && cap < per_xcd;
-  if (t->loop) per_xcd = cap;
-  t->blocks = (int)(8 * per_xcd);
-  const int64_t fchunk = (2 * (int64_t)t->n_tiles + 7) / 8;
+  nt.loop = bpc > 0 && cap < per_xcd;
+  if (nt.loop) per_xcd = cap;
+  nt.blocks = (int)(8 * per_xcd);
+  const int64_t fchunk = (2 * (int64_t)nt.n_tiles + 7) / 8;
   int64_t fper = std::max<int64_t>((fchunk + kWavesPerBlock - 1) / kWavesPerBlock, 1);
   if (bpc > 0) fper = std::min<int64_t>(fper, cap);
-  t->fused_blocks = (int)(8 * fper);
-  ERNN_TRY(dev_upload(&t->d_tiles, tiles));
-  ERNN_TRY(dev_upload(&t->d_src, sev));
-  ERNN_TRY(dev_upload(&t->d_rec, rev));
+  nt.fused_blocks = (int)(8 * fper);
+  int r = dev_upload(&nt.d_tiles, tiles);
+  if (r == ERNN_OK) r = dev_upload(&nt.d_src, sev);
+  if (r == ERNN_OK) r = dev_upload(&nt.d_rec, rev);
+  if (r == ERNN_OK) r = dev_upload(&nt.d_inst_first, first);
+  if (r != ERNN_OK) {
+    free_tiling(nt);
+    return r;
+  }
+  free_tiling(*t);
+  *t = nt;
   return ERNN_OK;
 }
 
-// Output tiles of (blk_waves(RW) RW - 2 KS) x (64 V - 2 KS) cells per instance, row-major;
-// each block's source list covers its whole region (the halo cells' states
-// feed the later steps), its receiver list only its output tile.
-int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
-  const bool adjoint = bt == &p->blk_adj;
-  if (KS <= 1) {
-    free_blocking(*bt);
-    if (adjoint) free_blocking(p->blk_ff);
-    return ERNN_OK;
-  }
-  if (!adjoint && !block_ok(p->mode, V, RW, KS))
-    return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
-                                "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8), "
-                                "or be (1, 8, 5)");
-  if (adjoint && !ablk_ok(p->mode, p->fuse, V, RW, KS)) return fail(ERNN_E_INVALID, kAblkShapes);
+// Block table of a time-blocked shape, built into *out (empty on entry; left
+// empty on failure).  Output tiles of (blk_waves(RW) RW - 2 KS) x (64 V - 2 KS)
+// cells per instance, row-major; each block's source list covers its whole
+// region (the halo cells' states feed the later steps).  Receivers: a
+// forward-style table (adjoint = false) lists those inside the output tile
+// (recording), an adjoint table those inside the whole region (injection).
+int make_blocking(const Plan* p, bool adjoint, int V, int RW, int KS, BlockTiling* out) {
   const int OR = blk_waves(RW) * RW - 2 * KS, OC = kWave * V - 2 * KS;
   std::vector<BlockDev> blocks;
   std::vector<SrcEv> sev;
@@ -388,8 +395,10 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
     if (a.i != b.i) return a.i < b.i;
     return a.j < b.j;
   };
+  std::vector<int> first(p->n_inst + 1, 0);
   for (int b = 0; b < p->n_inst; ++b) {
     const ernn_instance_t& I = p->inst_h[b];
+    first[b] = (int)blocks.size();
     const int nbi = (I.nx + OR - 1) / OR, nbj = (I.ny + OC - 1) / OC;
     std::vector<Ev> se, re;
     for (int e = 0; e < I.src_count; ++e) {
@@ -449,19 +458,73 @@ int build_blocking(Plan* p, BlockTiling* bt, int V, int RW, int KS) {
       }
     }
   }
+  first[p->n_inst] = (int)blocks.size();
   if (blocks.size() > ((size_t)1 << 30)) return fail(ERNN_E_CAPACITY, "too many blocks");
   if (adjoint && (int64_t)blocks.size() * blk_waves(RW) > p->partial_cap)
     return fail(ERNN_E_CAPACITY, "time-blocked adjoint needs more partial slots than the workspace holds");
-  free_blocking(*bt);
-  bt->V = V;
-  bt->RW = RW;
-  bt->KS = KS;
-  bt->n_blocks = (int)blocks.size();
-  bt->grid = 8 * ((bt->n_blocks + 7) / 8);
-  ERNN_TRY(dev_upload(&bt->d_blocks, blocks));
-  ERNN_TRY(dev_upload(&bt->d_src, sev));
-  ERNN_TRY(dev_upload(&bt->d_rec, rev));
-  if (adjoint && p->fuse) return build_blocking(p, &p->blk_ff, V, p->ff_rw ? p->ff_rw : RW, KS);
+  BlockTiling& nb = *out;
+  nb.V = V;
+  nb.RW = RW;
+  nb.KS = KS;
+  nb.n_blocks = (int)blocks.size();
+  nb.grid = 8 * ((nb.n_blocks + 7) / 8);
+  int r = dev_upload(&nb.d_blocks, blocks);
+  if (r == ERNN_OK) r = dev_upload(&nb.d_src, sev);
+  if (r == ERNN_OK) r = dev_upload(&nb.d_rec, rev);
+  if (r == ERNN_OK) r = dev_upload(&nb.d_inst_first, first);
+  if (r != ERNN_OK) free_blocking(nb);
+  return r;
+}
+
+const char* const kBlkShapes =
+    "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS "
+    "(damage field / six fields: not V = 2 with RW = 8), or be (1, 8, 5)";
+
+// The time-blocked forward (KS <= 1: one step per launch).  The plan keeps
+// its old table if the new one cannot be built.
+int set_fwd_blocking(Plan* p, int V, int RW, int KS) {
+  if (KS <= 1) {
+    free_blocking(p->blk);
+    return ERNN_OK;
+  }
+  if (!block_ok(p->mode, V, RW, KS)) return fail(ERNN_E_INVALID, kBlkShapes);
+  BlockTiling nb;
+  ERNN_TRY(make_blocking(p, false, V, RW, KS, &nb));
+  free_blocking(p->blk);
+  p->blk = nb;
+  return ERNN_OK;
+}
+
+// The time-blocked adjoint (KS <= 1: one step per launch) and, on fuse plans,
+// the block table of the fused kernel's recompute half with RWF rows per
+// wavefront (0: the adjoint's RW).  Both tables are built first and replace
+// the plan's together, so a failure leaves the plan as it was.
+int set_adj_blocking(Plan* p, int V, int RW, int KS, int RWF) {
+  if (KS <= 1) {
+    free_blocking(p->blk_adj);
+    free_blocking(p->blk_ff);
+    p->ff_rw = 0;
+    return ERNN_OK;
+  }
+  if (!ablk_ok(p->mode, p->fuse, V, RW, KS)) return fail(ERNN_E_INVALID, kAblkShapes);
+  const int rwf = RWF ? RWF : RW;
+  if (p->fuse && !fused_ff_ok(V, RW, KS, rwf))
+    return fail(ERNN_E_INVALID, "fused recompute rows per wavefront: the adjoint's RW, or 8 beside RW = 4 with "
+                                "KS in {2,4}");
+  BlockTiling na, nf;
+  ERNN_TRY(make_blocking(p, true, V, RW, KS, &na));
+  if (p->fuse) {
+    const int r = make_blocking(p, false, V, rwf, KS, &nf);
+    if (r != ERNN_OK) {
+      free_blocking(na);
+      return r;
+    }
+  }
+  free_blocking(p->blk_adj);
+  free_blocking(p->blk_ff);
+  p->blk_adj = na;
+  p->blk_ff = nf;
+  p->ff_rw = p->fuse ? rwf : 0;
   return ERNN_OK;
 }
 
@@ -505,6 +568,9 @@ int validate(const ernn_desc_t* d) {
                                 "or be (1, 8, 5)");
   if (d->ablk_ks > 1 && !ablk_ok(d->param_mode, d->fuse != 0, d->ablk_v, d->ablk_rw, d->ablk_ks))
     return fail(ERNN_E_INVALID, kAblkShapes);
+  if (d->ablk_ff_rw != 0 && (d->ablk_ks <= 1 || !d->fuse || !fused_ff_ok(d->ablk_v, d->ablk_rw, d->ablk_ks, d->ablk_ff_rw)))
+    return fail(ERNN_E_INVALID, "ablk_ff_rw: 0, or on a fuse plan with a time-blocked adjoint the adjoint's RW, or 8 "
+                                "beside RW = 4 with KS in {2,4}");
   if (!d->inst) return fail(ERNN_E_INVALID, "null instance array");
   if (d->n_src < 0 || d->n_rec < 0) return fail(ERNN_E_INVALID, "negative source/receiver count");
   if ((d->n_src > 0 && !d->src) || (d->n_rec > 0 && !d->rec))
@@ -686,8 +752,8 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   ERNN_TRY(build_tiling(p, &p->fwd, d->fwd_v ? d->fwd_v : 1, d->fwd_r ? d->fwd_r : 4, d->fwd_bpc));
   const int adj_r0 = (d->param_mode == ERNN_PARAM_FIELD6) ? 2 : 4;
   ERNN_TRY(build_tiling(p, &p->adj, d->adj_v ? d->adj_v : 1, d->adj_r ? d->adj_r : adj_r0, d->adj_bpc));
-  ERNN_TRY(build_blocking(p, &p->blk, d->blk_v, d->blk_rw, d->blk_ks));
-  ERNN_TRY(build_blocking(p, &p->blk_adj, d->ablk_v, d->ablk_rw, d->ablk_ks));
+  ERNN_TRY(set_fwd_blocking(p, d->blk_v, d->blk_rw, d->blk_ks));
+  ERNN_TRY(set_adj_blocking(p, d->ablk_v, d->ablk_rw, d->ablk_ks, d->ablk_ff_rw));
   ERNN_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
   return ERNN_OK;
 }
@@ -1004,11 +1070,11 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
   }
   if (p->mode != ERNN_PARAM_FIELD6) {
     if (p->blk_adj.KS > 1)
-      ERNN_HIP(launch_finalize_const_blocks(p->d_call, p->partial_off, p->blk_adj.d_blocks, p->blk_adj.n_blocks,
-                                            blk_waves(p->blk_adj.RW),
-                                            p->n_sets, st));
+      ERNN_HIP(launch_finalize_const(p->d_call, p->partial_off, p->d_set_list, p->d_set_begin,
+                                     p->blk_adj.d_inst_first, blk_waves(p->blk_adj.RW), p->n_sets, st));
     else
-      ERNN_HIP(launch_finalize_const(p->d_call, p->partial_off, p->adj.d_tiles, p->adj.n_tiles, p->n_sets, st));
+      ERNN_HIP(launch_finalize_const(p->d_call, p->partial_off, p->d_set_list, p->d_set_begin, p->adj.d_inst_first, 1,
+                                     p->n_sets, st));
   }
   if (p->n_acc) {
     for (int s = 0; s < p->n_sets; ++s) {
@@ -1127,6 +1193,7 @@ int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
   info->ablk_rw = p->blk_adj.RW;
   info->ablk_ks = p->blk_adj.KS;
   info->ablk_blocks = p->blk_adj.n_blocks;
+  info->ablk_ff_rw = p->ff_rw;
   return ERNN_OK;
 }
 
@@ -1182,14 +1249,18 @@ int ernn_plan_set_blocking(void* plan, int which, int v, int rw, int ks) {
     ERNN_HIP(hipSetDevice(p->device));
     ERNN_HIP(hipDeviceSynchronize());
     drop_graphs(p);
+    BlockTiling nf;
+    ERNN_TRY(make_blocking(p, false, v, rw, ks, &nf));
+    free_blocking(p->blk_ff);
+    p->blk_ff = nf;
     p->ff_rw = rw;
-    return build_blocking(p, &p->blk_ff, v, rw, ks);
+    return ERNN_OK;
   }
   ERNN_HIP(hipSetDevice(p->device));
   ERNN_HIP(hipDeviceSynchronize());
   drop_graphs(p);
-  if (which == 1) p->ff_rw = 0;   // a new adjoint shape resets the recompute half to it
-  return build_blocking(p, which == 0 ? &p->blk : &p->blk_adj, v, rw, ks);
+  // a new adjoint shape resets the recompute half to the adjoint's rows
+  return which == 0 ? set_fwd_blocking(p, v, rw, ks) : set_adj_blocking(p, v, rw, ks, 0);
 }
 
 int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, void* stream,
@@ -1204,9 +1275,17 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   ERNN_HIP(hipSetDevice(p->device));
   if (call->workspace == p->seg_ws) p->seg_ws = nullptr;   // the timed launches overwrite the state slots
   ERNN_HIP(launch_set_call(p->d_call, to_dev(call), st));
-  hipEvent_t e0, e1;
-  ERNN_HIP(hipEventCreate(&e0));
-  ERNN_HIP(hipEventCreate(&e1));
+  // both events are destroyed on every exit path
+  struct Events {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  ERNN_HIP(hipEventCreate(&ev.e0));
+  ERNN_HIP(hipEventCreate(&ev.e1));
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   ERNN_HIP(hipEventRecord(e0, st));
   const int nsteps = std::min(steps, p->nt_max);
   if (which == 4) {   // `steps` time-blocked adjoint launches of KS steps each, u^m from segment bank 0
@@ -1278,8 +1357,6 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   ERNN_HIP(hipEventRecord(e1, st));
   ERNN_HIP(hipEventSynchronize(e1));
   ERNN_HIP(hipEventElapsedTime(ms_out, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return ERNN_OK;
 }
 

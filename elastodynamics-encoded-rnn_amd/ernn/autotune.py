@@ -198,22 +198,25 @@ def tune_backward(survey, params, field, wavelets, steps: int = 48, rounds: int 
     each schedule is timed on its own temporary plan.  The survey's `fuse` is
     left as it was.
 
-    Returns (fuse, adj_blocking, {"fused": ms_per_step, "blocked": ms_per_step})."""
+    Returns (fuse, adj_blocking, ff_rw, {"fused": ms_per_step, "blocked": ms_per_step}):
+    ff_rw is the rows per wavefront of the fused kernel's recompute half the
+    fused time is taken with (Survey.adj_ff_rw), 0 for the unfused schedule."""
     from .engine import Engine
     if survey.mode == "field6":
-        return survey.fused, (0, 0, 0), {}
-    saved = survey.fuse
+        return survey.fused, (0, 0, 0), 0, {}
+    saved = survey.fuse, survey.adj_ff_rw
     times = {}
+    ff_rw = 0
     try:
-        survey.fuse = True
+        survey.fuse, survey.adj_ff_rw = True, 0
         eng = Engine(survey, device=params.device)
         try:
             tune_tiles(eng, params, field, wavelets, steps=steps, which=(2,), rounds=rounds)
             fz = tune_blocking(eng, params, field, wavelets, steps=steps, which=1, rounds=rounds)
             times["fused"] = fz[3]
-            _, ft = tune_fused_recompute(eng, params, field, wavelets, steps=steps, rounds=rounds)
+            ff_rw, ft = tune_fused_recompute(eng, params, field, wavelets, steps=steps, rounds=rounds)
             if ft:   # the fused kernel's recompute half on taller blocks
-                times["fused"] = min(times["fused"], min(ft.values()))
+                times["fused"] = min(ft.values())
         finally:
             eng.close()
         survey.fuse = False
@@ -226,6 +229,6 @@ def tune_backward(survey, params, field, wavelets, steps: int = 48, rounds: int 
         finally:
             eng.close()
     finally:
-        survey.fuse = saved
+        survey.fuse, survey.adj_ff_rw = saved
     fuse = times["fused"] <= times["blocked"]
-    return fuse, (fz[:3] if fuse else adj[:3]), times
+    return fuse, (fz[:3] if fuse else adj[:3]), (ff_rw if fuse else 0), times

@@ -39,8 +39,6 @@ class Engine:
             K = choose_ckpt(survey)
         self._build(K)
         self._pool: List[torch.Tensor] = []
-        # rows per wavefront of the fused kernel's recompute half (set_blocking which 2)
-        self.ff_rw = int(self.info.ablk_rw) if (self.info.fuse and self.info.ablk_ks > 1) else 0
 
     # ------------------------------------------------------------------ plan
     def _build(self, K: int) -> None:
@@ -76,6 +74,7 @@ class Engine:
         d.fuse = int(sv.fused)
         d.blk_v, d.blk_rw, d.blk_ks = (int(x) for x in sv.blocking)
         d.ablk_v, d.ablk_rw, d.ablk_ks = (int(x) for x in sv.adj_blocking)
+        d.ablk_ff_rw = int(sv.adj_ff_rw)
         d.inst = inst
         d.src, d.n_src = src, len(src_list)
         d.rec, d.n_rec = rec, len(rec_list)
@@ -217,6 +216,12 @@ class Engine:
         i = self.info
         return (int(i.ablk_v), int(i.ablk_rw), int(i.ablk_ks))
 
+    @property
+    def ff_rw(self) -> int:
+        """Rows per wavefront of the fused kernel's recompute half (0: the plan
+        has no fused time-blocked backward)."""
+        return int(self.info.ablk_ff_rw)
+
     def set_blocking(self, v: int, rw: int, ks: int, which: int = 0) -> None:
         """Time-blocked forward (which 0) or adjoint (which 1) with ks steps
         per launch (ks <= 1: one step per launch); which 2: rows per wavefront
@@ -224,10 +229,6 @@ class Engine:
         with torch.cuda.device(self.device):
             _lib.check(_lib.lib().ernn_plan_set_blocking(self._plan, which, v, rw, ks), "ernn_plan_set_blocking")
             _lib.check(_lib.lib().ernn_plan_info(self._plan, C.byref(self.info)), "ernn_plan_info")
-        if which == 2:
-            self.ff_rw = rw
-        elif which == 1:
-            self.ff_rw = rw if ks > 1 else 0
 
     def set_tiling(self, which: int, v: int, r: int, bpc: int = 0) -> None:
         with torch.cuda.device(self.device):

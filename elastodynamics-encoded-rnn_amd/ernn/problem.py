@@ -187,6 +187,9 @@ class Survey:
     # Time-blocked adjoint (V, RW, KS), const6 / dfield; with fuse, the shape of
     # the fused time-blocked backward ((1, 4, 2), (1, 4, 4) or (1, 8, 4))
     adj_blocking: Tuple[int, int, int] = (0, 0, 0)
+    # Fused time-blocked backward: rows per wavefront of the recompute half
+    # (0 = the adjoint's RW; 8 beside RW = 4 with KS in {2, 4})
+    adj_ff_rw: int = 0
 
     def __post_init__(self):
         if self.mode not in MODES:

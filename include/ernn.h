@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define ERNN_ABI_VERSION 3
+#define ERNN_ABI_VERSION 4
 
 #define ERNN_OK 0
 #define ERNN_E_INVALID (-1)   /* bad argument or descriptor */
@@ -104,6 +104,10 @@ typedef struct {
                            each launch also runs ablk_ks recompute steps of the previous
                            segment with the same shape, (1, 4, 2), (1, 4, 4) or (1, 8, 4).
                            ablk_ks <= 1 = off */
+  int32_t ablk_ff_rw;   /* fuse = 1 with a time-blocked adjoint: rows per wavefront of the
+                           fused kernel's recompute half, 0 = ablk_rw, or 8 beside ablk_rw = 4
+                           with ablk_ks in {2,4} (shorter grids: every block resident at once);
+                           otherwise 0 */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;
@@ -130,6 +134,8 @@ typedef struct {
   int32_t blk_blocks;       /* blocks per time-blocked launch */
   int32_t ablk_v, ablk_rw, ablk_ks; /* time-blocked adjoint shape (ablk_ks = 0: off) */
   int32_t ablk_blocks;
+  int32_t ablk_ff_rw;       /* rows per wavefront of the fused recompute half (0: no fused
+                               time-blocked backward) */
 } ernn_plan_info_t;
 
 /* Per-call device pointers.
@@ -186,7 +192,13 @@ int ernn_forward(void* plan, const ernn_call_t* call, void* stream);
 /* Gradient of a scalar loss J given dJ/d traces (replaces loss.backward(),
  * PI:447 / DI:511).  Needs the workspace of the matching ernn_forward.
  * Recomputes the forward from checkpoints segment by segment and runs the
- * reverse-time adjoint recurrence; FP64 accumulation.  The forward pass also
+ * reverse-time adjoint recurrence.  Accumulation: the constant gradients
+ * are summed in FP32 over one launch's cells and steps of a wavefront, then
+ * in FP64 (per-wavefront partials, a fixed-order FP64 reduction per set that
+ * depends only on the set's own instances); the per-cell gradients are
+ * summed in FP32 over one launch (at most 8 steps) by the time-blocked
+ * adjoint, or over one checkpoint segment by the one-step adjoint, and in
+ * FP64 across launches or segments.  The forward pass also
  * leaves every state of the last checkpoint segment in the workspace; the
  * first adjoint on that workspace after it starts from them instead of
  * recomputing the segment (later adjoints, or a workspace touched by

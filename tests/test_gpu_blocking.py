@@ -443,8 +443,11 @@ def test_fused_blocked_backward_tall_recompute_blocks(mode, shape):
     V, RW, KS = shape
     nx, ny, nt, h = 45, 150, 47, 0.2e-3
     rng = np.random.default_rng(14)
-    shot = make_shot(nx, ny, nt, h, sources=_edge_sources(nx, ny, h, V, RW, KS),
-                     receivers=_receivers(nx, ny, V, RW, KS))
+    # sources and receivers on and next to the output-tile edges of both block tables
+    # (the adjoint's 4-row bands and the recompute half's 8-row bands)
+    src = sorted(set(_edge_sources(nx, ny, h, V, RW, KS)) | set(_edge_sources(nx, ny, h, V, 8, KS)))
+    rec = np.unique(np.concatenate([_receivers(nx, ny, V, RW, KS), _receivers(nx, ny, V, 8, KS)]), axis=0)
+    shot = make_shot(nx, ny, nt, h, sources=src, receivers=rec)
     G = rng.standard_normal((2, nt, len(shot.receivers)))
     kw = {"params": C_ANISO}
     if mode == "dfield":
