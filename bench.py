@@ -802,8 +802,7 @@ def main():
 
     baseline = None
     if args.baseline and sv.n_sets == 1:
-        theta0 = torch.tensor(C_init / np.where(np.abs(C_init) > 0, np.abs(C_init), 5e9),
-                              dtype=torch.float32, device=dev)
+        theta0 = torch.tensor(C_init / model.scale.cpu().numpy(), dtype=torch.float32, device=dev)
         obs_shots = [eng.shot_traces(obs, b) for b in range(len(sv.shots))]
         D0 = None if sv.mode == "const6" else torch.tensor(D_init, device=dev)
         ag = time_autograd(torch, wl, theta0, model.scale, D0, obs_shots, args.baseline_window)

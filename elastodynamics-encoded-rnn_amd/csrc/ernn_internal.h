@@ -26,6 +26,18 @@ constexpr int kRowAlign = 16;
 
 enum ParamMode : int { kConst6 = 0, kDField = 1, kField6 = 2 };
 
+// Recurrence state.  Both recurrences, u^{n+1} = 2 u^n - u^{n-1} + d^n
+// (PI:286-291) and the reverse-time adjoint, are carried as (u, c, v) per
+// cell and component: v^n = u^n - u^{n-1} is the increment, u the value and c
+// the rounding error of the last sum u + v (Kahan compensation).  One step is
+//   v' = v + d,  y = v' + c,  u' = u + y,  c' = y - (u' - u),
+// so the FP32 rounding of the state does not accumulate over thousands of
+// steps.  A state slot holds the six padded fields u_x, u_y, c_x, c_y, v_x,
+// v_y of one time level; a bank slot holds u_x, u_y only (the recomputed
+// forward states the adjoint reads for its gradient terms).
+constexpr int kStateFields = 6;
+constexpr int kBankFields = 2;
+
 // fp64 partial sums per tile for the constant gradients (kConst6, kDField).
 // Slots 0-5 hold S11, S22, S12, S16, S26, S66 with the stencil weight h^-2 (or
 // h^-2 / 4 for slot 2) applied; slots 6-7 hold the mixed-derivative parts of
@@ -46,9 +58,9 @@ struct InstDev {
   float pad1;
   double gscale;    // (double)dt2 / (double)rho: d(u^{n+1})/dC = gscale * dA/dC u^n
   int64_t field;    // floats per padded field, (nx + 3) * pitch
-  int64_t pool_off; // float offset of this instance's state slots in the workspace
+  int64_t pool_off; // float offset of this instance's state slots [n_state][kStateFields][field]
+  int64_t bank_off; // float offset of its bank slots [n_bank][kBankFields][field]
   int64_t coef_off; // float offset of the packed coefficient fields of its set
-  int64_t acc_off;  // float offset of its fp32 per-cell segment accumulators
   int64_t acc64_off;// double offset of its fp64 per-cell accumulators
   int64_t trace_off;// float offset of its traces in the trace array
   int64_t wav_off;  // float offset of its source time function
@@ -87,15 +99,13 @@ struct StepArgs {
   int64_t partial_off;     // double offset in the workspace of [n_tiles][kPartials] fp64 partials
   int n_tiles;
   int mode;
-  int step;                // forward: n (produces u^{n+1}); adjoint: m (produces lam^m)
-  int s_prev, s_cur, s_next; // slots: fwd u^{n-1}, u^n, u^{n+1}; adj lam^{m+2}, lam^{m+1}, lam^m
-  int s_u;                 // adjoint: slot holding u^m
+  int step;                // forward: n (level n -> n+1); adjoint: m (lam level m+1 -> m)
+  int s_in, s_out;         // state slots of the input and output levels
+  int bank;                // forward: bank slot receiving u^{n+1} (or -1); adjoint: bank slot holding u^m
   int stride;              // recording stride P
   int record;              // forward: write traces on recorded steps
   int snapshot;            // traces are whole fields
-  int seg_top, seg_bot;    // adjoint: the segment runs m = seg_top .. seg_bot
   int rows;                // R
-  int pad;
 };
 
 // A launch that runs one adjoint step and, on the same tiles, one recompute
@@ -139,10 +149,11 @@ struct BlockArgs {
   const CallDev* call;
   int n_blocks;
   int mode;
-  int n0;                        // first step of the launch: reads u^{n0-1}, u^{n0}
+  int n0;                        // first step of the launch: starts from level n0
   int ks;                        // steps in this launch (<= KS of the kernel)
-  int s_prev, s_cur;             // slots of u^{n0-1}, u^{n0}
-  int store[kMaxBlockSteps];     // slot receiving u^{n0+1+s}, or -1
+  int s_in;                      // state slot of level n0
+  int st_store[kMaxBlockSteps];  // state slot receiving level n0+1+s, or -1
+  int bank_store[kMaxBlockSteps];// bank slot receiving u^{n0+1+s}, or -1
   int stride;                    // recording stride P
   int record;
   int snapshot;
@@ -159,12 +170,9 @@ struct BlockAdjArgs {
   int64_t partial_off;           // [blocks][blk_waves(RW)][kPartials] fp64 partials
   int n_blocks;
   int mode;
-  int m0, ks;
-  int s_l1, s_l2;                // slots of lam^{m0+1}, lam^{m0+2}
-  int store_in;                  // slot receiving lam^{m0+1} (a one-step launch), or -1
-  int store[kMaxBlockSteps];     // slot receiving lam^{m0-s}, or -1
-  int u_slot[kMaxBlockSteps];    // slot holding u^{m0-s}
-  int seg_top, seg_bot;          // the checkpoint segment this launch lies in
+  int m0, ks;                    // lam levels m0+1 -> m0-ks+1
+  int s_in, s_out;               // state slots of lam level m0+1 (input) and m0-ks+1 (output)
+  int bank_u[kMaxBlockSteps];    // bank slot holding u^{m0-s}
   int stride;
   int snapshot;
   int rw, ksmax;

@@ -218,13 +218,63 @@ __device__ __forceinline__ double wave_sum8(const double (&v)[8], int lane) {
 // halo_off): ernn_device.h.
 
 // --------------------------------------------------------------------------
-// Forward step: u^{n+1} from u^n, u^{n-1}  (PI:268-301 / DI:355-387)
+// State step (ernn_internal.h): level n + 1 from level n and the increment
+// d^n = (dt^2/rho) (L u^n + f^n) (forward, PI:286-291) or (dt^2/rho) A^T lam +
+// r (adjoint).  v' = v + d; u' = u + v' with the rounding error of that sum
+// kept in c and added back at the next step (Kahan).  Compiled without
+// contraction or reassociation: every operation rounds once, so the
+// compensation is exact and every kernel shape gives the same bits.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void advance(float& u, float& c, float& v, float d) {
+#pragma clang fp contract(off)
+  v = v + d;
+  const float y = v + c;
+  const float t = u + y;
+  c = y - (t - u);
+  u = t;
+}
+
+// --------------------------------------------------------------------------
+// Buffer addressing (time-blocked kernels).  A padded field is one buffer
+// resource (64-bit base in scalar registers); an access takes the lane's
+// column byte offset in one vector register and the row's byte offset in a
+// scalar register, so the kernels spend no vector instructions on addresses.
+// num_records is the field's size: an offset outside it reads 0 and writes
+// nothing.
+// --------------------------------------------------------------------------
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int kRsrcWord3 = 0x00020000;   // gfx9 raw buffer: 32-bit data, no swizzle
+
+__device__ __forceinline__ Rsrc field_rsrc(const gfloat* base, int64_t floats) {
+  const uint64_t bytes = (uint64_t)floats * sizeof(float);
+  return __builtin_amdgcn_make_buffer_rsrc((float*)const_cast<gfloat*>(base), 0,
+                                           (int)(uint32_t)(bytes > 0xffffffffull ? 0xffffffffull : bytes),
+                                           kRsrcWord3);
+}
+__device__ __forceinline__ float bload(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void bstore(float x, Rsrc r, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), r, (int)voff, (int)soff, 0);
+}
+
+// State slot q of an instance: field k (0, 1: u_x, u_y; 2, 3: c; 4, 5: v).
+__device__ __forceinline__ gfloat* state_field(const GCall& C, const InstDev& I, int q, int k) {
+  return C.ws + I.pool_off + ((int64_t)q * kStateFields + k) * I.field;
+}
+__device__ __forceinline__ gfloat* bank_field(const GCall& C, const InstDev& I, int b, int k) {
+  return C.ws + I.bank_off + ((int64_t)b * kBankFields + k) * I.field;
+}
+
+// --------------------------------------------------------------------------
+// Forward step: level n + 1 from level n (PI:268-301 / DI:355-387)
 //
 // R is a compile-time row count: the wavefront issues every load of its tile
-// (R + 2 rows of u^n, R rows of u^{n-1} and of the coefficients) before any
-// arithmetic, so a tile costs about one memory latency instead of R.  Rows
-// past the grid are zero ghost rows (the layout pads nx up to kRowAlign), so
-// the loads need no bounds checks; stores and events stop at row nx.
+// (R + 2 rows of u^n for the stencil, R rows of c^n, v^n and of the
+// coefficients) before any arithmetic, so a tile costs about one memory
+// latency instead of R.  Rows past the grid are zero ghost rows (the layout
+// pads nx up to kRowAlign), so the loads need no bounds checks; those rows are
+// computed and stored as zeros, and events stop at row nx.
 // --------------------------------------------------------------------------
 template <int MODE, int V, int R>
 __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, const TileDev& tl,
@@ -234,13 +284,12 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
   const uint32_t P = (uint32_t)I.pitch;
   const uint32_t lo = lane_off<V>(lane), ho = halo_off<V>(lane);
   const int j0 = tl.j0;
-  gfloat* pool = C.ws + I.pool_off;
-  const gfloat* u0x = pool + (int64_t)a.s_cur * 2 * I.field;
-  const gfloat* u0y = u0x + I.field;
-  const gfloat* u1x = pool + (int64_t)a.s_prev * 2 * I.field;
-  const gfloat* u1y = u1x + I.field;
-  gfloat* unx = pool + (int64_t)a.s_next * 2 * I.field;
-  gfloat* uny = unx + I.field;
+  const gfloat* ux = state_field(C, I, a.s_in, 0);
+  const gfloat* uy = state_field(C, I, a.s_in, 1);
+  const gfloat* cxi = state_field(C, I, a.s_in, 2);
+  const gfloat* cyi = state_field(C, I, a.s_in, 3);
+  const gfloat* vxi = state_field(C, I, a.s_in, 4);
+  const gfloat* vyi = state_field(C, I, a.s_in, 5);
   const gfloat* coef = C.ws + I.coef_off;
 
   const float kdt = (float)I.gscale;   // dt^2 / rho
@@ -259,19 +308,21 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
 
   constexpr int NC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
   RawRow<V> xraw[R + 2], yraw[R + 2];
-  float pvx[R][V], pvy[R][V];
+  float cx[R][V], cy[R][V], vx[R][V], vy[R][V];
   float cf[R][NC > 0 ? NC : 1][V];
 #pragma unroll
   for (int r = 0; r < R + 2; ++r) {
     const uint32_t ro = row_off(P, tl.i0 - 1 + r, j0);
-    xraw[r] = load_raw<V>(u0x, ro, lo, ho);
-    yraw[r] = load_raw<V>(u0y, ro, lo, ho);
+    xraw[r] = load_raw<V>(ux, ro, lo, ho);
+    yraw[r] = load_raw<V>(uy, ro, lo, ho);
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t o = row_off(P, tl.i0 + r, j0) + lo;
-    load_vec<V>(at(u1x, o), pvx[r]);
-    load_vec<V>(at(u1y, o), pvy[r]);
+    load_vec<V>(at(cxi, o), cx[r]);
+    load_vec<V>(at(cyi, o), cy[r]);
+    load_vec<V>(at(vxi, o), vx[r]);
+    load_vec<V>(at(vyi, o), vy[r]);
 #pragma unroll
     for (int q = 0; q < NC; ++q) load_vec<V>(at(coef + q * I.field, o), cf[r][q]);
   }
@@ -282,23 +333,29 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
     yr[r] = make_row<V>(yraw[r], lane);
   }
 
-  // Rows past the grid (i >= nx) are computed like the others and stored as
-  // zeros, which keeps those ghost rows zero.  A branch around them would let
-  // the compiler sink the tile's later loads below the first row's shuffles.
+  gfloat* oux = state_field(C, I, a.s_out, 0);
+  gfloat* ouy = state_field(C, I, a.s_out, 1);
+  gfloat* ocx = state_field(C, I, a.s_out, 2);
+  gfloat* ocy = state_field(C, I, a.s_out, 3);
+  gfloat* ovx = state_field(C, I, a.s_out, 4);
+  gfloat* ovy = state_field(C, I, a.s_out, 5);
+  gfloat* bx = a.bank >= 0 ? bank_field(C, I, a.bank, 0) : nullptr;
+  gfloat* by = a.bank >= 0 ? bank_field(C, I, a.bank, 1) : nullptr;
+
   int sp = tl.src_begin;
   int rp = tl.rec_begin;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int i = tl.i0 + r;
     const bool live = i < I.nx;
-    float nvx[V], nvy[V];
+    float dx[V], dy[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
       const UDeriv d = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1);
       float cq[NC > 0 ? NC : 1];
 #pragma unroll
       for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
-      fwd_op<MODE>(kf, gsa, cq, d, nvx[e], nvy[e]);
+      fwd_op<MODE>(kf, gsa, cq, d, dx[e], dy[e]);
     }
 
     // point forces on this row: (dt^2/rho) s_n w (PI:103-104, PI:270-277)
@@ -311,23 +368,36 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           if (e == lc % V) {
-            if (ev.comp == 0) nvx[e] = __builtin_fmaf(kdt, f, nvx[e]);
-            else nvy[e] = __builtin_fmaf(kdt, f, nvy[e]);
+            if (ev.comp == 0) dx[e] = __builtin_fmaf(kdt, f, dx[e]);
+            else dy[e] = __builtin_fmaf(kdt, f, dy[e]);
           }
         }
       }
       ++sp;
     }
+    float nx_[V], ny_[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      nvx[e] = __builtin_fmaf(2.f, xr[r + 1].v[e + 1], -pvx[r][e]) + nvx[e];
-      nvy[e] = __builtin_fmaf(2.f, yr[r + 1].v[e + 1], -pvy[r][e]) + nvy[e];
-      if (!live || j0 + lane * V + e >= I.ny) { nvx[e] = 0.f; nvy[e] = 0.f; }  // keep the ghosts zero
+      nx_[e] = xraw[r + 1].c[e];
+      ny_[e] = yraw[r + 1].c[e];
+      advance(nx_[e], cx[r][e], vx[r][e], dx[e]);
+      advance(ny_[e], cy[r][e], vy[r][e], dy[e]);
+      if (!live || j0 + lane * V + e >= I.ny) {   // keep the ghosts zero
+        nx_[e] = ny_[e] = 0.f;
+        cx[r][e] = cy[r][e] = vx[r][e] = vy[r][e] = 0.f;
+      }
     }
     const uint32_t so = row_off(P, i, j0) + lo;
-    store_vec<V>(at(unx, so), nvx);
-    store_vec<V>(at(uny, so), nvy);
-
+    store_vec<V>(at(oux, so), nx_);
+    store_vec<V>(at(ouy, so), ny_);
+    store_vec<V>(at(ocx, so), cx[r]);
+    store_vec<V>(at(ocy, so), cy[r]);
+    store_vec<V>(at(ovx, so), vx[r]);
+    store_vec<V>(at(ovy, so), vy[r]);
+    if (a.bank >= 0) {
+      store_vec<V>(at(bx, so), nx_);
+      store_vec<V>(at(by, so), ny_);
+    }
     if (rec_step && live) {
       if (a.snapshot) {
         gfloat* tx = C.traces + I.trace_off + ((int64_t)k * I.nx + i) * I.ny;
@@ -335,7 +405,7 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           const int j = j0 + lane * V + e;
-          if (j < I.ny) { tx[j] = nvx[e]; ty[j] = nvy[e]; }
+          if (j < I.ny) { tx[j] = nx_[e]; ty[j] = ny_[e]; }
         }
       } else {
         while (rp < tl.rec_end) {
@@ -347,7 +417,7 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
             gfloat* ty = tx + (int64_t)nf * I.n_rec;
 #pragma unroll
             for (int e = 0; e < V; ++e) {
-              if (e == lc % V) { *tx = nvx[e]; *ty = nvy[e]; }
+              if (e == lc % V) { *tx = nx_[e]; *ty = ny_[e]; }
             }
           }
           ++rp;
@@ -387,7 +457,8 @@ __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
 }
 
 // --------------------------------------------------------------------------
-// Adjoint step: lam^m = r^m + (2I + (dt^2/rho) A^T) lam^{m+1} - lam^{m+2}
+// Adjoint step: lam level m from level m + 1,
+//   lam^m = 2 lam^{m+1} - lam^{m+2} + (dt^2/rho) A^T lam^{m+1} + r^m,
 // with the gradient terms (dt^2/rho) <lam^{m+1}, dA/dC u^m>.  Same tile
 // scheme as the forward: all loads first, then R rows of arithmetic.
 // --------------------------------------------------------------------------
@@ -421,6 +492,19 @@ __device__ __forceinline__ AdjRaw<MODE, V> load_adj_raw(const gfloat* l1x, const
   return o;
 }
 
+// The six weighted fields of A^T lam for per-cell C (DI:151-163 transposed):
+// w_t = sum of C-weighted lam components that feed stencil t.
+__device__ __forceinline__ void weigh6(const float (&cr)[6], float x, float y, float (&o)[6]) {
+  const float c11 = cr[0], c22 = cr[1], c12 = cr[2], c16 = cr[3], c26 = cr[4], c66 = cr[5];
+  const float c1266 = c12 + c66;
+  o[0] = __builtin_fmaf(c11, x, c16 * y);           // -> Dxx, x row
+  o[1] = __builtin_fmaf(c66, x, c26 * y);           // -> Dyy, x row
+  o[2] = __builtin_fmaf(2.f * c16, x, c1266 * y);   // -> Dxy, x row
+  o[3] = __builtin_fmaf(c16, x, c66 * y);           // -> Dxx, y row
+  o[4] = __builtin_fmaf(c26, x, c22 * y);           // -> Dyy, y row
+  o[5] = __builtin_fmaf(c1266, x, 2.f * c26 * y);   // -> Dxy, y row
+}
+
 template <int MODE, int V>
 __device__ __forceinline__ AdjRows<MODE, V> make_adj_row(const AdjRaw<MODE, V>& raw, int lane) {
   AdjRows<MODE, V> o;
@@ -441,16 +525,11 @@ __device__ __forceinline__ AdjRows<MODE, V> make_adj_row(const AdjRaw<MODE, V>& 
     for (int q = 0; q < 6; ++q) cr[q] = make_row<V>(raw.cr[q], lane);
 #pragma unroll
     for (int e = 0; e < V + 2; ++e) {
-      const float c11 = cr[0].v[e], c22 = cr[1].v[e], c12 = cr[2].v[e];
-      const float c16 = cr[3].v[e], c26 = cr[4].v[e], c66 = cr[5].v[e];
-      const float x = lx.v[e], y = ly.v[e];
-      const float c1266 = c12 + c66;
-      o.w[0].v[e] = __builtin_fmaf(c11, x, c16 * y);           // -> Dxx, x row
-      o.w[1].v[e] = __builtin_fmaf(c66, x, c26 * y);           // -> Dyy, x row
-      o.w[2].v[e] = __builtin_fmaf(2.f * c16, x, c1266 * y);   // -> Dxy, x row
-      o.w[3].v[e] = __builtin_fmaf(c16, x, c66 * y);           // -> Dxx, y row
-      o.w[4].v[e] = __builtin_fmaf(c26, x, c22 * y);           // -> Dyy, y row
-      o.w[5].v[e] = __builtin_fmaf(c1266, x, 2.f * c26 * y);   // -> Dxy, y row
+      const float c[6] = {cr[0].v[e], cr[1].v[e], cr[2].v[e], cr[3].v[e], cr[4].v[e], cr[5].v[e]};
+      float wv[6];
+      weigh6(c, lx.v[e], ly.v[e], wv);
+#pragma unroll
+      for (int t = 0; t < 6; ++t) o.w[t].v[e] = wv[t];
     }
   }
   return o;
@@ -460,25 +539,24 @@ template <int MODE, int V, int R>
 __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, const TileDev& tl,
                                          const InstDev& I, int t, int lane) {
   // The adjoint is held to the gradient tolerance, not to the reference's
-  // rounding order: explicit fmas, and the compiler may fuse the rest.
+  // rounding order: explicit fmas, and the compiler may fuse the rest (not the
+  // state step, which sets its own contraction).
 #pragma clang fp contract(fast)
   const int m = a.step;
 
   const uint32_t P = (uint32_t)I.pitch;
   const uint32_t lo = lane_off<V>(lane), ho = halo_off<V>(lane);
   const int j0 = tl.j0;
-  gfloat* pool = C.ws + I.pool_off;
-  const gfloat* l1x = pool + (int64_t)a.s_cur * 2 * I.field;   // lam^{m+1}
-  const gfloat* l1y = l1x + I.field;
-  const gfloat* l2x = pool + (int64_t)a.s_prev * 2 * I.field;  // lam^{m+2}
-  const gfloat* l2y = l2x + I.field;
-  gfloat* lnx = pool + (int64_t)a.s_next * 2 * I.field;        // lam^m
-  gfloat* lny = lnx + I.field;
-  const gfloat* umx = pool + (int64_t)a.s_u * 2 * I.field;     // u^m
-  const gfloat* umy = umx + I.field;
+  const gfloat* l1x = state_field(C, I, a.s_in, 0);   // lam^{m+1}
+  const gfloat* l1y = state_field(C, I, a.s_in, 1);
+  const gfloat* cxi = state_field(C, I, a.s_in, 2);
+  const gfloat* cyi = state_field(C, I, a.s_in, 3);
+  const gfloat* wxi = state_field(C, I, a.s_in, 4);
+  const gfloat* wyi = state_field(C, I, a.s_in, 5);
+  const gfloat* umx = bank_field(C, I, a.bank, 0);    // u^m
+  const gfloat* umy = bank_field(C, I, a.bank, 1);
   const gfloat* coef = C.ws + I.coef_off;
 
-  const float kdt = (float)I.gscale;   // dt^2 / rho
   float c0[6];
   OpCoef kl, ku;   // kl: kdt A(C0) on the adjoint field; ku: A(C0) / a on u (dJ/dD)
   if constexpr (MODE != kField6) {
@@ -494,13 +572,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   const int k = (m - 1) / a.stride;
   const int nf = I.n_frames;
 
-  // per-cell accumulators (field modes): first / last adjoint step of the segment
-  const int top = min(a.seg_top, I.nt);
-  const bool seg_first = (m == top);
-  const bool seg_last = (m == a.seg_bot);
   constexpr int NACC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
-  constexpr int NW = AdjRows<MODE, V>::NW;
-  gfloat* acc32 = C.ws + I.acc_off;
   const double gsa = I.gscale * (double)I.hinv2;   // per-cell sums carry 1 / a
   gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
 
@@ -509,12 +581,14 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
   for (int r = 0; r < R + 2; ++r)
     lraw[r] = load_adj_raw<MODE, V>(l1x, l1y, coef, I.field, row_off(P, tl.i0 - 1 + r, j0), lo, ho);
-  float l2vx[R][V], l2vy[R][V];
+  float cx[R][V], cy[R][V], wx[R][V], wy[R][V];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t o = row_off(P, tl.i0 + r, j0) + lo;
-    load_vec<V>(at(l2x, o), l2vx[r]);
-    load_vec<V>(at(l2y, o), l2vy[r]);
+    load_vec<V>(at(cxi, o), cx[r]);
+    load_vec<V>(at(cyi, o), cy[r]);
+    load_vec<V>(at(wxi, o), wx[r]);
+    load_vec<V>(at(wyi, o), wy[r]);
   }
   constexpr int RU = (MODE == kConst6) ? 1 : R + 2;   // u^m rows with halo (field modes)
   float uvx[R][V], uvy[R][V];
@@ -534,22 +608,17 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
       uraw[r][1] = load_raw<V>(umy, ro, lo, ho);
     }
   }
-
-  // The accumulators this tile read-modify-writes are read together with its
-  // other loads, so the updates at the end of the tile wait for no memory.
-  float acc_in[NACC > 0 ? R : 1][NACC > 0 ? NACC : 1][V];
+  // the FP64 per-cell accumulators this tile updates are read with its other
+  // loads, so the update at the end of the tile waits for no memory
+  double acc_in[NACC > 0 ? R : 1][NACC > 0 ? NACC : 1][V];
   if constexpr (NACC > 0) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int q = 0; q < NACC; ++q) {
-        if (seg_first) {
+      for (int q = 0; q < NACC; ++q)
 #pragma unroll
-          for (int e = 0; e < V; ++e) acc_in[r][q][e] = 0.f;
-        } else {
-          load_vec<V>(at(acc32 + q * I.field, row_off(P, tl.i0 + r, j0) + lo), acc_in[r][q]);
-        }
-      }
+        for (int e = 0; e < V; ++e)
+          acc_in[r][q][e] = acc64[q * I.field + row_off(P, tl.i0 + r, j0) + lo + e];
   }
   gdouble* part = reinterpret_cast<gdouble*>(C.ws) + a.partial_off + (int64_t)t * kPartials + (lane >> 3);
   double part_in = 0.0;
@@ -572,6 +641,13 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
   for (int p = 0; p < kPartials; ++p) S[p] = 0.f;
 
+  gfloat* olx = state_field(C, I, a.s_out, 0);
+  gfloat* oly = state_field(C, I, a.s_out, 1);
+  gfloat* ocx = state_field(C, I, a.s_out, 2);
+  gfloat* ocy = state_field(C, I, a.s_out, 3);
+  gfloat* owx = state_field(C, I, a.s_out, 4);
+  gfloat* owy = state_field(C, I, a.s_out, 5);
+
   // Rows past the grid are computed, contribute nothing and are stored as
   // zeros (see fwd_tile).
   int rp = tl.rec_begin;
@@ -582,17 +658,15 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
     const AdjRows<MODE, V>& Lm = lr[r];
     const AdjRows<MODE, V>& Lc = lr[r + 1];
     const AdjRows<MODE, V>& Lp = lr[r + 2];
-    float nvx[V], nvy[V];
+    float dx[V], dy[V];
     float G[NACC > 0 ? NACC : 1][V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
       const int x = e + 1;
-      const float bx = __builtin_fmaf(2.f, Lc.cx[e], -l2vx[r][e]);   // 2 lam^{m+1} - lam^{m+2}
-      const float by = __builtin_fmaf(2.f, Lc.cy[e], -l2vy[r][e]);
       if constexpr (MODE == kConst6 || MODE == kDField) {
         // A^T lam = A(C0) (D * lam): constant stencils on the (weighted) adjoint field
         const UDeriv dl = uderivs<V>(Lm.w[0], Lc.w[0], Lp.w[0], Lm.w[1], Lc.w[1], Lp.w[1], x);
-        apply_op_u(kl, dl, bx, by, nvx[e], nvy[e]);
+        apply_op_sum(kl, dl, dx[e], dy[e]);
         float ux, uy;
         if constexpr (MODE == kConst6) { ux = uvx[r][e]; uy = uvy[r][e]; }
         else { ux = um[r + 1][0].v[x]; uy = um[r + 1][1].v[x]; }
@@ -609,8 +683,8 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
       } else {
         const float sx = uxx<V>(Lm.w[0], Lc.w[0], Lp.w[0], x) + uyy<V>(Lc.w[1], x);
         const float sy = uxx<V>(Lm.w[3], Lc.w[3], Lp.w[3], x) + uyy<V>(Lc.w[4], x);
-        nvx[e] = bx + __builtin_fmaf(ka, sx, kq * uxy<V>(Lm.w[2], Lp.w[2], x));
-        nvy[e] = by + __builtin_fmaf(ka, sy, kq * uxy<V>(Lm.w[5], Lp.w[5], x));
+        dx[e] = __builtin_fmaf(ka, sx, kq * uxy<V>(Lm.w[2], Lp.w[2], x));
+        dy[e] = __builtin_fmaf(ka, sy, kq * uxy<V>(Lm.w[5], Lp.w[5], x));
         const UDeriv du = uderivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
                                      um[r + 2][1], x);
         float g[6];
@@ -627,7 +701,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           const int j = j0 + lane * V + e;
-          if (live && j < I.ny) { nvx[e] += gx[j]; nvy[e] += gy[j]; }
+          if (live && j < I.ny) { dx[e] += gx[j]; dy[e] += gy[j]; }
         }
       } else {
         while (rp < tl.rec_end) {
@@ -640,38 +714,39 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
             const float vgx = *gx, vgy = *gy;
 #pragma unroll
             for (int e = 0; e < V; ++e) {
-              if (e == lc % V) { nvx[e] += vgx; nvy[e] += vgy; }
+              if (e == lc % V) { dx[e] += vgx; dy[e] += vgy; }
             }
           }
           ++rp;
         }
       }
     }
+    float nx_[V], ny_[V];
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      if (!live || j0 + lane * V + e >= I.ny) { nvx[e] = 0.f; nvy[e] = 0.f; }
+      nx_[e] = Lc.cx[e];
+      ny_[e] = Lc.cy[e];
+      advance(nx_[e], cx[r][e], wx[r][e], dx[e]);
+      advance(ny_[e], cy[r][e], wy[r][e], dy[e]);
+      if (!live || j0 + lane * V + e >= I.ny) {
+        nx_[e] = ny_[e] = 0.f;
+        cx[r][e] = cy[r][e] = wx[r][e] = wy[r][e] = 0.f;
+      }
     }
     const uint32_t so = row_off(P, i, j0) + lo;
-    store_vec<V>(at(lnx, so), nvx);
-    store_vec<V>(at(lny, so), nvy);
+    store_vec<V>(at(olx, so), nx_);
+    store_vec<V>(at(oly, so), ny_);
+    store_vec<V>(at(ocx, so), cx[r]);
+    store_vec<V>(at(ocy, so), cy[r]);
+    store_vec<V>(at(owx, so), wx[r]);
+    store_vec<V>(at(owy, so), wy[r]);
 
     if constexpr (NACC > 0) {
 #pragma unroll
-      for (int q = 0; q < NACC; ++q) {
-        gfloat* a32 = at(acc32 + q * I.field, so);
-        float v[V];
+      for (int q = 0; q < NACC; ++q)
 #pragma unroll
-        for (int e = 0; e < V; ++e) v[e] = live ? acc_in[r][q][e] + G[q][e] : 0.f;
-        if (seg_last) {
-          gdouble* a64 = acc64 + q * I.field + so;
-#pragma unroll
-          for (int e = 0; e < V; ++e) {
-            if (live && j0 + lane * V + e < I.ny) a64[e] += gsa * (double)v[e];
-          }
-        } else {
-          store_vec<V>(a32, v);
-        }
-      }
+        for (int e = 0; e < V; ++e)
+          if (live && j0 + lane * V + e < I.ny) acc64[q * I.field + so + e] = acc_in[r][q][e] + gsa * (double)G[q][e];
     }
   }
 
@@ -717,10 +792,10 @@ __global__ void __launch_bounds__(kBlock) adj_step_kernel(StepArgs a) {
 // --------------------------------------------------------------------------
 // Fused step: adjoint step m of checkpoint segment s and recompute step n of
 // segment s - 1 in one launch.  The two read and write disjoint slots (the
-// recompute fills the other bank of segment slots), so they are independent;
-// running them together halves the launches of the backward pass and lets the
-// bandwidth-bound recompute overlap the issue-bound adjoint on the same CUs.
-// Fused index u covers 2 x tiles: tile u / 2, adjoint if u is even.
+// recompute fills the other bank), so they are independent; running them
+// together halves the launches of the backward pass and lets the recompute
+// overlap the adjoint on the same CUs.  Fused index u covers 2 x tiles: tile
+// u / 2, adjoint if u is even.
 // --------------------------------------------------------------------------
 template <int MODE, int V, int R>
 __global__ void __launch_bounds__(kBlock) fused_step_kernel(FusedArgs fa) {
@@ -781,16 +856,22 @@ __device__ __forceinline__ uint32_t map_receivers(int (&map)[WB][RW][kWave * V],
 }
 
 // --------------------------------------------------------------------------
-// Time-blocked forward: KS steps per launch on a block region of
+// Time-blocked kernels: KS steps per launch on a block region of
 // blk_waves(RW) * RW rows x 64 V columns.  Wavefront w holds region rows
-// [w RW, (w + 1) RW) of u^n and u^{n-1} in registers; column neighbours come
-// by DPP shifts, and the rows just above and below a wavefront's band come
-// from its neighbours through LDS (one barrier per step).  The valid part of
-// the region shrinks by one cell per step on every side (its edge reads no
-// halo), so after KS steps the output tile (the region less KS cells on each
-// side) is exact.  Each cell's arithmetic is that of fwd_tile (fwd_op), so
-// the states are bitwise those of the single-step kernel.  Cells outside the grid are
-// kept at zero; only output cells are stored and recorded.
+// [w RW, (w + 1) RW) of the state in registers and updates it in place; column
+// neighbours come by DPP shifts, and the rows just above and below a
+// wavefront's band come from its neighbours through LDS (one barrier per
+// step).  The valid part of the region shrinks by one cell per step on every
+// side (its edge reads no halo), so after KS steps the output tile (the region
+// less KS cells on each side) is exact.  Cells outside the grid must stay
+// zero: with a per-cell coefficient (damage factor, six fields) they do by
+// themselves, since the ghost coefficients are zero; with six constants the
+// blocks on the grid edge zero the increment of those cells.  Only output
+// cells are stored.
+//
+// Addressing: buffer resources per field, the lane's column byte offset in a
+// vector register, the row's byte offset in a scalar register (wave-uniform).
+// Rows and columns outside the grid are clamped to the one-cell ghost frame.
 // --------------------------------------------------------------------------
 // The forward body reads its launch's arguments in place through a pointer
 // into the kernarg segment (address space 4): scalar loads, also for the
@@ -799,14 +880,50 @@ __device__ __forceinline__ uint32_t map_receivers(int (&map)[WB][RW][kWave * V],
 template <typename T>
 using KArg = const T __attribute__((address_space(4)));
 
+// Geometry of a wavefront's band shared by both time-blocked bodies.
+template <int V, int RW, int KS, int WB>
+struct Band {
+  static constexpr int kCols = kWave * V;
+  static constexpr int OR = WB * RW - 2 * KS;   // output rows
+  static constexpr int OC = kCols - 2 * KS;     // output columns
+  int gi0;              // grid row of the band's first row (wave-uniform)
+  int gj0;              // grid column of the lane's first column
+  uint32_t vcol[V];     // column byte offsets, clamped to the ghost frame
+  bool col_own;         // the lane's columns lie in the output tile (KS % V == 0: all or none)
+  bool col_in[V];       // column inside the grid
+  int own_lo, own_hi;   // band rows [own_lo, own_hi) lie in the output tile (wave-uniform)
+  bool edge;            // the block region reaches outside the grid (block-uniform)
+
+  __device__ __forceinline__ Band(const BlockDev& bl, const InstDev& I, int w, int lane) {
+    gi0 = __builtin_amdgcn_readfirstlane(bl.oi0 - KS + w * RW);
+    gj0 = bl.oj0 - KS + lane * V;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+      vcol[e] = (uint32_t)(kLeftPad + min(max(gj0 + e, -1), I.ny)) * (uint32_t)sizeof(float);
+      col_in[e] = gj0 + e >= 0 && gj0 + e < I.ny;
+    }
+    col_own = gj0 >= bl.oj0 && gj0 < bl.oj0 + OC;
+    own_lo = __builtin_amdgcn_readfirstlane(max(bl.oi0 - gi0, 0));
+    own_hi = __builtin_amdgcn_readfirstlane(min(min(bl.oi0 + OR, I.nx) - gi0, RW));
+    edge = bl.oi0 - KS < 0 || bl.oi0 - KS + WB * RW > I.nx || bl.oj0 - KS < 0 || bl.oj0 - KS + kCols > I.ny;
+  }
+  // byte offset of band row r (clamped), wave-uniform
+  __device__ __forceinline__ uint32_t srow(const InstDev& I, int r) const {
+    return (uint32_t)(min(max(gi0 + r, -1), I.nx) + 1) * (uint32_t)I.pitch * (uint32_t)sizeof(float);
+  }
+  __device__ __forceinline__ bool row_in(const InstDev& I, int r) const { return gi0 + r >= 0 && gi0 + r < I.nx; }
+  __device__ __forceinline__ bool owned(const InstDev& I, int r, int e) const {
+    return r >= own_lo && r < own_hi && col_own && col_in[e];
+  }
+};
+
 template <int MODE, int V, int RW, int KS>
 __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   const auto& a = *ap;
   constexpr int kCols = kWave * V;
   constexpr int WB = blk_waves(RW);                  // wavefronts
-  constexpr int OR = WB * RW - 2 * KS;               // output rows
-  constexpr int OC = kCols - 2 * KS;                 // output columns
   constexpr int NC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
+  static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
   __shared__ float xch[2][WB][2][2][kCols];          // [buffer][wavefront][top/bottom][comp][column]
 
   const int lane = threadIdx.x & (kWave - 1);
@@ -816,13 +933,10 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   const int t = (bid & 7) * chunk + (bid >> 3);
   // Bands of up to 4 rows load the call and block records side by side ahead
   // of the exits (a clamped, always valid index), so the state loads wait on
-  // one scalar round trip after the kernel arguments instead of four
-  // (single-512 forward 2.30 -> 2.18 us per step).  They stay there because
+  // one scalar round trip after the kernel arguments.  They stay there because
   // the exit tests a copy of n_blocks passed through an empty asm that takes
   // the loaded values as inputs; the asm is not volatile, so it writes no
   // memory as far as the compiler knows and later uniform loads stay scalar.
-  // 8-row bands keep the plain order, which the compiler gives fewer
-  // registers (96 against 100 VGPRs: 5 wavefronts per SIMD instead of 4).
   constexpr bool kEarly = RW <= 4;
   if (!kEarly && t >= a.n_blocks) return;
   const GCall C = global_call(*a.call);
@@ -830,37 +944,35 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   int n_blocks = a.n_blocks;
   if constexpr (kEarly)
     asm("" : "+s"(n_blocks) : "s"((uint64_t)C.ws), "s"(bl.I.nt), "s"(bl.I.field), "s"(bl.I.pool_off),
-        "s"(bl.oi0), "s"(a.n0), "s"(a.ks), "s"(a.s_prev), "s"(a.s_cur));
+        "s"(bl.oi0), "s"(a.n0), "s"(a.ks), "s"(a.s_in));
   if (t >= n_blocks) return;
   const InstDev I = bl.I;
   const int ks = min(a.ks, I.nt - a.n0);
   if (ks <= 0) return;   // this instance has finished its time loop
 
-  const uint32_t P = (uint32_t)I.pitch;
-  gfloat* pool = C.ws + I.pool_off;
-  const gfloat* u0x = pool + (int64_t)a.s_cur * 2 * I.field;
-  const gfloat* u1x = pool + (int64_t)a.s_prev * 2 * I.field;
-  const gfloat* coef = C.ws + I.coef_off;
-  const int gi0 = bl.oi0 - KS + w * RW;          // grid row of this wavefront's first region row
-  const int gj0 = bl.oj0 - KS + lane * V;        // grid column of this lane's first column
+  const Band<V, RW, KS, WB> bd(bl, I, w, lane);
+  const int gi0 = bd.gi0;
 
-  // ---- loads: u^n, u^{n-1} and the coefficients of the lane's cells; cells
-  // outside the grid read a zero ghost cell (clamped address)
-  float cx[RW][V], cy[RW][V], px[RW][V], py[RW][V];
+  // ---- loads: level n0 (u, c, v) and the coefficients of the lane's cells
+  Rsrc rin[kStateFields];
+#pragma unroll
+  for (int q = 0; q < kStateFields; ++q) rin[q] = field_rsrc(state_field(C, I, a.s_in, q), I.field);
+  float u[2][RW][V], c[2][RW][V], v[2][RW][V];
   float cf[RW][NC > 0 ? NC : 1][V];
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
-    const int ci = min(max(gi0 + r, -1), I.nx);
+    const uint32_t so = bd.srow(I, r);
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-      const int cj = min(max(gj0 + e, -1), I.ny);
-      const uint32_t o = row_off(P, ci, cj);
-      cx[r][e] = *at(u0x, o);
-      cy[r][e] = *at(u0x + I.field, o);
-      px[r][e] = *at(u1x, o);
-      py[r][e] = *at(u1x + I.field, o);
 #pragma unroll
-      for (int q = 0; q < NC; ++q) cf[r][q][e] = *at(coef + q * I.field, o);
+      for (int q = 0; q < 2; ++q) {
+        u[q][r][e] = bload(rin[q], bd.vcol[e], so);
+        c[q][r][e] = bload(rin[2 + q], bd.vcol[e], so);
+        v[q][r][e] = bload(rin[4 + q], bd.vcol[e], so);
+      }
+#pragma unroll
+      for (int q = 0; q < NC; ++q)
+        cf[r][q][e] = bload(field_rsrc(C.ws + I.coef_off + q * I.field, I.field), bd.vcol[e], so);
     }
   }
   const float kdt = (float)I.gscale;   // dt^2 / rho
@@ -873,32 +985,20 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   }
   const double gsa = I.gscale * (double)I.hinv2;
 
-  bool inside[RW][V], owned[RW][V];
-#pragma unroll
-  for (int r = 0; r < RW; ++r)
-#pragma unroll
-    for (int e = 0; e < V; ++e) {
-      const int gi = gi0 + r, gj = gj0 + e;
-      inside[r][e] = gi >= 0 && gi < I.nx && gj >= 0 && gj < I.ny;
-      owned[r][e] = inside[r][e] && gi >= bl.oi0 && gi < bl.oi0 + OR && gj >= bl.oj0 && gj < bl.oj0 + OC;
-    }
-
   __shared__ int recmap[WB][RW][kCols];
   const bool use_map = a.record && !a.snapshot && !bl.rec_shared && bl.rec_end > bl.rec_begin;   // block-uniform
-  // the recording stores read the LDS map: holding the indices in registers
-  // (ridx, left unused) cost the (1, 4) shapes 7 VGPRs for no measured gain
   int ridx[RW][V];
   const uint32_t rbits = use_map ? map_receivers<V, RW, WB>(recmap, ridx, a.rec, bl.rec_begin, bl.rec_end,
                                                             gi0, bl.oj0 - KS, w, lane) : 0u;
 
   auto publish = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < V; ++e) {
-      xch[buf][w][0][0][lane * V + e] = cx[0][e];
-      xch[buf][w][0][1][lane * V + e] = cy[0][e];
-      xch[buf][w][1][0][lane * V + e] = cx[RW - 1][e];
-      xch[buf][w][1][1][lane * V + e] = cy[RW - 1][e];
-    }
+    for (int e = 0; e < V; ++e)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        xch[buf][w][0][q][lane * V + e] = u[q][0][e];
+        xch[buf][w][1][q][lane * V + e] = u[q][RW - 1][e];
+      }
   };
   publish(0);
   __syncthreads();
@@ -923,12 +1023,12 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
       yr[RW + 1] = make_row_z<V>(by);
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
-        xr[r + 1] = make_row_z<V>(cx[r]);
-        yr[r + 1] = make_row_z<V>(cy[r]);
+        xr[r + 1] = make_row_z<V>(u[0][r]);
+        yr[r + 1] = make_row_z<V>(u[1][r]);
       }
     }
 
-    float nx_[RW][V], ny_[RW][V];
+    float dx[RW][V], dy[RW][V];
 #pragma unroll
     for (int r = 0; r < RW; ++r)
 #pragma unroll
@@ -937,7 +1037,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
         float cq[NC > 0 ? NC : 1];
 #pragma unroll
         for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
-        fwd_op<MODE>(kf, gsa, cq, d, nx_[r][e], ny_[r][e]);
+        fwd_op<MODE>(kf, gsa, cq, d, dx[r][e], dy[r][e]);
       }
 
     // point forces anywhere in the region (halo cells feed the later steps)
@@ -952,48 +1052,88 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           if (r == rr && e == lc % V) {
-            if (ev.comp == 0) nx_[r][e] = __builtin_fmaf(kdt, f, nx_[r][e]);
-            else ny_[r][e] = __builtin_fmaf(kdt, f, ny_[r][e]);
+            if (ev.comp == 0) dx[r][e] = __builtin_fmaf(kdt, f, dx[r][e]);
+            else dy[r][e] = __builtin_fmaf(kdt, f, dy[r][e]);
           }
         }
+    }
+    if constexpr (MODE == kConst6) {   // cells outside the grid keep their zero state
+      if (bd.edge) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+          for (int e = 0; e < V; ++e)
+            if (!(bd.row_in(I, r) && bd.col_in[e])) { dx[r][e] = 0.f; dy[r][e] = 0.f; }
+      }
     }
 #pragma unroll
     for (int r = 0; r < RW; ++r)
 #pragma unroll
       for (int e = 0; e < V; ++e) {
-        nx_[r][e] = __builtin_fmaf(2.f, cx[r][e], -px[r][e]) + nx_[r][e];
-        ny_[r][e] = __builtin_fmaf(2.f, cy[r][e], -py[r][e]) + ny_[r][e];
-        if (!inside[r][e]) { nx_[r][e] = 0.f; ny_[r][e] = 0.f; }
+        advance(u[0][r][e], c[0][r][e], v[0][r][e], dx[r][e]);
+        advance(u[1][r][e], c[1][r][e], v[1][r][e], dy[r][e]);
       }
 
-    // store the output cells of the states the schedule keeps
-    const int slot = a.store[s];
-    if (slot >= 0) {
-      gfloat* dx = pool + (int64_t)slot * 2 * I.field;
+    // store the output cells of the levels the schedule keeps
+    const int st_slot = a.st_store[s];
+    const int bk_slot = a.bank_store[s];
+    if (st_slot >= 0 || bk_slot >= 0) {
+      if (bd.col_own) {
+        if (st_slot >= 0) {
+          Rsrc ro[kStateFields];
 #pragma unroll
-      for (int r = 0; r < RW; ++r)
+          for (int q = 0; q < kStateFields; ++q) ro[q] = field_rsrc(state_field(C, I, st_slot, q), I.field);
 #pragma unroll
-        for (int e = 0; e < V; ++e)
-          if (owned[r][e]) {
-            const uint32_t o = row_off(P, gi0 + r, gj0 + e);
-            *at(dx, o) = nx_[r][e];
-            *at(dx + I.field, o) = ny_[r][e];
+          for (int r = 0; r < RW; ++r) {
+            if (r < bd.own_lo || r >= bd.own_hi) continue;
+            const uint32_t so = bd.srow(I, r);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              if (!bd.col_in[e]) continue;
+#pragma unroll
+              for (int q = 0; q < 2; ++q) {
+                bstore(u[q][r][e], ro[q], bd.vcol[e], so);
+                bstore(c[q][r][e], ro[2 + q], bd.vcol[e], so);
+                bstore(v[q][r][e], ro[4 + q], bd.vcol[e], so);
+              }
+            }
           }
+        }
+        if (bk_slot >= 0) {
+          const Rsrc bx = field_rsrc(bank_field(C, I, bk_slot, 0), I.field);
+          const Rsrc by = field_rsrc(bank_field(C, I, bk_slot, 1), I.field);
+#pragma unroll
+          for (int r = 0; r < RW; ++r) {
+            if (r < bd.own_lo || r >= bd.own_hi) continue;
+            const uint32_t so = bd.srow(I, r);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              if (!bd.col_in[e]) continue;
+              bstore(u[0][r][e], bx, bd.vcol[e], so);
+              bstore(u[1][r][e], by, bd.vcol[e], so);
+            }
+          }
+        }
+      }
     }
     // record frame k = n / P from u^{n+1} (PI:294-301)
     if (a.record && (n % a.stride == 0)) {
       const int k = n / a.stride;
       const int nf = I.n_frames;
       if (a.snapshot) {
+        if (bd.col_own) {
 #pragma unroll
-        for (int r = 0; r < RW; ++r)
+          for (int r = 0; r < RW; ++r) {
+            if (r < bd.own_lo || r >= bd.own_hi) continue;
 #pragma unroll
-          for (int e = 0; e < V; ++e)
-            if (owned[r][e]) {
-              gfloat* tx = C.traces + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + gj0 + e;
-              tx[0] = nx_[r][e];
-              tx[(int64_t)nf * I.nx * I.ny] = ny_[r][e];
-            }
+            for (int e = 0; e < V; ++e)
+              if (bd.col_in[e]) {
+                gfloat* tx = C.traces + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + bd.gj0 + e;
+                tx[0] = u[0][r][e];
+                tx[(int64_t)nf * I.nx * I.ny] = u[1][r][e];
+              }
+          }
+        }
       } else if (use_map) {
         if (rbits) {
           gfloat* tx = C.traces + I.trace_off + (int64_t)k * I.n_rec;
@@ -1004,8 +1144,8 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
             for (int e = 0; e < V; ++e)
               if ((rbits >> (r * V + e)) & 1u) {
                 const int rec = recmap[w][r][lane * V + e];
-                tx[rec] = nx_[r][e];
-                tx[rec + cy] = ny_[r][e];
+                tx[rec] = u[0][r][e];
+                tx[rec + cy] = u[1][r][e];
               }
         }
       } else {
@@ -1019,22 +1159,12 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
 #pragma unroll
             for (int e = 0; e < V; ++e)
               if (r == rr && e == lc % V) {
-                tx[0] = nx_[r][e];
-                tx[(int64_t)nf * I.n_rec] = ny_[r][e];
+                tx[0] = u[0][r][e];
+                tx[(int64_t)nf * I.n_rec] = u[1][r][e];
               }
         }
       }
     }
-
-#pragma unroll
-    for (int r = 0; r < RW; ++r)
-#pragma unroll
-      for (int e = 0; e < V; ++e) {
-        px[r][e] = cx[r][e];
-        py[r][e] = cy[r][e];
-        cx[r][e] = nx_[r][e];
-        cy[r][e] = ny_[r][e];
-      }
     if (s + 1 < ks) {
       publish(buf ^ 1);
       __syncthreads();
@@ -1044,20 +1174,334 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
 
 // --------------------------------------------------------------------------
 // Time-blocked adjoint: KS reverse steps per launch on the block regions of
-// the time-blocked forward.  Wavefront w holds lam^{m+1}, lam^{m+2} (and D)
-// of its region rows in registers; the stencilled field (lam, or D * lam for
-// the damage mode) crosses band edges through LDS.  Residuals are injected
-// on the whole region (its halo cells feed the later steps), the gradient
-// terms only on the output tile, where they are summed over the launch's
-// steps in registers: one butterfly for the constant-gradient partials (one
-// slot per wavefront) and one read-modify-write of the per-cell accumulators
-// per launch.  The reverse recurrence is that of adj_tile.
+// the time-blocked forward, lam levels m0 + 1 -> m0 - ks + 1.  Wavefront w
+// holds the lam state (and D) of its band in registers; the stencilled field
+// (lam, or D * lam for the damage mode) crosses band edges through LDS.
+// Residuals are injected on the whole region (its halo cells feed the later
+// steps); the gradient terms are taken only on the band rows inside the
+// output tile, summed over the launch's steps in registers: one butterfly for
+// the constant-gradient partials (one slot per wavefront, lanes outside the
+// output tile contribute nothing) and one FP64 update of the per-cell
+// damage gradient per launch.  The reverse recurrence is that of adj_tile.
 // --------------------------------------------------------------------------
 template <int MODE, int V, int RW, int KS>
-__global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockAdjArgs a) {
+__device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) {
 #pragma clang fp contract(fast)
-  const int bid = blockIdx.x;
-#include "ernn_adj_block_body.inc"
+  const auto& a = *ap;
+  constexpr int kCols = kWave * V;
+  constexpr int WB = blk_waves(RW);
+  constexpr bool kD = (MODE == kDField);
+  static_assert(MODE != kField6, "six-field adjoint: one step per launch");
+  static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
+  __shared__ float xch[2][WB][2][2][kCols];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x >> 6;
+  const int chunk = (a.n_blocks + 7) >> 3;
+  const int t = (bid & 7) * chunk + (bid >> 3);
+  const GCall C = global_call(*a.call);
+  const BlockDev bl = a.blocks[max(min(t, a.n_blocks - 1), 0)];
+  int n_blocks = a.n_blocks;
+  asm("" : "+s"(n_blocks) : "s"((uint64_t)C.ws), "s"(bl.I.nt), "s"(bl.I.field), "s"(bl.I.pool_off),
+      "s"(bl.oi0), "s"(a.m0), "s"(a.ks), "s"(a.s_in));
+  if (t >= n_blocks) return;
+  const InstDev I = bl.I;
+  if (a.m0 - a.ks + 1 > I.nt) return;   // this instance's reverse recurrence has not started
+
+  const Band<V, RW, KS, WB> bd(bl, I, w, lane);
+  const int gi0 = bd.gi0;
+  const float kdt = (float)I.gscale;
+  const double gsa = I.gscale * (double)I.hinv2;
+  float c0[6];
+#pragma unroll
+  for (int p = 0; p < 6; ++p) c0[p] = C.params[bl.set * 6 + p];
+  const OpCoef kl = op_coef(c0, gsa);    // (dt^2/rho) A(C0) on the (weighted) adjoint field
+  const OpCoef ku = op_coef(c0, 1.0);    // A(C0) / a on u (dJ/dD)
+  (void)kdt;
+
+  // ---- loads: lam level m0 + 1 (lam, c, w) and D
+  Rsrc rin[kStateFields];
+#pragma unroll
+  for (int q = 0; q < kStateFields; ++q) rin[q] = field_rsrc(state_field(C, I, a.s_in, q), I.field);
+  const Rsrc rD = field_rsrc(C.ws + I.coef_off, I.field);
+  float l[2][RW][V], c[2][RW][V], wv[2][RW][V], dd[RW][V];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const uint32_t so = bd.srow(I, r);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        l[q][r][e] = bload(rin[q], bd.vcol[e], so);
+        c[q][r][e] = bload(rin[2 + q], bd.vcol[e], so);
+        wv[q][r][e] = bload(rin[4 + q], bd.vcol[e], so);
+      }
+      dd[r][e] = kD ? bload(rD, bd.vcol[e], so) : 1.f;
+    }
+  }
+  gdouble* part = reinterpret_cast<gdouble*>(C.ws) + a.partial_off +
+                  ((int64_t)t * WB + w) * kPartials + (lane >> 3);
+  const double part_in = *part;
+  float gacc[RW][V];   // per-cell damage gradient of this launch (FP32 over at most KS steps)
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int e = 0; e < V; ++e) gacc[r][e] = 0.f;
+
+  // The damage mode walks the receiver list: its parallel injection would
+  // cost registers, and on the large grids it runs on other blocks hide the walk.
+  __shared__ int recmap[WB][RW][kCols];
+  const bool use_map = !kD && !a.snapshot && !bl.rec_shared && bl.rec_end > bl.rec_begin;   // block-uniform
+  int ridx[RW][V];
+  const uint32_t rbits = use_map ? map_receivers<V, RW, WB>(recmap, ridx, a.rec, bl.rec_begin, bl.rec_end,
+                                                            gi0, bl.oj0 - KS, w, lane) : 0u;
+
+  auto publish = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        xch[buf][w][0][q][lane * V + e] = dd[0][e] * l[q][0][e];
+        xch[buf][w][1][q][lane * V + e] = dd[RW - 1][e] * l[q][RW - 1][e];
+      }
+  };
+  // u^m of the lane's cells on the band rows in the output tile (the damage
+  // mode with one row of halo on each side for its stencil).  Step s + 1's
+  // loads are issued as soon as step s has used its u, before the band-edge
+  // exchange, so they arrive while the next update computes.
+  constexpr int RU = kD ? RW + 2 : RW;
+  constexpr int UO = kD ? 1 : 0;
+  float urx[RU][V], ury[RU][V];
+  const bool grad_rows = bd.own_lo < bd.own_hi;   // wave-uniform
+  auto load_u = [&](int s) {
+    const Rsrc bx = field_rsrc(bank_field(C, I, a.bank_u[s], 0), I.field);
+    const Rsrc by = field_rsrc(bank_field(C, I, a.bank_u[s], 1), I.field);
+#pragma unroll
+    for (int r = 0; r < RU; ++r) {
+      const uint32_t so = bd.srow(I, r - UO);
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        urx[r][e] = bload(bx, bd.vcol[e], so);
+        ury[r][e] = bload(by, bd.vcol[e], so);
+      }
+    }
+  };
+  if (grad_rows && a.m0 <= I.nt) load_u(0);
+  publish(0);
+  __syncthreads();
+
+  float S[kPartials];
+#pragma unroll
+  for (int p = 0; p < kPartials; ++p) S[p] = 0.f;
+  const int nf = I.n_frames;
+
+  for (int s = 0; s < a.ks; ++s) {
+    const int m = a.m0 - s;
+    const int buf = s & 1;
+    if (m <= I.nt) {   // block-uniform: an instance with fewer steps starts later, from its zero state
+      // ---- stencil rows of the (weighted) adjoint field: own rows and the band-edge rows
+      Row<V> xr[RW + 2], yr[RW + 2];
+      {
+        float tx[V], ty[V], bx[V], by[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
+          ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
+          bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
+          by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+        }
+        xr[0] = make_row_z<V>(tx);
+        yr[0] = make_row_z<V>(ty);
+        xr[RW + 1] = make_row_z<V>(bx);
+        yr[RW + 1] = make_row_z<V>(by);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          float rx[V], ry[V];
+#pragma unroll
+          for (int e = 0; e < V; ++e) { rx[e] = dd[r][e] * l[0][r][e]; ry[e] = dd[r][e] * l[1][r][e]; }
+          xr[r + 1] = make_row_z<V>(rx);
+          yr[r + 1] = make_row_z<V>(ry);
+        }
+      }
+      Row<V> uxr[kD ? RW + 2 : 1], uyr[kD ? RW + 2 : 1];
+      if constexpr (kD) {
+        if (grad_rows) {
+#pragma unroll
+          for (int r = 0; r < RW + 2; ++r) {
+            uxr[r] = make_row_z<V>(urx[r]);
+            uyr[r] = make_row_z<V>(ury[r]);
+          }
+        }
+      }
+
+      float dx[RW][V], dy[RW][V];
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const int x = e + 1;
+          const UDeriv dl = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], x);
+          apply_op_sum(kl, dl, dx[r][e], dy[r][e]);
+          // gradient terms on the band rows of the output tile (lam^{m+1} with u^m)
+          if (r >= bd.own_lo && r < bd.own_hi) {
+            sbp_sums(dl, kD ? uxr[r + 1].v[x] : urx[r][e], kD ? uyr[r + 1].v[x] : ury[r][e], S);
+            if constexpr (kD) {
+              const UDeriv du = uderivs<V>(uxr[r], uxr[r + 1], uxr[r + 2], uyr[r], uyr[r + 1], uyr[r + 2], x);
+              float ox, oy;
+              apply_op_sum(ku, du, ox, oy);
+              gacc[r][e] += __builtin_fmaf(l[0][r][e], ox, l[1][r][e] * oy);
+            }
+          }
+        }
+      // The gradient sums are complete here.  The empty asm statements pin
+      // them in place: left free, the compiler sinks the gradient arithmetic
+      // below the injection branch and keeps every row's derivatives live
+      // across it.
+#pragma unroll
+      for (int p = 0; p < kPartials; ++p) asm volatile("" : "+v"(S[p]));
+      if constexpr (kD) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r)
+#pragma unroll
+          for (int e = 0; e < V; ++e) asm volatile("" : "+v"(gacc[r][e]));
+      }
+
+      // residual injection on recorded steps m = kP + 1 (the whole region)
+      if (((m - 1) % a.stride == 0) && ((m - 1) / a.stride < nf)) {
+        const int k = (m - 1) / a.stride;
+        if (a.snapshot) {
+#pragma unroll
+          for (int r = 0; r < RW; ++r) {
+            if (!bd.row_in(I, r)) continue;
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              if (bd.col_in[e]) {
+                const gfloat* gx = C.gtr + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + bd.gj0 + e;
+                dx[r][e] += gx[0];
+                dy[r][e] += gx[(int64_t)nf * I.nx * I.ny];
+              }
+          }
+        } else if (use_map) {
+          if (rbits) {
+            const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec;
+            const int64_t cy = (int64_t)nf * I.n_rec;
+            // every row's residual loads are issued before the first is added
+            float rx[RW][V], ry[RW][V];
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
+#pragma unroll
+              for (int e = 0; e < V; ++e) {
+                rx[r][e] = 0.f;
+                ry[r][e] = 0.f;
+                if ((rbits >> (r * V + e)) & 1u) {
+                  rx[r][e] = gx[ridx[r][e]];
+                  ry[r][e] = gx[ridx[r][e] + cy];
+                }
+              }
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
+#pragma unroll
+              for (int e = 0; e < V; ++e) {
+                dx[r][e] += rx[r][e];
+                dy[r][e] += ry[r][e];
+              }
+          }
+        } else {
+          for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
+            const RecEv ev = a.rec[rp];
+            const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
+            if (rr < 0 || rr >= RW || lane != lc / V) continue;
+            const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+            const float vgx = gx[0], vgy = gx[(int64_t)nf * I.n_rec];
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
+#pragma unroll
+              for (int e = 0; e < V; ++e)
+                if (r == rr && e == lc % V) {
+                  dx[r][e] += vgx;
+                  dy[r][e] += vgy;
+                }
+          }
+        }
+      }
+      if constexpr (MODE == kConst6) {   // cells outside the grid keep their zero state
+        if (bd.edge) {
+#pragma unroll
+          for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              if (!(bd.row_in(I, r) && bd.col_in[e])) { dx[r][e] = 0.f; dy[r][e] = 0.f; }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          advance(l[0][r][e], c[0][r][e], wv[0][r][e], dx[r][e]);
+          advance(l[1][r][e], c[1][r][e], wv[1][r][e], dy[r][e]);
+        }
+    }
+    if (s + 1 < a.ks) {
+      if (grad_rows && m - 1 <= I.nt) load_u(s + 1);
+      publish(buf ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // ---- the launch's last level, on the output tile
+  if (bd.col_own) {
+    Rsrc ro[kStateFields];
+#pragma unroll
+    for (int q = 0; q < kStateFields; ++q) ro[q] = field_rsrc(state_field(C, I, a.s_out, q), I.field);
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (r < bd.own_lo || r >= bd.own_hi) continue;
+      const uint32_t so = bd.srow(I, r);
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        if (!bd.col_in[e]) continue;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          bstore(l[q][r][e], ro[q], bd.vcol[e], so);
+          bstore(c[q][r][e], ro[2 + q], bd.vcol[e], so);
+          bstore(wv[q][r][e], ro[4 + q], bd.vcol[e], so);
+        }
+      }
+    }
+  }
+  // constant-gradient partials of this wavefront (slot t WB + w; lane 8q holds slot q)
+  {
+    double Sd[kPartials];
+#pragma unroll
+    for (int p = 0; p < kPartials; ++p) Sd[p] = bd.col_own ? (double)S[p] : 0.0;
+    const double tot = wave_sum8(Sd, lane);
+    const int q = lane >> 3;
+    const double wq = (q == 2 || q >= 6) ? 0.25 * gsa : gsa;
+    if ((lane & 7) == 0) *part = part_in + wq * tot;
+  }
+  if constexpr (kD) {
+    // FP64 per-cell accumulator, read here (after the loop, as the asm keeps it)
+    asm volatile("" ::: "memory");
+    gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
+    if (bd.col_own) {
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        if (r < bd.own_lo || r >= bd.own_hi) continue;
+        const uint32_t so = bd.srow(I, r) / (uint32_t)sizeof(float);
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (bd.col_in[e]) {
+            gdouble* pa = acc64 + so + bd.vcol[e] / (uint32_t)sizeof(float);
+            *pa = *pa + gsa * (double)gacc[r][e];
+          }
+      }
+    }
+  }
+}
+
+template <int MODE, int V, int RW, int KS>
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockAdjArgs) {
+  adj_block_body<MODE, V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
 
 // the argument is read in place through the kernarg segment (offset 0)
@@ -1069,28 +1513,25 @@ __global__ void __launch_bounds__(kWave * blk_waves(RW)) fwd_block_kernel(BlockA
 // --------------------------------------------------------------------------
 // Fused time-blocked backward: one launch runs KS reverse steps of checkpoint
 // segment s (blocks [0, adj_grid)) and KS recompute steps of segment s - 1
-// (the remaining blocks), which fill the other segment bank.  The two halves
-// touch disjoint slots and have no order between them; on small grids, where
+// (the remaining blocks), which fill the other bank.  The two halves touch
+// disjoint slots and have no order between them; on small grids, where
 // either launch alone leaves most of the chip idle and each step is bound by
 // latency, running them side by side overlaps the two chains.  adj_grid is a
 // multiple of 8, so both halves keep the XCD banding of their block tables.
-// --------------------------------------------------------------------------
 // RWF: rows per wavefront of the recompute half (its own block table).  On
 // small grids taller recompute blocks (RWF = 8 beside RW = 4) cut the grid so
 // that every block of the launch is resident at once.
+// --------------------------------------------------------------------------
 template <int MODE, int V, int RW, int KS, int RWF>
 __global__ void __launch_bounds__(kWave * blk_waves(RW)) fused_block_kernel(FusedBlockArgs fa) {
-#pragma clang fp contract(fast)
   static_assert(blk_waves(RWF) == blk_waves(RW), "both halves run blocks of one size");
+  const auto base = (const char __attribute__((address_space(4)))*)__builtin_amdgcn_kernarg_segment_ptr();
   if (blockIdx.x >= fa.adj_grid) {   // block-uniform
-    const auto base = (const char __attribute__((address_space(4)))*)__builtin_amdgcn_kernarg_segment_ptr();
     fwd_block_body<MODE, V, RWF, KS>((KArg<BlockArgs>*)(base + offsetof(FusedBlockArgs, fwd)),
                                      blockIdx.x - fa.adj_grid);
     return;
   }
-  const BlockAdjArgs& a = fa.adj;
-  const int bid = blockIdx.x;
-#include "ernn_adj_block_body.inc"
+  adj_block_body<MODE, V, RW, KS>((KArg<BlockAdjArgs>*)(base + offsetof(FusedBlockArgs, adj)), blockIdx.x);
 }
 
 // --------------------------------------------------------------------------
@@ -1191,13 +1632,13 @@ __global__ void finalize_field_kernel(const CallDev* call, const InstDev* inst, 
 // --------------------------------------------------------------------------
 // Launchers (host side, called from ernn_plan.cpp)
 // --------------------------------------------------------------------------
-// Tile shapes with a compiled kernel: V columns per lane x R rows, V * R <= 16.
+// Tile shapes with a compiled kernel: V columns per lane x R <= 8 rows, V * R <= 16.
 // Every built kernel keeps its working set in architectural VGPRs: shapes
 // whose adjoint would spill (to AGPRs or scratch) are not built, the build
 // rejects any kernel that spills, and the host refuses the missing shapes
 // (adj_tile_ok in ernn_plan.cpp).
 #define ERNN_TILE_CASES(X) \
-  X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(1, 16) X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(4, 4)
+  X(1, 1) X(1, 2) X(1, 4) X(1, 8) X(2, 1) X(2, 2) X(2, 4) X(2, 8) X(4, 1) X(4, 2) X(4, 4)
 #define ERNN_TILE_CASES_DFIELD_ADJ(X) \
   X(1, 1) X(1, 2) X(1, 4) X(2, 1) X(2, 2) X(2, 4) X(4, 1) X(4, 2)
 #define ERNN_TILE_CASES_FIELD6_ADJ(X) X(1, 1) X(1, 2) X(2, 1) X(2, 2) X(4, 1)
@@ -1338,7 +1779,8 @@ hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t 
 // fused_ff_ok in ernn_plan.cpp): the small-grid shapes, constant and damage
 // modes; the recompute half runs RWF rows per wavefront (the adjoint's RW, or
 // 8 beside RW = 4).
-#define ERNN_FUSED_BLOCK_CASES(X) X(1, 4, 2, 4) X(1, 4, 4, 4) X(1, 8, 4, 8) X(1, 4, 2, 8) X(1, 4, 4, 8)
+#define ERNN_FUSED_BLOCK_CASES_FIELD(X) X(1, 4, 2, 4) X(1, 4, 4, 4) X(1, 4, 2, 8) X(1, 4, 4, 8)
+#define ERNN_FUSED_BLOCK_CASES(X) ERNN_FUSED_BLOCK_CASES_FIELD(X) X(1, 8, 4, 8)
 #define ERNN_FUSED_BLOCK_CASE(v_, r_, k_, f_)                                                         \
   if (V == v_ && fa.adj.rw == r_ && fa.adj.ksmax == k_ && fa.fwd.rw == f_) {                        \
     hipLaunchKernelGGL((fused_block_kernel<MODE, v_, r_, k_, f_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, \
@@ -1347,7 +1789,11 @@ hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t 
   }
 template <int MODE>
 static hipError_t launch_fused_block_m(const FusedBlockArgs& fa, int V, int grid, hipStream_t st) {
-  ERNN_FUSED_BLOCK_CASES(ERNN_FUSED_BLOCK_CASE)
+  if constexpr (MODE == kConst6) {
+    ERNN_FUSED_BLOCK_CASES(ERNN_FUSED_BLOCK_CASE)
+  } else {
+    ERNN_FUSED_BLOCK_CASES_FIELD(ERNN_FUSED_BLOCK_CASE)
+  }
   return hipErrorInvalidValue;
 }
 #undef ERNN_FUSED_BLOCK_CASE

@@ -3,24 +3,25 @@
 // recompute, reverse-time adjoint), optional hipGraph capture, and the C ABI
 // declared in include/ernn.h.
 //
-// Schedule (K = checkpoint interval, S = ceil(nt_max / K) checkpoints):
-//  forward   steps n = 0 .. nt-1 produce u^{n+1} from (u^n, u^{n-1}).  State m
-//            lives in a 3-slot ring unless m = sK - 1 or m = sK (1 <= s < S),
-//            which go straight into checkpoint s; checkpoint 0 is the zero
-//            initial state (u^-1, u^0) of PI:256-259.  The time-blocked
-//            forward runs KS steps per launch and keeps only the checkpoint
-//            states and each launch's last two states, which alternate between
-//            two pairs of a 4-slot ring by launch parity.
-//  adjoint   for s = S-1 .. 0: recompute u^{sK+1} .. u^{top} (top =
-//            min((s+1)K, nt)) from checkpoint s into K segment slots, then run
-//            the adjoint steps m = top .. sK+1, which read u^m from the segment
-//            and rotate lam through a 3-slot ring.  The time-blocked adjoint
-//            runs KS reverse steps per launch and hands its last two lam
-//            states to the next launch through the other pair of a 4-slot
-//            ring.  The forward has no atomics and a fixed order, so the
+// Schedule (K = checkpoint interval, S = ceil(nt_max / K) checkpoints).  A
+// time level of either recurrence is a state (u, c, v) in a state slot
+// (ernn_internal.h); every launch reads one level and writes its last level to
+// a state slot other than the one it reads.
+//  forward   steps n = 0 .. nt-1 take level n to level n + 1.  Levels sK (s <
+//            S) are checkpoints and go to their own state slots; checkpoint 0
+//            is the zero initial state of PI:256-259 (u^0 = u^-1 = 0).  Other
+//            levels alternate between two ring slots; the time-blocked forward
+//            runs KS steps per launch and writes only its last level and the
+//            checkpoints inside it.  Levels of the last checkpoint segment also
+//            go to the bank (u only), as its recompute would leave them.
+//  adjoint   for s = S-1 .. 0: recompute levels sK+1 .. top (top = min((s+1)K,
+//            nt)) from checkpoint s, writing every u^m to the segment's bank,
+//            then run the adjoint steps m = top .. sK+1 (lam level m + 1 ->
+//            m), which read u^m from the bank; lam's levels alternate between
+//            two slots.  The forward has no atomics and a fixed order, so the
 //            recomputed states equal the forward's bit for bit.
-//  fused     small grids: one launch runs adjoint step m of segment s and
-//            recompute step n of segment s - 1 (two segment banks).
+//  fused     small grids: one launch runs adjoint work of segment s and
+//            recompute work of segment s - 1 (two banks).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -79,10 +80,10 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 // Tile shapes with a compiled kernel (ERNN_TILE_CASES in ernn_kernels.hip):
-// V in {1, 2, 4} columns per lane, R a power of two rows, V * R <= 16.
+// V in {1, 2, 4} columns per lane, R a power of two rows up to 8, V * R <= 16.
 bool tile_ok(int V, int R) {
   if (!(V == 1 || V == 2 || V == 4)) return false;
-  if (R < 1 || R > kRowAlign || (R & (R - 1)) != 0) return false;
+  if (R < 1 || R > 8 || (R & (R - 1)) != 0) return false;
   return V * R <= 16;
 }
 // Adjoint shapes that would spill registers are not built (ERNN_TILE_CASES_*
@@ -125,7 +126,7 @@ bool adj_block_ok(int mode, int V, int RW, int KS) {
 // plan the adjoint blocking shape runs both halves of each fused launch.
 bool fused_block_ok(int mode, int V, int RW, int KS) {
   if (mode == ERNN_PARAM_FIELD6 || V != 1) return false;
-  return (RW == 4 && (KS == 2 || KS == 4)) || (RW == 8 && KS == 4);
+  return (RW == 4 && (KS == 2 || KS == 4)) || (mode == ERNN_PARAM_CONST6 && RW == 8 && KS == 4);
 }
 // Rows per wavefront of the fused kernel's recompute half (ERNN_FUSED_BLOCK_CASES):
 // the adjoint's, or 8 beside the 4-row adjoint shapes.
@@ -138,7 +139,7 @@ bool ablk_ok(int mode, bool fuse, int V, int RW, int KS) {
 const char* const kAblkShapes =
     "time-blocked adjoint shape (V, RW, KS): constant or damage mode; unfused: V = 1 with RW in {4,8}, "
     "KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) (1, 8, 5) or V = 2, RW = 4, KS in {2,4}; fused: "
-    "(1, 4, 2), (1, 4, 4) or (1, 8, 4)";
+    "(1, 4, 2), (1, 4, 4) or (constant) (1, 8, 4)";
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
@@ -201,41 +202,22 @@ struct Plan {
   const void* seg_ws = nullptr;
   hipStream_t cap_stream = nullptr;
 
-  // slots of one instance's state pool: ring (4), checkpoints, segment bank(s), lam ring (4)
+  // An instance's state slots (kStateFields fields each): two forward ring
+  // slots, the checkpoints ck(s) = level sK, two lam ring slots; its bank
+  // slots (u only): banks x K, bank(b, i) holding u^{sK+1+i} of a segment s
+  // with bank_of(s) = b.
   int banks() const { return fuse ? 2 : 1; }
   int ring(int r) const { return r; }
-  int ck(int s, int q) const { return 4 + 2 * s + q; }
-  int seg(int bank, int q) const { return 4 + 2 * S + bank * K + q; }
-  int lam(int r) const { return 4 + 2 * S + banks() * K + r; }
-  int n_slots() const { return 4 + 2 * S + banks() * K + 4; }
+  int ck(int s) const { return 2 + s; }
+  int lam(int r) const { return 2 + S + r; }
+  int n_state() const { return 4 + S; }
+  int n_bank() const { return banks() * K; }
   int bank_of(int s) const { return fuse ? (s & 1) : 0; }
-
-  int slot_fwd(int m) const {
-    if (m == -1) return ck(0, 0);
-    if (m == 0) return ck(0, 1);
-    if ((m + 1) % K == 0 && (m + 1) / K < S) return ck((m + 1) / K, 0);
-    if (m % K == 0 && m / K < S) return ck(m / K, 1);
-    return ring(m % 3);
-  }
-  int slot_seg(int s, int m) const {
-    if (m == s * K - 1) return ck(s, 0);
-    if (m == s * K) return ck(s, 1);
-    return seg(bank_of(s), m - s * K - 1);
-  }
-  // Time-blocked forward pass: where state m is kept, or -1.  State m comes
-  // from launch L = (m - 1) / KS at position (m - 1) % KS; checkpoint states
-  // keep their slots, and the last two states of launch L go to ring pair
-  // (L + 1) & 1, which launch L does not read.
-  int slot_blk(int m) const {
-    if (m <= 0) return slot_fwd(m);
-    const int sf = slot_fwd(m);
-    if (sf >= 4) return sf;
-    const int KS = blk.KS;
-    const int L = (m - 1) / KS, pos = (m - 1) % KS;
-    const int ks = std::min(KS, nt_max - L * KS);
-    if (pos >= ks - 2) return ring(2 * ((L + 1) & 1) + (pos - (ks - 2)));
-    return -1;
-  }
+  int bank(int s, int m) const { return bank_of(s) * K + (m - s * K - 1); }   // u^m, sK < m <= top
+  bool is_ck(int m) const { return m % K == 0 && m / K < S; }
+  int other_ring(int q) const { return q == ring(0) ? ring(1) : ring(0); }
+  int other_lam(int q) const { return q == lam(0) ? lam(1) : lam(0); }
+  int seg_top(int s) const { return std::min((s + 1) * K, nt_max); }
 };
 
 template <typename T>
@@ -279,7 +261,7 @@ int64_t tiles_for(const Plan& p, int V, int R) {
 
 int build_tiling(Plan* p, Tiling* t, int V, int R, int bpc) {
   if (!tile_ok(V, R))
-    return fail(ERNN_E_INVALID, "tile (V, R) must have V in {1,2,4}, R a power of two, V*R <= 16");
+    return fail(ERNN_E_INVALID, "tile (V, R) must have V in {1,2,4}, R a power of two up to 8, V*R <= 16");
   if (t == &p->fwd && !fwd_tile_ok(p->mode, V, R))
     return fail(ERNN_E_INVALID, "forward tile (V, R) with V*R > 8 is not built for six fields");
   if (t == &p->adj && !adj_tile_ok(p->mode, V, R))
@@ -704,7 +686,9 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
     D.gscale = (double)D.dt2 / (double)D.rho;
     D.field = (round_up(I.nx, kRowAlign) + 3) * (int64_t)D.pitch;
     D.pool_off = off;
-    off += round_up((int64_t)p->n_slots() * 2 * D.field, 64);
+    off += round_up((int64_t)p->n_state() * kStateFields * D.field, 64);
+    D.bank_off = off;
+    off += round_up((int64_t)p->n_bank() * kBankFields * D.field, 64);
     D.trace_off = p->trace_off[b];
     D.wav_off = 0;
   }
@@ -722,10 +706,6 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
       off += round_up((int64_t)p->n_coef * p->inst[p->set_first[s]].field, 64);
     }
     for (int b = 0; b < p->n_inst; ++b) p->inst[b].coef_off = coef_off[p->inst[b].set];
-    for (int b = 0; b < p->n_inst; ++b) {
-      p->inst[b].acc_off = off;
-      off += round_up((int64_t)p->n_acc * p->inst[b].field, 64);
-    }
     for (int b = 0; b < p->n_inst; ++b) {
       p->inst[b].acc64_off = off / 2;
       off += round_up((int64_t)2 * p->n_acc * p->inst[b].field, 64);
@@ -832,32 +812,42 @@ BlockArgs block_args(const Plan* p, const BlockTiling& bt) {
 }
 BlockArgs block_args(const Plan* p) { return block_args(p, p->blk); }
 
-// One time-blocked launch of the forward pass (record = 1) from step n0, up
-// to step `end` at most.
-int launch_blk_forward(const Plan* p, BlockArgs& a, int n0, int end, hipStream_t st) {
+// Time-blocked forward-pass launch from level n0 (state slot *in) up to level
+// `end` at most: writes the checkpoint levels inside it, its last level to the
+// ring slot it does not read (when that level is no checkpoint), and the
+// levels of the last checkpoint segment to its bank.  *in becomes the slot of
+// the last level.
+void blk_forward_args(const Plan* p, BlockArgs& a, int n0, int end, int* in) {
+  const int B = (p->S - 1) * p->K;
   a.n0 = n0;
-  a.ks = std::min(p->blk.KS, end - n0);
-  a.s_prev = p->slot_blk(n0 - 1);
-  a.s_cur = p->slot_blk(n0);
-  for (int s = 0; s < kMaxBlockSteps; ++s) a.store[s] = s < a.ks ? p->slot_blk(n0 + 1 + s) : -1;
-  ERNN_HIP(launch_fwd_block(a, p->blk.V, p->blk.grid, st));
-  return ERNN_OK;
+  a.ks = std::min(a.ksmax, end - n0);
+  a.s_in = *in;
+  for (int j = 0; j < kMaxBlockSteps; ++j) a.st_store[j] = a.bank_store[j] = -1;
+  for (int j = 0; j < a.ks; ++j) {
+    const int m = n0 + 1 + j;
+    if (p->is_ck(m)) a.st_store[j] = p->ck(m / p->K);
+    else if (j == a.ks - 1) a.st_store[j] = p->other_ring(*in);
+    if (m > B) a.bank_store[j] = p->bank(p->S - 1, m);
+  }
+  *in = a.st_store[a.ks - 1];
 }
 
-// Time-blocked recompute arguments for segment s from step n0: every state
-// is stored.
-void blk_recompute_args(const Plan* p, BlockArgs& a, int s, int n0, int top) {
+// Time-blocked recompute launch of segment s from level n0 (state slot *in):
+// every level into the segment's bank, the last one also to the ring slot the
+// launch does not read unless it ends the segment.
+void blk_recompute_args(const Plan* p, BlockArgs& a, int s, int n0, int* in) {
+  const int top = p->seg_top(s);
   a.n0 = n0;
   a.ks = std::min(a.ksmax, top - n0);
-  a.s_prev = p->slot_seg(s, n0 - 1);
-  a.s_cur = p->slot_seg(s, n0);
-  for (int q = 0; q < kMaxBlockSteps; ++q) a.store[q] = q < a.ks ? p->slot_seg(s, n0 + 1 + q) : -1;
-}
-int launch_blk_recompute(const Plan* p, const BlockTiling& bt, BlockArgs& a, int s, int n0, int top,
-                         hipStream_t st) {
-  blk_recompute_args(p, a, s, n0, top);
-  ERNN_HIP(launch_fwd_block(a, bt.V, bt.grid, st));
-  return ERNN_OK;
+  a.s_in = *in;
+  for (int j = 0; j < kMaxBlockSteps; ++j) {
+    a.st_store[j] = -1;
+    a.bank_store[j] = j < a.ks ? p->bank(s, n0 + 1 + j) : -1;
+  }
+  if (n0 + a.ks < top) {
+    a.st_store[a.ks - 1] = p->other_ring(*in);
+    *in = a.st_store[a.ks - 1];
+  }
 }
 
 BlockAdjArgs block_adj_args(const Plan* p) {
@@ -876,58 +866,63 @@ BlockAdjArgs block_adj_args(const Plan* p) {
   return a;
 }
 
-// Time-blocked adjoint launch L over m0 .. m0 - ks + 1: reads lam^{m0+1},
-// lam^{m0+2} from lam pair L & 1 (slots 1, 0), leaves its last two states in
-// pair (L + 1) & 1; u^m comes from slot u_of(m).
-template <typename UOf>
-void blk_adjoint_args(const Plan* p, BlockAdjArgs& a, int L, int m0, int ks, int seg_top, int seg_bot, UOf u_of) {
-  const int pr = 2 * (L & 1), pw = 2 * ((L + 1) & 1);
+// Time-blocked adjoint launch over lam levels m0 + 1 -> m0 - ks + 1 (state
+// slot *lin, then the other lam slot), reading u^m of segment s from its bank.
+void blk_adjoint_args(const Plan* p, BlockAdjArgs& a, int s, int m0, int ks, int* lin) {
   a.m0 = m0;
   a.ks = ks;
-  a.s_l1 = p->lam(pr + 1);
-  a.s_l2 = p->lam(pr);
-  a.store_in = ks == 1 ? p->lam(pw) : -1;
-  for (int j = 0; j < kMaxBlockSteps; ++j) {
-    a.store[j] = j == ks - 1 ? p->lam(pw + 1) : (j == ks - 2 ? p->lam(pw) : -1);
-    a.u_slot[j] = j < ks ? u_of(m0 - j) : 0;
-  }
-  a.seg_top = seg_top;
-  a.seg_bot = seg_bot;
+  a.s_in = *lin;
+  a.s_out = p->other_lam(*lin);
+  for (int j = 0; j < kMaxBlockSteps; ++j) a.bank_u[j] = j < ks ? p->bank(s, m0 - j) : 0;
+  *lin = a.s_out;
 }
-template <typename UOf>
-int launch_blk_adjoint(const Plan* p, BlockAdjArgs& a, int L, int m0, int ks, int seg_top, int seg_bot, UOf u_of,
-                       hipStream_t st) {
-  blk_adjoint_args(p, a, L, m0, ks, seg_top, seg_bot, u_of);
-  ERNN_HIP(launch_adj_block(a, p->blk_adj.V, p->blk_adj.grid, st));
-  return ERNN_OK;
+
+// forward step n of the single-step schedule: level n (slot in) -> n + 1
+void step_forward_args(const Plan* p, StepArgs& a, int n, int* in, bool bank_last_segment) {
+  const int m = n + 1;
+  a.step = n;
+  a.s_in = *in;
+  a.s_out = p->is_ck(m) ? p->ck(m / p->K) : p->other_ring(*in);
+  a.bank = (bank_last_segment && m > (p->S - 1) * p->K) ? p->bank(p->S - 1, m) : -1;
+  *in = a.s_out;
+}
+// recompute step n of segment s: level n -> n + 1 into the bank (and a ring slot)
+void step_recompute_args(const Plan* p, StepArgs& a, int s, int n, int* in) {
+  a.step = n;
+  a.s_in = *in;
+  a.s_out = p->other_ring(*in);
+  a.bank = p->bank(s, n + 1);
+  *in = a.s_out;
+}
+// adjoint step m of segment s: lam level m + 1 -> m, u^m from the bank
+void step_adjoint_args(const Plan* p, StepArgs& a, int s, int m, int* lin) {
+  a.step = m;
+  a.s_in = *lin;
+  a.s_out = p->other_lam(*lin);
+  a.bank = p->bank(s, m);
+  *lin = a.s_out;
 }
 
 // The forward pass keeps the checkpoints and, for the last checkpoint
-// segment (steps from B = (S - 1) K on), every state in that segment's bank,
-// as its recompute would: the adjoint that follows on the same workspace
-// starts on them instead of recomputing the segment (seg_ws).  States B - 1
-// and B are the segment's checkpoints under both slot maps.
+// segment, every level's u in that segment's bank, as its recompute would: the
+// adjoint that follows on the same workspace starts on them instead of
+// recomputing the segment (seg_ws).
 int enqueue_forward(const Plan* p, hipStream_t st) {
   ERNN_TRY(enqueue_pack(p, st));
-  const int B = (p->S - 1) * p->K;
+  int in = p->ck(0);
   if (p->blk.KS > 1) {
     BlockArgs b = block_args(p);
     b.record = 1;
-    for (int n0 = 0; n0 < B; n0 += p->blk.KS) ERNN_TRY(launch_blk_forward(p, b, n0, B, st));
-    for (int n0 = B; n0 < p->nt_max; n0 += p->blk.KS) {
-      blk_recompute_args(p, b, p->S - 1, n0, p->nt_max);
+    for (int n0 = 0; n0 < p->nt_max; n0 += p->blk.KS) {
+      blk_forward_args(p, b, n0, p->nt_max, &in);
       ERNN_HIP(launch_fwd_block(b, p->blk.V, p->blk.grid, st));
     }
     return ERNN_OK;
   }
-  const auto slot = [&](int m) { return m >= B - 1 ? p->slot_seg(p->S - 1, m) : p->slot_fwd(m); };
   StepArgs a = base_args(p, p->fwd);
   a.record = 1;
   for (int n = 0; n < p->nt_max; ++n) {
-    a.step = n;
-    a.s_prev = slot(n - 1);
-    a.s_cur = slot(n);
-    a.s_next = slot(n + 1);
+    step_forward_args(p, a, n, &in, true);
     ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
   }
   return ERNN_OK;
@@ -937,10 +932,12 @@ int enqueue_forward(const Plan* p, hipStream_t st) {
 // pass on this workspace), so its recompute is skipped.
 int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
   ERNN_TRY(enqueue_pack(p, st));
-  // zero: lam ring, per-tile partials, fp64 per-cell accumulators
+  // zero: the two lam slots (lam level nt + 1 is the zero state), per-tile
+  // partials, fp64 per-cell accumulators
   for (int b = 0; b < p->n_inst; ++b) {
     const InstDev& D = p->inst[b];
-    ERNN_HIP(launch_fill_ws(p->d_call, D.pool_off + (int64_t)p->lam(0) * 2 * D.field, 4 * 2 * D.field, st));
+    ERNN_HIP(launch_fill_ws(p->d_call, D.pool_off + (int64_t)p->lam(0) * kStateFields * D.field,
+                            2 * kStateFields * D.field, st));
     if (p->n_acc)
       ERNN_HIP(launch_fill_ws(p->d_call, 2 * D.acc64_off, 2 * (int64_t)p->n_acc * D.field, st));
   }
@@ -949,30 +946,26 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
   StepArgs f = base_args(p, p->fwd);
   f.record = 0;
   StepArgs a = base_args(p, p->adj);
-  // recompute step n of segment s (u^{n+1} into the segment's bank)
-  auto recompute = [&](StepArgs& r, int s, int n) {
-    r.step = n;
-    r.s_prev = p->slot_seg(s, n - 1);
-    r.s_cur = p->slot_seg(s, n);
-    r.s_next = p->slot_seg(s, n + 1);
-  };
-  auto seg_top = [&](int s) { return std::min((s + 1) * p->K, p->nt_max); };
   BlockArgs rb = block_args(p);
   rb.record = 0;
   // recompute segment s whole (time-blocked when enabled)
   auto recompute_segment = [&](int s) -> int {
-    const int top = seg_top(s);
+    int in = p->ck(s);
+    const int top = p->seg_top(s);
     if (p->blk.KS > 1) {
-      for (int n0 = s * p->K; n0 < top; n0 += p->blk.KS)
-        ERNN_TRY(launch_blk_recompute(p, p->blk, rb, s, n0, top, st));
+      for (int n0 = s * p->K; n0 < top; n0 += p->blk.KS) {
+        blk_recompute_args(p, rb, s, n0, &in);
+        ERNN_HIP(launch_fwd_block(rb, p->blk.V, p->blk.grid, st));
+      }
       return ERNN_OK;
     }
     for (int n = s * p->K; n < top; ++n) {
-      recompute(f, s, n);
+      step_recompute_args(p, f, s, n, &in);
       ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
     }
     return ERNN_OK;
   };
+  int lin = p->lam(0);
   if (p->fuse && p->blk_adj.KS > 1) {
     // The last segment is recomputed alone (unless stored); afterwards every
     // time-blocked adjoint launch of segment s carries the next KS recompute
@@ -986,50 +979,44 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
     fb.fwd.record = 0;
     fb.adj_grid = p->blk_adj.grid;
     const int grid = p->blk_adj.grid + p->blk_ff.grid;
-    int L = 0;
     for (int s = p->S - 1; s >= 0; --s) {
-      const int top = seg_top(s), bot = s * p->K + 1;
-      const auto u_of = [&](int m) { return p->slot_seg(s, m); };
+      const int top = p->seg_top(s), bot = s * p->K + 1;
       int n0 = (s - 1) * p->K;   // next recompute step of segment s - 1
-      for (int m0 = top; m0 >= bot; m0 -= KS, ++L) {
+      int rin = s >= 1 ? p->ck(s - 1) : 0;
+      for (int m0 = top; m0 >= bot; m0 -= KS) {
         const int ks = std::min(KS, m0 - bot + 1);
+        blk_adjoint_args(p, fb.adj, s, m0, ks, &lin);
         if (s >= 1 && n0 < s * p->K) {
-          blk_adjoint_args(p, fb.adj, L, m0, ks, top, bot, u_of);
-          blk_recompute_args(p, fb.fwd, s - 1, n0, s * p->K);
+          blk_recompute_args(p, fb.fwd, s - 1, n0, &rin);
           ERNN_HIP(launch_fused_block(fb, p->blk_adj.V, grid, st));
           n0 += KS;
         } else {
-          ERNN_TRY(launch_blk_adjoint(p, fb.adj, L, m0, ks, top, bot, u_of, st));
+          ERNN_HIP(launch_adj_block(fb.adj, p->blk_adj.V, p->blk_adj.grid, st));
         }
       }
       for (; s >= 1 && n0 < s * p->K; n0 += KS) {
         BlockArgs r = block_args(p, p->blk_ff);
         r.record = 0;
-        ERNN_TRY(launch_blk_recompute(p, p->blk_ff, r, s - 1, n0, s * p->K, st));
+        blk_recompute_args(p, r, s - 1, n0, &rin);
+        ERNN_HIP(launch_fwd_block(r, p->blk_ff.V, p->blk_ff.grid, st));
       }
     }
   } else if (!p->fuse && p->blk_adj.KS > 1) {
     BlockAdjArgs ba = block_adj_args(p);
-    int L = 0;
     for (int s = p->S - 1; s >= 0; --s) {
-      const int top = seg_top(s), bot = s * p->K + 1;
+      const int top = p->seg_top(s), bot = s * p->K + 1;
       if (s < p->S - 1 || !skip_last) ERNN_TRY(recompute_segment(s));
-      for (int m0 = top; m0 >= bot; m0 -= p->blk_adj.KS, ++L)
-        ERNN_TRY(launch_blk_adjoint(p, ba, L, m0, std::min(p->blk_adj.KS, m0 - bot + 1), top, bot,
-                                    [&](int m) { return p->slot_seg(s, m); }, st));
+      for (int m0 = top; m0 >= bot; m0 -= p->blk_adj.KS) {
+        blk_adjoint_args(p, ba, s, m0, std::min(p->blk_adj.KS, m0 - bot + 1), &lin);
+        ERNN_HIP(launch_adj_block(ba, p->blk_adj.V, p->blk_adj.grid, st));
+      }
     }
   } else if (!p->fuse) {
     for (int s = p->S - 1; s >= 0; --s) {
-      const int top = seg_top(s), bot = s * p->K + 1;
+      const int top = p->seg_top(s), bot = s * p->K + 1;
       if (s < p->S - 1 || !skip_last) ERNN_TRY(recompute_segment(s));
-      a.seg_top = top;
-      a.seg_bot = bot;
       for (int m = top; m >= bot; --m) {
-        a.step = m;
-        a.s_prev = p->lam((m + 2) % 3);
-        a.s_cur = p->lam((m + 1) % 3);
-        a.s_next = p->lam(m % 3);
-        a.s_u = p->seg(0, m - s * p->K - 1);
+        step_adjoint_args(p, a, s, m, &lin);
         ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, p->adj.loop, st));
       }
     }
@@ -1043,19 +1030,14 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
     fa.fwd = base_args(p, p->adj);   // the fused kernel runs both on the adjoint's tiles
     fa.fwd.record = 0;
     for (int s = p->S - 1; s >= 0; --s) {
-      const int top = seg_top(s), bot = s * p->K + 1;
-      a.seg_top = top;
-      a.seg_bot = bot;
+      const int top = p->seg_top(s), bot = s * p->K + 1;
+      int rin = s >= 1 ? p->ck(s - 1) : 0;
       int idx = 0;
       for (int m = top; m >= bot; --m, ++idx) {
-        a.step = m;
-        a.s_prev = p->lam((m + 2) % 3);
-        a.s_cur = p->lam((m + 1) % 3);
-        a.s_next = p->lam(m % 3);
-        a.s_u = p->seg(p->bank_of(s), m - s * p->K - 1);
+        step_adjoint_args(p, a, s, m, &lin);
         if (s >= 1 && idx < p->K) {
           fa.adj = a;
-          recompute(fa.fwd, s - 1, (s - 1) * p->K + idx);
+          step_recompute_args(p, fa.fwd, s - 1, (s - 1) * p->K + idx, &rin);
           ERNN_HIP(launch_fused_step(fa, p->adj.V, p->adj.fused_blocks, st));
         } else {
           ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, p->adj.loop, st));
@@ -1063,7 +1045,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
       }
       // a short last segment leaves recompute steps of segment s - 1 to run alone
       for (; s >= 1 && idx < p->K; ++idx) {
-        recompute(f, s - 1, (s - 1) * p->K + idx);
+        step_recompute_args(p, f, s - 1, (s - 1) * p->K + idx, &rin);
         ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
       }
     }
@@ -1288,68 +1270,67 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   hipEvent_t e0 = ev.e0, e1 = ev.e1;
   ERNN_HIP(hipEventRecord(e0, st));
   const int nsteps = std::min(steps, p->nt_max);
-  if (which == 4) {   // `steps` time-blocked adjoint launches of KS steps each, u^m from segment bank 0
-    // (fuse plans: fused launches, u^m from bank 1 while segment 0 is recomputed into bank 0)
-    BlockAdjArgs ba = block_adj_args(p);
+  // The timed launches follow the schedules' slot use: the adjoint reads u^m
+  // from bank 0 (fuse plans: bank 1, while segment 0 is recomputed into bank 0).
+  if (which == 4) {   // `steps` time-blocked adjoint launches of KS steps each
     const int KS = p->blk_adj.KS;
     const int launches = std::min(steps, (p->nt_max + KS - 1) / KS);
+    const int useg = p->fuse ? 1 : 0;   // bank_of(useg) is the bank the adjoint reads
     FusedBlockArgs fb{};
+    fb.adj = block_adj_args(p);
     fb.fwd = block_args(p, p->blk_ff);
     fb.fwd.record = 0;
     fb.adj_grid = p->blk_adj.grid;
+    int lin = p->lam(0), rin = p->ck(0), n0 = 0;
+    const int top0 = p->seg_top(0);
     for (int L = 0; L < launches; ++L) {
-      const int m0 = p->nt_max - L * KS;
-      const int bank = p->fuse ? 1 : 0;
-      const auto u_of = [&](int m) { return p->seg(bank, m % p->K); };
+      // reverse steps within one segment's bank: m0 cycles over useg K + KS .. useg K + K
+      const int span = std::max(KS, p->K - p->K % KS);
+      const int m0 = useg * p->K + KS + (L * KS) % std::max(span - KS + 1, 1);
+      blk_adjoint_args(p, fb.adj, useg, std::min(m0, useg * p->K + p->K), std::min(KS, p->K), &lin);
       if (!p->fuse) {
-        ERNN_TRY(launch_blk_adjoint(p, ba, L, m0, std::min(KS, m0), p->nt_max, 1, u_of, st));
+        ERNN_HIP(launch_adj_block(fb.adj, p->blk_adj.V, p->blk_adj.grid, st));
         continue;
       }
-      fb.adj = ba;
-      blk_adjoint_args(p, fb.adj, L, m0, std::min(KS, m0), p->nt_max, 1, u_of);
-      const int n0 = (L * KS) % std::max(KS, p->K - p->K % KS);
-      blk_recompute_args(p, fb.fwd, 0, n0, std::min(p->K, p->nt_max));
+      if (n0 >= top0) { n0 = 0; rin = p->ck(0); }
+      blk_recompute_args(p, fb.fwd, 0, n0, &rin);
+      n0 += KS;
       ERNN_HIP(launch_fused_block(fb, p->blk_adj.V, p->blk_adj.grid + p->blk_ff.grid, st));
     }
   } else if (which == 3) {   // `steps` time-blocked forward launches of KS steps each
     BlockArgs b = block_args(p);
     b.record = 1;
     const int launches = std::min(steps, (p->nt_max + p->blk.KS - 1) / p->blk.KS);
-    for (int L = 0; L < launches; ++L) ERNN_TRY(launch_blk_forward(p, b, L * p->blk.KS, p->nt_max, st));
+    int in = p->ck(0);
+    for (int L = 0; L < launches; ++L) {
+      blk_forward_args(p, b, L * p->blk.KS, p->nt_max, &in);
+      ERNN_HIP(launch_fwd_block(b, p->blk.V, p->blk.grid, st));
+    }
   } else if (which == 0) {
     StepArgs a = base_args(p, p->fwd);
     a.record = 1;
+    int in = p->ck(0);
     for (int n = 0; n < nsteps; ++n) {
-      a.step = n;
-      a.s_prev = p->slot_fwd(n - 1);
-      a.s_cur = p->slot_fwd(n);
-      a.s_next = p->slot_fwd(n + 1);
+      step_forward_args(p, a, n, &in, false);
       ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
     }
   } else {
     StepArgs a = base_args(p, p->adj);
-    a.seg_top = p->nt_max;
-    a.seg_bot = 1;
     FusedArgs fa{};
     fa.fwd = base_args(p, p->adj);
     fa.fwd.record = 0;
+    const int useg = which == 2 ? 1 : 0;
+    int lin = p->lam(0), rin = p->ck(0);
     for (int q = 0; q < nsteps; ++q) {
-      const int m = p->nt_max - q;
-      a.step = m;
-      a.s_prev = p->lam((m + 2) % 3);
-      a.s_cur = p->lam((m + 1) % 3);
-      a.s_next = p->lam(m % 3);
-      a.s_u = p->seg(0, q % p->K);
+      const int m = useg * p->K + p->K - q % p->K;   // u^m from segment useg's bank
+      step_adjoint_args(p, a, useg, m, &lin);
       if (which == 1) {
         ERNN_HIP(launch_adj_step(a, p->adj.V, p->adj.blocks, p->adj.loop, st));
       } else {   // recompute steps of segment 0 into bank 0 while the adjoint reads bank 1
-        a.s_u = p->seg(1, q % p->K);
-        fa.adj = a;
         const int n = q % p->K;
-        fa.fwd.step = n;
-        fa.fwd.s_prev = p->slot_seg(0, n - 1);
-        fa.fwd.s_cur = p->slot_seg(0, n);
-        fa.fwd.s_next = p->slot_seg(0, n + 1);
+        if (n == 0) rin = p->ck(0);
+        fa.adj = a;
+        step_recompute_args(p, fa.fwd, 0, n, &rin);
         ERNN_HIP(launch_fused_step(fa, p->adj.V, p->adj.fused_blocks, st));
       }
     }

@@ -1,12 +1,14 @@
 """Autotuning: checkpoint interval under a memory cap, and tile shapes by timing.
 
-Checkpoint interval K.  Each instance keeps 4 ring slots, 2 states per
-checkpoint (S = ceil(nt / K) checkpoints), K recompute slots per bank (two
-banks when the adjoint is fused with the next segment's recompute) and 4
-adjoint slots, each slot two padded fields.  Recompute cost is one extra
-forward whatever K is, so K only trades memory: slots(K) = 8 + 2 ceil(nt/K) +
-banks K is minimal near K = sqrt(2 nt / banks).  ``choose_ckpt`` returns that
-K, or raises if even the minimum exceeds the cap.
+Checkpoint interval K.  Each instance keeps 4 + S state slots of six padded
+fields (two forward ring slots, the S = ceil(nt / K) checkpoints, two adjoint
+slots; a state is u, its compensation c and its increment v per component)
+and K bank slots of two fields per bank (the recomputed u of one segment; two
+banks when the adjoint is fused with the next segment's recompute).
+Recompute cost is one extra forward whatever K is, so K only trades memory:
+fields(K) = 6 (4 + ceil(nt/K)) + 2 banks K is minimal near K = sqrt(3 nt /
+banks).  ``choose_ckpt`` returns that K, or raises if even the minimum
+exceeds the cap.
 
 Tiles.  ``tune_tiles`` times every candidate (V columns per lane, R rows per
 wavefront) of the forward and the adjoint kernel on the real problem for a
@@ -32,14 +34,19 @@ def padded_field_floats(nx: int, ny: int) -> int:
     return ((nx + ROW_ALIGN - 1) // ROW_ALIGN * ROW_ALIGN + 3) * pitch
 
 
-def slots(nt: int, K: int, banks: int = 1) -> int:
-    return 8 + 2 * math.ceil(nt / K) + banks * K
+STATE_FIELDS = 6   # u, c, v per component (the library's kStateFields)
+BANK_FIELDS = 2    # u per component
+
+
+def fields(nt: int, K: int, banks: int = 1) -> int:
+    """Padded fields per instance for nt steps and checkpoint interval K."""
+    return STATE_FIELDS * (4 + math.ceil(nt / K)) + BANK_FIELDS * banks * K
 
 
 def state_bytes(survey: Survey, K: int) -> int:
     nt = survey.nt_max
     b = 2 if survey.fused else 1
-    return sum(slots(nt, K, b) * 2 * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
+    return sum(fields(nt, K, b) * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
 
 
 def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
@@ -48,16 +55,16 @@ def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
     if nt < 4:
         return 2
     b = 2 if survey.fused else 1
-    best = min(range(2, nt + 1), key=lambda K: (slots(nt, K, b), -K))
+    best = min(range(2, nt + 1), key=lambda K: (fields(nt, K, b), -K))
     if mem_cap_bytes is not None and state_bytes(survey, best) > mem_cap_bytes:
         raise MemoryError(f"checkpointed state needs {state_bytes(survey, best) / 2**30:.2f} GiB "
                           f"> cap {mem_cap_bytes / 2**30:.2f} GiB")
     return best
 
 
-# every compiled tile shape: V columns per lane x R rows, V * R <= 16
+# every compiled tile shape: V columns per lane x R <= 8 rows, V * R <= 16
 TILE_SHAPES: Tuple[Tuple[int, int], ...] = (
-    (1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (2, 1), (2, 2), (2, 4), (2, 8), (4, 1), (4, 2), (4, 4))
+    (1, 1), (1, 2), (1, 4), (1, 8), (2, 1), (2, 2), (2, 4), (2, 8), (4, 1), (4, 2), (4, 4))
 # grid caps in blocks per CU (0: one tile per wavefront; otherwise persistent wavefronts)
 GRID_CAPS: Tuple[int, ...] = (0, 2, 4, 8)
 DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
@@ -69,7 +76,8 @@ BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = (
 # time-blocked adjoint shapes (constant and damage modes; V = 2 for constant stiffness only)
 ADJ_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 6, 4), (1, 8, 2), (1, 8, 4),
                                                       (1, 8, 5), (1, 8, 8), (2, 4, 2), (2, 4, 4))
-# time-blocked shapes of the fused backward (fuse plans): both halves of a launch share one shape
+# time-blocked shapes of the fused backward (fuse plans): both halves of a launch share one shape;
+# (1, 8, 4) is built for six constants only
 FUSED_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 8, 4))
 # largest V * R and R of the built adjoint kernels per mode (larger shapes would spill registers)
 ADJ_MAX_CELLS = {"const6": 16, "dfield": 8, "field6": 4}
@@ -130,7 +138,8 @@ def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int =
         cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or b[:2] != (2, 8)]
     else:
         shapes = (FUSED_BLOCK_SHAPES if fused else ADJ_BLOCK_SHAPES) if shapes is None else shapes
-        cands = [(0, 0, 0)] + ([b for b in shapes if mode == "const6" or (b[0] == 1 and b[2] != 5)]
+        cands = [(0, 0, 0)] + ([b for b in shapes if mode == "const6" or (b[0] == 1 and b[2] != 5 and
+                                                                           not (fused and b[1] == 8))]
                                if mode != "field6" else [])
     ws = engine.acquire_workspace()
     traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)

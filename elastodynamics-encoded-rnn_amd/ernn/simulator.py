@@ -253,8 +253,15 @@ class ERNN(nn.Module):
     """Receiver-trace ERNN over a batch of shots (the benchmark surveys).
 
     Parameters are linear and normalised: C = theta * scale per set and
-    constant (scale = |C_init|, or `zero_scale` for constants that start at 0),
-    which, unlike exp/clamp (PI:223-232), can reach zero and negative values.
+    constant, which, unlike exp/clamp (PI:223-232), can reach zero and negative
+    values.  The default scale is scale_p = |C_init,p|^q |C11_init|^(1-q) with
+    q = scale_power (constants that start at 0 use `zero_scale` for
+    |C_init,p|).  An Adam step of lr moves a constant by about lr * scale_p: q =
+    1 scales each constant by itself, q = 0 all by C11.  In between, the small
+    constants, which the waveforms constrain weakly until C11 has settled, move
+    at a larger fraction of their own value than C11 does; q = 0.4 brings the
+    plain-128 inversion within 2 % in 50 iterations at lr 1e-2, where q = 1
+    needs about 150.
     In dfield mode the per-cell factor D multiplies all six constants and is a
     parameter too (``learn_damage``).  ``trainable`` (six booleans) freezes
     constants known a priori, e.g. C16 = C26 = 0 in an orthotropic frame.
@@ -262,7 +269,7 @@ class ERNN(nn.Module):
     """
 
     def __init__(self, survey: Survey, C_init: np.ndarray, scale: Optional[np.ndarray] = None,
-                 zero_scale: float = 5e9, D_init: Optional[np.ndarray] = None,
+                 zero_scale: float = 5e9, scale_power: float = 0.4, D_init: Optional[np.ndarray] = None,
                  learn_constants: bool = True, learn_damage: bool = True, device=None,
                  autotune: bool = False, trainable: Optional[Sequence[bool]] = None):
         super().__init__()
@@ -270,7 +277,9 @@ class ERNN(nn.Module):
         dev = torch.device(device) if device is not None else torch.device("cuda")
         C_init = np.asarray(C_init, dtype=np.float64).reshape(survey.n_sets, 6)
         if scale is None:
-            scale = np.where(np.abs(C_init) > 0, np.abs(C_init), zero_scale)
+            own = np.where(np.abs(C_init) > 0, np.abs(C_init), zero_scale)
+            ref = np.maximum(np.abs(C_init[:, :1]), zero_scale)   # C11 of each set
+            scale = own ** scale_power * ref ** (1.0 - scale_power)
         self.register_buffer("scale", torch.tensor(np.asarray(scale).reshape(-1, 6), dtype=torch.float32, device=dev))
         self.theta = nn.Parameter(torch.tensor(C_init / scale, dtype=torch.float32, device=dev),
                                   requires_grad=learn_constants)

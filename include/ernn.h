@@ -83,7 +83,7 @@ typedef struct {
   int32_t snapshot;     /* 1: frames are whole fields (PI:304-305); 0: receiver traces */
   int32_t ckpt_every;   /* K >= 2: checkpoint interval of the adjoint recompute */
   int32_t fwd_v, fwd_r; /* forward tile: V in {1,2,4} columns per lane, R in {1,2,4,8,16}
-                           rows, V * R <= 16 (six fields: V * R <= 8); 0, 0 = default (1, 4) */
+                           rows up to 8, V * R <= 16 (six fields: V * R <= 8); 0, 0 = default (1, 4) */
   int32_t adj_v, adj_r; /* adjoint tile: as above, damage field V * R <= 8 and R <= 4, six
                            fields V * R <= 4 and R <= 2; 0, 0 = default (1, 4), six fields (1, 2) */
   int32_t use_graph;    /* 1: replay each pass as a captured hipGraph */
@@ -102,7 +102,7 @@ typedef struct {
                            (1, 6, 4), or (CONST6) (1, 8, 5) or V = 2 with RW = 4 and KS in
                            {2,4}.  fuse = 1:
                            each launch also runs ablk_ks recompute steps of the previous
-                           segment with the same shape, (1, 4, 2), (1, 4, 4) or (1, 8, 4).
+                           segment with the same shape, (1, 4, 2), (1, 4, 4) or (CONST6) (1, 8, 4).
                            ablk_ks <= 1 = off */
   int32_t ablk_ff_rw;   /* fuse = 1 with a time-blocked adjoint: rows per wavefront of the
                            fused kernel's recompute half, 0 = ablk_rw, or 8 beside ablk_rw = 4

@@ -236,6 +236,8 @@ def test_fused_blocked_backward_matches_unfused_and_oracle(mode, shape):
     adjoint launches cannot carry all of segment s - 1's recompute, which then
     runs alone) and one-step launches (7 mod KS); gradients vs the unfused
     single-step backward (same recurrence) and the FP64 oracle."""
+    if mode == "dfield" and shape[1] == 8:
+        pytest.skip("fused (1, 8, 4) is built for six constants only (the damage kernel would spill)")
     from gpu_helpers import oracle_run
     from oracle import ernn_fp64 as orc
     V, RW, KS = shape

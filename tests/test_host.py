@@ -138,14 +138,16 @@ def test_engine_refuses_without_gpu():
 
 
 # ------------------------------------------------------------------ helpers
-def test_checkpoint_interval_minimizes_slots():
+def test_checkpoint_interval_minimizes_state_memory():
+    """fields(K) = 6 (4 + ceil(nt / K)) + 2 banks K padded fields per instance is
+    smallest near K = sqrt(3 nt / banks)."""
     wl = configs.single_512(nt=2000)
     for fuse, banks in ((False, 1), (True, 2)):
         wl.survey.fuse = fuse
         assert wl.survey.fused == fuse
         K = autotune.choose_ckpt(wl.survey)
-        assert autotune.slots(2000, K, banks) == min(autotune.slots(2000, k, banks) for k in range(2, 2001))
-        assert abs(K - math.sqrt(2 * 2000 / banks)) < 8
+        assert autotune.fields(2000, K, banks) == min(autotune.fields(2000, k, banks) for k in range(2, 2001))
+        assert abs(K - math.sqrt(3 * 2000 / banks)) < 8
     with pytest.raises(MemoryError):
         autotune.choose_ckpt(wl.survey, mem_cap_bytes=1 << 20)
 
