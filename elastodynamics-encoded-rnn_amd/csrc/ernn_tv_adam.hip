@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "ernn.h"
@@ -273,6 +274,12 @@ __global__ void __launch_bounds__(kTvThreads) adam_kernel(AdamArgs a) {
 
 using namespace ernn;
 
+// Partial sums of the allocation-free entry point: one array per GPU, held
+// until the process ends; its device address comes from hipGetSymbolAddress
+// (the name itself is a host-side stand-in).
+constexpr int64_t kTvScratchDoubles = int64_t(1) << 16;   // 512 KiB
+static __device__ __align__(256) double scratch[kTvScratchDoubles];
+
 namespace {
 
 int64_t tv_blocks(int nx, int ny) { return ((int64_t)nx * ny + kTvThreads - 1) / kTvThreads; }
@@ -299,28 +306,18 @@ int check_desc(const ernn_tv_adam_desc_t* d) {
   return ERNN_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int ernn_tv_adam_workspace_bytes(const ernn_tv_adam_desc_t* desc, int64_t* bytes) {
-  if (!bytes) return set_last_error(ERNN_E_INVALID, "null argument");
-  const int r = check_desc(desc);
-  if (r != ERNN_OK) return r;
-  *bytes = tv_blocks(desc->nx, desc->ny) * desc->n_fields * (int64_t)sizeof(double);
-  return ERNN_OK;
-}
-
-int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c, void* stream) {
-  const int r = check_desc(d);
-  if (r != ERNN_OK) return r;
-  if (!c || !c->workspace) return set_last_error(ERNN_E_INVALID, "null call or workspace");
-  if ((reinterpret_cast<uintptr_t>(c->workspace) & 7) != 0)
-    return set_last_error(ERNN_E_INVALID, "workspace must be 8-byte aligned");
+int check_fields(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c) {
+  if (!c) return set_last_error(ERNN_E_INVALID, "null call");
   for (int f = 0; f < d->n_fields; ++f)
     if (!c->theta[f] || !c->grad[f] || !c->m[f] || !c->v[f])
       return set_last_error(ERNN_E_INVALID, "theta, grad, m and v of field " + std::to_string(f) + " are required");
-  hipStream_t st = static_cast<hipStream_t>(stream);
+  return ERNN_OK;
+}
+
+// The TV pass, the optional TV finalize and the Adam pass on `st`; `partial`
+// is device memory of at least tv_blocks(nx, ny) x n_fields doubles that every
+// launch writes before it reads (no clearing needed).
+int enqueue_tv_adam(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c, double* partial, hipStream_t st) {
   const int64_t n = (int64_t)d->nx * d->ny;
   // 16-byte accesses when every field pointer allows them
   bool aligned = true;
@@ -337,7 +334,7 @@ int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c
     t.lo[f] = d->lo[f];
     t.hi[f] = d->hi[f];
   }
-  t.partial = static_cast<double*>(c->workspace);
+  t.partial = partial;
   t.nx = d->nx;
   t.ny = d->ny;
   t.log_params = d->log_params ? 1 : 0;
@@ -352,7 +349,7 @@ int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c
   TV_HIP(hipGetLastError());
   if (c->tv) {
     hipLaunchKernelGGL(tv_finalize_kernel, dim3((unsigned)d->n_fields), dim3(kTvThreads), 0, st,
-                       static_cast<const double*>(c->workspace), (int)nb, c->tv);
+                       static_cast<const double*>(partial), (int)nb, c->tv);
     TV_HIP(hipGetLastError());
   }
   // Adam's scalars as torch forms them: Python floats (double), cast to FP32 where a kernel uses them
@@ -382,5 +379,60 @@ int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c
   TV_HIP(hipGetLastError());
   return ERNN_OK;
 }
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_current_device(double** out) {
+  int dev = -1;
+  TV_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_last_error(ERNN_E_INVALID, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    TV_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return set_last_error(ERNN_E_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return ERNN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ernn_tv_adam_workspace_bytes(const ernn_tv_adam_desc_t* desc, int64_t* bytes) {
+  if (!bytes) return set_last_error(ERNN_E_INVALID, "null argument");
+  const int r = check_desc(desc);
+  if (r != ERNN_OK) return r;
+  *bytes = tv_blocks(desc->nx, desc->ny) * desc->n_fields * (int64_t)sizeof(double);
+  return ERNN_OK;
+}
+
+int ernn_tv_adam_step(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c, void* stream) {
+  int r = check_desc(d);
+  if (r != ERNN_OK) return r;
+  if ((r = check_fields(d, c)) != ERNN_OK) return r;
+  if (!c->workspace) return set_last_error(ERNN_E_INVALID, "null workspace");
+  if ((reinterpret_cast<uintptr_t>(c->workspace) & 7) != 0)
+    return set_last_error(ERNN_E_INVALID, "workspace must be 8-byte aligned");
+  return enqueue_tv_adam(d, c, static_cast<double*>(c->workspace), static_cast<hipStream_t>(stream));
+}
+
+int ernn_tv_adam_step_static(const ernn_tv_adam_desc_t* d, const ernn_tv_adam_call_t* c, void* stream) {
+  int r = check_desc(d);
+  if (r != ERNN_OK) return r;
+  if ((r = check_fields(d, c)) != ERNN_OK) return r;
+  if (tv_blocks(d->nx, d->ny) * d->n_fields > kTvScratchDoubles)
+    return set_last_error(ERNN_E_CAPACITY, "ernn_tv_adam_step_static: n_fields x ceil(nx * ny / " + std::to_string(kTvThreads) +
+                                               ") must be <= " + std::to_string(kTvScratchDoubles) +
+                                               " (its static scratch); use ernn_tv_adam_step");
+  double* partial = nullptr;
+  if ((r = scratch_of_current_device(&partial)) != ERNN_OK) return r;
+  return enqueue_tv_adam(d, c, partial, static_cast<hipStream_t>(stream));
+}
+
 
 }  // extern "C"

@@ -109,6 +109,7 @@ class TvAdamCall(C.Structure):
 EXPORTS = {
     "ernn_tv_adam_workspace_bytes": ([C.POINTER(TvAdamDesc), C.POINTER(C.c_int64)], C.c_int),
     "ernn_tv_adam_step": ([C.POINTER(TvAdamDesc), C.POINTER(TvAdamCall), C.c_void_p], C.c_int),
+    "ernn_tv_adam_step_static": ([C.POINTER(TvAdamDesc), C.POINTER(TvAdamCall), C.c_void_p], C.c_int),
     "ernn_fem_plan_create": ([C.POINTER(FemDesc), C.POINTER(C.c_void_p)], C.c_int),
     "ernn_fem_plan_destroy": ([C.c_void_p], C.c_int),
     "ernn_fem_plan_info": ([C.c_void_p, C.POINTER(FemInfo)], C.c_int),

@@ -360,6 +360,16 @@ int ernn_tv_adam_workspace_bytes(const ernn_tv_adam_desc_t* desc, int64_t* bytes
  * m, v (torch's foreach arithmetic order).  Enqueued on `stream`. */
 int ernn_tv_adam_step(const ernn_tv_adam_desc_t* desc, const ernn_tv_adam_call_t* call, void* stream);
 
+/* ernn_tv_adam_step without a workspace: call->workspace is ignored and the
+ * FP64 partial sums go to a static device array of 65536 doubles (512 KiB)
+ * that each GPU holds until the process ends, so the call allocates nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Limit: n_fields x ceil(nx * ny / 256) <= 65536 (about 2.8 M
+ * cells per field for six fields), else ERNN_E_CAPACITY.  Calls on one GPU
+ * share the array: calls that overlap (other streams or threads) race. */
+int ernn_tv_adam_step_static(const ernn_tv_adam_desc_t* desc, const ernn_tv_adam_call_t* call, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -155,3 +155,14 @@ def test_tv_adam_workspace_size_and_bounds_check():
     assert "bounds of field 2" in _lib.lib().ernn_last_error().decode()
     call = _lib.TvAdamCall()
     assert _lib.lib().ernn_tv_adam_step(C.byref(_tv_desc()), C.byref(call), None) == -1   # null workspace
+
+
+def test_tv_adam_static_entry_point_rejects_fields_beyond_its_scratch():
+    d = _tv_desc(nx=2048, ny=2048)                       # 6 x 16384 partials > 65536
+    call = _lib.TvAdamCall()
+    for f in range(6):                                     # never dereferenced: the size check comes first
+        call.theta[f] = call.grad[f] = call.m[f] = call.v[f] = 256
+    assert _lib.lib().ernn_tv_adam_step_static(C.byref(d), C.byref(call), None) == -3
+    assert "static scratch" in _lib.lib().ernn_last_error().decode()
+    call2 = _lib.TvAdamCall()
+    assert _lib.lib().ernn_tv_adam_step_static(C.byref(_tv_desc()), C.byref(call2), None) == -1   # null fields
