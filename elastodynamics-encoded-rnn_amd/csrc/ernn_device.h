@@ -20,6 +20,32 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) f32x2 gfloat2;
 typedef __attribute__((address_space(1))) f32x4 gfloat4;
+
+// Reads of tables that no kernel writes while it runs (wavelets, source and
+// receiver events) through the constant address space: uniform ones become
+// scalar loads, counted by lgkmcnt.  As vector loads after a store they would
+// wait on vmcnt, which on CDNA also counts the stores still in flight.
+__device__ __forceinline__ float ld_const(const gfloat* p) {
+  return *(const float __attribute__((address_space(4)))*)p;
+}
+// Four dwords of a 16-byte record, loaded one by one so that they merge into
+// one s_load_dwordx4 (an aggregate copy can narrow to three dwords, which
+// only a vector load serves).
+__device__ __forceinline__ void ld_const4(const void* p, int (&q)[4]) {
+  const int __attribute__((address_space(4)))* c = (const int __attribute__((address_space(4)))*)p;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = c[k];
+}
+__device__ __forceinline__ SrcEv ld_src(const SrcEv* p) {
+  int q[4];
+  ld_const4(p, q);
+  return SrcEv{q[0], q[1], __int_as_float(q[2]), q[3]};
+}
+__device__ __forceinline__ RecEv ld_rec(const RecEv* p) {
+  int q[4];
+  ld_const4(p, q);
+  return RecEv{q[0], q[1], q[2], q[3]};
+}
 typedef __attribute__((address_space(1))) double gdouble;
 typedef __attribute__((address_space(1))) char gbyte;
 

@@ -302,7 +302,7 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
   }
   const double gsa = I.gscale * (double)I.hinv2;   // (dt^2 / rho) h^-2
   // step nt runs only for L^nt (damage mode): no force, nothing recorded
-  const float s_n = n < I.nt ? C.wav[I.wav_off + n] : 0.f;
+  const float s_n = n < I.nt ? ld_const(C.wav + I.wav_off + n) : 0.f;
   const bool rec_step = a.record && n < I.nt && (n % a.stride == 0);
   const int k = n / a.stride;
   const int nf = I.n_frames;
@@ -384,7 +384,7 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
 
     // point forces on this row: (dt^2/rho) s_n w (PI:103-104, PI:270-277)
     while (sp < tl.src_end) {
-      const SrcEv ev = a.src[sp];
+      const SrcEv ev = ld_src(a.src + sp);
       if (ev.i != i) break;
       const int lc = ev.j - j0;
       if (lane == lc / V) {
@@ -433,7 +433,7 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
         }
       } else {
         while (rp < tl.rec_end) {
-          const RecEv ev = a.rec[rp];
+          const RecEv ev = ld_rec(a.rec + rp);
           if (ev.i != i) break;
           const int lc = ev.j - j0;
           if (lane == lc / V) {
@@ -730,7 +730,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
         }
       } else {
         while (rp < tl.rec_end) {
-          const RecEv ev = a.rec[rp];
+          const RecEv ev = ld_rec(a.rec + rp);
           if (ev.i != i) break;
           const int lc = ev.j - j0;
           if (lane == lc / V) {
@@ -1095,9 +1095,9 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
 
     // point forces anywhere in the region (halo cells feed the later steps);
     // step nt (L^nt alone) has none
-    const float s_n = n < I.nt ? C.wav[I.wav_off + n] : 0.f;
+    const float s_n = n < I.nt ? ld_const(C.wav + I.wav_off + n) : 0.f;
     for (int sp = bl.src_begin; sp < bl.src_end; ++sp) {
-      const SrcEv ev = a.src[sp];
+      const SrcEv ev = ld_src(a.src + sp);
       const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
       if (rr < 0 || rr >= RW || lane != lc / V) continue;
       const float f = s_n * ev.w;
@@ -1204,7 +1204,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
         }
       } else {
         for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
-          const RecEv ev = a.rec[rp];
+          const RecEv ev = ld_rec(a.rec + rp);
           const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
           if (rr < 0 || rr >= RW || lane != lc / V) continue;
           gfloat* tx = C.traces + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
@@ -1453,7 +1453,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           }
         } else {
           for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
-            const RecEv ev = a.rec[rp];
+            const RecEv ev = ld_rec(a.rec + rp);
             const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
             if (rr < 0 || rr >= RW || lane != lc / V) continue;
             const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
