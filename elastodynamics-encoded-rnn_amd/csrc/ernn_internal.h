@@ -33,13 +33,10 @@ enum ParamMode : int { kConst6 = 0, kDField = 1, kField6 = 2 };
 //   v' = v + d,  y = v' + c,  u' = u + y,  c' = y - (u' - u),
 // so the FP32 rounding of the state does not accumulate over thousands of
 // steps.  A state slot holds the six padded fields u_x, u_y, c_x, c_y, v_x,
-// v_y of one time level; a bank slot holds what the adjoint's gradient terms
-// read of a recomputed level m: u^m (u_x, u_y), and in the damage mode also
-// L^m = (dt^2/rho) A(C0) u^m (L_x, L_y), the forward's operator before the
-// scaling by D, so that dJ/dD = sum_m lam^{m+1} . L^m needs no stencil of u.
+// v_y of one time level; a bank slot holds u_x, u_y only (the recomputed
+// forward states the adjoint reads for its gradient terms).
 constexpr int kStateFields = 6;
-constexpr int kBankFields = 2;      // six constants, six fields
-constexpr int kBankFieldsLu = 4;    // damage field
+constexpr int kBankFields = 2;
 
 // fp64 partial sums per tile for the constant gradients (kConst6, kDField).
 // Slots 0-5 hold S11, S22, S12, S16, S26, S66 with the stencil weight h^-2 (or
@@ -54,7 +51,7 @@ struct InstDev {
   int set;          // parameter set index
   int n_rec;        // receivers (receiver mode)
   int n_frames;     // recorded frames, ceil(nt / P)
-  int bank_fields;  // fields per bank slot (kBankFields or kBankFieldsLu)
+  int pad0;
   float hinv2;      // fl32(h^-2)
   float dt2;        // fl32(dt^2)
   float rho;        // fl32(rho)
@@ -104,9 +101,7 @@ struct StepArgs {
   int mode;
   int step;                // forward: n (level n -> n+1); adjoint: m (lam level m+1 -> m)
   int s_in, s_out;         // state slots of the input and output levels
-  int bank;                // forward: bank slot receiving u^{n+1} (or -1); adjoint: bank slot holding level m
-  int bank_lu;             // forward, damage mode: bank slot receiving L^n (or -1); step n = nt runs
-                           // for it alone (no state kept, nothing recorded)
+  int bank;                // forward: bank slot receiving u^{n+1} (or -1); adjoint: bank slot holding u^m
   int stride;              // recording stride P
   int record;              // forward: write traces on recorded steps
   int snapshot;            // traces are whole fields
@@ -159,8 +154,6 @@ struct BlockArgs {
   int s_in;                      // state slot of level n0
   int st_store[kMaxBlockSteps];  // state slot receiving level n0+1+s, or -1
   int bank_store[kMaxBlockSteps];// bank slot receiving u^{n0+1+s}, or -1
-  int lu_store[kMaxBlockSteps];  // damage mode: bank slot receiving L^{n0+s}, or -1
-  int extra;                     // 1: the launch may run step nt (for L^nt alone)
   int stride;                    // recording stride P
   int record;
   int snapshot;

@@ -263,7 +263,7 @@ __device__ __forceinline__ gfloat* state_field(const GCall& C, const InstDev& I,
   return C.ws + I.pool_off + ((int64_t)q * kStateFields + k) * I.field;
 }
 __device__ __forceinline__ gfloat* bank_field(const GCall& C, const InstDev& I, int b, int k) {
-  return C.ws + I.bank_off + ((int64_t)b * I.bank_fields + k) * I.field;
+  return C.ws + I.bank_off + ((int64_t)b * kBankFields + k) * I.field;
 }
 
 // --------------------------------------------------------------------------
@@ -301,9 +301,8 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
     kf = op_coef(c0, I.gscale * (double)I.hinv2);
   }
   const double gsa = I.gscale * (double)I.hinv2;   // (dt^2 / rho) h^-2
-  // step nt runs only for L^nt (damage mode): no force, nothing recorded
-  const float s_n = n < I.nt ? ld_const(C.wav + I.wav_off + n) : 0.f;
-  const bool rec_step = a.record && n < I.nt && (n % a.stride == 0);
+  const float s_n = ld_const(C.wav + I.wav_off + n);
+  const bool rec_step = a.record && (n % a.stride == 0);
   const int k = n / a.stride;
   const int nf = I.n_frames;
 
@@ -342,9 +341,6 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
   gfloat* ovy = state_field(C, I, a.s_out, 5);
   gfloat* bx = a.bank >= 0 ? bank_field(C, I, a.bank, 0) : nullptr;
   gfloat* by = a.bank >= 0 ? bank_field(C, I, a.bank, 1) : nullptr;
-  const bool lu = (MODE == kDField) && a.bank_lu >= 0;
-  gfloat* lux = lu ? bank_field(C, I, a.bank_lu, 2) : nullptr;
-  gfloat* luy = lu ? bank_field(C, I, a.bank_lu, 3) : nullptr;
 
   int sp = tl.src_begin;
   int rp = tl.rec_begin;
@@ -359,27 +355,7 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
       float cq[NC > 0 ? NC : 1];
 #pragma unroll
       for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
-      if constexpr (MODE == kDField) apply_op_sum(kf, d, dx[e], dy[e]);   // L^n; D applied below
-      else fwd_op<MODE>(kf, gsa, cq, d, dx[e], dy[e]);
-    }
-    if constexpr (MODE == kDField) {
-      if (lu) {   // L^n of the damage gradient (zero on the ghost rows)
-        const uint32_t so = row_off(P, i, j0) + lo;
-        float zx[V], zy[V];
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          const bool in = live && j0 + lane * V + e < I.ny;
-          zx[e] = in ? dx[e] : 0.f;
-          zy[e] = in ? dy[e] : 0.f;
-        }
-        store_vec<V>(at(lux, so), zx);
-        store_vec<V>(at(luy, so), zy);
-      }
-#pragma unroll
-      for (int e = 0; e < V; ++e) {
-        dx[e] = cf[r][0][e] * dx[e];
-        dy[e] = cf[r][0][e] * dy[e];
-      }
+      fwd_op<MODE>(kf, gsa, cq, d, dx[e], dy[e]);
     }
 
     // point forces on this row: (dt^2/rho) s_n w (PI:103-104, PI:270-277)
@@ -463,8 +439,7 @@ __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
   if constexpr (!LOOP) {
     const TileDev tl = a.tiles[tr.t];
     const InstDev I = a.inst[tl.inst];
-    // this instance has finished its time loop (step nt runs for L^nt alone)
-    if (a.step > I.nt || (a.step == I.nt && a.bank_lu < 0)) return;
+    if (a.step >= I.nt) return;   // this instance has finished its time loop
     fwd_tile<MODE, V, R>(a, C, tl, I, lane);
   } else {
     int cur = -1;
@@ -475,7 +450,7 @@ __global__ void __launch_bounds__(kBlock) fwd_step_kernel(StepArgs a) {
         I = a.inst[tl.inst];
         cur = tl.inst;
       }
-      if (a.step > I.nt || (a.step == I.nt && a.bank_lu < 0)) continue;
+      if (a.step >= I.nt) continue;
       fwd_tile<MODE, V, R>(a, C, tl, I, lane);
     }
   }
@@ -583,11 +558,12 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   const gfloat* coef = C.ws + I.coef_off;
 
   float c0[6];
-  OpCoef kl;   // (dt^2/rho) A(C0) on the (weighted) adjoint field
+  OpCoef kl, ku;   // kl: kdt A(C0) on the adjoint field; ku: A(C0) / a on u (dJ/dD)
   if constexpr (MODE != kField6) {
 #pragma unroll
     for (int p = 0; p < 6; ++p) c0[p] = C.params[tl.set * 6 + p];
     kl = op_coef(c0, I.gscale * (double)I.hinv2);
+    ku = op_coef(c0, 1.0);
   }
   const float ka = (float)(I.gscale * (double)I.hinv2), kq = 0.25f * ka;   // kField6: (dt^2/rho) a, a / 4
 
@@ -614,22 +590,15 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
     load_vec<V>(at(wxi, o), wx[r]);
     load_vec<V>(at(wyi, o), wy[r]);
   }
-  // u^m at the cells; the damage mode also L^m = (dt^2/rho) A(C0) u^m (bank
-  // fields 2, 3); the six-field mode u^m rows with halo for its stencils
-  constexpr int RU = (MODE == kField6) ? R + 2 : 1;
-  constexpr int RL = (MODE == kDField) ? R : 1;
-  float uvx[R][V], uvy[R][V], lvx[RL][V], lvy[RL][V];
+  constexpr int RU = (MODE == kConst6) ? 1 : R + 2;   // u^m rows with halo (field modes)
+  float uvx[R][V], uvy[R][V];
   RawRow<V> uraw[RU][2];
-  if constexpr (MODE != kField6) {
+  if constexpr (MODE == kConst6) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint32_t o = row_off(P, tl.i0 + r, j0) + lo;
       load_vec<V>(at(umx, o), uvx[r]);
       load_vec<V>(at(umy, o), uvy[r]);
-      if constexpr (MODE == kDField) {
-        load_vec<V>(at(bank_field(C, I, a.bank, 2), o), lvx[r]);
-        load_vec<V>(at(bank_field(C, I, a.bank, 3), o), lvy[r]);
-      }
     }
   } else {
 #pragma unroll
@@ -660,7 +629,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
   for (int r = 0; r < R + 2; ++r) lr[r] = make_adj_row<MODE, V>(lraw[r], lane);
   Row<V> um[RU][2];
-  if constexpr (MODE == kField6) {
+  if constexpr (MODE != kConst6) {
 #pragma unroll
     for (int r = 0; r < RU; ++r) {
       um[r][0] = make_row<V>(uraw[r][0], lane);
@@ -698,12 +667,18 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
         // A^T lam = A(C0) (D * lam): constant stencils on the (weighted) adjoint field
         const UDeriv dl = uderivs<V>(Lm.w[0], Lc.w[0], Lp.w[0], Lm.w[1], Lc.w[1], Lp.w[1], x);
         apply_op_sum(kl, dl, dx[e], dy[e]);
-        float ux = uvx[r][e], uy = uvy[r][e];
+        float ux, uy;
+        if constexpr (MODE == kConst6) { ux = uvx[r][e]; uy = uvy[r][e]; }
+        else { ux = um[r + 1][0].v[x]; uy = um[r + 1][1].v[x]; }
         if (!live) { ux = 0.f; uy = 0.f; }
         sbp_sums(dl, ux, uy, S);
         if constexpr (MODE == kDField) {
-          // dJ/dD(cell) = (dt^2/rho) lam . A(C0) u^m at the cell = lam . L^m
-          G[0][e] = __builtin_fmaf(Lc.cx[e], lvx[r][e], Lc.cy[e] * lvy[r][e]);
+          // dJ/dD(cell) = lam . A(C0) u^m at the cell (divided by a)
+          const UDeriv du = uderivs<V>(um[r][0], um[r + 1][0], um[r + 2][0], um[r][1], um[r + 1][1],
+                                       um[r + 2][1], x);
+          float ox, oy;
+          apply_op_sum(ku, du, ox, oy);
+          G[0][e] = __builtin_fmaf(Lc.cx[e], ox, Lc.cy[e] * oy);
         }
       } else {
         const float sx = uxx<V>(Lm.w[0], Lc.w[0], Lp.w[0], x) + uyy<V>(Lc.w[1], x);
@@ -771,8 +746,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
       for (int q = 0; q < NACC; ++q)
 #pragma unroll
         for (int e = 0; e < V; ++e)
-          if (live && j0 + lane * V + e < I.ny)   // L^m carries (dt^2/rho) h^-2 already; the fields' sums 1 / a
-            acc64[q * I.field + so + e] = acc_in[r][q][e] + (MODE == kDField ? 1.0 : gsa) * (double)G[q][e];
+          if (live && j0 + lane * V + e < I.ny) acc64[q * I.field + so + e] = acc_in[r][q][e] + gsa * (double)G[q][e];
     }
   }
 
@@ -839,7 +813,7 @@ __global__ void __launch_bounds__(kBlock) fused_step_kernel(FusedArgs fa) {
       cur = tl.inst;
     }
     if (u & 1) {
-      if (fa.fwd.step < I.nt || (fa.fwd.step == I.nt && fa.fwd.bank_lu >= 0)) fwd_tile<MODE, V, R>(fa.fwd, C, tl, I, lane);
+      if (fa.fwd.step < I.nt) fwd_tile<MODE, V, R>(fa.fwd, C, tl, I, lane);
     } else {
       if (fa.adj.step <= I.nt) adj_tile<MODE, V, R>(fa.adj, C, tl, I, t, lane);
     }
@@ -973,8 +947,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
         "s"(bl.oi0), "s"(a.n0), "s"(a.ks), "s"(a.s_in));
   if (t >= n_blocks) return;
   const InstDev I = bl.I;
-  // steps n0 .. nt - 1 of this instance, and step nt for L^nt alone (extra)
-  const int ks = min(a.ks, I.nt - a.n0 + a.extra);
+  const int ks = min(a.ks, I.nt - a.n0);
   if (ks <= 0) return;   // this instance has finished its time loop
 
   const Band<V, RW, KS, WB> bd(bl, I, w, lane);
@@ -1064,38 +1037,11 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
         float cq[NC > 0 ? NC : 1];
 #pragma unroll
         for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
-        if constexpr (MODE == kDField) apply_op_sum(kf, d, dx[r][e], dy[r][e]);   // L^n; D applied below
-        else fwd_op<MODE>(kf, gsa, cq, d, dx[r][e], dy[r][e]);
+        fwd_op<MODE>(kf, gsa, cq, d, dx[r][e], dy[r][e]);
       }
-    if constexpr (MODE == kDField) {
-      const int lu_slot = a.lu_store[s];
-      if (lu_slot >= 0 && bd.col_own) {   // L^n of the damage gradient on the output tile
-        const Rsrc lx = field_rsrc(bank_field(C, I, lu_slot, 2), I.field);
-        const Rsrc ly = field_rsrc(bank_field(C, I, lu_slot, 3), I.field);
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-          if (r < bd.own_lo || r >= bd.own_hi) continue;
-          const uint32_t so = bd.srow(I, r);
-#pragma unroll
-          for (int e = 0; e < V; ++e) {
-            if (!bd.col_in[e]) continue;
-            bstore(dx[r][e], lx, bd.vcol[e], so);
-            bstore(dy[r][e], ly, bd.vcol[e], so);
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          dx[r][e] = cf[r][0][e] * dx[r][e];
-          dy[r][e] = cf[r][0][e] * dy[r][e];
-        }
-    }
 
-    // point forces anywhere in the region (halo cells feed the later steps);
-    // step nt (L^nt alone) has none
-    const float s_n = n < I.nt ? ld_const(C.wav + I.wav_off + n) : 0.f;
+    // point forces anywhere in the region (halo cells feed the later steps)
+    const float s_n = ld_const(C.wav + I.wav_off + n);
     for (int sp = bl.src_begin; sp < bl.src_end; ++sp) {
       const SrcEv ev = ld_src(a.src + sp);
       const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
@@ -1171,7 +1117,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
       }
     }
     // record frame k = n / P from u^{n+1} (PI:294-301)
-    if (a.record && n < I.nt && (n % a.stride == 0)) {
+    if (a.record && (n % a.stride == 0)) {
       const int k = n / a.stride;
       const int nf = I.n_frames;
       if (a.snapshot) {
@@ -1270,6 +1216,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 #pragma unroll
   for (int p = 0; p < 6; ++p) c0[p] = C.params[bl.set * 6 + p];
   const OpCoef kl = op_coef(c0, gsa);    // (dt^2/rho) A(C0) on the (weighted) adjoint field
+  const OpCoef ku = op_coef(c0, 1.0);    // A(C0) / a on u (dJ/dD)
   (void)kdt;
 
   // ---- loads: lam level m0 + 1 (lam, c, w) and D
@@ -1318,30 +1265,24 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
         xch[buf][w][1][q][lane * V + e] = dd[RW - 1][e] * l[q][RW - 1][e];
       }
   };
-  // u^m (and in the damage mode L^m = (dt^2/rho) A(C0) u^m) of the lane's
-  // cells on the band rows in the output tile.  Step s + 1's loads are issued
-  // as soon as step s has used its values, before the band-edge exchange, so
-  // they arrive while the next update computes.
-  constexpr int RL = kD ? RW : 1;
-  float urx[RW][V], ury[RW][V], ulx[RL][V], uly[RL][V];
+  // u^m of the lane's cells on the band rows in the output tile (the damage
+  // mode with one row of halo on each side for its stencil).  Step s + 1's
+  // loads are issued as soon as step s has used its u, before the band-edge
+  // exchange, so they arrive while the next update computes.
+  constexpr int RU = kD ? RW + 2 : RW;
+  constexpr int UO = kD ? 1 : 0;
+  float urx[RU][V], ury[RU][V];
   const bool grad_rows = bd.own_lo < bd.own_hi;   // wave-uniform
   auto load_u = [&](int s) {
     const Rsrc bx = field_rsrc(bank_field(C, I, a.bank_u[s], 0), I.field);
     const Rsrc by = field_rsrc(bank_field(C, I, a.bank_u[s], 1), I.field);
-    const Rsrc lx = field_rsrc(bank_field(C, I, a.bank_u[s], kD ? 2 : 0), I.field);
-    const Rsrc ly = field_rsrc(bank_field(C, I, a.bank_u[s], kD ? 3 : 1), I.field);
 #pragma unroll
-    for (int r = 0; r < RW; ++r) {
-      if (r < bd.own_lo || r >= bd.own_hi) continue;
-      const uint32_t so = bd.srow(I, r);
+    for (int r = 0; r < RU; ++r) {
+      const uint32_t so = bd.srow(I, r - UO);
 #pragma unroll
       for (int e = 0; e < V; ++e) {
         urx[r][e] = bload(bx, bd.vcol[e], so);
         ury[r][e] = bload(by, bd.vcol[e], so);
-        if constexpr (kD) {
-          ulx[r][e] = bload(lx, bd.vcol[e], so);
-          uly[r][e] = bload(ly, bd.vcol[e], so);
-        }
       }
     }
   };
@@ -1382,6 +1323,16 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           yr[r + 1] = make_row_z<V>(ry);
         }
       }
+      Row<V> uxr[kD ? RW + 2 : 1], uyr[kD ? RW + 2 : 1];
+      if constexpr (kD) {
+        if (grad_rows) {
+#pragma unroll
+          for (int r = 0; r < RW + 2; ++r) {
+            uxr[r] = make_row_z<V>(urx[r]);
+            uyr[r] = make_row_z<V>(ury[r]);
+          }
+        }
+      }
 
       float dx[RW][V], dy[RW][V];
 #pragma unroll
@@ -1393,9 +1344,13 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           apply_op_sum(kl, dl, dx[r][e], dy[r][e]);
           // gradient terms on the band rows of the output tile (lam^{m+1} with u^m)
           if (r >= bd.own_lo && r < bd.own_hi) {
-            sbp_sums(dl, urx[r][e], ury[r][e], S);
-            // dJ/dD(cell) += (dt^2/rho) lam^{m+1} . A(C0) u^m = lam^{m+1} . L^m
-            if constexpr (kD) gacc[r][e] += __builtin_fmaf(l[0][r][e], ulx[r][e], l[1][r][e] * uly[r][e]);
+            sbp_sums(dl, kD ? uxr[r + 1].v[x] : urx[r][e], kD ? uyr[r + 1].v[x] : ury[r][e], S);
+            if constexpr (kD) {
+              const UDeriv du = uderivs<V>(uxr[r], uxr[r + 1], uxr[r + 2], uyr[r], uyr[r + 1], uyr[r + 2], x);
+              float ox, oy;
+              apply_op_sum(ku, du, ox, oy);
+              gacc[r][e] += __builtin_fmaf(l[0][r][e], ox, l[1][r][e] * oy);
+            }
           }
         }
       // The gradient sums are complete here.  The empty asm statements pin
@@ -1537,7 +1492,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
         for (int e = 0; e < V; ++e)
           if (bd.col_in[e]) {
             gdouble* pa = acc64 + so + bd.vcol[e] / (uint32_t)sizeof(float);
-            *pa = *pa + (double)gacc[r][e];
+            *pa = *pa + gsa * (double)gacc[r][e];
           }
       }
     }

@@ -171,9 +171,6 @@ struct Plan {
   bool use_graph = false;
   bool fuse = false;
   int ff_rw = 0;   // rows per wavefront of the fused recompute half (0: no fused time-blocked backward)
-  // damage mode: bank slots also hold L^m, and every recompute (and the
-  // forward pass's last segment) runs one step more, for L of its last level
-  int lu = 0;
   std::vector<ernn_instance_t> inst_h;
   std::vector<ernn_source_t> src_h;
   std::vector<ernn_receiver_t> rec_h;
@@ -215,7 +212,6 @@ struct Plan {
   int lam(int r) const { return 2 + S + r; }
   int n_state() const { return 4 + S; }
   int n_bank() const { return banks() * K; }
-  int bank_fields() const { return lu ? kBankFieldsLu : kBankFields; }
   int bank_of(int s) const { return fuse ? (s & 1) : 0; }
   int bank(int s, int m) const { return bank_of(s) * K + (m - s * K - 1); }   // u^m, sK < m <= top
   bool is_ck(int m) const { return m % K == 0 && m / K < S; }
@@ -628,7 +624,6 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   p->use_graph = d->use_graph != 0;
   p->fuse = d->fuse != 0;
   p->n_coef = p->mode == ERNN_PARAM_DFIELD ? 1 : (p->mode == ERNN_PARAM_FIELD6 ? 6 : 0);
-  p->lu = p->mode == ERNN_PARAM_DFIELD ? 1 : 0;
   p->n_acc = p->n_coef;
   p->inst_h.assign(d->inst, d->inst + d->n_inst);
   if (d->n_src) p->src_h.assign(d->src, d->src + d->n_src);
@@ -693,8 +688,7 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
     D.pool_off = off;
     off += round_up((int64_t)p->n_state() * kStateFields * D.field, 64);
     D.bank_off = off;
-    D.bank_fields = p->bank_fields();
-    off += round_up((int64_t)p->n_bank() * p->bank_fields() * D.field, 64);
+    off += round_up((int64_t)p->n_bank() * kBankFields * D.field, 64);
     D.trace_off = p->trace_off[b];
     D.wav_off = 0;
   }
@@ -818,48 +812,39 @@ BlockArgs block_args(const Plan* p, const BlockTiling& bt) {
 }
 BlockArgs block_args(const Plan* p) { return block_args(p, p->blk); }
 
-// Time-blocked forward-pass launch from level n0 (state slot *in) up to step
+// Time-blocked forward-pass launch from level n0 (state slot *in) up to level
 // `end` at most: writes the checkpoint levels inside it, its last level to the
 // ring slot it does not read (when that level is no checkpoint), and the
-// levels of the last checkpoint segment to its bank (with L^n in the damage
-// mode, whose step nt runs for L^nt alone).  *in becomes the slot of the last
-// level.
+// levels of the last checkpoint segment to its bank.  *in becomes the slot of
+// the last level.
 void blk_forward_args(const Plan* p, BlockArgs& a, int n0, int end, int* in) {
   const int B = (p->S - 1) * p->K;
   a.n0 = n0;
   a.ks = std::min(a.ksmax, end - n0);
   a.s_in = *in;
-  a.extra = p->lu;
-  for (int j = 0; j < kMaxBlockSteps; ++j) a.st_store[j] = a.bank_store[j] = a.lu_store[j] = -1;
+  for (int j = 0; j < kMaxBlockSteps; ++j) a.st_store[j] = a.bank_store[j] = -1;
   for (int j = 0; j < a.ks; ++j) {
-    const int n = n0 + j, m = n + 1;
-    if (m <= p->nt_max) {
-      if (p->is_ck(m)) a.st_store[j] = p->ck(m / p->K);
-      else if (j == a.ks - 1) a.st_store[j] = p->other_ring(*in);
-      if (m > B) a.bank_store[j] = p->bank(p->S - 1, m);
-    }
-    if (p->lu && n > B && n <= p->nt_max) a.lu_store[j] = p->bank(p->S - 1, n);
+    const int m = n0 + 1 + j;
+    if (p->is_ck(m)) a.st_store[j] = p->ck(m / p->K);
+    else if (j == a.ks - 1) a.st_store[j] = p->other_ring(*in);
+    if (m > B) a.bank_store[j] = p->bank(p->S - 1, m);
   }
-  if (a.st_store[a.ks - 1] >= 0) *in = a.st_store[a.ks - 1];
+  *in = a.st_store[a.ks - 1];
 }
 
 // Time-blocked recompute launch of segment s from level n0 (state slot *in):
-// every level into the segment's bank (damage mode: also L^n of levels sK + 1
-// .. top, which takes one step past the segment), the last level also to the
-// ring slot the launch does not read unless it ends the recompute.
+// every level into the segment's bank, the last one also to the ring slot the
+// launch does not read unless it ends the segment.
 void blk_recompute_args(const Plan* p, BlockArgs& a, int s, int n0, int* in) {
-  const int top = p->seg_top(s), end = top + p->lu;
+  const int top = p->seg_top(s);
   a.n0 = n0;
-  a.ks = std::min(a.ksmax, end - n0);
+  a.ks = std::min(a.ksmax, top - n0);
   a.s_in = *in;
-  a.extra = p->lu;
   for (int j = 0; j < kMaxBlockSteps; ++j) {
-    const int n = n0 + j;
     a.st_store[j] = -1;
-    a.bank_store[j] = (j < a.ks && n + 1 <= top) ? p->bank(s, n + 1) : -1;
-    a.lu_store[j] = (j < a.ks && p->lu && n > s * p->K) ? p->bank(s, n) : -1;
+    a.bank_store[j] = j < a.ks ? p->bank(s, n0 + 1 + j) : -1;
   }
-  if (n0 + a.ks < end) {
+  if (n0 + a.ks < top) {
     a.st_store[a.ks - 1] = p->other_ring(*in);
     *in = a.st_store[a.ks - 1];
   }
@@ -892,26 +877,21 @@ void blk_adjoint_args(const Plan* p, BlockAdjArgs& a, int s, int m0, int ks, int
   *lin = a.s_out;
 }
 
-// forward step n of the single-step schedule: level n (slot in) -> n + 1;
-// with bank_last_segment the levels of the last segment (and L^n) go to its
-// bank, and step nt runs for L^nt alone in the damage mode
+// forward step n of the single-step schedule: level n (slot in) -> n + 1
 void step_forward_args(const Plan* p, StepArgs& a, int n, int* in, bool bank_last_segment) {
-  const int m = n + 1, B = (p->S - 1) * p->K;
+  const int m = n + 1;
   a.step = n;
   a.s_in = *in;
-  a.s_out = p->is_ck(m) && m <= p->nt_max ? p->ck(m / p->K) : p->other_ring(*in);
-  a.bank = (bank_last_segment && m > B && m <= p->nt_max) ? p->bank(p->S - 1, m) : -1;
-  a.bank_lu = (bank_last_segment && p->lu && n > B && n <= p->nt_max) ? p->bank(p->S - 1, n) : -1;
+  a.s_out = p->is_ck(m) ? p->ck(m / p->K) : p->other_ring(*in);
+  a.bank = (bank_last_segment && m > (p->S - 1) * p->K) ? p->bank(p->S - 1, m) : -1;
   *in = a.s_out;
 }
-// recompute step n of segment s: level n -> n + 1 into the bank (and a ring
-// slot); damage mode: L^n too, for n = sK + 1 .. top
+// recompute step n of segment s: level n -> n + 1 into the bank (and a ring slot)
 void step_recompute_args(const Plan* p, StepArgs& a, int s, int n, int* in) {
   a.step = n;
   a.s_in = *in;
   a.s_out = p->other_ring(*in);
-  a.bank = n + 1 <= p->seg_top(s) ? p->bank(s, n + 1) : -1;
-  a.bank_lu = (p->lu && n > s * p->K) ? p->bank(s, n) : -1;
+  a.bank = p->bank(s, n + 1);
   *in = a.s_out;
 }
 // adjoint step m of segment s: lam level m + 1 -> m, u^m from the bank
@@ -930,19 +910,18 @@ void step_adjoint_args(const Plan* p, StepArgs& a, int s, int m, int* lin) {
 int enqueue_forward(const Plan* p, hipStream_t st) {
   ERNN_TRY(enqueue_pack(p, st));
   int in = p->ck(0);
-  const int end = p->nt_max + p->lu;   // damage mode: step nt for L^nt
   if (p->blk.KS > 1) {
     BlockArgs b = block_args(p);
     b.record = 1;
-    for (int n0 = 0; n0 < end; n0 += p->blk.KS) {
-      blk_forward_args(p, b, n0, end, &in);
+    for (int n0 = 0; n0 < p->nt_max; n0 += p->blk.KS) {
+      blk_forward_args(p, b, n0, p->nt_max, &in);
       ERNN_HIP(launch_fwd_block(b, p->blk.V, p->blk.grid, st));
     }
     return ERNN_OK;
   }
   StepArgs a = base_args(p, p->fwd);
   a.record = 1;
-  for (int n = 0; n < end; ++n) {
+  for (int n = 0; n < p->nt_max; ++n) {
     step_forward_args(p, a, n, &in, true);
     ERNN_HIP(launch_fwd_step(a, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
   }
@@ -972,15 +951,15 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
   // recompute segment s whole (time-blocked when enabled)
   auto recompute_segment = [&](int s) -> int {
     int in = p->ck(s);
-    const int end = p->seg_top(s) + p->lu;
+    const int top = p->seg_top(s);
     if (p->blk.KS > 1) {
-      for (int n0 = s * p->K; n0 < end; n0 += p->blk.KS) {
+      for (int n0 = s * p->K; n0 < top; n0 += p->blk.KS) {
         blk_recompute_args(p, rb, s, n0, &in);
         ERNN_HIP(launch_fwd_block(rb, p->blk.V, p->blk.grid, st));
       }
       return ERNN_OK;
     }
-    for (int n = s * p->K; n < end; ++n) {
+    for (int n = s * p->K; n < top; ++n) {
       step_recompute_args(p, f, s, n, &in);
       ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
     }
@@ -1007,7 +986,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
       for (int m0 = top; m0 >= bot; m0 -= KS) {
         const int ks = std::min(KS, m0 - bot + 1);
         blk_adjoint_args(p, fb.adj, s, m0, ks, &lin);
-        if (s >= 1 && n0 < s * p->K + p->lu) {
+        if (s >= 1 && n0 < s * p->K) {
           blk_recompute_args(p, fb.fwd, s - 1, n0, &rin);
           ERNN_HIP(launch_fused_block(fb, p->blk_adj.V, grid, st));
           n0 += KS;
@@ -1015,7 +994,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
           ERNN_HIP(launch_adj_block(fb.adj, p->blk_adj.V, p->blk_adj.grid, st));
         }
       }
-      for (; s >= 1 && n0 < s * p->K + p->lu; n0 += KS) {
+      for (; s >= 1 && n0 < s * p->K; n0 += KS) {
         BlockArgs r = block_args(p, p->blk_ff);
         r.record = 0;
         blk_recompute_args(p, r, s - 1, n0, &rin);
@@ -1056,7 +1035,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
       int idx = 0;
       for (int m = top; m >= bot; --m, ++idx) {
         step_adjoint_args(p, a, s, m, &lin);
-        if (s >= 1 && idx < p->K + p->lu) {
+        if (s >= 1 && idx < p->K) {
           fa.adj = a;
           step_recompute_args(p, fa.fwd, s - 1, (s - 1) * p->K + idx, &rin);
           ERNN_HIP(launch_fused_step(fa, p->adj.V, p->adj.fused_blocks, st));
@@ -1065,7 +1044,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
         }
       }
       // a short last segment leaves recompute steps of segment s - 1 to run alone
-      for (; s >= 1 && idx < p->K + p->lu; ++idx) {
+      for (; s >= 1 && idx < p->K; ++idx) {
         step_recompute_args(p, f, s - 1, (s - 1) * p->K + idx, &rin);
         ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
       }
@@ -1313,7 +1292,7 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
         ERNN_HIP(launch_adj_block(fb.adj, p->blk_adj.V, p->blk_adj.grid, st));
         continue;
       }
-      if (n0 >= top0 + p->lu) { n0 = 0; rin = p->ck(0); }
+      if (n0 >= top0) { n0 = 0; rin = p->ck(0); }
       blk_recompute_args(p, fb.fwd, 0, n0, &rin);
       n0 += KS;
       ERNN_HIP(launch_fused_block(fb, p->blk_adj.V, p->blk_adj.grid + p->blk_ff.grid, st));

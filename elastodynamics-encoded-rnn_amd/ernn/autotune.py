@@ -3,13 +3,12 @@
 Checkpoint interval K.  Each instance keeps 4 + S state slots of six padded
 fields (two forward ring slots, the S = ceil(nt / K) checkpoints, two adjoint
 slots; a state is u, its compensation c and its increment v per component)
-and K bank slots per bank (the recomputed levels of one segment, u per
-component, and in the damage mode also the operator L = (dt^2/rho) A(C0) u;
-two banks when the adjoint is fused with the next segment's recompute).
+and K bank slots of two fields per bank (the recomputed u of one segment; two
+banks when the adjoint is fused with the next segment's recompute).
 Recompute cost is one extra forward whatever K is, so K only trades memory:
-fields(K) = 6 (4 + ceil(nt/K)) + b banks K with b = 2 (4 with L) is minimal
-near K = sqrt(6 nt / (b banks)).  ``choose_ckpt`` returns that K, or raises if
-even the minimum exceeds the cap.
+fields(K) = 6 (4 + ceil(nt/K)) + 2 banks K is minimal near K = sqrt(3 nt /
+banks).  ``choose_ckpt`` returns that K, or raises if even the minimum
+exceeds the cap.
 
 Tiles.  ``tune_tiles`` times every candidate (V columns per lane, R rows per
 wavefront) of the forward and the adjoint kernel on the real problem for a
@@ -36,19 +35,18 @@ def padded_field_floats(nx: int, ny: int) -> int:
 
 
 STATE_FIELDS = 6   # u, c, v per component (the library's kStateFields)
-BANK_FIELDS = {"const6": 2, "dfield": 4, "field6": 2}   # u (and L) per component
+BANK_FIELDS = 2    # u per component
 
 
-def fields(nt: int, K: int, banks: int = 1, bank_fields: int = 2) -> int:
+def fields(nt: int, K: int, banks: int = 1) -> int:
     """Padded fields per instance for nt steps and checkpoint interval K."""
-    return STATE_FIELDS * (4 + math.ceil(nt / K)) + bank_fields * banks * K
+    return STATE_FIELDS * (4 + math.ceil(nt / K)) + BANK_FIELDS * banks * K
 
 
 def state_bytes(survey: Survey, K: int) -> int:
     nt = survey.nt_max
     b = 2 if survey.fused else 1
-    bf = BANK_FIELDS[survey.mode]
-    return sum(fields(nt, K, b, bf) * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
+    return sum(fields(nt, K, b) * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
 
 
 def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
@@ -57,8 +55,7 @@ def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
     if nt < 4:
         return 2
     b = 2 if survey.fused else 1
-    bf = BANK_FIELDS[survey.mode]
-    best = min(range(2, nt + 1), key=lambda K: (fields(nt, K, b, bf), -K))
+    best = min(range(2, nt + 1), key=lambda K: (fields(nt, K, b), -K))
     if mem_cap_bytes is not None and state_bytes(survey, best) > mem_cap_bytes:
         raise MemoryError(f"checkpointed state needs {state_bytes(survey, best) / 2**30:.2f} GiB "
                           f"> cap {mem_cap_bytes / 2**30:.2f} GiB")
