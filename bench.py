@@ -748,11 +748,15 @@ def main():
     # steps is credited KS steps of algorithmic bytes; the recompute half of a
     # fused launch is overhead and is reported apart.
     kern = {}
+    # recomputed steps: every segment but the last (the forward pass keeps the
+    # last segment's states); the fused schedule recomputes the last one alone
+    S = math.ceil(nt / K)
+    n_rec = K if sv.fused else (S - 1) * K
     if KS > 1:
-        n_blk = math.ceil(nt / KS) + (math.ceil(K / KS) if sv.fused else math.ceil(nt / KS))
+        n_blk = math.ceil(nt / KS) + (math.ceil(K / KS) if sv.fused else (S - 1) * math.ceil(K / KS))
         kern["fwd_block_kernel"] = (blk_ms, n_blk, bf * KS, 0)
     else:
-        kern["fwd_step_kernel"] = (fwd_ms, nt + K if sv.fused else 2 * nt, bf, 0)
+        kern["fwd_step_kernel"] = (fwd_ms, nt + n_rec, bf, 0)
     if sv.fused and AKS > 1:   # fused time-blocked backward: AKS adjoint steps + AKS recompute steps
         kern["fused_block_kernel"] = (ablk_ms, math.ceil(nt / AKS), ba * AKS, bf * AKS)
     elif sv.fused:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
@@ -800,6 +804,10 @@ def main():
         f"{ablk_ms / AKS * 1e3 if ablk_ms else float('nan'):.2f} us per step; "
         f"{name}: {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.2%} of HBM peak)")
 
+    # the pooled workspaces (all of a segment's states) go back before the
+    # baselines allocate their own
+    eng.trim()
+    torch.cuda.empty_cache()
     baseline = None
     if args.baseline and sv.n_sets == 1:
         theta0 = torch.tensor(C_init / model.scale.cpu().numpy(), dtype=torch.float32, device=dev)

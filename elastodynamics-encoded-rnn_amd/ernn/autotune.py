@@ -5,10 +5,13 @@ fields (two forward ring slots, the S = ceil(nt / K) checkpoints, two adjoint
 slots; a state is u, its compensation c and its increment v per component)
 and K bank slots of two fields per bank (the recomputed u of one segment; two
 banks when the adjoint is fused with the next segment's recompute).
-Recompute cost is one extra forward whatever K is, so K only trades memory:
 fields(K) = 6 (4 + ceil(nt/K)) + 2 banks K is minimal near K = sqrt(3 nt /
-banks).  ``choose_ckpt`` returns that K, or raises if even the minimum
-exceeds the cap.
+banks).  The forward pass leaves the last segment's states in the bank, so
+the backward recomputes nt - (last segment) steps: a larger K costs memory
+and saves recompute, and K = nt (one segment) recomputes nothing.
+``choose_ckpt`` returns, among the K whose workspace fits a memory budget,
+the one that recomputes the fewest steps, or raises if even the K of least
+memory exceeds the cap.
 
 Tiles.  ``tune_tiles`` times every candidate (V columns per lane, R rows per
 wavefront) of the forward and the adjoint kernel on the real problem for a
@@ -49,8 +52,11 @@ def state_bytes(survey: Survey, K: int) -> int:
     return sum(fields(nt, K, b) * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
 
 
-def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
-    """Checkpoint interval minimizing state memory; raises if it exceeds the cap."""
+def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None, budget_bytes: Optional[int] = None) -> int:
+    """Checkpoint interval.  Without a budget: the K of least state memory.
+    With one: among the K from that one up whose workspace fits it, the K
+    that recomputes the fewest steps (nt: none).  Raises if the K of least
+    memory exceeds the cap."""
     nt = survey.nt_max
     if nt < 4:
         return 2
@@ -59,7 +65,17 @@ def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None) -> int:
     if mem_cap_bytes is not None and state_bytes(survey, best) > mem_cap_bytes:
         raise MemoryError(f"checkpointed state needs {state_bytes(survey, best) / 2**30:.2f} GiB "
                           f"> cap {mem_cap_bytes / 2**30:.2f} GiB")
+    if budget_bytes:
+        fit = [K for K in range(best, nt + 1) if state_bytes(survey, K) <= budget_bytes]
+        if fit:   # fewest recomputed steps (every segment but the last), then least memory
+            best = min(fit, key=lambda K: (recomputed_steps(nt, K), state_bytes(survey, K)))
     return best
+
+
+def recomputed_steps(nt: int, K: int) -> int:
+    """Steps the backward recomputes: every checkpoint segment but the last,
+    whose states the forward pass leaves in the bank."""
+    return (math.ceil(nt / K) - 1) * K
 
 
 # every compiled tile shape: V columns per lane x R <= 8 rows, V * R <= 16
@@ -213,11 +229,12 @@ def tune_backward(survey, params, field, wavelets, steps: int = 48, rounds: int 
     from .engine import Engine
     if survey.mode == "field6":
         return survey.fused, (0, 0, 0), 0, {}
-    saved = survey.fuse, survey.adj_ff_rw
+    saved = survey.fuse, survey.adj_ff_rw, survey.ckpt_budget
     times = {}
     ff_rw = 0
     try:
-        survey.fuse, survey.adj_ff_rw = True, 0
+        # the temporary plans only time steps: checkpoint interval of least memory
+        survey.fuse, survey.adj_ff_rw, survey.ckpt_budget = True, 0, 0
         eng = Engine(survey, device=params.device)
         try:
             tune_tiles(eng, params, field, wavelets, steps=steps, which=(2,), rounds=rounds)
@@ -238,6 +255,6 @@ def tune_backward(survey, params, field, wavelets, steps: int = 48, rounds: int 
         finally:
             eng.close()
     finally:
-        survey.fuse, survey.adj_ff_rw = saved
+        survey.fuse, survey.adj_ff_rw, survey.ckpt_budget = saved
     fuse = times["fused"] <= times["blocked"]
     return fuse, (fz[:3] if fuse else adj[:3]), (ff_rw if fuse else 0), times

@@ -36,7 +36,11 @@ class Engine:
         K = ckpt_every or survey.ckpt_every
         if not K:
             from .autotune import choose_ckpt
-            K = choose_ckpt(survey)
+            budget = survey.ckpt_budget
+            if budget is None:
+                free, _ = torch.cuda.mem_get_info(self.device)
+                budget = int(0.8 * free)
+            K = choose_ckpt(survey, budget_bytes=budget)
         self._build(K)
         self._pool: List[torch.Tensor] = []
 
@@ -125,6 +129,11 @@ class Engine:
     def release_workspace(self, ws: torch.Tensor) -> None:
         if len(self._pool) < 2:
             self._pool.append(ws)
+
+    def trim(self) -> None:
+        """Drop the pooled workspaces (torch's allocator keeps the memory
+        cached until torch.cuda.empty_cache())."""
+        self._pool.clear()
 
     def _call(self, ws, params=None, field=None, wavelets=None, traces=None, grad_traces=None,
               grad_params=None, grad_field=None) -> _lib.Call:

@@ -170,6 +170,12 @@ class Survey:
     rec_stride: int = 1       # frame k records u^{kP+1} (PI:294-301)
     snapshot: bool = False    # record whole fields (the reference's output)
     ckpt_every: int = 0       # 0 = choose by memory (autotune.choose_ckpt)
+    # HBM bytes the checkpointed states may take when ckpt_every is 0: the
+    # interval is the one within them that recomputes the fewest steps (the
+    # forward pass keeps the last segment's states, so an interval of nt
+    # recomputes nothing).  None = 80 % of the device's free memory when the
+    # engine is built; 0 = the interval of least memory.
+    ckpt_budget: Optional[int] = None
     # forward V, R, adjoint V, R (0 = default), then grid caps in blocks per CU
     # for the forward and adjoint (0 = one tile per wavefront)
     tiles: Tuple[int, ...] = (0, 0, 0, 0, 0, 0)

@@ -152,6 +152,30 @@ def test_checkpoint_interval_minimizes_state_memory():
         autotune.choose_ckpt(wl.survey, mem_cap_bytes=1 << 20)
 
 
+def test_checkpoint_interval_fills_the_memory_budget():
+    """With a budget the interval is the one that fits it and recomputes the
+    fewest steps: the whole run (nothing recomputed) when it fits, the
+    least-memory K when nothing larger fits, and in between the K brute force
+    over every interval picks."""
+    wl = configs.damage_1024()
+    sv = wl.survey
+    nt = sv.nt_max
+    k_min = autotune.choose_ckpt(sv)
+    assert autotune.choose_ckpt(sv, budget_bytes=1 << 20) == k_min
+    full = autotune.state_bytes(sv, nt)
+    assert full < 230 << 30                              # 8 x 1024^2 x 3000 steps of u: about 210 GB
+    assert autotune.choose_ckpt(sv, budget_bytes=full) == nt
+    assert autotune.recomputed_steps(nt, nt) == 0
+    assert autotune.recomputed_steps(3000, 70) == 42 * 70     # 43 segments, the last (60 steps) kept
+    for frac in (0.3, 0.5, 0.9):
+        budget = int(full * frac)
+        K = autotune.choose_ckpt(sv, budget_bytes=budget)
+        assert autotune.state_bytes(sv, K) <= budget
+        fit = [k for k in range(k_min, nt + 1) if autotune.state_bytes(sv, k) <= budget]
+        assert autotune.recomputed_steps(nt, K) == min(autotune.recomputed_steps(nt, k) for k in fit)
+        assert autotune.recomputed_steps(nt, K) <= nt - K   # the kept last segment is a full one
+
+
 def test_damage_1024_fits_the_32_gb_cap():
     wl = configs.damage_1024()
     K = autotune.choose_ckpt(wl.survey, mem_cap_bytes=32 << 30)
