@@ -711,7 +711,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
           if (lane == lc / V) {
             const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
             const gfloat* gy = gx + (int64_t)nf * I.n_rec;
-            const float vgx = *gx, vgy = *gy;
+            const float vgx = ld_const(gx), vgy = ld_const(gy);
 #pragma unroll
             for (int e = 0; e < V; ++e) {
               if (e == lc % V) { dx[e] += vgx; dy[e] += vgy; }
@@ -1412,7 +1412,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
             const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
             if (rr < 0 || rr >= RW || lane != lc / V) continue;
             const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
-            const float vgx = gx[0], vgy = gx[(int64_t)nf * I.n_rec];
+            const float vgx = ld_const(gx), vgy = ld_const(gx + (int64_t)nf * I.n_rec);
 #pragma unroll
             for (int r = 0; r < RW; ++r)
 #pragma unroll
