@@ -117,13 +117,16 @@ def _weights(torch, h, device, dtype):
     return s * a, s * b, s * c
 
 
-def autograd_traces(torch, shot, C, faithful, D=None, nsteps=None):
+def autograd_traces(torch, shot, C, faithful, D=None, nsteps=None, fields=None, stride=0):
     """Receiver traces (2, nsteps, n_rec) through the reference's op sequence.
 
     C: six 0-dim tensors; D: optional (nx, ny) damage field, in which case the
-    per-cell operator of DI:100-176 runs on the fields D * C_p."""
+    per-cell operator of DI:100-176 runs on the fields D * C_p; fields: six
+    (nx, ny) stiffness fields for that operator instead (C unused).  stride > 0:
+    whole-field frames u^(kP+1) every stride steps (DI:367-393), returned as
+    (X, Y), each (1, n_frames, nx, ny)."""
     import torch.nn.functional as F
-    dev = C[0].device
+    dev = (fields if fields is not None else C)[0].device
     nx, ny = shot.nx, shot.ny
     nt = shot.nt if nsteps is None else nsteps
     sig = torch.tensor(shot.wavelet, device=dev)
@@ -131,11 +134,14 @@ def autograd_traces(torch, shot, C, faithful, D=None, nsteps=None):
     for (i, j, w, comp) in shot.sources:
         g[0, i, j] += w
     rho = torch.tensor(shot.rho, device=dev)
-    ri = torch.as_tensor(shot.receivers[:, 0], device=dev).long()
-    rj = torch.as_tensor(shot.receivers[:, 1], device=dev).long()
-    if D is None:
+    if not stride:
+        ri = torch.as_tensor(shot.receivers[:, 0], device=dev).long()
+        rj = torch.as_tensor(shot.receivers[:, 1], device=dev).long()
+    if fields is not None:   # per-cell fields on the transposed view (DI:143-148)
+        C11, C22, C12, C16, C26, C66 = (f.unsqueeze(0).transpose(1, 2) for f in fields)
+    elif D is None:
         C11, C22, C12, C16, C26, C66 = C
-    else:   # per-cell fields on the transposed view (DI:143-148)
+    else:
         C11, C22, C12, C16, C26, C66 = ((D * c).unsqueeze(0).transpose(1, 2) for c in C)
     W = None if faithful else _weights(torch, shot.h, dev, torch.float32)
     ux1 = torch.zeros(1, nx, ny, device=dev); uy1 = torch.zeros_like(ux1)
@@ -156,10 +162,15 @@ def autograd_traces(torch, shot, C, faithful, D=None, nsteps=None):
         uxn = 2 * ux1 - ux2 + shot.dt ** 2 * Lx
         uyn = 2 * uy1 - uy2 + shot.dt ** 2 * Ly
         ux2, uy2, ux1, uy1 = ux1, uy1, uxn, uyn
-        if faithful:
+        if stride:
+            if n % stride == 0:
+                hx.append(ux1); hy.append(uy1)
+        elif faithful:
             hx.append(ux1); hy.append(uy1)
         else:
             hx.append(ux1[0, ri, rj]); hy.append(uy1[0, ri, rj])
+    if stride:
+        return torch.stack(hx, dim=1), torch.stack(hy, dim=1)
     if faithful:
         X = torch.stack(hx, dim=1); Y = torch.stack(hy, dim=1)
         return torch.stack([X[0][:, ri, rj], Y[0][:, ri, rj]])
@@ -605,6 +616,231 @@ def tv_adam_main(args):
     print(json.dumps(line), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# Six-field damage inversion (Damage_inversion/Inversion_adam.py, DI) at the
+# reference's own size: 301^2 grid over 0.4 m, 4001 steps over 120 us, one
+# Gaussian y-force at the centre (f0 100 kHz, t0 10 us, amplitude 1e10,
+# a = 5e6, DI:885-895), rho 1610, whole-field frames every P = 5 steps,
+# six per-cell log-stiffness fields.  One step = one epoch of DI:465-560:
+# forward, 5 MSE_x + MSE_y on every second frame, backward, TV (1e-21) +
+# Adam (lr 0.02), here with the fused TV + Adam update.
+# ---------------------------------------------------------------------------
+DI_TRUE = (130.74e9, 11.50e9, 10.49e9, 30.67e9, 3.75e9, 14.77e9)   # DI:838-843
+
+
+def di_fields(n, seed=5):
+    """True fields: the healthy constants scaled by three elliptical damage
+    regions (D = 0.5-0.8), as DF's damaged elements (DF:145, DF:183-212)."""
+    import numpy as np
+    from ernn import ellipse_damage
+    D = ellipse_damage(n, n, 3, np.random.default_rng(seed), semi=(12, 30), level=(0.5, 0.8), taper=3)
+    return [(D * c).astype(np.float32) for c in DI_TRUE], D
+
+
+def di_autograd(torch, shot, fields0, obs, window, lam, stride):
+    """s/epoch of the reference's own DI epoch on the GPU: clamp(exp(theta)) per
+    cell (DI:304-322), the per-cell operator + frames (DI:100-176, DI:331-393),
+    5 MSE_x + MSE_y on every second frame, lam TV (DI:441-463), torch Adam.
+    Timed over `window` steps (memory-capped) and scaled to nt."""
+    bounds = [(50e9, 250e9), (5e9, 50e9), (5e9, 50e9), (0.0, 60e9), (0.0, 20e9), (5e9, 30e9)]
+    tape_step = 48 * 4 * shot.nx * shot.ny
+    mem_cap = torch.cuda.get_device_properties(0).total_memory // 2
+    W = max(10, min(window, shot.nt, mem_cap // tape_step))
+    thetas = [torch.nn.Parameter(f.clamp_min(1e6).log()) for f in fields0]
+    opt = torch.optim.Adam(thetas, lr=0.02)
+    nf = (W + stride - 1) // stride
+
+    def tv(c):
+        return (c[1:, :] - c[:-1, :]).abs().sum() + (c[:, 1:] - c[:, :-1]).abs().sum()
+
+    def it():
+        opt.zero_grad()
+        C = [torch.clamp(torch.exp(t), *b) for t, b in zip(thetas, bounds)]
+        X, Y = autograd_traces(torch, shot, None, True, nsteps=W, fields=C, stride=stride)
+        loss = 5 * torch.mean((X[:, ::2] - obs[0][:, :nf:2]) ** 2) + torch.mean((Y[:, ::2] - obs[1][:, :nf:2]) ** 2)
+        loss = loss + lam * sum(tv(c) for c in C)
+        loss.backward()
+        opt.step()
+    it()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) * shot.nt / W
+    log(f"autograd DI epoch on the GPU: {t:.3f} s" + (f" (extrapolated from {W} of {shot.nt} steps)" if W < shot.nt else ""))
+    return t, W
+
+
+def di_cpu_baseline(torch, shot, fields, budget_s, stride):
+    """The op-for-op CPU port (oracle/plain_torch.py, per-cell operator) with
+    autograd on the host cores over a bounded window of steps."""
+    sys.path.insert(0, REPO)
+    from oracle import plain_torch as pt
+    threads = torch.get_num_threads()
+
+    def run(nsteps):
+        C = [torch.tensor(f).unsqueeze(0).requires_grad_() for f in fields]
+        sig = torch.tensor(shot.wavelet[:nsteps]).unsqueeze(0)
+        spatial = torch.zeros(shot.nx, shot.ny)
+        for (i, j, w, comp) in shot.sources:
+            spatial[i, j] += w
+        t0 = time.perf_counter()
+        ux, uy = pt.forward(C, shot.h, shot.dt, nsteps, stride, sig, spatial, torch.tensor(shot.rho),
+                            per_cell=True, hoist_weights=True)
+        (ux.square().sum() + uy.square().sum()).backward()
+        return time.perf_counter() - t0
+
+    t_probe = run(10)
+    n = int(max(10, min(shot.nt, 10 * budget_s / max(t_probe, 1e-3))))
+    t = run(n)
+    return {"value": round(shot.nx * shot.ny * n / t / 1e9, 5), "unit": "Gcell-steps/s (fwd+adjoint)",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle/plain_torch.py per-cell forward + autograd backward, {shot.nx}x{shot.ny} grid, "
+                      f"{n} of {shot.nt} steps, torch {threads} threads, {t:.1f} s"}
+
+
+def di_main(args):
+    import numpy as np
+    import torch
+    import ernn
+    from ernn.autotune import tune_blocking, tune_tiles
+    from ernn.inversion import add_noise_rms
+
+    dev = torch.device("cuda:0")
+    n, P, lam = 301, 5, 1e-21
+    nt = args.nt or 4001
+    h, dt = 0.4 / (n - 1), 120e-6 / 4000
+    sim = ernn.SpatiallyVaryingAnisotropicSimulator((n, n), h, dt, nt, P, total_time=(nt - 1) * dt,
+                                                    rho_known=1610.0).to(dev)
+    names = ("C11", "C22", "C12", "C16", "C26", "C66")
+    thetas = [getattr(sim, "log_" + k) for k in names]
+    init = [t.detach().clone() for t in thetas]
+    true, D = di_fields(n)
+    eng = sim.engine(False)
+    shot = eng.survey.shots[0]
+    log(f"workload di-301: {n}^2 x {nt}, six per-cell fields, frames every {P} steps, "
+        f"{len(shot.sources)} source cells; {n * n * nt / 1e9:.3f} Gcell-steps per pass")
+    with torch.no_grad():
+        for t, f in zip(thetas, true):
+            t.copy_(torch.tensor(f, device=dev).clamp_min(1e6).log().unsqueeze(0))
+        ux, uy = sim()
+        gen = torch.Generator().manual_seed(1)
+        obs_x = add_noise_rms(ux, 0.01, gen)
+        obs_y = add_noise_rms(uy, 0.01, gen)
+        field_true = torch.cat([torch.tensor(f, device=dev).reshape(-1) for f in true])
+        for t, t0 in zip(thetas, init):
+            t.copy_(t0)
+    wav = sim.source_signal[0].contiguous()
+    steps = max(16, min(96, 40000 // max(1, n * n // 1000)))
+    if args.autotune:
+        tune = tune_tiles(eng, None, field_true, wav, steps=steps, which=(0, 2) if eng.info.fuse else (0, 1))
+        log("autotune: " + ", ".join(f"{('fwd', 'adj', 'fused')[w]} V={v} R={r} B={b} {ms * 1e3:.1f} us/step"
+                                     for w, (v, r, b, ms, _) in tune.items()))
+        bv, brw, bks, bms, btimes = tune_blocking(eng, None, field_true, wav, steps=steps)
+        log(f"autotune: forward blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
+            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f}; {_ranked(btimes)})")
+        bv, brw, bks, bms, btimes = tune_blocking(eng, None, field_true, wav, steps=steps, which=1)
+        log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
+            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f}; {_ranked(btimes)})")
+    inv = ernn.SpatiallyVaryingParameterInverter(sim, obs_x, obs_y, secondary_time_stride=2, lambda_tv=lam,
+                                                 verbose=False)
+    opt = inv.fused_optimizer(lr=0.02)
+
+    def epoch():
+        opt.zero_grad()
+        px, py = sim()
+        data = inv.data_loss(px, py)
+        data.backward()
+        opt.step()
+        return data
+
+    for _ in range(args.warmup):
+        epoch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = epoch()
+    torch.cuda.synchronize()
+    dt_iter = (time.perf_counter() - t0) / args.steps
+    cells = n * n
+    value = cells * nt / dt_iter / 1e9
+    log(f"HIP: {dt_iter * 1e3:.2f} ms/epoch, {value:.2f} Gcell-steps/s fwd+adjoint, data loss {loss.item():.4e}")
+
+    # per-kernel launch durations (hipEvents on the launch stream)
+    ws = eng.acquire_workspace()
+    tr0 = torch.zeros(eng.trace_floats, device=dev)
+    field = torch.cat([c.reshape(-1) for c in sim.get_material_params()[:6]]).detach().contiguous()
+    nk = min(200, nt)
+    eng.time_steps(0, 10, None, field, wav, ws, tr0, tr0)
+    fwd_ms = eng.time_steps(0, nk, None, field, wav, ws, tr0, tr0) / nk
+    adj_ms = eng.time_steps(1, nk, None, field, wav, ws, tr0, tr0) / nk
+    fused_ms = eng.time_steps(2, nk, None, field, wav, ws, tr0, tr0) / nk if eng.info.fuse else None
+    KS, AKS = eng.blocking[2], eng.adj_blocking[2]
+    blk_ms = ablk_ms = None
+    if KS > 1:
+        L = max(2, nk // KS)
+        eng.time_steps(3, 2, None, field, wav, ws, tr0, tr0)
+        blk_ms = eng.time_steps(3, L, None, field, wav, ws, tr0, tr0) / L
+    if AKS > 1:
+        L = max(2, nk // AKS)
+        eng.time_steps(4, 2, None, field, wav, ws, tr0, tr0)
+        ablk_ms = eng.time_steps(4, L, None, field, wav, ws, tr0, tr0) / L
+    eng.release_workspace(ws)
+    bf, ba = BYTES_PER_CELL[("field6", 0)], BYTES_PER_CELL[("field6", 1)]
+    K = int(eng.info.ckpt_every)
+    S = math.ceil(nt / K)
+    if AKS > 1:
+        kern = {"adj_block_kernel": (ablk_ms, sum(math.ceil(min(K, nt - s * K) / AKS) for s in range(S)), ba * AKS)}
+    else:
+        kern = {"adj_step_kernel": (adj_ms, nt, ba)}
+    if KS > 1:
+        kern["fwd_block_kernel"] = (blk_ms, math.ceil(nt / KS) + (S - 1) * math.ceil(K / KS), bf * KS)
+    else:
+        kern["fwd_step_kernel"] = (fwd_ms, nt + (S - 1) * K, bf)
+    name = max(kern, key=lambda k: kern[k][0] * kern[k][1])
+    ms, launches, bpc = kern[name]
+    achieved = cells * bpc / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_cell": bpc,
+                "cells_per_launch": cells, "avg_launch_us": round(ms * 1e3, 3), "launches_per_iter": launches,
+                "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),
+                "fused_step_us": round(fused_ms * 1e3, 3) if fused_ms else None,
+                "fwd_block_us_per_step": round(blk_ms / KS * 1e3, 3) if blk_ms else None,
+                "adj_block_us_per_step": round(ablk_ms / AKS * 1e3, 3) if ablk_ms else None}
+    log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3:.2f} us, fused "
+        f"{fused_ms * 1e3 if fused_ms else float('nan'):.2f} us per launch, time-blocked fwd "
+        f"{blk_ms / KS * 1e3 if blk_ms else float('nan'):.2f} us, adj "
+        f"{ablk_ms / AKS * 1e3 if ablk_ms else float('nan'):.2f} us per step; {name}: {achieved:.0f} GB/s algorithmic")
+    eng.trim()
+    torch.cuda.empty_cache()
+
+    baseline = None
+    if args.baseline:
+        fields0 = [torch.tensor(np.full((n, n), c, np.float32), device=dev) for c in DI_TRUE]
+        t_ag, W = di_autograd(torch, shot, fields0, (obs_x, obs_y), args.baseline_window, lam, P)
+        baseline = {"s_per_epoch": round(t_ag, 4), "value": round(cells * nt / t_ag / 1e9, 4),
+                    "unit": "Gcell-steps/s", "window_steps": W, "extrapolated": W < nt}
+    cpu = None
+    if args.cpu:
+        torch.set_num_threads(min(os.cpu_count() or 1, 16))
+        cpu = di_cpu_baseline(torch, shot, true, args.cpu_seconds, P)
+        log(f"cpu baseline: {cpu['value']} Gcell-steps/s on {cpu['cores']} threads")
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Gcell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt_iter * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "di-301", "grid": f"{n}x{n}", "nt": nt, "fields": 6, "frame_stride": P,
+                   "secondary_stride": 2, "lambda_tv": lam, "mode": "field6", "ckpt_every": K,
+                   "fused_backward": bool(eng.info.fuse), "tiles": list(eng.tiles), "blocking": list(eng.blocking),
+                   "adj_blocking": list(eng.adj_blocking), "parallelism": "none"},
+        "s_per_epoch": round(dt_iter, 5),
+        "autograd_baseline": baseline,
+        "speedup_vs_autograd": round(baseline["s_per_epoch"] / dt_iter, 2) if baseline else None,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -615,6 +851,8 @@ def main():
         return fem_main(args)
     if args.workload == "tv-adam-2048":
         return tv_adam_main(args)
+    if args.workload == "di-301":
+        return di_main(args)
     import numpy as np
     import torch
     import ernn
@@ -657,7 +895,7 @@ def main():
         log(f"tiles fixed: fwd V={vf} R={rf} B={bf}, adj V={va} R={ra} B={ba}")
     elif args.autotune:
         tune = tune_tiles(eng, params_true, field_true, model.wavelets, steps=max(20, min(100, 40000 // cells_k)),
-                          which=(0, 2) if sv.fused else (0, 1))
+                          which=(0, 2) if eng.info.fuse else (0, 1))
         log("autotune: " + ", ".join(f"{('fwd', 'adj', 'fused')[w]} V={v} R={r} B={b} {ms * 1e3:.1f} us/step"
                                      for w, (v, r, b, ms, _) in tune.items()))
     if args.blocking not in ("auto", "0"):
@@ -726,7 +964,7 @@ def main():
     eng.time_steps(0, 10, p, f, model.wavelets, ws, tr0, tr0)
     fwd_ms = eng.time_steps(0, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
     adj_ms = eng.time_steps(1, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
-    fused_ms = eng.time_steps(2, nk, p, f, model.wavelets, ws, tr0, tr0) / nk if sv.fused else None
+    fused_ms = eng.time_steps(2, nk, p, f, model.wavelets, ws, tr0, tr0) / nk if eng.info.fuse else None
     KS = eng.blocking[2]
     blk_ms = None
     if KS > 1:   # per time-blocked launch of KS steps
@@ -751,15 +989,15 @@ def main():
     # recomputed steps: every segment but the last (the forward pass keeps the
     # last segment's states); the fused schedule recomputes the last one alone
     S = math.ceil(nt / K)
-    n_rec = K if sv.fused else (S - 1) * K
+    n_rec = K if eng.info.fuse else (S - 1) * K
     if KS > 1:
-        n_blk = math.ceil(nt / KS) + (math.ceil(K / KS) if sv.fused else (S - 1) * math.ceil(K / KS))
+        n_blk = math.ceil(nt / KS) + (math.ceil(K / KS) if eng.info.fuse else (S - 1) * math.ceil(K / KS))
         kern["fwd_block_kernel"] = (blk_ms, n_blk, bf * KS, 0)
     else:
         kern["fwd_step_kernel"] = (fwd_ms, nt + n_rec, bf, 0)
-    if sv.fused and AKS > 1:   # fused time-blocked backward: AKS adjoint steps + AKS recompute steps
+    if eng.info.fuse and AKS > 1:   # fused time-blocked backward: AKS adjoint steps + AKS recompute steps
         kern["fused_block_kernel"] = (ablk_ms, math.ceil(nt / AKS), ba * AKS, bf * AKS)
-    elif sv.fused:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
+    elif eng.info.fuse:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
         kern["fused_step_kernel"] = (fused_ms, nt, ba, bf)
     elif AKS > 1:
         kern["adj_block_kernel"] = (ablk_ms, sum(math.ceil(min(K, nt - s * K) / AKS)
@@ -841,7 +1079,7 @@ def main():
         "config": {"workload": wl.name, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
                    "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
                    "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every), "graph": bool(sv.use_graph),
-                   "fused_backward": bool(sv.fused),
+                   "fused_backward": bool(eng.info.fuse),
                    "blocking": list(eng.blocking),
                    "adj_blocking": list(eng.adj_blocking), "fused_recompute_rw": eng.ff_rw,
                    "tiles": list(eng.tiles),

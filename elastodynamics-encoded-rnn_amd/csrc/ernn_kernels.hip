@@ -1191,7 +1191,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   constexpr int kCols = kWave * V;
   constexpr int WB = blk_waves(RW);
   constexpr bool kD = (MODE == kDField);
-  static_assert(MODE != kField6, "six-field adjoint: one step per launch");
+  static_assert(MODE != kField6, "six-field adjoint: adj_block_body6");
   static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
   __shared__ float xch[2][WB][2][2][kCols];
 
@@ -1499,9 +1499,242 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   }
 }
 
+// --------------------------------------------------------------------------
+// Six-field time-blocked adjoint (per-cell C_p: the operator of DI:151-163
+// transposed).  The blocks, bands, state layout and schedule are those of
+// adj_block_body; what the stencil reads are the six weighted fields of
+// weigh6 (each cell's own C_p times its lam), formed by every lane for its
+// own cells, so a column neighbour's value is one DPP shift away:
+//   (A^T lam)_x = a (Dxx w0 + Dyy w1) + (a/4) Dxy w2,  likewise y from w3..w5,
+// the arithmetic of adj_tile's six-field branch.  Only w0 (Dxx), w3 (Dxx)
+// and w2, w5 (Dxy) reach across rows, so those four cross band edges through
+// LDS.  The per-cell gradients lam^{m+1} . dA/dC_p u^m (field_grads, a stencil
+// of the banked u with one halo row) are summed over the launch's steps in
+// registers and added to the FP64 accumulators once per launch.
+// --------------------------------------------------------------------------
+template <int RW, int KS>
+__device__ __forceinline__ void adj_block_body6(KArg<BlockAdjArgs>* ap, int bid) {
+#pragma clang fp contract(fast)
+  constexpr int V = 1;
+  const auto& a = *ap;
+  constexpr int kCols = kWave;
+  constexpr int WB = blk_waves(RW);
+  __shared__ float xch[2][WB][2][4][kCols];   // [buffer][wavefront][top/bottom][w0, w2, w3, w5][column]
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x >> 6;
+  const int chunk = (a.n_blocks + 7) >> 3;
+  const int t = (bid & 7) * chunk + (bid >> 3);
+  const GCall C = global_call(*a.call);
+  const BlockDev bl = a.blocks[max(min(t, a.n_blocks - 1), 0)];
+  int n_blocks = a.n_blocks;
+  asm("" : "+s"(n_blocks) : "s"((uint64_t)C.ws), "s"(bl.I.nt), "s"(bl.I.field), "s"(bl.I.pool_off),
+      "s"(bl.oi0), "s"(a.m0), "s"(a.ks), "s"(a.s_in));
+  if (t >= n_blocks) return;
+  const InstDev I = bl.I;
+  if (a.m0 - a.ks + 1 > I.nt) return;   // this instance's reverse recurrence has not started
+
+  const Band<V, RW, KS, WB> bd(bl, I, w, lane);
+  const int gi0 = bd.gi0;
+  const float ka = (float)(I.gscale * (double)I.hinv2), kq = 0.25f * ka;   // (dt^2/rho) a, a / 4
+  const double gsa = I.gscale * (double)I.hinv2;                           // per-cell sums carry 1 / a
+
+  // ---- loads: lam level m0 + 1 (lam, c, w) and the six stiffness fields
+  Rsrc rin[kStateFields];
+#pragma unroll
+  for (int q = 0; q < kStateFields; ++q) rin[q] = field_rsrc(state_field(C, I, a.s_in, q), I.field);
+  float l[2][RW], c[2][RW], wv[2][RW], cf[6][RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const uint32_t so = bd.srow(I, r);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      l[q][r] = bload(rin[q], bd.vcol[0], so);
+      c[q][r] = bload(rin[2 + q], bd.vcol[0], so);
+      wv[q][r] = bload(rin[4 + q], bd.vcol[0], so);
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) cf[q][r] = bload(field_rsrc(C.ws + I.coef_off + q * I.field, I.field), bd.vcol[0], so);
+  }
+  float gacc[6][RW];   // per-cell gradients of this launch (FP32 over at most KS steps)
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+#pragma unroll
+    for (int r = 0; r < RW; ++r) gacc[q][r] = 0.f;
+
+  auto weights = [&](int r, float (&o)[6]) {
+    const float cr[6] = {cf[0][r], cf[1][r], cf[2][r], cf[3][r], cf[4][r], cf[5][r]};
+    weigh6(cr, l[0][r], l[1][r], o);
+  };
+  auto publish = [&](int buf) {
+    float o[6];
+    weights(0, o);
+    xch[buf][w][0][0][lane] = o[0];
+    xch[buf][w][0][1][lane] = o[2];
+    xch[buf][w][0][2][lane] = o[3];
+    xch[buf][w][0][3][lane] = o[5];
+    weights(RW - 1, o);
+    xch[buf][w][1][0][lane] = o[0];
+    xch[buf][w][1][1][lane] = o[2];
+    xch[buf][w][1][2][lane] = o[3];
+    xch[buf][w][1][3][lane] = o[5];
+  };
+  // u^m on the band rows with one halo row on each side; step s + 1's loads
+  // are issued as soon as step s has used its u
+  float urx[RW + 2], ury[RW + 2];
+  const bool grad_rows = bd.own_lo < bd.own_hi;   // wave-uniform
+  auto load_u = [&](int s) {
+    const Rsrc bx = field_rsrc(bank_field(C, I, a.bank_u[s], 0), I.field);
+    const Rsrc by = field_rsrc(bank_field(C, I, a.bank_u[s], 1), I.field);
+#pragma unroll
+    for (int r = 0; r < RW + 2; ++r) {
+      const uint32_t so = bd.srow(I, r - 1);
+      urx[r] = bload(bx, bd.vcol[0], so);
+      ury[r] = bload(by, bd.vcol[0], so);
+    }
+  };
+  if (grad_rows && a.m0 <= I.nt) load_u(0);
+  publish(0);
+  __syncthreads();
+  const int nf = I.n_frames;
+
+  for (int s = 0; s < a.ks; ++s) {
+    const int m = a.m0 - s;
+    const int buf = s & 1;
+    if (m <= I.nt) {   // block-uniform: an instance with fewer steps starts later, from its zero state
+      // weighted rows -1 .. RW: w0, w3 (Dxx) by value, w2, w5 (Dxy) with their
+      // column neighbours; w1, w4 (Dyy) on the band's own rows
+      float W0[RW + 2], W3[RW + 2];
+      Row<V> W2[RW + 2], W5[RW + 2], W1[RW], W4[RW];
+      {
+        const float z = 0.f;
+        const bool up = w > 0, dn = w < WB - 1;
+        W0[0] = up ? xch[buf][w - 1][1][0][lane] : z;
+        W3[0] = up ? xch[buf][w - 1][1][2][lane] : z;
+        W0[RW + 1] = dn ? xch[buf][w + 1][0][0][lane] : z;
+        W3[RW + 1] = dn ? xch[buf][w + 1][0][2][lane] : z;
+        const float t2[1] = {up ? xch[buf][w - 1][1][1][lane] : z}, t5[1] = {up ? xch[buf][w - 1][1][3][lane] : z};
+        const float b2[1] = {dn ? xch[buf][w + 1][0][1][lane] : z}, b5[1] = {dn ? xch[buf][w + 1][0][3][lane] : z};
+        W2[0] = make_row_z<V>(t2);
+        W5[0] = make_row_z<V>(t5);
+        W2[RW + 1] = make_row_z<V>(b2);
+        W5[RW + 1] = make_row_z<V>(b5);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          float o[6];
+          weights(r, o);
+          W0[r + 1] = o[0];
+          W3[r + 1] = o[3];
+          const float o1[1] = {o[1]}, o2[1] = {o[2]}, o4[1] = {o[4]}, o5[1] = {o[5]};
+          W1[r] = make_row_z<V>(o1);
+          W2[r + 1] = make_row_z<V>(o2);
+          W4[r] = make_row_z<V>(o4);
+          W5[r + 1] = make_row_z<V>(o5);
+        }
+      }
+      Row<V> ux[RW + 2], uy[RW + 2];
+      if (grad_rows) {
+#pragma unroll
+        for (int r = 0; r < RW + 2; ++r) {
+          const float ox[1] = {urx[r]}, oy[1] = {ury[r]};
+          ux[r] = make_row_z<V>(ox);
+          uy[r] = make_row_z<V>(oy);
+        }
+      }
+      constexpr int x = 1;
+      float dx[RW], dy[RW];
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        const float sx = __builtin_fmaf(-2.f, W0[r + 1], W0[r] + W0[r + 2]) + uyy<V>(W1[r], x);
+        const float sy = __builtin_fmaf(-2.f, W3[r + 1], W3[r] + W3[r + 2]) + uyy<V>(W4[r], x);
+        dx[r] = __builtin_fmaf(ka, sx, kq * uxy<V>(W2[r], W2[r + 2], x));
+        dy[r] = __builtin_fmaf(ka, sy, kq * uxy<V>(W5[r], W5[r + 2], x));
+        // per-cell gradients on the band rows of the output tile (lam^{m+1} with u^m)
+        if (r >= bd.own_lo && r < bd.own_hi) {
+          const UDeriv du = uderivs<V>(ux[r], ux[r + 1], ux[r + 2], uy[r], uy[r + 1], uy[r + 2], x);
+          float g[6];
+          field_grads(du, l[0][r], l[1][r], g);
+#pragma unroll
+          for (int q = 0; q < 6; ++q) gacc[q][r] += g[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+#pragma unroll
+        for (int r = 0; r < RW; ++r) asm volatile("" : "+v"(gacc[q][r]));
+
+      // residual injection on recorded steps m = kP + 1 (the whole region)
+      if (((m - 1) % a.stride == 0) && ((m - 1) / a.stride < nf)) {
+        const int k = (m - 1) / a.stride;
+        if (a.snapshot) {
+#pragma unroll
+          for (int r = 0; r < RW; ++r) {
+            if (!bd.row_in(I, r) || !bd.col_in[0]) continue;
+            const gfloat* gx = C.gtr + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + bd.gj0;
+            dx[r] += gx[0];
+            dy[r] += gx[(int64_t)nf * I.nx * I.ny];
+          }
+        } else {
+          for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
+            const RecEv ev = ld_rec(a.rec + rp);
+            const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
+            if (rr < 0 || rr >= RW || lane != lc) continue;
+            const gfloat* gx = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+            const float vgx = ld_const(gx), vgy = ld_const(gx + (int64_t)nf * I.n_rec);
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
+              if (r == rr) {
+                dx[r] += vgx;
+                dy[r] += vgy;
+              }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        advance(l[0][r], c[0][r], wv[0][r], dx[r]);
+        advance(l[1][r], c[1][r], wv[1][r], dy[r]);
+      }
+    }
+    if (s + 1 < a.ks) {
+      if (grad_rows && m - 1 <= I.nt) load_u(s + 1);
+      publish(buf ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // ---- the launch's last level and the per-cell gradients, on the output tile
+  if (bd.col_own && bd.col_in[0]) {
+    Rsrc ro[kStateFields];
+#pragma unroll
+    for (int q = 0; q < kStateFields; ++q) ro[q] = field_rsrc(state_field(C, I, a.s_out, q), I.field);
+    gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (r < bd.own_lo || r >= bd.own_hi) continue;
+      const uint32_t so = bd.srow(I, r);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        bstore(l[q][r], ro[q], bd.vcol[0], so);
+        bstore(c[q][r], ro[2 + q], bd.vcol[0], so);
+        bstore(wv[q][r], ro[4 + q], bd.vcol[0], so);
+      }
+      const uint32_t cell = so / (uint32_t)sizeof(float) + bd.vcol[0] / (uint32_t)sizeof(float);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        gdouble* pa = acc64 + (int64_t)q * I.field + cell;
+        *pa = *pa + gsa * (double)gacc[q][r];
+      }
+    }
+  }
+}
+
 template <int MODE, int V, int RW, int KS>
 __global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockAdjArgs) {
-  adj_block_body<MODE, V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
+  if constexpr (MODE == kField6)
+    adj_block_body6<RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
+  else
+    adj_block_body<MODE, V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
 
 // the argument is read in place through the kernarg segment (offset 0)
@@ -1750,6 +1983,7 @@ hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st)
 // checked by the build).  RW = 6 keeps the damage adjoint at three
 // wavefronts per SIMD (166 VGPRs), which outruns RW = 8 at two.
 #define ERNN_ADJ_BLOCK_CASES_DFIELD(X) X(1, 4, 2) X(1, 4, 4) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 8)
+#define ERNN_ADJ_BLOCK_CASES_FIELD6(X) X(1, 2, 2) X(1, 2, 4) X(1, 4, 2) X(1, 4, 4)
 #define ERNN_ADJ_BLOCK_CASES(X) ERNN_ADJ_BLOCK_CASES_DFIELD(X) X(1, 8, 5) X(2, 4, 2) X(2, 4, 4)
 #define ERNN_ADJ_BLOCK_CASE(v_, r_, k_)                                                             \
   if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
@@ -1760,8 +1994,10 @@ template <int MODE>
 static hipError_t launch_adj_block_m(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
   if constexpr (MODE == kConst6) {
     ERNN_ADJ_BLOCK_CASES(ERNN_ADJ_BLOCK_CASE)
-  } else {
+  } else if constexpr (MODE == kDField) {
     ERNN_ADJ_BLOCK_CASES_DFIELD(ERNN_ADJ_BLOCK_CASE)
+  } else {
+    ERNN_ADJ_BLOCK_CASES_FIELD6(ERNN_ADJ_BLOCK_CASE)
   }
   return hipErrorInvalidValue;
 }
@@ -1771,6 +2007,7 @@ hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t 
   switch (a.mode) {
     case kConst6: return launch_adj_block_m<kConst6>(a, V, grid, st);
     case kDField: return launch_adj_block_m<kDField>(a, V, grid, st);
+    case kField6: return launch_adj_block_m<kField6>(a, V, grid, st);
   }
   return hipErrorInvalidValue;
 }

@@ -112,10 +112,11 @@ bool block_ok(int mode, int V, int RW, int KS) {
   return !(mode != ERNN_PARAM_CONST6 && V == 2 && RW == 8);
 }
 // Time-blocked adjoint shapes (ERNN_ADJ_BLOCK_CASES): constant and damage
-// modes; the forward's V = 1 shapes and (1, 6, 4); V = 2 (RW = 4) for
-// constant stiffness only.
+// modes: the forward's V = 1 shapes and (1, 6, 4); V = 2 (RW = 4) for
+// constant stiffness only.  Six fields (ERNN_ADJ_BLOCK_CASES_FIELD6): V = 1,
+// RW in {2, 4} (16-row regions), KS in {2, 4}.
 bool adj_block_ok(int mode, int V, int RW, int KS) {
-  if (mode == ERNN_PARAM_FIELD6) return false;
+  if (mode == ERNN_PARAM_FIELD6) return V == 1 && (RW == 2 || RW == 4) && (KS == 2 || KS == 4);
   if (RW == 6) return V == 1 && KS == 4;
   if (KS == 5) return mode == ERNN_PARAM_CONST6 && V == 1 && RW == 8;
   if (!block_ok(mode, V, RW, KS)) return false;
@@ -137,9 +138,9 @@ bool ablk_ok(int mode, bool fuse, int V, int RW, int KS) {
   return fuse ? fused_block_ok(mode, V, RW, KS) : adj_block_ok(mode, V, RW, KS);
 }
 const char* const kAblkShapes =
-    "time-blocked adjoint shape (V, RW, KS): constant or damage mode; unfused: V = 1 with RW in {4,8}, "
-    "KS in {2,4,8}, 4 RW > 2 KS, or (1, 6, 4), or (constant) (1, 8, 5) or V = 2, RW = 4, KS in {2,4}; fused: "
-    "(1, 4, 2), (1, 4, 4) or (constant) (1, 8, 4)";
+    "time-blocked adjoint shape (V, RW, KS): unfused: V = 1 with RW in {4,8}, KS in {2,4,8}, 4 RW > 2 KS, or "
+    "(1, 6, 4), or (constant) (1, 8, 5) or V = 2, RW = 4, KS in {2,4}; six fields: V = 1, RW in {2,4}, KS in "
+    "{2,4}; fused (constant or damage mode): (1, 4, 2), (1, 4, 4) or (constant) (1, 8, 4)";
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 

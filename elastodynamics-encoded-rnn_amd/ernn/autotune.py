@@ -92,6 +92,8 @@ BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = (
 # time-blocked adjoint shapes (constant and damage modes; V = 2 for constant stiffness only)
 ADJ_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 6, 4), (1, 8, 2), (1, 8, 4),
                                                       (1, 8, 5), (1, 8, 8), (2, 4, 2), (2, 4, 4))
+# time-blocked adjoint shapes of the six-field mode (16-row regions: 8 waves x 2 rows or 4 x 4)
+ADJ_BLOCK_SHAPES_FIELD6: Tuple[Tuple[int, int, int], ...] = ((1, 2, 2), (1, 2, 4), (1, 4, 2), (1, 4, 4))
 # time-blocked shapes of the fused backward (fuse plans): both halves of a launch share one shape;
 # (1, 8, 4) is built for six constants only
 FUSED_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 8, 4))
@@ -153,10 +155,12 @@ def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int =
         shapes = BLOCK_SHAPES if shapes is None else shapes
         cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or b[:2] != (2, 8)]
     else:
-        shapes = (FUSED_BLOCK_SHAPES if fused else ADJ_BLOCK_SHAPES) if shapes is None else shapes
-        cands = [(0, 0, 0)] + ([b for b in shapes if mode == "const6" or (b[0] == 1 and b[2] != 5 and
-                                                                           not (fused and b[1] == 8))]
-                               if mode != "field6" else [])
+        if mode == "field6":
+            cands = [(0, 0, 0)] + ([] if fused else list(ADJ_BLOCK_SHAPES_FIELD6 if shapes is None else shapes))
+        else:
+            shapes = (FUSED_BLOCK_SHAPES if fused else ADJ_BLOCK_SHAPES) if shapes is None else shapes
+            cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or (b[0] == 1 and b[2] != 5 and
+                                                                             not (fused and b[1] == 8))]
     ws = engine.acquire_workspace()
     traces = torch.zeros(engine.trace_floats, dtype=torch.float32, device=engine.device)
     samples = {c: [] for c in cands}

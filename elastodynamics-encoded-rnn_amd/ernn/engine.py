@@ -41,6 +41,10 @@ class Engine:
                 free, _ = torch.cuda.mem_get_info(self.device)
                 budget = int(0.8 * free)
             K = choose_ckpt(survey, budget_bytes=budget)
+        # The fused backward overlaps each segment's adjoint with the next
+        # segment's recompute; with one segment (every state kept) there is no
+        # recompute to overlap, so the automatic choice is the unfused one.
+        self._fuse = survey.fused and not (survey.fuse is None and survey.nt_max <= K)
         self._build(K)
         self._pool: List[torch.Tensor] = []
 
@@ -75,7 +79,7 @@ class Engine:
         t = tuple(sv.tiles) + (0,) * (6 - len(sv.tiles))
         d.fwd_v, d.fwd_r, d.adj_v, d.adj_r, d.fwd_bpc, d.adj_bpc = t[:6]
         d.use_graph = int(sv.use_graph)
-        d.fuse = int(sv.fused)
+        d.fuse = int(self._fuse)
         d.blk_v, d.blk_rw, d.blk_ks = (int(x) for x in sv.blocking)
         d.ablk_v, d.ablk_rw, d.ablk_ks = (int(x) for x in sv.adj_blocking)
         d.ablk_ff_rw = int(sv.adj_ff_rw)

@@ -59,7 +59,8 @@ def _tried(eng, which):
     ("dfield", False, {s for s in autotune.ADJ_BLOCK_SHAPES if s[0] == 1 and s[2] != 5}),
     ("const6", True, set(autotune.FUSED_BLOCK_SHAPES)),
     ("dfield", True, {s for s in autotune.FUSED_BLOCK_SHAPES if s[1] == 4}),
-    ("field6", False, set()),
+    ("field6", False, set(autotune.ADJ_BLOCK_SHAPES_FIELD6)),
+    ("field6", True, set()),
 ])
 def test_adjoint_blocking_candidates_follow_the_built_shapes(mode, fuse, expect):
     eng = StubEngine(mode, fuse)

@@ -180,6 +180,76 @@ def test_blocked_adjoint_matches_single_step_and_oracle(mode, shape):
     e0.close(); e1.close()
 
 
+FIELD6_ADJ_SHAPES = [(1, 2, 2), (1, 2, 4), (1, 4, 2), (1, 4, 4)]
+
+
+def _field6_pair(shots, ablk, K, snapshot=False, stride=1, **kw):
+    def sv(a):
+        return ernn.Survey(shots, mode="field6", ckpt_every=K, tiles=(2, 2, 1, 2, 0, 0), snapshot=snapshot,
+                           rec_stride=stride, fuse=False, adj_blocking=a)
+    return run_hip(sv((0, 0, 0)), **kw), run_hip(sv(ablk), **kw)
+
+
+def _field6_errors(g_blk, g_one, gcell, nx, ny):
+    a = g_blk.reshape(6, nx, ny)
+    b = g_one.reshape(6, nx, ny)
+    e_ss = max(rel_l2(a[p], b[p]) for p in range(6))
+    e_or = max(np.abs(a[p] - gcell[p]).max() / np.abs(gcell[p]).max() for p in range(6))
+    return e_ss, e_or
+
+
+@pytest.mark.parametrize("shape", FIELD6_ADJ_SHAPES)
+def test_blocked_six_field_adjoint_matches_single_step_and_oracle(shape):
+    """The six-field time-blocked adjoint (per-cell C_p) vs the single-step
+    adjoint (the same lam recurrence; the per-cell gradients are summed per
+    launch before their FP64 update) and vs the FP64 oracle, with sources and
+    receivers on output-tile edges and in halos (the walked receiver list);
+    K = 7 leaves one-step launches at segment ends."""
+    from gpu_helpers import oracle_run
+    V, RW, KS = shape
+    nx, ny, nt, h = 45, 150, 47, 0.2e-3
+    rng = np.random.default_rng(17)
+    shot = make_shot(nx, ny, nt, h, sources=_edge_sources(nx, ny, h, 1, 4, KS),
+                     receivers=_receivers(nx, ny, 1, 4, KS))
+    fields = np.stack([rng.uniform(0.6, 1.0, (nx, ny)) * c for c in C_ANISO]).astype(np.float32)
+    G = rng.standard_normal((2, nt, len(shot.receivers)))
+    (e0, t0, g0), (e1, t1, g1) = _field6_pair([shot], shape, K=7, field=fields, G=G)
+    assert e1.adj_blocking == shape
+    assert np.array_equal(t0, t1)
+    assert g1[0] is None
+    _, gcell = oracle_run(shot, [f.astype(np.float64) for f in fields], 1, False, G)
+    e_ss, e_or = _field6_errors(g1[1], g0[1], gcell, nx, ny)
+    print(f"field6 adj {shape}: dJ/dC_p(cell) vs single-step rel-L2 {e_ss:.2e}, vs FP64 worst max rel {e_or:.2e}")
+    assert e_ss < 1e-5
+    assert e_or < 1e-4
+    e0.close(); e1.close()
+
+
+@pytest.mark.parametrize("shape", [(1, 2, 4), (1, 4, 4)])
+def test_blocked_six_field_adjoint_snapshot(shape):
+    """The damage inversion's layout: whole-field residuals every P = 5 steps, a
+    Gaussian force spread over several cells, a grid of a few blocks with an
+    odd size; blocked vs single-step and vs the FP64 oracle."""
+    from gpu_helpers import oracle_run
+    nx, ny, nt, h, P = 53, 97, 43, 0.2e-3, 5
+    rng = np.random.default_rng(19)
+    ci, cj = nx // 2, ny // 2
+    src = [(ci + di, cj + dj, float(np.exp(-0.5 * (di * di + dj * dj))) / h ** 2, 1)
+           for di in (-1, 0, 1) for dj in (-1, 0, 1)]
+    shot = make_shot(nx, ny, nt, h, sources=src)
+    fields = np.stack([rng.uniform(0.6, 1.0, (nx, ny)) * c for c in C_ANISO]).astype(np.float32)
+    nf = (nt + P - 1) // P
+    G = rng.standard_normal((2, nf, nx, ny))
+    (e0, t0, g0), (e1, t1, g1) = _field6_pair([shot], shape, K=9, snapshot=True, stride=P, field=fields, G=G)
+    assert np.array_equal(t0, t1)
+    _, gcell = oracle_run(shot, [f.astype(np.float64) for f in fields], P, True, G)
+    e_ss, e_or = _field6_errors(g1[1], g0[1], gcell, nx, ny)
+    print(f"field6 snapshot adj {shape}: vs single-step rel-L2 {e_ss:.2e}, vs FP64 worst max rel {e_or:.2e}")
+    assert e_ss < 1e-5
+    assert e_or < 1e-4
+    e0.close(); e1.close()
+
+
 def test_blocked_adjoint_snapshot_and_uneven_batch():
     """Whole-field residuals with stride 2; instances of different sizes and step
     counts (each starts its reverse recurrence at its own nt) in one launch."""

@@ -82,12 +82,15 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
     ({}, {"blk_v": 1, "blk_rw": 4, "blk_ks": 3}, "time-blocked"),
     ({}, {"param_mode": 1, "blk_v": 2, "blk_rw": 8, "blk_ks": 4}, "time-blocked"),
     ({}, {"ablk_v": 2, "ablk_rw": 8, "ablk_ks": 4}, "time-blocked adjoint"),
-    ({}, {"param_mode": 2, "ablk_v": 1, "ablk_rw": 4, "ablk_ks": 2}, "time-blocked adjoint"),
+    ({}, {"param_mode": 2, "fuse": 1, "ablk_v": 1, "ablk_rw": 4, "ablk_ks": 2}, "time-blocked adjoint"),
     ({}, {"param_mode": 1, "ablk_v": 2, "ablk_rw": 4, "ablk_ks": 2}, "time-blocked adjoint"),
     ({}, {"blk_v": 1, "blk_rw": 6, "blk_ks": 4}, "time-blocked forward"),
     ({}, {"ablk_v": 1, "ablk_rw": 6, "ablk_ks": 2}, "time-blocked adjoint"),
     ({}, {"ablk_v": 2, "ablk_rw": 6, "ablk_ks": 4}, "time-blocked adjoint"),
     ({}, {"param_mode": 2, "ablk_v": 1, "ablk_rw": 6, "ablk_ks": 4}, "time-blocked adjoint"),
+    ({}, {"param_mode": 2, "ablk_v": 1, "ablk_rw": 8, "ablk_ks": 4}, "time-blocked adjoint"),
+    ({}, {"param_mode": 2, "ablk_v": 1, "ablk_rw": 2, "ablk_ks": 8}, "time-blocked adjoint"),
+    ({}, {"param_mode": 2, "ablk_v": 2, "ablk_rw": 2, "ablk_ks": 2}, "time-blocked adjoint"),
 ])
 def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
     inst = dict(GOOD, **patch)
