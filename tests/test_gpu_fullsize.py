@@ -218,6 +218,71 @@ def test_damage_field_every_shape_full_grid(name):
     _sweep(sv, K, C0, D, G, ref_tr, ref_g, name)
 
 
+# ------------------------------------------------------------------- di-301
+DI_TRUE = (130.74e9, 11.50e9, 10.49e9, 30.67e9, 3.75e9, 14.77e9)   # DI:838-843
+DI_NAMES = ("C11", "C22", "C12", "C16", "C26", "C66")
+
+
+def test_di_301_timed_shapes_full_grid():
+    """The damage inversion at its own grid (301^2 over 0.4 m, dt = 30 ns, the
+    centred Gaussian force, rho 1610, frames every 5 steps, six per-cell fields
+    with three damage ellipses), 600 of its 4001 steps: the shapes bench.py
+    times on di-301 (forward (1, 4, 4), adjoint (1, 2, 4)) vs the one-step
+    kernels (frames bitwise, gradients within 1e-5) and vs the reference's op
+    sequence with autograd through clamp(exp(theta)) (DI:304-322,
+    DI:100-176, DI:331-393): frames within 1e-5 relative L2, gradients of the
+    six log-fields within 1e-4."""
+    n, nt, P = 301, 600, 5
+    h, dt = 0.4 / (n - 1), 120e-6 / 4000
+    D = ernn.ellipse_damage(n, n, 3, np.random.default_rng(5), semi=(12, 30), level=(0.5, 0.8), taper=3)
+    true = [(D * c).astype(np.float32) for c in DI_TRUE]
+    nf = (nt + P - 1) // P
+    rng = np.random.default_rng(29)
+    Gx = torch.tensor(rng.standard_normal((1, nf, n, n)), dtype=torch.float32, device=DEV)
+    Gy = torch.tensor(rng.standard_normal((1, nf, n, n)), dtype=torch.float32, device=DEV)
+
+    def make():
+        sim = ernn.SpatiallyVaryingAnisotropicSimulator((n, n), h, dt, nt, P, total_time=(nt - 1) * dt,
+                                                        rho_known=1610.0).to(DEV)
+        with torch.no_grad():
+            for k, f in zip(DI_NAMES, true):
+                getattr(sim, "log_" + k).copy_(torch.tensor(f, device=DEV).log().unsqueeze(0))
+        return sim
+
+    def hip(fwd, adj):
+        sim = make()
+        eng = sim.engine(False)
+        eng.set_blocking(*fwd)
+        eng.set_blocking(*adj, which=1)
+        assert eng.info.ckpt_every == nt and not eng.info.fuse   # every state kept, nothing recomputed
+        ux, uy = sim()
+        ((ux * Gx).sum() + (uy * Gy).sum()).backward()
+        grads = [getattr(sim, "log_" + k).grad.detach().double().cpu().numpy() for k in DI_NAMES]
+        out = ux.detach().cpu().numpy(), uy.detach().cpu().numpy(), grads
+        eng.close()
+        return out
+
+    x1, y1, g1 = hip((0, 0, 0), (0, 0, 0))
+    xt, yt, gt = hip((1, 4, 4), (1, 2, 4))
+    assert np.array_equal(x1, xt) and np.array_equal(y1, yt)
+    e_ts = max(rel_l2(a, b) for a, b in zip(gt, g1))
+
+    sim = make()
+    thetas = [getattr(sim, "log_" + k) for k in DI_NAMES]
+    C = sim.get_material_params()[:6]
+    ux, uy = pt.forward(list(C), h, dt, nt, P, sim.source_signal, sim.gaussian_dist, sim.rho, per_cell=True)
+    ((ux * Gx).sum() + (uy * Gy).sum()).backward()
+    gref = [t.grad.detach().double().cpu().numpy() for t in thetas]
+    e_tr = max(rel_l2(x1, ux.detach().cpu().numpy()), rel_l2(y1, uy.detach().cpu().numpy()))
+    e_g1 = max(rel_l2(a, b) for a, b in zip(g1, gref))
+    e_gt = max(rel_l2(a, b) for a, b in zip(gt, gref))
+    print(f"di-301 301^2 x {nt}: timed shapes grads vs one step {e_ts:.2e}; vs plain autograd: frames {e_tr:.2e}, "
+          f"log-field grads one step {e_g1:.2e}, timed {e_gt:.2e}")
+    assert e_ts < 1e-5
+    assert e_tr < 1e-5
+    assert e_g1 < 1e-4 and e_gt < 1e-4
+
+
 # ------------------------------------------------------------- batched-uneven
 def test_batched_uneven_full_workload():
     """The batched-uneven workload itself: 24 seeded instances of
