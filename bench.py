@@ -189,7 +189,7 @@ def time_autograd(torch, wl, theta0, scale, D0, observed, window):
     shots = wl.survey.shots
     s0 = shots[0]
     tape_step = 48 * 4 * s0.nx * s0.ny
-    mem_cap = torch.cuda.get_device_properties(0).total_memory // 2
+    mem_cap = min(torch.cuda.get_device_properties(0).total_memory // 2, int(0.8 * torch.cuda.mem_get_info()[0]))
     W = max(10, min(window, s0.nt, mem_cap // tape_step))
     scale_f = sum(s.nt for s in shots) / W
     for faithful in (True, False):
@@ -644,7 +644,7 @@ def di_autograd(torch, shot, fields0, obs, window, lam, stride):
     Timed over `window` steps (memory-capped) and scaled to nt."""
     bounds = [(50e9, 250e9), (5e9, 50e9), (5e9, 50e9), (0.0, 60e9), (0.0, 20e9), (5e9, 30e9)]
     tape_step = 48 * 4 * shot.nx * shot.ny
-    mem_cap = torch.cuda.get_device_properties(0).total_memory // 2
+    mem_cap = min(torch.cuda.get_device_properties(0).total_memory // 2, int(0.8 * torch.cuda.mem_get_info()[0]))
     W = max(10, min(window, shot.nt, mem_cap // tape_step))
     thetas = [torch.nn.Parameter(f.clamp_min(1e6).log()) for f in fields0]
     opt = torch.optim.Adam(thetas, lr=0.02)
@@ -786,6 +786,7 @@ def di_main(args):
         eng.time_steps(4, 2, None, field, wav, ws, tr0, tr0)
         ablk_ms = eng.time_steps(4, L, None, field, wav, ws, tr0, tr0) / L
     eng.release_workspace(ws)
+    del ws, tr0
     bf, ba = BYTES_PER_CELL[("field6", 0)], BYTES_PER_CELL[("field6", 1)]
     K = int(eng.info.ckpt_every)
     S = math.ceil(nt / K)
@@ -1044,6 +1045,7 @@ def main():
 
     # the pooled workspaces (all of a segment's states) go back before the
     # baselines allocate their own
+    del ws, tr0
     eng.trim()
     torch.cuda.empty_cache()
     baseline = None
