@@ -842,6 +842,146 @@ def di_main(args):
     print(json.dumps(line), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# Observed-data file (HDF5) of the damage inversion at the reference's size:
+# DF main (301^2 nodes over 0.4 m, 4001 samples over 120 us, four damage
+# regions, frames every 5 steps) written by the FEM generator through
+# ernn.hdf5 and read back by the loader; beside it the same arrays written and
+# read through the HDF5 C library (the library h5py calls), where installed.
+# ---------------------------------------------------------------------------
+DF_REGIONS = [{"type": "circle", "center": (0.1, 0.1), "size": 0.02},
+              {"type": "square", "center": (-0.1, -0.1), "size": 0.02},
+              {"type": "square", "center": (-0.1, -0.06), "size": 0.02},
+              {"type": "square", "center": (-0.06, -0.1), "size": 0.02}]   # DF:1510-1545
+
+
+def _libhdf5():
+    import ctypes as C
+    for name in ("libhdf5.so", "libhdf5.so.103", "/opt/conda/lib/libhdf5.so.103"):
+        try:
+            lib = C.CDLL(name)
+        except OSError:
+            continue
+        hid = C.c_int64
+        for fn, res, argt in [("H5Fopen", hid, [C.c_char_p, C.c_uint, hid]), ("H5Fclose", C.c_int, [hid]),
+                              ("H5Fcreate", hid, [C.c_char_p, C.c_uint, hid, hid]),
+                              ("H5Dopen2", hid, [hid, C.c_char_p, hid]), ("H5Dclose", C.c_int, [hid]),
+                              ("H5Dread", C.c_int, [hid, hid, hid, hid, hid, C.c_void_p]),
+                              ("H5Dwrite", C.c_int, [hid, hid, hid, hid, hid, C.c_void_p]),
+                              ("H5Dcreate2", hid, [hid, C.c_char_p, hid, hid, hid, hid, hid]),
+                              ("H5Pcreate", hid, [hid]), ("H5Pclose", C.c_int, [hid]),
+                              ("H5Pset_chunk", C.c_int, [hid, C.c_int, C.POINTER(C.c_ulonglong)]),
+                              ("H5Pset_deflate", C.c_int, [hid, C.c_uint]),
+                              ("H5Screate_simple", hid, [C.c_int, C.POINTER(C.c_ulonglong), C.c_void_p]),
+                              ("H5Sclose", C.c_int, [hid])]:
+            f = getattr(lib, fn)
+            f.restype, f.argtypes = res, argt
+        lib.H5open()
+        return lib
+    return None
+
+
+def _lib_read(lib, path, shape):
+    import ctypes as C
+    import numpy as np
+    flt = C.c_int64.in_dll(lib, "H5T_NATIVE_FLOAT_g").value
+    f = lib.H5Fopen(path.encode(), 0, 0)
+    out = []
+    for k in ("ux", "uy"):
+        d = lib.H5Dopen2(f, k.encode(), 0)
+        a = np.empty(shape, np.float32)
+        if lib.H5Dread(d, flt, 0, 0, 0, a.ctypes.data) < 0:
+            raise RuntimeError(f"H5Dread {k}")
+        lib.H5Dclose(d)
+        out.append(a)
+    lib.H5Fclose(f)
+    return out
+
+
+def _lib_write(lib, path, arrays):
+    import ctypes as C
+    flt = C.c_int64.in_dll(lib, "H5T_NATIVE_FLOAT_g").value
+    dcpl_cls = C.c_int64.in_dll(lib, "H5P_CLS_DATASET_CREATE_ID_g").value
+    f = lib.H5Fcreate(path.encode(), 2, 0, 0)
+    for k, a in arrays.items():
+        dims = (C.c_ulonglong * 3)(*a.shape)
+        sp = lib.H5Screate_simple(3, dims, None)
+        dcpl = lib.H5Pcreate(dcpl_cls)
+        lib.H5Pset_chunk(dcpl, 3, (C.c_ulonglong * 3)(1, a.shape[1], a.shape[2]))
+        lib.H5Pset_deflate(dcpl, 4)
+        d = lib.H5Dcreate2(f, k.encode(), flt, sp, 0, dcpl, 0)
+        if lib.H5Dwrite(d, flt, 0, 0, 0, a.ctypes.data) < 0:
+            raise RuntimeError(f"H5Dwrite {k}")
+        lib.H5Dclose(d); lib.H5Pclose(dcpl); lib.H5Sclose(sp)
+    lib.H5Fclose(f)
+
+
+def hdf5_main(args):
+    import tempfile
+    import numpy as np
+    import torch
+    from ernn import data, fem, hdf5
+    dev = torch.device("cuda:0")
+    threads = min(os.cpu_count() or 1, 16)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "anisotropic_ricker_damage_data_ds5_FEM_fixed_bc.mat")
+        t0 = time.perf_counter()
+        path = fem.generate_reference_data(path, damage_configs=DF_REGIONS, stride=5, nx=301, ny=301, Lx=0.4, Ly=0.4,
+                                           nt=args.nt or 4001, t_total=120e-6, source_amplitude=1e10, source_f0=100e3,
+                                           source_t0=10e-6, gaussian_param=5e6, rho=1610.0, device=dev,
+                                           C11=130.74e9, C22=11.50e9, C12=10.49e9, C16=30.67e9, C26=3.75e9,
+                                           C66=14.77e9)
+        t_gen = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        with hdf5.File(path) as f:
+            shape = f["ux"].shape
+        payload = 2 * 4 * int(np.prod(shape))
+        log(f"hdf5: FEM run + file ({shape[0]} frames of {shape[1]}x{shape[2]}, {size / 1e6:.1f} MB on disk, "
+            f"{payload / 1e6:.1f} MB of float32) in {t_gen:.2f} s")
+
+        def ours_read():
+            with hdf5.File(path) as f:
+                return f["ux"][:], f["uy"][:]
+
+        def timed(fn, k):
+            fn()
+            t = time.perf_counter()
+            for _ in range(k):
+                r = fn()
+            return (time.perf_counter() - t) / k, r
+
+        k = max(1, args.steps)
+        t_read, (ux, uy) = timed(ours_read, k)
+        t_load, _ = timed(lambda: data.load_fem_damage_data(path, device=dev), k)
+        wpath = os.path.join(d, "w.mat")
+        t_write, _ = timed(lambda: hdf5.write(wpath, {"ux": ux, "uy": uy}, chunked={"ux": (1,) + ux.shape[1:],
+                                                                                    "uy": (1,) + uy.shape[1:]}), k)
+        log(f"hdf5: read {t_read * 1e3:.1f} ms ({payload / t_read / 1e9:.2f} GB/s, {threads} threads), loader to "
+            f"the GPU {t_load * 1e3:.1f} ms, write {t_write * 1e3:.1f} ms ({payload / t_write / 1e9:.2f} GB/s)")
+        lib = _libhdf5()
+        library = None
+        if lib is not None:
+            t_lread, (lx, ly) = timed(lambda: _lib_read(lib, path, shape), k)
+            assert np.array_equal(lx, ux) and np.array_equal(ly, uy)
+            lpath = os.path.join(d, "lib.mat")
+            t_lwrite, _ = timed(lambda: _lib_write(lib, lpath, {"ux": ux, "uy": uy}), k)
+            library = {"read_ms": round(t_lread * 1e3, 2), "read_GBps": round(payload / t_lread / 1e9, 3),
+                       "write_ms": round(t_lwrite * 1e3, 2), "write_GBps": round(payload / t_lwrite / 1e9, 3),
+                       "cores": 1, "what": "HDF5 C library (the calls h5py makes), same file / arrays"}
+            log(f"hdf5: HDF5 C library read {t_lread * 1e3:.1f} ms ({payload / t_lread / 1e9:.2f} GB/s), write "
+                f"{t_lwrite * 1e3:.1f} ms")
+    line = {
+        "metric": "observed-data HDF5 read, GB/s of float32 payload", "value": round(payload / t_read / 1e9, 3),
+        "unit": "GB/s", "n_gpus": 1, "steps": k, "warmup": 1, "ms_per_step": round(t_read * 1e3, 2),
+        "higher_is_better": True, "scaling": "none", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "hdf5-df", "frames": shape[0], "grid": f"{shape[1]}x{shape[2]}", "gzip": 4,
+                   "chunk": [1, shape[1], shape[2]], "file_MB": round(size / 1e6, 1), "threads": threads},
+        "write_ms": round(t_write * 1e3, 2), "loader_to_gpu_ms": round(t_load * 1e3, 2),
+        "roofline": None, "cpu_baseline": library,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -854,6 +994,8 @@ def main():
         return tv_adam_main(args)
     if args.workload == "di-301":
         return di_main(args)
+    if args.workload == "hdf5-df":
+        return hdf5_main(args)
     import numpy as np
     import torch
     import ernn

@@ -8,12 +8,14 @@ save_reference_data    writes the same layout (the reference's writers,
                        it as HDF5)
 
 The reference keeps its data in HDF5 files: datasets ux, uy (nt_ds, nx, ny)
-float32, x (nx,), y (ny,), t (nt_ds,), optional damage_mask (nx, ny) uint8,
-groups source_info and damage_info of scalars, and the primary time stride in
-the file name ("_ds5").  h5py is not a dependency here, so the same layout is
-read from and written to NumPy .npz archives (group members as
-"source_info/<key>" entries, loaded with allow_pickle=False).  HDF5 files
-(.h5 / .mat) are read through h5py where it is installed.
+float32 in chunks of one frame with gzip 4, x (nx,), y (ny,), t (nt_ds,),
+optional damage_mask (nx, ny) uint8, groups source_info and damage_info of
+scalars (damage_info with region_<i> subgroups), and the primary time stride
+in the file name ("_ds5").  HDF5 files (.mat, .h5, .hdf5) are read through
+h5py where it is installed and through this package's own reader
+(ernn.hdf5) where it is not, and written in that layout by ernn.hdf5.write.
+The same layout also goes to NumPy .npz archives (group members as
+"source_info/<key>" entries, loaded with allow_pickle=False).
 """
 from __future__ import annotations
 
@@ -49,18 +51,24 @@ def _read(path: str) -> Dict[str, object]:
         return out
     try:
         import h5py
-    except ImportError as e:   # the layout is HDF5 in the reference; h5py is optional here
-        raise ImportError(f"{path}: reading HDF5 needs h5py, which is not installed; convert the file to .npz "
-                          "with save_reference_data (or np.savez with the same keys) where h5py is available") from e
+        opener = lambda p: h5py.File(p, "r")
+        is_group = lambda o: isinstance(o, h5py.Group)
+    except ImportError:   # the reference reads with h5py; without it, this package's reader
+        from . import hdf5
+        opener = hdf5.File
+        is_group = lambda o: isinstance(o, hdf5.Group)
     out = {}
-    with h5py.File(path, "r") as f:
+    with opener(path) as f:
         for k in ("ux", "uy", "x", "y", "t", "damage_mask"):
             if k in f:
                 out[k] = f[k][:]
-        for g in _GROUPS:
+        for g in _GROUPS:   # the group's datasets, as DI:631-645 reads them (region_<i> subgroups aside)
             if g in f:
-                for k in f[g].keys():
-                    out[f"{g}/{k}"] = f[g][k][()]
+                grp = f[g]
+                for k in grp.keys():
+                    obj = grp[k]
+                    if not is_group(obj):
+                        out[f"{g}/{k}"] = obj[()]
     return out
 
 
@@ -110,13 +118,19 @@ def load_fem_damage_data(path: str, device=None):
     return ux, uy, meta
 
 
+HDF5_EXTENSIONS = (".mat", ".h5", ".hdf5")
+
+
 def save_reference_data(path: str, ux, uy, x, y, t, source_info: Optional[Dict[str, object]] = None,
                         damage_mask=None, damage_info: Optional[Dict[str, object]] = None) -> str:
     """Write ux, uy (nt, nx, ny) float32 and the coordinate / metadata entries
-    of the reference layout to an .npz archive (compressed); returns the path.
-    Put the primary stride in the name ("_ds5") as the reference does."""
-    if not path.endswith(".npz"):
-        raise ValueError("save_reference_data writes .npz archives")
+    of the reference layout; returns the path.  .mat / .h5 / .hdf5: an HDF5
+    file as the reference's writers make it (PF:908-945, DF:1627-1684: ux, uy
+    in chunks of one frame with gzip 4; damage_info keys "region_<i>/<name>"
+    become region subgroups); .npz: a compressed NumPy archive.  Put the
+    primary stride in the name ("_ds5") as the reference does."""
+    if not path.endswith(".npz") and not path.endswith(HDF5_EXTENSIONS):
+        raise ValueError(f"save_reference_data writes .npz archives or HDF5 files ({', '.join(HDF5_EXTENSIONS)})")
     arrays = {"ux": np.asarray(ux, np.float32), "uy": np.asarray(uy, np.float32), "x": np.asarray(x),
               "y": np.asarray(y), "t": np.asarray(t)}
     if arrays["ux"].ndim != 3 or arrays["ux"].shape != arrays["uy"].shape:
@@ -129,5 +143,18 @@ def save_reference_data(path: str, ux, uy, x, y, t, source_info: Optional[Dict[s
             if a.dtype == object:
                 raise ValueError(f"{g}/{k}: only numeric or string values are stored")
             arrays[f"{g}/{k}"] = a
-    np.savez_compressed(path, **arrays)
+    if path.endswith(".npz"):
+        np.savez_compressed(path, **arrays)
+        return path
+    from . import hdf5
+    tree: Dict[str, object] = {}
+    for key, a in arrays.items():
+        node = tree
+        parts = key.split("/")
+        for part in parts[:-1]:
+            node = node.setdefault(part, {})
+        node[parts[-1]] = a.item().decode("utf-8") if a.dtype.kind == "S" and a.ndim == 0 else \
+            (str(a) if a.dtype.kind == "U" and a.ndim == 0 else a)
+    nt, nx, ny = arrays["ux"].shape
+    hdf5.write(path, tree, chunked={"ux": (1, nx, ny), "uy": (1, nx, ny)}, level=4)
     return path

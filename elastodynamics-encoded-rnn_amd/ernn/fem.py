@@ -16,7 +16,7 @@ libernn_hip.so (ernn_fem_* in include/ernn.h).  There is no PyTorch fallback.
   apply_damage_to_stiffness             DF:183-212
   solve_2d_anisotropic_wave_fem_ricker_fixed         PF:196-590
   solve_2d_anisotropic_wave_fem_ricker_fixed_damage  DF:770-1224
-  generate_reference_data               PF:823-956, DF:1486-1700 (file written as .npz)
+  generate_reference_data               PF:823-956, DF:1486-1700 (HDF5 as the reference writes it, or .npz)
 """
 from __future__ import annotations
 
@@ -357,9 +357,11 @@ def solve_2d_anisotropic_wave_fem_ricker_fixed_damage(nx=301, ny=301, Lx=0.4, Ly
 def generate_reference_data(path: str, damage_configs=None, stride: int = 5, **solver_kwargs) -> str:
     """Run the solver and write the observed-data file of PF:900-945 /
     DF:1619-1684 (ux, uy as (nt_ds, nx, ny) float32, x, y, t[::stride],
-    source_info, and for damage runs damage_mask and damage_info) as an .npz
-    archive read by data.load_reference_data / load_fem_damage_data.  The
-    stride goes into the name as "_ds<stride>" when the name lacks it."""
+    source_info, and for damage runs damage_mask and damage_info with its
+    region_<i> subgroups): an HDF5 file in the reference's layout for a .mat /
+    .h5 name, an .npz archive for an .npz name; data.load_reference_data /
+    load_fem_damage_data read both.  The stride goes into the name as
+    "_ds<stride>" when the name lacks it."""
     damaged = damage_configs is not None
     solver = solve_2d_anisotropic_wave_fem_ricker_fixed_damage if damaged else solve_2d_anisotropic_wave_fem_ricker_fixed
     kw = dict(solver_kwargs, record_stride=stride)

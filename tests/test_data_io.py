@@ -61,14 +61,43 @@ def test_bad_inputs(tmp_path):
     with pytest.raises(ValueError):
         data.save_reference_data(str(tmp_path / "x.npz"), ux, uy[:, :4], 0, 0, 0)
     with pytest.raises(ValueError):
-        data.save_reference_data(str(tmp_path / "x.h5"), ux, uy, 0, 0, 0)
+        data.save_reference_data(str(tmp_path / "x.txt"), ux, uy, 0, 0, 0)
     with pytest.raises(FileNotFoundError):
         data.load_reference_data(str(tmp_path / "missing_ds5.npz"))
 
 
 @pytest.mark.skipif(importlib.util.find_spec("h5py") is not None, reason="h5py is installed")
-def test_hdf5_without_h5py_names_the_conversion(tmp_path):
+def test_truncated_hdf5_is_an_error_without_h5py(tmp_path):
+    """Without h5py the package's own reader takes HDF5 files; a file that is
+    only a signature is refused with its error."""
+    from ernn import hdf5
     p = tmp_path / "anisotropic_ricker_data_ds5_FEM_fixed_bc.mat"
     p.write_bytes(b"\x89HDF\r\n\x1a\n")
-    with pytest.raises(ImportError, match="h5py"):
+    with pytest.raises((hdf5.HDF5Error, IndexError, ValueError)):
         data.load_reference_data(str(p))
+    q = tmp_path / "not_hdf5_ds5.mat"
+    q.write_bytes(b"plain text, no signature" * 40)
+    with pytest.raises(hdf5.HDF5Error, match="signature"):
+        data.load_reference_data(str(q))
+
+
+def test_reference_layout_round_trip_through_hdf5(tmp_path):
+    """save_reference_data to an HDF5 name writes the reference's layout (ux,
+    uy in gzip frames, region subgroups) and the loaders read it back."""
+    ux, uy = _fields(nt=9, nx=6, ny=5, seed=4)
+    x, y, t = np.linspace(0, 1, 6, dtype=np.float32), np.linspace(0, 2, 5, dtype=np.float32), np.arange(9, dtype=np.float32)
+    mask = (np.arange(30).reshape(6, 5) % 4 == 0)
+    p = data.save_reference_data(str(tmp_path / "anisotropic_ricker_damage_data_ds5_FEM_fixed_bc.mat"), ux, uy, x, y, t,
+                                 source_info={"source_f0": 100e3, "component": "y"}, damage_mask=mask,
+                                 damage_info={"total_damaged_points": 8, "damage_percentage": 26.7,
+                                              "region_0/type": "ellipse", "region_0/center": np.array([0.1, 0.2])})
+    gx, gy, meta = data.load_fem_damage_data(p)
+    assert np.array_equal(gx[0].numpy(), ux) and np.array_equal(gy[0].numpy(), uy)
+    assert meta["primary_stride"] == 5 and np.array_equal(meta["t_primary"], t)
+    assert np.array_equal(meta["damage_mask"], mask.astype(np.uint8))
+    assert meta["damage_info"]["total_damaged_points"] == 8
+    assert meta["source_info"]["source_f0"] == 100e3 and meta["source_info"]["component"] == b"y"
+    from ernn import hdf5
+    with hdf5.File(p) as f:
+        assert f["damage_info/region_0/type"][()] == b"ellipse"
+        assert np.array_equal(f["damage_info"]["region_0"]["center"][:], [0.1, 0.2])

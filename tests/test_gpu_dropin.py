@@ -30,7 +30,8 @@ LR = 0.04
 
 def test_parameter_inversion_main_matches_plain_path(tmp_path):
     # observed data: the FEM generator's file (PF:823-945) at the reference's size
-    path = fem.generate_reference_data(str(tmp_path / "anisotropic_ricker_data_FEM_fixed_bc.npz"), stride=5,
+    # the reference's own file name and format (PF:905-945): HDF5, ux / uy in gzip frames
+    path = fem.generate_reference_data(str(tmp_path / "anisotropic_ricker_data_ds5_FEM_fixed_bc.mat"), stride=5,
                                        nx=301, ny=301, Lx=0.4, Ly=0.4, nt=2001, t_total=60e-6,
                                        source_amplitude=1e10, source_f0=100e3, source_t0=10e-6,
                                        gaussian_param=5e6, rho=1610.0, device=DEV, **PF_C)

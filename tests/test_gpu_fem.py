@@ -204,9 +204,10 @@ def test_fem_solver_api_and_reference_file_round_trip(tmp_path):
     assert tuple(fx.shape) == (1, 40, 41, 37) and meta["primary_stride"] == 5
     np.testing.assert_array_equal(fx[0].cpu().numpy(), np.moveaxis(ux[:, :, ::5], -1, 0))
     assert float(meta["source_info"]["C16"]) == 30.67e9
-    dpath = fem.generate_reference_data(str(tmp_path / "damage_FEM_ds5.npz"),
+    dpath = fem.generate_reference_data(str(tmp_path / "anisotropic_ricker_damage_data_ds5_FEM_fixed_bc.mat"),
                                         damage_configs=[{"type": "square", "center": (0.01, 0.0), "size": 0.005}],
                                         stride=5, **kw)
+    assert dpath.endswith("_ds5_FEM_fixed_bc.mat")   # HDF5, as DF:1622-1684 writes it
     dx_, dy_, dmeta = data.load_fem_damage_data(dpath, device=DEV)
     assert dmeta["damage_mask"].sum() > 0 and int(dmeta["damage_info"]["total_damaged_points"]) > 0
     assert rel_l2(dy_.cpu().numpy(), fy.cpu().numpy()) > 1e-4
