@@ -6,7 +6,8 @@ shots, MSE on receiver traces, gradient (checkpointed recompute + reverse-time
 adjoint), Adam update.  Default workload: damage-1024 (1024^2 grid, 3000
 steps, 8 top-edge shots, 512 receivers on the top and bottom rows, damage
 field D(x) of three ellipses scaling all six constants; the gradient covers
-the six constants and the 1 Mi-cell D field), FP32, synthetic data (observed
+the six constants and the 1 Mi-cell D field; checkpoint interval chosen under
+a 32 GB cap on the checkpointed states), FP32, synthetic data (observed
 traces = the same solver at the true constants and D + 1 % RMS seeded noise).
 
 Also reported on the same line:
@@ -28,6 +29,9 @@ Also reported on the same line:
                      1; `traffic` is the measured HBM bytes per launch.
   cpu_baseline       the op-for-op CPU port (oracle/plain_torch.py, autograd) on
                      the host cores over a bounded window of steps.
+  hbm_budget_run     workloads with a checkpoint cap: the same iteration with the
+                     interval that 80 % of the free HBM gives (fewer recomputed
+                     steps); `value` stays the capped figure.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -97,6 +101,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--baseline-window", type=int, default=2000,
                     help="steps of one shot timed for the autograd baseline (scaled to the survey)")
+    ap.add_argument("--hbm-run", type=int, default=1,
+                    help="workloads with a checkpoint memory cap: also time the interval that 80 %% of the free "
+                         "HBM gives (reported beside the capped value)")
     ap.add_argument("--per-instance", type=int, default=1,
                     help="batched surveys: time every instance alone for p50 / max")
     ap.add_argument("--fem-n", type=int, default=301,
@@ -982,6 +989,52 @@ def hdf5_main(args):
     print(json.dumps(line), flush=True)
 
 
+def time_hbm_budget(torch, ernn, sv, eng, C_init, D_init, obs, dev, args):
+    """The same Adam iteration with the checkpoint interval that 80 % of the
+    free HBM gives instead of the workload's cap (same tiles and blocking).
+    The capped engine's workspaces are released first."""
+    import dataclasses
+    from ernn.autotune import state_bytes
+    eng.trim()
+    torch.cuda.empty_cache()
+    sv2 = dataclasses.replace(sv, ckpt_budget=None, fuse=bool(eng.info.fuse))
+    model = ernn.ERNN(sv2, C_init, D_init=D_init, device=dev)
+    e2 = model.engine
+    t = eng.tiles
+    e2.set_tiling(0, t[0], t[1], t[4])
+    e2.set_tiling(1, t[2], t[3], t[5])
+    e2.set_blocking(*eng.blocking)
+    e2.set_blocking(*eng.adj_blocking, which=1)
+    if e2.info.fuse and eng.ff_rw:
+        v, _, ks = eng.adj_blocking
+        e2.set_blocking(v, eng.ff_rw, ks, which=2)
+    opt = torch.optim.Adam(model.parameters(), lr=1e-2)
+
+    def it():
+        opt.zero_grad()
+        loss = torch.mean((model() - obs) ** 2)
+        loss.backward()
+        opt.step()
+
+    for _ in range(max(1, args.warmup)):
+        it()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        it()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    K = int(e2.info.ckpt_every)
+    out = {"budget": "80 % of free HBM", "ckpt_every": K, "ckpt_state_bytes": state_bytes(sv2, K),
+           "fused_backward": bool(e2.info.fuse), "s_per_adam_iter": round(dt, 5),
+           "value": round(sv.cell_steps() / dt / 1e9, 3), "unit": "Gcell-steps/s"}
+    log(f"whole-HBM checkpoint budget: K = {K}, {dt * 1e3:.2f} ms/Adam-iter, {out['value']} Gcell-steps/s")
+    e2.close()
+    del model, opt
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -1000,7 +1053,7 @@ def main():
     import torch
     import ernn
     from ernn import configs
-    from ernn.autotune import tune_backward, tune_blocking, tune_fused_recompute, tune_tiles
+    from ernn.autotune import state_bytes, tune_backward, tune_blocking, tune_fused_recompute, tune_tiles
 
     dev = torch.device("cuda:0")
     kw = {"nt": args.nt} if args.nt else {}
@@ -1214,6 +1267,10 @@ def main():
         cpu["fp64_oracle"] = cpu_oracle(wl, args.cpu_seconds / 3)
         log(f"cpu FP64 oracle: {cpu['fp64_oracle']['value']} Gcell-steps/s on one core")
 
+    hbm = None
+    if args.hbm_run and sv.ckpt_budget:
+        hbm = time_hbm_budget(torch, ernn, sv, eng, C_init, D_init, obs, dev, args)
+
     s0 = sv.shots[0]
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Gcell-steps/s", "n_gpus": 1,
@@ -1222,7 +1279,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
                    "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
-                   "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every), "graph": bool(sv.use_graph),
+                   "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every),
+                   "ckpt_cap_bytes": sv.ckpt_budget, "ckpt_state_bytes": state_bytes(sv, int(eng.info.ckpt_every)),
+                   "graph": bool(sv.use_graph),
                    "fused_backward": bool(eng.info.fuse),
                    "blocking": list(eng.blocking),
                    "adj_blocking": list(eng.adj_blocking), "fused_recompute_rw": eng.ff_rw,
@@ -1232,6 +1291,7 @@ def main():
         "autograd_baseline": baseline,
         "speedup_vs_autograd": (round(value / baseline["value"], 2) if baseline else None),
         "per_instance_alone": per_instance,
+        "hbm_budget_run": hbm,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -1448,7 +1448,29 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     }
   }
 
-  // ---- the launch's last level, on the output tile
+  // ---- the launch's last level, on the output tile.  The damage mode's FP64
+  // per-cell accumulators are read first, every row at once, so their update
+  // costs one memory round trip: read after the state stores, or row by row,
+  // each read would wait for every store and read issued before it.
+  asm volatile("" ::: "memory");   // the reads stay after the loop (registers)
+  gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
+  double acc_in[kD ? RW : 1][V];
+  if constexpr (kD) {
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+#pragma unroll
+      for (int e = 0; e < V; ++e) acc_in[r][e] = 0.0;
+    if (bd.col_own) {
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        if (r < bd.own_lo || r >= bd.own_hi) continue;
+        const uint32_t so = bd.srow(I, r) / (uint32_t)sizeof(float);
+#pragma unroll
+        for (int e = 0; e < V; ++e)
+          if (bd.col_in[e]) acc_in[r][e] = acc64[so + bd.vcol[e] / (uint32_t)sizeof(float)];
+      }
+    }
+  }
   if (bd.col_own) {
     Rsrc ro[kStateFields];
 #pragma unroll
@@ -1480,9 +1502,6 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     if ((lane & 7) == 0) *part = part_in + wq * tot;
   }
   if constexpr (kD) {
-    // FP64 per-cell accumulator, read here (after the loop, as the asm keeps it)
-    asm volatile("" ::: "memory");
-    gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
     if (bd.col_own) {
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
@@ -1490,10 +1509,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
         const uint32_t so = bd.srow(I, r) / (uint32_t)sizeof(float);
 #pragma unroll
         for (int e = 0; e < V; ++e)
-          if (bd.col_in[e]) {
-            gdouble* pa = acc64 + so + bd.vcol[e] / (uint32_t)sizeof(float);
-            *pa = *pa + gsa * (double)gacc[r][e];
-          }
+          if (bd.col_in[e]) acc64[so + bd.vcol[e] / (uint32_t)sizeof(float)] = acc_in[r][e] + gsa * (double)gacc[r][e];
       }
     }
   }
@@ -1703,12 +1719,26 @@ __device__ __forceinline__ void adj_block_body6(KArg<BlockAdjArgs>* ap, int bid)
     }
   }
 
-  // ---- the launch's last level and the per-cell gradients, on the output tile
+  // ---- the launch's last level and the per-cell gradients, on the output
+  // tile.  Every FP64 accumulator of the tile is read before the first store,
+  // so the update costs one memory round trip (a read behind the stores would
+  // wait for all of them).
+  asm volatile("" ::: "memory");   // the reads stay after the loop (registers)
   if (bd.col_own && bd.col_in[0]) {
+    gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
+    double acc_in[6][RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc_in[q][r] = 0.0;
+      if (r < bd.own_lo || r >= bd.own_hi) continue;
+      const uint32_t cell = bd.srow(I, r) / (uint32_t)sizeof(float) + bd.vcol[0] / (uint32_t)sizeof(float);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc_in[q][r] = acc64[(int64_t)q * I.field + cell];
+    }
     Rsrc ro[kStateFields];
 #pragma unroll
     for (int q = 0; q < kStateFields; ++q) ro[q] = field_rsrc(state_field(C, I, a.s_out, q), I.field);
-    gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
       if (r < bd.own_lo || r >= bd.own_hi) continue;
@@ -1719,12 +1749,13 @@ __device__ __forceinline__ void adj_block_body6(KArg<BlockAdjArgs>* ap, int bid)
         bstore(c[q][r], ro[2 + q], bd.vcol[0], so);
         bstore(wv[q][r], ro[4 + q], bd.vcol[0], so);
       }
-      const uint32_t cell = so / (uint32_t)sizeof(float) + bd.vcol[0] / (uint32_t)sizeof(float);
+    }
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        gdouble* pa = acc64 + (int64_t)q * I.field + cell;
-        *pa = *pa + gsa * (double)gacc[q][r];
-      }
+    for (int r = 0; r < RW; ++r) {
+      if (r < bd.own_lo || r >= bd.own_hi) continue;
+      const uint32_t cell = bd.srow(I, r) / (uint32_t)sizeof(float) + bd.vcol[0] / (uint32_t)sizeof(float);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc64[(int64_t)q * I.field + cell] = acc_in[q][r] + gsa * (double)gacc[q][r];
     }
   }
 }

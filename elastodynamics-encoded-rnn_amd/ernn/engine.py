@@ -40,7 +40,9 @@ class Engine:
             if budget is None:
                 free, _ = torch.cuda.mem_get_info(self.device)
                 budget = int(0.8 * free)
-            K = choose_ckpt(survey, budget_bytes=budget)
+                K = choose_ckpt(survey, budget_bytes=budget)
+            else:   # a stated budget is a cap: MemoryError if even the least-memory interval exceeds it
+                K = choose_ckpt(survey, mem_cap_bytes=budget or None, budget_bytes=budget)
         # The fused backward overlaps each segment's adjoint with the next
         # segment's recompute; with one segment (every state kept) there is no
         # recompute to overlap, so the automatic choice is the unfused one.

@@ -181,8 +181,20 @@ def test_checkpoint_interval_fills_the_memory_budget():
 
 def test_damage_1024_fits_the_32_gb_cap():
     wl = configs.damage_1024()
-    K = autotune.choose_ckpt(wl.survey, mem_cap_bytes=32 << 30)
-    assert autotune.state_bytes(wl.survey, K) < 32 << 30
+    sv = wl.survey
+    assert sv.ckpt_budget == configs.DAMAGE_MEM_CAP == 32 * 10**9
+    # the interval the engine takes for the workload (a stated budget is a cap)
+    K = autotune.choose_ckpt(sv, mem_cap_bytes=sv.ckpt_budget, budget_bytes=sv.ckpt_budget)
+    assert autotune.state_bytes(sv, K) <= 32 * 10**9
+    assert K == 375 and autotune.recomputed_steps(sv.nt_max, K) == 2625   # 8 segments of 375 steps
+    # nothing larger fits, and no K that fits recomputes fewer steps
+    fit = [k for k in range(2, sv.nt_max + 1) if autotune.state_bytes(sv, k) <= 32 * 10**9]
+    assert min(autotune.recomputed_steps(sv.nt_max, k) for k in fit) == 2625
+    # the least-memory interval is well under the cap; a cap below it is refused
+    k_min = autotune.choose_ckpt(sv)
+    assert autotune.state_bytes(sv, k_min) < 16 * 10**9
+    with pytest.raises(MemoryError):
+        autotune.choose_ckpt(sv, mem_cap_bytes=autotune.state_bytes(sv, k_min) - 1)
 
 
 def test_padded_layout_matches_library_convention():
