@@ -24,7 +24,10 @@ constexpr int kLeftPad = 32;
 constexpr int kColAlign = 256;
 constexpr int kRowAlign = 16;
 
-enum ParamMode : int { kConst6 = 0, kDField = 1, kField6 = 2 };
+// Kernel modes: the ABI's parameter modes, plus kDOnly for a damage field
+// over known constants (fixed_c0): the bank then holds (dt^2/rho) A(C0) u^m
+// instead of u^m and the adjoint forms dJ/dD only.
+enum ParamMode : int { kConst6 = 0, kDField = 1, kField6 = 2, kDOnly = 3 };
 
 // Recurrence state.  Both recurrences, u^{n+1} = 2 u^n - u^{n-1} + d^n
 // (PI:286-291) and the reverse-time adjoint, are carried as (u, c, v) per
@@ -34,7 +37,8 @@ enum ParamMode : int { kConst6 = 0, kDField = 1, kField6 = 2 };
 // so the FP32 rounding of the state does not accumulate over thousands of
 // steps.  A state slot holds the six padded fields u_x, u_y, c_x, c_y, v_x,
 // v_y of one time level; a bank slot holds u_x, u_y only (the recomputed
-// forward states the adjoint reads for its gradient terms).
+// forward states the adjoint reads for its gradient terms), or with kDOnly
+// the two components of (dt^2/rho) A(C0) u.
 constexpr int kStateFields = 6;
 constexpr int kBankFields = 2;
 

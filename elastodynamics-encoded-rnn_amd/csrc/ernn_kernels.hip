@@ -177,7 +177,7 @@ __device__ __forceinline__ void fwd_op(const OpCoef& kf, double gsa, const float
                                        float& ox, float& oy) {
   if constexpr (MODE == kConst6) {
     apply_op_sum(kf, d, ox, oy);
-  } else if constexpr (MODE == kDField) {
+  } else if constexpr (MODE == kDField || MODE == kDOnly) {
     apply_op_sum(kf, d, ox, oy);
     ox = cq[0] * ox;
     oy = cq[0] * oy;
@@ -306,7 +306,7 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
   const int k = n / a.stride;
   const int nf = I.n_frames;
 
-  constexpr int NC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
+  constexpr int NC = (MODE == kDField || MODE == kDOnly) ? 1 : (MODE == kField6 ? 6 : 0);
   RawRow<V> xraw[R + 2], yraw[R + 2];
   float cx[R][V], cy[R][V], vx[R][V], vy[R][V];
   float cf[R][NC > 0 ? NC : 1][V];
@@ -349,13 +349,20 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
     const int i = tl.i0 + r;
     const bool live = i < I.nx;
     float dx[V], dy[V];
+    float lbx[V], lby[V];   // kDOnly: (dt^2/rho) A(C0) u^n, the bank's value for level n
 #pragma unroll
     for (int e = 0; e < V; ++e) {
       const UDeriv d = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1);
       float cq[NC > 0 ? NC : 1];
 #pragma unroll
       for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
-      fwd_op<MODE>(kf, gsa, cq, d, dx[e], dy[e]);
+      if constexpr (MODE == kDOnly) {   // the arithmetic of fwd_op<kDField>, keeping the operator value
+        apply_op_sum(kf, d, lbx[e], lby[e]);
+        dx[e] = cq[0] * lbx[e];
+        dy[e] = cq[0] * lby[e];
+      } else {
+        fwd_op<MODE>(kf, gsa, cq, d, dx[e], dy[e]);
+      }
     }
 
     // point forces on this row: (dt^2/rho) s_n w (PI:103-104, PI:270-277)
@@ -385,6 +392,7 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
       if (!live || j0 + lane * V + e >= I.ny) {   // keep the ghosts zero
         nx_[e] = ny_[e] = 0.f;
         cx[r][e] = cy[r][e] = vx[r][e] = vy[r][e] = 0.f;
+        if constexpr (MODE == kDOnly) lbx[e] = lby[e] = 0.f;
       }
     }
     const uint32_t so = row_off(P, i, j0) + lo;
@@ -394,9 +402,14 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
     store_vec<V>(at(ocy, so), cy[r]);
     store_vec<V>(at(ovx, so), vx[r]);
     store_vec<V>(at(ovy, so), vy[r]);
-    if (a.bank >= 0) {
-      store_vec<V>(at(bx, so), nx_);
-      store_vec<V>(at(by, so), ny_);
+    if (a.bank >= 0) {   // u^{n+1}; kDOnly: (dt^2/rho) A(C0) u^n
+      if constexpr (MODE == kDOnly) {
+        store_vec<V>(at(bx, so), lbx);
+        store_vec<V>(at(by, so), lby);
+      } else {
+        store_vec<V>(at(bx, so), nx_);
+        store_vec<V>(at(by, so), ny_);
+      }
     }
     if (rec_step && live) {
       if (a.snapshot) {
@@ -475,7 +488,7 @@ struct AdjRows {
 // the weighted fields need (kDField: D, kField6: the six fields).
 template <int MODE, int V>
 struct AdjRaw {
-  static constexpr int NCR = (MODE == kField6) ? 6 : (MODE == kDField ? 1 : 0);
+  static constexpr int NCR = (MODE == kField6) ? 6 : ((MODE == kDField || MODE == kDOnly) ? 1 : 0);
   RawRow<V> lx, ly;
   RawRow<V> cr[NCR > 0 ? NCR : 1];
 };
@@ -515,7 +528,7 @@ __device__ __forceinline__ AdjRows<MODE, V> make_adj_row(const AdjRaw<MODE, V>& 
   if constexpr (MODE == kConst6) {
     o.w[0] = lx;
     o.w[1] = ly;
-  } else if constexpr (MODE == kDField) {
+  } else if constexpr (MODE == kDField || MODE == kDOnly) {
     const Row<V> D = make_row<V>(raw.cr[0], lane);
     o.w[0] = mul_row<V>(D, lx);
     o.w[1] = mul_row<V>(D, ly);
@@ -553,7 +566,7 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   const gfloat* cyi = state_field(C, I, a.s_in, 3);
   const gfloat* wxi = state_field(C, I, a.s_in, 4);
   const gfloat* wyi = state_field(C, I, a.s_in, 5);
-  const gfloat* umx = bank_field(C, I, a.bank, 0);    // u^m
+  const gfloat* umx = bank_field(C, I, a.bank, 0);    // u^m (kDOnly: (dt^2/rho) A(C0) u^m)
   const gfloat* umy = bank_field(C, I, a.bank, 1);
   const gfloat* coef = C.ws + I.coef_off;
 
@@ -572,8 +585,13 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   const int k = (m - 1) / a.stride;
   const int nf = I.n_frames;
 
-  constexpr int NACC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
+  constexpr int NACC = (MODE == kDField || MODE == kDOnly) ? 1 : (MODE == kField6 ? 6 : 0);
+  constexpr bool kPart = MODE == kConst6 || MODE == kDField;   // constant-gradient partials
   const double gsa = I.gscale * (double)I.hinv2;   // per-cell sums carry 1 / a
+  // kDOnly: the banked (dt^2/rho) A(C0) u^m carries the scale; the gradient
+  // term of the last level (lam^{nt+1} = 0) is skipped, its bank slot unwritten
+  const double gsc = (MODE == kDOnly) ? 1.0 : gsa;
+  const bool gterm = MODE != kDOnly || m < I.nt;
   gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
 
   // ---- loads (all issued before the first shuffle)
@@ -590,10 +608,11 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
     load_vec<V>(at(wxi, o), wx[r]);
     load_vec<V>(at(wyi, o), wy[r]);
   }
-  constexpr int RU = (MODE == kConst6) ? 1 : R + 2;   // u^m rows with halo (field modes)
+  constexpr bool kPointU = MODE == kConst6 || MODE == kDOnly;   // the bank read pointwise
+  constexpr int RU = kPointU ? 1 : R + 2;   // u^m rows with halo (field modes)
   float uvx[R][V], uvy[R][V];
   RawRow<V> uraw[RU][2];
-  if constexpr (MODE == kConst6) {
+  if constexpr (kPointU) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const uint32_t o = row_off(P, tl.i0 + r, j0) + lo;
@@ -622,14 +641,14 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
   }
   gdouble* part = reinterpret_cast<gdouble*>(C.ws) + a.partial_off + (int64_t)t * kPartials + (lane >> 3);
   double part_in = 0.0;
-  if constexpr (MODE != kField6) part_in = *part;
+  if constexpr (kPart) part_in = *part;
 
   // ---- shuffles
   AdjRows<MODE, V> lr[R + 2];
 #pragma unroll
   for (int r = 0; r < R + 2; ++r) lr[r] = make_adj_row<MODE, V>(lraw[r], lane);
   Row<V> um[RU][2];
-  if constexpr (MODE != kConst6) {
+  if constexpr (!kPointU) {
 #pragma unroll
     for (int r = 0; r < RU; ++r) {
       um[r][0] = make_row<V>(uraw[r][0], lane);
@@ -663,7 +682,12 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
 #pragma unroll
     for (int e = 0; e < V; ++e) {
       const int x = e + 1;
-      if constexpr (MODE == kConst6 || MODE == kDField) {
+      if constexpr (MODE == kDOnly) {
+        // A^T lam = A(C0) (D * lam); dJ/dD(cell) = lam . (dt^2/rho) A(C0) u^m from the bank
+        const UDeriv dl = uderivs<V>(Lm.w[0], Lc.w[0], Lp.w[0], Lm.w[1], Lc.w[1], Lp.w[1], x);
+        apply_op_sum(kl, dl, dx[e], dy[e]);
+        G[0][e] = gterm ? __builtin_fmaf(Lc.cx[e], uvx[r][e], Lc.cy[e] * uvy[r][e]) : 0.f;
+      } else if constexpr (MODE == kConst6 || MODE == kDField) {
         // A^T lam = A(C0) (D * lam): constant stencils on the (weighted) adjoint field
         const UDeriv dl = uderivs<V>(Lm.w[0], Lc.w[0], Lp.w[0], Lm.w[1], Lc.w[1], Lp.w[1], x);
         apply_op_sum(kl, dl, dx[e], dy[e]);
@@ -746,11 +770,11 @@ __device__ __forceinline__ void adj_tile(const StepArgs& a, const GCall& C, cons
       for (int q = 0; q < NACC; ++q)
 #pragma unroll
         for (int e = 0; e < V; ++e)
-          if (live && j0 + lane * V + e < I.ny) acc64[q * I.field + so + e] = acc_in[r][q][e] + gsa * (double)G[q][e];
+          if (live && j0 + lane * V + e < I.ny) acc64[q * I.field + so + e] = acc_in[r][q][e] + gsc * (double)G[q][e];
     }
   }
 
-  if constexpr (MODE != kField6) {
+  if constexpr (kPart) {
     // per-tile fp64 partials of the constant gradients (deterministic: one
     // wavefront owns slot t in every launch); lane 8q holds slot q
     double Sd[kPartials];
@@ -922,7 +946,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   const auto& a = *ap;
   constexpr int kCols = kWave * V;
   constexpr int WB = blk_waves(RW);                  // wavefronts
-  constexpr int NC = (MODE == kDField) ? 1 : (MODE == kField6 ? 6 : 0);
+  constexpr int NC = (MODE == kDField || MODE == kDOnly) ? 1 : (MODE == kField6 ? 6 : 0);
   static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
   __shared__ float xch[2][WB][2][2][kCols];          // [buffer][wavefront][top/bottom][comp][column]
 
@@ -1029,6 +1053,11 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
     }
 
     float dx[RW][V], dy[RW][V];
+    // kDOnly banks (dt^2/rho) A(C0) u^n of this step's level n on the output
+    // tile as it is formed (the bank slot of level n, or -1)
+    const int lb_slot = MODE == kDOnly ? a.bank_store[s] : -1;
+    const Rsrc lbx = field_rsrc(bank_field(C, I, max(lb_slot, 0), 0), I.field);
+    const Rsrc lby = field_rsrc(bank_field(C, I, max(lb_slot, 0), 1), I.field);
 #pragma unroll
     for (int r = 0; r < RW; ++r)
 #pragma unroll
@@ -1037,7 +1066,19 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
         float cq[NC > 0 ? NC : 1];
 #pragma unroll
         for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
-        fwd_op<MODE>(kf, gsa, cq, d, dx[r][e], dy[r][e]);
+        if constexpr (MODE == kDOnly) {   // the arithmetic of fwd_op<kDField>, keeping the operator value
+          float ox, oy;
+          apply_op_sum(kf, d, ox, oy);
+          if (lb_slot >= 0 && r >= bd.own_lo && r < bd.own_hi && bd.col_own && bd.col_in[e]) {
+            const uint32_t so = bd.srow(I, r);
+            bstore(ox, lbx, bd.vcol[e], so);
+            bstore(oy, lby, bd.vcol[e], so);
+          }
+          dx[r][e] = cq[0] * ox;
+          dy[r][e] = cq[0] * oy;
+        } else {
+          fwd_op<MODE>(kf, gsa, cq, d, dx[r][e], dy[r][e]);
+        }
       }
 
     // point forces anywhere in the region (halo cells feed the later steps)
@@ -1074,9 +1115,10 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
         advance(u[1][r][e], c[1][r][e], v[1][r][e], dy[r][e]);
       }
 
-    // store the output cells of the levels the schedule keeps
+    // store the output cells of the levels the schedule keeps (kDOnly: the
+    // bank took the operator value above)
     const int st_slot = a.st_store[s];
-    const int bk_slot = a.bank_store[s];
+    const int bk_slot = MODE == kDOnly ? -1 : a.bank_store[s];
     if (st_slot >= 0 || bk_slot >= 0) {
       if (bd.col_own) {
         if (st_slot >= 0) {
@@ -1190,7 +1232,8 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   const auto& a = *ap;
   constexpr int kCols = kWave * V;
   constexpr int WB = blk_waves(RW);
-  constexpr bool kD = (MODE == kDField);
+  constexpr bool kD = (MODE == kDField || MODE == kDOnly);   // D-weighted adjoint field
+  constexpr bool kL = (MODE == kDOnly);   // bank = (dt^2/rho) A(C0) u^m, dJ/dD only
   static_assert(MODE != kField6, "six-field adjoint: adj_block_body6");
   static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
   __shared__ float xch[2][WB][2][2][kCols];
@@ -1241,7 +1284,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   }
   gdouble* part = reinterpret_cast<gdouble*>(C.ws) + a.partial_off +
                   ((int64_t)t * WB + w) * kPartials + (lane >> 3);
-  const double part_in = *part;
+  const double part_in = kL ? 0.0 : *part;
   float gacc[RW][V];   // per-cell damage gradient of this launch (FP32 over at most KS steps)
 #pragma unroll
   for (int r = 0; r < RW; ++r)
@@ -1266,11 +1309,13 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
       }
   };
   // u^m of the lane's cells on the band rows in the output tile (the damage
-  // mode with one row of halo on each side for its stencil).  Step s + 1's
-  // loads are issued as soon as step s has used its u, before the band-edge
-  // exchange, so they arrive while the next update computes.
-  constexpr int RU = kD ? RW + 2 : RW;
-  constexpr int UO = kD ? 1 : 0;
+  // mode with one row of halo on each side for its stencil; kDOnly: the
+  // banked operator value, pointwise).  Step s + 1's loads are issued as soon
+  // as step s has used its u, before the band-edge exchange, so they arrive
+  // while the next update computes.
+  constexpr bool kUS = kD && !kL;   // a stencil of u
+  constexpr int RU = kUS ? RW + 2 : RW;
+  constexpr int UO = kUS ? 1 : 0;
   float urx[RU][V], ury[RU][V];
   const bool grad_rows = bd.own_lo < bd.own_hi;   // wave-uniform
   auto load_u = [&](int s) {
@@ -1323,8 +1368,8 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           yr[r + 1] = make_row_z<V>(ry);
         }
       }
-      Row<V> uxr[kD ? RW + 2 : 1], uyr[kD ? RW + 2 : 1];
-      if constexpr (kD) {
+      Row<V> uxr[kUS ? RW + 2 : 1], uyr[kUS ? RW + 2 : 1];
+      if constexpr (kUS) {
         if (grad_rows) {
 #pragma unroll
           for (int r = 0; r < RW + 2; ++r) {
@@ -1344,8 +1389,12 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           apply_op_sum(kl, dl, dx[r][e], dy[r][e]);
           // gradient terms on the band rows of the output tile (lam^{m+1} with u^m)
           if (r >= bd.own_lo && r < bd.own_hi) {
-            sbp_sums(dl, kD ? uxr[r + 1].v[x] : urx[r][e], kD ? uyr[r + 1].v[x] : ury[r][e], S);
-            if constexpr (kD) {
+            if constexpr (kL) {   // the last level's term (lam^{nt+1} = 0) is skipped: its bank slot is unwritten
+              if (m < I.nt) gacc[r][e] += __builtin_fmaf(l[0][r][e], urx[r][e], l[1][r][e] * ury[r][e]);
+            } else {
+              sbp_sums(dl, kUS ? uxr[r + 1].v[x] : urx[r][e], kUS ? uyr[r + 1].v[x] : ury[r][e], S);
+            }
+            if constexpr (kUS) {
               const UDeriv du = uderivs<V>(uxr[r], uxr[r + 1], uxr[r + 2], uyr[r], uyr[r + 1], uyr[r + 2], x);
               float ox, oy;
               apply_op_sum(ku, du, ox, oy);
@@ -1357,8 +1406,10 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
       // them in place: left free, the compiler sinks the gradient arithmetic
       // below the injection branch and keeps every row's derivatives live
       // across it.
+      if constexpr (!kL) {
 #pragma unroll
-      for (int p = 0; p < kPartials; ++p) asm volatile("" : "+v"(S[p]));
+        for (int p = 0; p < kPartials; ++p) asm volatile("" : "+v"(S[p]));
+      }
       if constexpr (kD) {
 #pragma unroll
         for (int r = 0; r < RW; ++r)
@@ -1492,7 +1543,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     }
   }
   // constant-gradient partials of this wavefront (slot t WB + w; lane 8q holds slot q)
-  {
+  if constexpr (!kL) {
     double Sd[kPartials];
 #pragma unroll
     for (int p = 0; p < kPartials; ++p) Sd[p] = bd.col_own ? (double)S[p] : 0.0;
@@ -1509,7 +1560,8 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
         const uint32_t so = bd.srow(I, r) / (uint32_t)sizeof(float);
 #pragma unroll
         for (int e = 0; e < V; ++e)
-          if (bd.col_in[e]) acc64[so + bd.vcol[e] / (uint32_t)sizeof(float)] = acc_in[r][e] + gsa * (double)gacc[r][e];
+          if (bd.col_in[e])
+            acc64[so + bd.vcol[e] / (uint32_t)sizeof(float)] = acc_in[r][e] + (kL ? 1.0 : gsa) * (double)gacc[r][e];
       }
     }
   }
@@ -1940,7 +1992,7 @@ template <int MODE>
 static hipError_t launch_adj_m(const StepArgs& a, int V, int blocks, bool loop, hipStream_t st) {
   if constexpr (MODE == kField6) {
     ERNN_TILE_CASES_FIELD6_ADJ(ERNN_ADJ_CASE)
-  } else if constexpr (MODE == kDField) {
+  } else if constexpr (MODE == kDField || MODE == kDOnly) {
     ERNN_TILE_CASES_DFIELD_ADJ(ERNN_ADJ_CASE)
   } else {
     ERNN_TILE_CASES(ERNN_ADJ_CASE)
@@ -2006,6 +2058,7 @@ hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st)
     case kConst6: return launch_block_m<kConst6>(a, V, grid, st);
     case kDField: return launch_block_m<kDField>(a, V, grid, st);
     case kField6: return launch_block_m<kField6>(a, V, grid, st);
+    case kDOnly: return launch_block_m<kDOnly>(a, V, grid, st);
   }
   return hipErrorInvalidValue;
 }
@@ -2025,7 +2078,7 @@ template <int MODE>
 static hipError_t launch_adj_block_m(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
   if constexpr (MODE == kConst6) {
     ERNN_ADJ_BLOCK_CASES(ERNN_ADJ_BLOCK_CASE)
-  } else if constexpr (MODE == kDField) {
+  } else if constexpr (MODE == kDField || MODE == kDOnly) {
     ERNN_ADJ_BLOCK_CASES_DFIELD(ERNN_ADJ_BLOCK_CASE)
   } else {
     ERNN_ADJ_BLOCK_CASES_FIELD6(ERNN_ADJ_BLOCK_CASE)
@@ -2039,6 +2092,7 @@ hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t 
     case kConst6: return launch_adj_block_m<kConst6>(a, V, grid, st);
     case kDField: return launch_adj_block_m<kDField>(a, V, grid, st);
     case kField6: return launch_adj_block_m<kField6>(a, V, grid, st);
+    case kDOnly: return launch_adj_block_m<kDOnly>(a, V, grid, st);
   }
   return hipErrorInvalidValue;
 }
@@ -2083,6 +2137,7 @@ hipError_t launch_fwd_step(const StepArgs& a, int V, int blocks, bool loop, hipS
     case kConst6: return launch_fwd_m<kConst6>(a, V, blocks, loop, st);
     case kDField: return launch_fwd_m<kDField>(a, V, blocks, loop, st);
     case kField6: return launch_fwd_m<kField6>(a, V, blocks, loop, st);
+    case kDOnly: return launch_fwd_m<kDOnly>(a, V, blocks, loop, st);
   }
   return hipErrorInvalidValue;
 }
@@ -2092,6 +2147,7 @@ hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, bool loop, hipS
     case kConst6: return launch_adj_m<kConst6>(a, V, blocks, loop, st);
     case kDField: return launch_adj_m<kDField>(a, V, blocks, loop, st);
     case kField6: return launch_adj_m<kField6>(a, V, blocks, loop, st);
+    case kDOnly: return launch_adj_m<kDOnly>(a, V, blocks, loop, st);
   }
   return hipErrorInvalidValue;
 }

@@ -22,6 +22,12 @@
 //            recomputed states equal the forward's bit for bit.
 //  fused     small grids: one launch runs adjoint work of segment s and
 //            recompute work of segment s - 1 (two banks).
+//  fixed_c0  (damage field over known constants, kernel mode kDOnly) the bank
+//            slot of level m holds L^m = (dt^2/rho) A(C0) u^m instead of u^m:
+//            step n banks L^n of the level it leaves, so a recompute of
+//            segment s runs steps sK .. top (one past the segment, for
+//            L^top; the last segment stops at nt - 1, and the adjoint skips
+//            the term of level nt, whose lam^{nt+1} is zero).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -172,6 +178,12 @@ struct Plan {
   bool use_graph = false;
   bool fuse = false;
   int ff_rw = 0;   // rows per wavefront of the fused recompute half (0: no fused time-blocked backward)
+  // fixed_c0 (damage field over known constants): the bank holds the operator
+  // value L^m = (dt^2/rho) A(C0) u^m of level m instead of u^m, written by the
+  // step that leaves level m (a recompute runs one step past its segment for
+  // L^top), and the kernels run in mode kDOnly
+  bool lbank = false;
+  int kmode = 0;   // kernel mode (ParamMode)
   std::vector<ernn_instance_t> inst_h;
   std::vector<ernn_source_t> src_h;
   std::vector<ernn_receiver_t> rec_h;
@@ -205,8 +217,8 @@ struct Plan {
 
   // An instance's state slots (kStateFields fields each): two forward ring
   // slots, the checkpoints ck(s) = level sK, two lam ring slots; its bank
-  // slots (u only): banks x K, bank(b, i) holding u^{sK+1+i} of a segment s
-  // with bank_of(s) = b.
+  // slots (u only, or L with lbank): banks x K, bank(b, i) holding u^{sK+1+i}
+  // of a segment s with bank_of(s) = b.
   int banks() const { return fuse ? 2 : 1; }
   int ring(int r) const { return r; }
   int ck(int s) const { return 2 + s; }
@@ -551,6 +563,10 @@ int validate(const ernn_desc_t* d) {
                                 "or be (1, 8, 5)");
   if (d->ablk_ks > 1 && !ablk_ok(d->param_mode, d->fuse != 0, d->ablk_v, d->ablk_rw, d->ablk_ks))
     return fail(ERNN_E_INVALID, kAblkShapes);
+  if (d->fixed_c0 != 0 && d->fixed_c0 != 1) return fail(ERNN_E_INVALID, "fixed_c0 must be 0 or 1");
+  if (d->fixed_c0 && d->param_mode != ERNN_PARAM_DFIELD)
+    return fail(ERNN_E_INVALID, "fixed_c0 applies to the damage-field mode (ERNN_PARAM_DFIELD) only");
+  if (d->fixed_c0 && d->fuse) return fail(ERNN_E_INVALID, "fixed_c0 needs fuse = 0");
   if (d->ablk_ff_rw != 0 && (d->ablk_ks <= 1 || !d->fuse || !fused_ff_ok(d->ablk_v, d->ablk_rw, d->ablk_ks, d->ablk_ff_rw)))
     return fail(ERNN_E_INVALID, "ablk_ff_rw: 0, or on a fuse plan with a time-blocked adjoint the adjoint's RW, or 8 "
                                 "beside RW = 4 with KS in {2,4}");
@@ -624,6 +640,8 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   p->K = d->ckpt_every;
   p->use_graph = d->use_graph != 0;
   p->fuse = d->fuse != 0;
+  p->lbank = d->fixed_c0 == 1 && p->mode == ERNN_PARAM_DFIELD;
+  p->kmode = p->lbank ? kDOnly : p->mode;
   p->n_coef = p->mode == ERNN_PARAM_DFIELD ? 1 : (p->mode == ERNN_PARAM_FIELD6 ? 6 : 0);
   p->n_acc = p->n_coef;
   p->inst_h.assign(d->inst, d->inst + d->n_inst);
@@ -763,7 +781,7 @@ int check_call(const Plan* p, const ernn_call_t* c, bool adjoint, bool grads = t
   if (!adjoint && !c->traces) return fail(ERNN_E_INVALID, "null traces");
   if (adjoint) {
     if (!c->grad_traces) return fail(ERNN_E_INVALID, "null grad_traces");
-    if (grads && p->mode != ERNN_PARAM_FIELD6 && !c->grad_params)
+    if (grads && p->mode != ERNN_PARAM_FIELD6 && !p->lbank && !c->grad_params)
       return fail(ERNN_E_INVALID, "null grad_params");
     if (grads && p->mode != ERNN_PARAM_CONST6 && !c->grad_field)
       return fail(ERNN_E_INVALID, "null grad_field");
@@ -780,7 +798,7 @@ StepArgs base_args(const Plan* p, const Tiling& t) {
   a.call = p->d_call;
   a.partial_off = p->partial_off;
   a.n_tiles = t.n_tiles;
-  a.mode = p->mode;
+  a.mode = p->kmode;
   a.stride = p->P;
   a.snapshot = p->snapshot;
   a.rows = t.R;
@@ -804,7 +822,7 @@ BlockArgs block_args(const Plan* p, const BlockTiling& bt) {
   a.rec = bt.d_rec;
   a.call = p->d_call;
   a.n_blocks = bt.n_blocks;
-  a.mode = p->mode;
+  a.mode = p->kmode;
   a.stride = p->P;
   a.snapshot = p->snapshot;
   a.rw = bt.RW;
@@ -816,8 +834,8 @@ BlockArgs block_args(const Plan* p) { return block_args(p, p->blk); }
 // Time-blocked forward-pass launch from level n0 (state slot *in) up to level
 // `end` at most: writes the checkpoint levels inside it, its last level to the
 // ring slot it does not read (when that level is no checkpoint), and the
-// levels of the last checkpoint segment to its bank.  *in becomes the slot of
-// the last level.
+// levels of the last checkpoint segment to its bank (lbank: step n banks L^n
+// of the level it leaves).  *in becomes the slot of the last level.
 void blk_forward_args(const Plan* p, BlockArgs& a, int n0, int end, int* in) {
   const int B = (p->S - 1) * p->K;
   a.n0 = n0;
@@ -828,24 +846,36 @@ void blk_forward_args(const Plan* p, BlockArgs& a, int n0, int end, int* in) {
     const int m = n0 + 1 + j;
     if (p->is_ck(m)) a.st_store[j] = p->ck(m / p->K);
     else if (j == a.ks - 1) a.st_store[j] = p->other_ring(*in);
-    if (m > B) a.bank_store[j] = p->bank(p->S - 1, m);
+    const int lev = p->lbank ? m - 1 : m;   // the level this step banks
+    if (lev > B) a.bank_store[j] = p->bank(p->S - 1, lev);
   }
   *in = a.st_store[a.ks - 1];
 }
 
-// Time-blocked recompute launch of segment s from level n0 (state slot *in):
-// every level into the segment's bank, the last one also to the ring slot the
-// launch does not read unless it ends the segment.
-void blk_recompute_args(const Plan* p, BlockArgs& a, int s, int n0, int* in) {
+// Last step + 1 of the recompute of segment s: its top level, or with lbank
+// one step further, whose step banks L^top (the last segment's top is the
+// last level, whose term the adjoint skips: lam^{nt+1} = 0).
+int recompute_end(const Plan* p, int s) {
   const int top = p->seg_top(s);
+  return p->lbank ? std::min(top + 1, p->nt_max) : top;
+}
+
+// Time-blocked recompute launch of segment s from level n0 (state slot *in):
+// every level into the segment's bank (lbank: L^n of every level n > sK
+// the launch leaves), the last one also to the ring slot the launch does not
+// read unless it ends the recompute.
+void blk_recompute_args(const Plan* p, BlockArgs& a, int s, int n0, int* in) {
+  const int end = recompute_end(p, s);
   a.n0 = n0;
-  a.ks = std::min(a.ksmax, top - n0);
+  a.ks = std::min(a.ksmax, end - n0);
   a.s_in = *in;
   for (int j = 0; j < kMaxBlockSteps; ++j) {
     a.st_store[j] = -1;
-    a.bank_store[j] = j < a.ks ? p->bank(s, n0 + 1 + j) : -1;
+    if (j >= a.ks) a.bank_store[j] = -1;
+    else if (!p->lbank) a.bank_store[j] = p->bank(s, n0 + 1 + j);
+    else a.bank_store[j] = n0 + j > s * p->K ? p->bank(s, n0 + j) : -1;
   }
-  if (n0 + a.ks < top) {
+  if (n0 + a.ks < end) {
     a.st_store[a.ks - 1] = p->other_ring(*in);
     *in = a.st_store[a.ks - 1];
   }
@@ -859,7 +889,7 @@ BlockAdjArgs block_adj_args(const Plan* p) {
   a.call = p->d_call;
   a.partial_off = p->partial_off;
   a.n_blocks = p->blk_adj.n_blocks;
-  a.mode = p->mode;
+  a.mode = p->kmode;
   a.stride = p->P;
   a.snapshot = p->snapshot;
   a.rw = p->blk_adj.RW;
@@ -884,15 +914,17 @@ void step_forward_args(const Plan* p, StepArgs& a, int n, int* in, bool bank_las
   a.step = n;
   a.s_in = *in;
   a.s_out = p->is_ck(m) ? p->ck(m / p->K) : p->other_ring(*in);
-  a.bank = (bank_last_segment && m > (p->S - 1) * p->K) ? p->bank(p->S - 1, m) : -1;
+  const int lev = p->lbank ? n : m;   // the level this step banks (u^m, or L^n)
+  a.bank = (bank_last_segment && lev > (p->S - 1) * p->K) ? p->bank(p->S - 1, lev) : -1;
   *in = a.s_out;
 }
-// recompute step n of segment s: level n -> n + 1 into the bank (and a ring slot)
+// recompute step n of segment s: level n -> n + 1 into the bank (lbank: L^n,
+// for n > sK) and a ring slot
 void step_recompute_args(const Plan* p, StepArgs& a, int s, int n, int* in) {
   a.step = n;
   a.s_in = *in;
   a.s_out = p->other_ring(*in);
-  a.bank = p->bank(s, n + 1);
+  a.bank = !p->lbank ? p->bank(s, n + 1) : (n > s * p->K ? p->bank(s, n) : -1);
   *in = a.s_out;
 }
 // adjoint step m of segment s: lam level m + 1 -> m, u^m from the bank
@@ -952,15 +984,15 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
   // recompute segment s whole (time-blocked when enabled)
   auto recompute_segment = [&](int s) -> int {
     int in = p->ck(s);
-    const int top = p->seg_top(s);
+    const int end = recompute_end(p, s);
     if (p->blk.KS > 1) {
-      for (int n0 = s * p->K; n0 < top; n0 += p->blk.KS) {
+      for (int n0 = s * p->K; n0 < end; n0 += p->blk.KS) {
         blk_recompute_args(p, rb, s, n0, &in);
         ERNN_HIP(launch_fwd_block(rb, p->blk.V, p->blk.grid, st));
       }
       return ERNN_OK;
     }
-    for (int n = s * p->K; n < top; ++n) {
+    for (int n = s * p->K; n < end; ++n) {
       step_recompute_args(p, f, s, n, &in);
       ERNN_HIP(launch_fwd_step(f, p->fwd.V, p->fwd.blocks, p->fwd.loop, st));
     }
@@ -1051,7 +1083,7 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
       }
     }
   }
-  if (p->mode != ERNN_PARAM_FIELD6) {
+  if (p->mode != ERNN_PARAM_FIELD6 && !p->lbank) {
     if (p->blk_adj.KS > 1)
       ERNN_HIP(launch_finalize_const(p->d_call, p->partial_off, p->d_set_list, p->d_set_begin,
                                      p->blk_adj.d_inst_first, blk_waves(p->blk_adj.RW), p->n_sets, st));
@@ -1177,6 +1209,7 @@ int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
   info->ablk_ks = p->blk_adj.KS;
   info->ablk_blocks = p->blk_adj.n_blocks;
   info->ablk_ff_rw = p->ff_rw;
+  info->fixed_c0 = p->lbank ? 1 : 0;
   return ERNN_OK;
 }
 

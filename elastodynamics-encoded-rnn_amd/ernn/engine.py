@@ -85,6 +85,7 @@ class Engine:
         d.blk_v, d.blk_rw, d.blk_ks = (int(x) for x in sv.blocking)
         d.ablk_v, d.ablk_rw, d.ablk_ks = (int(x) for x in sv.adj_blocking)
         d.ablk_ff_rw = int(sv.adj_ff_rw)
+        d.fixed_c0 = int(bool(sv.fixed_c0))
         d.inst = inst
         d.src, d.n_src = src, len(src_list)
         d.rec, d.n_rec = rec, len(rec_list)
@@ -112,6 +113,11 @@ class Engine:
             pass
 
     # ------------------------------------------------------------- buffers
+    @property
+    def fixed_c0(self) -> bool:
+        """The plan returns dJ/dD only (dfield over known constants)."""
+        return bool(self.info.fixed_c0)
+
     @property
     def workspace_bytes(self) -> int:
         return int(self.info.workspace_bytes)
@@ -184,12 +190,14 @@ class Engine:
 
     def adjoint(self, params, field, wavelets, grad_traces: torch.Tensor, ws: torch.Tensor
                 ) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor]]:
-        """Gradient of J given dJ/d traces; returns (grad_params, grad_field) in fp64."""
+        """Gradient of J given dJ/d traces; returns (grad_params, grad_field) in fp64
+        (grad_params None for six fields and for a fixed_c0 plan)."""
         self._check_inputs(params, field, wavelets)
         if grad_traces.numel() != self.trace_floats or grad_traces.dtype != torch.float32:
             raise ValueError("grad_traces must be float32 with the trace layout")
         grad_traces = grad_traces.contiguous()
-        gp = torch.empty(self.survey.n_sets * 6, dtype=torch.float64, device=self.device) if self.mode != 2 else None
+        gp = (torch.empty(self.survey.n_sets * 6, dtype=torch.float64, device=self.device)
+              if self.mode != 2 and not self.fixed_c0 else None)
         gf = (torch.empty(int(self.info.grad_field_doubles), dtype=torch.float64, device=self.device)
               if self.mode != 0 else None)
         c = self._call(ws, params, field, wavelets, grad_traces=grad_traces, grad_params=gp, grad_field=gf)

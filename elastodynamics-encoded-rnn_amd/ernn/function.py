@@ -28,6 +28,8 @@ class ERNNFunction(torch.autograd.Function):
         f = field if engine.mode != 0 else None
         ws = engine.acquire_workspace()
         traces = engine.forward(p, f, wavelets, ws)
+        if ctx.needs_input_grad[1] and engine.fixed_c0:
+            raise RuntimeError("this plan returns dJ/dD only (Survey.fixed_c0): the constants must not require grad")
         needs = ctx.needs_input_grad[1] or ctx.needs_input_grad[2]
         if needs:
             ctx.engine = engine

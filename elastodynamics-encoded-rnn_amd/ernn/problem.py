@@ -196,13 +196,25 @@ class Survey:
     # Fused time-blocked backward: rows per wavefront of the recompute half
     # (0 = the adjoint's RW; 8 beside RW = 4 with KS in {2, 4})
     adj_ff_rw: int = 0
+    # dfield only: the six background constants are known (stiffness-degradation
+    # imaging) and the adjoint returns dJ/dD alone.  The checkpoint bank then
+    # holds (dt^2/rho) A(C0) u^m, which the forward forms anyway, instead of
+    # u^m, and the adjoint reads it pointwise instead of taking a stencil of
+    # u (no constant-gradient sums).  Needs the unfused backward.
+    fixed_c0: bool = False
 
     def __post_init__(self):
         if self.mode not in MODES:
             raise ValueError(f"mode must be one of {sorted(MODES)}")
+        if self.fixed_c0 and self.mode != "dfield":
+            raise ValueError("fixed_c0 applies to the dfield mode only")
+        if self.fixed_c0 and self.fuse:
+            raise ValueError("fixed_c0 needs the unfused backward (fuse False or None)")
 
     @property
     def fused(self) -> bool:
+        if self.fixed_c0:
+            return False
         if self.fuse is not None:
             return bool(self.fuse)
         return sum(s.nx * s.ny for s in self.shots) <= FUSE_MAX_CELLS

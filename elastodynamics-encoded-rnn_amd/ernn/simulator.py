@@ -263,7 +263,8 @@ class ERNN(nn.Module):
     plain-128 inversion within 2 % in 50 iterations at lr 1e-2, where q = 1
     needs about 150.
     In dfield mode the per-cell factor D multiplies all six constants and is a
-    parameter too (``learn_damage``).  ``trainable`` (six booleans) freezes
+    parameter too (``learn_damage``).  A survey with ``fixed_c0`` (background
+    constants known, dJ/dD only) needs ``learn_constants=False``.  ``trainable`` (six booleans) freezes
     constants known a priori, e.g. C16 = C26 = 0 in an orthotropic frame.
     forward() returns the flat trace tensor; ``shot_traces``/``stacked`` give views.
     """
@@ -273,6 +274,8 @@ class ERNN(nn.Module):
                  learn_constants: bool = True, learn_damage: bool = True, device=None,
                  autotune: bool = False, trainable: Optional[Sequence[bool]] = None):
         super().__init__()
+        if survey.fixed_c0 and learn_constants:
+            raise ValueError("the survey's plan returns dJ/dD only (fixed_c0): pass learn_constants=False")
         self.survey = survey
         dev = torch.device(device) if device is not None else torch.device("cuda")
         C_init = np.asarray(C_init, dtype=np.float64).reshape(survey.n_sets, 6)

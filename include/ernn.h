@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define ERNN_ABI_VERSION 4
+#define ERNN_ABI_VERSION 5
 
 #define ERNN_OK 0
 #define ERNN_E_INVALID (-1)   /* bad argument or descriptor */
@@ -108,6 +108,14 @@ typedef struct {
                            fused kernel's recompute half, 0 = ablk_rw, or 8 beside ablk_rw = 4
                            with ablk_ks in {2,4} (shorter grids: every block resident at once);
                            otherwise 0 */
+  int32_t fixed_c0;     /* DFIELD only, 0 or 1.  1: the six background constants are known
+                           (stiffness-degradation imaging, C_p = D C0_p with C0 fixed) and the
+                           adjoint returns dJ/dD only.  dJ/dD(cell) = sum over steps of
+                           lam^{m+1} . (dt^2/rho) A(C0) u^m needs u^m only through A(C0) u^m,
+                           which the forward computes anyway: the checkpoint bank then holds
+                           (dt^2/rho) A(C0) u^m instead of u^m (same size), and the adjoint
+                           reads it pointwise instead of taking a stencil of u.  Requires
+                           fuse = 0; grad_params is not written. */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;
@@ -136,6 +144,7 @@ typedef struct {
   int32_t ablk_blocks;
   int32_t ablk_ff_rw;       /* rows per wavefront of the fused recompute half (0: no fused
                                time-blocked backward) */
+  int32_t fixed_c0;         /* the descriptor's fixed_c0 (DFIELD: dJ/dD only) */
 } ernn_plan_info_t;
 
 /* Per-call device pointers.
@@ -149,7 +158,7 @@ typedef struct {
  *              rec_count (receiver mode, ordered as the instance's receiver
  *              slice) or nx*ny (snapshot mode); component 0 = ux, 1 = uy.
  *  grad_traces adjoint input dJ/d traces, same layout as traces
- *  grad_params [n_sets][6] fp64 output (CONST6, DFIELD)
+ *  grad_params [n_sets][6] fp64 output (CONST6, DFIELD without fixed_c0)
  *  grad_field  fp64 output, per set [n_coef][nx*ny] (DFIELD: dJ/dD; FIELD6:
  *              dJ/dC_p fields), sets back to back */
 typedef struct {

@@ -91,6 +91,10 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
     ({}, {"param_mode": 2, "ablk_v": 1, "ablk_rw": 8, "ablk_ks": 4}, "time-blocked adjoint"),
     ({}, {"param_mode": 2, "ablk_v": 1, "ablk_rw": 2, "ablk_ks": 8}, "time-blocked adjoint"),
     ({}, {"param_mode": 2, "ablk_v": 2, "ablk_rw": 2, "ablk_ks": 2}, "time-blocked adjoint"),
+    ({}, {"fixed_c0": 1}, "damage-field mode"),
+    ({}, {"param_mode": 2, "fixed_c0": 1}, "damage-field mode"),
+    ({}, {"param_mode": 1, "fixed_c0": 2}, "fixed_c0 must be 0 or 1"),
+    ({}, {"param_mode": 1, "fixed_c0": 1, "fuse": 1}, "fuse = 0"),
 ])
 def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
     inst = dict(GOOD, **patch)
