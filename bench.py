@@ -83,6 +83,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="damage-1024")
     ap.add_argument("--nt", type=int, default=0, help="override the step count (profiling runs)")
+    ap.add_argument("--joint-c0", type=int, default=0,
+                    help="damage-1024: also invert the six background constants (joint gradient; the "
+                         "workload's own setting inverts D over known constants)")
+    ap.add_argument("--ckpt-every", type=int, default=0,
+                    help="fix the checkpoint interval (profiling runs; 0 = the workload's memory budget)")
     ap.add_argument("--graph", type=int, default=1, help="replay passes as captured hipGraphs")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--tiles", default="", help="fixed tiles 'Vf,Rf,Va,Ra,Bf,Ba' (skips autotuning)")
@@ -184,8 +189,9 @@ def autograd_traces(torch, shot, C, faithful, D=None, nsteps=None, fields=None, 
     return torch.stack([torch.stack(hx), torch.stack(hy)])
 
 
-def time_autograd(torch, wl, theta0, scale, D0, observed, window):
+def time_autograd(torch, wl, theta0, scale, D0, observed, window, learn_c=True):
     """s/Adam-iter of the plain path (shots one at a time, as the reference).
+    learn_c False: the constants are known (no gradient), only D is inverted.
 
     If the survey is longer than `window` steps or has several shots, one shot
     is timed over `window` steps and the per-step cost is scaled to
@@ -200,8 +206,8 @@ def time_autograd(torch, wl, theta0, scale, D0, observed, window):
     W = max(10, min(window, s0.nt, mem_cap // tape_step))
     scale_f = sum(s.nt for s in shots) / W
     for faithful in (True, False):
-        theta = torch.nn.Parameter(theta0.clone())
-        params = [theta]
+        theta = torch.nn.Parameter(theta0.clone(), requires_grad=learn_c)
+        params = [theta] if learn_c else []
         D = None
         if D0 is not None:
             D = torch.nn.Parameter(D0.clone())
@@ -299,8 +305,8 @@ def time_instances(torch, ernn, wl, C_init, D_init, dev, iters):
     for b, s in enumerate(wl.survey.shots):
         one = ernn.Shot(**{**s.__dict__, "set": 0})
         sv1 = ernn.Survey([one], mode=wl.survey.mode, n_sets=1, ckpt_every=wl.survey.ckpt_every,
-                          use_graph=wl.survey.use_graph)
-        m = ernn.ERNN(sv1, C_init[s.set:s.set + 1], D_init=D_init, device=dev)
+                          use_graph=wl.survey.use_graph, fixed_c0=wl.survey.fixed_c0)
+        m = ernn.ERNN(sv1, C_init[s.set:s.set + 1], D_init=D_init, device=dev, learn_constants=not sv1.fixed_c0)
         for k in range(iters + 1):
             if k == 1:
                 torch.cuda.synchronize()
@@ -710,7 +716,7 @@ def di_main(args):
     import numpy as np
     import torch
     import ernn
-    from ernn.autotune import tune_blocking, tune_tiles
+    from ernn.autotune import timing_buffers, tune_blocking, tune_tiles
     from ernn.inversion import add_noise_rms
 
     dev = torch.device("cuda:0")
@@ -775,25 +781,25 @@ def di_main(args):
 
     # per-kernel launch durations (hipEvents on the launch stream)
     ws = eng.acquire_workspace()
-    tr0 = torch.zeros(eng.trace_floats, device=dev)
+    tr0, gr0 = timing_buffers(eng)
     field = torch.cat([c.reshape(-1) for c in sim.get_material_params()[:6]]).detach().contiguous()
     nk = min(200, nt)
-    eng.time_steps(0, 10, None, field, wav, ws, tr0, tr0)
-    fwd_ms = eng.time_steps(0, nk, None, field, wav, ws, tr0, tr0) / nk
-    adj_ms = eng.time_steps(1, nk, None, field, wav, ws, tr0, tr0) / nk
-    fused_ms = eng.time_steps(2, nk, None, field, wav, ws, tr0, tr0) / nk if eng.info.fuse else None
+    eng.time_steps(0, 10, None, field, wav, ws, tr0, gr0)
+    fwd_ms = eng.time_steps(0, nk, None, field, wav, ws, tr0, gr0) / nk
+    adj_ms = eng.time_steps(1, nk, None, field, wav, ws, tr0, gr0) / nk
+    fused_ms = eng.time_steps(2, nk, None, field, wav, ws, tr0, gr0) / nk if eng.info.fuse else None
     KS, AKS = eng.blocking[2], eng.adj_blocking[2]
     blk_ms = ablk_ms = None
     if KS > 1:
         L = max(2, nk // KS)
-        eng.time_steps(3, 2, None, field, wav, ws, tr0, tr0)
-        blk_ms = eng.time_steps(3, L, None, field, wav, ws, tr0, tr0) / L
+        eng.time_steps(3, 2, None, field, wav, ws, tr0, gr0)
+        blk_ms = eng.time_steps(3, L, None, field, wav, ws, tr0, gr0) / L
     if AKS > 1:
         L = max(2, nk // AKS)
-        eng.time_steps(4, 2, None, field, wav, ws, tr0, tr0)
-        ablk_ms = eng.time_steps(4, L, None, field, wav, ws, tr0, tr0) / L
+        eng.time_steps(4, 2, None, field, wav, ws, tr0, gr0)
+        ablk_ms = eng.time_steps(4, L, None, field, wav, ws, tr0, gr0) / L
     eng.release_workspace(ws)
-    del ws, tr0
+    del ws, tr0, gr0
     bf, ba = BYTES_PER_CELL[("field6", 0)], BYTES_PER_CELL[("field6", 1)]
     K = int(eng.info.ckpt_every)
     S = math.ceil(nt / K)
@@ -998,7 +1004,7 @@ def time_hbm_budget(torch, ernn, sv, eng, C_init, D_init, obs, dev, args):
     eng.trim()
     torch.cuda.empty_cache()
     sv2 = dataclasses.replace(sv, ckpt_budget=None, fuse=bool(eng.info.fuse))
-    model = ernn.ERNN(sv2, C_init, D_init=D_init, device=dev)
+    model = ernn.ERNN(sv2, C_init, D_init=D_init, device=dev, learn_constants=not sv2.fixed_c0)
     e2 = model.engine
     t = eng.tiles
     e2.set_tiling(0, t[0], t[1], t[4])
@@ -1053,18 +1059,26 @@ def main():
     import torch
     import ernn
     from ernn import configs
-    from ernn.autotune import state_bytes, tune_backward, tune_blocking, tune_fused_recompute, tune_tiles
+    from ernn.autotune import (state_bytes, timing_buffers, tune_backward, tune_blocking, tune_fused_recompute,
+                               tune_tiles)
 
     dev = torch.device("cuda:0")
     kw = {"nt": args.nt} if args.nt else {}
     wl = configs.WORKLOADS[args.workload](**kw)
     sv = wl.survey
     sv.use_graph = bool(args.graph)
+    if args.ckpt_every:
+        sv.ckpt_every = args.ckpt_every
+    variant = ""
+    if args.joint_c0 and sv.fixed_c0:
+        sv.fixed_c0, variant = False, "-joint"   # the profiles of this variant carry the suffix
     rng = np.random.default_rng(0)
     C_true = wl.C_true
     pert = 1.0 + rng.uniform(-0.2, 0.2, size=C_true.shape)
     C_init = C_true * pert
     C_init[:, 3:5] = C_true[:, 3:5] + rng.uniform(-0.2, 0.2, size=(C_true.shape[0], 2)) * C_true[:, 5:6]
+    if sv.fixed_c0:   # stiffness-degradation imaging: the background constants are known, D is inverted
+        C_init = C_true.copy()
     D_init = None if sv.mode == "const6" else np.ones_like(wl.D_true)
     log(f"workload {wl.name}: {wl.description}; {sv.cell_steps() / 1e9:.3f} Gcell-steps per pass")
 
@@ -1081,7 +1095,7 @@ def main():
         log(f"autotune: backward {'fused' if fuse else 'unfused'} (fused {bt['fused'] * 1e3:.2f} vs unfused "
             f"time-blocked {bt['blocked'] * 1e3:.2f} us per reverse step)")
 
-    model = ernn.ERNN(sv, C_init, D_init=D_init, device=dev)
+    model = ernn.ERNN(sv, C_init, D_init=D_init, device=dev, learn_constants=not sv.fixed_c0)
     eng = model.engine
     tune = None
     if args.tiles:
@@ -1153,42 +1167,64 @@ def main():
 
     # per-kernel launch durations (hipEvents around back-to-back launches on the launch stream)
     ws = eng.acquire_workspace()
-    tr0 = torch.zeros(eng.trace_floats, device=dev)
+    tr0, gr0 = timing_buffers(eng)
     nk = min(200, sv.nt_max)
     p = model.stiffness().detach().reshape(-1).contiguous()
     f = None if sv.mode == "const6" else model.damage.detach()
-    eng.time_steps(0, 10, p, f, model.wavelets, ws, tr0, tr0)
-    fwd_ms = eng.time_steps(0, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
-    adj_ms = eng.time_steps(1, nk, p, f, model.wavelets, ws, tr0, tr0) / nk
-    fused_ms = eng.time_steps(2, nk, p, f, model.wavelets, ws, tr0, tr0) / nk if eng.info.fuse else None
+    eng.time_steps(0, 10, p, f, model.wavelets, ws, tr0, gr0)
+    fwd_ms = eng.time_steps(0, nk, p, f, model.wavelets, ws, tr0, gr0) / nk
+    adj_ms = eng.time_steps(1, nk, p, f, model.wavelets, ws, tr0, gr0) / nk
+    fused_ms = eng.time_steps(2, nk, p, f, model.wavelets, ws, tr0, gr0) / nk if eng.info.fuse else None
     KS = eng.blocking[2]
-    blk_ms = None
+    blk_ms = blk_bank_ms = None
+    nt, K = sv.nt_max, int(eng.info.ckpt_every)
+    S = math.ceil(nt / K)
+    B = (S - 1) * K    # the forward pass banks from level B on (the last checkpoint segment)
     if KS > 1:   # per time-blocked launch of KS steps
-        L = max(2, nk // KS)
-        eng.time_steps(3, 2, p, f, model.wavelets, ws, tr0, tr0)
-        blk_ms = eng.time_steps(3, L, p, f, model.wavelets, ws, tr0, tr0) / L
+        # the first launches of the forward pass (no banking while L KS <= B),
+        # then the whole pass, whose launches past B also bank: their average
+        # is the banking launch's duration (as in the segment recompute)
+        L = max(2, min(nk // KS, B // KS)) if B >= 2 * KS else max(2, nk // KS)
+        eng.time_steps(3, 2, p, f, model.wavelets, ws, tr0, gr0)
+        blk_ms = eng.time_steps(3, L, p, f, model.wavelets, ws, tr0, gr0) / L
+        n_fwd, n_nb = math.ceil(nt / KS), B // KS
+        if B >= 2 * KS and n_fwd > n_nb:
+            t_all = eng.time_steps(3, n_fwd, p, f, model.wavelets, ws, tr0, gr0)
+            blk_bank_ms = max(t_all - n_nb * blk_ms, 0.0) / (n_fwd - n_nb)
+        else:   # one checkpoint segment: every launch of the pass banks
+            blk_bank_ms = blk_ms
     AKS = eng.adj_blocking[2]
     ablk_ms = None
     if AKS > 1:
         L = max(2, nk // AKS)
-        eng.time_steps(4, 2, p, f, model.wavelets, ws, tr0, tr0)
-        ablk_ms = eng.time_steps(4, L, p, f, model.wavelets, ws, tr0, tr0) / L
+        eng.time_steps(4, 2, p, f, model.wavelets, ws, tr0, gr0)
+        ablk_ms = eng.time_steps(4, L, p, f, model.wavelets, ws, tr0, gr0) / L
     eng.release_workspace(ws)
     cells = sum(s.nx * s.ny for s in sv.shots)
     bf, ba = BYTES_PER_CELL[(sv.mode, 0)], BYTES_PER_CELL[(sv.mode, 1)]
-    nt, K = sv.nt_max, int(eng.info.ckpt_every)
-    # (ms per launch, launches per iteration, algorithmic bytes per cell per
-    # launch, recompute bytes per cell per launch not counted): a launch of KS
-    # steps is credited KS steps of algorithmic bytes; the recompute half of a
-    # fused launch is overhead and is reported apart.
+    # (average ms per launch, launches per iteration, algorithmic bytes per
+    # cell per launch, recompute bytes per cell per launch not counted): a
+    # launch of KS steps is credited KS steps of algorithmic bytes; recompute
+    # work (a segment recompute launch, the recompute half of a fused launch)
+    # is overhead and is reported apart.
     kern = {}
     # recomputed steps: every segment but the last (the forward pass keeps the
-    # last segment's states); the fused schedule recomputes the last one alone
-    S = math.ceil(nt / K)
-    n_rec = K if eng.info.fuse else (S - 1) * K
+    # last segment's states; a fixed_c0 plan runs one step past each
+    # segment); the fused schedule recomputes the last one alone
+    seg = K + (1 if sv.fixed_c0 else 0)
+    n_rec = K if eng.info.fuse else (S - 1) * seg
+    blk_split = None
     if KS > 1:
-        n_blk = math.ceil(nt / KS) + (math.ceil(K / KS) if eng.info.fuse else (S - 1) * math.ceil(K / KS))
-        kern["fwd_block_kernel"] = (blk_ms, n_blk, bf * KS, 0)
+        n_fwd, n_nb = math.ceil(nt / KS), min(B // KS, math.ceil(nt / KS))
+        n_rc = math.ceil(K / KS) if eng.info.fuse else (S - 1) * math.ceil(seg / KS)
+        n_bank = n_fwd - n_nb + n_rc
+        total = n_nb * blk_ms + n_bank * blk_bank_ms
+        n_blk = n_fwd + n_rc
+        # launches of the forward pass carry its algorithmic bytes, recompute launches none
+        kern["fwd_block_kernel"] = (total / n_blk, n_blk, bf * KS * n_fwd / n_blk, bf * KS * n_rc / n_blk)
+        blk_split = {"launches_no_bank": n_nb, "launches_bank": n_bank, "us_no_bank": round(blk_ms * 1e3, 3),
+                     "us_bank": round(blk_bank_ms * 1e3, 3), "forward_pass_launches": n_fwd,
+                     "recompute_launches": n_rc}
     else:
         kern["fwd_step_kernel"] = (fwd_ms, nt + n_rec, bf, 0)
     if eng.info.fuse and AKS > 1:   # fused time-blocked backward: AKS adjoint steps + AKS recompute steps
@@ -1215,10 +1251,17 @@ def main():
     traffic, traffic_tiles, valu = None, None, None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        entry = json.load(open(tpath)).get(f"{wl.name}/{name}")
+        entry = json.load(open(tpath)).get(f"{wl.name}{variant}/{name}")
         if entry and entry.get("shape") == kshape:
             traffic, traffic_tiles = entry["bytes"], entry["tiles"]
-            if entry.get("valu_insts"):   # SQ_INSTS_VALU per launch, same counter runs
+            vi = entry.get("valu_insts")
+            if name == "fwd_block_kernel" and blk_split and "bytes_no_bank" in entry:
+                # the iteration's mix of launches that bank u (or L) and launches that do not
+                nb, bk = blk_split["launches_no_bank"], blk_split["launches_bank"]
+                traffic = int((nb * entry["bytes_no_bank"] + bk * entry["bytes"]) / (nb + bk))
+                vi = int((nb * entry["valu_insts_no_bank"] + bk * entry["valu_insts"]) / (nb + bk))
+            if vi:   # SQ_INSTS_VALU per launch, same counter runs
+                entry = dict(entry, valu_insts=vi)
                 rate = entry["valu_insts"] / (ms * 1e-3)
                 valu = {"insts_per_launch": entry["valu_insts"], "issue_rate": round(rate / 1e9, 1),
                         "peak": VALU_PEAK_WIPS / 1e9, "unit": "G wave-instructions/s",
@@ -1231,7 +1274,15 @@ def main():
                 "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),
                 "fused_step_us": round(fused_ms * 1e3, 3) if fused_ms else None,
                 "fwd_block_us_per_step": round(blk_ms / KS * 1e3, 3) if blk_ms else None,
-                "adj_block_us_per_step": round(ablk_ms / AKS * 1e3, 3) if ablk_ms else None}
+                "adj_block_us_per_step": round(ablk_ms / AKS * 1e3, 3) if ablk_ms else None,
+                "fwd_block_launches": blk_split,
+                "kernel_ms_per_iter": {k: round(v[0] * v[1], 2) for k, v in kern.items()},
+                # the whole iteration against the algorithmic bytes of one forward and one adjoint
+                # step per cell (recompute excluded): the HBM bound of the fwd + adjoint rate
+                "iteration": {"bytes_per_cell_step": bf + ba,
+                              "achieved": round((bf + ba) * sv.cell_steps() / dt_iter / 1e9, 1),
+                              "frac": round((bf + ba) * sv.cell_steps() / dt_iter / 1e9 / HBM_PEAK_GBS, 4),
+                              "bound_gcell_steps": round(HBM_PEAK_GBS / (bf + ba), 1)}}
     log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3:.2f} us, "
         f"fused {fused_ms * 1e3 if fused_ms else float('nan'):.2f} us per launch, time-blocked fwd "
         f"{blk_ms / KS * 1e3 if blk_ms else float('nan'):.2f} us per step, time-blocked adj "
@@ -1240,7 +1291,7 @@ def main():
 
     # the pooled workspaces (all of a segment's states) go back before the
     # baselines allocate their own
-    del ws, tr0
+    del ws, tr0, gr0
     eng.trim()
     torch.cuda.empty_cache()
     baseline = None
@@ -1248,7 +1299,8 @@ def main():
         theta0 = torch.tensor(C_init / model.scale.cpu().numpy(), dtype=torch.float32, device=dev)
         obs_shots = [eng.shot_traces(obs, b) for b in range(len(sv.shots))]
         D0 = None if sv.mode == "const6" else torch.tensor(D_init, device=dev)
-        ag = time_autograd(torch, wl, theta0, model.scale, D0, obs_shots, args.baseline_window)
+        ag = time_autograd(torch, wl, theta0, model.scale, D0, obs_shots, args.baseline_window,
+                           learn_c=not sv.fixed_c0)
         best = min(ag["faithful"], ag["hoisted"])
         baseline = {"s_per_adam_iter_faithful": round(ag["faithful"], 4),
                     "s_per_adam_iter_hoisted": round(ag["hoisted"], 4),
@@ -1277,9 +1329,9 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt_iter * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": wl.name, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
+        "config": {"workload": wl.name + variant, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
                    "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
-                   "mode": sv.mode, "ckpt_every": int(eng.info.ckpt_every),
+                   "mode": sv.mode, "fixed_c0": bool(sv.fixed_c0), "ckpt_every": int(eng.info.ckpt_every),
                    "ckpt_cap_bytes": sv.ckpt_budget, "ckpt_state_bytes": state_bytes(sv, int(eng.info.ckpt_every)),
                    "graph": bool(sv.use_graph),
                    "fused_backward": bool(eng.info.fuse),

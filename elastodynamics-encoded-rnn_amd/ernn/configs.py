@@ -11,8 +11,9 @@ C13, C15, C35, C55).
   plain-128       128^2, h 0.2 mm, 600 steps, 1 shot at top-centre, 64 top-row receivers
   single-512      512^2, h 0.05 mm, 2000 steps, 1 shot, 256 top-row receivers, K = 50
   damage-1024     1024^2, h 0.05 mm, 3000 steps, 8 top-edge shots, 512 receivers on the top
-                  and bottom rows, damage field D with 3 elliptical regions, checkpoint
-                  interval chosen under a 32 GB cap on the checkpointed states
+                  and bottom rows, damage field D with 3 elliptical regions over the known
+                  orthotropic background (dJ/dD only: fixed_c0), checkpoint interval chosen
+                  under a 32 GB cap on the checkpointed states
   batched-uneven  24 instances, sizes {256,512,768,1024}^2, h 0.05 mm, 2000 steps,
                   constants +/-30 %, own top-edge source, 256 top-row receivers each
   aniso-2048      2048^2, h 0.05 mm, 4000 steps, 16 shots, 1024 receivers, stiffness
@@ -70,14 +71,16 @@ DAMAGE_MEM_CAP = 32 * 10**9   # bytes of checkpointed state the damage-imaging w
 
 
 def damage_1024(nt: int = 3000, n: int = 1024, shots: int = 8, seed: int = 2,
-                mem_cap: Optional[int] = DAMAGE_MEM_CAP) -> Workload:
+                mem_cap: Optional[int] = DAMAGE_MEM_CAP, fixed_c0: bool = True) -> Workload:
     """mem_cap: HBM bytes for the checkpointed states (Survey.ckpt_budget);
-    None = the engine's default, 80 % of the free device memory."""
+    None = the engine's default, 80 % of the free device memory.  fixed_c0:
+    stiffness-degradation imaging inverts D over the known background
+    constants (dJ/dD only); False also returns dJ/dC0 (the joint gradient)."""
     h = 0.05e-3
     dt = cfl_dt(C_ORTHO, RHO, h)
     rec = np.concatenate([row_receivers(n, n - 1, n // 4), row_receivers(n, 0, n // 4)])
     sh = [_shot(n, h, nt, dt, int((k + 0.5) * n / shots), rec) for k in range(shots)]
-    sv = Survey(sh, mode="dfield", ckpt_budget=mem_cap)
+    sv = Survey(sh, mode="dfield", ckpt_budget=mem_cap, fixed_c0=fixed_c0)
     D = ellipse_damage(n, n, 3, np.random.default_rng(seed), semi=(20, 80), level=(0.3, 0.7), taper=5)
     return Workload("damage-1024", sv, np.array([C_ORTHO]), D,
                     description=f"{n}^2 x {nt}, {shots} shots, 512 receivers, D field 3 ellipses"

@@ -190,7 +190,7 @@ def test_fixed_c0_damage_1024_full_grid():
     the tuner ranks: traces bitwise equal to the joint plan, dJ/dD within 1e-5
     of it; K = 24 (eleven recomputed segments) and K = nt (none)."""
     from test_gpu_fullsize import _run, _variant
-    wl = configs.damage_1024(nt=300, shots=2)
+    wl = configs.damage_1024(nt=300, shots=2, fixed_c0=False)
     sv = wl.survey
     C0 = np.asarray(wl.C_true[0], np.float32)
     D = np.asarray(wl.D_true, np.float32).reshape(-1)
@@ -223,7 +223,8 @@ def test_fixed_c0_full_length_vs_fp64_oracle():
     from test_gpu_fullsize import _run, _variant
     n, nt, K = 128, 3000, 375
     wl = configs.damage_1024(nt=nt, n=n, shots=2)
-    sv = dataclasses.replace(wl.survey, fixed_c0=True)
+    sv = wl.survey
+    assert sv.fixed_c0
     C0 = np.asarray(wl.C_true[0], np.float32)
     D = np.asarray(wl.D_true, np.float32)
     probe = ernn.Engine(_variant(sv, ckpt_every=K), device=DEV)

@@ -183,7 +183,7 @@ def test_damage_field_every_shape_full_grid(name):
     300 steps, K = 24: every time-blocked shape vs the one-step path, the
     one-step path and the timed shapes vs per-cell plain autograd."""
     if name == "damage-1024":
-        wl = configs.damage_1024(nt=300, shots=2)
+        wl = configs.damage_1024(nt=300, shots=2, fixed_c0=False)
     else:
         wl = configs.aniso_2048(nt=300, n=1024, shots=2)
     sv = wl.survey
@@ -352,7 +352,7 @@ def test_full_length_gradient_vs_fp64_oracle(name, nt, K):
     from oracle import ernn_fp64 as orc
     n = 128
     if name == "damage-1024":
-        wl = configs.damage_1024(nt=nt, n=n, shots=2)
+        wl = configs.damage_1024(nt=nt, n=n, shots=2, fixed_c0=False)
     else:
         wl = configs.aniso_2048(nt=nt, n=n, shots=2)
     sv = wl.survey

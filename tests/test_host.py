@@ -253,3 +253,18 @@ def test_ellipse_damage_properties():
     assert D.shape == (200, 180) and D.dtype == np.float32
     assert D.max() == 1.0 and 0.3 <= D.min() <= 0.7
     assert (D < 1).mean() > 0.02
+
+
+def test_integration_stub_structs_match_the_binding():
+    """The ctypes stub in INTEGRATION.md declares the descriptor and plan-info
+    structs with the library's layout (the binding in ernn/_lib.py)."""
+    import re
+    doc = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "INTEGRATION.md")).read()
+    code = doc[doc.index("class Instance(C.Structure):"):doc.index("def check(rc):")]
+    ns = {"C": C}
+    exec(code, ns)
+    for name, ref in (("Desc", _lib.Desc), ("Info", _lib.PlanInfo), ("Call", _lib.Call), ("Instance", _lib.Instance)):
+        stub = ns[name]
+        assert C.sizeof(stub) == C.sizeof(ref), name
+        assert [f[0] for f in stub._fields_] == [f[0] for f in ref._fields_], name
+    assert re.search(r"Desc\(1, 1, 0, P, 1, K(, \d+){16}, inst,", doc)
