@@ -1251,7 +1251,10 @@ def main():
     traffic, traffic_tiles, valu = None, None, None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        entry = json.load(open(tpath)).get(f"{wl.name}{variant}/{name}")
+        tj = json.load(open(tpath))
+        # entries per kernel shape ("<workload>/<kernel>@<shape>"), or one entry with its shape inside
+        entry = (tj.get(f"{wl.name}{variant}/{name}@" + ",".join(map(str, kshape)))
+                 or tj.get(f"{wl.name}{variant}/{name}"))
         if entry and entry.get("shape") == kshape:
             traffic, traffic_tiles = entry["bytes"], entry["tiles"]
             vi = entry.get("valu_insts")

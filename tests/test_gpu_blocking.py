@@ -54,12 +54,11 @@ def _pair(shots, mode, blocking, tiles=(2, 2, 1, 4, 0, 0), K=7, snapshot=False, 
     return a, b
 
 
-@pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("mode", ["const6", "dfield", "field6"])
+# (2, 8, ·) is built for six constants only (the field modes' registers would spill)
+@pytest.mark.parametrize("mode,shape", [(m, s) for m in ("const6", "dfield", "field6") for s in SHAPES
+                                        if m == "const6" or s[:2] != (2, 8)])
 def test_blocked_forward_and_gradients_are_bitwise_single_step(mode, shape):
     V, RW, KS = shape
-    if mode != "const6" and (V, RW) == (2, 8):
-        pytest.skip("not built for field modes (register spill)")
     nx, ny, nt, h = 45, 150, 47, 0.2e-3
     rng = np.random.default_rng(7)
     shot = make_shot(nx, ny, nt, h, sources=_edge_sources(nx, ny, h, V, RW, KS),
@@ -136,14 +135,13 @@ def _adj_pair(shots, mode, ablk, K, snapshot=False, stride=1, **kw):
     return run_hip(sv((0, 0, 0)), **kw), run_hip(sv(ablk), **kw)
 
 
-@pytest.mark.parametrize("shape", ADJ_SHAPES)
-@pytest.mark.parametrize("mode", ["const6", "dfield"])
+# V = 2 and KS = 5 adjoint shapes are built for six constants only
+@pytest.mark.parametrize("mode,shape", [(m, s) for m in ("const6", "dfield") for s in ADJ_SHAPES
+                                        if m == "const6" or (s[0] == 1 and s[2] != 5)])
 def test_blocked_adjoint_matches_single_step_and_oracle(mode, shape):
     """Gradients of the time-blocked adjoint vs the single-step adjoint (same
     recurrence, gradient sums taken in another order) and vs the FP64 oracle;
     K = 7 leaves one-step launches (7 mod KS = 1 or 3) at segment ends."""
-    if mode == "dfield" and (shape[0] == 2 or shape[2] == 5):
-        pytest.skip("not built for the damage mode")
     from gpu_helpers import oracle_run
     from oracle import ernn_fp64 as orc
     V, RW, KS = shape
@@ -298,16 +296,15 @@ def test_receiver_line_and_shared_cells(mode):
 
 
 # ------------------------------------------------------ fused time-blocked backward
-@pytest.mark.parametrize("shape", [(1, 4, 2), (1, 4, 4), (1, 8, 4)])
-@pytest.mark.parametrize("mode", ["const6", "dfield"])
+# fused (1, 8, 4) is built for six constants only (the damage kernel would spill)
+@pytest.mark.parametrize("mode,shape", [("const6", (1, 4, 2)), ("const6", (1, 4, 4)), ("const6", (1, 8, 4)),
+                                        ("dfield", (1, 4, 2)), ("dfield", (1, 4, 4))])
 def test_fused_blocked_backward_matches_unfused_and_oracle(mode, shape):
     """Each launch runs KS reverse steps of segment s beside KS recompute steps
     of segment s - 1.  K = 7 with 47 steps leaves a short last segment (its
     adjoint launches cannot carry all of segment s - 1's recompute, which then
     runs alone) and one-step launches (7 mod KS); gradients vs the unfused
     single-step backward (same recurrence) and the FP64 oracle."""
-    if mode == "dfield" and shape[1] == 8:
-        pytest.skip("fused (1, 8, 4) is built for six constants only (the damage kernel would spill)")
     from gpu_helpers import oracle_run
     from oracle import ernn_fp64 as orc
     V, RW, KS = shape
