@@ -244,14 +244,22 @@ def cpu_baseline(torch, wl, budget_s):
     s = wl.survey.shots[0]
     threads = torch.get_num_threads()
 
+    import numpy as np
+    dfield = wl.survey.mode == "dfield"
+
     def run(nsteps):
-        C = [torch.tensor(float(c), requires_grad=True) for c in wl.C_true[0]]
         sig = torch.tensor(s.wavelet[:nsteps]).unsqueeze(0)
         spatial = torch.zeros(s.nx, s.ny)
         for (i, j, w, comp) in s.sources:
             spatial[i, j] += w
         t0 = time.perf_counter()
-        ux, uy = pt.forward(C, s.h, s.dt, nsteps, 1, sig, spatial, torch.tensor(s.rho), hoist_weights=True)
+        if dfield:   # the per-cell operator of DI:100-176 on C_p = D C0_p, gradient w.r.t. D
+            D = torch.tensor(np.asarray(wl.D_true, np.float32), requires_grad=True)
+            C = [(D * float(c)).unsqueeze(0) for c in wl.C_true[0]]
+        else:
+            C = [torch.tensor(float(c), requires_grad=True) for c in wl.C_true[0]]
+        ux, uy = pt.forward(C, s.h, s.dt, nsteps, 1, sig, spatial, torch.tensor(s.rho), hoist_weights=True,
+                            per_cell=dfield)
         (ux.square().sum() + uy.square().sum()).backward()
         return time.perf_counter() - t0
 
@@ -268,7 +276,8 @@ def cpu_baseline(torch, wl, budget_s):
         pass
     return {"value": round(rate, 5), "unit": "Gcell-steps/s (fwd+adjoint)", "cores": threads,
             "kind": "port", "cpu": cpu,
-            "sample": f"oracle/plain_torch.py forward + autograd backward, {s.nx}x{s.ny} grid, "
+            "sample": f"oracle/plain_torch.py forward + autograd backward"
+                      + (" (per-cell operator, dJ/dD)" if dfield else "") + f", {s.nx}x{s.ny} grid, "
                       f"{n} of {s.nt} steps, torch {threads} threads, {t:.1f} s"}
 
 
