@@ -64,6 +64,9 @@ BYTES_PER_CELL = {
     ("dfield", 0): 28, ("dfield", 1): 36,
     ("field6", 0): 48, ("field6", 1): 56,
 }
+# The reversed-u adjoint (reverse_u plans) also carries u backward: per step it
+# reads u^m, u^{m+1} and writes u^{m-1} on top of the damage adjoint's 36 B.
+REV_BYTES_PER_CELL = 36 + 24
 
 
 def log(msg):
@@ -86,6 +89,9 @@ def parse():
     ap.add_argument("--joint-c0", type=int, default=0,
                     help="damage-1024: also invert the six background constants (joint gradient; the "
                          "workload's own setting inverts D over known constants)")
+    ap.add_argument("--reverse-u", type=int, default=1,
+                    help="damage-1024 over known constants: 1 = the adjoint reconstructs u backward from the "
+                         "checkpoints (no bank, no recompute); 0 = banked L^m with segment recompute")
     ap.add_argument("--ckpt-every", type=int, default=0,
                     help="fix the checkpoint interval (profiling runs; 0 = the workload's memory budget)")
     ap.add_argument("--graph", type=int, default=1, help="replay passes as captured hipGraphs")
@@ -314,7 +320,8 @@ def time_instances(torch, ernn, wl, C_init, D_init, dev, iters):
     for b, s in enumerate(wl.survey.shots):
         one = ernn.Shot(**{**s.__dict__, "set": 0})
         sv1 = ernn.Survey([one], mode=wl.survey.mode, n_sets=1, ckpt_every=wl.survey.ckpt_every,
-                          use_graph=wl.survey.use_graph, fixed_c0=wl.survey.fixed_c0)
+                          use_graph=wl.survey.use_graph, fixed_c0=wl.survey.fixed_c0,
+                          reverse_u=wl.survey.reverse_u)
         m = ernn.ERNN(sv1, C_init[s.set:s.set + 1], D_init=D_init, device=dev, learn_constants=not sv1.fixed_c0)
         for k in range(iters + 1):
             if k == 1:
@@ -1081,6 +1088,10 @@ def main():
     variant = ""
     if args.joint_c0 and sv.fixed_c0:
         sv.fixed_c0, variant = False, "-joint"   # the profiles of this variant carry the suffix
+    if sv.fixed_c0 and args.reverse_u:
+        sv.reverse_u = True
+    elif sv.fixed_c0:
+        variant = "-banked"
     rng = np.random.default_rng(0)
     C_true = wl.C_true
     pert = 1.0 + rng.uniform(-0.2, 0.2, size=C_true.shape)
@@ -1114,7 +1125,7 @@ def main():
         log(f"tiles fixed: fwd V={vf} R={rf} B={bf}, adj V={va} R={ra} B={ba}")
     elif args.autotune:
         tune = tune_tiles(eng, params_true, field_true, model.wavelets, steps=max(20, min(100, 40000 // cells_k)),
-                          which=(0, 2) if eng.info.fuse else (0, 1))
+                          which=(0, 2) if eng.info.fuse else ((0,) if eng.reverse_u else (0, 1)))
         log("autotune: " + ", ".join(f"{('fwd', 'adj', 'fused')[w]} V={v} R={r} B={b} {ms * 1e3:.1f} us/step"
                                      for w, (v, r, b, ms, _) in tune.items()))
     if args.blocking not in ("auto", "0"):
@@ -1131,8 +1142,9 @@ def main():
     elif args.adj_blocking == "auto" and args.autotune and sv.mode != "field6":
         bv, brw, bks, bms, btimes = tune_blocking(eng, params_true, field_true, model.wavelets,
                                                   steps=max(16, min(96, 40000 // cells_k)), which=1)
-        log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step "
-            f"(one step per launch {btimes[(0, 0, 0)] * 1e3:.2f}; {_ranked(btimes)})")
+        one = btimes.get((0, 0, 0))
+        log(f"autotune: adjoint blocking (V, RW, KS) = ({bv}, {brw}, {bks}) {bms * 1e3:.2f} us/step ("
+            + (f"one step per launch {one * 1e3:.2f}; " if one else "") + f"{_ranked(btimes)})")
     if eng.info.fuse and eng.adj_blocking[2] > 1:
         if args.ff_rw != "auto":
             v, _, ks = eng.adj_blocking
@@ -1182,13 +1194,16 @@ def main():
     f = None if sv.mode == "const6" else model.damage.detach()
     eng.time_steps(0, 10, p, f, model.wavelets, ws, tr0, gr0)
     fwd_ms = eng.time_steps(0, nk, p, f, model.wavelets, ws, tr0, gr0) / nk
-    adj_ms = eng.time_steps(1, nk, p, f, model.wavelets, ws, tr0, gr0) / nk
+    adj_ms = None if eng.reverse_u else eng.time_steps(1, nk, p, f, model.wavelets, ws, tr0, gr0) / nk
     fused_ms = eng.time_steps(2, nk, p, f, model.wavelets, ws, tr0, gr0) / nk if eng.info.fuse else None
     KS = eng.blocking[2]
     blk_ms = blk_bank_ms = None
     nt, K = sv.nt_max, int(eng.info.ckpt_every)
     S = math.ceil(nt / K)
-    B = (S - 1) * K    # the forward pass banks from level B on (the last checkpoint segment)
+    rev = eng.reverse_u
+    # the forward pass banks from level B on (the last checkpoint segment); a
+    # reverse_u plan banks nothing
+    B = nt if rev else (S - 1) * K
     if KS > 1:   # per time-blocked launch of KS steps
         # the first launches of the forward pass (no banking while L KS <= B),
         # then the whole pass, whose launches past B also bank: their average
@@ -1197,7 +1212,9 @@ def main():
         eng.time_steps(3, 2, p, f, model.wavelets, ws, tr0, gr0)
         blk_ms = eng.time_steps(3, L, p, f, model.wavelets, ws, tr0, gr0) / L
         n_fwd, n_nb = math.ceil(nt / KS), B // KS
-        if B >= 2 * KS and n_fwd > n_nb:
+        if rev:   # no launch banks
+            blk_bank_ms = blk_ms
+        elif B >= 2 * KS and n_fwd > n_nb:
             t_all = eng.time_steps(3, n_fwd, p, f, model.wavelets, ws, tr0, gr0)
             blk_bank_ms = max(t_all - n_nb * blk_ms, 0.0) / (n_fwd - n_nb)
         else:   # one checkpoint segment: every launch of the pass banks
@@ -1221,11 +1238,12 @@ def main():
     # last segment's states; a fixed_c0 plan runs one step past each
     # segment); the fused schedule recomputes the last one alone
     seg = K + (1 if sv.fixed_c0 else 0)
-    n_rec = K if eng.info.fuse else (S - 1) * seg
+    n_rec = 0 if rev else (K if eng.info.fuse else (S - 1) * seg)
     blk_split = None
     if KS > 1:
-        n_fwd, n_nb = math.ceil(nt / KS), min(B // KS, math.ceil(nt / KS))
-        n_rc = math.ceil(K / KS) if eng.info.fuse else (S - 1) * math.ceil(seg / KS)
+        n_fwd = math.ceil(nt / KS)
+        n_nb = n_fwd if rev else min(B // KS, n_fwd)
+        n_rc = 0 if rev else (math.ceil(K / KS) if eng.info.fuse else (S - 1) * math.ceil(seg / KS))
         n_bank = n_fwd - n_nb + n_rc
         total = n_nb * blk_ms + n_bank * blk_bank_ms
         n_blk = n_fwd + n_rc
@@ -1241,8 +1259,9 @@ def main():
     elif eng.info.fuse:   # backward: ~nt fused adjoint + recompute launches (+ the last segment alone)
         kern["fused_step_kernel"] = (fused_ms, nt, ba, bf)
     elif AKS > 1:
-        kern["adj_block_kernel"] = (ablk_ms, sum(math.ceil(min(K, nt - s * K) / AKS)
-                                                for s in range(math.ceil(nt / K))), ba * AKS, 0)
+        kern["rev_block_kernel" if rev else "adj_block_kernel"] = (
+            ablk_ms, sum(math.ceil(min(K, nt - s * K) / AKS) for s in range(math.ceil(nt / K))),
+            (REV_BYTES_PER_CELL if rev else ba) * AKS, 0)
     else:
         kern["adj_step_kernel"] = (adj_ms, nt, ba, 0)
     name = max(kern, key=lambda k: kern[k][0] * kern[k][1])
@@ -1256,6 +1275,7 @@ def main():
     # now (time-blocked forward: blocking; time-blocked adjoint: adj_blocking;
     # fused time-blocked: adj_blocking + recompute rows; one-step: the tiles).
     kshape = {"fwd_block_kernel": list(eng.blocking), "adj_block_kernel": list(eng.adj_blocking),
+              "rev_block_kernel": list(eng.adj_blocking),
               "fused_block_kernel": list(eng.adj_blocking) + [eng.ff_rw]}.get(name, list(eng.tiles))
     traffic, traffic_tiles, valu = None, None, None
     tpath = os.path.join(REPO, "profiles", "traffic.json")
@@ -1283,19 +1303,20 @@ def main():
                 "traffic_tiles": traffic_tiles, "kernel_shape": kshape, "valu": valu,
                 "bytes_per_cell": bpc, "recompute_bytes_per_cell": rbpc, "cells_per_launch": cells,
                 "avg_launch_us": round(ms * 1e3, 3), "launches_per_iter": launches,
-                "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3),
+                "fwd_step_us": round(fwd_ms * 1e3, 3), "adj_step_us": round(adj_ms * 1e3, 3) if adj_ms else None,
                 "fused_step_us": round(fused_ms * 1e3, 3) if fused_ms else None,
                 "fwd_block_us_per_step": round(blk_ms / KS * 1e3, 3) if blk_ms else None,
                 "adj_block_us_per_step": round(ablk_ms / AKS * 1e3, 3) if ablk_ms else None,
                 "fwd_block_launches": blk_split,
                 "kernel_ms_per_iter": {k: round(v[0] * v[1], 2) for k, v in kern.items()},
                 # the whole iteration against the algorithmic bytes of one forward and one adjoint
-                # step per cell (recompute excluded): the HBM bound of the fwd + adjoint rate
+                # step per cell with u^m at hand (recompute and u's reversal excluded): the HBM
+                # bound of the fwd + adjoint rate, the same for every schedule
                 "iteration": {"bytes_per_cell_step": bf + ba,
                               "achieved": round((bf + ba) * sv.cell_steps() / dt_iter / 1e9, 1),
                               "frac": round((bf + ba) * sv.cell_steps() / dt_iter / 1e9 / HBM_PEAK_GBS, 4),
                               "bound_gcell_steps": round(HBM_PEAK_GBS / (bf + ba), 1)}}
-    log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3:.2f} us, "
+    log(f"kernels: fwd {fwd_ms * 1e3:.2f} us, adj {adj_ms * 1e3 if adj_ms else float('nan'):.2f} us, "
         f"fused {fused_ms * 1e3 if fused_ms else float('nan'):.2f} us per launch, time-blocked fwd "
         f"{blk_ms / KS * 1e3 if blk_ms else float('nan'):.2f} us per step, time-blocked adj "
         f"{ablk_ms / AKS * 1e3 if ablk_ms else float('nan'):.2f} us per step; "
@@ -1332,7 +1353,7 @@ def main():
         log(f"cpu FP64 oracle: {cpu['fp64_oracle']['value']} Gcell-steps/s on one core")
 
     hbm = None
-    if args.hbm_run and sv.ckpt_budget:
+    if args.hbm_run and sv.ckpt_budget and not sv.reverse_u:   # reverse_u: the interval is not memory-bound
         hbm = time_hbm_budget(torch, ernn, sv, eng, C_init, D_init, obs, dev, args)
 
     s0 = sv.shots[0]
@@ -1343,7 +1364,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name + variant, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
                    "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
-                   "mode": sv.mode, "fixed_c0": bool(sv.fixed_c0), "ckpt_every": int(eng.info.ckpt_every),
+                   "mode": sv.mode, "fixed_c0": bool(sv.fixed_c0), "reverse_u": bool(sv.reverse_u),
+                   "ckpt_every": int(eng.info.ckpt_every),
                    "ckpt_cap_bytes": sv.ckpt_budget, "ckpt_state_bytes": state_bytes(sv, int(eng.info.ckpt_every)),
                    "graph": bool(sv.use_graph),
                    "fused_backward": bool(eng.info.fuse),

@@ -110,6 +110,9 @@ struct StepArgs {
   int record;              // forward: write traces on recorded steps
   int snapshot;            // traces are whole fields
   int rows;                // R
+  // reversed-u plans (forward): the step that reaches an instance's last level
+  // I.nt also stores it into state slot fin_base + ceil(I.nt / fin_k) (fin_k = 0: off)
+  int fin_base, fin_k;
 };
 
 // A launch that runs one adjoint step and, on the same tiles, one recompute
@@ -162,6 +165,7 @@ struct BlockArgs {
   int record;
   int snapshot;
   int rw, ksmax;                 // compiled shape: rows per wavefront, steps per launch
+  int fin_base, fin_k;           // as StepArgs: an instance's last level also into slot fin_base + ceil(I.nt / fin_k)
 };
 
 // Time-blocked adjoint (constant and damage modes): one launch runs the
@@ -169,6 +173,7 @@ struct BlockArgs {
 struct BlockAdjArgs {
   const InstDev* inst;
   const BlockDev* blocks;        // receiver lists cover the whole region (injection)
+  const SrcEv* src;              // the region's point forces (the reversed-u adjoint)
   const RecEv* rec;
   const CallDev* call;
   int64_t partial_off;           // [blocks][blk_waves(RW)][kPartials] fp64 partials
@@ -180,6 +185,12 @@ struct BlockAdjArgs {
   int stride;
   int snapshot;
   int rw, ksmax;
+  // reversed-u launches (rev_block_kernel): u runs backward beside lam.  The
+  // launch that holds an instance's first level of the segment, min(seg_top,
+  // I.nt), loads (u, c, v) of that level from checkpoint slot u_ck; later
+  // launches read (u^{m0}, c, v^{m0+1}) from u ring slot u_in.  The launch
+  // writes (u, c, v) of its level m0 - ks, v of the level above, to u_out.
+  int u_in, u_out, u_ck, seg_top;
 };
 
 // Fused time-blocked backward: a time-blocked adjoint launch of segment s and

@@ -234,6 +234,16 @@ __device__ __forceinline__ void advance(float& u, float& c, float& v, float d) {
   u = t;
 }
 
+// The compensated sum of advance run backward: (u, c) of the level below,
+// u' + c' = u + c - v, with v the increment that produced this level.
+__device__ __forceinline__ void retreat(float& u, float& c, float v) {
+#pragma clang fp contract(off)
+  const float y = c - v;
+  const float t = u + y;
+  c = y - (t - u);
+  u = t;
+}
+
 // --------------------------------------------------------------------------
 // Buffer addressing (time-blocked kernels).  A padded field is one buffer
 // resource (64-bit base in scalar registers); an access takes the lane's
@@ -402,6 +412,17 @@ __device__ __forceinline__ void fwd_tile(const StepArgs& a, const GCall& C, cons
     store_vec<V>(at(ocy, so), cy[r]);
     store_vec<V>(at(ovx, so), vx[r]);
     store_vec<V>(at(ovy, so), vy[r]);
+    if (a.fin_k > 0 && n + 1 == I.nt) {   // the instance's last level, for a reversed-u adjoint (tile-uniform)
+      const int fin = a.fin_base + (I.nt + a.fin_k - 1) / a.fin_k;
+      if (fin != a.s_out) {
+        store_vec<V>(at(state_field(C, I, fin, 0), so), nx_);
+        store_vec<V>(at(state_field(C, I, fin, 1), so), ny_);
+        store_vec<V>(at(state_field(C, I, fin, 2), so), cx[r]);
+        store_vec<V>(at(state_field(C, I, fin, 3), so), cy[r]);
+        store_vec<V>(at(state_field(C, I, fin, 4), so), vx[r]);
+        store_vec<V>(at(state_field(C, I, fin, 5), so), vy[r]);
+      }
+    }
     if (a.bank >= 0) {   // u^{n+1}; kDOnly: (dt^2/rho) A(C0) u^n
       if constexpr (MODE == kDOnly) {
         store_vec<V>(at(bx, so), lbx);
@@ -1116,9 +1137,31 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
       }
 
     // store the output cells of the levels the schedule keeps (kDOnly: the
-    // bank took the operator value above)
+    // bank took the operator value above); a reversed-u plan also keeps each
+    // instance's last level in its checkpoint slot (block-uniform)
     const int st_slot = a.st_store[s];
     const int bk_slot = MODE == kDOnly ? -1 : a.bank_store[s];
+    const int fin_slot = (a.fin_k > 0 && a.n0 + s + 1 == I.nt) ? a.fin_base + (I.nt + a.fin_k - 1) / a.fin_k : -1;
+    if (fin_slot >= 0 && fin_slot != st_slot && bd.col_own) {
+      Rsrc ro[kStateFields];
+#pragma unroll
+      for (int q = 0; q < kStateFields; ++q) ro[q] = field_rsrc(state_field(C, I, fin_slot, q), I.field);
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        if (r < bd.own_lo || r >= bd.own_hi) continue;
+        const uint32_t so = bd.srow(I, r);
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          if (!bd.col_in[e]) continue;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            bstore(u[q][r][e], ro[q], bd.vcol[e], so);
+            bstore(c[q][r][e], ro[2 + q], bd.vcol[e], so);
+            bstore(v[q][r][e], ro[4 + q], bd.vcol[e], so);
+          }
+        }
+      }
+    }
     if (st_slot >= 0 || bk_slot >= 0) {
       if (bd.col_own) {
         if (st_slot >= 0) {
@@ -1568,6 +1611,289 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 }
 
 // --------------------------------------------------------------------------
+// Reversed-u time-blocked adjoint (damage field over known constants, plans
+// with reverse_u).  A launch runs the reverse steps m = m0 .. m0 - ks + 1 of
+// lam as adj_block_body<kDOnly> does and, beside it, runs the forward
+// recurrence backward in time instead of reading a recompute bank.  The
+// state (u^m, c^m, v^{m+1}) of a cell gives, per step:
+//   L^m = (dt^2/rho) A(C0) u^m          the forward's operator value at step m
+//   v^m = v^{m+1} - (D L^m + (dt^2/rho) f^m)   (forward: v^{m+1} = v^m + d^m)
+//   u^{m-1} + c^{m-1} = u^m + c^m - v^m      (the forward's compensated sum, reversed)
+// and dJ/dD takes lam^{m+1} . L^m.  A segment starts from the checkpoint of
+// its top level (u, c, v of that level, v^T taken as is).  The leapfrog
+// recurrence is time-symmetric, so the reversed states stay within rounding
+// of the forward's; u's halo shrinks with lam's, one cell per step.
+// --------------------------------------------------------------------------
+template <int V, int RW, int KS>
+__device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) {
+#pragma clang fp contract(fast)
+  const auto& a = *ap;
+  constexpr int kCols = kWave * V;
+  constexpr int WB = blk_waves(RW);
+  static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
+  // band-edge rows: [buffer][wavefront][top/bottom][D lam_x, D lam_y, u_x, u_y][column]
+  __shared__ float xch[2][WB][2][4][kCols];
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x >> 6;
+  const int chunk = (a.n_blocks + 7) >> 3;
+  const int t = (bid & 7) * chunk + (bid >> 3);
+  const GCall C = global_call(*a.call);
+  const BlockDev bl = a.blocks[max(min(t, a.n_blocks - 1), 0)];
+  int n_blocks = a.n_blocks;
+  asm("" : "+s"(n_blocks) : "s"((uint64_t)C.ws), "s"(bl.I.nt), "s"(bl.I.field), "s"(bl.I.pool_off),
+      "s"(bl.oi0), "s"(a.m0), "s"(a.ks), "s"(a.s_in));
+  if (t >= n_blocks) return;
+  const InstDev I = bl.I;
+  if (a.m0 - a.ks + 1 > I.nt) return;   // this instance's reverse recurrence has not started
+  // the instance's first level in this segment; the launch holding it loads u
+  // from the segment's checkpoint slot (block-uniform)
+  const int ustart = min(a.seg_top, I.nt);
+  const bool from_ck = a.m0 >= ustart;
+
+  const Band<V, RW, KS, WB> bd(bl, I, w, lane);
+  const int gi0 = bd.gi0;
+  const float kdt = (float)I.gscale;
+  float c0[6];
+#pragma unroll
+  for (int p = 0; p < 6; ++p) c0[p] = C.params[bl.set * 6 + p];
+  // (dt^2/rho) A(C0): on D lam for the adjoint, on u for L (the forward's coefficients)
+  const OpCoef kl = op_coef(c0, I.gscale * (double)I.hinv2);
+
+  // ---- loads: lam level m0 + 1 (lam, c, w), u (u, c, v) and D
+  Rsrc rin[kStateFields], uin[kStateFields];
+  const int uslot = from_ck ? a.u_ck : a.u_in;
+#pragma unroll
+  for (int q = 0; q < kStateFields; ++q) {
+    rin[q] = field_rsrc(state_field(C, I, a.s_in, q), I.field);
+    uin[q] = field_rsrc(state_field(C, I, uslot, q), I.field);
+  }
+  const Rsrc rD = field_rsrc(C.ws + I.coef_off, I.field);
+  float l[2][RW][V], c[2][RW][V], wv[2][RW][V], dd[RW][V];
+  float uu[2][RW][V], cu[2][RW][V], vu[2][RW][V];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const uint32_t so = bd.srow(I, r);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        l[q][r][e] = bload(rin[q], bd.vcol[e], so);
+        c[q][r][e] = bload(rin[2 + q], bd.vcol[e], so);
+        wv[q][r][e] = bload(rin[4 + q], bd.vcol[e], so);
+        uu[q][r][e] = bload(uin[q], bd.vcol[e], so);
+        cu[q][r][e] = bload(uin[2 + q], bd.vcol[e], so);
+        vu[q][r][e] = bload(uin[4 + q], bd.vcol[e], so);
+      }
+      dd[r][e] = bload(rD, bd.vcol[e], so);
+    }
+  }
+  float gacc[RW][V];   // per-cell damage gradient of this launch (FP32 over at most KS steps)
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int e = 0; e < V; ++e) gacc[r][e] = 0.f;
+
+  auto publish = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < V; ++e)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        xch[buf][w][0][q][lane * V + e] = dd[0][e] * l[q][0][e];
+        xch[buf][w][1][q][lane * V + e] = dd[RW - 1][e] * l[q][RW - 1][e];
+        xch[buf][w][0][2 + q][lane * V + e] = uu[q][0][e];
+        xch[buf][w][1][2 + q][lane * V + e] = uu[q][RW - 1][e];
+      }
+  };
+  publish(0);
+  __syncthreads();
+  const int nf = I.n_frames;
+
+  for (int s = 0; s < a.ks; ++s) {
+    const int m = a.m0 - s;
+    const int buf = s & 1;
+    if (m <= I.nt) {   // block-uniform: an instance with fewer steps starts later
+      const bool first = from_ck && m == ustart;   // v^m is the checkpoint's own (block-uniform)
+      // ---- stencil rows of D lam and of u: own rows and the band-edge rows
+      Row<V> xr[RW + 2], yr[RW + 2], pxr[RW + 2], pyr[RW + 2];
+      {
+        float e0[4][V], e1[4][V];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            e0[q][e] = (w > 0) ? xch[buf][w - 1][1][q][lane * V + e] : 0.f;
+            e1[q][e] = (w < WB - 1) ? xch[buf][w + 1][0][q][lane * V + e] : 0.f;
+          }
+        xr[0] = make_row_z<V>(e0[0]);
+        yr[0] = make_row_z<V>(e0[1]);
+        pxr[0] = make_row_z<V>(e0[2]);
+        pyr[0] = make_row_z<V>(e0[3]);
+        xr[RW + 1] = make_row_z<V>(e1[0]);
+        yr[RW + 1] = make_row_z<V>(e1[1]);
+        pxr[RW + 1] = make_row_z<V>(e1[2]);
+        pyr[RW + 1] = make_row_z<V>(e1[3]);
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          float rx[V], ry[V];
+#pragma unroll
+          for (int e = 0; e < V; ++e) { rx[e] = dd[r][e] * l[0][r][e]; ry[e] = dd[r][e] * l[1][r][e]; }
+          xr[r + 1] = make_row_z<V>(rx);
+          yr[r + 1] = make_row_z<V>(ry);
+          pxr[r + 1] = make_row_z<V>(uu[0][r]);
+          pyr[r + 1] = make_row_z<V>(uu[1][r]);
+        }
+      }
+
+      float dx[RW][V], dy[RW][V], gx[RW][V], gy[RW][V];
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const int x = e + 1;
+          const UDeriv dl = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], x);
+          apply_op_sum(kl, dl, dx[r][e], dy[r][e]);
+          const UDeriv du = uderivs<V>(pxr[r], pxr[r + 1], pxr[r + 2], pyr[r], pyr[r + 1], pyr[r + 2], x);
+          float lx, ly;
+          apply_op_sum(kl, du, lx, ly);   // L^m, the forward's operator value
+          // dJ/dD on the band rows of the output tile; the last level's term (lam^{nt+1} = 0) is skipped
+          if (r >= bd.own_lo && r < bd.own_hi && m < I.nt)
+            gacc[r][e] += __builtin_fmaf(l[0][r][e], lx, l[1][r][e] * ly);
+          gx[r][e] = dd[r][e] * lx;   // D L^m: the forward's increment before its force
+          gy[r][e] = dd[r][e] * ly;
+        }
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) asm volatile("" : "+v"(gacc[r][e]));
+
+      if (!first) {
+        // point forces of step m, as the forward added them: d^m = fma(dt^2/rho, s_m w, D L^m)
+        const float s_m = ld_const(C.wav + I.wav_off + m);
+        for (int sp = bl.src_begin; sp < bl.src_end; ++sp) {
+          const SrcEv ev = ld_src(a.src + sp);
+          const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
+          if (rr < 0 || rr >= RW || lane != lc / V) continue;
+          const float f = s_m * ev.w;
+#pragma unroll
+          for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+              if (r == rr && e == lc % V) {
+                if (ev.comp == 0) gx[r][e] = __builtin_fmaf(kdt, f, gx[r][e]);
+                else gy[r][e] = __builtin_fmaf(kdt, f, gy[r][e]);
+              }
+            }
+        }
+      }
+
+      // residual injection on recorded steps m = kP + 1 (the whole region; walked receiver list)
+      if (((m - 1) % a.stride == 0) && ((m - 1) / a.stride < nf)) {
+        const int k = (m - 1) / a.stride;
+        if (a.snapshot) {
+#pragma unroll
+          for (int r = 0; r < RW; ++r) {
+            if (!bd.row_in(I, r)) continue;
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              if (bd.col_in[e]) {
+                const gfloat* g = C.gtr + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + bd.gj0 + e;
+                dx[r][e] += g[0];
+                dy[r][e] += g[(int64_t)nf * I.nx * I.ny];
+              }
+          }
+        } else {
+          for (int rp = bl.rec_begin; rp < bl.rec_end; ++rp) {
+            const RecEv ev = ld_rec(a.rec + rp);
+            const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
+            if (rr < 0 || rr >= RW || lane != lc / V) continue;
+            const gfloat* g = C.gtr + I.trace_off + (int64_t)k * I.n_rec + ev.rec;
+            const float vgx = ld_const(g), vgy = ld_const(g + (int64_t)nf * I.n_rec);
+#pragma unroll
+            for (int r = 0; r < RW; ++r)
+#pragma unroll
+              for (int e = 0; e < V; ++e)
+                if (r == rr && e == lc % V) {
+                  dx[r][e] += vgx;
+                  dy[r][e] += vgy;
+                }
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          advance(l[0][r][e], c[0][r][e], wv[0][r][e], dx[r][e]);
+          advance(l[1][r][e], c[1][r][e], wv[1][r][e], dy[r][e]);
+          if (!first) {   // v^m = v^{m+1} - d^m
+            vu[0][r][e] = vu[0][r][e] - gx[r][e];
+            vu[1][r][e] = vu[1][r][e] - gy[r][e];
+          }
+          retreat(uu[0][r][e], cu[0][r][e], vu[0][r][e]);
+          retreat(uu[1][r][e], cu[1][r][e], vu[1][r][e]);
+        }
+    }
+    if (s + 1 < a.ks) {
+      publish(buf ^ 1);
+      __syncthreads();
+    }
+  }
+
+  // ---- the launch's last levels on the output tile: lam, the reversed u, and
+  // the FP64 per-cell accumulators (read first, every row at once)
+  asm volatile("" ::: "memory");
+  gdouble* acc64 = reinterpret_cast<gdouble*>(C.ws) + I.acc64_off;
+  double acc_in[RW][V];
+#pragma unroll
+  for (int r = 0; r < RW; ++r)
+#pragma unroll
+    for (int e = 0; e < V; ++e) acc_in[r][e] = 0.0;
+  if (bd.col_own) {
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (r < bd.own_lo || r >= bd.own_hi) continue;
+      const uint32_t so = bd.srow(I, r) / (uint32_t)sizeof(float);
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+        if (bd.col_in[e]) acc_in[r][e] = acc64[so + bd.vcol[e] / (uint32_t)sizeof(float)];
+    }
+    Rsrc ro[kStateFields], uo[kStateFields];
+#pragma unroll
+    for (int q = 0; q < kStateFields; ++q) {
+      ro[q] = field_rsrc(state_field(C, I, a.s_out, q), I.field);
+      uo[q] = field_rsrc(state_field(C, I, a.u_out, q), I.field);
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (r < bd.own_lo || r >= bd.own_hi) continue;
+      const uint32_t so = bd.srow(I, r);
+#pragma unroll
+      for (int e = 0; e < V; ++e) {
+        if (!bd.col_in[e]) continue;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          bstore(l[q][r][e], ro[q], bd.vcol[e], so);
+          bstore(c[q][r][e], ro[2 + q], bd.vcol[e], so);
+          bstore(wv[q][r][e], ro[4 + q], bd.vcol[e], so);
+          bstore(uu[q][r][e], uo[q], bd.vcol[e], so);
+          bstore(cu[q][r][e], uo[2 + q], bd.vcol[e], so);
+          bstore(vu[q][r][e], uo[4 + q], bd.vcol[e], so);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (r < bd.own_lo || r >= bd.own_hi) continue;
+      const uint32_t so = bd.srow(I, r) / (uint32_t)sizeof(float);
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+        if (bd.col_in[e]) acc64[so + bd.vcol[e] / (uint32_t)sizeof(float)] = acc_in[r][e] + (double)gacc[r][e];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
 // Six-field time-blocked adjoint (per-cell C_p: the operator of DI:151-163
 // transposed).  The blocks, bands, state layout and schedule are those of
 // adj_block_body; what the stencil reads are the six weighted fields of
@@ -1818,6 +2144,12 @@ __global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockA
     adj_block_body6<RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
   else
     adj_block_body<MODE, V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
+}
+
+// Reversed-u adjoint (damage field over known constants; rev_block_body).
+template <int V, int RW, int KS>
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) rev_block_kernel(BlockAdjArgs) {
+  rev_block_body<V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
 
 // the argument is read in place through the kernarg segment (offset 0)
@@ -2086,6 +2418,21 @@ static hipError_t launch_adj_block_m(const BlockAdjArgs& a, int V, int grid, hip
   return hipErrorInvalidValue;
 }
 #undef ERNN_ADJ_BLOCK_CASE
+
+// Reversed-u adjoint shapes (rev_block_ok in ernn_plan.cpp): V = 1, RW in
+// {4, 6, 8} (lam and u both held in registers), KS in {2, 4}.
+#define ERNN_REV_BLOCK_CASES(X) X(1, 4, 2) X(1, 4, 4) X(1, 6, 2) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4)
+#define ERNN_REV_BLOCK_CASE(v_, r_, k_)                                                             \
+  if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
+    hipLaunchKernelGGL((rev_block_kernel<v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
+    return hipGetLastError();                                                                      \
+  }
+hipError_t launch_rev_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
+  if (a.mode != kDOnly) return hipErrorInvalidValue;
+  ERNN_REV_BLOCK_CASES(ERNN_REV_BLOCK_CASE)
+  return hipErrorInvalidValue;
+}
+#undef ERNN_REV_BLOCK_CASE
 
 hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
   switch (a.mode) {

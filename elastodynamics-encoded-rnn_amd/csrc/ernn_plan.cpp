@@ -46,6 +46,7 @@ hipError_t launch_adj_step(const StepArgs& a, int V, int blocks, bool loop, hipS
 hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t st);
 hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st);
 hipError_t launch_adj_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st);
+hipError_t launch_rev_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st);
 hipError_t launch_fused_block(const FusedBlockArgs& fa, int V, int grid, hipStream_t st);
 hipError_t launch_set_call(CallDev* dst, const CallDev& v, hipStream_t st);
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t st);
@@ -140,6 +141,10 @@ bool fused_block_ok(int mode, int V, int RW, int KS) {
 bool fused_ff_ok(int V, int RW, int KS, int RWF) {
   return RWF == RW || (V == 1 && RW == 4 && RWF == 8 && (KS == 2 || KS == 4));
 }
+// Reversed-u adjoint shapes (ERNN_REV_BLOCK_CASES): V = 1, RW in {4, 6, 8}, KS in {2, 4}.
+bool rev_block_ok(int V, int RW, int KS) {
+  return V == 1 && (RW == 4 || RW == 6 || RW == 8) && (KS == 2 || KS == 4);
+}
 bool ablk_ok(int mode, bool fuse, int V, int RW, int KS) {
   return fuse ? fused_block_ok(mode, V, RW, KS) : adj_block_ok(mode, V, RW, KS);
 }
@@ -184,6 +189,11 @@ struct Plan {
   // L^top), and the kernels run in mode kDOnly
   bool lbank = false;
   int kmode = 0;   // kernel mode (ParamMode)
+  // reverse_u (with fixed_c0): no bank; the adjoint runs u backward from the
+  // checkpoints (rev_block_kernel).  Slots: checkpoint S holds the last level,
+  // and each instance's own last level goes to checkpoint ceil(nt_b / K);
+  // two more ring slots carry the reversed u between launches.
+  bool rtr = false;
   std::vector<ernn_instance_t> inst_h;
   std::vector<ernn_source_t> src_h;
   std::vector<ernn_receiver_t> rec_h;
@@ -221,15 +231,17 @@ struct Plan {
   // of a segment s with bank_of(s) = b.
   int banks() const { return fuse ? 2 : 1; }
   int ring(int r) const { return r; }
-  int ck(int s) const { return 2 + s; }
-  int lam(int r) const { return 2 + S + r; }
-  int n_state() const { return 4 + S; }
-  int n_bank() const { return banks() * K; }
+  int ck(int s) const { return 2 + s; }   // s <= S with rtr (checkpoint S: the last level)
+  int lam(int r) const { return 2 + S + (rtr ? 1 : 0) + r; }
+  int ur(int r) const { return 4 + S + 1 + r; }   // rtr only
+  int n_state() const { return 4 + S + (rtr ? 3 : 0); }
+  int n_bank() const { return rtr ? 0 : banks() * K; }
   int bank_of(int s) const { return fuse ? (s & 1) : 0; }
   int bank(int s, int m) const { return bank_of(s) * K + (m - s * K - 1); }   // u^m, sK < m <= top
   bool is_ck(int m) const { return m % K == 0 && m / K < S; }
   int other_ring(int q) const { return q == ring(0) ? ring(1) : ring(0); }
   int other_lam(int q) const { return q == lam(0) ? lam(1) : lam(0); }
+  int other_ur(int q) const { return q == ur(0) ? ur(1) : ur(0); }
   int seg_top(int s) const { return std::min((s + 1) * K, nt_max); }
 };
 
@@ -495,13 +507,15 @@ int set_fwd_blocking(Plan* p, int V, int RW, int KS) {
 // wavefront (0: the adjoint's RW).  Both tables are built first and replace
 // the plan's together, so a failure leaves the plan as it was.
 int set_adj_blocking(Plan* p, int V, int RW, int KS, int RWF) {
+  if (p->rtr && (KS <= 1 || !rev_block_ok(V, RW, KS)))
+    return fail(ERNN_E_INVALID, "a reverse_u plan runs the reversed adjoint: (1, RW, KS), RW in {4, 6, 8}, KS in {2, 4}");
   if (KS <= 1) {
     free_blocking(p->blk_adj);
     free_blocking(p->blk_ff);
     p->ff_rw = 0;
     return ERNN_OK;
   }
-  if (!ablk_ok(p->mode, p->fuse, V, RW, KS)) return fail(ERNN_E_INVALID, kAblkShapes);
+  if (!p->rtr && !ablk_ok(p->mode, p->fuse, V, RW, KS)) return fail(ERNN_E_INVALID, kAblkShapes);
   const int rwf = RWF ? RWF : RW;
   if (p->fuse && !fused_ff_ok(V, RW, KS, rwf))
     return fail(ERNN_E_INVALID, "fused recompute rows per wavefront: the adjoint's RW, or 8 beside RW = 4 with "
@@ -561,12 +575,17 @@ int validate(const ernn_desc_t* d) {
     return fail(ERNN_E_INVALID, "time-blocked forward shape (V, RW, KS) must have V in {1,2}, RW in {4,8}, "
                                 "KS in {2,4,8}, 4 RW > 2 KS (damage field / six fields: not V = 2 with RW = 8), "
                                 "or be (1, 8, 5)");
-  if (d->ablk_ks > 1 && !ablk_ok(d->param_mode, d->fuse != 0, d->ablk_v, d->ablk_rw, d->ablk_ks))
+  if (d->ablk_ks > 1 && !d->reverse_u && !ablk_ok(d->param_mode, d->fuse != 0, d->ablk_v, d->ablk_rw, d->ablk_ks))
     return fail(ERNN_E_INVALID, kAblkShapes);
   if (d->fixed_c0 != 0 && d->fixed_c0 != 1) return fail(ERNN_E_INVALID, "fixed_c0 must be 0 or 1");
   if (d->fixed_c0 && d->param_mode != ERNN_PARAM_DFIELD)
     return fail(ERNN_E_INVALID, "fixed_c0 applies to the damage-field mode (ERNN_PARAM_DFIELD) only");
   if (d->fixed_c0 && d->fuse) return fail(ERNN_E_INVALID, "fixed_c0 needs fuse = 0");
+  if (d->reverse_u != 0 && d->reverse_u != 1) return fail(ERNN_E_INVALID, "reverse_u must be 0 or 1");
+  if (d->reverse_u && !d->fixed_c0) return fail(ERNN_E_INVALID, "reverse_u needs fixed_c0 = 1");
+  if (d->reverse_u && (d->ablk_ks <= 1 || !rev_block_ok(d->ablk_v, d->ablk_rw, d->ablk_ks)))
+    return fail(ERNN_E_INVALID, "reverse_u needs a time-blocked adjoint of shape (1, RW, KS), RW in {4, 6, 8}, "
+                                "KS in {2, 4}");
   if (d->ablk_ff_rw != 0 && (d->ablk_ks <= 1 || !d->fuse || !fused_ff_ok(d->ablk_v, d->ablk_rw, d->ablk_ks, d->ablk_ff_rw)))
     return fail(ERNN_E_INVALID, "ablk_ff_rw: 0, or on a fuse plan with a time-blocked adjoint the adjoint's RW, or 8 "
                                 "beside RW = 4 with KS in {2,4}");
@@ -641,6 +660,7 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   p->use_graph = d->use_graph != 0;
   p->fuse = d->fuse != 0;
   p->lbank = d->fixed_c0 == 1 && p->mode == ERNN_PARAM_DFIELD;
+  p->rtr = p->lbank && d->reverse_u == 1;
   p->kmode = p->lbank ? kDOnly : p->mode;
   p->n_coef = p->mode == ERNN_PARAM_DFIELD ? 1 : (p->mode == ERNN_PARAM_FIELD6 ? 6 : 0);
   p->n_acc = p->n_coef;
@@ -802,6 +822,8 @@ StepArgs base_args(const Plan* p, const Tiling& t) {
   a.stride = p->P;
   a.snapshot = p->snapshot;
   a.rows = t.R;
+  a.fin_base = p->rtr ? p->ck(0) : 0;   // reversed u starts each segment from a checkpoint
+  a.fin_k = p->rtr ? p->K : 0;
   return a;
 }
 
@@ -827,6 +849,8 @@ BlockArgs block_args(const Plan* p, const BlockTiling& bt) {
   a.snapshot = p->snapshot;
   a.rw = bt.RW;
   a.ksmax = bt.KS;
+  a.fin_base = p->rtr ? p->ck(0) : 0;
+  a.fin_k = p->rtr ? p->K : 0;
   return a;
 }
 BlockArgs block_args(const Plan* p) { return block_args(p, p->blk); }
@@ -847,7 +871,7 @@ void blk_forward_args(const Plan* p, BlockArgs& a, int n0, int end, int* in) {
     if (p->is_ck(m)) a.st_store[j] = p->ck(m / p->K);
     else if (j == a.ks - 1) a.st_store[j] = p->other_ring(*in);
     const int lev = p->lbank ? m - 1 : m;   // the level this step banks
-    if (lev > B) a.bank_store[j] = p->bank(p->S - 1, lev);
+    if (lev > B && !p->rtr) a.bank_store[j] = p->bank(p->S - 1, lev);
   }
   *in = a.st_store[a.ks - 1];
 }
@@ -885,6 +909,7 @@ BlockAdjArgs block_adj_args(const Plan* p) {
   BlockAdjArgs a{};
   a.inst = p->d_inst;
   a.blocks = p->blk_adj.d_blocks;
+  a.src = p->blk_adj.d_src;
   a.rec = p->blk_adj.d_rec;
   a.call = p->d_call;
   a.partial_off = p->partial_off;
@@ -908,6 +933,23 @@ void blk_adjoint_args(const Plan* p, BlockAdjArgs& a, int s, int m0, int ks, int
   *lin = a.s_out;
 }
 
+// Reversed-u adjoint launch over lam levels m0 + 1 -> m0 - ks + 1 of segment
+// s: u runs backward from checkpoint s + 1 (the level min(top, nt) of each
+// instance) through the u ring.
+void rev_adjoint_args(const Plan* p, BlockAdjArgs& a, int s, int m0, int ks, int* lin, int* uin) {
+  a.m0 = m0;
+  a.ks = ks;
+  a.s_in = *lin;
+  a.s_out = p->other_lam(*lin);
+  a.u_in = *uin;
+  a.u_out = p->other_ur(*uin);
+  a.u_ck = p->ck(s + 1);
+  a.seg_top = p->seg_top(s);
+  for (int j = 0; j < kMaxBlockSteps; ++j) a.bank_u[j] = 0;
+  *lin = a.s_out;
+  *uin = a.u_out;
+}
+
 // forward step n of the single-step schedule: level n (slot in) -> n + 1
 void step_forward_args(const Plan* p, StepArgs& a, int n, int* in, bool bank_last_segment) {
   const int m = n + 1;
@@ -915,7 +957,7 @@ void step_forward_args(const Plan* p, StepArgs& a, int n, int* in, bool bank_las
   a.s_in = *in;
   a.s_out = p->is_ck(m) ? p->ck(m / p->K) : p->other_ring(*in);
   const int lev = p->lbank ? n : m;   // the level this step banks (u^m, or L^n)
-  a.bank = (bank_last_segment && lev > (p->S - 1) * p->K) ? p->bank(p->S - 1, lev) : -1;
+  a.bank = (bank_last_segment && !p->rtr && lev > (p->S - 1) * p->K) ? p->bank(p->S - 1, lev) : -1;
   *in = a.s_out;
 }
 // recompute step n of segment s: level n -> n + 1 into the bank (lbank: L^n,
@@ -1032,6 +1074,18 @@ int enqueue_adjoint(const Plan* p, hipStream_t st, bool skip_last) {
         r.record = 0;
         blk_recompute_args(p, r, s - 1, n0, &rin);
         ERNN_HIP(launch_fwd_block(r, p->blk_ff.V, p->blk_ff.grid, st));
+      }
+    }
+  } else if (p->rtr) {
+    // reversed u: each segment's launches run u backward from the checkpoint
+    // of its top level beside lam; no recompute, no bank
+    BlockAdjArgs ba = block_adj_args(p);
+    int uin = p->ur(0);
+    for (int s = p->S - 1; s >= 0; --s) {
+      const int top = p->seg_top(s), bot = s * p->K + 1;
+      for (int m0 = top; m0 >= bot; m0 -= p->blk_adj.KS) {
+        rev_adjoint_args(p, ba, s, m0, std::min(p->blk_adj.KS, m0 - bot + 1), &lin, &uin);
+        ERNN_HIP(launch_rev_block(ba, p->blk_adj.V, p->blk_adj.grid, st));
       }
     }
   } else if (!p->fuse && p->blk_adj.KS > 1) {
@@ -1210,6 +1264,7 @@ int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
   info->ablk_blocks = p->blk_adj.n_blocks;
   info->ablk_ff_rw = p->ff_rw;
   info->fixed_c0 = p->lbank ? 1 : 0;
+  info->reverse_u = p->rtr ? 1 : 0;
   return ERNN_OK;
 }
 
@@ -1286,6 +1341,8 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   if (which == 2 && !p->fuse) return fail(ERNN_E_INVALID, "fused timing needs a plan with fuse = 1");
   if (which == 3 && p->blk.KS <= 1) return fail(ERNN_E_INVALID, "time-blocked timing needs blk_ks > 1");
   if (which == 4 && p->blk_adj.KS <= 1) return fail(ERNN_E_INVALID, "time-blocked adjoint timing needs ablk_ks > 1");
+  if (which == 1 && p->rtr)
+    return fail(ERNN_E_INVALID, "a reverse_u plan has no single-step adjoint (which = 4 times its reversed launches)");
   ERNN_TRY(check_call(p, call, which == 1 || which == 2 || which == 4, /*grads=*/false));  // never finalizes
   hipStream_t st = static_cast<hipStream_t>(stream);
   ERNN_HIP(hipSetDevice(p->device));
@@ -1306,7 +1363,20 @@ int ernn_time_steps(void* plan, int which, int steps, const ernn_call_t* call, v
   const int nsteps = std::min(steps, p->nt_max);
   // The timed launches follow the schedules' slot use: the adjoint reads u^m
   // from bank 0 (fuse plans: bank 1, while segment 0 is recomputed into bank 0).
-  if (which == 4) {   // `steps` time-blocked adjoint launches of KS steps each
+  if (which == 4 && p->rtr) {   // `steps` reversed-u adjoint launches within segment 0
+    const int KS = p->blk_adj.KS;
+    const int launches = std::min(steps, (p->nt_max + KS - 1) / KS);
+    BlockAdjArgs ba = block_adj_args(p);
+    int lin = p->lam(0), uin = p->ur(0);
+    const int top0 = p->seg_top(0);
+    for (int L = 0; L < launches; ++L) {
+      // m0 walks down segment 0 from its top and starts over below KS
+      const int span = std::max(top0 - KS + 1, 1);
+      const int m0 = top0 - (L * KS) % span;
+      rev_adjoint_args(p, ba, 0, m0, std::min(KS, m0), &lin, &uin);
+      ERNN_HIP(launch_rev_block(ba, p->blk_adj.V, p->blk_adj.grid, st));
+    }
+  } else if (which == 4) {   // `steps` time-blocked adjoint launches of KS steps each
     const int KS = p->blk_adj.KS;
     const int launches = std::min(steps, (p->nt_max + KS - 1) / KS);
     const int useg = p->fuse ? 1 : 0;   // bank_of(useg) is the bank the adjoint reads

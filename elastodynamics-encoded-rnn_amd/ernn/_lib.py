@@ -43,7 +43,7 @@ class Desc(C.Structure):
                 ("adj_bpc", C.c_int32), ("fuse", C.c_int32),
                 ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
                 ("ablk_v", C.c_int32), ("ablk_rw", C.c_int32), ("ablk_ks", C.c_int32),
-                ("ablk_ff_rw", C.c_int32), ("fixed_c0", C.c_int32),
+                ("ablk_ff_rw", C.c_int32), ("fixed_c0", C.c_int32), ("reverse_u", C.c_int32),
                 ("inst", C.POINTER(Instance)), ("src", C.POINTER(Source)), ("n_src", C.c_int64),
                 ("rec", C.POINTER(Receiver)), ("n_rec", C.c_int64)]
 
@@ -59,7 +59,7 @@ class PlanInfo(C.Structure):
                 ("fuse", C.c_int32), ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
                 ("blk_blocks", C.c_int32), ("ablk_v", C.c_int32), ("ablk_rw", C.c_int32),
                 ("ablk_ks", C.c_int32), ("ablk_blocks", C.c_int32), ("ablk_ff_rw", C.c_int32),
-                ("fixed_c0", C.c_int32)]
+                ("fixed_c0", C.c_int32), ("reverse_u", C.c_int32)]
 
 
 class Call(C.Structure):

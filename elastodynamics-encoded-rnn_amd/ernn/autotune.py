@@ -11,7 +11,8 @@ the backward recomputes nt - (last segment) steps: a larger K costs memory
 and saves recompute, and K = nt (one segment) recomputes nothing.
 ``choose_ckpt`` returns, among the K whose workspace fits a memory budget,
 the one that recomputes the fewest steps, or raises if even the K of least
-memory exceeds the cap.
+memory exceeds the cap.  A reverse_u plan has no bank and recomputes nothing:
+its K is REV_SEGMENT (or the shortest longer interval that fits).
 
 Tiles.  ``tune_tiles`` times every candidate (V columns per lane, R rows per
 wavefront) of the forward and the adjoint kernel on the real problem for a
@@ -41,15 +42,24 @@ STATE_FIELDS = 6   # u, c, v per component (the library's kStateFields)
 BANK_FIELDS = 2    # u per component
 
 
-def fields(nt: int, K: int, banks: int = 1) -> int:
-    """Padded fields per instance for nt steps and checkpoint interval K."""
+# Checkpoint interval of a reverse_u plan when none is named: each segment's
+# reversed reconstruction of u starts over from an exact checkpoint, so the
+# interval bounds how far rounding can carry it.
+REV_SEGMENT = 64
+
+
+def fields(nt: int, K: int, banks: int = 1, reverse_u: bool = False) -> int:
+    """Padded fields per instance for nt steps and checkpoint interval K
+    (reverse_u: no bank; S + 1 checkpoints and two more ring states)."""
+    if reverse_u:
+        return STATE_FIELDS * (7 + math.ceil(nt / K))
     return STATE_FIELDS * (4 + math.ceil(nt / K)) + BANK_FIELDS * banks * K
 
 
 def state_bytes(survey: Survey, K: int) -> int:
     nt = survey.nt_max
     b = 2 if survey.fused else 1
-    return sum(fields(nt, K, b) * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
+    return sum(fields(nt, K, b, survey.reverse_u) * padded_field_floats(s.nx, s.ny) * 4 for s in survey.shots)
 
 
 def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None, budget_bytes: Optional[int] = None) -> int:
@@ -60,6 +70,13 @@ def choose_ckpt(survey: Survey, mem_cap_bytes: Optional[int] = None, budget_byte
     nt = survey.nt_max
     if nt < 4:
         return 2
+    if survey.reverse_u:   # REV_SEGMENT steps, or the shortest interval from there that fits
+        fit = [K for K in range(min(REV_SEGMENT, nt), nt + 1)
+               if state_bytes(survey, K) <= (budget_bytes or mem_cap_bytes or float("inf"))]
+        if not fit:
+            raise MemoryError(f"checkpointed state needs {state_bytes(survey, nt) / 2**30:.2f} GiB or more "
+                              f"> cap {(budget_bytes or mem_cap_bytes) / 2**30:.2f} GiB")
+        return fit[0]
     b = 2 if survey.fused else 1
     best = min(range(2, nt + 1), key=lambda K: (fields(nt, K, b), -K))
     if mem_cap_bytes is not None and state_bytes(survey, best) > mem_cap_bytes:
@@ -97,6 +114,9 @@ ADJ_BLOCK_SHAPES_FIELD6: Tuple[Tuple[int, int, int], ...] = ((1, 2, 2), (1, 2, 4
 # time-blocked shapes of the fused backward (fuse plans): both halves of a launch share one shape;
 # (1, 8, 4) is built for six constants only
 FUSED_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 8, 4))
+# reversed-u adjoint shapes (reverse_u plans)
+REV_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 6, 2), (1, 6, 4), (1, 8, 2),
+                                                      (1, 8, 4))
 # largest V * R and R of the built adjoint kernels per mode (larger shapes would spill registers)
 ADJ_MAX_CELLS = {"const6": 16, "dfield": 8, "field6": 4}
 ADJ_MAX_ROWS = {"const6": 16, "dfield": 4, "field6": 2}
@@ -121,8 +141,11 @@ def tune_tiles(engine, params, field, wavelets, steps: int = 50,
     drift hit every candidate alike) and ranked by their median.
     A candidate is (V, R, blocks per CU).  which: 0 forward, 1 adjoint,
     2 fused adjoint + recompute (sets the adjoint tiling; fuse plans).
-    Returns {which: (V, R, bpc, ms_per_step, {candidate: ms})}.
+    Returns {which: (V, R, bpc, ms_per_step, {candidate: ms})}.  A reverse_u
+    plan has no single-step adjoint: its which 1 is left out.
     """
+    if getattr(engine, "reverse_u", False):
+        which = tuple(w for w in which if w != 1)
     ws = engine.acquire_workspace()
     traces, grads = timing_buffers(engine)
     for w in which:   # bring clocks and caches to steady state before the first candidate
@@ -158,10 +181,13 @@ def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int =
     plus recompute against the fused single-step kernel.
 
     Returns (V, RW, KS, ms_per_step, {shape: ms_per_step}); (0, 0, 0) is the
-    single-step kernel."""
+    single-step kernel.  A reverse_u plan times its reversed adjoint shapes
+    (REV_BLOCK_SHAPES; it has no single-step adjoint)."""
     mode = engine.survey.mode
     fused = which == 1 and bool(engine.info.fuse)
-    if which == 0:
+    if which == 1 and engine.reverse_u:
+        cands = list(REV_BLOCK_SHAPES if shapes is None else shapes)
+    elif which == 0:
         shapes = BLOCK_SHAPES if shapes is None else shapes
         cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or b[:2] != (2, 8)]
     else:

@@ -16,6 +16,9 @@ import torch
 from . import _lib
 from .problem import MODES, Survey
 
+# reversed adjoint shape of a reverse_u survey that names none
+DEFAULT_REV_SHAPE = (1, 4, 4)
+
 
 def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
     return None if t is None else t.data_ptr()
@@ -83,9 +86,13 @@ class Engine:
         d.use_graph = int(sv.use_graph)
         d.fuse = int(self._fuse)
         d.blk_v, d.blk_rw, d.blk_ks = (int(x) for x in sv.blocking)
-        d.ablk_v, d.ablk_rw, d.ablk_ks = (int(x) for x in sv.adj_blocking)
+        ab = tuple(sv.adj_blocking)
+        if sv.reverse_u and ab[2] <= 1:
+            ab = DEFAULT_REV_SHAPE
+        d.ablk_v, d.ablk_rw, d.ablk_ks = (int(x) for x in ab)
         d.ablk_ff_rw = int(sv.adj_ff_rw)
         d.fixed_c0 = int(bool(sv.fixed_c0))
+        d.reverse_u = int(bool(sv.reverse_u))
         d.inst = inst
         d.src, d.n_src = src, len(src_list)
         d.rec, d.n_rec = rec, len(rec_list)
@@ -117,6 +124,11 @@ class Engine:
     def fixed_c0(self) -> bool:
         """The plan returns dJ/dD only (dfield over known constants)."""
         return bool(self.info.fixed_c0)
+
+    @property
+    def reverse_u(self) -> bool:
+        """The adjoint reconstructs u backward from the checkpoints (no bank, no recompute)."""
+        return bool(self.info.reverse_u)
 
     @property
     def workspace_bytes(self) -> int:

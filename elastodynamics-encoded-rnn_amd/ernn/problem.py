@@ -202,6 +202,13 @@ class Survey:
     # u^m, and the adjoint reads it pointwise instead of taking a stencil of
     # u (no constant-gradient sums).  Needs the unfused backward.
     fixed_c0: bool = False
+    # With fixed_c0: no bank and no recompute.  The adjoint runs the forward
+    # recurrence backward in time from each segment's top checkpoint (the
+    # leapfrog step is time-symmetric) and forms (dt^2/rho) A(C0) u^m on the
+    # way.  Needs a reversed time-blocked adjoint shape (REV_BLOCK_SHAPES;
+    # (0, 0, 0) = (1, 4, 4)); the checkpoint interval bounds how far each
+    # reconstruction runs from its checkpoint (ckpt_every 0: REV_SEGMENT).
+    reverse_u: bool = False
 
     def __post_init__(self):
         if self.mode not in MODES:
@@ -210,6 +217,8 @@ class Survey:
             raise ValueError("fixed_c0 applies to the dfield mode only")
         if self.fixed_c0 and self.fuse:
             raise ValueError("fixed_c0 needs the unfused backward (fuse False or None)")
+        if self.reverse_u and not self.fixed_c0:
+            raise ValueError("reverse_u needs fixed_c0 (dfield over known constants)")
 
     @property
     def fused(self) -> bool:

@@ -116,6 +116,14 @@ typedef struct {
                            (dt^2/rho) A(C0) u^m instead of u^m (same size), and the adjoint
                            reads it pointwise instead of taking a stencil of u.  Requires
                            fuse = 0; grad_params is not written. */
+  int32_t reverse_u;    /* with fixed_c0 = 1, 0 or 1.  1: no bank and no recompute; the
+                           adjoint runs the forward recurrence backward in time from the
+                           checkpoints beside lam (the leapfrog step is time-symmetric:
+                           u^{m-1} = u^m - v^m, v^m = v^{m+1} - d^m), forming A(C0) u^m on
+                           the way.  The checkpoints hold (u, c, v) at every K-th level and
+                           each instance's last level.  Requires a time-blocked adjoint
+                           (ablk_v, ablk_rw, ablk_ks) = (1, RW, KS), RW in {4, 6, 8}, KS in
+                           {2, 4}. */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;
@@ -145,6 +153,7 @@ typedef struct {
   int32_t ablk_ff_rw;       /* rows per wavefront of the fused recompute half (0: no fused
                                time-blocked backward) */
   int32_t fixed_c0;         /* the descriptor's fixed_c0 (DFIELD: dJ/dD only) */
+  int32_t reverse_u;        /* the descriptor's reverse_u */
 } ernn_plan_info_t;
 
 /* Per-call device pointers.

@@ -12,9 +12,10 @@ from ernn import autotune
 class StubEngine:
     """Records shape switches; time_steps returns a cost model of the current shape."""
 
-    def __init__(self, mode, fuse=False, cost=None):
+    def __init__(self, mode, fuse=False, cost=None, reverse_u=False):
         self.survey = types.SimpleNamespace(mode=mode)
         self.info = types.SimpleNamespace(fuse=int(fuse))
+        self.reverse_u = reverse_u
         self.trace_floats = 4
         self.device = torch.device("cpu")
         self.tiling = {0: None, 1: None}
@@ -70,6 +71,15 @@ def test_adjoint_blocking_candidates_follow_the_built_shapes(mode, fuse, expect)
     assert shapes == expect
     # on a fuse plan the one-step-per-launch candidate is the fused single-step kernel
     assert (2 in eng.calls) == fuse and (1 in eng.calls) == (not fuse)
+
+
+def test_reverse_u_adjoint_candidates_are_the_reversed_shapes_only():
+    eng = StubEngine("dfield", reverse_u=True, cost=lambda which, s: 0.5 if s == (1, 6, 4) else 1.0)
+    seen = _tried(eng, 1)
+    v, rw, ks, ms, times = autotune.tune_blocking(eng, None, None, None, steps=8, which=1, rounds=2)
+    # no single-step candidate: a reverse_u plan has no one-step adjoint
+    assert {s for s, w in seen if w == 1} == set(autotune.REV_BLOCK_SHAPES) and set(times) == set(autotune.REV_BLOCK_SHAPES)
+    assert (v, rw, ks) == (1, 6, 4) and eng.blocking[1] == (1, 6, 4) and set(eng.calls) == {4}
 
 
 def test_forward_blocking_candidates_exclude_wide_rows_for_field_modes():

@@ -95,6 +95,13 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
     ({}, {"param_mode": 2, "fixed_c0": 1}, "damage-field mode"),
     ({}, {"param_mode": 1, "fixed_c0": 2}, "fixed_c0 must be 0 or 1"),
     ({}, {"param_mode": 1, "fixed_c0": 1, "fuse": 1}, "fuse = 0"),
+    ({}, {"param_mode": 1, "fixed_c0": 1, "reverse_u": 2}, "reverse_u must be 0 or 1"),
+    ({}, {"param_mode": 1, "reverse_u": 1, "ablk_v": 1, "ablk_rw": 4, "ablk_ks": 4}, "needs fixed_c0"),
+    ({}, {"param_mode": 1, "fixed_c0": 1, "reverse_u": 1}, "needs a time-blocked adjoint of shape (1, RW, KS)"),
+    ({}, {"param_mode": 1, "fixed_c0": 1, "reverse_u": 1, "ablk_v": 1, "ablk_rw": 8, "ablk_ks": 8},
+     "needs a time-blocked adjoint of shape (1, RW, KS)"),
+    ({}, {"param_mode": 1, "fixed_c0": 1, "reverse_u": 1, "ablk_v": 2, "ablk_rw": 4, "ablk_ks": 4},
+     "needs a time-blocked adjoint of shape (1, RW, KS)"),
 ])
 def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
     inst = dict(GOOD, **patch)
@@ -201,6 +208,28 @@ def test_damage_1024_fits_the_32_gb_cap():
         autotune.choose_ckpt(sv, mem_cap_bytes=autotune.state_bytes(sv, k_min) - 1)
 
 
+def test_reverse_u_checkpoints_every_rev_segment_steps_without_a_bank():
+    """A reverse_u plan keeps S + 1 checkpoints and four ring states, no bank:
+    its interval is REV_SEGMENT unless the budget forces a longer one."""
+    wl = configs.damage_1024()
+    sv = wl.survey
+    sv.reverse_u = True
+    nt = sv.nt_max
+    assert autotune.fields(nt, 64, reverse_u=True) == 6 * (7 + 47)
+    per_field = sum(autotune.padded_field_floats(s.nx, s.ny) * 4 for s in sv.shots)
+    assert autotune.state_bytes(sv, 64) == 6 * (7 + 47) * per_field
+    K = autotune.choose_ckpt(sv, mem_cap_bytes=sv.ckpt_budget, budget_bytes=sv.ckpt_budget)
+    assert K == autotune.REV_SEGMENT and autotune.state_bytes(sv, K) < 12 * 10**9
+    tight = autotune.state_bytes(sv, 200)
+    K = autotune.choose_ckpt(sv, budget_bytes=tight)
+    assert autotune.state_bytes(sv, K) <= tight < autotune.state_bytes(sv, K - 1)
+    with pytest.raises(MemoryError):
+        autotune.choose_ckpt(sv, mem_cap_bytes=autotune.state_bytes(sv, nt) - 1,
+                             budget_bytes=autotune.state_bytes(sv, nt) - 1)
+    with pytest.raises(ValueError, match="fixed_c0"):
+        ernn.Survey(sv.shots, mode="dfield", reverse_u=True)
+
+
 def test_padded_layout_matches_library_convention():
     # pitch = 32 + round_up(ny, 256); rows round_up(nx, 16) + 3
     assert autotune.padded_field_floats(128, 128) == 131 * 288
@@ -267,4 +296,4 @@ def test_integration_stub_structs_match_the_binding():
         stub = ns[name]
         assert C.sizeof(stub) == C.sizeof(ref), name
         assert [f[0] for f in stub._fields_] == [f[0] for f in ref._fields_], name
-    assert re.search(r"Desc\(1, 1, 0, P, 1, K(, \d+){16}, inst,", doc)
+    assert re.search(r"Desc\(1, 1, 0, P, 1, K(, \d+){17}, inst,", doc)
