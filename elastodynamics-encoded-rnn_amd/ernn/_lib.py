@@ -15,7 +15,7 @@ import torch  # noqa: F401  (must precede loading the HIP library)
 _DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_DIR, "libernn_hip.so")
 
-ERNN_ABI_VERSION = 5
+ERNN_ABI_VERSION = 6
 PARAM_CONST6, PARAM_DFIELD, PARAM_FIELD6 = 0, 1, 2
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "capacity exceeded"}
 

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define ERNN_ABI_VERSION 5
+#define ERNN_ABI_VERSION 6
 
 #define ERNN_OK 0
 #define ERNN_E_INVALID (-1)   /* bad argument or descriptor */
