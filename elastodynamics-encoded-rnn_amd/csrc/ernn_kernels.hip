@@ -1620,9 +1620,16 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 //   v^m = v^{m+1} - (D L^m + (dt^2/rho) f^m)   (forward: v^{m+1} = v^m + d^m)
 //   u^{m-1} + c^{m-1} = u^m + c^m - v^m      (the forward's compensated sum, reversed)
 // and dJ/dD takes lam^{m+1} . L^m.  A segment starts from the checkpoint of
-// its top level (u, c, v of that level, v^T taken as is).  The leapfrog
+// its top level (u, c, v of that level, v^T taken as is).  Between launches
+// the lam and u rings carry (lam, w) and (u, v) only; both compensations
+// restart at zero, which saves a third of the launch's HBM traffic for one
+// rounding of lam and of u every KS steps (within a launch the sums stay
+// compensated).  The leapfrog
 // recurrence is time-symmetric, so the reversed states stay within rounding
-// of the forward's; u's halo shrinks with lam's, one cell per step.
+// of the forward's; u's halo shrinks with lam's, one cell per step.  Point
+// forces and residuals are added to the velocities before each step's row
+// loop, so the loop is one branch-free block of arithmetic whose rows are
+// stepped as soon as their stencils are formed.
 // --------------------------------------------------------------------------
 template <int V, int RW, int KS>
 __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) {
@@ -1679,10 +1686,12 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         l[q][r][e] = bload(rin[q], bd.vcol[e], so);
-        c[q][r][e] = bload(rin[2 + q], bd.vcol[e], so);
+        c[q][r][e] = 0.f;   // lam's compensation, like u's, restarts every launch
         wv[q][r][e] = bload(rin[4 + q], bd.vcol[e], so);
         uu[q][r][e] = bload(uin[q], bd.vcol[e], so);
-        cu[q][r][e] = bload(uin[2 + q], bd.vcol[e], so);
+        // the u ring between launches carries (u, v) only: the compensation
+        // restarts at zero (the checkpoints keep theirs)
+        cu[q][r][e] = from_ck ? bload(uin[2 + q], bd.vcol[e], so) : 0.f;
         vu[q][r][e] = bload(uin[4 + q], bd.vcol[e], so);
       }
       dd[r][e] = bload(rD, bd.vcol[e], so);
@@ -1709,85 +1718,37 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   __syncthreads();
   const int nf = I.n_frames;
 
-  for (int s = 0; s < a.ks; ++s) {
+  // unrolled: the row window keeps its registers from step to step (two
+  // shapes stay rolled to fit three wavefronts per SIMD without spilling)
+  constexpr int kUnroll = (RW == 8 && (KS == 4 || KS == 6)) ? 1 : KS;
+#pragma unroll kUnroll
+  for (int s = 0; s < KS; ++s) {
+    if (s >= a.ks) continue;   // block-uniform
     const int m = a.m0 - s;
     const int buf = s & 1;
     if (m <= I.nt) {   // block-uniform: an instance with fewer steps starts later
       const bool first = from_ck && m == ustart;   // v^m is the checkpoint's own (block-uniform)
-      // ---- stencil rows of D lam and of u: own rows and the band-edge rows
-      Row<V> xr[RW + 2], yr[RW + 2], pxr[RW + 2], pyr[RW + 2];
-      {
-        float e0[4][V], e1[4][V];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int e = 0; e < V; ++e) {
-            e0[q][e] = (w > 0) ? xch[buf][w - 1][1][q][lane * V + e] : 0.f;
-            e1[q][e] = (w < WB - 1) ? xch[buf][w + 1][0][q][lane * V + e] : 0.f;
-          }
-        xr[0] = make_row_z<V>(e0[0]);
-        yr[0] = make_row_z<V>(e0[1]);
-        pxr[0] = make_row_z<V>(e0[2]);
-        pyr[0] = make_row_z<V>(e0[3]);
-        xr[RW + 1] = make_row_z<V>(e1[0]);
-        yr[RW + 1] = make_row_z<V>(e1[1]);
-        pxr[RW + 1] = make_row_z<V>(e1[2]);
-        pyr[RW + 1] = make_row_z<V>(e1[3]);
-#pragma unroll
-        for (int r = 0; r < RW; ++r) {
-          float rx[V], ry[V];
-#pragma unroll
-          for (int e = 0; e < V; ++e) { rx[e] = dd[r][e] * l[0][r][e]; ry[e] = dd[r][e] * l[1][r][e]; }
-          xr[r + 1] = make_row_z<V>(rx);
-          yr[r + 1] = make_row_z<V>(ry);
-          pxr[r + 1] = make_row_z<V>(uu[0][r]);
-          pyr[r + 1] = make_row_z<V>(uu[1][r]);
-        }
-      }
-
-      float dx[RW][V], dy[RW][V], gx[RW][V], gy[RW][V];
-#pragma unroll
-      for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e) {
-          const int x = e + 1;
-          const UDeriv dl = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], x);
-          apply_op_sum(kl, dl, dx[r][e], dy[r][e]);
-          const UDeriv du = uderivs<V>(pxr[r], pxr[r + 1], pxr[r + 2], pyr[r], pyr[r + 1], pyr[r + 2], x);
-          float lx, ly;
-          apply_op_sum(kl, du, lx, ly);   // L^m, the forward's operator value
-          // dJ/dD on the band rows of the output tile; the last level's term (lam^{nt+1} = 0) is skipped
-          if (r >= bd.own_lo && r < bd.own_hi && m < I.nt)
-            gacc[r][e] += __builtin_fmaf(l[0][r][e], lx, l[1][r][e] * ly);
-          gx[r][e] = dd[r][e] * lx;   // D L^m: the forward's increment before its force
-          gy[r][e] = dd[r][e] * ly;
-        }
-#pragma unroll
-      for (int r = 0; r < RW; ++r)
-#pragma unroll
-        for (int e = 0; e < V; ++e) asm volatile("" : "+v"(gacc[r][e]));
-
+      // Point forces and residuals enter the velocities before the row loop
+      // (neither is read by a stencil), which keeps that loop free of branches:
+      //   u: v^{m+1} - (dt^2/rho) f^m, the rest of d^m = D L^m follows per row;
+      //   lam: w + r^m on recorded steps m = kP + 1, the stencil part follows.
       if (!first) {
-        // point forces of step m, as the forward added them: d^m = fma(dt^2/rho, s_m w, D L^m)
-        const float s_m = ld_const(C.wav + I.wav_off + m);
+        const float s_m = ld_const(C.wav + I.wav_off + m);   // m < nt here
         for (int sp = bl.src_begin; sp < bl.src_end; ++sp) {
           const SrcEv ev = ld_src(a.src + sp);
           const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
           if (rr < 0 || rr >= RW || lane != lc / V) continue;
-          const float f = s_m * ev.w;
+          const float f = kdt * (s_m * ev.w);
 #pragma unroll
           for (int r = 0; r < RW; ++r)
 #pragma unroll
-            for (int e = 0; e < V; ++e) {
+            for (int e = 0; e < V; ++e)
               if (r == rr && e == lc % V) {
-                if (ev.comp == 0) gx[r][e] = __builtin_fmaf(kdt, f, gx[r][e]);
-                else gy[r][e] = __builtin_fmaf(kdt, f, gy[r][e]);
+                if (ev.comp == 0) vu[0][r][e] = vu[0][r][e] - f;
+                else vu[1][r][e] = vu[1][r][e] - f;
               }
-            }
         }
       }
-
-      // residual injection on recorded steps m = kP + 1 (the whole region; walked receiver list)
       if (((m - 1) % a.stride == 0) && ((m - 1) / a.stride < nf)) {
         const int k = (m - 1) / a.stride;
         if (a.snapshot) {
@@ -1798,8 +1759,8 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
             for (int e = 0; e < V; ++e)
               if (bd.col_in[e]) {
                 const gfloat* g = C.gtr + I.trace_off + ((int64_t)k * I.nx + gi0 + r) * I.ny + bd.gj0 + e;
-                dx[r][e] += g[0];
-                dy[r][e] += g[(int64_t)nf * I.nx * I.ny];
+                wv[0][r][e] = wv[0][r][e] + g[0];
+                wv[1][r][e] = wv[1][r][e] + g[(int64_t)nf * I.nx * I.ny];
               }
           }
         } else {
@@ -1814,25 +1775,73 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 #pragma unroll
               for (int e = 0; e < V; ++e)
                 if (r == rr && e == lc % V) {
-                  dx[r][e] += vgx;
-                  dy[r][e] += vgy;
+                  wv[0][r][e] = wv[0][r][e] + vgx;
+                  wv[1][r][e] = wv[1][r][e] + vgy;
                 }
           }
         }
       }
+      // Rows r - 1, r, r + 1 of D lam and of u as they were at the start of
+      // the step, in a window that slides down the band: row r's state is
+      // stepped as soon as its stencils are formed (the window keeps the old
+      // values its neighbours still read), so only three rows of taps are
+      // live at a time.
+      float e0[4][V], e1[4][V];
 #pragma unroll
-      for (int r = 0; r < RW; ++r)
+      for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          advance(l[0][r][e], c[0][r][e], wv[0][r][e], dx[r][e]);
-          advance(l[1][r][e], c[1][r][e], wv[1][r][e], dy[r][e]);
-          if (!first) {   // v^m = v^{m+1} - d^m
-            vu[0][r][e] = vu[0][r][e] - gx[r][e];
-            vu[1][r][e] = vu[1][r][e] - gy[r][e];
+          e0[q][e] = (w > 0) ? xch[buf][w - 1][1][q][lane * V + e] : 0.f;
+          e1[q][e] = (w < WB - 1) ? xch[buf][w + 1][0][q][lane * V + e] : 0.f;
+        }
+      auto dl_row = [&](int r, int q) {
+        float t[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) t[e] = dd[r][e] * l[q][r][e];
+        return make_row_z<V>(t);
+      };
+      Row<V> xm = make_row_z<V>(e0[0]), ym = make_row_z<V>(e0[1]);
+      Row<V> pxm = make_row_z<V>(e0[2]), pym = make_row_z<V>(e0[3]);
+      Row<V> xc = dl_row(0, 0), yc = dl_row(0, 1);
+      Row<V> pxc = make_row_z<V>(uu[0][0]), pyc = make_row_z<V>(uu[1][0]);
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        Row<V> xp, yp, pxp, pyp;
+        if (r + 1 < RW) {
+          xp = dl_row(r + 1, 0);
+          yp = dl_row(r + 1, 1);
+          pxp = make_row_z<V>(uu[0][r + 1]);
+          pyp = make_row_z<V>(uu[1][r + 1]);
+        } else {
+          xp = make_row_z<V>(e1[0]);
+          yp = make_row_z<V>(e1[1]);
+          pxp = make_row_z<V>(e1[2]);
+          pyp = make_row_z<V>(e1[3]);
+        }
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const int x = e + 1;
+          float dx, dy, lx, ly;
+          const UDeriv dlam = uderivs<V>(xm, xc, xp, ym, yc, yp, x);
+          apply_op_sum(kl, dlam, dx, dy);
+          const UDeriv du = uderivs<V>(pxm, pxc, pxp, pym, pyc, pyp, x);
+          apply_op_sum(kl, du, lx, ly);   // L^m, the forward's operator value
+          // dJ/dD with lam^{m+1}, not yet stepped (zero at m = nt); rows
+          // outside the output tile are never stored
+          gacc[r][e] += __builtin_fmaf(l[0][r][e], lx, l[1][r][e] * ly);
+          // step row r: lam to level m, u to level m - 1
+          advance(l[0][r][e], c[0][r][e], wv[0][r][e], dx);
+          advance(l[1][r][e], c[1][r][e], wv[1][r][e], dy);
+          if (!first) {   // v^m = v^{m+1} - d^m, d^m = D L^m + (dt^2/rho) f^m
+            vu[0][r][e] = vu[0][r][e] - dd[r][e] * lx;
+            vu[1][r][e] = vu[1][r][e] - dd[r][e] * ly;
           }
           retreat(uu[0][r][e], cu[0][r][e], vu[0][r][e]);
           retreat(uu[1][r][e], cu[1][r][e], vu[1][r][e]);
         }
+        xm = xc; xc = xp; ym = yc; yc = yp;
+        pxm = pxc; pxc = pxp; pym = pyc; pyc = pyp;
+      }
     }
     if (s + 1 < a.ks) {
       publish(buf ^ 1);
@@ -1874,10 +1883,8 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
           bstore(l[q][r][e], ro[q], bd.vcol[e], so);
-          bstore(c[q][r][e], ro[2 + q], bd.vcol[e], so);
           bstore(wv[q][r][e], ro[4 + q], bd.vcol[e], so);
           bstore(uu[q][r][e], uo[q], bd.vcol[e], so);
-          bstore(cu[q][r][e], uo[2 + q], bd.vcol[e], so);
           bstore(vu[q][r][e], uo[4 + q], bd.vcol[e], so);
         }
       }
@@ -2148,7 +2155,8 @@ __global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockA
 
 // Reversed-u adjoint (damage field over known constants; rev_block_body).
 template <int V, int RW, int KS>
-__global__ void __launch_bounds__(kWave * blk_waves(RW)) rev_block_kernel(BlockAdjArgs) {
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) __attribute__((amdgpu_waves_per_eu(3)))
+rev_block_kernel(BlockAdjArgs) {
   rev_block_body<V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
 
@@ -2421,7 +2429,8 @@ static hipError_t launch_adj_block_m(const BlockAdjArgs& a, int V, int grid, hip
 
 // Reversed-u adjoint shapes (rev_block_ok in ernn_plan.cpp): V = 1, RW in
 // {4, 6, 8} (lam and u both held in registers), KS in {2, 4}.
-#define ERNN_REV_BLOCK_CASES(X) X(1, 4, 2) X(1, 4, 4) X(1, 6, 2) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4)
+#define ERNN_REV_BLOCK_CASES(X) \
+  X(1, 4, 2) X(1, 4, 4) X(1, 6, 2) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 5) X(1, 8, 6)
 #define ERNN_REV_BLOCK_CASE(v_, r_, k_)                                                             \
   if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
     hipLaunchKernelGGL((rev_block_kernel<v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \

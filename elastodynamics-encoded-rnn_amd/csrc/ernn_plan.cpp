@@ -141,9 +141,10 @@ bool fused_block_ok(int mode, int V, int RW, int KS) {
 bool fused_ff_ok(int V, int RW, int KS, int RWF) {
   return RWF == RW || (V == 1 && RW == 4 && RWF == 8 && (KS == 2 || KS == 4));
 }
-// Reversed-u adjoint shapes (ERNN_REV_BLOCK_CASES): V = 1, RW in {4, 6, 8}, KS in {2, 4}.
+// Reversed-u adjoint shapes (ERNN_REV_BLOCK_CASES): V = 1, RW in {4, 6, 8}, KS in {2, 4}, and
+// (1, 8, 5), (1, 8, 6).
 bool rev_block_ok(int V, int RW, int KS) {
-  return V == 1 && (RW == 4 || RW == 6 || RW == 8) && (KS == 2 || KS == 4);
+  return V == 1 && (((RW == 4 || RW == 6 || RW == 8) && (KS == 2 || KS == 4)) || (RW == 8 && (KS == 5 || KS == 6)));
 }
 bool ablk_ok(int mode, bool fuse, int V, int RW, int KS) {
   return fuse ? fused_block_ok(mode, V, RW, KS) : adj_block_ok(mode, V, RW, KS);
@@ -508,7 +509,8 @@ int set_fwd_blocking(Plan* p, int V, int RW, int KS) {
 // the plan's together, so a failure leaves the plan as it was.
 int set_adj_blocking(Plan* p, int V, int RW, int KS, int RWF) {
   if (p->rtr && (KS <= 1 || !rev_block_ok(V, RW, KS)))
-    return fail(ERNN_E_INVALID, "a reverse_u plan runs the reversed adjoint: (1, RW, KS), RW in {4, 6, 8}, KS in {2, 4}");
+    return fail(ERNN_E_INVALID, "a reverse_u plan runs the reversed adjoint: (1, RW, KS), RW in {4, 6, 8}, KS in {2, 4}, "
+                                "or (1, 8, 5), (1, 8, 6)");
   if (KS <= 1) {
     free_blocking(p->blk_adj);
     free_blocking(p->blk_ff);
@@ -585,7 +587,7 @@ int validate(const ernn_desc_t* d) {
   if (d->reverse_u && !d->fixed_c0) return fail(ERNN_E_INVALID, "reverse_u needs fixed_c0 = 1");
   if (d->reverse_u && (d->ablk_ks <= 1 || !rev_block_ok(d->ablk_v, d->ablk_rw, d->ablk_ks)))
     return fail(ERNN_E_INVALID, "reverse_u needs a time-blocked adjoint of shape (1, RW, KS), RW in {4, 6, 8}, "
-                                "KS in {2, 4}");
+                                "KS in {2, 4}, or (1, 8, 5), (1, 8, 6)");
   if (d->ablk_ff_rw != 0 && (d->ablk_ks <= 1 || !d->fuse || !fused_ff_ok(d->ablk_v, d->ablk_rw, d->ablk_ks, d->ablk_ff_rw)))
     return fail(ERNN_E_INVALID, "ablk_ff_rw: 0, or on a fuse plan with a time-blocked adjoint the adjoint's RW, or 8 "
                                 "beside RW = 4 with KS in {2,4}");

@@ -41,12 +41,14 @@ from test_gpu_fixed_c0 import _oracle_dD
 pytestmark = pytest.mark.gpu
 
 REV = list(autotune.REV_BLOCK_SHAPES)
+# banked damage-mode adjoint shapes (V = 1, KS != 5)
+DFIELD_ADJ = {s for s in autotune.ADJ_BLOCK_SHAPES if s[0] == 1 and s[2] != 5}
 
 
 def _trio(shots, fwd, adj, K, n_sets=1, snapshot=False, stride=1, **kw):
     """(banked fixed_c0 run, reverse_u run): each (engine, traces, grads).  The
     banked plan takes the reversed shape where it has that shape, else (1, 4, 4)."""
-    banked = adj if adj in autotune.ADJ_BLOCK_SHAPES else (1, 4, 4)
+    banked = adj if adj in DFIELD_ADJ else (1, 4, 4)
     base = ernn.Survey(shots, mode="dfield", ckpt_every=K, tiles=(2, 2, 1, 4, 0, 0), snapshot=snapshot,
                        rec_stride=stride, fuse=False, blocking=fwd, adj_blocking=banked, n_sets=n_sets,
                        fixed_c0=True)
