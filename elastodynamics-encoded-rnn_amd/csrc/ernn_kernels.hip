@@ -1048,93 +1048,138 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   publish(0);
   __syncthreads();
 
-  for (int s = 0; s < ks; ++s) {
+  // the damage-field modes unroll their steps (registers stay put between them)
+  constexpr int kUnroll = (MODE == kDOnly || MODE == kDField) ? KS : 1;
+  static_assert(MODE != kDOnly || NC == 1, "one coefficient per cell: D");
+#pragma unroll kUnroll
+  for (int s = 0; s < KS; ++s) {
+    if (s >= ks) continue;   // block-uniform
     const int n = a.n0 + s;
     const int buf = s & 1;
-    // rows -1 .. RW of this wavefront's band; the region's own edge rows get zeros
-    Row<V> xr[RW + 2], yr[RW + 2];
-    {
-      float tx[V], ty[V], bx[V], by[V];
+    // Damage-field steps over known constants with no point force in the
+    // block and nothing to bank: a window of rows r - 1, r, r + 1 slides down
+    // the band and each row is stepped as soon as its stencil is formed (the
+    // window holds the values its neighbours still read), one branch-free
+    // block per step.  The arithmetic per cell is the general path's, so both
+    // give the same bits.  (The joint damage mode keeps the general path: the
+    // second path would cost it a wavefront per SIMD.)
+    bool stepped = false;
+    if constexpr (MODE == kDOnly) {
+      if (a.bank_store[s] < 0 && bl.src_begin == bl.src_end) {
+        float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
-      for (int e = 0; e < V; ++e) {
-        tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
-        ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
-        bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
-        by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
-      }
-      xr[0] = make_row_z<V>(tx);
-      yr[0] = make_row_z<V>(ty);
-      xr[RW + 1] = make_row_z<V>(bx);
-      yr[RW + 1] = make_row_z<V>(by);
+        for (int e = 0; e < V; ++e) {
+          tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
+          ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
+          bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
+          by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+        }
+        Row<V> xm = make_row_z<V>(tx), ym = make_row_z<V>(ty);
+        Row<V> xc = make_row_z<V>(u[0][0]), yc = make_row_z<V>(u[1][0]);
 #pragma unroll
-      for (int r = 0; r < RW; ++r) {
-        xr[r + 1] = make_row_z<V>(u[0][r]);
-        yr[r + 1] = make_row_z<V>(u[1][r]);
+        for (int r = 0; r < RW; ++r) {
+          const Row<V> xp = (r + 1 < RW) ? make_row_z<V>(u[0][r + 1]) : make_row_z<V>(bx);
+          const Row<V> yp = (r + 1 < RW) ? make_row_z<V>(u[1][r + 1]) : make_row_z<V>(by);
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            const UDeriv d = uderivs<V>(xm, xc, xp, ym, yc, yp, e + 1);
+            float ox, oy;   // the arithmetic of fwd_op<kDField>
+            apply_op_sum(kf, d, ox, oy);
+            const float dx = cf[r][0][e] * ox, dy = cf[r][0][e] * oy;
+            advance(u[0][r][e], c[0][r][e], v[0][r][e], dx);
+            advance(u[1][r][e], c[1][r][e], v[1][r][e], dy);
+          }
+          xm = xc; xc = xp; ym = yc; yc = yp;
+        }
+        stepped = true;
       }
     }
-
-    float dx[RW][V], dy[RW][V];
-    // kDOnly banks (dt^2/rho) A(C0) u^n of this step's level n on the output
-    // tile as it is formed (the bank slot of level n, or -1)
-    const int lb_slot = MODE == kDOnly ? a.bank_store[s] : -1;
-    const Rsrc lbx = field_rsrc(bank_field(C, I, max(lb_slot, 0), 0), I.field);
-    const Rsrc lby = field_rsrc(bank_field(C, I, max(lb_slot, 0), 1), I.field);
+    if (!stepped) {
+      // rows -1 .. RW of this wavefront's band; the region's own edge rows get zeros
+      Row<V> xr[RW + 2], yr[RW + 2];
+      {
+        float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
-    for (int r = 0; r < RW; ++r)
+        for (int e = 0; e < V; ++e) {
+          tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
+          ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
+          bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
+          by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+        }
+        xr[0] = make_row_z<V>(tx);
+        yr[0] = make_row_z<V>(ty);
+        xr[RW + 1] = make_row_z<V>(bx);
+        yr[RW + 1] = make_row_z<V>(by);
 #pragma unroll
-      for (int e = 0; e < V; ++e) {
-        const UDeriv d = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1);
-        float cq[NC > 0 ? NC : 1];
-#pragma unroll
-        for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
-        if constexpr (MODE == kDOnly) {   // the arithmetic of fwd_op<kDField>, keeping the operator value
-          float ox, oy;
-          apply_op_sum(kf, d, ox, oy);
-          if (lb_slot >= 0 && r >= bd.own_lo && r < bd.own_hi && bd.col_own && bd.col_in[e]) {
-            const uint32_t so = bd.srow(I, r);
-            bstore(ox, lbx, bd.vcol[e], so);
-            bstore(oy, lby, bd.vcol[e], so);
-          }
-          dx[r][e] = cq[0] * ox;
-          dy[r][e] = cq[0] * oy;
-        } else {
-          fwd_op<MODE>(kf, gsa, cq, d, dx[r][e], dy[r][e]);
+        for (int r = 0; r < RW; ++r) {
+          xr[r + 1] = make_row_z<V>(u[0][r]);
+          yr[r + 1] = make_row_z<V>(u[1][r]);
         }
       }
 
-    // point forces anywhere in the region (halo cells feed the later steps)
-    const float s_n = ld_const(C.wav + I.wav_off + n);
-    for (int sp = bl.src_begin; sp < bl.src_end; ++sp) {
-      const SrcEv ev = ld_src(a.src + sp);
-      const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
-      if (rr < 0 || rr >= RW || lane != lc / V) continue;
-      const float f = s_n * ev.w;
+      float dx[RW][V], dy[RW][V];
+      // kDOnly banks (dt^2/rho) A(C0) u^n of this step's level n on the output
+      // tile as it is formed (the bank slot of level n, or -1)
+      const int lb_slot = MODE == kDOnly ? a.bank_store[s] : -1;
+      const Rsrc lbx = field_rsrc(bank_field(C, I, max(lb_slot, 0), 0), I.field);
+      const Rsrc lby = field_rsrc(bank_field(C, I, max(lb_slot, 0), 1), I.field);
 #pragma unroll
       for (int r = 0; r < RW; ++r)
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          if (r == rr && e == lc % V) {
-            if (ev.comp == 0) dx[r][e] = __builtin_fmaf(kdt, f, dx[r][e]);
-            else dy[r][e] = __builtin_fmaf(kdt, f, dy[r][e]);
+          const UDeriv d = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1);
+          float cq[NC > 0 ? NC : 1];
+#pragma unroll
+          for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
+          if constexpr (MODE == kDOnly) {   // the arithmetic of fwd_op<kDField>, keeping the operator value
+            float ox, oy;
+            apply_op_sum(kf, d, ox, oy);
+            if (lb_slot >= 0 && r >= bd.own_lo && r < bd.own_hi && bd.col_own && bd.col_in[e]) {
+              const uint32_t so = bd.srow(I, r);
+              bstore(ox, lbx, bd.vcol[e], so);
+              bstore(oy, lby, bd.vcol[e], so);
+            }
+            dx[r][e] = cq[0] * ox;
+            dy[r][e] = cq[0] * oy;
+          } else {
+            fwd_op<MODE>(kf, gsa, cq, d, dx[r][e], dy[r][e]);
           }
         }
-    }
-    if constexpr (MODE == kConst6) {   // cells outside the grid keep their zero state
-      if (bd.edge) {
+
+      // point forces anywhere in the region (halo cells feed the later steps)
+      const float s_n = ld_const(C.wav + I.wav_off + n);
+      for (int sp = bl.src_begin; sp < bl.src_end; ++sp) {
+        const SrcEv ev = ld_src(a.src + sp);
+        const int rr = ev.i - gi0, lc = ev.j - (bl.oj0 - KS);
+        if (rr < 0 || rr >= RW || lane != lc / V) continue;
+        const float f = s_n * ev.w;
 #pragma unroll
         for (int r = 0; r < RW; ++r)
 #pragma unroll
-          for (int e = 0; e < V; ++e)
-            if (!(bd.row_in(I, r) && bd.col_in[e])) { dx[r][e] = 0.f; dy[r][e] = 0.f; }
+          for (int e = 0; e < V; ++e) {
+            if (r == rr && e == lc % V) {
+              if (ev.comp == 0) dx[r][e] = __builtin_fmaf(kdt, f, dx[r][e]);
+              else dy[r][e] = __builtin_fmaf(kdt, f, dy[r][e]);
+            }
+          }
       }
+      if constexpr (MODE == kConst6) {   // cells outside the grid keep their zero state
+        if (bd.edge) {
+#pragma unroll
+          for (int r = 0; r < RW; ++r)
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+              if (!(bd.row_in(I, r) && bd.col_in[e])) { dx[r][e] = 0.f; dy[r][e] = 0.f; }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          advance(u[0][r][e], c[0][r][e], v[0][r][e], dx[r][e]);
+          advance(u[1][r][e], c[1][r][e], v[1][r][e], dy[r][e]);
+        }
     }
-#pragma unroll
-    for (int r = 0; r < RW; ++r)
-#pragma unroll
-      for (int e = 0; e < V; ++e) {
-        advance(u[0][r][e], c[0][r][e], v[0][r][e], dx[r][e]);
-        advance(u[1][r][e], c[1][r][e], v[1][r][e], dy[r][e]);
-      }
 
     // store the output cells of the levels the schedule keeps (kDOnly: the
     // bank took the operator value above); a reversed-u plan also keeps each
