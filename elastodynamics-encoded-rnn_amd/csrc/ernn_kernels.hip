@@ -234,15 +234,6 @@ __device__ __forceinline__ void advance(float& u, float& c, float& v, float d) {
   u = t;
 }
 
-// The compensated sum of advance run backward: (u, c) of the level below,
-// u' + c' = u + c - v, with v the increment that produced this level.
-__device__ __forceinline__ void retreat(float& u, float& c, float v) {
-#pragma clang fp contract(off)
-  const float y = c - v;
-  const float t = u + y;
-  c = y - (t - u);
-  u = t;
-}
 
 // --------------------------------------------------------------------------
 // Buffer addressing (time-blocked kernels).  A padded field is one buffer
@@ -1660,21 +1651,20 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 // with reverse_u).  A launch runs the reverse steps m = m0 .. m0 - ks + 1 of
 // lam as adj_block_body<kDOnly> does and, beside it, runs the forward
 // recurrence backward in time instead of reading a recompute bank.  The
-// state (u^m, c^m, v^{m+1}) of a cell gives, per step:
+// state (u^m, v^{m+1}) of a cell gives, per step:
 //   L^m = (dt^2/rho) A(C0) u^m          the forward's operator value at step m
 //   v^m = v^{m+1} - (D L^m + (dt^2/rho) f^m)   (forward: v^{m+1} = v^m + d^m)
-//   u^{m-1} + c^{m-1} = u^m + c^m - v^m      (the forward's compensated sum, reversed)
+//   u^{m-1} = u^m - v^m                  (forward: u^m = u^{m-1} + v^m)
 // and dJ/dD takes lam^{m+1} . L^m.  A segment starts from the checkpoint of
-// its top level (u, c, v of that level, v^T taken as is).  Between launches
-// the lam and u rings carry (lam, w) and (u, v) only; both compensations
-// restart at zero, which saves a third of the launch's HBM traffic for one
-// rounding of lam and of u every KS steps (within a launch the sums stay
-// compensated).  The leapfrog
+// its top level (u + c and v of that level, v^T taken as is).  The leapfrog
 // recurrence is time-symmetric, so the reversed states stay within rounding
-// of the forward's; u's halo shrinks with lam's, one cell per step.  Point
-// forces and residuals are added to the velocities before each step's row
-// loop, so the loop is one branch-free block of arithmetic whose rows are
-// stepped as soon as their stencils are formed.
+// of the forward's; u's halo shrinks with lam's, one cell per step.  Both
+// recurrences run in plain increment form (no compensation): each segment
+// restarts from an exact checkpoint, and the rings between launches carry
+// (lam, w) and (u, v) only.  Point forces and residuals are added to the
+// velocities before each step's row loop, so the loop is one branch-free
+// block of arithmetic whose rows are stepped as soon as their stencils are
+// formed.
 // --------------------------------------------------------------------------
 template <int V, int RW, int KS>
 __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) {
@@ -1721,8 +1711,8 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     uin[q] = field_rsrc(state_field(C, I, uslot, q), I.field);
   }
   const Rsrc rD = field_rsrc(C.ws + I.coef_off, I.field);
-  float l[2][RW][V], c[2][RW][V], wv[2][RW][V], dd[RW][V];
-  float uu[2][RW][V], cu[2][RW][V], vu[2][RW][V];
+  float l[2][RW][V], wv[2][RW][V], dd[RW][V];
+  float uu[2][RW][V], vu[2][RW][V];
 #pragma unroll
   for (int r = 0; r < RW; ++r) {
     const uint32_t so = bd.srow(I, r);
@@ -1731,15 +1721,21 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         l[q][r][e] = bload(rin[q], bd.vcol[e], so);
-        c[q][r][e] = 0.f;   // lam's compensation, like u's, restarts every launch
         wv[q][r][e] = bload(rin[4 + q], bd.vcol[e], so);
         uu[q][r][e] = bload(uin[q], bd.vcol[e], so);
-        // the u ring between launches carries (u, v) only: the compensation
-        // restarts at zero (the checkpoints keep theirs)
-        cu[q][r][e] = from_ck ? bload(uin[2 + q], bd.vcol[e], so) : 0.f;
         vu[q][r][e] = bload(uin[4 + q], bd.vcol[e], so);
       }
       dd[r][e] = bload(rD, bd.vcol[e], so);
+    }
+  }
+  if (from_ck) {   // a checkpoint's u + c, rounded once (the rings carry (u, v) only)
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const uint32_t so = bd.srow(I, r);
+#pragma unroll
+      for (int e = 0; e < V; ++e)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) uu[q][r][e] = uu[q][r][e] + bload(uin[2 + q], bd.vcol[e], so);
     }
   }
   float gacc[RW][V];   // per-cell damage gradient of this launch (FP32 over at most KS steps)
@@ -1765,7 +1761,7 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 
   // unrolled: the row window keeps its registers from step to step (two
   // shapes stay rolled to fit three wavefronts per SIMD without spilling)
-  constexpr int kUnroll = (RW == 8 && (KS == 4 || KS == 6)) ? 1 : KS;
+  constexpr int kUnroll = KS;
 #pragma unroll kUnroll
   for (int s = 0; s < KS; ++s) {
     if (s >= a.ks) continue;   // block-uniform
@@ -1875,14 +1871,16 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           // outside the output tile are never stored
           gacc[r][e] += __builtin_fmaf(l[0][r][e], lx, l[1][r][e] * ly);
           // step row r: lam to level m, u to level m - 1
-          advance(l[0][r][e], c[0][r][e], wv[0][r][e], dx);
-          advance(l[1][r][e], c[1][r][e], wv[1][r][e], dy);
+          wv[0][r][e] = wv[0][r][e] + dx;   // lam in increment form (no compensation)
+          wv[1][r][e] = wv[1][r][e] + dy;
+          l[0][r][e] = l[0][r][e] + wv[0][r][e];
+          l[1][r][e] = l[1][r][e] + wv[1][r][e];
           if (!first) {   // v^m = v^{m+1} - d^m, d^m = D L^m + (dt^2/rho) f^m
             vu[0][r][e] = vu[0][r][e] - dd[r][e] * lx;
             vu[1][r][e] = vu[1][r][e] - dd[r][e] * ly;
           }
-          retreat(uu[0][r][e], cu[0][r][e], vu[0][r][e]);
-          retreat(uu[1][r][e], cu[1][r][e], vu[1][r][e]);
+          uu[0][r][e] = uu[0][r][e] - vu[0][r][e];   // u^{m-1} = u^m - v^m
+          uu[1][r][e] = uu[1][r][e] - vu[1][r][e];
         }
         xm = xc; xc = xp; ym = yc; yc = yp;
         pxm = pxc; pxc = pxp; pym = pyc; pyc = pyp;
@@ -2200,7 +2198,7 @@ __global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockA
 
 // Reversed-u adjoint (damage field over known constants; rev_block_body).
 template <int V, int RW, int KS>
-__global__ void __launch_bounds__(kWave * blk_waves(RW)) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) __attribute__((amdgpu_waves_per_eu(RW <= 6 ? 4 : 3)))
 rev_block_kernel(BlockAdjArgs) {
   rev_block_body<V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
@@ -2475,7 +2473,7 @@ static hipError_t launch_adj_block_m(const BlockAdjArgs& a, int V, int grid, hip
 // Reversed-u adjoint shapes (rev_block_ok in ernn_plan.cpp): V = 1, RW in
 // {4, 6, 8} (lam and u both held in registers), KS in {2, 4}.
 #define ERNN_REV_BLOCK_CASES(X) \
-  X(1, 4, 2) X(1, 4, 4) X(1, 6, 2) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 5) X(1, 8, 6)
+  X(1, 4, 2) X(1, 4, 4) X(1, 6, 2) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 5) X(1, 8, 6) X(1, 10, 5) X(1, 10, 6)
 #define ERNN_REV_BLOCK_CASE(v_, r_, k_)                                                             \
   if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
     hipLaunchKernelGGL((rev_block_kernel<v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \

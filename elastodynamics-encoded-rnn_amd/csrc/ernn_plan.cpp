@@ -142,9 +142,10 @@ bool fused_ff_ok(int V, int RW, int KS, int RWF) {
   return RWF == RW || (V == 1 && RW == 4 && RWF == 8 && (KS == 2 || KS == 4));
 }
 // Reversed-u adjoint shapes (ERNN_REV_BLOCK_CASES): V = 1, RW in {4, 6, 8}, KS in {2, 4}, and
-// (1, 8, 5), (1, 8, 6).
+// RW in {8, 10}, KS in {5, 6}.
 bool rev_block_ok(int V, int RW, int KS) {
-  return V == 1 && (((RW == 4 || RW == 6 || RW == 8) && (KS == 2 || KS == 4)) || (RW == 8 && (KS == 5 || KS == 6)));
+  return V == 1 && (((RW == 4 || RW == 6 || RW == 8) && (KS == 2 || KS == 4)) ||
+                    ((RW == 8 || RW == 10) && (KS == 5 || KS == 6)));
 }
 bool ablk_ok(int mode, bool fuse, int V, int RW, int KS) {
   return fuse ? fused_block_ok(mode, V, RW, KS) : adj_block_ok(mode, V, RW, KS);
@@ -510,7 +511,7 @@ int set_fwd_blocking(Plan* p, int V, int RW, int KS) {
 int set_adj_blocking(Plan* p, int V, int RW, int KS, int RWF) {
   if (p->rtr && (KS <= 1 || !rev_block_ok(V, RW, KS)))
     return fail(ERNN_E_INVALID, "a reverse_u plan runs the reversed adjoint: (1, RW, KS), RW in {4, 6, 8}, KS in {2, 4}, "
-                                "or (1, 8, 5), (1, 8, 6)");
+                                "or RW in {8, 10}, KS in {5, 6}");
   if (KS <= 1) {
     free_blocking(p->blk_adj);
     free_blocking(p->blk_ff);
@@ -587,7 +588,7 @@ int validate(const ernn_desc_t* d) {
   if (d->reverse_u && !d->fixed_c0) return fail(ERNN_E_INVALID, "reverse_u needs fixed_c0 = 1");
   if (d->reverse_u && (d->ablk_ks <= 1 || !rev_block_ok(d->ablk_v, d->ablk_rw, d->ablk_ks)))
     return fail(ERNN_E_INVALID, "reverse_u needs a time-blocked adjoint of shape (1, RW, KS), RW in {4, 6, 8}, "
-                                "KS in {2, 4}, or (1, 8, 5), (1, 8, 6)");
+                                "KS in {2, 4}, or RW in {8, 10}, KS in {5, 6}");
   if (d->ablk_ff_rw != 0 && (d->ablk_ks <= 1 || !d->fuse || !fused_ff_ok(d->ablk_v, d->ablk_rw, d->ablk_ks, d->ablk_ff_rw)))
     return fail(ERNN_E_INVALID, "ablk_ff_rw: 0, or on a fuse plan with a time-blocked adjoint the adjoint's RW, or 8 "
                                 "beside RW = 4 with KS in {2,4}");

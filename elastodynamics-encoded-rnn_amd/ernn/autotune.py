@@ -116,7 +116,7 @@ ADJ_BLOCK_SHAPES_FIELD6: Tuple[Tuple[int, int, int], ...] = ((1, 2, 2), (1, 2, 4
 FUSED_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 8, 4))
 # reversed-u adjoint shapes (reverse_u plans)
 REV_BLOCK_SHAPES: Tuple[Tuple[int, int, int], ...] = ((1, 4, 2), (1, 4, 4), (1, 6, 2), (1, 6, 4), (1, 8, 2),
-                                                      (1, 8, 4), (1, 8, 5), (1, 8, 6))
+                                                      (1, 8, 4), (1, 8, 5), (1, 8, 6), (1, 10, 5), (1, 10, 6))
 # largest V * R and R of the built adjoint kernels per mode (larger shapes would spill registers)
 ADJ_MAX_CELLS = {"const6": 16, "dfield": 8, "field6": 4}
 ADJ_MAX_ROWS = {"const6": 16, "dfield": 4, "field6": 2}

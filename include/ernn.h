@@ -123,7 +123,7 @@ typedef struct {
                            the way.  The checkpoints hold (u, c, v) at every K-th level and
                            each instance's last level.  Requires a time-blocked adjoint
                            (ablk_v, ablk_rw, ablk_ks) = (1, RW, KS), RW in {4, 6, 8}, KS in
-                           {2, 4}, or (1, 8, 5), (1, 8, 6). */
+                           {2, 4}, or RW in {8, 10}, KS in {5, 6}. */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;
