@@ -25,6 +25,7 @@
 // restatement of the reference (PI:44-106, PI:286-291), well inside the 1e-5
 // parity bar, and every cell's arithmetic is fixed, so results do not depend
 // on the tiling.
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include "ernn_internal.h"
 #include "ernn_device.h"   // address-space views, register rows, tile order
@@ -96,6 +97,31 @@ __device__ __forceinline__ UDeriv uderivs(const Row<V>& xm, const Row<V>& xc, co
   d.xx_y = uxx<V>(ym, yc, yp, x);
   d.yy_y = uyy<V>(yc, x);
   d.xy_y = uxy<V>(ym, yp, x);
+  return d;
+}
+
+// One row's taps at a lane's single cell (V = 1): the value, the sum of its
+// column neighbours and their difference (x+1 minus x-1).  Formed once per row
+// from two DPP shifts (the x+1 shift folds into the add and the subtract), they
+// serve the row's own Uyy and its neighbour rows' Uxy; tap_derivs gives the
+// bits of uderivs<1>.
+struct Taps {
+  float c, s, d;
+};
+__device__ __forceinline__ Taps make_taps(float v) {
+  const float lft = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+  const float rgt = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+  return Taps{v, lft + rgt, rgt - lft};
+}
+__device__ __forceinline__ UDeriv tap_derivs(const Taps& xm, const Taps& xc, const Taps& xp, const Taps& ym,
+                                             const Taps& yc, const Taps& yp) {
+  UDeriv d;
+  d.xx_x = __builtin_fmaf(-2.f, xc.c, xm.c + xp.c);
+  d.yy_x = __builtin_fmaf(-2.f, xc.c, xc.s);
+  d.xy_x = xp.d - xm.d;
+  d.xx_y = __builtin_fmaf(-2.f, yc.c, ym.c + yp.c);
+  d.yy_y = __builtin_fmaf(-2.f, yc.c, yc.s);
+  d.xy_y = yp.d - ym.d;
   return d;
 }
 
@@ -1065,15 +1091,24 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
           bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
           by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
         }
-        Row<V> xm = make_row_z<V>(tx), ym = make_row_z<V>(ty);
-        Row<V> xc = make_row_z<V>(u[0][0]), yc = make_row_z<V>(u[1][0]);
+        // one cell per lane: rows travel as Taps (one DPP shift per row and
+        // component instead of two); wider lanes keep their Row of V + 2 cells
+        using RowT = std::conditional_t<V == 1, Taps, Row<V>>;
+        auto mk = [&](const float (&vals)[V]) -> RowT {
+          if constexpr (V == 1) return make_taps(vals[0]);
+          else return make_row_z<V>(vals);
+        };
+        RowT xm = mk(tx), ym = mk(ty);
+        RowT xc = mk(u[0][0]), yc = mk(u[1][0]);
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
-          const Row<V> xp = (r + 1 < RW) ? make_row_z<V>(u[0][r + 1]) : make_row_z<V>(bx);
-          const Row<V> yp = (r + 1 < RW) ? make_row_z<V>(u[1][r + 1]) : make_row_z<V>(by);
+          const RowT xp = (r + 1 < RW) ? mk(u[0][r + 1]) : mk(bx);
+          const RowT yp = (r + 1 < RW) ? mk(u[1][r + 1]) : mk(by);
 #pragma unroll
           for (int e = 0; e < V; ++e) {
-            const UDeriv d = uderivs<V>(xm, xc, xp, ym, yc, yp, e + 1);
+            UDeriv d;
+            if constexpr (V == 1) d = tap_derivs(xm, xc, xp, ym, yc, yp);
+            else d = uderivs<V>(xm, xc, xp, ym, yc, yp, e + 1);
             float ox, oy;   // the arithmetic of fwd_op<kDField>
             apply_op_sum(kf, d, ox, oy);
             const float dx = cf[r][0][e] * ox, dy = cf[r][0][e] * oy;
@@ -1672,7 +1707,7 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   const auto& a = *ap;
   constexpr int kCols = kWave * V;
   constexpr int WB = blk_waves(RW);
-  static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
+  static_assert(V == 1, "one cell per lane: the rows travel as Taps");
   // band-edge rows: [buffer][wavefront][top/bottom][D lam_x, D lam_y, u_x, u_y][column]
   __shared__ float xch[2][WB][2][4][kCols];
 
@@ -1835,37 +1870,30 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           e0[q][e] = (w > 0) ? xch[buf][w - 1][1][q][lane * V + e] : 0.f;
           e1[q][e] = (w < WB - 1) ? xch[buf][w + 1][0][q][lane * V + e] : 0.f;
         }
-      auto dl_row = [&](int r, int q) {
-        float t[V];
-#pragma unroll
-        for (int e = 0; e < V; ++e) t[e] = dd[r][e] * l[q][r][e];
-        return make_row_z<V>(t);
-      };
-      Row<V> xm = make_row_z<V>(e0[0]), ym = make_row_z<V>(e0[1]);
-      Row<V> pxm = make_row_z<V>(e0[2]), pym = make_row_z<V>(e0[3]);
-      Row<V> xc = dl_row(0, 0), yc = dl_row(0, 1);
-      Row<V> pxc = make_row_z<V>(uu[0][0]), pyc = make_row_z<V>(uu[1][0]);
+      Taps xm = make_taps(e0[0][0]), ym = make_taps(e0[1][0]);
+      Taps pxm = make_taps(e0[2][0]), pym = make_taps(e0[3][0]);
+      Taps xc = make_taps(dd[0][0] * l[0][0][0]), yc = make_taps(dd[0][0] * l[1][0][0]);
+      Taps pxc = make_taps(uu[0][0][0]), pyc = make_taps(uu[1][0][0]);
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
-        Row<V> xp, yp, pxp, pyp;
+        Taps xp, yp, pxp, pyp;
         if (r + 1 < RW) {
-          xp = dl_row(r + 1, 0);
-          yp = dl_row(r + 1, 1);
-          pxp = make_row_z<V>(uu[0][r + 1]);
-          pyp = make_row_z<V>(uu[1][r + 1]);
+          xp = make_taps(dd[r + 1][0] * l[0][r + 1][0]);
+          yp = make_taps(dd[r + 1][0] * l[1][r + 1][0]);
+          pxp = make_taps(uu[0][r + 1][0]);
+          pyp = make_taps(uu[1][r + 1][0]);
         } else {
-          xp = make_row_z<V>(e1[0]);
-          yp = make_row_z<V>(e1[1]);
-          pxp = make_row_z<V>(e1[2]);
-          pyp = make_row_z<V>(e1[3]);
+          xp = make_taps(e1[0][0]);
+          yp = make_taps(e1[1][0]);
+          pxp = make_taps(e1[2][0]);
+          pyp = make_taps(e1[3][0]);
         }
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          const int x = e + 1;
           float dx, dy, lx, ly;
-          const UDeriv dlam = uderivs<V>(xm, xc, xp, ym, yc, yp, x);
+          const UDeriv dlam = tap_derivs(xm, xc, xp, ym, yc, yp);
           apply_op_sum(kl, dlam, dx, dy);
-          const UDeriv du = uderivs<V>(pxm, pxc, pxp, pym, pyc, pyp, x);
+          const UDeriv du = tap_derivs(pxm, pxc, pxp, pym, pyc, pyp);
           apply_op_sum(kl, du, lx, ly);   // L^m, the forward's operator value
           // dJ/dD with lam^{m+1}, not yet stepped (zero at m = nt); rows
           // outside the output tile are never stored
