@@ -125,6 +125,27 @@ __device__ __forceinline__ UDeriv tap_derivs(const Taps& xm, const Taps& xc, con
   return d;
 }
 
+// Register rows of the time-blocked kernels: Taps for one cell per lane, the
+// V + 2 cells of Row otherwise; the same bits either way.
+template <int V>
+using RowT = std::conditional_t<V == 1, Taps, Row<V>>;
+template <int V>
+__device__ __forceinline__ RowT<V> mk_row(const float (&c)[V]) {
+  if constexpr (V == 1) return make_taps(c[0]);
+  else return make_row_z<V>(c);
+}
+template <int V>
+__device__ __forceinline__ UDeriv row_derivs(const RowT<V>& xm, const RowT<V>& xc, const RowT<V>& xp,
+                                             const RowT<V>& ym, const RowT<V>& yc, const RowT<V>& yp, int x) {
+  if constexpr (V == 1) return tap_derivs(xm, xc, xp, ym, yc, yp);
+  else return uderivs<V>(xm, xc, xp, ym, yc, yp, x);
+}
+template <int V>
+__device__ __forceinline__ float row_val(const RowT<V>& r, int x) {
+  if constexpr (V == 1) return r.c;
+  else return r.v[x];
+}
+
 // Operator coefficients for unit-weight derivatives, scaled by fa = f a:
 // k_p = fa C_p on the xx / yy taps, q_* = (fa / 4) C on the mixed taps.  Each
 // is formed in FP64 and rounded once to FP32, so the effective stiffness of
@@ -1122,7 +1143,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
     }
     if (!stepped) {
       // rows -1 .. RW of this wavefront's band; the region's own edge rows get zeros
-      Row<V> xr[RW + 2], yr[RW + 2];
+      RowT<V> xr[RW + 2], yr[RW + 2];
       {
         float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
@@ -1132,14 +1153,14 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
           bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
           by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
         }
-        xr[0] = make_row_z<V>(tx);
-        yr[0] = make_row_z<V>(ty);
-        xr[RW + 1] = make_row_z<V>(bx);
-        yr[RW + 1] = make_row_z<V>(by);
+        xr[0] = mk_row<V>(tx);
+        yr[0] = mk_row<V>(ty);
+        xr[RW + 1] = mk_row<V>(bx);
+        yr[RW + 1] = mk_row<V>(by);
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
-          xr[r + 1] = make_row_z<V>(u[0][r]);
-          yr[r + 1] = make_row_z<V>(u[1][r]);
+          xr[r + 1] = mk_row<V>(u[0][r]);
+          yr[r + 1] = mk_row<V>(u[1][r]);
         }
       }
 
@@ -1153,7 +1174,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
       for (int r = 0; r < RW; ++r)
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          const UDeriv d = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1);
+          const UDeriv d = row_derivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], e + 1);
           float cq[NC > 0 ? NC : 1];
 #pragma unroll
           for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
@@ -1459,7 +1480,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     const int buf = s & 1;
     if (m <= I.nt) {   // block-uniform: an instance with fewer steps starts later, from its zero state
       // ---- stencil rows of the (weighted) adjoint field: own rows and the band-edge rows
-      Row<V> xr[RW + 2], yr[RW + 2];
+      RowT<V> xr[RW + 2], yr[RW + 2];
       {
         float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
@@ -1469,26 +1490,26 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
           by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
         }
-        xr[0] = make_row_z<V>(tx);
-        yr[0] = make_row_z<V>(ty);
-        xr[RW + 1] = make_row_z<V>(bx);
-        yr[RW + 1] = make_row_z<V>(by);
+        xr[0] = mk_row<V>(tx);
+        yr[0] = mk_row<V>(ty);
+        xr[RW + 1] = mk_row<V>(bx);
+        yr[RW + 1] = mk_row<V>(by);
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
           float rx[V], ry[V];
 #pragma unroll
           for (int e = 0; e < V; ++e) { rx[e] = dd[r][e] * l[0][r][e]; ry[e] = dd[r][e] * l[1][r][e]; }
-          xr[r + 1] = make_row_z<V>(rx);
-          yr[r + 1] = make_row_z<V>(ry);
+          xr[r + 1] = mk_row<V>(rx);
+          yr[r + 1] = mk_row<V>(ry);
         }
       }
-      Row<V> uxr[kUS ? RW + 2 : 1], uyr[kUS ? RW + 2 : 1];
+      RowT<V> uxr[kUS ? RW + 2 : 1], uyr[kUS ? RW + 2 : 1];
       if constexpr (kUS) {
         if (grad_rows) {
 #pragma unroll
           for (int r = 0; r < RW + 2; ++r) {
-            uxr[r] = make_row_z<V>(urx[r]);
-            uyr[r] = make_row_z<V>(ury[r]);
+            uxr[r] = mk_row<V>(urx[r]);
+            uyr[r] = mk_row<V>(ury[r]);
           }
         }
       }
@@ -1499,17 +1520,17 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 #pragma unroll
         for (int e = 0; e < V; ++e) {
           const int x = e + 1;
-          const UDeriv dl = uderivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], x);
+          const UDeriv dl = row_derivs<V>(xr[r], xr[r + 1], xr[r + 2], yr[r], yr[r + 1], yr[r + 2], x);
           apply_op_sum(kl, dl, dx[r][e], dy[r][e]);
           // gradient terms on the band rows of the output tile (lam^{m+1} with u^m)
           if (r >= bd.own_lo && r < bd.own_hi) {
             if constexpr (kL) {   // the last level's term (lam^{nt+1} = 0) is skipped: its bank slot is unwritten
               if (m < I.nt) gacc[r][e] += __builtin_fmaf(l[0][r][e], urx[r][e], l[1][r][e] * ury[r][e]);
             } else {
-              sbp_sums(dl, kUS ? uxr[r + 1].v[x] : urx[r][e], kUS ? uyr[r + 1].v[x] : ury[r][e], S);
+              sbp_sums(dl, kUS ? row_val<V>(uxr[r + 1], x) : urx[r][e], kUS ? row_val<V>(uyr[r + 1], x) : ury[r][e], S);
             }
             if constexpr (kUS) {
-              const UDeriv du = uderivs<V>(uxr[r], uxr[r + 1], uxr[r + 2], uyr[r], uyr[r + 1], uyr[r + 2], x);
+              const UDeriv du = row_derivs<V>(uxr[r], uxr[r + 1], uxr[r + 2], uyr[r], uyr[r + 1], uyr[r + 2], x);
               float ox, oy;
               apply_op_sum(ku, du, ox, oy);
               gacc[r][e] += __builtin_fmaf(l[0][r][e], ox, l[1][r][e] * oy);
