@@ -39,7 +39,8 @@ Parity pinning of each function (tests/test_oracle_fp64.py):
   apply_AT          dot-product test <A u, v> = <u, A^T v> for per-cell C.
   forward           u^1 = dt^2 f^0 / rho (recording offset), linearity in
                     the source, discrete energy conservation without source,
-                    1-D plane-wave dispersion relation.
+                    1-D plane-wave dispersion relation, time symmetry (states
+                    rebuilt backward from checkpoints equal the stored ones).
   adjoint_gradient  central finite differences of forward+loss, and PyTorch
                     autograd of the plain_torch restatement in float64.
 """
