@@ -1722,7 +1722,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 // block of arithmetic whose rows are stepped as soon as their stencils are
 // formed.
 // --------------------------------------------------------------------------
-template <int V, int RW, int KS>
+template <bool kJ, int V, int RW, int KS>
 __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) {
 #pragma clang fp contract(fast)
   const auto& a = *ap;
@@ -1799,6 +1799,15 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   for (int r = 0; r < RW; ++r)
 #pragma unroll
     for (int e = 0; e < V; ++e) gacc[r][e] = 0.f;
+  // joint gradient (kJ): the constant-gradient sums of adj_block_body<kDField>,
+  // <D_p(D lam^{m+1}), u^m> by parts, over this launch's steps (FP32), added to
+  // the wavefront's FP64 partial slot at the end
+  float S[kPartials];
+#pragma unroll
+  for (int p = 0; p < kPartials; ++p) S[p] = 0.f;
+  float own_row[RW];   // 1 on the band rows of the output tile (wave-uniform)
+#pragma unroll
+  for (int r = 0; r < RW; ++r) own_row[r] = (r >= bd.own_lo && r < bd.own_hi) ? 1.f : 0.f;
 
   auto publish = [&](int buf) {
 #pragma unroll
@@ -1919,6 +1928,7 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           // dJ/dD with lam^{m+1}, not yet stepped (zero at m = nt); rows
           // outside the output tile are never stored
           gacc[r][e] += __builtin_fmaf(l[0][r][e], lx, l[1][r][e] * ly);
+          if constexpr (kJ) sbp_sums(dlam, own_row[r] * uu[0][r][e], own_row[r] * uu[1][r][e], S);
           // step row r: lam to level m, u to level m - 1
           wv[0][r][e] = wv[0][r][e] + dx;   // lam in increment form (no compensation)
           wv[1][r][e] = wv[1][r][e] + dy;
@@ -1989,6 +1999,17 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
       for (int e = 0; e < V; ++e)
         if (bd.col_in[e]) acc64[so + bd.vcol[e] / (uint32_t)sizeof(float)] = acc_in[r][e] + (double)gacc[r][e];
     }
+  }
+  if constexpr (kJ) {   // constant-gradient partials of this wavefront (slot t WB + w; lane 8q holds slot q)
+    gdouble* part = reinterpret_cast<gdouble*>(C.ws) + a.partial_off + ((int64_t)t * WB + w) * kPartials + (lane >> 3);
+    double Sd[kPartials];
+#pragma unroll
+    for (int p = 0; p < kPartials; ++p) Sd[p] = bd.col_own ? (double)S[p] : 0.0;
+    const double tot = wave_sum8(Sd, lane);
+    const int q = lane >> 3;
+    const double gsa = I.gscale * (double)I.hinv2;
+    const double wq = (q == 2 || q >= 6) ? 0.25 * gsa : gsa;
+    if ((lane & 7) == 0) *part = *part + wq * tot;
   }
 }
 
@@ -2246,10 +2267,10 @@ __global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockA
 }
 
 // Reversed-u adjoint (damage field over known constants; rev_block_body).
-template <int V, int RW, int KS>
+template <bool kJ, int V, int RW, int KS>
 __global__ void __launch_bounds__(kWave * blk_waves(RW)) __attribute__((amdgpu_waves_per_eu(RW <= 6 ? 4 : 3)))
 rev_block_kernel(BlockAdjArgs) {
-  rev_block_body<V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
+  rev_block_body<kJ, V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
 
 // the argument is read in place through the kernarg segment (offset 0)
@@ -2521,16 +2542,28 @@ static hipError_t launch_adj_block_m(const BlockAdjArgs& a, int V, int grid, hip
 
 // Reversed-u adjoint shapes (rev_block_ok in ernn_plan.cpp): V = 1, RW in
 // {4, 6, 8} (lam and u both held in registers), KS in {2, 4}.
-#define ERNN_REV_BLOCK_CASES(X) \
-  X(1, 4, 2) X(1, 4, 4) X(1, 6, 2) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 5) X(1, 8, 6) X(1, 10, 5) X(1, 10, 6)
+#define ERNN_REV_BLOCK_CASES_JOINT(X) \
+  X(1, 4, 2) X(1, 4, 4) X(1, 6, 2) X(1, 6, 4) X(1, 8, 2) X(1, 8, 4) X(1, 8, 5) X(1, 8, 6)
+#define ERNN_REV_BLOCK_CASES(X) ERNN_REV_BLOCK_CASES_JOINT(X) X(1, 10, 5) X(1, 10, 6)
 #define ERNN_REV_BLOCK_CASE(v_, r_, k_)                                                             \
   if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
-    hipLaunchKernelGGL((rev_block_kernel<v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
+    hipLaunchKernelGGL((rev_block_kernel<kJ, v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
     return hipGetLastError();                                                                      \
   }
+template <bool kJ>
+static hipError_t launch_rev_block_j(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
+  if constexpr (kJ) {   // the joint sums' registers leave out the ten-row bands (they would spill)
+    ERNN_REV_BLOCK_CASES_JOINT(ERNN_REV_BLOCK_CASE)
+  } else {
+    ERNN_REV_BLOCK_CASES(ERNN_REV_BLOCK_CASE)
+  }
+  return hipErrorInvalidValue;
+}
+// a.mode: kDOnly (dJ/dD over known constants) or kDField (joint: dJ/dD and the
+// constant-gradient partials)
 hipError_t launch_rev_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
-  if (a.mode != kDOnly) return hipErrorInvalidValue;
-  ERNN_REV_BLOCK_CASES(ERNN_REV_BLOCK_CASE)
+  if (a.mode == kDOnly) return launch_rev_block_j<false>(a, V, grid, st);
+  if (a.mode == kDField) return launch_rev_block_j<true>(a, V, grid, st);
   return hipErrorInvalidValue;
 }
 #undef ERNN_REV_BLOCK_CASE

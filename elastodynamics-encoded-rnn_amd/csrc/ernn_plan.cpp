@@ -142,8 +142,9 @@ bool fused_ff_ok(int V, int RW, int KS, int RWF) {
   return RWF == RW || (V == 1 && RW == 4 && RWF == 8 && (KS == 2 || KS == 4));
 }
 // Reversed-u adjoint shapes (ERNN_REV_BLOCK_CASES): V = 1, RW in {4, 6, 8}, KS in {2, 4}, and
-// RW in {8, 10}, KS in {5, 6}.
-bool rev_block_ok(int V, int RW, int KS) {
+// RW in {8, 10}, KS in {5, 6}; the joint gradient leaves out RW = 10.
+bool rev_block_ok(int V, int RW, int KS, bool joint = false) {
+  if (joint && RW == 10) return false;   // ERNN_REV_BLOCK_CASES_JOINT
   return V == 1 && (((RW == 4 || RW == 6 || RW == 8) && (KS == 2 || KS == 4)) ||
                     ((RW == 8 || RW == 10) && (KS == 5 || KS == 6)));
 }
@@ -509,9 +510,9 @@ int set_fwd_blocking(Plan* p, int V, int RW, int KS) {
 // wavefront (0: the adjoint's RW).  Both tables are built first and replace
 // the plan's together, so a failure leaves the plan as it was.
 int set_adj_blocking(Plan* p, int V, int RW, int KS, int RWF) {
-  if (p->rtr && (KS <= 1 || !rev_block_ok(V, RW, KS)))
+  if (p->rtr && (KS <= 1 || !rev_block_ok(V, RW, KS, !p->lbank)))
     return fail(ERNN_E_INVALID, "a reverse_u plan runs the reversed adjoint: (1, RW, KS), RW in {4, 6, 8}, KS in {2, 4}, "
-                                "or RW in {8, 10}, KS in {5, 6}");
+                                "or RW in {8, 10}, KS in {5, 6} (joint gradient: not RW = 10)");
   if (KS <= 1) {
     free_blocking(p->blk_adj);
     free_blocking(p->blk_ff);
@@ -585,10 +586,12 @@ int validate(const ernn_desc_t* d) {
     return fail(ERNN_E_INVALID, "fixed_c0 applies to the damage-field mode (ERNN_PARAM_DFIELD) only");
   if (d->fixed_c0 && d->fuse) return fail(ERNN_E_INVALID, "fixed_c0 needs fuse = 0");
   if (d->reverse_u != 0 && d->reverse_u != 1) return fail(ERNN_E_INVALID, "reverse_u must be 0 or 1");
-  if (d->reverse_u && !d->fixed_c0) return fail(ERNN_E_INVALID, "reverse_u needs fixed_c0 = 1");
-  if (d->reverse_u && (d->ablk_ks <= 1 || !rev_block_ok(d->ablk_v, d->ablk_rw, d->ablk_ks)))
+  if (d->reverse_u && d->param_mode != ERNN_PARAM_DFIELD)
+    return fail(ERNN_E_INVALID, "reverse_u applies to the damage-field mode (ERNN_PARAM_DFIELD) only");
+  if (d->reverse_u && d->fuse) return fail(ERNN_E_INVALID, "reverse_u needs fuse = 0");
+  if (d->reverse_u && (d->ablk_ks <= 1 || !rev_block_ok(d->ablk_v, d->ablk_rw, d->ablk_ks, d->fixed_c0 != 1)))
     return fail(ERNN_E_INVALID, "reverse_u needs a time-blocked adjoint of shape (1, RW, KS), RW in {4, 6, 8}, "
-                                "KS in {2, 4}, or RW in {8, 10}, KS in {5, 6}");
+                                "KS in {2, 4}, or RW in {8, 10}, KS in {5, 6} (joint gradient: not RW = 10)");
   if (d->ablk_ff_rw != 0 && (d->ablk_ks <= 1 || !d->fuse || !fused_ff_ok(d->ablk_v, d->ablk_rw, d->ablk_ks, d->ablk_ff_rw)))
     return fail(ERNN_E_INVALID, "ablk_ff_rw: 0, or on a fuse plan with a time-blocked adjoint the adjoint's RW, or 8 "
                                 "beside RW = 4 with KS in {2,4}");
@@ -663,8 +666,10 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   p->use_graph = d->use_graph != 0;
   p->fuse = d->fuse != 0;
   p->lbank = d->fixed_c0 == 1 && p->mode == ERNN_PARAM_DFIELD;
-  p->rtr = p->lbank && d->reverse_u == 1;
-  p->kmode = p->lbank ? kDOnly : p->mode;
+  p->rtr = p->mode == ERNN_PARAM_DFIELD && d->reverse_u == 1;
+  // kernels over a damage field: kDOnly banks L (fixed_c0) and, with reversed u
+  // (nothing banked), runs the forward's sliding-row path with the bits of kDField
+  p->kmode = (p->lbank || p->rtr) ? kDOnly : p->mode;
   p->n_coef = p->mode == ERNN_PARAM_DFIELD ? 1 : (p->mode == ERNN_PARAM_FIELD6 ? 6 : 0);
   p->n_acc = p->n_coef;
   p->inst_h.assign(d->inst, d->inst + d->n_inst);
@@ -917,7 +922,7 @@ BlockAdjArgs block_adj_args(const Plan* p) {
   a.call = p->d_call;
   a.partial_off = p->partial_off;
   a.n_blocks = p->blk_adj.n_blocks;
-  a.mode = p->kmode;
+  a.mode = (p->rtr && !p->lbank) ? kDField : p->kmode;   // reversed u: joint or dJ/dD only
   a.stride = p->P;
   a.snapshot = p->snapshot;
   a.rw = p->blk_adj.RW;

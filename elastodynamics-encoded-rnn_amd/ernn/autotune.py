@@ -187,6 +187,8 @@ def tune_blocking(engine, params, field, wavelets, steps: int = 64, which: int =
     fused = which == 1 and bool(engine.info.fuse)
     if which == 1 and engine.reverse_u:
         cands = list(REV_BLOCK_SHAPES if shapes is None else shapes)
+        if not getattr(engine, "fixed_c0", True):   # the joint gradient is not built for ten-row bands
+            cands = [c for c in cands if c[1] != 10]
     elif which == 0:
         shapes = BLOCK_SHAPES if shapes is None else shapes
         cands = [(0, 0, 0)] + [b for b in shapes if mode == "const6" or b[:2] != (2, 8)]

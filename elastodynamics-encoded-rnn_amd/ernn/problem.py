@@ -202,12 +202,13 @@ class Survey:
     # u^m, and the adjoint reads it pointwise instead of taking a stencil of
     # u (no constant-gradient sums).  Needs the unfused backward.
     fixed_c0: bool = False
-    # With fixed_c0: no bank and no recompute.  The adjoint runs the forward
-    # recurrence backward in time from each segment's top checkpoint (the
-    # leapfrog step is time-symmetric) and forms (dt^2/rho) A(C0) u^m on the
-    # way.  Needs a reversed time-blocked adjoint shape (REV_BLOCK_SHAPES;
-    # (0, 0, 0) = (1, 4, 4)); the checkpoint interval bounds how far each
-    # reconstruction runs from its checkpoint (ckpt_every 0: REV_SEGMENT).
+    # dfield (joint, or with fixed_c0): no bank and no recompute.  The adjoint
+    # runs the forward recurrence backward in time from each segment's top
+    # checkpoint (the leapfrog step is time-symmetric) and forms
+    # (dt^2/rho) A(C0) u^m on the way.  Needs the unfused backward and a
+    # reversed time-blocked adjoint shape (REV_BLOCK_SHAPES; (0, 0, 0) =
+    # (1, 4, 4)); the checkpoint interval bounds how far each reconstruction
+    # runs from its checkpoint (ckpt_every 0: REV_SEGMENT).
     reverse_u: bool = False
 
     def __post_init__(self):
@@ -217,12 +218,14 @@ class Survey:
             raise ValueError("fixed_c0 applies to the dfield mode only")
         if self.fixed_c0 and self.fuse:
             raise ValueError("fixed_c0 needs the unfused backward (fuse False or None)")
-        if self.reverse_u and not self.fixed_c0:
-            raise ValueError("reverse_u needs fixed_c0 (dfield over known constants)")
+        if self.reverse_u and self.mode != "dfield":
+            raise ValueError("reverse_u applies to the dfield mode only")
+        if self.reverse_u and self.fuse:
+            raise ValueError("reverse_u needs the unfused backward (fuse False or None)")
 
     @property
     def fused(self) -> bool:
-        if self.fixed_c0:
+        if self.fixed_c0 or self.reverse_u:
             return False
         if self.fuse is not None:
             return bool(self.fuse)

@@ -116,14 +116,15 @@ typedef struct {
                            (dt^2/rho) A(C0) u^m instead of u^m (same size), and the adjoint
                            reads it pointwise instead of taking a stencil of u.  Requires
                            fuse = 0; grad_params is not written. */
-  int32_t reverse_u;    /* with fixed_c0 = 1, 0 or 1.  1: no bank and no recompute; the
-                           adjoint runs the forward recurrence backward in time from the
-                           checkpoints beside lam (the leapfrog step is time-symmetric:
-                           u^{m-1} = u^m - v^m, v^m = v^{m+1} - d^m), forming A(C0) u^m on
-                           the way.  The checkpoints hold (u, c, v) at every K-th level and
-                           each instance's last level.  Requires a time-blocked adjoint
-                           (ablk_v, ablk_rw, ablk_ks) = (1, RW, KS), RW in {4, 6, 8}, KS in
-                           {2, 4}, or RW in {8, 10}, KS in {5, 6}. */
+  int32_t reverse_u;    /* ERNN_PARAM_DFIELD (joint, or with fixed_c0 = 1), 0 or 1.  1: no
+                           bank and no recompute; the adjoint runs the forward recurrence
+                           backward in time from the checkpoints beside lam (the leapfrog
+                           step is time-symmetric: u^{m-1} = u^m - v^m, v^m = v^{m+1} - d^m),
+                           forming A(C0) u^m on the way (and, for the joint gradient, the
+                           constant-gradient sums).  The checkpoints hold (u, c, v) at every
+                           K-th level and each instance's last level.  Requires fuse = 0 and
+                           a time-blocked adjoint (ablk_v, ablk_rw, ablk_ks) = (1, RW, KS),
+                           RW in {4, 6, 8}, KS in {2, 4}, or RW in {8, 10}, KS in {5, 6}. */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;

@@ -121,6 +121,80 @@ def test_reverse_u_uneven_batch_and_snapshot(K):
         e0.close(); e1.close()
 
 
+def _oracle_joint(shots, C0, field, G, off, stride=1, snapshot=False):
+    """FP64 (dJ/dC0, dJ/dD) over the shots of one damage field (one set)."""
+    from oracle import ernn_fp64 as orc
+    from gpu_helpers import oracle_run
+    D = np.asarray(field, np.float64)
+    gcell = 0.0
+    for b, s in enumerate(shots):
+        cells = [(np.float32(field) * np.float32(c)).astype(np.float64) for c in C0]
+        gcell = gcell + oracle_run(s, cells, stride, snapshot, G[off[b]:off[b + 1]])[1]
+    return orc.dfield_gradient(gcell, np.asarray(C0, np.float32).astype(np.float64), D)
+
+
+def _max_rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+REV_JOINT = [s for s in REV if s[1] != 10]
+
+
+@pytest.mark.parametrize("adj", REV_JOINT)
+@pytest.mark.parametrize("fwd", [(0, 0, 0), (1, 8, 5)])
+def test_reverse_u_joint_matches_banked_joint_plan_and_oracle(fwd, adj):
+    """The joint damage gradient (dJ/dC0 and dJ/dD) with u reversed: against the
+    banked joint plan and the FP64 adjoint, every joint reversed shape, K = 7,
+    47 steps, sources and receivers on block edges."""
+    nx, ny, nt, h = 45, 150, 47, 0.2e-3
+    rng = np.random.default_rng(97)
+    shot = make_shot(nx, ny, nt, h, sources=_edge_sources(nx, ny, h, 1, adj[1], adj[2]),
+                     receivers=_receivers(nx, ny, 1, adj[1], adj[2]))
+    D = rng.uniform(0.4, 1.0, (nx, ny)).astype(np.float32)
+    G = rng.standard_normal((2, nt, len(shot.receivers)))
+    banked = adj if adj in DFIELD_ADJ else (1, 4, 4)
+    base = ernn.Survey([shot], mode="dfield", ckpt_every=7, tiles=(2, 2, 1, 4, 0, 0), fuse=False, blocking=fwd,
+                       adj_blocking=banked)
+    e0, t0, g0 = run_hip(base, params=C_ANISO, field=D, G=G)
+    e1, t1, g1 = run_hip(dataclasses.replace(base, reverse_u=True, adj_blocking=adj), params=C_ANISO, field=D, G=G)
+    assert e1.reverse_u and not e1.fixed_c0 and np.array_equal(t0, t1)
+    gC0, gD = _oracle_joint([shot], C_ANISO, D, G.reshape(-1), [0, G.size])
+    e_c, e_d = _max_rel(g1[0], g0[0]), rel_l2(g1[1], g0[1])
+    o_c, o_d = _max_rel(g1[0], gC0), rel_l2(g1[1].reshape(nx, ny), gD)
+    print(f"reverse_u joint fwd {fwd} adj {adj}: vs banked dJ/dC0 {e_c:.2e} dJ/dD {e_d:.2e}; "
+          f"vs FP64 {o_c:.2e} / {o_d:.2e}")
+    assert e_c < 1e-5 and e_d < 1e-5
+    assert o_c < 1e-4 and o_d < 1e-4
+    e0.close(); e1.close()
+
+
+@pytest.mark.parametrize("K", [5, 60])
+def test_reverse_u_joint_uneven_batch(K):
+    """Three damage fields (sets) of different grids and step counts, whole-field
+    frames every 3 steps, joint gradient: per-set dJ/dC0 and every dJ/dD against
+    the banked joint plan."""
+    h = 0.2e-3
+    shots = [make_shot(nx, ny, nt, h, set_=k) for k, (nx, ny, nt) in enumerate(((5, 3, 20), (40, 90, 41),
+                                                                                 (33, 70, 30)))]
+    rng = np.random.default_rng(K + 300)
+    base = ernn.Survey(shots, mode="dfield", n_sets=3, ckpt_every=K, snapshot=True, rec_stride=3, fuse=False,
+                       blocking=(1, 8, 5), adj_blocking=(1, 8, 4))
+    probe = ernn.Engine(base, device=DEV)
+    off = list(probe.trace_off)
+    nfield = int(probe.info.field_floats)
+    probe.close()
+    D = rng.uniform(0.4, 1.0, nfield).astype(np.float32)
+    G = rng.standard_normal(off[-1])
+    P = np.tile(np.asarray(C_ANISO, np.float32), 3)
+    e0, t0, g0 = run_hip(base, params=P, field=D, G=G)
+    e1, t1, g1 = run_hip(dataclasses.replace(base, reverse_u=True, adj_blocking=(1, 8, 6)), params=P, field=D, G=G)
+    assert np.array_equal(t0, t1)
+    e_c, e_d = _max_rel(g1[0], g0[0]), rel_l2(g1[1], g0[1])
+    print(f"reverse_u joint K={K} uneven snapshot batch: vs banked dJ/dC0 {e_c:.2e}, dJ/dD {e_d:.2e}")
+    assert e_c < 1e-5 and e_d < 1e-5
+    e0.close(); e1.close()
+
+
 @pytest.mark.parametrize("stride", [2, 3])
 def test_reverse_u_receiver_traces_with_stride(stride):
     """Receiver traces recorded every `stride` steps (frame k = u^{kP+1}), so the
@@ -246,6 +320,17 @@ def test_reverse_u_damage_1024_full_grid():
             e = rel_l2(g[1], ref_g[1])
             print(f"damage-1024 1024^2 x 300, reverse_u, K {K}, adjoint {adj}: dJ/dD vs joint plan {e:.2e}")
             assert e < 1e-5
+    # the joint gradient with u reversed
+    for adj in ((1, 8, 6), (1, 8, 4), (1, 6, 4)):
+        sv1 = dataclasses.replace(_variant(sv, ckpt_every=autotune.REV_SEGMENT, tiles=tiles, blocking=(1, 8, 5),
+                                           adj_blocking=adj), reverse_u=True)
+        print(f"damage-1024 joint reverse_u adj {adj}", flush=True)
+        tr, g, info = _run(sv1, C0, D, G)
+        assert np.array_equal(tr, ref_tr)
+        e_c, e_d = _max_rel(g[0], ref_g[0]), rel_l2(g[1], ref_g[1])
+        print(f"damage-1024 1024^2 x 300, joint reverse_u, K 64, adjoint {adj}: vs joint plan dJ/dC0 {e_c:.2e}, "
+              f"dJ/dD {e_d:.2e}")
+        assert e_c < 1e-5 and e_d < 1e-5
 
 
 def test_reverse_u_full_length_vs_fp64_oracle():
@@ -275,6 +360,10 @@ def test_reverse_u_full_length_vs_fp64_oracle():
                            D.reshape(-1), G)
         assert np.array_equal(tr, ref_tr)
         res[K] = g[1]
+    print(f"reverse_u {nt} steps: joint gradient, K 64", flush=True)
+    trj, gj, _ = _run(dataclasses.replace(_variant(sv, ckpt_every=autotune.REV_SEGMENT, **timed), fixed_c0=False,
+                                          reverse_u=True), C0, D.reshape(-1), G)
+    assert np.array_equal(trj, ref_tr)
     cells = [D.astype(np.float64) * float(c) for c in C0]
     gcell = 0.0
     for b, s in enumerate(sv.shots):
@@ -284,7 +373,10 @@ def test_reverse_u_full_length_vs_fp64_oracle():
         gcell = gcell + orc.adjoint_gradient(cells, s.h, s.dt, s.rho, s.nt, n, n, st,
                                              G[off[b]:off[b + 1]].reshape(ref.shape), 1, s.receivers)
         del st
-    gD = orc.dfield_gradient(gcell, C0.astype(np.float64), D.astype(np.float64))[1]
+    gC0, gD = orc.dfield_gradient(gcell, C0.astype(np.float64), D.astype(np.float64))
+    e_jc, e_jd = _max_rel(gj[0], gC0), rel_l2(gj[1].reshape(n, n), gD)
+    print(f"joint reverse_u crop {n}^2 x {nt}, K 64: dJ/dC0 max rel vs FP64 {e_jc:.2e}, dJ/dD rel-L2 {e_jd:.2e}")
+    assert e_jc < 1e-4 and e_jd < 1e-4
     e_bank = rel_l2(ref_g[1].reshape(n, n), gD)
     print(f"banked fixed_c0 crop {n}^2 x {nt}, K 375: dJ/dD rel-L2 vs FP64 {e_bank:.2e}")
     for K, gd in res.items():

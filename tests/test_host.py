@@ -96,7 +96,8 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
     ({}, {"param_mode": 1, "fixed_c0": 2}, "fixed_c0 must be 0 or 1"),
     ({}, {"param_mode": 1, "fixed_c0": 1, "fuse": 1}, "fuse = 0"),
     ({}, {"param_mode": 1, "fixed_c0": 1, "reverse_u": 2}, "reverse_u must be 0 or 1"),
-    ({}, {"param_mode": 1, "reverse_u": 1, "ablk_v": 1, "ablk_rw": 4, "ablk_ks": 4}, "needs fixed_c0"),
+    ({}, {"param_mode": 0, "reverse_u": 1, "ablk_v": 1, "ablk_rw": 4, "ablk_ks": 4}, "damage-field mode"),
+    ({}, {"param_mode": 1, "reverse_u": 1, "fuse": 1, "ablk_v": 1, "ablk_rw": 4, "ablk_ks": 4}, "fuse = 0"),
     ({}, {"param_mode": 1, "fixed_c0": 1, "reverse_u": 1}, "needs a time-blocked adjoint of shape (1, RW, KS)"),
     ({}, {"param_mode": 1, "fixed_c0": 1, "reverse_u": 1, "ablk_v": 1, "ablk_rw": 8, "ablk_ks": 8},
      "needs a time-blocked adjoint of shape (1, RW, KS)"),
@@ -226,8 +227,11 @@ def test_reverse_u_checkpoints_every_rev_segment_steps_without_a_bank():
     with pytest.raises(MemoryError):
         autotune.choose_ckpt(sv, mem_cap_bytes=autotune.state_bytes(sv, nt) - 1,
                              budget_bytes=autotune.state_bytes(sv, nt) - 1)
-    with pytest.raises(ValueError, match="fixed_c0"):
-        ernn.Survey(sv.shots, mode="dfield", reverse_u=True)
+    with pytest.raises(ValueError, match="dfield"):
+        ernn.Survey(sv.shots, mode="const6", reverse_u=True)
+    with pytest.raises(ValueError, match="unfused"):
+        ernn.Survey(sv.shots, mode="dfield", reverse_u=True, fuse=True)
+    assert not ernn.Survey(sv.shots, mode="dfield", reverse_u=True).fused
 
 
 def test_padded_layout_matches_library_convention():
