@@ -90,8 +90,8 @@ def parse():
                     help="damage-1024: also invert the six background constants (joint gradient; the "
                          "workload's own setting inverts D over known constants)")
     ap.add_argument("--reverse-u", type=int, default=1,
-                    help="damage-1024 over known constants: 1 = the adjoint reconstructs u backward from the "
-                         "checkpoints (no bank, no recompute); 0 = banked L^m with segment recompute")
+                    help="damage-field workloads: 1 = the adjoint reconstructs u backward from the checkpoints "
+                         "(no bank, no recompute); 0 = banked states with segment recompute")
     ap.add_argument("--ckpt-every", type=int, default=0,
                     help="fix the checkpoint interval (profiling runs; 0 = the workload's memory budget)")
     ap.add_argument("--graph", type=int, default=1, help="replay passes as captured hipGraphs")
@@ -1088,10 +1088,11 @@ def main():
     variant = ""
     if args.joint_c0 and sv.fixed_c0:
         sv.fixed_c0, variant = False, "-joint"   # the profiles of this variant carry the suffix
-    if sv.fixed_c0 and args.reverse_u:
-        sv.reverse_u = True
-    elif sv.fixed_c0:
-        variant = "-banked"
+    if sv.mode == "dfield":   # damage field: reversed u by default, the banked schedule as a variant
+        if args.reverse_u:
+            sv.reverse_u = True
+        else:
+            variant += "-banked"
     rng = np.random.default_rng(0)
     C_true = wl.C_true
     pert = 1.0 + rng.uniform(-0.2, 0.2, size=C_true.shape)
