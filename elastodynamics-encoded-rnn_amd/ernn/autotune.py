@@ -44,8 +44,10 @@ BANK_FIELDS = 2    # u per component
 
 # Checkpoint interval of a reverse_u plan when none is named: each segment's
 # reversed reconstruction of u starts over from an exact checkpoint, so the
-# interval bounds how far rounding can carry it.
-REV_SEGMENT = 64
+# interval bounds how far rounding can carry it.  60 is a multiple of every
+# step count per launch up to 6 (2, 4, 5, 6): no segment ends in a short
+# reversed launch and the forward's checkpoints fall on launch ends.
+REV_SEGMENT = 60
 
 
 def fields(nt: int, K: int, banks: int = 1, reverse_u: bool = False) -> int:

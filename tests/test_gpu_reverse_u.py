@@ -331,7 +331,7 @@ def test_reverse_u_damage_1024_full_grid():
         tr, g, info = _run(sv1, C0, D, G)
         assert np.array_equal(tr, ref_tr)
         e_c, e_d = _max_rel(g[0], ref_g[0]), rel_l2(g[1], ref_g[1])
-        print(f"damage-1024 1024^2 x 300, joint reverse_u, K 64, adjoint {adj}: vs joint plan dJ/dC0 {e_c:.2e}, "
+        print(f"damage-1024 1024^2 x 300, joint reverse_u, K {autotune.REV_SEGMENT}, adjoint {adj}: vs joint plan dJ/dC0 {e_c:.2e}, "
               f"dJ/dD {e_d:.2e}")
         assert e_c < 1e-5 and e_d < 1e-5
 
@@ -363,7 +363,7 @@ def test_reverse_u_full_length_vs_fp64_oracle():
                            D.reshape(-1), G)
         assert np.array_equal(tr, ref_tr)
         res[K] = g[1]
-    print(f"reverse_u {nt} steps: joint gradient, K 64", flush=True)
+    print(f"reverse_u {nt} steps: joint gradient, K {autotune.REV_SEGMENT}", flush=True)
     trj, gj, _ = _run(dataclasses.replace(_variant(sv, ckpt_every=autotune.REV_SEGMENT, **timed), fixed_c0=False,
                                           reverse_u=True), C0, D.reshape(-1), G)
     assert np.array_equal(trj, ref_tr)
@@ -378,7 +378,7 @@ def test_reverse_u_full_length_vs_fp64_oracle():
         del st
     gC0, gD = orc.dfield_gradient(gcell, C0.astype(np.float64), D.astype(np.float64))
     e_jc, e_jd = _max_rel(gj[0], gC0), rel_l2(gj[1].reshape(n, n), gD)
-    print(f"joint reverse_u crop {n}^2 x {nt}, K 64: dJ/dC0 max rel vs FP64 {e_jc:.2e}, dJ/dD rel-L2 {e_jd:.2e}")
+    print(f"joint reverse_u crop {n}^2 x {nt}, K {autotune.REV_SEGMENT}: dJ/dC0 max rel vs FP64 {e_jc:.2e}, dJ/dD rel-L2 {e_jd:.2e}")
     assert e_jc < 1e-4 and e_jd < 1e-4
     e_bank = rel_l2(ref_g[1].reshape(n, n), gD)
     print(f"banked fixed_c0 crop {n}^2 x {nt}, K 375: dJ/dD rel-L2 vs FP64 {e_bank:.2e}")
