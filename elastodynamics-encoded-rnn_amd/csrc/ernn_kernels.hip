@@ -1834,6 +1834,7 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     const int buf = s & 1;
     if (m <= I.nt) {   // block-uniform: an instance with fewer steps starts later
       const bool first = from_ck && m == ustart;   // v^m is the checkpoint's own (block-uniform)
+      const float keep = first ? 0.f : 1.f;
       // Point forces and residuals enter the velocities before the row loop
       // (neither is read by a stencil), which keeps that loop free of branches:
       //   u: v^{m+1} - (dt^2/rho) f^m, the rest of d^m = D L^m follows per row;
@@ -1934,10 +1935,11 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
           wv[1][r][e] = wv[1][r][e] + dy;
           l[0][r][e] = l[0][r][e] + wv[0][r][e];
           l[1][r][e] = l[1][r][e] + wv[1][r][e];
-          if (!first) {   // v^m = v^{m+1} - d^m, d^m = D L^m + (dt^2/rho) f^m
-            vu[0][r][e] = vu[0][r][e] - dd[r][e] * lx;
-            vu[1][r][e] = vu[1][r][e] - dd[r][e] * ly;
-          }
+          // v^m = v^{m+1} - d^m, d^m = D L^m + (dt^2/rho) f^m; nothing at the
+          // segment's first level (its v is the checkpoint's): a 0/1 factor on D
+          const float dk = keep * dd[r][e];
+          vu[0][r][e] = vu[0][r][e] - dk * lx;
+          vu[1][r][e] = vu[1][r][e] - dk * ly;
           uu[0][r][e] = uu[0][r][e] - vu[0][r][e];   // u^{m-1} = u^m - v^m
           uu[1][r][e] = uu[1][r][e] - vu[1][r][e];
         }
