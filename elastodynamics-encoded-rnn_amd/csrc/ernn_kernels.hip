@@ -1730,7 +1730,9 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   constexpr int WB = blk_waves(RW);
   static_assert(V == 1, "one cell per lane: the rows travel as Taps");
   // band-edge rows: [buffer][wavefront][top/bottom][D lam_x, D lam_y, u_x, u_y][column]
-  __shared__ float xch[2][WB][2][4][kCols];
+  // slots 1 .. WB hold the wavefronts' rows; slots 0 and WB + 1 stay zero (the
+  // region's outer edges), so every wavefront reads its neighbours unconditionally
+  __shared__ float xch[2][WB + 2][2][4][kCols];
 
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x >> 6;
@@ -1814,12 +1816,23 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     for (int e = 0; e < V; ++e)
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        xch[buf][w][0][q][lane * V + e] = dd[0][e] * l[q][0][e];
-        xch[buf][w][1][q][lane * V + e] = dd[RW - 1][e] * l[q][RW - 1][e];
-        xch[buf][w][0][2 + q][lane * V + e] = uu[q][0][e];
-        xch[buf][w][1][2 + q][lane * V + e] = uu[q][RW - 1][e];
+        xch[buf][w + 1][0][q][lane * V + e] = dd[0][e] * l[q][0][e];
+        xch[buf][w + 1][1][q][lane * V + e] = dd[RW - 1][e] * l[q][RW - 1][e];
+        xch[buf][w + 1][0][2 + q][lane * V + e] = uu[q][0][e];
+        xch[buf][w + 1][1][2 + q][lane * V + e] = uu[q][RW - 1][e];
       }
   };
+  if (w < 2) {   // the guard slots of both buffers: wavefront 0 the top one, wavefront 1 the bottom one
+    const int g = w == 0 ? 0 : WB + 1;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int side = 0; side < 2; ++side)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int e = 0; e < V; ++e) xch[b][g][side][q][lane * V + e] = 0.f;
+  }
   publish(0);
   __syncthreads();
   const int nf = I.n_frames;
@@ -1898,8 +1911,8 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
       for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          e0[q][e] = (w > 0) ? xch[buf][w - 1][1][q][lane * V + e] : 0.f;
-          e1[q][e] = (w < WB - 1) ? xch[buf][w + 1][0][q][lane * V + e] : 0.f;
+          e0[q][e] = xch[buf][w][1][q][lane * V + e];
+          e1[q][e] = xch[buf][w + 2][0][q][lane * V + e];
         }
       Taps xm = make_taps(e0[0][0]), ym = make_taps(e0[1][0]);
       Taps pxm = make_taps(e0[2][0]), pym = make_taps(e0[3][0]);
