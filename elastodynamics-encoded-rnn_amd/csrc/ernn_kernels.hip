@@ -1007,7 +1007,9 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   constexpr int WB = blk_waves(RW);                  // wavefronts
   constexpr int NC = (MODE == kDField || MODE == kDOnly) ? 1 : (MODE == kField6 ? 6 : 0);
   static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
-  __shared__ float xch[2][WB][2][2][kCols];          // [buffer][wavefront][top/bottom][comp][column]
+  // [buffer][wavefront + 1][top/bottom][comp][column]: slots 0 and WB + 1 stay zero (the
+  // region's outer edges), so every wavefront reads its neighbours unconditionally
+  __shared__ float xch[2][WB + 2][2][2][kCols];
 
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x >> 6;
@@ -1079,10 +1081,21 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
     for (int e = 0; e < V; ++e)
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        xch[buf][w][0][q][lane * V + e] = u[q][0][e];
-        xch[buf][w][1][q][lane * V + e] = u[q][RW - 1][e];
+        xch[buf][w + 1][0][q][lane * V + e] = u[q][0][e];
+        xch[buf][w + 1][1][q][lane * V + e] = u[q][RW - 1][e];
       }
   };
+  if (w < 2) {   // the guard slots of both buffers: wavefront 0 the top one, wavefront 1 the bottom one
+    const int g = w == 0 ? 0 : WB + 1;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int side = 0; side < 2; ++side)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int e = 0; e < V; ++e) xch[b][g][side][q][lane * V + e] = 0.f;
+  }
   publish(0);
   __syncthreads();
 
@@ -1107,10 +1120,10 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
         float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
-          ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
-          bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
-          by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+          tx[e] = xch[buf][w][1][0][lane * V + e];
+          ty[e] = xch[buf][w][1][1][lane * V + e];
+          bx[e] = xch[buf][w + 2][0][0][lane * V + e];
+          by[e] = xch[buf][w + 2][0][1][lane * V + e];
         }
         // one cell per lane: rows travel as Taps (one DPP shift per row and
         // component instead of two); wider lanes keep their Row of V + 2 cells
@@ -1148,10 +1161,10 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
         float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
-          ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
-          bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
-          by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+          tx[e] = xch[buf][w][1][0][lane * V + e];
+          ty[e] = xch[buf][w][1][1][lane * V + e];
+          bx[e] = xch[buf][w + 2][0][0][lane * V + e];
+          by[e] = xch[buf][w + 2][0][1][lane * V + e];
         }
         xr[0] = mk_row<V>(tx);
         yr[0] = mk_row<V>(ty);
@@ -1371,7 +1384,9 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
   constexpr bool kL = (MODE == kDOnly);   // bank = (dt^2/rho) A(C0) u^m, dJ/dD only
   static_assert(MODE != kField6, "six-field adjoint: adj_block_body6");
   static_assert(V == 1 || KS % V == 0, "the output tile starts on a lane boundary");
-  __shared__ float xch[2][WB][2][2][kCols];
+  // [buffer][wavefront + 1][top/bottom][comp][column]: slots 0 and WB + 1 stay zero (the
+  // region's outer edges), so every wavefront reads its neighbours unconditionally
+  __shared__ float xch[2][WB + 2][2][2][kCols];
 
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x >> 6;
@@ -1439,8 +1454,8 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     for (int e = 0; e < V; ++e)
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        xch[buf][w][0][q][lane * V + e] = dd[0][e] * l[q][0][e];
-        xch[buf][w][1][q][lane * V + e] = dd[RW - 1][e] * l[q][RW - 1][e];
+        xch[buf][w + 1][0][q][lane * V + e] = dd[0][e] * l[q][0][e];
+        xch[buf][w + 1][1][q][lane * V + e] = dd[RW - 1][e] * l[q][RW - 1][e];
       }
   };
   // u^m of the lane's cells on the band rows in the output tile (the damage
@@ -1467,6 +1482,17 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
     }
   };
   if (grad_rows && a.m0 <= I.nt) load_u(0);
+  if (w < 2) {   // the guard slots of both buffers: wavefront 0 the top one, wavefront 1 the bottom one
+    const int g = w == 0 ? 0 : WB + 1;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int side = 0; side < 2; ++side)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int e = 0; e < V; ++e) xch[b][g][side][q][lane * V + e] = 0.f;
+  }
   publish(0);
   __syncthreads();
 
@@ -1485,10 +1511,10 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
         float tx[V], ty[V], bx[V], by[V];
 #pragma unroll
         for (int e = 0; e < V; ++e) {
-          tx[e] = (w > 0) ? xch[buf][w - 1][1][0][lane * V + e] : 0.f;
-          ty[e] = (w > 0) ? xch[buf][w - 1][1][1][lane * V + e] : 0.f;
-          bx[e] = (w < WB - 1) ? xch[buf][w + 1][0][0][lane * V + e] : 0.f;
-          by[e] = (w < WB - 1) ? xch[buf][w + 1][0][1][lane * V + e] : 0.f;
+          tx[e] = xch[buf][w][1][0][lane * V + e];
+          ty[e] = xch[buf][w][1][1][lane * V + e];
+          bx[e] = xch[buf][w + 2][0][0][lane * V + e];
+          by[e] = xch[buf][w + 2][0][1][lane * V + e];
         }
         xr[0] = mk_row<V>(tx);
         yr[0] = mk_row<V>(ty);
