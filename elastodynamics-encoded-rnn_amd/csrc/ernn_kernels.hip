@@ -2309,7 +2309,7 @@ __global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockA
 
 // Reversed-u adjoint (damage field over known constants; rev_block_body).
 template <bool kJ, int V, int RW, int KS>
-__global__ void __launch_bounds__(kWave * blk_waves(RW)) __attribute__((amdgpu_waves_per_eu(RW <= 6 ? 4 : 3)))
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) __attribute__((amdgpu_waves_per_eu((RW <= 6 || (!kJ && RW <= 8)) ? 4 : 3)))
 rev_block_kernel(BlockAdjArgs) {
   rev_block_body<kJ, V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
