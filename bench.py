@@ -321,7 +321,7 @@ def time_instances(torch, ernn, wl, C_init, D_init, dev, iters):
         one = ernn.Shot(**{**s.__dict__, "set": 0})
         sv1 = ernn.Survey([one], mode=wl.survey.mode, n_sets=1, ckpt_every=wl.survey.ckpt_every,
                           use_graph=wl.survey.use_graph, fixed_c0=wl.survey.fixed_c0,
-                          reverse_u=wl.survey.reverse_u)
+                          reverse_u=wl.survey.reverse_u, orthotropic=wl.survey.orthotropic)
         m = ernn.ERNN(sv1, C_init[s.set:s.set + 1], D_init=D_init, device=dev, learn_constants=not sv1.fixed_c0)
         for k in range(iters + 1):
             if k == 1:
@@ -1087,7 +1087,7 @@ def main():
         sv.ckpt_every = args.ckpt_every
     variant = ""
     if args.joint_c0 and sv.fixed_c0:
-        sv.fixed_c0, variant = False, "-joint"   # the profiles of this variant carry the suffix
+        sv.fixed_c0, sv.orthotropic, variant = False, False, "-joint"   # the profiles of this variant carry the suffix
     if sv.mode == "dfield":   # damage field: reversed u by default, the banked schedule as a variant
         if args.reverse_u:
             sv.reverse_u = True
@@ -1366,6 +1366,7 @@ def main():
         "config": {"workload": wl.name + variant, "grid": f"{s0.nx}x{s0.ny}", "nt": sv.nt_max,
                    "shots": len(sv.shots), "receivers": len(s0.receivers) if s0.receivers is not None else 0,
                    "mode": sv.mode, "fixed_c0": bool(sv.fixed_c0), "reverse_u": bool(sv.reverse_u),
+                   "orthotropic": bool(sv.orthotropic),
                    "ckpt_every": int(eng.info.ckpt_every),
                    "ckpt_cap_bytes": sv.ckpt_budget, "ckpt_state_bytes": state_bytes(sv, int(eng.info.ckpt_every)),
                    "graph": bool(sv.use_graph),

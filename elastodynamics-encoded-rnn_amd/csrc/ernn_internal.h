@@ -166,6 +166,7 @@ struct BlockArgs {
   int snapshot;
   int rw, ksmax;                 // compiled shape: rows per wavefront, steps per launch
   int fin_base, fin_k;           // as StepArgs: an instance's last level also into slot fin_base + ceil(I.nt / fin_k)
+  int ortho;                     // kDOnly: C16 = C26 = 0 in every set (the operator sums drop those terms)
 };
 
 // Time-blocked adjoint (constant and damage modes): one launch runs the
@@ -191,6 +192,7 @@ struct BlockAdjArgs {
   // launches read (u^{m0}, c, v^{m0+1}) from u ring slot u_in.  The launch
   // writes (u, c, v) of its level m0 - ks, v of the level above, to u_out.
   int u_in, u_out, u_ck, seg_top;
+  int ortho;                     // as BlockArgs (reversed u over known constants)
 };
 
 // Fused time-blocked backward: a time-blocked adjoint launch of segment s and

@@ -179,6 +179,19 @@ __device__ __forceinline__ void apply_op_sum(const OpCoef& k, const UDeriv& d, f
        __builtin_fmaf(k.k16, d.xx_x, __builtin_fmaf(k.k26, d.yy_x, k.q226 * d.xy_y)))));
 }
 
+// apply_op_sum with k16 = k26 = 0 (a background orthotropic along the grid
+// axes, then q216 = q226 = 0 too): the same bits, three instructions per
+// component instead of six (the dropped terms add exact zeros).
+template <bool kOrtho>
+__device__ __forceinline__ void apply_op(const OpCoef& k, const UDeriv& d, float& ox, float& oy) {
+  if constexpr (kOrtho) {
+    ox = __builtin_fmaf(k.k11, d.xx_x, __builtin_fmaf(k.k66, d.yy_x, k.q1266 * d.xy_y));
+    oy = __builtin_fmaf(k.k66, d.xx_y, __builtin_fmaf(k.k22, d.yy_y, k.q1266 * d.xy_x));
+  } else {
+    apply_op_sum(k, d, ox, oy);
+  }
+}
+
 // (bx, by) + f (A u): the operator sum first, then one addition.
 __device__ __forceinline__ void apply_op_u(const OpCoef& k, const UDeriv& d, float bx, float by,
                                            float& ox, float& oy) {
@@ -1000,7 +1013,7 @@ struct Band {
   }
 };
 
-template <int MODE, int V, int RW, int KS>
+template <int MODE, int V, int RW, int KS, bool kO = false>
 __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
   const auto& a = *ap;
   constexpr int kCols = kWave * V;
@@ -1144,7 +1157,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
             if constexpr (V == 1) d = tap_derivs(xm, xc, xp, ym, yc, yp);
             else d = uderivs<V>(xm, xc, xp, ym, yc, yp, e + 1);
             float ox, oy;   // the arithmetic of fwd_op<kDField>
-            apply_op_sum(kf, d, ox, oy);
+            apply_op<kO>(kf, d, ox, oy);
             const float dx = cf[r][0][e] * ox, dy = cf[r][0][e] * oy;
             advance(u[0][r][e], c[0][r][e], v[0][r][e], dx);
             advance(u[1][r][e], c[1][r][e], v[1][r][e], dy);
@@ -1193,7 +1206,7 @@ __device__ __forceinline__ void fwd_block_body(KArg<BlockArgs>* ap, int bid) {
           for (int q = 0; q < NC; ++q) cq[q] = cf[r][q][e];
           if constexpr (MODE == kDOnly) {   // the arithmetic of fwd_op<kDField>, keeping the operator value
             float ox, oy;
-            apply_op_sum(kf, d, ox, oy);
+            apply_op<kO>(kf, d, ox, oy);
             if (lb_slot >= 0 && r >= bd.own_lo && r < bd.own_hi && bd.col_own && bd.col_in[e]) {
               const uint32_t so = bd.srow(I, r);
               bstore(ox, lbx, bd.vcol[e], so);
@@ -1748,7 +1761,7 @@ __device__ __forceinline__ void adj_block_body(KArg<BlockAdjArgs>* ap, int bid) 
 // block of arithmetic whose rows are stepped as soon as their stencils are
 // formed.
 // --------------------------------------------------------------------------
-template <bool kJ, int V, int RW, int KS>
+template <bool kJ, bool kO, int V, int RW, int KS>
 __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) {
 #pragma clang fp contract(fast)
   const auto& a = *ap;
@@ -1962,9 +1975,9 @@ __device__ __forceinline__ void rev_block_body(KArg<BlockAdjArgs>* ap, int bid) 
         for (int e = 0; e < V; ++e) {
           float dx, dy, lx, ly;
           const UDeriv dlam = tap_derivs(xm, xc, xp, ym, yc, yp);
-          apply_op_sum(kl, dlam, dx, dy);
+          apply_op<kO>(kl, dlam, dx, dy);
           const UDeriv du = tap_derivs(pxm, pxc, pxp, pym, pyc, pyp);
-          apply_op_sum(kl, du, lx, ly);   // L^m, the forward's operator value
+          apply_op<kO>(kl, du, lx, ly);   // L^m, the forward's operator value
           // dJ/dD with lam^{m+1}, not yet stepped (zero at m = nt); rows
           // outside the output tile are never stored
           gacc[r][e] += __builtin_fmaf(l[0][r][e], lx, l[1][r][e] * ly);
@@ -2308,16 +2321,23 @@ __global__ void __launch_bounds__(kWave * blk_waves(RW)) adj_block_kernel(BlockA
 }
 
 // Reversed-u adjoint (damage field over known constants; rev_block_body).
-template <bool kJ, int V, int RW, int KS>
+// kO: the plan's promise of a background orthotropic along the grid axes
+// (BlockAdjArgs.ortho), whose operator sums drop the C16, C26 terms.
+template <bool kJ, bool kO, int V, int RW, int KS>
 __global__ void __launch_bounds__(kWave * blk_waves(RW)) __attribute__((amdgpu_waves_per_eu((RW <= 6 || (!kJ && RW <= 8)) ? 4 : 3)))
 rev_block_kernel(BlockAdjArgs) {
-  rev_block_body<kJ, V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
+  rev_block_body<kJ, kO, V, RW, KS>((KArg<BlockAdjArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
 
 // the argument is read in place through the kernarg segment (offset 0)
 template <int MODE, int V, int RW, int KS>
 __global__ void __launch_bounds__(kWave * blk_waves(RW)) fwd_block_kernel(BlockArgs) {
   fwd_block_body<MODE, V, RW, KS>((KArg<BlockArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
+}
+// damage over known constants with the orthotropic promise (BlockArgs.ortho)
+template <int V, int RW, int KS>
+__global__ void __launch_bounds__(kWave * blk_waves(RW)) fwd_block_kernel_ortho(BlockArgs) {
+  fwd_block_body<kDOnly, V, RW, KS, true>((KArg<BlockArgs>*)__builtin_amdgcn_kernarg_segment_ptr(), blockIdx.x);
 }
 
 // --------------------------------------------------------------------------
@@ -2536,16 +2556,28 @@ hipError_t launch_fused_step(const FusedArgs& fa, int V, int blocks, hipStream_t
     hipLaunchKernelGGL((fwd_block_kernel<MODE, v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
     return hipGetLastError();                                                                      \
   }
+#define ERNN_BLOCK_CASE_ORTHO(v_, r_, k_)                                                           \
+  if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
+    hipLaunchKernelGGL((fwd_block_kernel_ortho<v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
+    return hipGetLastError();                                                                      \
+  }
 template <int MODE>
 static hipError_t launch_block_m(const BlockArgs& a, int V, int grid, hipStream_t st) {
   if constexpr (MODE == kConst6) {
     ERNN_BLOCK_CASES(ERNN_BLOCK_CASE)
+  } else if constexpr (MODE == kDOnly) {
+    if (a.ortho) {
+      ERNN_BLOCK_CASES_FIELD(ERNN_BLOCK_CASE_ORTHO)
+    } else {
+      ERNN_BLOCK_CASES_FIELD(ERNN_BLOCK_CASE)
+    }
   } else {
     ERNN_BLOCK_CASES_FIELD(ERNN_BLOCK_CASE)
   }
   return hipErrorInvalidValue;
 }
 #undef ERNN_BLOCK_CASE
+#undef ERNN_BLOCK_CASE_ORTHO
 
 hipError_t launch_fwd_block(const BlockArgs& a, int V, int grid, hipStream_t st) {
   switch (a.mode) {
@@ -2588,10 +2620,10 @@ static hipError_t launch_adj_block_m(const BlockAdjArgs& a, int V, int grid, hip
 #define ERNN_REV_BLOCK_CASES(X) ERNN_REV_BLOCK_CASES_JOINT(X) X(1, 10, 5) X(1, 10, 6)
 #define ERNN_REV_BLOCK_CASE(v_, r_, k_)                                                             \
   if (V == v_ && a.rw == r_ && a.ksmax == k_) {                                                    \
-    hipLaunchKernelGGL((rev_block_kernel<kJ, v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
+    hipLaunchKernelGGL((rev_block_kernel<kJ, kO, v_, r_, k_>), dim3(grid), dim3(kWave * blk_waves(r_)), 0, st, a); \
     return hipGetLastError();                                                                      \
   }
-template <bool kJ>
+template <bool kJ, bool kO>
 static hipError_t launch_rev_block_j(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
   if constexpr (kJ) {   // the joint sums' registers leave out the ten-row bands (they would spill)
     ERNN_REV_BLOCK_CASES_JOINT(ERNN_REV_BLOCK_CASE)
@@ -2603,8 +2635,9 @@ static hipError_t launch_rev_block_j(const BlockAdjArgs& a, int V, int grid, hip
 // a.mode: kDOnly (dJ/dD over known constants) or kDField (joint: dJ/dD and the
 // constant-gradient partials)
 hipError_t launch_rev_block(const BlockAdjArgs& a, int V, int grid, hipStream_t st) {
-  if (a.mode == kDOnly) return launch_rev_block_j<false>(a, V, grid, st);
-  if (a.mode == kDField) return launch_rev_block_j<true>(a, V, grid, st);
+  if (a.mode == kDOnly)
+    return a.ortho ? launch_rev_block_j<false, true>(a, V, grid, st) : launch_rev_block_j<false, false>(a, V, grid, st);
+  if (a.mode == kDField) return launch_rev_block_j<true, false>(a, V, grid, st);   // joint: never ortho
   return hipErrorInvalidValue;
 }
 #undef ERNN_REV_BLOCK_CASE

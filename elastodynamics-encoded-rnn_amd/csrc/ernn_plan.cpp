@@ -197,6 +197,9 @@ struct Plan {
   // and each instance's own last level goes to checkpoint ceil(nt_b / K);
   // two more ring slots carry the reversed u between launches.
   bool rtr = false;
+  // ortho (with fixed_c0): the caller promises C16 = C26 = 0 in every cell, so
+  // the damage kernels' operator sums drop those terms (the same bits)
+  bool ortho = false;
   std::vector<ernn_instance_t> inst_h;
   std::vector<ernn_source_t> src_h;
   std::vector<ernn_receiver_t> rec_h;
@@ -589,6 +592,9 @@ int validate(const ernn_desc_t* d) {
   if (d->reverse_u && d->param_mode != ERNN_PARAM_DFIELD)
     return fail(ERNN_E_INVALID, "reverse_u applies to the damage-field mode (ERNN_PARAM_DFIELD) only");
   if (d->reverse_u && d->fuse) return fail(ERNN_E_INVALID, "reverse_u needs fuse = 0");
+  if (d->ortho != 0 && d->ortho != 1) return fail(ERNN_E_INVALID, "ortho must be 0 or 1");
+  if (d->ortho && d->fixed_c0 != 1)
+    return fail(ERNN_E_INVALID, "ortho applies to a damage field over known constants (fixed_c0 = 1) only");
   if (d->reverse_u && (d->ablk_ks <= 1 || !rev_block_ok(d->ablk_v, d->ablk_rw, d->ablk_ks, d->fixed_c0 != 1)))
     return fail(ERNN_E_INVALID, "reverse_u needs a time-blocked adjoint of shape (1, RW, KS), RW in {4, 6, 8}, "
                                 "KS in {2, 4}, or RW in {8, 10}, KS in {5, 6} (joint gradient: not RW = 10)");
@@ -667,6 +673,7 @@ int build_plan(const ernn_desc_t* d, Plan* p) {
   p->fuse = d->fuse != 0;
   p->lbank = d->fixed_c0 == 1 && p->mode == ERNN_PARAM_DFIELD;
   p->rtr = p->mode == ERNN_PARAM_DFIELD && d->reverse_u == 1;
+  p->ortho = p->lbank && d->ortho == 1;
   // kernels over a damage field: kDOnly banks L (fixed_c0) and, with reversed u
   // (nothing banked), runs the forward's sliding-row path with the bits of kDField
   p->kmode = (p->lbank || p->rtr) ? kDOnly : p->mode;
@@ -859,6 +866,7 @@ BlockArgs block_args(const Plan* p, const BlockTiling& bt) {
   a.ksmax = bt.KS;
   a.fin_base = p->rtr ? p->ck(0) : 0;
   a.fin_k = p->rtr ? p->K : 0;
+  a.ortho = p->ortho && a.mode == kDOnly;
   return a;
 }
 BlockArgs block_args(const Plan* p) { return block_args(p, p->blk); }
@@ -927,6 +935,7 @@ BlockAdjArgs block_adj_args(const Plan* p) {
   a.snapshot = p->snapshot;
   a.rw = p->blk_adj.RW;
   a.ksmax = p->blk_adj.KS;
+  a.ortho = p->ortho && a.mode == kDOnly;
   return a;
 }
 
@@ -1273,6 +1282,7 @@ int ernn_plan_info(const void* plan, ernn_plan_info_t* info) {
   info->ablk_ff_rw = p->ff_rw;
   info->fixed_c0 = p->lbank ? 1 : 0;
   info->reverse_u = p->rtr ? 1 : 0;
+  info->ortho = p->ortho ? 1 : 0;
   return ERNN_OK;
 }
 

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (must precede loading the HIP library)
 _DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_DIR, "libernn_hip.so")
 
-ERNN_ABI_VERSION = 6
+ERNN_ABI_VERSION = 7
 PARAM_CONST6, PARAM_DFIELD, PARAM_FIELD6 = 0, 1, 2
 ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "capacity exceeded"}
 
@@ -44,6 +44,7 @@ class Desc(C.Structure):
                 ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
                 ("ablk_v", C.c_int32), ("ablk_rw", C.c_int32), ("ablk_ks", C.c_int32),
                 ("ablk_ff_rw", C.c_int32), ("fixed_c0", C.c_int32), ("reverse_u", C.c_int32),
+                ("ortho", C.c_int32),
                 ("inst", C.POINTER(Instance)), ("src", C.POINTER(Source)), ("n_src", C.c_int64),
                 ("rec", C.POINTER(Receiver)), ("n_rec", C.c_int64)]
 
@@ -59,7 +60,7 @@ class PlanInfo(C.Structure):
                 ("fuse", C.c_int32), ("blk_v", C.c_int32), ("blk_rw", C.c_int32), ("blk_ks", C.c_int32),
                 ("blk_blocks", C.c_int32), ("ablk_v", C.c_int32), ("ablk_rw", C.c_int32),
                 ("ablk_ks", C.c_int32), ("ablk_blocks", C.c_int32), ("ablk_ff_rw", C.c_int32),
-                ("fixed_c0", C.c_int32), ("reverse_u", C.c_int32)]
+                ("fixed_c0", C.c_int32), ("reverse_u", C.c_int32), ("ortho", C.c_int32)]
 
 
 class Call(C.Structure):

@@ -71,16 +71,20 @@ DAMAGE_MEM_CAP = 32 * 10**9   # bytes of checkpointed state the damage-imaging w
 
 
 def damage_1024(nt: int = 3000, n: int = 1024, shots: int = 8, seed: int = 2,
-                mem_cap: Optional[int] = DAMAGE_MEM_CAP, fixed_c0: bool = True) -> Workload:
+                mem_cap: Optional[int] = DAMAGE_MEM_CAP, fixed_c0: bool = True,
+                orthotropic: Optional[bool] = None) -> Workload:
     """mem_cap: HBM bytes for the checkpointed states (Survey.ckpt_budget);
     None = the engine's default, 80 % of the free device memory.  fixed_c0:
     stiffness-degradation imaging inverts D over the known background
-    constants (dJ/dD only); False also returns dJ/dC0 (the joint gradient)."""
+    constants (dJ/dD only); False also returns dJ/dC0 (the joint gradient).
+    orthotropic (None: as fixed_c0): the plan takes C16 = C26 = 0 of the
+    orthotropic background as given (Survey.orthotropic)."""
     h = 0.05e-3
     dt = cfl_dt(C_ORTHO, RHO, h)
     rec = np.concatenate([row_receivers(n, n - 1, n // 4), row_receivers(n, 0, n // 4)])
     sh = [_shot(n, h, nt, dt, int((k + 0.5) * n / shots), rec) for k in range(shots)]
-    sv = Survey(sh, mode="dfield", ckpt_budget=mem_cap, fixed_c0=fixed_c0)
+    ortho = fixed_c0 if orthotropic is None else orthotropic
+    sv = Survey(sh, mode="dfield", ckpt_budget=mem_cap, fixed_c0=fixed_c0, orthotropic=ortho)
     D = ellipse_damage(n, n, 3, np.random.default_rng(seed), semi=(20, 80), level=(0.3, 0.7), taper=5)
     return Workload("damage-1024", sv, np.array([C_ORTHO]), D,
                     description=f"{n}^2 x {nt}, {shots} shots, 512 receivers, D field 3 ellipses"

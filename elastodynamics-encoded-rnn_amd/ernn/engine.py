@@ -93,6 +93,7 @@ class Engine:
         d.ablk_ff_rw = int(sv.adj_ff_rw)
         d.fixed_c0 = int(bool(sv.fixed_c0))
         d.reverse_u = int(bool(sv.reverse_u))
+        d.ortho = int(bool(sv.orthotropic))
         d.inst = inst
         d.src, d.n_src = src, len(src_list)
         d.rec, d.n_rec = rec, len(rec_list)
@@ -129,6 +130,11 @@ class Engine:
     def reverse_u(self) -> bool:
         """The adjoint reconstructs u backward from the checkpoints (no bank, no recompute)."""
         return bool(self.info.reverse_u)
+
+    @property
+    def ortho(self) -> bool:
+        """The damage kernels take C16 = C26 = 0 as given (Survey.orthotropic)."""
+        return bool(self.info.ortho)
 
     @property
     def workspace_bytes(self) -> int:

@@ -210,6 +210,11 @@ class Survey:
     # (1, 4, 4)); the checkpoint interval bounds how far each reconstruction
     # runs from its checkpoint (ckpt_every 0: REV_SEGMENT).
     reverse_u: bool = False
+    # with fixed_c0: the known constants have C16 = C26 = 0 in every set (a
+    # background orthotropic along the grid axes), so the damage kernels drop
+    # those terms of the operator sums (the same bits, fewer instructions).
+    # Constants that break the promise give wrong results; ERNN checks C_init.
+    orthotropic: bool = False
 
     def __post_init__(self):
         if self.mode not in MODES:
@@ -222,6 +227,8 @@ class Survey:
             raise ValueError("reverse_u applies to the dfield mode only")
         if self.reverse_u and self.fuse:
             raise ValueError("reverse_u needs the unfused backward (fuse False or None)")
+        if self.orthotropic and not self.fixed_c0:
+            raise ValueError("orthotropic applies to a dfield survey over known constants (fixed_c0) only")
 
     @property
     def fused(self) -> bool:

@@ -279,6 +279,8 @@ class ERNN(nn.Module):
         self.survey = survey
         dev = torch.device(device) if device is not None else torch.device("cuda")
         C_init = np.asarray(C_init, dtype=np.float64).reshape(survey.n_sets, 6)
+        if survey.orthotropic and np.any(C_init[:, 3:5] != 0):
+            raise ValueError("the survey promises C16 = C26 = 0 (orthotropic): C_init breaks it")
         if scale is None:
             own = np.where(np.abs(C_init) > 0, np.abs(C_init), zero_scale)
             ref = np.maximum(np.abs(C_init[:, :1]), zero_scale)   # C11 of each set

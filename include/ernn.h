@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define ERNN_ABI_VERSION 6
+#define ERNN_ABI_VERSION 7
 
 #define ERNN_OK 0
 #define ERNN_E_INVALID (-1)   /* bad argument or descriptor */
@@ -125,6 +125,11 @@ typedef struct {
                            K-th level and each instance's last level.  Requires fuse = 0 and
                            a time-blocked adjoint (ablk_v, ablk_rw, ablk_ks) = (1, RW, KS),
                            RW in {4, 6, 8}, KS in {2, 4}, or RW in {8, 10}, KS in {5, 6}. */
+  int32_t ortho;        /* with fixed_c0 = 1, 0 or 1.  1: the caller promises C16 = C26 = 0 in
+                           every set's constants (a background orthotropic along the grid axes);
+                           the time-blocked forward and the reversed adjoint drop those terms of
+                           the operator sums (the same bits as with them, fewer instructions).
+                           Constants that break the promise give wrong results. */
   const ernn_instance_t* inst;  /* host array [n_inst] */
   const ernn_source_t* src;     /* host array [n_src] */
   int64_t n_src;
@@ -155,6 +160,7 @@ typedef struct {
                                time-blocked backward) */
   int32_t fixed_c0;         /* the descriptor's fixed_c0 (DFIELD: dJ/dD only) */
   int32_t reverse_u;        /* the descriptor's reverse_u */
+  int32_t ortho;            /* the descriptor's ortho */
 } ernn_plan_info_t;
 
 /* Per-call device pointers.

@@ -364,7 +364,7 @@ def test_reverse_u_full_length_vs_fp64_oracle():
         assert np.array_equal(tr, ref_tr)
         res[K] = g[1]
     print(f"reverse_u {nt} steps: joint gradient, K {autotune.REV_SEGMENT}", flush=True)
-    trj, gj, _ = _run(dataclasses.replace(_variant(sv, ckpt_every=autotune.REV_SEGMENT, **timed), fixed_c0=False,
+    trj, gj, _ = _run(dataclasses.replace(_variant(sv, ckpt_every=autotune.REV_SEGMENT, **timed), fixed_c0=False, orthotropic=False,
                                           reverse_u=True), C0, D.reshape(-1), G)
     assert np.array_equal(trj, ref_tr)
     cells = [D.astype(np.float64) * float(c) for c in C0]

@@ -103,6 +103,9 @@ GOOD = dict(nx=16, ny=16, nt=10, set=0, h=1e-3, dt=1e-8, rho=1600.0,
      "needs a time-blocked adjoint of shape (1, RW, KS)"),
     ({}, {"param_mode": 1, "fixed_c0": 1, "reverse_u": 1, "ablk_v": 2, "ablk_rw": 4, "ablk_ks": 4},
      "needs a time-blocked adjoint of shape (1, RW, KS)"),
+    ({}, {"param_mode": 1, "fixed_c0": 1, "ortho": 2}, "ortho must be 0 or 1"),
+    ({}, {"param_mode": 1, "ortho": 1}, "fixed_c0 = 1"),
+    ({}, {"param_mode": 0, "ortho": 1}, "fixed_c0 = 1"),
 ])
 def test_plan_create_rejects_bad_descriptors(patch, kw, msg):
     inst = dict(GOOD, **patch)
@@ -300,4 +303,19 @@ def test_integration_stub_structs_match_the_binding():
         stub = ns[name]
         assert C.sizeof(stub) == C.sizeof(ref), name
         assert [f[0] for f in stub._fields_] == [f[0] for f in ref._fields_], name
-    assert re.search(r"Desc\(1, 1, 0, P, 1, K(, \d+){17}, inst,", doc)
+    assert re.search(r"Desc\(1, 1, 0, P, 1, K(, \d+){18}, inst,", doc)
+
+
+def test_orthotropic_survey_needs_known_constants_with_c16_c26_zero():
+    """Survey.orthotropic is a promise about the known constants: it needs
+    fixed_c0, and ERNN rejects a C_init with C16 or C26 nonzero before any
+    device work."""
+    from ernn.simulator import ERNN
+    w = configs.damage_1024(nt=20, n=64, shots=1)
+    assert w.survey.orthotropic and w.survey.fixed_c0
+    assert not configs.damage_1024(nt=20, n=64, shots=1, fixed_c0=False).survey.orthotropic
+    with pytest.raises(ValueError, match="fixed_c0"):
+        Survey(w.survey.shots, mode="dfield", orthotropic=True)
+    bad = np.array([configs.C_ORTHO]); bad[0, 3] = 1e9
+    with pytest.raises(ValueError, match="C16 = C26 = 0"):
+        ERNN(w.survey, bad, learn_constants=False)
